@@ -1,0 +1,180 @@
+/*
+ * wurst_amd.h -- C-ABI of the MI355X-native `nextevent_path_tracer`.
+ *
+ * The reference has no FFI: its integrator is the C++ class PathNeeRenderer
+ * (src/renderer/pathnextevent.h:15-196) reached from the scene-JSON dispatch
+ * `"integrator": "nextevent_path_tracer"` (src/sceneio/simple2parser.h:192-195).
+ * This header is the boundary the project's C++ host (and any FFI: ctypes,
+ * cgo, JNI) binds; every entry point below names the reference interface it
+ * replaces.  Plain pointers and sizes only; nothing throws across the ABI.
+ *
+ * Conventions
+ *   - Every function returns WR_OK (0) or a WR_ERR_* code; wr_last_error()
+ *     gives a thread-local message for the last failure on the calling thread.
+ *   - Handles are opaque and owned by the caller; destroy them explicitly.
+ *   - Host arrays are caller-owned and only read during the call.
+ *   - Calls on one scene handle (and films rendered from it) must be
+ *     serialised by the caller; distinct scene handles may be used
+ *     concurrently (each owns a HIP stream).
+ *   - Rays are 8 floats: origin.xyz, tmin, direction.xyz, tmax.  A hit is
+ *     accepted for t in (tmin, tmax], the interval rtcIntersect1/rtcOccluded1
+ *     use in the reference (src/accel/embreeaccel.h:336-422).
+ */
+#ifndef WURST_AMD_H
+#define WURST_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WR_OK 0
+#define WR_ERR_INVALID_ARGUMENT 1
+#define WR_ERR_DEVICE 2          /* HIP runtime error */
+#define WR_ERR_OUT_OF_MEMORY 3
+#define WR_ERR_UNSUPPORTED 4
+
+#define WR_INVALID_ID 0xFFFFFFFFu /* RTC_INVALID_GEOMETRY_ID in the reference */
+
+enum wr_material_type {
+    WR_LAMBERT = 0,     /* LambertDiffuseBsdf        src/bsdf/lambertdiffuse.h:7  */
+    WR_OREN_NAYAR = 1,  /* OrenNayarBsdf             src/bsdf/orennayar.h:8       */
+    WR_PHONG = 2,       /* PhongBsdf (modified)      src/bsdf/phong.h:12          */
+    WR_MIXED = 3        /* MixedBsdf{Lambert, Phong} src/bsdf/mixed.h:6           */
+};
+
+/* Flattened Bsdf (src/common/bsdf.h:16-43).  twosided != 0 wraps the BSDF in
+ * TwoSidedBsdf (src/bsdf/twosided.h:8), as the OBJ loader does for every
+ * material (src/sceneio/meshio/assimpio.h:79-121). */
+typedef struct {
+    int32_t type;
+    int32_t twosided;
+    float rho_d[3];     /* diffuse reflectance (Lambert, Oren-Nayar, Mixed) */
+    float rho_s[3];     /* specular reflectance (Phong, Mixed) */
+    float sigma;        /* Oren-Nayar standard deviation */
+    float exponent;     /* Phong exponent */
+} wr_material;
+
+/* DiffuseAreaLight (src/light/diffusearealight.h:11) over the triangles of
+ * geometry `geom`; emits radiance L on the side of Ng = (v1-v0) x (v2-v0). */
+typedef struct {
+    int32_t geom;
+    float L[3];
+} wr_light;
+
+/* ThinlensCamera (src/camera/thinlens.h:13-55); aperture_radius 0 = pinhole. */
+typedef struct {
+    float pos[3], lookat[3], up[3];
+    float fovy_deg;
+    int32_t width, height;
+    float aperture_radius;
+    float focal_distance;
+} wr_camera;
+
+/* Scene = the triangle meshes the reference attaches to EmbreeAccel
+ * (embreeaccel.h:274-315), in attach order: geometry g owns the contiguous
+ * triangles with tri_geom == g, and (geomID, primID) = (g, index within g). */
+typedef struct {
+    const float *verts;            uint32_t n_verts;   /* [n_verts*3] */
+    const uint32_t *tris;          uint32_t n_tris;    /* [n_tris*3]  */
+    const uint32_t *tri_geom;                          /* [n_tris]    */
+    uint32_t n_geoms;
+    const int32_t *geom_material;                      /* [n_geoms], -1 for emitters */
+    const int32_t *geom_light;                         /* [n_geoms], -1 if not emissive */
+    const wr_material *materials;  uint32_t n_materials;
+    const wr_light *lights;        uint32_t n_lights;
+    wr_camera camera;
+} wr_scene_desc;
+
+/* PathNeeRenderer parameters: num_path_vertices / num_spp are the JSON keys of
+ * PathNeeRenderer::FromSimple2Json (pathnextevent.h:17-26). */
+typedef struct {
+    int32_t spp;                 /* samples per pixel rendered by this call */
+    int32_t num_path_vertices;   /* N (default 10); max depth = N - 2 */
+    uint32_t seed;
+    int32_t use_mis;             /* NEE_USE_MIS (pathnextevent.h:13); 1 = on */
+    int32_t first_sample;        /* index of the first sample (progressive calls) */
+    int32_t max_paths_in_flight; /* 0 = library default */
+    int32_t profile;             /* 1: time each kernel class with HIP events */
+    int32_t count_traversal;     /* 1: count visited nodes / tested triangles */
+} wr_render_params;
+
+typedef struct {
+    uint64_t paths;
+    uint64_t extend_rays;        /* closest-hit rays traced */
+    uint64_t shadow_rays;        /* any-hit rays traced (NEE + light->camera) */
+    double wall_ms;              /* host wall time of the call */
+    /* with profile = 1: summed HIP-event time and launch count per kernel class */
+    double extend_ms, shadow_ms, shade_ms, generate_ms, accumulate_ms;
+    uint64_t extend_launches, shadow_launches;
+    /* with count_traversal = 1 */
+    uint64_t extend_nodes, extend_tris, shadow_nodes, shadow_tris;
+} wr_render_stats;
+
+typedef struct wr_device wr_device;
+typedef struct wr_scene wr_scene;
+typedef struct wr_film wr_film;
+
+const char *wr_last_error(void);
+/* Library/ABI version string. */
+const char *wr_version(void);
+
+/* Binds a GPU (HIP device ordinal).  No reference counterpart (the reference
+ * is CPU-only); it stands where Parallel's thread pool is created
+ * (src/common/parallel.h:13-21). */
+int wr_device_create(int ordinal, wr_device **out);
+void wr_device_destroy(wr_device *dev);
+
+/* Checks a scene description without touching a GPU (indices, materials,
+ * lights, camera).  The reference reports these as exceptions or asserts
+ * (simple2parser.h:90, cdftable.h:36-44 on zero lights, embreeaccel.h:31-61);
+ * here they are status codes.  wr_scene_create runs the same checks. */
+int wr_scene_validate(const wr_scene_desc *desc);
+
+/* Builds the BVH on the host and verifies its invariants (every triangle in
+ * exactly one leaf, child boxes enclose their triangles, stack bound within the
+ * traversal stack).  Host only; no GPU needed. */
+int wr_bvh_selftest(const wr_scene_desc *desc, uint64_t *n_nodes, uint32_t *max_stack, uint32_t *depth);
+
+/* Uploads geometry, materials, lights and camera and builds the BVH.
+ * Replaces EmbreeAccel::addGeometry + commit (embreeaccel.h:274-315,466-469),
+ * GeneralSubpathSampler's light CdfTable (generalsubpathsampler.h:162-183) and
+ * ThinlensCamera construction (thinlens.h:40-55). */
+int wr_scene_create(wr_device *dev, const wr_scene_desc *desc, wr_scene **out);
+void wr_scene_destroy(wr_scene *scene);
+/* BVH statistics: nodes, leaf triangles, max stack depth, bytes of nodes+tris. */
+int wr_scene_info(const wr_scene *scene, uint64_t *n_nodes, uint64_t *n_tris, uint32_t *max_stack,
+                  uint64_t *device_bytes);
+
+/* FP64 film in device memory: a base buffer (own-pixel samples) and a splat
+ * buffer (light->camera connections).  Replaces Film's base and atomic buffers
+ * (src/common/film.h:35-68). */
+int wr_film_create(wr_scene *scene, wr_film **out);
+void wr_film_destroy(wr_film *film);
+int wr_film_clear(wr_film *film);
+/* Copies the raw sums to host arrays [H][W][3] (row 0 = top of the image).
+ * image = base / spp + splat / (spp * W * H)   (pathnextevent.h:154-157). */
+int wr_film_read(wr_film *film, double *base, double *splat);
+
+/* PathNeeRenderer::render (pathnextevent.h:152-191): adds spp samples per pixel
+ * of the NEE + MIS estimator into `film`.  Blocking. */
+int wr_render_nee(wr_scene *scene, wr_film *film, const wr_render_params *params, wr_render_stats *stats);
+
+/* Closest hit for n host rays (EmbreeAccel::intersect -> rtcIntersect1,
+ * embreeaccel.h:353-422).  Misses give geom = prim = WR_INVALID_ID, t = +inf. */
+int wr_trace_closest(wr_scene *scene, const float *rays, uint32_t n, uint32_t *geom, uint32_t *prim, float *t);
+/* Any hit (EmbreeAccel::isIntersect -> rtcOccluded1, embreeaccel.h:336-351). */
+int wr_trace_any(wr_scene *scene, const float *rays, uint32_t n, uint8_t *occluded);
+
+/* Device-resident variants (pointers from hipMalloc, e.g. torch tensors):
+ * rays [n*8] float, hits [n] of {uint32 leaf-tri or WR_INVALID_ID, float t}.
+ * Launched on the scene's stream; `ms` (nullable) receives the kernel time
+ * measured with HIP events on that stream. */
+int wr_trace_closest_device(wr_scene *scene, const float *d_rays, uint32_t n, void *d_hits, float *ms);
+int wr_trace_any_device(wr_scene *scene, const float *d_rays, uint32_t n, uint8_t *d_occluded, float *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

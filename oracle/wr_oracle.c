@@ -1,0 +1,1121 @@
+/*
+ * wr_oracle.c -- CPU FP64 restatement of the `nextevent_path_tracer` hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker).  Used by tests/, by
+ * __graft_entry__.smoke() and by bench.py's cpu_baseline leg; never by the
+ * product.  See wr_oracle.h for what pins each function.
+ *
+ * Reference files restated here (paths relative to the reference checkout):
+ *   src/renderer/pathnextevent.h          evalContribution / render
+ *   src/pathsampler/generalsubpathsampler.h  randomWalk / createLightSubpath /
+ *                                          createCameraSubpath / evalLightPdfA
+ *   src/common/path.h                     GeometryTerm / ConvertToAreaFromSolidAngle
+ *   src/common/scene.h                    transmittance (visibility part)
+ *   src/accel/embreeaccel.h               FP32 ray casts, hit-vertex construction
+ *   src/bsdf/{lambertdiffuse,orennayar,phong,mixed,twosided}.h
+ *   src/light/diffusearealight.h, src/geometry/trianglemesh.h, src/common/cdftable.h
+ *   src/camera/thinlens.h, src/common/camera.h, src/common/film.h
+ *   src/common/mapping.h, src/common/math/{coordframe,local,ray}.h
+ */
+#define _GNU_SOURCE
+#include "wr_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <pthread.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+#include <unistd.h>
+
+#define PI 3.141592653589793238462643383279502884197169399375105820974
+#define INV_PI 0.318309886183790671537767526745028724068919291480912897495
+#define SMALL_VALUE 1e-12 /* scalarmath.h:32 */
+#define RAY_EPS 1e-4      /* ray.h:8 */
+
+static __thread char g_err[256];
+const char *orc_last_error(void) { return g_err; }
+static int fail(const char *msg) {
+    snprintf(g_err, sizeof g_err, "%s", msg);
+    return 1;
+}
+
+/* ------------------------------------------------------------------ vec3 */
+typedef struct { double x, y, z; } v3;
+static inline v3 V(double x, double y, double z) { v3 r = {x, y, z}; return r; }
+static inline v3 vadd(v3 a, v3 b) { return V(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline v3 vsub(v3 a, v3 b) { return V(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline v3 vmul(v3 a, double s) { return V(a.x * s, a.y * s, a.z * s); }
+static inline v3 vneg(v3 a) { return V(-a.x, -a.y, -a.z); }
+static inline double vdot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline v3 vcross(v3 a, v3 b) {
+    return V(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+static inline double vlen2(v3 a) { return vdot(a, a); }
+static inline v3 vnorm(v3 a) { return vmul(a, 1.0 / sqrt(vlen2(a))); }
+static inline double vget(v3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
+static inline v3 flipy(v3 a) { return V(a.x, -a.y, a.z); }
+
+/* ------------------------------------------------------------------ RNG
+ * Counter-based: uniform(seed, pixel, sample, dim).  A pure integer function,
+ * so the GPU path draws bit-identical numbers.  The reference uses
+ * mt19937_64 per tile (src/common/rng.h:5-29); the stream itself is not part
+ * of the estimator, only the dimension order (one 2-D draw per decision). */
+static inline uint32_t mix32(uint32_t x) {
+    x ^= x >> 16; x *= 0x7feb352du;
+    x ^= x >> 15; x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+static inline uint32_t path_key(uint32_t seed, uint32_t pixel, uint32_t sample) {
+    uint32_t h = mix32(seed ^ 0xa511e9b3u);
+    h = mix32(h ^ pixel);
+    h = mix32(h ^ (sample * 0x9e3779b9u + 0x7f4a7c15u));
+    return h;
+}
+static inline double u01(uint32_t key, uint32_t dim) {
+    return (double)(mix32(key + dim * 0x9e3779b9u + 0x632be5abu) >> 8) * (1.0 / 16777216.0);
+}
+uint32_t orc_rng_u32(uint32_t seed, uint32_t pixel, uint32_t sample, uint32_t dim) {
+    return mix32(path_key(seed, pixel, sample) + dim * 0x9e3779b9u + 0x632be5abu);
+}
+double orc_rng_uniform(uint32_t seed, uint32_t pixel, uint32_t sample, uint32_t dim) {
+    return u01(path_key(seed, pixel, sample), dim);
+}
+/* dimension layout per path */
+enum { DIM_LIGHT = 0, DIM_LENS = 2, DIM_PIXEL = 4, DIM_BOUNCE0 = 6, DIMS_PER_BOUNCE = 6 };
+enum { BD_MEDIUM = 0, BD_BSDF = 2 };
+
+/* ------------------------------------------------------------------ frames, mappings */
+typedef struct { v3 x, y, z; } frame3;
+/* coordframe.h:46-60: Duff et al. 2017 ONB with y = normal */
+static frame3 make_frame(v3 n) {
+    frame3 f;
+    f.y = n;
+    double sign = copysign(1.0, n.y);
+    double a = -1.0 / (sign + n.y);
+    double b = n.z * n.x * a;
+    f.x = V(sign + n.x * n.x * a, -n.x, b);
+    f.z = V(sign * b, -sign * n.z, 1.0 + sign * n.z * n.z * a);
+    return f;
+}
+static inline v3 to_local(const frame3 *f, v3 g) { return V(vdot(g, f->x), vdot(g, f->y), vdot(g, f->z)); }
+static inline v3 to_world(const frame3 *f, v3 l) {
+    return vadd(vadd(vmul(f->x, l.x), vmul(f->y, l.y)), vmul(f->z, l.z));
+}
+void orc_onb(const double n[3], double x[3], double z[3]) {
+    frame3 f = make_frame(V(n[0], n[1], n[2]));
+    x[0] = f.x.x; x[1] = f.x.y; x[2] = f.x.z;
+    z[0] = f.z.x; z[1] = f.z.y; z[2] = f.z.z;
+}
+/* mapping.h:42-49 */
+static v3 cosine_hemisphere(double u0, double u1) {
+    double sinphi = sqrt(1.0 - u0);
+    double theta = PI * 2.0 * u1;
+    return V(cos(theta) * sinphi, sqrt(u0), sin(theta) * sinphi);
+}
+void orc_cosine_hemisphere(const double u[2], double out[3]) {
+    v3 o = cosine_hemisphere(u[0], u[1]);
+    out[0] = o.x; out[1] = o.y; out[2] = o.z;
+}
+/* mapping.h:21-27 */
+static void barycentric_from_square(double u0, double u1, double *b1, double *b2) {
+    double s = sqrt(u0);
+    *b1 = s * (1.0 - u1);
+    *b2 = u1 * s;
+}
+void orc_barycentric_from_square(const double u[2], double b[2]) { barycentric_from_square(u[0], u[1], &b[0], &b[1]); }
+/* mapping.h:53-73 (concentric map) */
+static void disk_from_square(double u0, double u1, double *dx, double *dy) {
+    double a = u0 * 2.0 - 1.0, b = u1 * 2.0 - 1.0;
+    if (a == 0 && b == 0) { *dx = 0; *dy = 0; return; }
+    double r, phi;
+    if (a * a > b * b) { r = a; phi = (PI / 4.0) * (b / a); }
+    else { r = b; phi = (PI / 2.0) - (PI / 4.0) * (a / b); }
+    *dx = r * cos(phi);
+    *dy = r * sin(phi);
+}
+void orc_disk_from_square(const double u[2], double d[2]) { disk_from_square(u[0], u[1], &d[0], &d[1]); }
+
+/* ------------------------------------------------------------------ CDF table (cdftable.h:10-44) */
+typedef struct { int n; double *pdf, *cdf; } cdf_table;
+static int cdf_build(cdf_table *t, const double *w, int n) {
+    t->n = n; t->pdf = NULL; t->cdf = NULL;
+    if (n == 0) return 0;
+    t->pdf = (double *)malloc(sizeof(double) * n);
+    t->cdf = (double *)malloc(sizeof(double) * n);
+    if (!t->pdf || !t->cdf) return 1;
+    double sum = 0.0;
+    for (int i = 0; i < n; i++) sum += fabs(w[i]);
+    for (int i = 0; i < n; i++) t->pdf[i] = fabs(w[i]) / sum;
+    t->cdf[0] = fabs(w[0]) / sum;
+    t->cdf[n - 1] = 1.0;
+    for (int i = 1; i < n - 1; i++) t->cdf[i] = t->cdf[i - 1] + t->pdf[i];
+    return 0;
+}
+static int cdf_sample(const cdf_table *t, double u, double *pdf, double *remapped) {
+    /* std::lower_bound: first index whose cdf >= u */
+    int lo = 0, hi = t->n;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (t->cdf[mid] < u) lo = mid + 1; else hi = mid;
+    }
+    int idx = lo;
+    if (idx >= t->n) idx = t->n - 1; /* unreachable for u < 1 (cdf[n-1] == 1) */
+    if (pdf) *pdf = t->pdf[idx];
+    if (remapped) {
+        double r = (idx == 0) ? u / t->pdf[idx] : (u - t->cdf[idx - 1]) / t->pdf[idx];
+        *remapped = fmin(r, 1.0 - SMALL_VALUE);
+    }
+    return idx;
+}
+static void cdf_free(cdf_table *t) { free(t->pdf); free(t->cdf); }
+int orc_cdf_sample(const double *w, int n, double u, double *pdf, double *remapped) {
+    cdf_table t;
+    if (n <= 0 || cdf_build(&t, w, n)) return -1;
+    int i = cdf_sample(&t, u, pdf, remapped);
+    cdf_free(&t);
+    return i;
+}
+
+/* ------------------------------------------------------------------ BSDFs (local frame, y = normal) */
+typedef struct { double rho_d[3], rho_s[3], sigma, n; int type, twosided; } bsdf_t;
+static void mat_load(bsdf_t *b, const orc_material *m) {
+    for (int i = 0; i < 3; i++) { b->rho_d[i] = m->rho_d[i]; b->rho_s[i] = m->rho_s[i]; }
+    b->sigma = m->sigma; b->n = m->exponent; b->type = m->type; b->twosided = m->twosided;
+}
+typedef struct { double r, g, b; } rgb;
+static inline rgb RGB(double r, double g, double b) { rgb c = {r, g, b}; return c; }
+static inline rgb rgb3(const double *a) { return RGB(a[0], a[1], a[2]); }
+static inline rgb cmul(rgb a, rgb b) { return RGB(a.r * b.r, a.g * b.g, a.b * b.b); }
+static inline rgb cscale(rgb a, double s) { return RGB(a.r * s, a.g * s, a.b * s); }
+static inline rgb cadd(rgb a, rgb b) { return RGB(a.r + b.r, a.g + b.g, a.b + b.b); }
+static inline int czero(rgb a) { return a.r == 0.0 && a.g == 0.0 && a.b == 0.0; } /* vecsimd.h:258-262 */
+static const rgb BLACK = {0, 0, 0};
+
+/* lambertdiffuse.h:16-51 */
+static rgb lambert_eval(const bsdf_t *b, v3 wi, v3 wo) {
+    if (wi.y <= 0.0 || wo.y <= 0.0) return BLACK;
+    return cscale(rgb3(b->rho_d), INV_PI);
+}
+static double lambert_pdf(v3 wi, v3 wo) {
+    if (wi.y <= 0.0 || wo.y <= 0.0) return 0.0;
+    return wo.y * INV_PI;
+}
+static rgb lambert_sample(const bsdf_t *b, v3 wi, double u0, double u1, v3 *wo, double *pdf) {
+    if (wi.y <= 0.0) return BLACK;
+    v3 o = cosine_hemisphere(u0, u1);
+    *wo = o;
+    if (pdf) *pdf = fmax(o.y, 0.0) * INV_PI;
+    if (o.y <= 0.0) return BLACK;
+    return rgb3(b->rho_d);
+}
+/* orennayar.h:14-83 (pbrt-v3) */
+static rgb oren_eval(const bsdf_t *b, v3 wi, v3 wo) {
+    if (wi.y <= 0.0 || wo.y <= 0.0) return BLACK;
+    double sinI = sqrt(fmin(fmax(1.0 - wi.y * wi.y, 0.0), 1.0));
+    double sinO = sqrt(fmin(fmax(1.0 - wo.y * wo.y, 0.0), 1.0));
+    double maxCos = 0.0, sinAlpha = 0.0, tanBeta = 0.0;
+    if (sinI > 1e-4 && sinO > 1e-4) {
+        double sinPI = wi.z / sinI, cosPI = wi.x / sinI;
+        double sinPO = wo.z / sinO, cosPO = wo.x / sinO;
+        double dcos = cosPI * cosPO + sinPI * sinPO;
+        maxCos = fmax(0.0, dcos);
+        if (fabs(wi.y) > fabs(wo.y)) { sinAlpha = sinO; tanBeta = sinI / wi.y; }
+        else { sinAlpha = sinI; tanBeta = sinO / wo.y; }
+    }
+    double s2 = b->sigma * b->sigma;
+    double A = 1.0 - s2 / (2 * (s2 + 0.33));
+    double B = 0.45 * s2 / (s2 + 0.09);
+    return cscale(rgb3(b->rho_d), INV_PI * (A + B * maxCos * sinAlpha * tanBeta));
+}
+static rgb oren_sample(const bsdf_t *b, v3 wi, double u0, double u1, v3 *wo, double *pdf) {
+    if (wi.y < 0.0) { if (pdf) *pdf = 0.0; return BLACK; }
+    v3 o = cosine_hemisphere(u0, u1);
+    *wo = o;
+    if (pdf) *pdf = o.y * INV_PI;
+    return cscale(oren_eval(b, o, wi), PI); /* orennayar.h:74 passes (out, incoming) */
+}
+/* phong.h:22-93 (modified Phong, Lafortune & Willems 1994) */
+static inline v3 reflect_local(v3 wi) { return V(-wi.x, wi.y, -wi.z); } /* Local3::Reflect(-wi) */
+static rgb phong_eval(const bsdf_t *b, v3 wi, v3 wo) {
+    if (wi.y <= 0.0 || wo.y <= 0.0) return BLACK;
+    v3 r = reflect_local(wi);
+    if (r.y <= 0.0) return BLACK;
+    double c = vdot(r, wo);
+    if (c <= 0.0) return BLACK;
+    return cscale(rgb3(b->rho_s), (b->n + 2.0) * pow(c, b->n) * INV_PI * 0.5);
+}
+static double phong_pdf(const bsdf_t *b, v3 wi, v3 wo) {
+    if (wi.y <= 0.0 && wo.y <= 0.0) return 0.0;
+    v3 r = reflect_local(wi);
+    double c = vdot(r, wo);
+    if (c <= 0.0) return 0.0;
+    return (b->n + 1.0) * pow(c, b->n) * INV_PI * 0.5;
+}
+static rgb phong_sample(const bsdf_t *b, v3 wi, double u0, double u1, v3 *wo, double *pdf) {
+    if (wi.y <= 0.0) return BLACK;
+    double n = b->n;
+    v3 r = reflect_local(wi);
+    double cosT = pow(u0, 1.0 / (n + 1.0));
+    double sinT = sqrt(1.0 - cosT * cosT);
+    double phi = 2.0 * PI * u1;
+    v3 lo = V(sinT * cos(phi), cosT, sinT * sin(phi));
+    frame3 rb = make_frame(r);
+    v3 o = to_world(&rb, lo);
+    double c = vdot(o, r);
+    *wo = o;
+    if (pdf) *pdf = (n + 1.0) * 0.5 * INV_PI * pow(c, n);
+    if (r.y <= 0.0 || o.y <= 0.0 || c <= 0.0) return BLACK;
+    return cscale(rgb3(b->rho_s), (n + 2.0) / (n + 1.0) * o.y);
+}
+/* one-sided dispatch (the material's own BSDF) */
+static rgb inner_eval(const bsdf_t *b, v3 wi, v3 wo) {
+    switch (b->type) {
+    case ORC_LAMBERT: return lambert_eval(b, wi, wo);
+    case ORC_OREN_NAYAR: return oren_eval(b, wi, wo);
+    case ORC_PHONG: return phong_eval(b, wi, wo);
+    default: return cadd(lambert_eval(b, wi, wo), phong_eval(b, wi, wo)); /* mixed.h:20-29 */
+    }
+}
+static double inner_pdf(const bsdf_t *b, v3 wi, v3 wo) {
+    switch (b->type) {
+    case ORC_LAMBERT: case ORC_OREN_NAYAR: return lambert_pdf(wi, wo); /* orennayar.h:57-64 same form */
+    case ORC_PHONG: return phong_pdf(b, wi, wo);
+    default: return (lambert_pdf(wi, wo) + phong_pdf(b, wi, wo)) / 2.0; /* mixed.h:31-40 */
+    }
+}
+static rgb inner_sample(const bsdf_t *b, v3 wi, double u0, double u1, v3 *wo, double *pdf) {
+    switch (b->type) {
+    case ORC_LAMBERT: return lambert_sample(b, wi, u0, u1, wo, pdf);
+    case ORC_OREN_NAYAR: return oren_sample(b, wi, u0, u1, wo, pdf);
+    case ORC_PHONG: return phong_sample(b, wi, u0, u1, wo, pdf);
+    default: { /* mixed.h:42-60: two components, chosen uniformly */
+        unsigned nc = 2;
+        unsigned ci = (unsigned)floor(u0 * nc);
+        if (ci > nc - 1) ci = nc - 1;
+        double ur = fmin(u0 * nc - ci, 1.0 - SMALL_VALUE);
+        v3 o = V(0, 0, 0);
+        if (ci == 0) lambert_sample(b, wi, ur, u1, &o, NULL);
+        else phong_sample(b, wi, ur, u1, &o, NULL);
+        *wo = o;
+        rgb f = inner_eval(b, wi, o);
+        double p = inner_pdf(b, wi, o);
+        if (pdf) *pdf = p;
+        if (p == 0.0) return BLACK;
+        return cscale(f, o.y / p);
+    }
+    }
+}
+/* twosided.h:25-63 wrapper */
+static rgb bsdf_eval(const bsdf_t *b, v3 wi, v3 wo) {
+    if (!b->twosided) return inner_eval(b, wi, wo);
+    if (wi.y > 0.0 && wo.y > 0.0) return inner_eval(b, wi, wo);
+    if (!(wi.y > 0.0) && !(wo.y > 0.0)) return inner_eval(b, flipy(wi), flipy(wo));
+    return BLACK;
+}
+static double bsdf_pdf(const bsdf_t *b, v3 wi, v3 wo) {
+    if (!b->twosided) return inner_pdf(b, wi, wo);
+    if (wi.y > 0.0 && wo.y > 0.0) return inner_pdf(b, wi, wo);
+    if (!(wi.y > 0.0) && !(wo.y > 0.0)) return inner_pdf(b, flipy(wi), flipy(wo));
+    return 0.0;
+}
+static rgb bsdf_sample(const bsdf_t *b, v3 wi, double u0, double u1, v3 *wo, double *pdf) {
+    if (!b->twosided || wi.y > 0.0) return inner_sample(b, wi, u0, u1, wo, pdf);
+    rgb r = inner_sample(b, flipy(wi), u0, u1, wo, pdf);
+    *wo = flipy(*wo);
+    return r;
+}
+void orc_bsdf_eval(const orc_material *m, const double wi[3], const double wo[3], double f[3], double *pdf) {
+    bsdf_t b; mat_load(&b, m);
+    v3 i = V(wi[0], wi[1], wi[2]), o = V(wo[0], wo[1], wo[2]);
+    rgb c = bsdf_eval(&b, i, o);
+    f[0] = c.r; f[1] = c.g; f[2] = c.b;
+    *pdf = bsdf_pdf(&b, i, o);
+}
+void orc_bsdf_eval_n(const orc_material *m, int n, const double *wi, const double *wo, double *f, double *pdf) {
+    for (int i = 0; i < n; i++) orc_bsdf_eval(m, wi + 3 * i, wo + 3 * i, f + 3 * i, pdf + i);
+}
+void orc_bsdf_sample(const orc_material *m, const double wi[3], const double u[2], double wo[3], double w[3], double *pdf) {
+    bsdf_t b; mat_load(&b, m);
+    v3 o = V(0, 0, 0);
+    double p = 0.0;
+    rgb c = bsdf_sample(&b, V(wi[0], wi[1], wi[2]), u[0], u[1], &o, &p);
+    wo[0] = o.x; wo[1] = o.y; wo[2] = o.z;
+    w[0] = c.r; w[1] = c.g; w[2] = c.b;
+    *pdf = p;
+}
+
+/* ------------------------------------------------------------------ scene + BVH */
+typedef struct { double lo[3], hi[3]; int32_t left_or_first, count; } bnode; /* count>0: leaf */
+
+typedef struct {
+    int geom;
+    rgb L;
+    double area, inv_area;
+    uint32_t first_tri, n_tris;
+    cdf_table tri_cdf;
+} light_t;
+
+struct orc_scene {
+    uint32_t n_verts, n_tris, n_geoms;
+    v3 *verts;              /* FP32 vertices as doubles */
+    uint32_t *tris;         /* [n_tris*3] */
+    uint32_t *tri_geom;
+    uint32_t *geom_first;   /* first triangle of each geometry */
+    int32_t *geom_material, *geom_light;
+    bsdf_t *mats; uint32_t n_mats;
+    light_t *lights; uint32_t n_lights;
+    cdf_table light_cdf;
+    /* BVH */
+    bnode *nodes; uint32_t n_nodes;
+    uint32_t *tri_order;    /* leaf order -> triangle index */
+    /* camera (thinlens.h:40-55, camera.h:9-17) */
+    v3 cam_origin; frame3 cam_frame;
+    int W, H;
+    double cam_d, per_film_norm, per_pixel_norm, lens_area, aperture, focal;
+};
+
+typedef struct { double c[3]; uint32_t tri; } bref;
+
+static void box_tri(const orc_scene *s, uint32_t t, double lo[3], double hi[3]) {
+    for (int k = 0; k < 3; k++) { lo[k] = DBL_MAX; hi[k] = -DBL_MAX; }
+    for (int j = 0; j < 3; j++) {
+        v3 p = s->verts[s->tris[3 * t + j]];
+        for (int k = 0; k < 3; k++) {
+            double v = vget(p, k);
+            if (v < lo[k]) lo[k] = v;
+            if (v > hi[k]) hi[k] = v;
+        }
+    }
+}
+static double half_area(const double lo[3], const double hi[3]) {
+    double dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+    if (dx < 0 || dy < 0 || dz < 0) return 0.0;
+    return dx * dy + dy * dz + dz * dx;
+}
+
+typedef struct { bnode *nodes; uint32_t n, cap; } nodevec;
+static uint32_t push_node(nodevec *nv) {
+    if (nv->n == nv->cap) {
+        nv->cap = nv->cap ? nv->cap * 2 : 1024;
+        nv->nodes = (bnode *)realloc(nv->nodes, sizeof(bnode) * nv->cap);
+    }
+    return nv->n++;
+}
+
+#define NBINS 16
+/* Binned SAH build (test-side BVH; any correct BVH serves the oracle). */
+static void build_rec(const orc_scene *s, bref *refs, double (*tlo)[3], double (*thi)[3],
+                      uint32_t begin, uint32_t end, nodevec *nv, uint32_t node) {
+    double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+    double clo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, chi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+    for (uint32_t i = begin; i < end; i++) {
+        uint32_t t = refs[i].tri;
+        for (int k = 0; k < 3; k++) {
+            if (tlo[t][k] < lo[k]) lo[k] = tlo[t][k];
+            if (thi[t][k] > hi[k]) hi[k] = thi[t][k];
+            if (refs[i].c[k] < clo[k]) clo[k] = refs[i].c[k];
+            if (refs[i].c[k] > chi[k]) chi[k] = refs[i].c[k];
+        }
+    }
+    bnode *nd = &nv->nodes[node];
+    memcpy(nd->lo, lo, sizeof lo);
+    memcpy(nd->hi, hi, sizeof hi);
+    uint32_t n = end - begin;
+    if (n <= 4) { nd->left_or_first = (int32_t)begin; nd->count = (int32_t)n; return; }
+    double best_cost = DBL_MAX; int best_axis = -1, best_split = 0;
+    for (int k = 0; k < 3; k++) {
+        double ext = chi[k] - clo[k];
+        if (ext <= 0) continue;
+        uint32_t cnt[NBINS] = {0};
+        double blo[NBINS][3], bhi[NBINS][3];
+        for (int b = 0; b < NBINS; b++) for (int j = 0; j < 3; j++) { blo[b][j] = DBL_MAX; bhi[b][j] = -DBL_MAX; }
+        for (uint32_t i = begin; i < end; i++) {
+            int b = (int)((refs[i].c[k] - clo[k]) / ext * NBINS);
+            if (b >= NBINS) b = NBINS - 1;
+            if (b < 0) b = 0;
+            cnt[b]++;
+            uint32_t t = refs[i].tri;
+            for (int j = 0; j < 3; j++) {
+                if (tlo[t][j] < blo[b][j]) blo[b][j] = tlo[t][j];
+                if (thi[t][j] > bhi[b][j]) bhi[b][j] = thi[t][j];
+            }
+        }
+        double rarea[NBINS]; uint32_t rcnt[NBINS];
+        double alo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, ahi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+        uint32_t acc = 0;
+        for (int b = NBINS - 1; b > 0; b--) {
+            for (int j = 0; j < 3; j++) { if (blo[b][j] < alo[j]) alo[j] = blo[b][j]; if (bhi[b][j] > ahi[j]) ahi[j] = bhi[b][j]; }
+            acc += cnt[b];
+            rarea[b] = half_area(alo, ahi); rcnt[b] = acc;
+        }
+        for (int j = 0; j < 3; j++) { alo[j] = DBL_MAX; ahi[j] = -DBL_MAX; }
+        acc = 0;
+        for (int b = 0; b < NBINS - 1; b++) {
+            for (int j = 0; j < 3; j++) { if (blo[b][j] < alo[j]) alo[j] = blo[b][j]; if (bhi[b][j] > ahi[j]) ahi[j] = bhi[b][j]; }
+            acc += cnt[b];
+            if (acc == 0 || rcnt[b + 1] == 0) continue;
+            double cost = half_area(alo, ahi) * acc + rarea[b + 1] * rcnt[b + 1];
+            if (cost < best_cost) { best_cost = cost; best_axis = k; best_split = b; }
+        }
+    }
+    uint32_t mid;
+    if (best_axis < 0) {
+        mid = begin + n / 2; /* all centroids coincide: split by count */
+    } else {
+        double ext = chi[best_axis] - clo[best_axis];
+        uint32_t i = begin, j = end;
+        while (i < j) {
+            int b = (int)((refs[i].c[best_axis] - clo[best_axis]) / ext * NBINS);
+            if (b >= NBINS) b = NBINS - 1;
+            if (b <= best_split) i++;
+            else { bref tmp = refs[i]; refs[i] = refs[--j]; refs[j] = tmp; }
+        }
+        mid = i;
+        if (mid == begin || mid == end) mid = begin + n / 2;
+    }
+    uint32_t l = push_node(nv);
+    uint32_t r = push_node(nv);
+    nv->nodes[node].left_or_first = (int32_t)l;
+    nv->nodes[node].count = 0;
+    build_rec(s, refs, tlo, thi, begin, mid, nv, l);
+    build_rec(s, refs, tlo, thi, mid, end, nv, r);
+}
+
+static int build_bvh(orc_scene *s) {
+    uint32_t n = s->n_tris;
+    bref *refs = (bref *)malloc(sizeof(bref) * (n ? n : 1));
+    double (*tlo)[3] = malloc(sizeof(double[3]) * (n ? n : 1));
+    double (*thi)[3] = malloc(sizeof(double[3]) * (n ? n : 1));
+    if (!refs || !tlo || !thi) return 1;
+    for (uint32_t t = 0; t < n; t++) {
+        box_tri(s, t, tlo[t], thi[t]);
+        for (int k = 0; k < 3; k++) refs[t].c[k] = 0.5 * (tlo[t][k] + thi[t][k]);
+        refs[t].tri = t;
+    }
+    nodevec nv = {0};
+    uint32_t root = push_node(&nv);
+    if (n == 0) {
+        for (int k = 0; k < 3; k++) { nv.nodes[root].lo[k] = 1; nv.nodes[root].hi[k] = -1; }
+        nv.nodes[root].left_or_first = 0; nv.nodes[root].count = 0;
+        /* count 0 on the root with no children: traversal treats an empty box as a miss */
+    } else {
+        build_rec(s, refs, tlo, thi, 0, n, &nv, root);
+    }
+    s->nodes = nv.nodes; s->n_nodes = nv.n;
+    s->tri_order = (uint32_t *)malloc(sizeof(uint32_t) * (n ? n : 1));
+    for (uint32_t i = 0; i < n; i++) s->tri_order[i] = refs[i].tri;
+    free(refs); free(tlo); free(thi);
+    return 0;
+}
+
+/* ------------------------------------------------------------------ ray / triangle (FP64) */
+typedef struct { double o[3], d[3], tmin, tmax; int kx, ky, kz; double Sx, Sy, Sz; double inv[3]; } fray;
+static void ray_setup(fray *r, const float o[3], const float d[3], float tmin, float tmax) {
+    for (int k = 0; k < 3; k++) { r->o[k] = o[k]; r->d[k] = d[k]; }
+    r->tmin = tmin; r->tmax = tmax;
+    int kz = 0;
+    if (fabs(r->d[1]) > fabs(r->d[kz])) kz = 1;
+    if (fabs(r->d[2]) > fabs(r->d[kz])) kz = 2;
+    int kx = (kz + 1) % 3, ky = (kx + 1) % 3;
+    if (r->d[kz] < 0) { int t = kx; kx = ky; ky = t; }
+    r->kx = kx; r->ky = ky; r->kz = kz;
+    r->Sx = r->d[kx] / r->d[kz]; r->Sy = r->d[ky] / r->d[kz]; r->Sz = 1.0 / r->d[kz];
+    for (int k = 0; k < 3; k++) r->inv[k] = 1.0 / r->d[k];
+}
+/* Watertight test (Woop, Benthin, Wald, JCGT 2013) in FP64, double-sided.
+ * Returns 1 on a hit in (tmin, tmax]; always writes the barycentric margin. */
+static int tri_isect(const fray *r, v3 p0, v3 p1, v3 p2, double *t_out, double *margin, double *t_any) {
+    double A[3] = {p0.x - r->o[0], p0.y - r->o[1], p0.z - r->o[2]};
+    double B[3] = {p1.x - r->o[0], p1.y - r->o[1], p1.z - r->o[2]};
+    double C[3] = {p2.x - r->o[0], p2.y - r->o[1], p2.z - r->o[2]};
+    int kx = r->kx, ky = r->ky, kz = r->kz;
+    double Ax = A[kx] - r->Sx * A[kz], Ay = A[ky] - r->Sy * A[kz];
+    double Bx = B[kx] - r->Sx * B[kz], By = B[ky] - r->Sy * B[kz];
+    double Cx = C[kx] - r->Sx * C[kz], Cy = C[ky] - r->Sy * C[kz];
+    double U = Cx * By - Cy * Bx, Vv = Ax * Cy - Ay * Cx, W = Bx * Ay - By * Ax;
+    double det = U + Vv + W;
+    *margin = -1.0; *t_any = INFINITY;
+    if (det == 0.0) return 0;
+    double Az = r->Sz * A[kz], Bz = r->Sz * B[kz], Cz = r->Sz * C[kz];
+    double T = U * Az + Vv * Bz + W * Cz;
+    double t = T / det;
+    double m = fmin(fmin(U / det, Vv / det), W / det);
+    *margin = m;
+    *t_any = t;
+    if ((U < 0 || Vv < 0 || W < 0) && (U > 0 || Vv > 0 || W > 0)) return 0;
+    if (!(t > r->tmin && t <= r->tmax)) return 0;
+    *t_out = t;
+    return 1;
+}
+static int box_hit(const fray *r, const bnode *n, double tmax) {
+    double t0 = r->tmin, t1 = tmax;
+    for (int k = 0; k < 3; k++) {
+        if (r->d[k] == 0.0) {
+            if (r->o[k] < n->lo[k] || r->o[k] > n->hi[k]) return 0;
+            continue;
+        }
+        double a = (n->lo[k] - r->o[k]) * r->inv[k];
+        double b = (n->hi[k] - r->o[k]) * r->inv[k];
+        if (a > b) { double t = a; a = b; b = t; }
+        a -= 1e-9 * fabs(a) + 1e-300; b += 1e-9 * fabs(b) + 1e-300; /* conservative */
+        if (a > t0) t0 = a;
+        if (b < t1) t1 = b;
+        if (t0 > t1) return 0;
+    }
+    return 1;
+}
+static inline v3 tri_v(const orc_scene *s, uint32_t t, int j) { return s->verts[s->tris[3 * t + j]]; }
+
+/* closest hit through the BVH; returns triangle or UINT32_MAX */
+static uint32_t bvh_closest(const orc_scene *s, const fray *r, double *t_hit) {
+    if (s->n_tris == 0) return UINT32_MAX;
+    uint32_t stack[128]; int sp = 0;
+    stack[sp++] = 0;
+    double best = r->tmax; uint32_t best_tri = UINT32_MAX;
+    fray rr = *r;
+    while (sp) {
+        const bnode *n = &s->nodes[stack[--sp]];
+        rr.tmax = best;
+        if (!box_hit(&rr, n, best)) continue;
+        if (n->count > 0) {
+            for (int i = 0; i < n->count; i++) {
+                uint32_t t = s->tri_order[n->left_or_first + i];
+                double th, m, ta;
+                if (tri_isect(&rr, tri_v(s, t, 0), tri_v(s, t, 1), tri_v(s, t, 2), &th, &m, &ta)) {
+                    if (th <= best) { best = th; best_tri = t; rr.tmax = best; }
+                }
+            }
+        } else {
+            stack[sp++] = (uint32_t)n->left_or_first + 1;
+            stack[sp++] = (uint32_t)n->left_or_first;
+        }
+    }
+    *t_hit = best;
+    return best_tri;
+}
+static int bvh_any(const orc_scene *s, const fray *r) {
+    if (s->n_tris == 0) return 0;
+    uint32_t stack[128]; int sp = 0;
+    stack[sp++] = 0;
+    while (sp) {
+        const bnode *n = &s->nodes[stack[--sp]];
+        if (!box_hit(r, n, r->tmax)) continue;
+        if (n->count > 0) {
+            for (int i = 0; i < n->count; i++) {
+                uint32_t t = s->tri_order[n->left_or_first + i];
+                double th, m, ta;
+                if (tri_isect(r, tri_v(s, t, 0), tri_v(s, t, 1), tri_v(s, t, 2), &th, &m, &ta)) return 1;
+            }
+        } else {
+            stack[sp++] = (uint32_t)n->left_or_first + 1;
+            stack[sp++] = (uint32_t)n->left_or_first;
+        }
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------ scene create */
+int orc_scene_create(const orc_scene_desc *d, orc_scene **out) {
+    *out = NULL;
+    if (!d) return fail("null desc");
+    if (d->n_lights == 0) return fail("scene has no lights");
+    if (d->camera.width <= 0 || d->camera.height <= 0) return fail("bad resolution");
+    orc_scene *s = (orc_scene *)calloc(1, sizeof(orc_scene));
+    s->n_verts = d->n_verts; s->n_tris = d->n_tris; s->n_geoms = d->n_geoms;
+    s->verts = (v3 *)malloc(sizeof(v3) * (d->n_verts ? d->n_verts : 1));
+    for (uint32_t i = 0; i < d->n_verts; i++)
+        s->verts[i] = V(d->verts[3 * i], d->verts[3 * i + 1], d->verts[3 * i + 2]);
+    s->tris = (uint32_t *)malloc(sizeof(uint32_t) * 3 * (d->n_tris ? d->n_tris : 1));
+    memcpy(s->tris, d->tris, sizeof(uint32_t) * 3 * d->n_tris);
+    for (uint32_t i = 0; i < 3 * d->n_tris; i++)
+        if (s->tris[i] >= d->n_verts) { orc_scene_destroy(s); return fail("triangle index out of range"); }
+    s->tri_geom = (uint32_t *)malloc(sizeof(uint32_t) * (d->n_tris ? d->n_tris : 1));
+    memcpy(s->tri_geom, d->tri_geom, sizeof(uint32_t) * d->n_tris);
+    s->geom_first = (uint32_t *)malloc(sizeof(uint32_t) * (d->n_geoms ? d->n_geoms : 1));
+    for (uint32_t g = 0; g < d->n_geoms; g++) s->geom_first[g] = UINT32_MAX;
+    for (uint32_t t = 0; t < d->n_tris; t++) {
+        uint32_t g = d->tri_geom[t];
+        if (g >= d->n_geoms) { orc_scene_destroy(s); return fail("tri_geom out of range"); }
+        if (s->geom_first[g] == UINT32_MAX) s->geom_first[g] = t;
+    }
+    s->geom_material = (int32_t *)malloc(sizeof(int32_t) * (d->n_geoms ? d->n_geoms : 1));
+    s->geom_light = (int32_t *)malloc(sizeof(int32_t) * (d->n_geoms ? d->n_geoms : 1));
+    memcpy(s->geom_material, d->geom_material, sizeof(int32_t) * d->n_geoms);
+    memcpy(s->geom_light, d->geom_light, sizeof(int32_t) * d->n_geoms);
+    s->n_mats = d->n_materials;
+    s->mats = (bsdf_t *)malloc(sizeof(bsdf_t) * (d->n_materials ? d->n_materials : 1));
+    for (uint32_t m = 0; m < d->n_materials; m++) mat_load(&s->mats[m], &d->materials[m]);
+    for (uint32_t g = 0; g < d->n_geoms; g++) {
+        if (s->geom_light[g] < 0 && (s->geom_material[g] < 0 || (uint32_t)s->geom_material[g] >= s->n_mats)) {
+            orc_scene_destroy(s); return fail("geometry without material or light");
+        }
+    }
+    /* lights: DiffuseAreaLight on a TriangleMeshGeometry (diffusearealight.h:21-24,
+     * trianglemesh.h:110-129) */
+    s->n_lights = d->n_lights;
+    s->lights = (light_t *)calloc(d->n_lights, sizeof(light_t));
+    double *w = (double *)malloc(sizeof(double) * d->n_lights);
+    for (uint32_t l = 0; l < d->n_lights; l++) {
+        light_t *L = &s->lights[l];
+        L->geom = d->lights[l].geom;
+        L->L = RGB(d->lights[l].L[0], d->lights[l].L[1], d->lights[l].L[2]);
+        if (L->geom < 0 || (uint32_t)L->geom >= d->n_geoms || s->geom_first[L->geom] == UINT32_MAX) {
+            free(w); orc_scene_destroy(s); return fail("light geometry invalid");
+        }
+        L->first_tri = s->geom_first[L->geom];
+        uint32_t c = 0;
+        while (L->first_tri + c < d->n_tris && d->tri_geom[L->first_tri + c] == (uint32_t)L->geom) c++;
+        L->n_tris = c;
+        double *areas = (double *)malloc(sizeof(double) * c);
+        double sum = 0;
+        for (uint32_t i = 0; i < c; i++) {
+            uint32_t t = L->first_tri + i;
+            v3 a = tri_v(s, t, 0), b = tri_v(s, t, 1), cc = tri_v(s, t, 2);
+            areas[i] = sqrt(vlen2(vcross(vsub(b, a), vsub(cc, a)))) / 2.0;
+            sum += areas[i];
+        }
+        if (!(sum > 0)) { free(areas); free(w); orc_scene_destroy(s); return fail("light with zero area"); }
+        L->area = sum; L->inv_area = 1.0 / sum;
+        cdf_build(&L->tri_cdf, areas, (int)c);
+        free(areas);
+        w[l] = sqrt(L->L.r * L->L.r + L->L.g * L->L.g + L->L.b * L->L.b);
+    }
+    cdf_build(&s->light_cdf, w, (int)d->n_lights);
+    free(w);
+    /* camera */
+    const orc_camera *c = &d->camera;
+    s->W = c->width; s->H = c->height;
+    s->cam_origin = V(c->pos[0], c->pos[1], c->pos[2]);
+    v3 la = V(c->lookat[0], c->lookat[1], c->lookat[2]), up = V(c->up[0], c->up[1], c->up[2]);
+    double fovy = (double)c->fovy_deg * PI / 180.0;
+    s->cam_frame.z = vnorm(vsub(la, s->cam_origin));
+    s->cam_frame.x = vnorm(vcross(up, s->cam_frame.z));
+    s->cam_frame.y = vnorm(vcross(s->cam_frame.z, s->cam_frame.x));
+    s->cam_d = (double)s->H * 0.5 / tan(fovy * 0.5);
+    s->per_pixel_norm = s->cam_d * s->cam_d / ((double)s->W * (double)s->H);
+    s->per_film_norm = s->cam_d * s->cam_d;
+    s->aperture = c->aperture_radius;
+    s->focal = c->focal_distance;
+    s->lens_area = PI * s->aperture * s->aperture;
+    if (!(s->focal > 0)) { orc_scene_destroy(s); return fail("focal distance must be > 0"); }
+    if (build_bvh(s)) { orc_scene_destroy(s); return fail("bvh build failed"); }
+    *out = s;
+    return 0;
+}
+void orc_scene_destroy(orc_scene *s) {
+    if (!s) return;
+    free(s->verts); free(s->tris); free(s->tri_geom); free(s->geom_first);
+    free(s->geom_material); free(s->geom_light); free(s->mats);
+    if (s->lights) for (uint32_t l = 0; l < s->n_lights; l++) cdf_free(&s->lights[l].tri_cdf);
+    free(s->lights);
+    cdf_free(&s->light_cdf);
+    free(s->nodes); free(s->tri_order);
+    free(s);
+}
+
+/* ------------------------------------------------------------------ light + camera sampling */
+typedef struct { v3 pos, n; frame3 f; rgb contrib; double pdfA; int light; } lvert;
+/* createLightSubpath with numVertices = 1 (generalsubpathsampler.h:185-255),
+ * DiffuseAreaLight::sampleLe0ByPdf (diffusearealight.h:53-72),
+ * TriangleMeshGeometry::sample (trianglemesh.h:131-159) */
+static void sample_light(const orc_scene *s, double u0, double u1, lvert *lv) {
+    double pdfLight, ur;
+    int li = cdf_sample(&s->light_cdf, u0, &pdfLight, &ur);
+    const light_t *L = &s->lights[li];
+    double ur2;
+    int ti = cdf_sample(&L->tri_cdf, ur, NULL, &ur2);
+    double b1, b2;
+    barycentric_from_square(ur2, u1, &b1, &b2);
+    uint32_t t = L->first_tri + (uint32_t)ti;
+    v3 p0 = tri_v(s, t, 0), p1 = tri_v(s, t, 1), p2 = tri_v(s, t, 2);
+    lv->pos = vadd(vadd(vmul(p0, 1.0 - b1 - b2), vmul(p1, b1)), vmul(p2, b2));
+    lv->n = vnorm(vcross(vsub(p1, p0), vsub(p2, p0)));
+    lv->f = make_frame(lv->n);
+    double pdfPos = L->inv_area;
+    lv->contrib = cscale(cscale(L->L, PI / pdfPos), 1.0 / pdfLight);
+    lv->pdfA = pdfLight * pdfPos;
+    lv->light = li;
+}
+int orc_light_sample(const orc_scene *s, const double u[2], double pos[3], double normal[3], double contrib[3], double *pdfA) {
+    lvert lv;
+    sample_light(s, u[0], u[1], &lv);
+    pos[0] = lv.pos.x; pos[1] = lv.pos.y; pos[2] = lv.pos.z;
+    normal[0] = lv.n.x; normal[1] = lv.n.y; normal[2] = lv.n.z;
+    contrib[0] = lv.contrib.r; contrib[1] = lv.contrib.g; contrib[2] = lv.contrib.b;
+    *pdfA = lv.pdfA;
+    return lv.light;
+}
+/* ThinlensCamera::sampleWe0ByPdf (thinlens.h:121-136) */
+static v3 camera_position(const orc_scene *s, double u0, double u1, v3 *lens_local) {
+    double dx, dy;
+    disk_from_square(u0, u1, &dx, &dy);
+    v3 ll = V(dx * s->aperture, dy * s->aperture, 0.0);
+    *lens_local = ll;
+    return vadd(to_world(&s->cam_frame, ll), s->cam_origin);
+}
+/* samplePerPixelWe1CosByPdf -> sampleWe1CosByPdf (thinlens.h:138-166, camera.h:19-25) */
+static v3 camera_direction(const orc_scene *s, v3 lens_local, int px, int py, double j0, double j1,
+                           double raster[2], double *pdfW) {
+    double invW = 1.0 / (double)s->W, invH = 1.0 / (double)s->H;
+    double ns0 = ((double)px + j0) * invW, ns1 = ((double)py + j1) * invH;
+    raster[0] = ns0 * s->W; raster[1] = ns1 * s->H;
+    v3 film = V((ns0 - 0.5) * s->W, (ns1 - 0.5) * s->H, -s->cam_d);
+    v3 focal = vmul(film, s->focal / film.z);
+    v3 dl = vnorm(vsub(focal, lens_local));
+    double c = dl.z;
+    *pdfW = s->per_film_norm / (c * c * c) * (invW * invH);
+    return to_world(&s->cam_frame, dl);
+}
+void orc_camera_ray(const orc_scene *s, int px, int py, const double ul[2], const double up[2],
+                    double org[3], double dir[3], double raster[2], double *pdfW) {
+    v3 ll;
+    v3 o = camera_position(s, ul[0], ul[1], &ll);
+    v3 d = camera_direction(s, ll, px, py, up[0], up[1], raster, pdfW);
+    org[0] = o.x; org[1] = o.y; org[2] = o.z;
+    dir[0] = d.x; dir[1] = d.y; dir[2] = d.z;
+}
+/* ThinlensCamera::evalWe1 (thinlens.h:62-80); returns 0 outside the film */
+static double camera_we1(const orc_scene *s, v3 cam_pos, v3 w, double raster[2]) {
+    v3 pos = to_local(&s->cam_frame, vsub(cam_pos, s->cam_origin));
+    v3 dir = to_local(&s->cam_frame, w);
+    double c = dir.z;
+    if (vdot(w, s->cam_frame.z) < 0.0) return 0.0;
+    v3 fp = vadd(vmul(vmul(dir, 1.0 / c), s->focal), pos);
+    v3 film = vmul(vmul(fp, -1.0 / s->focal), s->cam_d);
+    double rx = (film.x / s->W + 0.5) * s->W;
+    double ry = (film.y / s->H + 0.5) * s->H;
+    if (!(rx >= 0.0 && rx < s->W && ry >= 0.0 && ry < s->H)) return 0.0; /* bound.h:32-38 */
+    raster[0] = rx; raster[1] = ry;
+    double c2 = c * c;
+    return s->per_film_norm / (c2 * c2);
+}
+double orc_camera_we1(const orc_scene *s, const double cp[3], const double w[3], double raster[2]) {
+    return camera_we1(s, V(cp[0], cp[1], cp[2]), V(w[0], w[1], w[2]), raster);
+}
+
+/* ------------------------------------------------------------------ queries in FP32 (embreeaccel.h:336-422) */
+typedef struct { uint64_t ext, shd; } counters;
+static int occluded(const orc_scene *s, v3 o, v3 d, double tmin, double tmax, counters *cn) {
+    float of[3] = {(float)o.x, (float)o.y, (float)o.z}, df[3] = {(float)d.x, (float)d.y, (float)d.z};
+    fray r;
+    ray_setup(&r, of, df, (float)tmin, (float)tmax);
+    cn->shd++;
+    return bvh_any(s, &r);
+}
+/* returns triangle index or UINT32_MAX; writes the FP32-rounded t as the reference does */
+static uint32_t closest(const orc_scene *s, v3 o, v3 d, double *t, counters *cn) {
+    float of[3] = {(float)o.x, (float)o.y, (float)o.z}, df[3] = {(float)d.x, (float)d.y, (float)d.z};
+    fray r;
+    ray_setup(&r, of, df, (float)RAY_EPS, INFINITY);
+    cn->ext++;
+    double th;
+    uint32_t tri = bvh_closest(s, &r, &th);
+    if (tri != UINT32_MAX) *t = (double)(float)th;
+    return tri;
+}
+
+/* ------------------------------------------------------------------ the estimator */
+/* pathnextevent.h:41-150 with the random walk of generalsubpathsampler.h:13-160.
+ * base[3] receives splats[0].mContrib; sp_px, sp_py and sp[3] receive splats[1]. */
+static void eval_contribution(const orc_scene *s, int N, int use_mis, uint32_t key, int px, int py,
+                              double base[3], int *sp_ok, int *sp_px, int *sp_py, double sp[3], counters *cn) {
+    base[0] = base[1] = base[2] = 0.0;
+    *sp_ok = 0;
+    /* light subpath: one vertex */
+    lvert l;
+    sample_light(s, u01(key, DIM_LIGHT), u01(key, DIM_LIGHT + 1), &l);
+    double lPdfA = use_mis ? l.pdfA : 1.0;
+    /* camera vertex */
+    v3 lens_local;
+    v3 cpos = camera_position(s, u01(key, DIM_LENS), u01(key, DIM_LENS + 1), &lens_local);
+    v3 cn_ = s->cam_frame.z; /* camera geometry normal (thinlens.h:130) */
+    /* t = 1: light -> camera connection (pathnextevent.h:75-99) */
+    {
+        v3 uv = vsub(cpos, l.pos);
+        double len2 = vlen2(uv);
+        v3 nuv = vmul(uv, 1.0 / sqrt(len2));
+        double G = fabs(vdot(l.n, nuv)) * fabs(vdot(cn_, nuv)) / len2;
+        if (G != 0.0) {
+            double raster[2];
+            v3 wcam = vnorm(vsub(l.pos, cpos));
+            double we1 = camera_we1(s, cpos, wcam, raster);
+            if (we1 != 0.0) {
+                v3 wl = to_local(&l.f, vnorm(vsub(cpos, l.pos)));
+                double le1 = (wl.y < 0.0) ? 0.0 : INV_PI;
+                if (le1 != 0.0) {
+                    double len = sqrt(len2);
+                    int occ = occluded(s, l.pos, vmul(uv, 1.0 / len), RAY_EPS, len - RAY_EPS, cn);
+                    if (!occ) {
+                        double mis = 1.0;
+                        if (use_mis) {
+                            /* c0->pdfA(CameraImaginary, l) = perPixel pdfW * conversion to l */
+                            v3 dl = to_local(&s->cam_frame, wcam);
+                            double c = dl.z;
+                            double pdfW = s->per_pixel_norm / (c * c * c);
+                            double camPdfA = pdfW * fabs(vdot(l.n, vmul(nuv, -1.0))) / len2;
+                            mis = l.pdfA / (l.pdfA + camPdfA);
+                        }
+                        double k = mis * we1 * G * le1;
+                        sp[0] = k * l.contrib.r; sp[1] = k * l.contrib.g; sp[2] = k * l.contrib.b;
+                        *sp_px = (int)raster[0]; *sp_py = (int)raster[1];
+                        *sp_ok = 1;
+                    }
+                }
+            }
+        }
+    }
+    /* camera direction */
+    double raster0[2], pdfW;
+    v3 dir = camera_direction(s, lens_local, px, py, u01(key, DIM_PIXEL), u01(key, DIM_PIXEL + 1), raster0, &pdfW);
+    /* random walk */
+    v3 ro = cpos, rd = dir;
+    v3 prev_pos = cpos;
+    rgb prev_contrib = RGB(1, 1, 1);  /* contrib1 of samplePerPixelWe1CosByPdf */
+    rgb prev_path = RGB(1, 1, 1);     /* camera vertex mPathContrib = We0/pdf = 1 */
+    double prevPdfW = use_mis ? pdfW : 0.0;
+    int cNum = 1;
+    int count = 0;
+    for (int bounce = 0;; bounce++) {
+        double t;
+        uint32_t tri = closest(s, ro, rd, &t, cn);
+        if (tri == UINT32_MAX) break; /* no envmap: callback(nullptr) -> stop */
+        /* hit vertex (embreeaccel.h:398-420) */
+        v3 p = vadd(ro, vmul(rd, t));
+        v3 p0 = tri_v(s, tri, 0), p1 = tri_v(s, tri, 1), p2 = tri_v(s, tri, 2);
+        v3 ng = vnorm(vcross(vsub(p1, p0), vsub(p2, p0)));
+        frame3 f = make_frame(ng);
+        uint32_t g = s->tri_geom[tri];
+        rgb path = cmul(prev_contrib, prev_path);
+        /* area pdf of the sampling that produced this vertex (path.h:47-60) */
+        double cPdfA = prevPdfW;
+        if (use_mis) {
+            v3 uv = vsub(p, prev_pos);
+            double len2 = vlen2(uv);
+            v3 nuv = vmul(uv, 1.0 / sqrt(len2));
+            cPdfA *= fabs(vdot(ng, nuv)) / len2;
+        }
+        cNum++;
+        int is_light = s->geom_light[g] >= 0;
+        if (!is_light && cNum != N) {
+            /* NEE (pathnextevent.h:106-127) */
+            v3 uv = vsub(l.pos, p);
+            double len2 = vlen2(uv);
+            double len = sqrt(len2);
+            v3 nuv = vmul(uv, 1.0 / len);
+            double G = fabs(vdot(ng, nuv)) * fabs(vdot(l.n, nuv)) / len2;
+            if (G != 0.0) {
+                int occ = occluded(s, p, nuv, RAY_EPS, len - RAY_EPS, cn);
+                if (!occ) {
+                    const bsdf_t *b = &s->mats[s->geom_material[g]];
+                    v3 wi = to_local(&f, vnorm(vsub(prev_pos, p)));
+                    v3 wo = to_local(&f, vnorm(vsub(l.pos, p)));
+                    rgb fs = bsdf_eval(b, wi, wo);
+                    if (!czero(fs)) {
+                        v3 wl = to_local(&l.f, vnorm(vsub(p, l.pos)));
+                        double le1 = (wl.y < 0.0) ? 0.0 : INV_PI;
+                        if (le1 != 0.0) {
+                            double mis = 1.0;
+                            if (use_mis) {
+                                double bsdfPdfA = bsdf_pdf(b, wi, wo) * fabs(vdot(l.n, nuv)) / len2;
+                                mis = lPdfA / (lPdfA + bsdfPdfA);
+                            }
+                            rgb c = cmul(cmul(path, fs), l.contrib);
+                            double k = mis * le1 * G;
+                            base[0] += k * c.r; base[1] += k * c.g; base[2] += k * c.b;
+                        }
+                    }
+                }
+            }
+        } else if (is_light) {
+            /* emitter hit (pathnextevent.h:128-146) */
+            const light_t *L = &s->lights[s->geom_light[g]];
+            double mis = 0.0;
+            if (use_mis) {
+                double lightPdfA = s->light_cdf.pdf[s->geom_light[g]] * L->inv_area;
+                mis = cPdfA / (lightPdfA + cPdfA);
+            }
+            v3 wl = to_local(&f, vnorm(vsub(prev_pos, p)));
+            double le1 = (wl.y < 0.0) ? 0.0 : INV_PI;
+            double k = mis * le1 * PI;
+            base[0] += k * path.r * L->L.r; base[1] += k * path.g * L->L.g; base[2] += k * path.b * L->L.b;
+        } else {
+            break; /* surface at the last vertex: callback returns false */
+        }
+        if (++count == N - 1) break;
+        if (is_light) break;
+        /* BSDF sampling (generalsubpathsampler.h:81-130) */
+        const bsdf_t *b = &s->mats[s->geom_material[g]];
+        uint32_t dim = DIM_BOUNCE0 + DIMS_PER_BOUNCE * (uint32_t)bounce + BD_BSDF;
+        v3 wi = to_local(&f, vneg(rd));
+        v3 wo;
+        double pdf = 0.0;
+        rgb w = bsdf_sample(b, wi, u01(key, dim), u01(key, dim + 1), &wo, &pdf);
+        if (czero(w)) break;
+        prev_contrib = w;
+        prevPdfW = use_mis ? pdf : 0.0;
+        prev_path = path;
+        prev_pos = p;
+        ro = p;
+        rd = to_world(&f, wo);
+    }
+}
+
+/* ------------------------------------------------------------------ render driver (pathnextevent.h:152-191) */
+typedef struct {
+    const orc_scene *s;
+    const orc_render_params *p;
+    double *base, *splat;
+    int tiles_x, tiles_y, x0, y0, x1, y1;
+    volatile int next_tile;
+    counters total;
+    pthread_mutex_t mu;
+} job_t;
+#define TILE 16
+static void atomic_add_double(double *addr, double v) {
+    uint64_t *a = (uint64_t *)addr;
+    uint64_t oldb = __atomic_load_n(a, __ATOMIC_RELAXED), newb;
+    do {
+        double o; memcpy(&o, &oldb, 8);
+        double n = o + v;
+        memcpy(&newb, &n, 8);
+    } while (!__atomic_compare_exchange_n(a, &oldb, newb, 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED));
+}
+static void *worker(void *arg) {
+    job_t *j = (job_t *)arg;
+    const orc_scene *s = j->s;
+    const orc_render_params *p = j->p;
+    counters cn = {0, 0};
+    int ntiles = j->tiles_x * j->tiles_y;
+    for (;;) {
+        int tile = __atomic_fetch_add(&j->next_tile, 1, __ATOMIC_RELAXED);
+        if (tile >= ntiles) break;
+        int tx = tile % j->tiles_x, ty = tile / j->tiles_x;
+        int xa = j->x0 + tx * TILE, ya = j->y0 + ty * TILE;
+        int xb = xa + TILE < j->x1 ? xa + TILE : j->x1;
+        int yb = ya + TILE < j->y1 ? ya + TILE : j->y1;
+        for (int py = ya; py < yb; py++)
+            for (int px = xa; px < xb; px++) {
+                uint32_t pixel = (uint32_t)(py * s->W + px);
+                double acc[3] = {0, 0, 0};
+                for (int k = 0; k < p->spp; k++) {
+                    uint32_t sample = (uint32_t)(p->first_sample + k);
+                    uint32_t key = path_key(p->seed, pixel, sample);
+                    double b[3], sp[3];
+                    int ok, sx, sy;
+                    eval_contribution(s, p->num_path_vertices, p->use_mis, key, px, py, b, &ok, &sx, &sy, sp, &cn);
+                    acc[0] += b[0]; acc[1] += b[1]; acc[2] += b[2];
+                    if (ok) {
+                        double *dst = j->splat + 3 * ((size_t)sy * s->W + sx);
+                        atomic_add_double(dst + 0, sp[0]);
+                        atomic_add_double(dst + 1, sp[1]);
+                        atomic_add_double(dst + 2, sp[2]);
+                    }
+                }
+                double *dst = j->base + 3 * (size_t)pixel;
+                dst[0] += acc[0]; dst[1] += acc[1]; dst[2] += acc[2];
+            }
+    }
+    pthread_mutex_lock(&j->mu);
+    j->total.ext += cn.ext; j->total.shd += cn.shd;
+    pthread_mutex_unlock(&j->mu);
+    return NULL;
+}
+int orc_render_nee(const orc_scene *s, const orc_render_params *p, double *base, double *splat, orc_stats *st) {
+    if (!s || !p || !base || !splat) return fail("null argument");
+    if (p->num_path_vertices <= 1) { if (st) memset(st, 0, sizeof *st); return 0; } /* pathnextevent.h:163 */
+    if (p->spp < 0) return fail("negative spp");
+    job_t j;
+    memset(&j, 0, sizeof j);
+    j.s = s; j.p = p; j.base = base; j.splat = splat;
+    j.x0 = p->px0; j.y0 = p->py0; j.x1 = p->px1; j.y1 = p->py1;
+    if (j.x1 == 0) { j.x0 = 0; j.y0 = 0; j.x1 = s->W; j.y1 = s->H; }
+    if (j.x0 < 0 || j.y0 < 0 || j.x1 > s->W || j.y1 > s->H || j.x0 >= j.x1 || j.y0 >= j.y1) return fail("bad pixel window");
+    j.tiles_x = (j.x1 - j.x0 + TILE - 1) / TILE;
+    j.tiles_y = (j.y1 - j.y0 + TILE - 1) / TILE;
+    pthread_mutex_init(&j.mu, NULL);
+    int nt = p->threads > 0 ? p->threads : (int)sysconf(_SC_NPROCESSORS_ONLN);
+    if (nt < 1) nt = 1;
+    if (nt > 256) nt = 256;
+    struct timespec t0, t1;
+    clock_gettime(CLOCK_MONOTONIC, &t0);
+    pthread_t th[256];
+    for (int i = 0; i < nt; i++) pthread_create(&th[i], NULL, worker, &j);
+    for (int i = 0; i < nt; i++) pthread_join(th[i], NULL);
+    clock_gettime(CLOCK_MONOTONIC, &t1);
+    pthread_mutex_destroy(&j.mu);
+    if (st) {
+        st->paths = (uint64_t)(j.x1 - j.x0) * (uint64_t)(j.y1 - j.y0) * (uint64_t)p->spp;
+        st->extend_rays = j.total.ext;
+        st->shadow_rays = j.total.shd;
+        st->seconds = (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------ trace API */
+static void unpack_ray(const float *r, fray *fr) { ray_setup(fr, r, r + 4, r[3], r[7]); }
+int orc_trace_closest(const orc_scene *s, const float *rays, uint32_t n, uint32_t *geom, uint32_t *prim, float *t) {
+    for (uint32_t i = 0; i < n; i++) {
+        fray r; unpack_ray(rays + 8 * (size_t)i, &r);
+        double th;
+        uint32_t tri = bvh_closest(s, &r, &th);
+        if (tri == UINT32_MAX) { geom[i] = UINT32_MAX; prim[i] = UINT32_MAX; t[i] = INFINITY; }
+        else { geom[i] = s->tri_geom[tri]; prim[i] = tri - s->geom_first[geom[i]]; t[i] = (float)th; }
+    }
+    return 0;
+}
+int orc_trace_any(const orc_scene *s, const float *rays, uint32_t n, uint8_t *occ) {
+    for (uint32_t i = 0; i < n; i++) {
+        fray r; unpack_ray(rays + 8 * (size_t)i, &r);
+        occ[i] = (uint8_t)bvh_any(s, &r);
+    }
+    return 0;
+}
+#define NEAR_EPS 1e-5
+int orc_trace_closest_brute(const orc_scene *s, const float *rays, uint32_t n, uint32_t *geom, uint32_t *prim,
+                            float *t, float *second_t, float *margin) {
+    for (uint32_t i = 0; i < n; i++) {
+        fray r; unpack_ray(rays + 8 * (size_t)i, &r);
+        double best = INFINITY, bm = -1.0; uint32_t bt = UINT32_MAX;
+        for (uint32_t k = 0; k < s->n_tris; k++) {
+            double th, m, ta;
+            if (tri_isect(&r, tri_v(s, k, 0), tri_v(s, k, 1), tri_v(s, k, 2), &th, &m, &ta)) {
+                if (th < best) { best = th; bt = k; bm = m; }
+            }
+        }
+        double sec = INFINITY;
+        for (uint32_t k = 0; k < s->n_tris; k++) {
+            if (k == bt) continue;
+            double th, m, ta;
+            tri_isect(&r, tri_v(s, k, 0), tri_v(s, k, 1), tri_v(s, k, 2), &th, &m, &ta);
+            if (m >= -NEAR_EPS && ta > r.tmin * (1 - NEAR_EPS) && ta <= r.tmax * (1 + NEAR_EPS) && ta < sec) sec = ta;
+        }
+        if (bt == UINT32_MAX) { geom[i] = UINT32_MAX; prim[i] = UINT32_MAX; t[i] = INFINITY; margin[i] = -1.0f; }
+        else { geom[i] = s->tri_geom[bt]; prim[i] = bt - s->geom_first[geom[i]]; t[i] = (float)best; margin[i] = (float)bm; }
+        second_t[i] = (float)sec;
+    }
+    return 0;
+}
+int orc_trace_any_brute(const orc_scene *s, const float *rays, uint32_t n, uint8_t *occ, uint8_t *amb) {
+    for (uint32_t i = 0; i < n; i++) {
+        fray r; unpack_ray(rays + 8 * (size_t)i, &r);
+        int hit = 0, a = 0;
+        for (uint32_t k = 0; k < s->n_tris; k++) {
+            double th, m, ta;
+            int h = tri_isect(&r, tri_v(s, k, 0), tri_v(s, k, 1), tri_v(s, k, 2), &th, &m, &ta);
+            if (h) {
+                if (m >= NEAR_EPS && th > r.tmin * (1 + NEAR_EPS) + NEAR_EPS * 1e-3 &&
+                    th < r.tmax * (1 - NEAR_EPS)) { hit = 1; }
+                else a = 1;
+            } else if (m >= -NEAR_EPS && ta > r.tmin * (1 - NEAR_EPS) - 1e-7 && ta <= r.tmax * (1 + NEAR_EPS)) {
+                a = 1;
+            }
+        }
+        occ[i] = (uint8_t)(hit || a);
+        amb[i] = (uint8_t)(!hit && a);
+    }
+    return 0;
+}

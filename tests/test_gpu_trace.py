@@ -1,0 +1,172 @@
+"""GPU traversal parity: closest-hit and any-hit against the FP64 oracle.
+
+Bar: hit (geomID, primID) bit-exact with the oracle, misses identical, t within
+the FP32 error of the watertight test, excluding documented ties:
+  - another triangle hits, or nearly hits (min barycentric >= -1e-5), within
+    a relative 1e-5 of the closest t (coplanar / shared-edge candidates), or
+  - the closest hit lies within barycentric 1e-5 of an edge;
+for a miss, a near hit anywhere on the ray.  t tolerance: |dt| <= 1e-5 t +
+4e-7 C, with C the largest coordinate magnitude (the FP32 rounding of
+vertex - origin, a few ulp of C, dominates for short rays from surfaces)."""
+import numpy as np
+import pytest
+
+import oracle
+import wurstrenderer_amd as wr
+from wurstrenderer_amd import scenes
+
+pytestmark = pytest.mark.gpu
+INV = 0xFFFFFFFF
+
+
+def _ties(ot, sec, mar, og):
+    near_second = np.isfinite(sec) & (sec <= np.where(np.isfinite(ot), ot * (1 + 1e-5), np.inf))
+    edge = (og != INV) & (mar < 1e-5)
+    return near_second | edge
+
+
+def _check_closest(gs, osc, rays, sc, max_tie_frac=0.01):
+    print(f"closest parity: {sc.name} rays={len(rays)}", flush=True)
+    g, p, t = gs.trace_closest(rays)
+    og, op, ot, sec, mar = osc.trace_closest(rays, brute=True)
+    tie = _ties(ot, sec, mar, og)
+    ok = ~tie
+    bad = ok & ((g != og) | (p != op))
+    assert bad.sum() == 0, (np.nonzero(bad)[0][:10], g[bad][:5], og[bad][:5])
+    hit = ok & (og != INV)
+    C = float(np.abs(sc.verts).max())
+    tol = 1e-5 * ot[hit] + 4e-7 * C
+    assert (np.abs(t[hit] - ot[hit]) <= tol).all()
+    assert tie.mean() <= max_tie_frac, tie.mean()
+    assert np.isinf(t[ok & (og == INV)]).all()
+    return hit.mean()
+
+
+def test_closest_soup(gpu_device):
+    sc = scenes.triangle_soup(3000, seed=11)
+    gs, osc = wr.GpuScene(gpu_device, sc), oracle.OracleScene(sc)
+    rays = np.concatenate([scenes.random_rays(sc, 20000, seed=1), scenes.random_rays(sc, 10000, seed=2, inside=True)])
+    frac = _check_closest(gs, osc, rays, sc)
+    assert 0.3 < frac < 0.98
+
+
+def test_closest_cornell_camera_rays(gpu_device):
+    sc = scenes.cornell(1)
+    gs, osc = wr.GpuScene(gpu_device, sc), oracle.OracleScene(sc)
+    rays = scenes.camera_rays(sc, spp=1, seed=3)
+    frac = _check_closest(gs, osc, rays, sc)
+    assert frac > 0.99  # the box is closed except the front
+
+
+def test_closest_icospheres_random(gpu_device):
+    sc = scenes.icospheres(2, n_spheres=12, level=3, resolution=(64, 36))
+    gs, osc = wr.GpuScene(gpu_device, sc), oracle.OracleScene(sc)
+    rays = np.concatenate([scenes.random_rays(sc, 6000, seed=4), scenes.random_rays(sc, 4000, seed=5, inside=True)])
+    _check_closest(gs, osc, rays, sc)
+
+
+def test_closest_full_scene_camera_rays(gpu_device):
+    """The 1.31M-triangle workload: all primary rays of a 480x270 frame against
+    the oracle's BVH, and a brute-force subset with tie classification."""
+    sc = scenes.icospheres(2, resolution=(480, 270))
+    gs, osc = wr.GpuScene(gpu_device, sc), oracle.OracleScene(sc)
+    rays = scenes.camera_rays(sc, spp=1, seed=6)
+    print("full scene: gpu", flush=True)
+    g, p, t = gs.trace_closest(rays)
+    og, op, ot = osc.trace_closest(rays)
+    mism = (g != og) | (p != op)
+    assert mism.mean() < 1e-4, mism.sum()
+    same = ~mism & (og != INV)
+    assert (np.abs(t[same] - ot[same]) <= 1e-5 * ot[same] + 4e-7 * float(np.abs(sc.verts).max())).all()
+    sub = rays[np.random.default_rng(0).choice(len(rays), 300, replace=False)]
+    _check_closest(gs, osc, sub, sc, max_tie_frac=0.05)
+
+
+def test_any_hit_soup_and_cornell_shadow_rays(gpu_device):
+    sc = scenes.triangle_soup(3000, seed=12)
+    gs, osc = wr.GpuScene(gpu_device, sc), oracle.OracleScene(sc)
+    rays = np.concatenate([scenes.random_rays(sc, 20000, seed=7), scenes.random_rays(sc, 10000, seed=8, inside=True)])
+    rays[:, 7] = np.random.default_rng(1).uniform(0.05, 10.0, len(rays))
+    occ = gs.trace_any(rays)
+    oocc, amb = osc.trace_any(rays, brute=True)
+    ok = amb == 0
+    assert (occ[ok] == oocc[ok]).all()
+    assert 0.2 < oocc[ok].mean() < 0.9
+    assert amb.mean() < 0.01
+    # NEE-like rays in the Cornell box: surface points to points on the light
+    sc = scenes.cornell(1)
+    gs, osc = wr.GpuScene(gpu_device, sc), oracle.OracleScene(sc)
+    cam = scenes.camera_rays(sc, spp=1, seed=9)[::7]
+    g, p, t = osc.trace_closest(cam)
+    hit = g != INV
+    pos = cam[hit, :3].astype(np.float64) + cam[hit, 4:7] * t[hit, None]
+    rng = np.random.default_rng(2)
+    lp = np.stack([rng.uniform(-0.25, 0.25, hit.sum()), np.full(hit.sum(), 0.999), rng.uniform(-0.25, 0.25, hit.sum())], 1)
+    d = lp - pos
+    ln = np.linalg.norm(d, axis=1)
+    sh = np.zeros((hit.sum(), 8), np.float32)
+    sh[:, :3] = pos
+    sh[:, 3] = 1e-4
+    sh[:, 4:7] = d / ln[:, None]
+    sh[:, 7] = ln - 1e-4
+    occ = gs.trace_any(sh)
+    oocc, amb = osc.trace_any(sh, brute=True)
+    ok = amb == 0
+    assert (occ[ok] == oocc[ok]).all()
+    assert 0.05 < oocc[ok].mean() < 0.95
+    assert amb.mean() < 0.02
+
+
+def test_watertight_closed_icosphere_far_from_origin(gpu_device):
+    """Rays from inside a closed mesh placed far from the origin must all hit
+    (the property of Embree's WatertightTest, tutorials/verify/verify.cpp:2711-2792)."""
+    v, f = scenes.icosphere(4)
+    b = scenes.SceneBuilder()
+    m = b.material(scenes.LAMBERT, rho_d=(0.5, 0.5, 0.5))
+    centre = np.array([1000.0, -500.0, 250.0])
+    b.mesh(v * 2.0 + centre, f, material=m)
+    b.mesh(*scenes.quad(centre + [0, 50, 0], (1, 0, 0), (0, 0, 1)), light_L=(1, 1, 1))
+    sc = b.build(dict(pos=(0, 0, 0), lookat=(1, 0, 0), fovy=40.0, resolution=(8, 8)), "ico_far")
+    gs = wr.GpuScene(gpu_device, sc)
+    rng = np.random.default_rng(5)
+    n = 200000
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, :3] = centre + rng.uniform(-0.5, 0.5, size=(n, 3))
+    rays[:, 4:7] = d
+    rays[:, 7] = np.inf
+    g, p, t = gs.trace_closest(rays)
+    assert (g == 0).all(), (g != 0).sum()
+    # rays through vertices and edges of the mesh: aim exactly at vertex positions
+    verts = (v * 2.0 + centre).astype(np.float32)
+    o = np.repeat(centre[None].astype(np.float32), len(verts), 0)
+    dd = verts.astype(np.float64) - centre
+    dd /= np.linalg.norm(dd, axis=1, keepdims=True)
+    r2 = np.zeros((len(verts), 8), np.float32)
+    r2[:, :3] = o
+    r2[:, 4:7] = dd
+    r2[:, 7] = np.inf
+    g2, _, _ = gs.trace_closest(r2)
+    assert (g2 == 0).all()
+
+
+def test_device_resident_entry_point_matches_host_one(gpu_device):
+    import ctypes as C
+    import torch
+    sc = scenes.triangle_soup(2000, seed=13)
+    gs = wr.GpuScene(gpu_device, sc)
+    rays = scenes.random_rays(sc, 50000, seed=3)
+    g, p, t = gs.trace_closest(rays)
+    d_rays = torch.from_numpy(rays).cuda()
+    d_hits = torch.zeros((len(rays), 2), dtype=torch.int32, device="cuda")
+    ms = C.c_float()
+    wr._check(wr.lib().wr_trace_closest_device(gs.h, d_rays.data_ptr(), len(rays), d_hits.data_ptr(), C.byref(ms)))
+    hits = d_hits.cpu().numpy().view(np.uint32)
+    t_dev = hits[:, 1].view(np.float32)
+    np.testing.assert_array_equal(np.isinf(t_dev), np.isinf(t))
+    np.testing.assert_array_equal(t_dev[np.isfinite(t)], t[np.isfinite(t)])
+    assert ms.value > 0
+    d_occ = torch.zeros(len(rays), dtype=torch.uint8, device="cuda")
+    wr._check(wr.lib().wr_trace_any_device(gs.h, d_rays.data_ptr(), len(rays), d_occ.data_ptr(), None))
+    np.testing.assert_array_equal(d_occ.cpu().numpy(), gs.trace_any(rays))

@@ -1,0 +1,243 @@
+"""MI355X-native `nextevent_path_tracer`: Python binding of libwurst_amd.so.
+
+The library exposes the C-ABI declared in include/wurst_amd.h.  This module is
+the ctypes stub over that ABI (the same binding a ctypes user of the reference
+path would write); the compute runs in the HIP kernels of the library.  There
+is no CPU fallback: if the library is missing or no GPU is present the calls
+raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from . import scenes  # noqa: F401
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libwurst_amd.so")
+
+WR_OK = 0
+WR_INVALID_ID = 0xFFFFFFFF
+
+
+class WrError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"wurst_amd error {code}: {msg}")
+        self.code = code
+
+
+class _Camera(C.Structure):
+    _fields_ = [("pos", C.c_float * 3), ("lookat", C.c_float * 3), ("up", C.c_float * 3),
+                ("fovy_deg", C.c_float), ("width", C.c_int32), ("height", C.c_int32),
+                ("aperture_radius", C.c_float), ("focal_distance", C.c_float)]
+
+
+class _SceneDesc(C.Structure):
+    _fields_ = [("verts", C.c_void_p), ("n_verts", C.c_uint32), ("tris", C.c_void_p), ("n_tris", C.c_uint32),
+                ("tri_geom", C.c_void_p), ("n_geoms", C.c_uint32), ("geom_material", C.c_void_p),
+                ("geom_light", C.c_void_p), ("materials", C.c_void_p), ("n_materials", C.c_uint32),
+                ("lights", C.c_void_p), ("n_lights", C.c_uint32), ("camera", _Camera)]
+
+
+class RenderParams(C.Structure):
+    _fields_ = [("spp", C.c_int32), ("num_path_vertices", C.c_int32), ("seed", C.c_uint32),
+                ("use_mis", C.c_int32), ("first_sample", C.c_int32), ("max_paths_in_flight", C.c_int32),
+                ("profile", C.c_int32), ("count_traversal", C.c_int32)]
+
+
+class RenderStats(C.Structure):
+    _fields_ = [("paths", C.c_uint64), ("extend_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
+                ("wall_ms", C.c_double), ("extend_ms", C.c_double), ("shadow_ms", C.c_double),
+                ("shade_ms", C.c_double), ("generate_ms", C.c_double), ("accumulate_ms", C.c_double),
+                ("extend_launches", C.c_uint64), ("shadow_launches", C.c_uint64),
+                ("extend_nodes", C.c_uint64), ("extend_tris", C.c_uint64),
+                ("shadow_nodes", C.c_uint64), ("shadow_tris", C.c_uint64)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+# name -> (restype, argtypes); every symbol declared in include/wurst_amd.h
+P = C.c_void_p
+EXPORTS = {
+    "wr_last_error": (C.c_char_p, []),
+    "wr_version": (C.c_char_p, []),
+    "wr_device_create": (C.c_int, [C.c_int, C.POINTER(P)]),
+    "wr_device_destroy": (None, [P]),
+    "wr_scene_validate": (C.c_int, [C.POINTER(_SceneDesc)]),
+    "wr_bvh_selftest": (C.c_int, [C.POINTER(_SceneDesc), C.POINTER(C.c_uint64), C.POINTER(C.c_uint32),
+                                  C.POINTER(C.c_uint32)]),
+    "wr_scene_create": (C.c_int, [P, C.POINTER(_SceneDesc), C.POINTER(P)]),
+    "wr_scene_destroy": (None, [P]),
+    "wr_scene_info": (C.c_int, [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint32),
+                                C.POINTER(C.c_uint64)]),
+    "wr_film_create": (C.c_int, [P, C.POINTER(P)]),
+    "wr_film_destroy": (None, [P]),
+    "wr_film_clear": (C.c_int, [P]),
+    "wr_film_read": (C.c_int, [P, P, P]),
+    "wr_render_nee": (C.c_int, [P, P, C.POINTER(RenderParams), C.POINTER(RenderStats)]),
+    "wr_trace_closest": (C.c_int, [P, P, C.c_uint32, P, P, P]),
+    "wr_trace_any": (C.c_int, [P, P, C.c_uint32, P]),
+    "wr_trace_closest_device": (C.c_int, [P, P, C.c_uint32, P, C.POINTER(C.c_float)]),
+    "wr_trace_any_device": (C.c_int, [P, P, C.c_uint32, P, C.POINTER(C.c_float)]),
+}
+
+_lib = None
+
+
+def build() -> str:
+    """Compile libwurst_amd.so in-tree (hipcc, gfx950)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return LIB_PATH
+
+
+def lib():
+    """Load libwurst_amd.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build() or make -C wurstrenderer_amd")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in EXPORTS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc: int):
+    if rc != WR_OK:
+        raise WrError(rc, lib().wr_last_error().decode())
+
+
+class Device:
+    def __init__(self, ordinal: int = 0):
+        h = C.c_void_p()
+        _check(lib().wr_device_create(ordinal, C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().wr_device_destroy(self.h)
+            self.h = None
+
+
+def scene_desc(sc: scenes.Scene):
+    """Builds the C descriptor; returns (desc, keepalive arrays)."""
+    keep = [np.ascontiguousarray(a) for a in
+            (sc.verts, sc.tris, sc.tri_geom, sc.geom_material, sc.geom_light, sc.materials, sc.lights)]
+    v, t, tg, gm, gl, mats, lights = keep
+    d = _SceneDesc()
+    d.verts, d.n_verts = v.ctypes.data, v.shape[0]
+    d.tris, d.n_tris = t.ctypes.data, t.shape[0]
+    d.tri_geom, d.n_geoms = tg.ctypes.data, gm.shape[0]
+    d.geom_material, d.geom_light = gm.ctypes.data, gl.ctypes.data
+    d.materials, d.n_materials = mats.ctypes.data, mats.shape[0]
+    d.lights, d.n_lights = lights.ctypes.data, lights.shape[0]
+    C.memmove(C.byref(d.camera), sc.camera.tobytes(), C.sizeof(_Camera))
+    return d, keep
+
+
+def validate(sc: scenes.Scene) -> None:
+    """Host-side scene checks (raises WrError)."""
+    d, keep = scene_desc(sc)
+    _check(lib().wr_scene_validate(C.byref(d)))
+
+
+def bvh_selftest(sc: scenes.Scene) -> dict:
+    """Builds the BVH on the host and verifies its invariants (raises WrError)."""
+    d, keep = scene_desc(sc)
+    nn, ms, dp = C.c_uint64(), C.c_uint32(), C.c_uint32()
+    _check(lib().wr_bvh_selftest(C.byref(d), C.byref(nn), C.byref(ms), C.byref(dp)))
+    return dict(nodes=nn.value, max_stack=ms.value, depth=dp.value)
+
+
+class GpuScene:
+    """A scene resident in HBM (BVH, triangles, materials, lights, camera)."""
+
+    def __init__(self, device: Device, sc: scenes.Scene):
+        d, keep = scene_desc(sc)
+        h = C.c_void_p()
+        _check(lib().wr_scene_create(device.h, C.byref(d), C.byref(h)))
+        self.h = h
+        self.device = device
+        self.W, self.H = sc.resolution
+        del keep
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().wr_scene_destroy(self.h)
+            self.h = None
+
+    def info(self) -> dict:
+        nn, nt, nb = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        ms = C.c_uint32()
+        _check(lib().wr_scene_info(self.h, C.byref(nn), C.byref(nt), C.byref(ms), C.byref(nb)))
+        return dict(nodes=nn.value, tris=nt.value, max_stack=ms.value, device_bytes=nb.value)
+
+    def trace_closest(self, rays: np.ndarray):
+        rays = np.ascontiguousarray(rays, np.float32)
+        n = rays.shape[0]
+        geom = np.zeros(n, np.uint32)
+        prim = np.zeros(n, np.uint32)
+        t = np.zeros(n, np.float32)
+        _check(lib().wr_trace_closest(self.h, rays.ctypes.data, n, geom.ctypes.data, prim.ctypes.data, t.ctypes.data))
+        return geom, prim, t
+
+    def trace_any(self, rays: np.ndarray):
+        rays = np.ascontiguousarray(rays, np.float32)
+        n = rays.shape[0]
+        occ = np.zeros(n, np.uint8)
+        _check(lib().wr_trace_any(self.h, rays.ctypes.data, n, occ.ctypes.data))
+        return occ
+
+
+class Film:
+    """FP64 base + splat buffers in HBM (Film, src/common/film.h:35-68)."""
+
+    def __init__(self, scene: GpuScene):
+        h = C.c_void_p()
+        _check(lib().wr_film_create(scene.h, C.byref(h)))
+        self.h = h
+        self.scene = scene
+        self.spp = 0
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().wr_film_destroy(self.h)
+            self.h = None
+
+    def clear(self):
+        _check(lib().wr_film_clear(self.h))
+        self.spp = 0
+
+    def read(self):
+        W, H = self.scene.W, self.scene.H
+        base = np.zeros((H, W, 3), np.float64)
+        splat = np.zeros((H, W, 3), np.float64)
+        _check(lib().wr_film_read(self.h, base.ctypes.data, splat.ctypes.data))
+        return base, splat
+
+    def image(self):
+        """Film::getImage: base/spp + splat/(spp*W*H) (film.h:78-90, pathnextevent.h:154-157)."""
+        base, splat = self.read()
+        W, H = self.scene.W, self.scene.H
+        return base / self.spp + splat / (self.spp * W * H)
+
+
+def render_nee(scene: GpuScene, film: Film, spp: int, num_path_vertices: int = 10, seed: int = 0,
+               use_mis: bool = True, first_sample: int | None = None, max_paths_in_flight: int = 0,
+               profile: bool = False, count_traversal: bool = False) -> RenderStats:
+    """PathNeeRenderer::render on the GPU: adds `spp` samples per pixel into `film`."""
+    if first_sample is None:
+        first_sample = film.spp
+    p = RenderParams(spp, num_path_vertices, seed, int(bool(use_mis)), first_sample, max_paths_in_flight,
+                     int(bool(profile)), int(bool(count_traversal)))
+    st = RenderStats()
+    _check(lib().wr_render_nee(scene.h, film.h, C.byref(p), C.byref(st)))
+    film.spp += spp
+    return st

@@ -1,0 +1,348 @@
+// wr_bvh.cpp -- host BVH build: binned SAH over triangle bounds (BVH2), then
+// collapsed to the 4-wide, 128-byte-node layout of wr_layout.h.
+//
+// Replaces rtcCommitScene's BVH8/Triangle4 SAH build (reference:
+// src/accel/embreeaccel.h:466-469; Embree kernels/common/scene.cpp:120-140).
+// The hit semantics do not depend on the tree: any BVH gives the same closest
+// hit, so the tree shape is a performance choice only.
+#include "wr_layout.h"
+
+#include <algorithm>
+#include <atomic>
+#include <cfloat>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <future>
+#include <new>
+#include <vector>
+
+namespace wr {
+namespace {
+
+constexpr int kBins = 32;
+constexpr float kTraversalCost = 1.0f;
+constexpr float kTriangleCost = 1.0f;
+constexpr uint32_t kParallelThreshold = 65536;
+
+struct Ref {
+    float lo[3], hi[3], c[3];
+    uint32_t tri;
+};
+
+struct B2 {
+    float lo[3], hi[3];
+    uint32_t left;         // internal: index of left child (right = left + 1)
+    uint32_t first, count; // leaf range in refs (count > 0)
+};
+
+struct Aabb {
+    float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX};
+    float hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    void grow(const float *l, const float *h) {
+        for (int k = 0; k < 3; k++) {
+            lo[k] = std::min(lo[k], l[k]);
+            hi[k] = std::max(hi[k], h[k]);
+        }
+    }
+    void grow(const Aabb &b) { grow(b.lo, b.hi); }
+    float half_area() const {
+        float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+        if (dx < 0 || dy < 0 || dz < 0) return 0.f;
+        return dx * dy + dy * dz + dz * dx;
+    }
+};
+
+struct Builder {
+    std::vector<Ref> refs;
+    std::vector<B2> nodes;
+    std::atomic<uint32_t> n_nodes{0};
+
+    uint32_t alloc(uint32_t k) { return n_nodes.fetch_add(k); }
+
+    void make_leaf(uint32_t ni, uint32_t begin, uint32_t end, const Aabb &box) {
+        B2 &n = nodes[ni];
+        std::memcpy(n.lo, box.lo, sizeof n.lo);
+        std::memcpy(n.hi, box.hi, sizeof n.hi);
+        n.left = 0;
+        n.first = begin;
+        n.count = end - begin;
+    }
+
+    void build(uint32_t ni, uint32_t begin, uint32_t end, int depth) {
+        Aabb box, cbox;
+        for (uint32_t i = begin; i < end; i++) {
+            box.grow(refs[i].lo, refs[i].hi);
+            cbox.grow(refs[i].c, refs[i].c);
+        }
+        const uint32_t n = end - begin;
+        if (n == 1) { make_leaf(ni, begin, end, box); return; }
+        // binned SAH
+        float best_cost = FLT_MAX;
+        int best_axis = -1, best_split = -1;
+        for (int k = 0; k < 3; k++) {
+            float ext = cbox.hi[k] - cbox.lo[k];
+            if (!(ext > 0.f)) continue;
+            float scale = kBins / ext;
+            Aabb bb[kBins];
+            uint32_t bc[kBins] = {0};
+            for (uint32_t i = begin; i < end; i++) {
+                int b = std::min(kBins - 1, std::max(0, int((refs[i].c[k] - cbox.lo[k]) * scale)));
+                bc[b]++;
+                bb[b].grow(refs[i].lo, refs[i].hi);
+            }
+            float right_area[kBins];
+            uint32_t right_count[kBins];
+            Aabb acc;
+            uint32_t cnt = 0;
+            for (int b = kBins - 1; b > 0; b--) {
+                acc.grow(bb[b]);
+                cnt += bc[b];
+                right_area[b] = acc.half_area();
+                right_count[b] = cnt;
+            }
+            Aabb lacc;
+            uint32_t lcnt = 0;
+            for (int b = 0; b < kBins - 1; b++) {
+                lacc.grow(bb[b]);
+                lcnt += bc[b];
+                if (lcnt == 0 || right_count[b + 1] == 0) continue;
+                float cost = lacc.half_area() * lcnt + right_area[b + 1] * right_count[b + 1];
+                if (cost < best_cost) { best_cost = cost; best_axis = k; best_split = b; }
+            }
+        }
+        const float parent_area = box.half_area();
+        const float leaf_cost = kTriangleCost * n;
+        const float split_cost = parent_area > 0.f ? kTraversalCost + kTriangleCost * best_cost / parent_area : FLT_MAX;
+        if (n <= (uint32_t)kMaxLeafTris && (best_axis < 0 || leaf_cost <= split_cost)) {
+            make_leaf(ni, begin, end, box);
+            return;
+        }
+        uint32_t mid;
+        if (best_axis < 0) {
+            mid = begin + n / 2;  // coincident centroids: split by count
+        } else {
+            const int k = best_axis;
+            const float scale = kBins / (cbox.hi[k] - cbox.lo[k]);
+            auto it = std::partition(refs.begin() + begin, refs.begin() + end, [&](const Ref &r) {
+                int b = std::min(kBins - 1, std::max(0, int((r.c[k] - cbox.lo[k]) * scale)));
+                return b <= best_split;
+            });
+            mid = uint32_t(it - refs.begin());
+            if (mid == begin || mid == end) mid = begin + n / 2;
+        }
+        const uint32_t l = alloc(2);
+        B2 &nd = nodes[ni];
+        std::memcpy(nd.lo, box.lo, sizeof nd.lo);
+        std::memcpy(nd.hi, box.hi, sizeof nd.hi);
+        nd.left = l;
+        nd.first = 0;
+        nd.count = 0;
+        if (n > kParallelThreshold && depth < 8) {
+            auto fut = std::async(std::launch::async, [=] { build(l, begin, mid, depth + 1); });
+            build(l + 1, mid, end, depth + 1);
+            fut.get();
+        } else {
+            build(l, begin, mid, depth + 1);
+            build(l + 1, mid, end, depth + 1);
+        }
+    }
+};
+
+struct Collapser {
+    const std::vector<B2> &b2;
+    std::vector<Node4> out;
+    uint32_t max_stack = 0, max_depth = 0;
+
+    explicit Collapser(const std::vector<B2> &b) : b2(b) {}
+
+    static float area(const B2 &n) {
+        float dx = n.hi[0] - n.lo[0], dy = n.hi[1] - n.lo[1], dz = n.hi[2] - n.lo[2];
+        return dx * dy + dy * dz + dz * dx;
+    }
+
+    // stack_before: entries that may sit on the stack when this node is visited
+    uint32_t collapse(uint32_t root_children[], int nc, uint32_t stack_before, uint32_t depth) {
+        uint32_t kids[4];
+        int k = 0;
+        for (int i = 0; i < nc; i++) kids[k++] = root_children[i];
+        for (;;) {
+            if (k == 4) break;
+            int pick = -1;
+            float best = -1.f;
+            for (int i = 0; i < k; i++) {
+                const B2 &n = b2[kids[i]];
+                if (n.count == 0 && area(n) > best) { best = area(n); pick = i; }
+            }
+            if (pick < 0) break;
+            uint32_t l = b2[kids[pick]].left;
+            kids[pick] = l;
+            kids[k++] = l + 1;
+        }
+        const uint32_t me = uint32_t(out.size());
+        out.emplace_back();
+        Node4 nd;
+        for (int i = 0; i < 4; i++) {
+            nd.lo_x[i] = nd.lo_y[i] = nd.lo_z[i] = FLT_MAX;
+            nd.hi_x[i] = nd.hi_y[i] = nd.hi_z[i] = -FLT_MAX;
+            nd.child[i] = kEmptyChild;
+            nd.count[i] = 0;
+        }
+        // worst case: all k children hit; k-1 stay on the stack while descending
+        const uint32_t stack_here = stack_before + uint32_t(k);
+        max_stack = std::max(max_stack, stack_here);
+        max_depth = std::max(max_depth, depth + 1);
+        for (int i = 0; i < k; i++) {
+            const B2 &c = b2[kids[i]];
+            nd.lo_x[i] = c.lo[0]; nd.lo_y[i] = c.lo[1]; nd.lo_z[i] = c.lo[2];
+            nd.hi_x[i] = c.hi[0]; nd.hi_y[i] = c.hi[1]; nd.hi_z[i] = c.hi[2];
+            if (c.count > 0) {
+                nd.child[i] = kLeafBit | ((c.count - 1u) << 28) | c.first;
+                nd.count[i] = c.count;
+            } else {
+                uint32_t gk[2] = {c.left, c.left + 1};
+                nd.child[i] = collapse(gk, 2, stack_before + uint32_t(k) - 1, depth + 1);
+                nd.count[i] = 0;
+            }
+        }
+        out[me] = nd;
+        return me;
+    }
+};
+
+}  // namespace
+
+bool build_bvh4(const float *verts, uint32_t nv, const uint32_t *tris, uint32_t nt, const uint32_t *tri_geom,
+                BvhBuildResult *res) {
+    *res = BvhBuildResult{};
+    Builder b;
+    try {
+        b.refs.resize(nt);
+        b.nodes.resize(nt > 0 ? 2 * size_t(nt) - 1 : 1);
+    } catch (const std::bad_alloc &) {
+        return false;
+    }
+    for (uint32_t t = 0; t < nt; t++) {
+        Ref &r = b.refs[t];
+        for (int k = 0; k < 3; k++) { r.lo[k] = FLT_MAX; r.hi[k] = -FLT_MAX; }
+        for (int j = 0; j < 3; j++) {
+            const float *p = verts + 3 * size_t(tris[3 * size_t(t) + j]);
+            for (int k = 0; k < 3; k++) {
+                r.lo[k] = std::min(r.lo[k], p[k]);
+                r.hi[k] = std::max(r.hi[k], p[k]);
+            }
+        }
+        for (int k = 0; k < 3; k++) r.c[k] = 0.5f * (r.lo[k] + r.hi[k]);
+        r.tri = t;
+    }
+    std::vector<Node4> nodes;
+    uint32_t max_stack = 1, depth = 1;
+    if (nt == 0) {
+        Node4 nd;
+        for (int i = 0; i < 4; i++) {
+            nd.lo_x[i] = nd.lo_y[i] = nd.lo_z[i] = FLT_MAX;
+            nd.hi_x[i] = nd.hi_y[i] = nd.hi_z[i] = -FLT_MAX;
+            nd.child[i] = kEmptyChild;
+            nd.count[i] = 0;
+        }
+        nodes.push_back(nd);
+    } else {
+        const uint32_t root = b.alloc(1);
+        b.build(root, 0, nt, 0);
+        b.nodes.resize(b.n_nodes.load());
+        Collapser c(b.nodes);
+        c.out.reserve(b.nodes.size() / 2 + 1);
+        uint32_t rc[2];
+        int nc;
+        if (b.nodes[root].count > 0) { rc[0] = root; nc = 1; }
+        else { rc[0] = b.nodes[root].left; rc[1] = rc[0] + 1; nc = 2; }
+        c.collapse(rc, nc, 0, 0);
+        nodes.swap(c.out);
+        max_stack = c.max_stack;
+        depth = c.max_depth;
+    }
+    res->n_nodes = uint32_t(nodes.size());
+    res->nodes = static_cast<Node4 *>(std::aligned_alloc(128, sizeof(Node4) * nodes.size()));
+    const size_t tri_bytes = (sizeof(TriRecord) * std::max<size_t>(nt, 1) + 127) / 128 * 128;
+    res->tris = static_cast<TriRecord *>(std::aligned_alloc(128, tri_bytes));
+    if (!res->nodes || !res->tris) { free_bvh(res); return false; }
+    std::memcpy(res->nodes, nodes.data(), sizeof(Node4) * nodes.size());
+    for (uint32_t i = 0; i < nt; i++) {
+        const uint32_t t = b.refs[i].tri;
+        TriRecord &r = res->tris[i];
+        const float *p0 = verts + 3 * size_t(tris[3 * size_t(t) + 0]);
+        const float *p1 = verts + 3 * size_t(tris[3 * size_t(t) + 1]);
+        const float *p2 = verts + 3 * size_t(tris[3 * size_t(t) + 2]);
+        for (int k = 0; k < 3; k++) { r.v0[k] = p0[k]; r.v1[k] = p1[k]; r.v2[k] = p2[k]; }
+        r.orig = t;
+        r.geom = tri_geom[t];
+        r.pad = 0;
+    }
+    res->n_tris = nt;
+    res->max_stack = max_stack;
+    res->depth = depth;
+    return true;
+}
+
+const char *verify_bvh4(const BvhBuildResult &r, const float *verts, const uint32_t *tris, uint32_t nt) {
+    if (r.n_tris != nt) return "leaf triangle count differs from the input";
+    std::vector<uint8_t> seen(nt, 0);
+    for (uint32_t i = 0; i < nt; i++) {
+        const uint32_t o = r.tris[i].orig;
+        if (o >= nt || seen[o]) return "triangle missing or duplicated in leaf order";
+        seen[o] = 1;
+        const float *p[3] = {verts + 3 * size_t(tris[3 * size_t(o)]), verts + 3 * size_t(tris[3 * size_t(o) + 1]),
+                             verts + 3 * size_t(tris[3 * size_t(o) + 2])};
+        const float *q[3] = {r.tris[i].v0, r.tris[i].v1, r.tris[i].v2};
+        for (int j = 0; j < 3; j++)
+            for (int k = 0; k < 3; k++)
+                if (p[j][k] != q[j][k]) return "leaf triangle vertices differ from the input";
+    }
+    std::vector<uint32_t> leaf_refs(nt, 0);
+    std::vector<uint32_t> node_refs(r.n_nodes, 0);
+    for (uint32_t n = 0; n < r.n_nodes; n++) {
+        const Node4 &nd = r.nodes[n];
+        for (int c = 0; c < 4; c++) {
+            const uint32_t ch = nd.child[c];
+            if (ch == kEmptyChild) continue;
+            const float lo[3] = {nd.lo_x[c], nd.lo_y[c], nd.lo_z[c]}, hi[3] = {nd.hi_x[c], nd.hi_y[c], nd.hi_z[c]};
+            if (ch & kLeafBit) {
+                const uint32_t first = ch & 0x0FFFFFFFu, cnt = ((ch >> 28) & 7u) + 1u;
+                if (cnt != nd.count[c] || first + cnt > nt) return "bad leaf reference";
+                for (uint32_t i = first; i < first + cnt; i++) {
+                    leaf_refs[i]++;
+                    const float *q[3] = {r.tris[i].v0, r.tris[i].v1, r.tris[i].v2};
+                    for (int j = 0; j < 3; j++)
+                        for (int k = 0; k < 3; k++)
+                            if (q[j][k] < lo[k] || q[j][k] > hi[k]) return "leaf box does not enclose its triangle";
+                }
+            } else {
+                if (ch >= r.n_nodes || ch <= n) return "bad child node index";
+                node_refs[ch]++;
+                const Node4 &cn = r.nodes[ch];
+                for (int cc = 0; cc < 4; cc++) {
+                    if (cn.child[cc] == kEmptyChild) continue;
+                    const float clo[3] = {cn.lo_x[cc], cn.lo_y[cc], cn.lo_z[cc]};
+                    const float chi[3] = {cn.hi_x[cc], cn.hi_y[cc], cn.hi_z[cc]};
+                    for (int k = 0; k < 3; k++)
+                        if (clo[k] < lo[k] || chi[k] > hi[k]) return "child box not enclosed by its parent entry";
+                }
+            }
+        }
+    }
+    for (uint32_t i = 0; i < nt; i++)
+        if (leaf_refs[i] != 1) return "triangle referenced by zero or several leaves";
+    for (uint32_t n = 1; n < r.n_nodes; n++)
+        if (node_refs[n] != 1) return "node referenced by zero or several parents";
+    if (r.max_stack > uint32_t(kStackDepth)) return "stack bound exceeds the traversal stack";
+    return "";
+}
+
+void free_bvh(BvhBuildResult *r) {
+    std::free(r->nodes);
+    std::free(r->tris);
+    r->nodes = nullptr;
+    r->tris = nullptr;
+}
+
+}  // namespace wr

@@ -1,0 +1,322 @@
+// wr_device.h -- device-side math, RNG, sampling and BSDFs (FP32).
+//
+// Each function names the reference code whose semantics it reproduces; the
+// arithmetic is FP32 (the reference is FP64 throughout, README.md:83).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "wr_layout.h"
+
+namespace wr {
+
+constexpr float kPi = 3.14159265358979323846f;
+constexpr float kInvPi = 0.31830988618379067154f;
+constexpr float kRayEps = 1e-4f;                // Ray3::Epsilon (src/common/math/ray.h:8)
+constexpr float kOneMinusEps = 0x1.fffffep-1f;  // CdfTable remap clamp 1 - 1e-12 in FP32 (cdftable.h:41)
+
+enum : int { kLambert = 0, kOrenNayar = 1, kPhong = 2, kMixed = 3 };
+
+struct DMat {  // 48 B
+    int type, twosided;
+    float sigma, n;
+    float rho_d[4];
+    float rho_s[4];
+};
+
+struct DLight {  // 48 B
+    float L[3];
+    float pdf_select;   // CdfTable pdf of this light (weights = |L|_2, diffusearealight.h:21-24)
+    float cdf_select;   // CdfTable cdf
+    float inv_area;
+    float area;
+    uint32_t first_ltri;
+    uint32_t n_ltri;
+    uint32_t pad[3];
+};
+
+struct DCamera {
+    float ox, oy, oz;
+    float xx, xy, xz, yx, yy, yz, zx, zy, zz;  // frame (thinlens.h:46-50)
+    float d;                                     // lens-to-film distance (camera.h:16)
+    int W, H;
+    float invW, invH;
+    float per_pixel_norm, per_film_norm;
+    float aperture, focal;
+};
+
+struct DScene {
+    const Node4 *nodes;
+    const float4 *tris;       // 3 float4 per leaf-order triangle
+    const int2 *geom_info;    // (material, light) per geometry
+    const DMat *mats;
+    const DLight *lights;
+    const float4 *ltris;      // 3 float4 per emitter triangle
+    uint32_t n_lights;
+    DCamera cam;
+};
+
+// ---------------------------------------------------------------- float3 helpers
+__device__ __forceinline__ float3 f3(float x, float y, float z) { return make_float3(x, y, z); }
+__device__ __forceinline__ float3 operator+(float3 a, float3 b) { return f3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ float3 operator-(float3 a, float3 b) { return f3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ float3 operator*(float3 a, float s) { return f3(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ float3 operator*(float3 a, float3 b) { return f3(a.x * b.x, a.y * b.y, a.z * b.z); }
+__device__ __forceinline__ float3 operator-(float3 a) { return f3(-a.x, -a.y, -a.z); }
+__device__ __forceinline__ float dot3(float3 a, float3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ float3 cross3(float3 a, float3 b) {
+    return f3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ float3 normalize3(float3 a) { return a * (1.0f / sqrtf(dot3(a, a))); }
+__device__ __forceinline__ float3 xyz(float4 v) { return f3(v.x, v.y, v.z); }
+__device__ __forceinline__ bool is_black(float3 c) { return c.x == 0.f && c.y == 0.f && c.z == 0.f; }
+
+// ---------------------------------------------------------------- RNG
+// Counter-based: a pure integer function of (seed, pixel, sample, dim); the
+// oracle computes the same integers.  Dimension layout per path: 0-1 light,
+// 2-3 lens, 4-5 pixel jitter, then 6 per bounce (0-1 medium, 2-3 BSDF).
+__device__ __forceinline__ uint32_t mix32(uint32_t x) {
+    x ^= x >> 16; x *= 0x7feb352du;
+    x ^= x >> 15; x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+__device__ __forceinline__ uint32_t path_key(uint32_t seed, uint32_t pixel, uint32_t sample) {
+    uint32_t h = mix32(seed ^ 0xa511e9b3u);
+    h = mix32(h ^ pixel);
+    return mix32(h ^ (sample * 0x9e3779b9u + 0x7f4a7c15u));
+}
+__device__ __forceinline__ float u01(uint32_t key, uint32_t dim) {
+    return float(mix32(key + dim * 0x9e3779b9u + 0x632be5abu) >> 8) * 0x1p-24f;
+}
+enum : uint32_t { kDimLight = 0, kDimLens = 2, kDimPixel = 4, kDimBounce0 = 6, kDimsPerBounce = 6, kBdBsdf = 2 };
+
+// ---------------------------------------------------------------- frames and warps
+struct Frame { float3 x, y, z; };
+// CoordFrame3(normal): Duff et al. 2017 ONB with y = normal (coordframe.h:46-60)
+__device__ __forceinline__ Frame make_frame(float3 n) {
+    Frame f;
+    f.y = n;
+    float sign = copysignf(1.0f, n.y);
+    float a = -1.0f / (sign + n.y);
+    float b = n.z * n.x * a;
+    f.x = f3(sign + n.x * n.x * a, -n.x, b);
+    f.z = f3(sign * b, -sign * n.z, 1.0f + sign * n.z * n.z * a);
+    return f;
+}
+__device__ __forceinline__ float3 to_local(const Frame &f, float3 v) { return f3(dot3(v, f.x), dot3(v, f.y), dot3(v, f.z)); }
+__device__ __forceinline__ float3 to_world(const Frame &f, float3 v) { return f.x * v.x + f.y * v.y + f.z * v.z; }
+__device__ __forceinline__ float3 flipy(float3 v) { return f3(v.x, -v.y, v.z); }
+// Mapping::CosineWeightedHemisphereFromSquare (mapping.h:42-49)
+__device__ __forceinline__ float3 cosine_hemisphere(float u0, float u1) {
+    float s, c;
+    sincospif(2.0f * u1, &s, &c);
+    float sinphi = sqrtf(1.0f - u0);
+    return f3(c * sinphi, sqrtf(u0), s * sinphi);
+}
+// Mapping::DiskFromSquare, concentric map (mapping.h:53-73)
+__device__ __forceinline__ void disk_from_square(float u0, float u1, float &dx, float &dy) {
+    float a = u0 * 2.0f - 1.0f, b = u1 * 2.0f - 1.0f;
+    if (a == 0.f && b == 0.f) { dx = 0.f; dy = 0.f; return; }
+    float r, phi;
+    if (a * a > b * b) { r = a; phi = 0.25f * (b / a); }
+    else { r = b; phi = 0.5f - 0.25f * (a / b); }
+    float s, c;
+    sincospif(phi, &s, &c);
+    dx = r * c; dy = r * s;
+}
+
+// ---------------------------------------------------------------- BSDFs (local frame, y = normal)
+__device__ __forceinline__ float3 rho_d(const DMat &m) { return f3(m.rho_d[0], m.rho_d[1], m.rho_d[2]); }
+__device__ __forceinline__ float3 rho_s(const DMat &m) { return f3(m.rho_s[0], m.rho_s[1], m.rho_s[2]); }
+// LambertDiffuseBsdf (lambertdiffuse.h:16-51)
+__device__ __forceinline__ float3 lambert_eval(const DMat &m, float3 wi, float3 wo) {
+    if (wi.y <= 0.f || wo.y <= 0.f) return f3(0, 0, 0);
+    return rho_d(m) * kInvPi;
+}
+__device__ __forceinline__ float lambert_pdf(float3 wi, float3 wo) {
+    if (wi.y <= 0.f || wo.y <= 0.f) return 0.f;
+    return wo.y * kInvPi;
+}
+// OrenNayarBsdf (orennayar.h:14-83)
+__device__ __forceinline__ float3 oren_eval(const DMat &m, float3 wi, float3 wo) {
+    if (wi.y <= 0.f || wo.y <= 0.f) return f3(0, 0, 0);
+    float sinI = sqrtf(fminf(fmaxf(1.f - wi.y * wi.y, 0.f), 1.f));
+    float sinO = sqrtf(fminf(fmaxf(1.f - wo.y * wo.y, 0.f), 1.f));
+    float maxCos = 0.f, sinAlpha = 0.f, tanBeta = 0.f;
+    if (sinI > 1e-4f && sinO > 1e-4f) {
+        float dcos = (wi.x / sinI) * (wo.x / sinO) + (wi.z / sinI) * (wo.z / sinO);
+        maxCos = fmaxf(0.f, dcos);
+        if (fabsf(wi.y) > fabsf(wo.y)) { sinAlpha = sinO; tanBeta = sinI / wi.y; }
+        else { sinAlpha = sinI; tanBeta = sinO / wo.y; }
+    }
+    float s2 = m.sigma * m.sigma;
+    float A = 1.f - s2 / (2.f * (s2 + 0.33f));
+    float B = 0.45f * s2 / (s2 + 0.09f);
+    return rho_d(m) * (kInvPi * (A + B * maxCos * sinAlpha * tanBeta));
+}
+// PhongBsdf, modified Phong (phong.h:22-93)
+__device__ __forceinline__ float3 phong_eval(const DMat &m, float3 wi, float3 wo) {
+    if (wi.y <= 0.f || wo.y <= 0.f) return f3(0, 0, 0);
+    float3 r = f3(-wi.x, wi.y, -wi.z);
+    if (r.y <= 0.f) return f3(0, 0, 0);
+    float c = dot3(r, wo);
+    if (c <= 0.f) return f3(0, 0, 0);
+    return rho_s(m) * ((m.n + 2.f) * powf(c, m.n) * (kInvPi * 0.5f));
+}
+__device__ __forceinline__ float phong_pdf(const DMat &m, float3 wi, float3 wo) {
+    if (wi.y <= 0.f && wo.y <= 0.f) return 0.f;
+    float3 r = f3(-wi.x, wi.y, -wi.z);
+    float c = dot3(r, wo);
+    if (c <= 0.f) return 0.f;
+    return (m.n + 1.f) * powf(c, m.n) * (kInvPi * 0.5f);
+}
+__device__ __forceinline__ float3 phong_dir(const DMat &m, float3 wi, float u0, float u1) {
+    float3 r = f3(-wi.x, wi.y, -wi.z);
+    float cosT = powf(u0, 1.0f / (m.n + 1.0f));
+    float sinT = sqrtf(fmaxf(1.0f - cosT * cosT, 0.f));
+    float s, c;
+    sincospif(2.0f * u1, &s, &c);
+    Frame rb = make_frame(r);
+    return to_world(rb, f3(sinT * c, cosT, sinT * s));
+}
+__device__ __forceinline__ float3 inner_eval(const DMat &m, float3 wi, float3 wo) {
+    switch (m.type) {
+    case kLambert: return lambert_eval(m, wi, wo);
+    case kOrenNayar: return oren_eval(m, wi, wo);
+    case kPhong: return phong_eval(m, wi, wo);
+    default: return lambert_eval(m, wi, wo) + phong_eval(m, wi, wo);  // MixedBsdf (mixed.h:20-29)
+    }
+}
+__device__ __forceinline__ float inner_pdf(const DMat &m, float3 wi, float3 wo) {
+    switch (m.type) {
+    case kLambert: case kOrenNayar: return lambert_pdf(wi, wo);
+    case kPhong: return phong_pdf(m, wi, wo);
+    default: return 0.5f * (lambert_pdf(wi, wo) + phong_pdf(m, wi, wo));  // mixed.h:31-40
+    }
+}
+// sampleBsdfCosByPdf of each BSDF: returns f*cos/pdf, writes wo and pdf
+__device__ __forceinline__ float3 inner_sample(const DMat &m, float3 wi, float u0, float u1, float3 &wo, float &pdf) {
+    const float3 zero = f3(0, 0, 0);
+    switch (m.type) {
+    case kLambert: {
+        if (wi.y <= 0.f) return zero;
+        float3 o = cosine_hemisphere(u0, u1);
+        wo = o;
+        pdf = fmaxf(o.y, 0.f) * kInvPi;
+        if (o.y <= 0.f) return zero;
+        return rho_d(m);
+    }
+    case kOrenNayar: {
+        if (wi.y < 0.f) { pdf = 0.f; return zero; }
+        float3 o = cosine_hemisphere(u0, u1);
+        wo = o;
+        pdf = o.y * kInvPi;
+        return oren_eval(m, o, wi) * kPi;
+    }
+    case kPhong: {
+        if (wi.y <= 0.f) return zero;
+        float3 r = f3(-wi.x, wi.y, -wi.z);
+        float3 o = phong_dir(m, wi, u0, u1);
+        float c = dot3(o, r);
+        wo = o;
+        pdf = (m.n + 1.f) * 0.5f * kInvPi * powf(c, m.n);
+        if (r.y <= 0.f || o.y <= 0.f || c <= 0.f) return zero;
+        return rho_s(m) * ((m.n + 2.f) / (m.n + 1.f) * o.y);
+    }
+    default: {  // MixedBsdf::sampleBsdfCosByPdf (mixed.h:42-60), two components
+        float3 o = f3(0, 0, 0);
+        float ci = fminf(floorf(u0 * 2.f), 1.f);
+        float ur = fminf(u0 * 2.f - ci, kOneMinusEps);
+        if (wi.y > 0.f) {
+            if (ci == 0.f) o = cosine_hemisphere(ur, u1);
+            else o = phong_dir(m, wi, ur, u1);
+        }
+        wo = o;
+        float3 f = inner_eval(m, wi, o);
+        float p = inner_pdf(m, wi, o);
+        pdf = p;
+        if (p == 0.f) return zero;
+        return f * (o.y / p);
+    }
+    }
+}
+// TwoSidedBsdf wrapper (twosided.h:25-63)
+__device__ __forceinline__ float3 bsdf_eval(const DMat &m, float3 wi, float3 wo) {
+    if (!m.twosided) return inner_eval(m, wi, wo);
+    if (wi.y > 0.f && wo.y > 0.f) return inner_eval(m, wi, wo);
+    if (!(wi.y > 0.f) && !(wo.y > 0.f)) return inner_eval(m, flipy(wi), flipy(wo));
+    return f3(0, 0, 0);
+}
+__device__ __forceinline__ float bsdf_pdf(const DMat &m, float3 wi, float3 wo) {
+    if (!m.twosided) return inner_pdf(m, wi, wo);
+    if (wi.y > 0.f && wo.y > 0.f) return inner_pdf(m, wi, wo);
+    if (!(wi.y > 0.f) && !(wo.y > 0.f)) return inner_pdf(m, flipy(wi), flipy(wo));
+    return 0.f;
+}
+__device__ __forceinline__ float3 bsdf_sample(const DMat &m, float3 wi, float u0, float u1, float3 &wo, float &pdf) {
+    if (!m.twosided || wi.y > 0.f) return inner_sample(m, wi, u0, u1, wo, pdf);
+    float3 r = inner_sample(m, flipy(wi), u0, u1, wo, pdf);
+    wo = flipy(wo);
+    return r;
+}
+
+// ---------------------------------------------------------------- lights
+struct LightSample { float3 pos, n, contrib; float pdfA; };
+// createLightSubpath(numVertices = 1) -> CdfTable::sample -> DiffuseAreaLight::sampleLe0ByPdf
+// -> TriangleMeshGeometry::sample (generalsubpathsampler.h:199-255, diffusearealight.h:53-72,
+// trianglemesh.h:131-159, cdftable.h:36-44, mapping.h:21-27)
+__device__ __forceinline__ LightSample sample_light(const DScene &s, float u0, float u1) {
+    uint32_t li = 0;
+    while (li + 1 < s.n_lights && s.lights[li].cdf_select < u0) li++;
+    const DLight &L = s.lights[li];
+    float pdfL = L.pdf_select;
+    float prev_cdf = li == 0 ? 0.f : s.lights[li - 1].cdf_select;
+    float r = fminf((li == 0 ? u0 : (u0 - prev_cdf)) / pdfL, kOneMinusEps);
+    // triangle by area: lower_bound over the per-light cdf stored in p0.w
+    uint32_t lo = 0, hi = L.n_ltri - 1;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (s.ltris[3 * (L.first_ltri + mid)].w < r) lo = mid + 1; else hi = mid;
+    }
+    const float4 p0 = s.ltris[3 * (L.first_ltri + lo) + 0];
+    const float4 p1 = s.ltris[3 * (L.first_ltri + lo) + 1];
+    const float4 p2 = s.ltris[3 * (L.first_ltri + lo) + 2];
+    float prev_t = lo == 0 ? 0.f : s.ltris[3 * (L.first_ltri + lo - 1)].w;
+    float r2 = fminf((lo == 0 ? r : (r - prev_t)) / p1.w, kOneMinusEps);
+    float sq = sqrtf(r2);
+    float b1 = sq * (1.f - u1), b2 = u1 * sq;
+    LightSample ls;
+    float3 a = xyz(p0), b = xyz(p1), c = xyz(p2);
+    ls.pos = a * (1.f - b1 - b2) + b * b1 + c * b2;
+    ls.n = normalize3(cross3(b - a, c - a));
+    float k = kPi * L.area / pdfL;  // (pi L / pdfPos) / pdfLight
+    ls.contrib = f3(L.L[0] * k, L.L[1] * k, L.L[2] * k);
+    ls.pdfA = pdfL * L.inv_area;
+    return ls;
+}
+
+// ---------------------------------------------------------------- camera (thinlens.h, camera.h)
+__device__ __forceinline__ float3 cam_to_world(const DCamera &c, float3 l) {
+    return f3(c.xx * l.x + c.yx * l.y + c.zx * l.z, c.xy * l.x + c.yy * l.y + c.zy * l.z,
+              c.xz * l.x + c.yz * l.y + c.zz * l.z);
+}
+__device__ __forceinline__ float3 cam_to_local(const DCamera &c, float3 w) {
+    return f3(c.xx * w.x + c.xy * w.y + c.xz * w.z, c.yx * w.x + c.yy * w.y + c.yz * w.z,
+              c.zx * w.x + c.zy * w.y + c.zz * w.z);
+}
+// ThinlensCamera::evalWe1 (thinlens.h:62-80): returns 0 outside the film
+__device__ __forceinline__ float camera_we1(const DCamera &c, float3 lens_local, float3 w, float &rx, float &ry) {
+    float3 dir = cam_to_local(c, w);
+    float cz = dir.z;
+    if (cz < 0.f) return 0.f;
+    float3 fp = dir * (c.focal / cz) + lens_local;
+    float3 film = fp * (-c.d / c.focal);
+    rx = (film.x * c.invW + 0.5f) * float(c.W);
+    ry = (film.y * c.invH + 0.5f) * float(c.H);
+    if (!(rx >= 0.f && rx < float(c.W) && ry >= 0.f && ry < float(c.H))) return 0.f;
+    float c2 = cz * cz;
+    return c.per_film_norm / (c2 * c2);
+}
+
+}  // namespace wr

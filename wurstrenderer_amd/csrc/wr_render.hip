@@ -1,0 +1,1017 @@
+// wr_render.hip -- wavefront NEE + MIS path tracer for gfx950 and the C-ABI.
+//
+// Reference: PathNeeRenderer (src/renderer/pathnextevent.h:15-196) and the
+// random walk of GeneralSubpathSampler (src/pathsampler/generalsubpathsampler.h:13-160).
+// The recursive per-sample loop of the reference becomes, per batch of paths:
+//
+//   generate    camera ray + light vertex + light->camera (t = 1) connection
+//   repeat N-1 times:
+//     extend    closest hit for the active queue               (BVH4 traversal)
+//     shade     emitter hit with MIS, NEE with MIS, BSDF sample  (SoA path state)
+//     shadow    any hit for NEE and t = 1 rays; add visible contributions
+//     rotate    next queue becomes current (device-side counters)
+//   accumulate  per-pixel FP64 sums of the batch's path radiance
+//
+// Path state lives in HBM as SoA arrays indexed by path slot; queues hold slot
+// indices and are compacted with one atomic per wave (ballot + mbcnt).  Every
+// kernel reads its work count from device memory, so a batch needs no host
+// round trip.  Radiance is accumulated per path in FP32 and summed into the
+// FP64 film per pixel; t = 1 splats go to the FP64 splat film with atomics.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/wurst_amd.h"
+#include "wr_device.h"
+#include "wr_layout.h"
+#include "wr_traverse.h"
+
+using namespace wr;
+
+// ============================================================== errors
+static thread_local std::string g_error;
+static int set_error(int code, const std::string &msg) {
+    g_error = msg;
+    return code;
+}
+#define WR_HIP(expr)                                                                                     \
+    do {                                                                                                 \
+        hipError_t e_ = (expr);                                                                          \
+        if (e_ != hipSuccess)                                                                            \
+            return set_error(e_ == hipErrorOutOfMemory ? WR_ERR_OUT_OF_MEMORY : WR_ERR_DEVICE,           \
+                             std::string(#expr) + ": " + hipGetErrorString(e_));                         \
+    } while (0)
+
+// ============================================================== kernel arguments
+constexpr uint32_t kSplatFlag = 0x80000000u;
+constexpr int kBlock = 64;  // one wavefront per block; LDS stack indexed by lane
+
+struct PathBuffers {
+    float4 *ray_o, *ray_d;    // origin / direction of the ray to extend
+    float4 *thr;              // rgb: path throughput of the next vertex; w: pdfW of that direction
+    float4 *rad;              // rgb: accumulated radiance (FP32)
+    uint32_t *key;            // RNG key of the path
+    uint2 *hit;               // leaf-order triangle (or kInvalid), t bits
+    uint32_t *queue[2];       // active slot queues (ping-pong)
+    float4 *sh_o, *sh_d, *sh_c;  // shadow rays: o.xyz+tmax, d.xyz+target, contribution
+    uint32_t *counters;       // [0] active, [1] next, [2] shadow
+    unsigned long long *stats;  // [0] extend rays [1] shadow rays [2] ext nodes [3] ext tris [4] shd nodes [5] shd tris
+    double *film_base, *film_splat;
+};
+
+struct RenderArgs {
+    uint32_t n_paths;         // paths in this batch = spp_batch * W * H
+    uint32_t wh;
+    uint32_t first_sample;    // sample index of slot 0
+    uint32_t seed;
+    int N;                    // num_path_vertices
+    int use_mis;
+};
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+// Wave-aggregated queue append: one atomic per wave.  All lanes must call it.
+__device__ __forceinline__ uint32_t wave_append(uint32_t *counter, bool pred) {
+    const unsigned long long mask = __ballot(pred);
+    if (mask == 0) return kInvalid;
+    const uint32_t lane = lane_id();
+    const uint32_t leader = __ffsll((long long)mask) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+    base = __shfl(base, leader);
+    const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+    return pred ? base + below : kInvalid;
+}
+
+__device__ __forceinline__ void wave_add_stat(unsigned long long *dst, uint32_t v) {
+    // sum over the wave, one atomic
+    uint32_t s = v;
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if (lane_id() == 0 && s) atomicAdd(dst, (unsigned long long)s);
+}
+
+// ============================================================== small control kernels
+__global__ void k_reset(PathBuffers pb, uint32_t n_active) {
+    pb.counters[0] = n_active;
+    pb.counters[1] = 0;
+    pb.counters[2] = 0;
+}
+__global__ void k_rotate(PathBuffers pb) {
+    pb.stats[0] += pb.counters[0];
+    pb.stats[1] += pb.counters[2];
+    pb.counters[0] = pb.counters[1];
+    pb.counters[1] = 0;
+    pb.counters[2] = 0;
+}
+__global__ void k_fill_u32(uint32_t *p, uint32_t v, size_t n) {
+    for (size_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) p[i] = v;
+}
+__global__ void k_fill_f64(double *p, size_t n) {
+    for (size_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) p[i] = 0.0;
+}
+
+// ============================================================== generate
+// createLightSubpath + createCameraSubpath up to the first ray, including the
+// t = 1 light->camera connection (pathnextevent.h:53-99; thinlens.h:121-166).
+__global__ __launch_bounds__(kBlock) void k_generate(DScene s, PathBuffers pb, RenderArgs a) {
+    const DCamera &cam = s.cam;
+    for (uint32_t base = blockIdx.x * kBlock; base < a.n_paths; base += gridDim.x * kBlock) {
+        const uint32_t slot = base + lane_id();
+        const bool valid = slot < a.n_paths;
+        bool splat = false;
+        float4 so = make_float4(0, 0, 0, 0), sd = so, sc = so;
+        if (valid) {
+            const uint32_t pixel = slot % a.wh;
+            const uint32_t sample = a.first_sample + slot / a.wh;
+            const uint32_t px = pixel % uint32_t(cam.W), py = pixel / uint32_t(cam.W);
+            const uint32_t key = path_key(a.seed, pixel, sample);
+            const LightSample l = sample_light(s, u01(key, kDimLight), u01(key, kDimLight + 1));
+            // camera vertex (sampleWe0ByPdf)
+            float lx = 0.f, ly = 0.f;
+            if (cam.aperture > 0.f) {
+                disk_from_square(u01(key, kDimLens), u01(key, kDimLens + 1), lx, ly);
+                lx *= cam.aperture; ly *= cam.aperture;
+            }
+            const float3 lens = f3(lx, ly, 0.f);
+            const float3 cpos = cam_to_world(cam, lens) + f3(cam.ox, cam.oy, cam.oz);
+            const float3 cz = f3(cam.zx, cam.zy, cam.zz);
+            // t = 1 connection
+            {
+                const float3 uv = cpos - l.pos;
+                const float len2 = dot3(uv, uv);
+                const float len = sqrtf(len2);
+                const float3 nuv = uv * (1.0f / len);
+                const float cos_l = dot3(l.n, nuv);
+                const float G = fabsf(cos_l) * fabsf(dot3(cz, nuv)) / len2;
+                if (G != 0.f && cos_l >= 0.f) {
+                    float rx, ry;
+                    const float3 wcam = -nuv;
+                    const float we1 = camera_we1(cam, lens, wcam, rx, ry);
+                    if (we1 != 0.f) {
+                        float mis = 1.f;
+                        if (a.use_mis) {
+                            const float c = cam_to_local(cam, wcam).z;
+                            const float camPdfA = cam.per_pixel_norm / (c * c * c) * fabsf(cos_l) / len2;
+                            mis = l.pdfA / (l.pdfA + camPdfA);
+                        }
+                        const float k = mis * we1 * G * kInvPi;
+                        const uint32_t target = kSplatFlag | (uint32_t(ry) * uint32_t(cam.W) + uint32_t(rx));
+                        so = make_float4(l.pos.x, l.pos.y, l.pos.z, len - kRayEps);
+                        sd = make_float4(nuv.x, nuv.y, nuv.z, __uint_as_float(target));
+                        sc = make_float4(k * l.contrib.x, k * l.contrib.y, k * l.contrib.z, 0.f);
+                        splat = true;
+                    }
+                }
+            }
+            // primary ray (samplePerPixelWe1CosByPdf)
+            const float j0 = u01(key, kDimPixel), j1 = u01(key, kDimPixel + 1);
+            const float ns0 = (float(px) + j0) * cam.invW, ns1 = (float(py) + j1) * cam.invH;
+            const float3 film = f3((ns0 - 0.5f) * float(cam.W), (ns1 - 0.5f) * float(cam.H), -cam.d);
+            const float3 focal = film * (cam.focal / film.z);
+            const float3 dl = normalize3(focal - lens);
+            const float pdfW = cam.per_pixel_norm / (dl.z * dl.z * dl.z);
+            const float3 dir = cam_to_world(cam, dl);
+            pb.ray_o[slot] = make_float4(cpos.x, cpos.y, cpos.z, 0.f);
+            pb.ray_d[slot] = make_float4(dir.x, dir.y, dir.z, 0.f);
+            pb.thr[slot] = make_float4(1.f, 1.f, 1.f, a.use_mis ? pdfW : 0.f);
+            pb.rad[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+            pb.key[slot] = key;
+            pb.queue[0][slot] = slot;
+        }
+        const uint32_t idx = wave_append(&pb.counters[2], splat);
+        if (splat) {
+            pb.sh_o[idx] = so;
+            pb.sh_d[idx] = sd;
+            pb.sh_c[idx] = sc;
+        }
+    }
+}
+
+// ============================================================== extend
+template <bool COUNT>
+__global__ __launch_bounds__(kBlock) void k_extend(DScene s, PathBuffers pb, const uint32_t *__restrict__ queue, uint2 *spill) {
+    __shared__ uint32_t st_node[kLdsStackDepth * kBlock];
+    __shared__ float st_t[kLdsStackDepth * kBlock];
+    LaneStack<kBlock> st{st_node + lane_id(), st_t + lane_id(),
+                         spill + size_t(blockIdx.x * kBlock + lane_id()) * kSpillDepth, 0};
+    const uint32_t n = pb.counters[0];
+    TraversalCounters cnt;
+    for (uint32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
+        const uint32_t i = base + lane_id();
+        if (i < n) {
+            const uint32_t p = queue[i];
+            const float4 o = pb.ray_o[p], d = pb.ray_d[p];
+            const RaySetup r = setup_ray(xyz(o), xyz(d), kRayEps);
+            float t = INFINITY;
+            const uint32_t tri = traverse<false, COUNT, kBlock>(s, r, t, st, cnt);
+            pb.hit[p] = make_uint2(tri, __float_as_uint(t));
+        }
+    }
+    if (COUNT) {
+        wave_add_stat(&pb.stats[2], cnt.nodes);
+        wave_add_stat(&pb.stats[3], cnt.tris);
+    }
+}
+
+// ============================================================== shade
+// One vertex of randomWalk with the PathNeeRenderer callback
+// (generalsubpathsampler.h:34-130, pathnextevent.h:100-149).
+__global__ __launch_bounds__(kBlock) void k_shade(DScene s, PathBuffers pb, RenderArgs a, const uint32_t *__restrict__ queue,
+                                                  uint32_t *__restrict__ next_queue, int bounce) {
+    const uint32_t n = pb.counters[0];
+    const int cNum = bounce + 2;  // camera is vertex 1
+    for (uint32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
+        const uint32_t i = base + lane_id();
+        bool push_shadow = false, cont = false;
+        uint32_t p = 0;
+        float4 so = make_float4(0, 0, 0, 0), sd = so, sc = so;
+        float4 new_o = so, new_d = so, new_thr = so;
+        if (i < n) {
+            p = queue[i];
+            const uint2 h = pb.hit[p];
+            if (h.x != kInvalid) {
+                const float t = __uint_as_float(h.y);
+                const float3 ro = xyz(pb.ray_o[p]), rd = xyz(pb.ray_d[p]);
+                const float4 v0 = s.tris[3 * size_t(h.x) + 0];
+                const float4 v1 = s.tris[3 * size_t(h.x) + 1];
+                const float4 v2 = s.tris[3 * size_t(h.x) + 2];
+                const float3 pos = ro + rd * t;
+                const float3 ng = normalize3(cross3(xyz(v1) - xyz(v0), xyz(v2) - xyz(v0)));
+                const int2 gi = s.geom_info[__float_as_uint(v1.w)];
+                const float4 thr = pb.thr[p];
+                const float3 T = xyz(thr);
+                const uint32_t key = pb.key[p];
+                if (gi.y >= 0) {
+                    // emitter hit: MIS against light sampling (pathnextevent.h:128-146)
+                    const float cos_l = -dot3(ng, rd);
+                    if (cos_l >= 0.f) {
+                        float mis = 0.f;
+                        if (a.use_mis) {
+                            const float cPdfA = thr.w * fabsf(cos_l) / (t * t);
+                            const DLight &L = s.lights[gi.y];
+                            const float lightPdfA = L.pdf_select * L.inv_area;
+                            mis = cPdfA / (lightPdfA + cPdfA);
+                        }
+                        const DLight &L = s.lights[gi.y];
+                        float4 rr = pb.rad[p];
+                        rr.x += mis * T.x * L.L[0];
+                        rr.y += mis * T.y * L.L[1];
+                        rr.z += mis * T.z * L.L[2];
+                        pb.rad[p] = rr;
+                    }
+                } else if (cNum != a.N) {
+                    const DMat m = s.mats[gi.x];
+                    const Frame f = make_frame(ng);
+                    const float3 wi = to_local(f, -rd);
+                    // NEE with MIS (pathnextevent.h:106-127)
+                    const LightSample l = sample_light(s, u01(key, kDimLight), u01(key, kDimLight + 1));
+                    const float3 uv = l.pos - pos;
+                    const float len2 = dot3(uv, uv);
+                    const float len = sqrtf(len2);
+                    const float3 nuv = uv * (1.0f / len);
+                    const float cos_lv = -dot3(l.n, nuv);
+                    const float G = fabsf(dot3(ng, nuv)) * fabsf(cos_lv) / len2;
+                    if (G != 0.f && cos_lv >= 0.f) {
+                        const float3 wo = to_local(f, nuv);
+                        const float3 fs = bsdf_eval(m, wi, wo);
+                        if (!is_black(fs)) {
+                            float mis = 1.f;
+                            if (a.use_mis) {
+                                const float bsdfPdfA = bsdf_pdf(m, wi, wo) * fabsf(cos_lv) / len2;
+                                mis = l.pdfA / (l.pdfA + bsdfPdfA);
+                            }
+                            const float k = mis * G * kInvPi;
+                            const float3 c = T * fs * l.contrib * k;
+                            if (!is_black(c)) {
+                                so = make_float4(pos.x, pos.y, pos.z, len - kRayEps);
+                                sd = make_float4(nuv.x, nuv.y, nuv.z, __uint_as_float(p));
+                                sc = make_float4(c.x, c.y, c.z, 0.f);
+                                push_shadow = true;
+                            }
+                        }
+                    }
+                    // BSDF sampling (generalsubpathsampler.h:81-130); here cNum < N
+                    const uint32_t dim = kDimBounce0 + kDimsPerBounce * uint32_t(bounce) + kBdBsdf;
+                    float3 wo;
+                    float pdf = 0.f;
+                    const float3 w = bsdf_sample(m, wi, u01(key, dim), u01(key, dim + 1), wo, pdf);
+                    if (!is_black(w)) {
+                        const float3 nd = to_world(f, wo);
+                        new_o = make_float4(pos.x, pos.y, pos.z, 0.f);
+                        new_d = make_float4(nd.x, nd.y, nd.z, 0.f);
+                        new_thr = make_float4(w.x * T.x, w.y * T.y, w.z * T.z, a.use_mis ? pdf : 0.f);
+                        cont = true;
+                    }
+                }
+            }
+        }
+        const uint32_t si = wave_append(&pb.counters[2], push_shadow);
+        if (push_shadow) {
+            pb.sh_o[si] = so;
+            pb.sh_d[si] = sd;
+            pb.sh_c[si] = sc;
+        }
+        const uint32_t qi = wave_append(&pb.counters[1], cont);
+        if (cont) {
+            next_queue[qi] = p;
+            pb.ray_o[p] = new_o;
+            pb.ray_d[p] = new_d;
+            pb.thr[p] = new_thr;
+        }
+    }
+}
+
+// ============================================================== shadow
+template <bool COUNT>
+__global__ __launch_bounds__(kBlock) void k_shadow(DScene s, PathBuffers pb, uint2 *spill) {
+    __shared__ uint32_t st_node[kLdsStackDepth * kBlock];
+    __shared__ float st_t[kLdsStackDepth * kBlock];
+    LaneStack<kBlock> st{st_node + lane_id(), st_t + lane_id(),
+                         spill + size_t(blockIdx.x * kBlock + lane_id()) * kSpillDepth, 0};
+    const uint32_t n = pb.counters[2];
+    TraversalCounters cnt;
+    for (uint32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
+        const uint32_t i = base + lane_id();
+        if (i < n) {
+            const float4 o = pb.sh_o[i], d = pb.sh_d[i];
+            const RaySetup r = setup_ray(xyz(o), xyz(d), kRayEps);
+            float t = o.w;
+            const uint32_t tri = traverse<true, COUNT, kBlock>(s, r, t, st, cnt);
+            if (tri == kInvalid) {
+                const float4 c = pb.sh_c[i];
+                const uint32_t target = __float_as_uint(d.w);
+                if (target & kSplatFlag) {
+                    double *dst = pb.film_splat + 3 * size_t(target & ~kSplatFlag);
+                    atomicAdd(dst + 0, double(c.x));
+                    atomicAdd(dst + 1, double(c.y));
+                    atomicAdd(dst + 2, double(c.z));
+                } else {
+                    float4 rr = pb.rad[target];
+                    rr.x += c.x; rr.y += c.y; rr.z += c.z;
+                    pb.rad[target] = rr;
+                }
+            }
+        }
+    }
+    if (COUNT) {
+        wave_add_stat(&pb.stats[4], cnt.nodes);
+        wave_add_stat(&pb.stats[5], cnt.tris);
+    }
+}
+
+// ============================================================== accumulate
+// Film::addSample for the batch (film.h:42-48): pixel-owned FP64 sums.
+__global__ void k_accumulate(PathBuffers pb, uint32_t wh, uint32_t spp_batch) {
+    for (uint32_t px = blockIdx.x * blockDim.x + threadIdx.x; px < wh; px += gridDim.x * blockDim.x) {
+        double r = 0.0, g = 0.0, b = 0.0;
+        for (uint32_t k = 0; k < spp_batch; k++) {
+            const float4 v = pb.rad[size_t(k) * wh + px];
+            r += v.x; g += v.y; b += v.z;
+        }
+        double *dst = pb.film_base + 3 * size_t(px);
+        dst[0] += r; dst[1] += g; dst[2] += b;
+    }
+}
+
+// ============================================================== trace kernels (parity / benchmark)
+template <bool ANY>
+__global__ __launch_bounds__(kBlock) void k_trace(DScene s, const float4 *__restrict__ rays, uint32_t n, uint2 *hits,
+                                                  uint8_t *occ, uint2 *spill) {
+    __shared__ uint32_t st_node[kLdsStackDepth * kBlock];
+    __shared__ float st_t[kLdsStackDepth * kBlock];
+    LaneStack<kBlock> st{st_node + lane_id(), st_t + lane_id(),
+                         spill + size_t(blockIdx.x * kBlock + lane_id()) * kSpillDepth, 0};
+    TraversalCounters cnt;
+    for (uint32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
+        const uint32_t i = base + lane_id();
+        if (i < n) {
+            const float4 a = rays[2 * size_t(i)], b = rays[2 * size_t(i) + 1];
+            const RaySetup r = setup_ray(xyz(a), xyz(b), a.w);
+            float t = b.w;
+            const uint32_t tri = traverse<ANY, false, kBlock>(s, r, t, st, cnt);
+            if (ANY) occ[i] = tri != kInvalid;
+            else hits[i] = make_uint2(tri, __float_as_uint(tri == kInvalid ? INFINITY : t));
+        }
+    }
+}
+
+// ============================================================== host objects
+struct wr_device {
+    int ordinal = 0;
+    int n_cu = 256;
+};
+
+struct wr_scene {
+    wr_device *dev = nullptr;
+    hipStream_t stream = nullptr;
+    DScene ds{};
+    // device allocations
+    Node4 *d_nodes = nullptr;
+    float4 *d_tris = nullptr, *d_ltris = nullptr;
+    uint2 *d_spill = nullptr;   // traversal stack spill: kSpillDepth entries per lane of the largest grid
+    int max_trav_blocks = 0;
+    int2 *d_geom = nullptr;
+    DMat *d_mats = nullptr;
+    DLight *d_lights = nullptr;
+    uint64_t n_nodes = 0, n_tris = 0, device_bytes = 0;
+    uint32_t max_stack = 0;
+    // host side: triangle index -> (geom, prim)
+    std::vector<uint32_t> tri_geom, geom_first, leaf_to_orig;
+    int W = 0, H = 0;
+    // path buffers (lazily sized)
+    PathBuffers pb{};
+    uint32_t capacity = 0;
+    void *pool = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+};
+
+struct wr_film {
+    wr_scene *scene = nullptr;
+    double *d_base = nullptr, *d_splat = nullptr;
+    int W = 0, H = 0;
+};
+
+static int set_device(wr_device *dev) {
+    WR_HIP(hipSetDevice(dev->ordinal));
+    return WR_OK;
+}
+
+// The entry points below inherit C linkage from their declarations in wurst_amd.h.
+
+const char *wr_last_error(void) { return g_error.c_str(); }
+const char *wr_version(void) { return "wurst_amd 0.1 (gfx950, BVH4-128B, wavefront NEE+MIS)"; }
+
+int wr_device_create(int ordinal, wr_device **out) {
+    if (!out) return set_error(WR_ERR_INVALID_ARGUMENT, "out is null");
+    *out = nullptr;
+    int count = 0;
+    WR_HIP(hipGetDeviceCount(&count));
+    if (ordinal < 0 || ordinal >= count) return set_error(WR_ERR_INVALID_ARGUMENT, "device ordinal out of range");
+    WR_HIP(hipSetDevice(ordinal));
+    hipDeviceProp_t prop;
+    WR_HIP(hipGetDeviceProperties(&prop, ordinal));
+    wr_device *d = new wr_device;
+    d->ordinal = ordinal;
+    d->n_cu = prop.multiProcessorCount;
+    *out = d;
+    return WR_OK;
+}
+
+void wr_device_destroy(wr_device *dev) { delete dev; }
+
+void wr_scene_destroy(wr_scene *s) {
+    if (!s) return;
+    (void)hipSetDevice(s->dev->ordinal);
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    (void)hipFree(s->d_nodes);
+    (void)hipFree(s->d_tris);
+    (void)hipFree(s->d_ltris);
+    (void)hipFree(s->d_geom);
+    (void)hipFree(s->d_mats);
+    (void)hipFree(s->d_lights);
+    (void)hipFree(s->d_spill);
+    (void)hipFree(s->pool);
+    for (auto &e : s->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+}
+
+static int validate_desc(const wr_scene_desc *d) {
+    if (!d) return set_error(WR_ERR_INVALID_ARGUMENT, "desc is null");
+    if (d->n_tris > 0 && (!d->verts || !d->tris || !d->tri_geom))
+        return set_error(WR_ERR_INVALID_ARGUMENT, "null geometry arrays");
+    if (d->n_tris >= kMaxTriangles) return set_error(WR_ERR_UNSUPPORTED, "too many triangles (limit 2^28)");
+    if (d->n_lights == 0) return set_error(WR_ERR_INVALID_ARGUMENT, "scene has no lights");
+    if (d->camera.width <= 0 || d->camera.height <= 0) return set_error(WR_ERR_INVALID_ARGUMENT, "bad resolution");
+    if ((uint64_t)d->camera.width * d->camera.height >= (1ull << 31))
+        return set_error(WR_ERR_UNSUPPORTED, "resolution too large");
+    if (!(d->camera.focal_distance > 0.f))
+        return set_error(WR_ERR_INVALID_ARGUMENT, "focal distance must be > 0 (the JSON key \"radius\" sets it)");
+    if (!(d->camera.fovy_deg > 0.f && d->camera.fovy_deg < 180.f))
+        return set_error(WR_ERR_INVALID_ARGUMENT, "fovy must be in (0, 180) degrees");
+    for (uint32_t i = 0; i < 3 * d->n_tris; i++)
+        if (d->tris[i] >= d->n_verts) return set_error(WR_ERR_INVALID_ARGUMENT, "triangle vertex index out of range");
+    uint32_t prev = 0;
+    for (uint32_t t = 0; t < d->n_tris; t++) {
+        if (d->tri_geom[t] >= d->n_geoms) return set_error(WR_ERR_INVALID_ARGUMENT, "tri_geom out of range");
+        if (d->tri_geom[t] < prev) return set_error(WR_ERR_INVALID_ARGUMENT, "triangles of a geometry must be contiguous and ordered");
+        prev = d->tri_geom[t];
+    }
+    for (uint32_t g = 0; g < d->n_geoms; g++) {
+        const int m = d->geom_material[g], l = d->geom_light[g];
+        if (l >= (int)d->n_lights) return set_error(WR_ERR_INVALID_ARGUMENT, "geom_light out of range");
+        if (l < 0 && (m < 0 || m >= (int)d->n_materials))
+            return set_error(WR_ERR_INVALID_ARGUMENT, "geometry without a valid material");
+    }
+    for (uint32_t m = 0; m < d->n_materials; m++)
+        if (d->materials[m].type < 0 || d->materials[m].type > 3) return set_error(WR_ERR_INVALID_ARGUMENT, "unknown material type");
+    for (uint32_t l = 0; l < d->n_lights; l++)
+        if (d->lights[l].geom < 0 || (uint32_t)d->lights[l].geom >= d->n_geoms)
+            return set_error(WR_ERR_INVALID_ARGUMENT, "light geometry out of range");
+    return WR_OK;
+}
+
+int wr_scene_validate(const wr_scene_desc *desc) { return validate_desc(desc); }
+
+int wr_bvh_selftest(const wr_scene_desc *d, uint64_t *n_nodes, uint32_t *max_stack, uint32_t *depth) {
+    int rc = validate_desc(d);
+    if (rc) return rc;
+    BvhBuildResult bvh;
+    if (!build_bvh4(d->verts, d->n_verts, d->tris, d->n_tris, d->tri_geom, &bvh))
+        return set_error(WR_ERR_OUT_OF_MEMORY, "host allocation failed during BVH build");
+    const std::string err = verify_bvh4(bvh, d->verts, d->tris, d->n_tris);
+    if (n_nodes) *n_nodes = bvh.n_nodes;
+    if (max_stack) *max_stack = bvh.max_stack;
+    if (depth) *depth = bvh.depth;
+    free_bvh(&bvh);
+    if (!err.empty()) return set_error(WR_ERR_DEVICE, "BVH invariant violated: " + err);
+    return WR_OK;
+}
+
+template <typename T>
+static int upload(T **dst, const void *src, size_t bytes, uint64_t &total) {
+    WR_HIP(hipMalloc(reinterpret_cast<void **>(dst), std::max<size_t>(bytes, 128)));
+    if (bytes) WR_HIP(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+    total += bytes;
+    return WR_OK;
+}
+
+static int scene_create_impl(wr_device *dev, const wr_scene_desc *d, wr_scene *s) {
+    s->dev = dev;
+    int rc = set_device(dev);
+    if (rc) return rc;
+    WR_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    WR_HIP(hipEventCreate(&s->ev[0]));
+    WR_HIP(hipEventCreate(&s->ev[1]));
+    s->W = d->camera.width;
+    s->H = d->camera.height;
+    // host tables
+    s->tri_geom.assign(d->tri_geom, d->tri_geom + d->n_tris);
+    s->geom_first.assign(d->n_geoms, kInvalid);
+    for (uint32_t t = 0; t < d->n_tris; t++)
+        if (s->geom_first[d->tri_geom[t]] == kInvalid) s->geom_first[d->tri_geom[t]] = t;
+    // BVH
+    BvhBuildResult bvh;
+    if (!build_bvh4(d->verts, d->n_verts, d->tris, d->n_tris, d->tri_geom, &bvh))
+        return set_error(WR_ERR_OUT_OF_MEMORY, "host allocation failed during BVH build");
+    if (bvh.max_stack > (uint32_t)kStackDepth) {
+        free_bvh(&bvh);
+        return set_error(WR_ERR_UNSUPPORTED, "BVH deeper than the traversal stack");
+    }
+    s->max_trav_blocks = dev->n_cu * kMaxTraversalBlocksPerCU;
+    {
+        const size_t spill_bytes = sizeof(uint2) * size_t(s->max_trav_blocks) * kBlock * kSpillDepth;
+        hipError_t e = hipMalloc(&s->d_spill, spill_bytes);
+        if (e != hipSuccess) {
+            free_bvh(&bvh);
+            return set_error(WR_ERR_OUT_OF_MEMORY, "traversal stack spill allocation failed");
+        }
+        s->device_bytes += spill_bytes;
+    }
+    s->leaf_to_orig.resize(d->n_tris);
+    for (uint32_t i = 0; i < d->n_tris; i++) s->leaf_to_orig[i] = bvh.tris[i].orig;
+    s->n_nodes = bvh.n_nodes;
+    s->n_tris = d->n_tris;
+    s->max_stack = bvh.max_stack;
+    rc = upload(&s->d_nodes, bvh.nodes, sizeof(Node4) * bvh.n_nodes, s->device_bytes);
+    if (!rc) rc = upload(&s->d_tris, bvh.tris, sizeof(TriRecord) * bvh.n_tris, s->device_bytes);
+    free_bvh(&bvh);
+    if (rc) return rc;
+    // geometry info
+    std::vector<int2> gi(d->n_geoms);
+    for (uint32_t g = 0; g < d->n_geoms; g++) gi[g] = make_int2(d->geom_material[g], d->geom_light[g]);
+    rc = upload(&s->d_geom, gi.data(), sizeof(int2) * gi.size(), s->device_bytes);
+    if (rc) return rc;
+    std::vector<DMat> mats(d->n_materials);
+    for (uint32_t m = 0; m < d->n_materials; m++) {
+        const wr_material &src = d->materials[m];
+        DMat &dm = mats[m];
+        dm.type = src.type;
+        dm.twosided = src.twosided;
+        dm.sigma = src.sigma;
+        dm.n = src.exponent;
+        for (int k = 0; k < 3; k++) { dm.rho_d[k] = src.rho_d[k]; dm.rho_s[k] = src.rho_s[k]; }
+        dm.rho_d[3] = dm.rho_s[3] = 0.f;
+    }
+    rc = upload(&s->d_mats, mats.data(), sizeof(DMat) * mats.size(), s->device_bytes);
+    if (rc) return rc;
+    // lights: selection CdfTable over |L|_2, per-light triangle-area CdfTable (FP64 on the host)
+    std::vector<DLight> lights(d->n_lights);
+    std::vector<float4> ltris;
+    std::vector<double> w(d->n_lights);
+    double wsum = 0.0;
+    for (uint32_t l = 0; l < d->n_lights; l++) {
+        const float *L = d->lights[l].L;
+        w[l] = std::sqrt(double(L[0]) * L[0] + double(L[1]) * L[1] + double(L[2]) * L[2]);
+        wsum += w[l];
+    }
+    if (!(wsum > 0.0)) return set_error(WR_ERR_INVALID_ARGUMENT, "all lights have zero radiance");
+    double cdf = 0.0;
+    for (uint32_t l = 0; l < d->n_lights; l++) {
+        DLight &dl = lights[l];
+        const uint32_t g = (uint32_t)d->lights[l].geom;
+        const double pdf = w[l] / wsum;
+        cdf += pdf;
+        dl.pdf_select = float(pdf);
+        dl.cdf_select = (l + 1 == d->n_lights) ? 1.0f : float(cdf);
+        for (int k = 0; k < 3; k++) dl.L[k] = d->lights[l].L[k];
+        uint32_t first = s->geom_first[g];
+        if (first == kInvalid) return set_error(WR_ERR_INVALID_ARGUMENT, "light geometry has no triangles");
+        uint32_t cnt = 0;
+        while (first + cnt < d->n_tris && d->tri_geom[first + cnt] == g) cnt++;
+        std::vector<double> area(cnt);
+        double asum = 0.0;
+        for (uint32_t i = 0; i < cnt; i++) {
+            const uint32_t *t = d->tris + 3 * size_t(first + i);
+            const float *a = d->verts + 3 * size_t(t[0]), *b = d->verts + 3 * size_t(t[1]), *c = d->verts + 3 * size_t(t[2]);
+            double e1[3], e2[3];
+            for (int k = 0; k < 3; k++) { e1[k] = double(b[k]) - a[k]; e2[k] = double(c[k]) - a[k]; }
+            const double cx = e1[1] * e2[2] - e1[2] * e2[1], cy = e1[2] * e2[0] - e1[0] * e2[2], cz = e1[0] * e2[1] - e1[1] * e2[0];
+            area[i] = std::sqrt(cx * cx + cy * cy + cz * cz) / 2.0;
+            asum += area[i];
+        }
+        if (!(asum > 0.0)) return set_error(WR_ERR_INVALID_ARGUMENT, "light with zero area");
+        dl.area = float(asum);
+        dl.inv_area = float(1.0 / asum);
+        dl.first_ltri = uint32_t(ltris.size() / 3);
+        dl.n_ltri = cnt;
+        double tc = 0.0;
+        for (uint32_t i = 0; i < cnt; i++) {
+            const uint32_t *t = d->tris + 3 * size_t(first + i);
+            const double tp = area[i] / asum;
+            tc += tp;
+            const float tcdf = (i + 1 == cnt) ? 1.0f : float(tc);
+            const float *a = d->verts + 3 * size_t(t[0]), *b = d->verts + 3 * size_t(t[1]), *c = d->verts + 3 * size_t(t[2]);
+            ltris.push_back(make_float4(a[0], a[1], a[2], tcdf));
+            ltris.push_back(make_float4(b[0], b[1], b[2], float(tp)));
+            ltris.push_back(make_float4(c[0], c[1], c[2], 0.f));
+        }
+    }
+    rc = upload(&s->d_lights, lights.data(), sizeof(DLight) * lights.size(), s->device_bytes);
+    if (!rc) rc = upload(&s->d_ltris, ltris.data(), sizeof(float4) * ltris.size(), s->device_bytes);
+    if (rc) return rc;
+    // camera (ThinlensCamera ctor, thinlens.h:40-55; Camera ctor, camera.h:9-17), FP64 then rounded
+    const wr_camera &c = d->camera;
+    double o[3] = {c.pos[0], c.pos[1], c.pos[2]}, la[3] = {c.lookat[0], c.lookat[1], c.lookat[2]},
+           up[3] = {c.up[0], c.up[1], c.up[2]};
+    auto norm = [](double v[3]) {
+        double l = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        for (int k = 0; k < 3; k++) v[k] /= l;
+    };
+    auto cross = [](const double a[3], const double b[3], double r[3]) {
+        r[0] = a[1] * b[2] - a[2] * b[1];
+        r[1] = a[2] * b[0] - a[0] * b[2];
+        r[2] = a[0] * b[1] - a[1] * b[0];
+    };
+    double z[3] = {la[0] - o[0], la[1] - o[1], la[2] - o[2]}, x[3], y[3];
+    norm(z);
+    cross(up, z, x);
+    norm(x);
+    cross(z, x, y);
+    norm(y);
+    DCamera &dc = s->ds.cam;
+    dc.ox = c.pos[0]; dc.oy = c.pos[1]; dc.oz = c.pos[2];
+    dc.xx = float(x[0]); dc.xy = float(x[1]); dc.xz = float(x[2]);
+    dc.yx = float(y[0]); dc.yy = float(y[1]); dc.yz = float(y[2]);
+    dc.zx = float(z[0]); dc.zy = float(z[1]); dc.zz = float(z[2]);
+    const double fovy = double(c.fovy_deg) * 3.14159265358979323846 / 180.0;
+    const double dist = double(c.height) * 0.5 / std::tan(fovy * 0.5);
+    dc.d = float(dist);
+    dc.W = c.width;
+    dc.H = c.height;
+    dc.invW = float(1.0 / c.width);
+    dc.invH = float(1.0 / c.height);
+    dc.per_pixel_norm = float(dist * dist / (double(c.width) * double(c.height)));
+    dc.per_film_norm = float(dist * dist);
+    dc.aperture = c.aperture_radius;
+    dc.focal = c.focal_distance;
+    s->ds.nodes = s->d_nodes;
+    s->ds.tris = s->d_tris;
+    s->ds.geom_info = s->d_geom;
+    s->ds.mats = s->d_mats;
+    s->ds.lights = s->d_lights;
+    s->ds.ltris = s->d_ltris;
+    s->ds.n_lights = d->n_lights;
+    return WR_OK;
+}
+
+int wr_scene_create(wr_device *dev, const wr_scene_desc *desc, wr_scene **out) {
+    if (!dev || !out) return set_error(WR_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    int rc = validate_desc(desc);
+    if (rc) return rc;
+    wr_scene *s = new wr_scene;
+    rc = scene_create_impl(dev, desc, s);
+    if (rc) {
+        std::string msg = g_error;
+        wr_scene_destroy(s);
+        g_error = msg;
+        return rc;
+    }
+    *out = s;
+    return WR_OK;
+}
+
+int wr_scene_info(const wr_scene *s, uint64_t *n_nodes, uint64_t *n_tris, uint32_t *max_stack, uint64_t *bytes) {
+    if (!s) return set_error(WR_ERR_INVALID_ARGUMENT, "scene is null");
+    if (n_nodes) *n_nodes = s->n_nodes;
+    if (n_tris) *n_tris = s->n_tris;
+    if (max_stack) *max_stack = s->max_stack;
+    if (bytes) *bytes = s->device_bytes;
+    return WR_OK;
+}
+
+int wr_film_create(wr_scene *s, wr_film **out) {
+    if (!s || !out) return set_error(WR_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    int rc = set_device(s->dev);
+    if (rc) return rc;
+    wr_film *f = new wr_film;
+    f->scene = s;
+    f->W = s->W;
+    f->H = s->H;
+    const size_t bytes = sizeof(double) * 3 * size_t(f->W) * f->H;
+    hipError_t e = hipMalloc(&f->d_base, bytes);
+    if (e == hipSuccess) e = hipMalloc(&f->d_splat, bytes);
+    if (e != hipSuccess) {
+        (void)hipFree(f->d_base);
+        delete f;
+        return set_error(e == hipErrorOutOfMemory ? WR_ERR_OUT_OF_MEMORY : WR_ERR_DEVICE, hipGetErrorString(e));
+    }
+    *out = f;
+    return wr_film_clear(f);
+}
+
+void wr_film_destroy(wr_film *f) {
+    if (!f) return;
+    (void)hipSetDevice(f->scene->dev->ordinal);
+    (void)hipStreamSynchronize(f->scene->stream);
+    (void)hipFree(f->d_base);
+    (void)hipFree(f->d_splat);
+    delete f;
+}
+
+int wr_film_clear(wr_film *f) {
+    if (!f) return set_error(WR_ERR_INVALID_ARGUMENT, "film is null");
+    int rc = set_device(f->scene->dev);
+    if (rc) return rc;
+    const size_t n = 3 * size_t(f->W) * f->H;
+    k_fill_f64<<<1024, 256, 0, f->scene->stream>>>(f->d_base, n);
+    k_fill_f64<<<1024, 256, 0, f->scene->stream>>>(f->d_splat, n);
+    WR_HIP(hipGetLastError());
+    WR_HIP(hipStreamSynchronize(f->scene->stream));
+    return WR_OK;
+}
+
+int wr_film_read(wr_film *f, double *base, double *splat) {
+    if (!f) return set_error(WR_ERR_INVALID_ARGUMENT, "film is null");
+    int rc = set_device(f->scene->dev);
+    if (rc) return rc;
+    const size_t bytes = sizeof(double) * 3 * size_t(f->W) * f->H;
+    WR_HIP(hipStreamSynchronize(f->scene->stream));
+    if (base) WR_HIP(hipMemcpy(base, f->d_base, bytes, hipMemcpyDeviceToHost));
+    if (splat) WR_HIP(hipMemcpy(splat, f->d_splat, bytes, hipMemcpyDeviceToHost));
+    return WR_OK;
+}
+
+// Path-state pool: per slot 16*4 (ray_o, ray_d, thr, rad) + 4 key + 8 hit + 8 queues
+// + 2 shadow entries * 48.
+static int ensure_capacity(wr_scene *s, uint32_t cap) {
+    if (s->capacity >= cap) return WR_OK;
+    if (s->pool) {
+        WR_HIP(hipStreamSynchronize(s->stream));
+        WR_HIP(hipFree(s->pool));
+        s->pool = nullptr;
+        s->capacity = 0;
+    }
+    auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+    const size_t c = cap;
+    const size_t sizes[] = {al(16 * c), al(16 * c), al(16 * c), al(16 * c), al(4 * c), al(8 * c),
+                            al(4 * c),  al(4 * c),  al(32 * c), al(32 * c), al(32 * c), al(64), al(64)};
+    size_t total = 0;
+    for (size_t b : sizes) total += b;
+    char *p = nullptr;
+    WR_HIP(hipMalloc(reinterpret_cast<void **>(&p), total));
+    s->pool = p;
+    PathBuffers &pb = s->pb;
+    size_t off = 0;
+    auto take = [&](size_t i) { char *q = p + off; off += sizes[i]; return q; };
+    pb.ray_o = reinterpret_cast<float4 *>(take(0));
+    pb.ray_d = reinterpret_cast<float4 *>(take(1));
+    pb.thr = reinterpret_cast<float4 *>(take(2));
+    pb.rad = reinterpret_cast<float4 *>(take(3));
+    pb.key = reinterpret_cast<uint32_t *>(take(4));
+    pb.hit = reinterpret_cast<uint2 *>(take(5));
+    pb.queue[0] = reinterpret_cast<uint32_t *>(take(6));
+    pb.queue[1] = reinterpret_cast<uint32_t *>(take(7));
+    pb.sh_o = reinterpret_cast<float4 *>(take(8));
+    pb.sh_d = reinterpret_cast<float4 *>(take(9));
+    pb.sh_c = reinterpret_cast<float4 *>(take(10));
+    pb.counters = reinterpret_cast<uint32_t *>(take(11));
+    pb.stats = reinterpret_cast<unsigned long long *>(take(12));
+    s->capacity = cap;
+    return WR_OK;
+}
+
+static int grid_for(const wr_scene *s, uint32_t n, int per_cu) {
+    const uint32_t blocks = (n + kBlock - 1) / kBlock;
+    return int(std::max<uint32_t>(1, std::min<uint32_t>(blocks, uint32_t(s->dev->n_cu) * per_cu)));
+}
+// Traversal grids index the spill region by block, so they never exceed max_trav_blocks.
+static int trav_grid(const wr_scene *s, uint32_t n, int per_cu) {
+    return std::min(grid_for(s, n, std::min(per_cu, kMaxTraversalBlocksPerCU)), s->max_trav_blocks);
+}
+
+int wr_render_nee(wr_scene *s, wr_film *f, const wr_render_params *prm, wr_render_stats *stats) {
+    if (!s || !f || !prm) return set_error(WR_ERR_INVALID_ARGUMENT, "null argument");
+    if (f->scene != s) return set_error(WR_ERR_INVALID_ARGUMENT, "film belongs to another scene");
+    if (prm->spp < 0) return set_error(WR_ERR_INVALID_ARGUMENT, "spp must be >= 0");
+    const auto t0 = std::chrono::steady_clock::now();
+    wr_render_stats st{};
+    int rc = set_device(s->dev);
+    if (rc) return rc;
+    const int N = prm->num_path_vertices;
+    const uint32_t wh = uint32_t(s->W) * uint32_t(s->H);
+    if (N <= 1 || prm->spp == 0) {  // pathnextevent.h:163
+        if (stats) *stats = st;
+        return WR_OK;
+    }
+    uint64_t max_paths = prm->max_paths_in_flight > 0 ? uint64_t(prm->max_paths_in_flight) : (uint64_t(1) << 22);
+    max_paths = std::max<uint64_t>(max_paths, wh);
+    if (max_paths > (uint64_t(1) << 30)) return set_error(WR_ERR_UNSUPPORTED, "too many paths in flight");
+    const uint32_t spp_per_batch = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(max_paths / wh, uint64_t(prm->spp))));
+    rc = ensure_capacity(s, spp_per_batch * wh);
+    if (rc) return rc;
+    PathBuffers pb = s->pb;
+    pb.film_base = f->d_base;
+    pb.film_splat = f->d_splat;
+    k_fill_u32<<<1, 64, 0, s->stream>>>(reinterpret_cast<uint32_t *>(pb.stats), 0u, 12);
+    WR_HIP(hipGetLastError());
+    const bool prof = prm->profile != 0;
+    const bool count = prm->count_traversal != 0;
+    std::vector<hipEvent_t> evs;
+    struct Mark { int kind; size_t e0; };
+    std::vector<Mark> marks;
+    auto ev_record = [&]() -> size_t {
+        hipEvent_t e;
+        (void)hipEventCreate(&e);
+        (void)hipEventRecord(e, s->stream);
+        evs.push_back(e);
+        return evs.size() - 1;
+    };
+    auto begin = [&](int kind) { if (prof) marks.push_back({kind, ev_record()}); };
+    auto end = [&]() { if (prof) ev_record(); };
+    const int g_trav = 0;  // per-CU multiplier chosen below
+    (void)g_trav;
+    for (uint32_t done = 0; done < uint32_t(prm->spp); done += spp_per_batch) {
+        const uint32_t k = std::min<uint32_t>(spp_per_batch, uint32_t(prm->spp) - done);
+        RenderArgs a;
+        a.n_paths = k * wh;
+        a.wh = wh;
+        a.first_sample = uint32_t(prm->first_sample) + done;
+        a.seed = prm->seed;
+        a.N = N;
+        a.use_mis = prm->use_mis ? 1 : 0;
+        k_reset<<<1, 1, 0, s->stream>>>(pb, a.n_paths);
+        begin(3);
+        k_generate<<<grid_for(s, a.n_paths, 16), kBlock, 0, s->stream>>>(s->ds, pb, a);
+        end();
+        for (int b = 0; b < N - 1; b++) {
+            uint32_t *qc = pb.queue[b & 1], *qn = pb.queue[(b + 1) & 1];
+            begin(0);
+            if (count) k_extend<true><<<trav_grid(s, a.n_paths, 8), kBlock, 0, s->stream>>>(s->ds, pb, qc, s->d_spill);
+            else k_extend<false><<<trav_grid(s, a.n_paths, 8), kBlock, 0, s->stream>>>(s->ds, pb, qc, s->d_spill);
+            end();
+            begin(2);
+            k_shade<<<grid_for(s, a.n_paths, 16), kBlock, 0, s->stream>>>(s->ds, pb, a, qc, qn, b);
+            end();
+            begin(1);
+            if (count) k_shadow<true><<<trav_grid(s, 2 * a.n_paths, 8), kBlock, 0, s->stream>>>(s->ds, pb, s->d_spill);
+            else k_shadow<false><<<trav_grid(s, 2 * a.n_paths, 8), kBlock, 0, s->stream>>>(s->ds, pb, s->d_spill);
+            end();
+            k_rotate<<<1, 1, 0, s->stream>>>(pb);
+        }
+        begin(4);
+        k_accumulate<<<grid_for(s, wh, 16), 256, 0, s->stream>>>(pb, wh, k);
+        end();
+        WR_HIP(hipGetLastError());
+    }
+    unsigned long long hs[6] = {0};
+    WR_HIP(hipMemcpyAsync(hs, pb.stats, sizeof hs, hipMemcpyDeviceToHost, s->stream));
+    WR_HIP(hipStreamSynchronize(s->stream));
+    st.paths = uint64_t(prm->spp) * wh;
+    st.extend_rays = hs[0];
+    st.shadow_rays = hs[1];
+    st.extend_nodes = hs[2];
+    st.extend_tris = hs[3];
+    st.shadow_nodes = hs[4];
+    st.shadow_tris = hs[5];
+    if (prof) {
+        for (const Mark &m : marks) {
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, evs[m.e0], evs[m.e0 + 1]);
+            switch (m.kind) {
+            case 0: st.extend_ms += ms; st.extend_launches++; break;
+            case 1: st.shadow_ms += ms; st.shadow_launches++; break;
+            case 2: st.shade_ms += ms; break;
+            case 3: st.generate_ms += ms; break;
+            default: st.accumulate_ms += ms; break;
+            }
+        }
+    }
+    for (hipEvent_t e : evs) (void)hipEventDestroy(e);
+    st.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (stats) *stats = st;
+    return WR_OK;
+}
+
+static int trace_impl(wr_scene *s, const float *d_rays, uint32_t n, void *d_hits, uint8_t *d_occ, bool any, float *ms) {
+    int rc = set_device(s->dev);
+    if (rc) return rc;
+    if (n == 0) { if (ms) *ms = 0.f; return WR_OK; }
+    const int grid = trav_grid(s, n, 8);
+    if (ms) WR_HIP(hipEventRecord(s->ev[0], s->stream));
+    if (any) k_trace<true><<<grid, kBlock, 0, s->stream>>>(s->ds, reinterpret_cast<const float4 *>(d_rays), n, nullptr,
+                                                           d_occ, s->d_spill);
+    else k_trace<false><<<grid, kBlock, 0, s->stream>>>(s->ds, reinterpret_cast<const float4 *>(d_rays), n,
+                                                         reinterpret_cast<uint2 *>(d_hits), nullptr, s->d_spill);
+    WR_HIP(hipGetLastError());
+    if (ms) {
+        WR_HIP(hipEventRecord(s->ev[1], s->stream));
+        WR_HIP(hipEventSynchronize(s->ev[1]));
+        WR_HIP(hipEventElapsedTime(ms, s->ev[0], s->ev[1]));
+    }
+    return WR_OK;
+}
+
+int wr_trace_closest_device(wr_scene *s, const float *d_rays, uint32_t n, void *d_hits, float *ms) {
+    if (!s || (n && (!d_rays || !d_hits))) return set_error(WR_ERR_INVALID_ARGUMENT, "null argument");
+    return trace_impl(s, d_rays, n, d_hits, nullptr, false, ms);
+}
+
+int wr_trace_any_device(wr_scene *s, const float *d_rays, uint32_t n, uint8_t *d_occ, float *ms) {
+    if (!s || (n && (!d_rays || !d_occ))) return set_error(WR_ERR_INVALID_ARGUMENT, "null argument");
+    return trace_impl(s, d_rays, n, nullptr, d_occ, true, ms);
+}
+
+int wr_trace_closest(wr_scene *s, const float *rays, uint32_t n, uint32_t *geom, uint32_t *prim, float *t) {
+    if (!s || (n && (!rays || !geom || !prim || !t))) return set_error(WR_ERR_INVALID_ARGUMENT, "null argument");
+    if (n == 0) return WR_OK;
+    int rc = set_device(s->dev);
+    if (rc) return rc;
+    float *d_rays = nullptr;
+    uint2 *d_hits = nullptr;
+    WR_HIP(hipMalloc(&d_rays, sizeof(float) * 8 * size_t(n)));
+    hipError_t e = hipMalloc(&d_hits, sizeof(uint2) * size_t(n));
+    if (e != hipSuccess) { (void)hipFree(d_rays); return set_error(WR_ERR_OUT_OF_MEMORY, hipGetErrorString(e)); }
+    std::vector<uint2> hits(n);
+    e = hipMemcpy(d_rays, rays, sizeof(float) * 8 * size_t(n), hipMemcpyHostToDevice);
+    if (e == hipSuccess) rc = trace_impl(s, d_rays, n, d_hits, nullptr, false, nullptr);
+    if (e == hipSuccess && rc == WR_OK) e = hipStreamSynchronize(s->stream);
+    if (e == hipSuccess && rc == WR_OK) e = hipMemcpy(hits.data(), d_hits, sizeof(uint2) * size_t(n), hipMemcpyDeviceToHost);
+    (void)hipFree(d_rays);
+    (void)hipFree(d_hits);
+    if (rc) return rc;
+    if (e != hipSuccess) return set_error(WR_ERR_DEVICE, hipGetErrorString(e));
+    for (uint32_t i = 0; i < n; i++) {
+        if (hits[i].x == kInvalid) {
+            geom[i] = prim[i] = WR_INVALID_ID;
+            t[i] = INFINITY;
+        } else {
+            const uint32_t orig = s->leaf_to_orig[hits[i].x];
+            geom[i] = s->tri_geom[orig];
+            prim[i] = orig - s->geom_first[geom[i]];
+            std::memcpy(&t[i], &hits[i].y, 4);
+        }
+    }
+    return WR_OK;
+}
+
+int wr_trace_any(wr_scene *s, const float *rays, uint32_t n, uint8_t *occluded) {
+    if (!s || (n && (!rays || !occluded))) return set_error(WR_ERR_INVALID_ARGUMENT, "null argument");
+    if (n == 0) return WR_OK;
+    int rc = set_device(s->dev);
+    if (rc) return rc;
+    float *d_rays = nullptr;
+    uint8_t *d_occ = nullptr;
+    WR_HIP(hipMalloc(&d_rays, sizeof(float) * 8 * size_t(n)));
+    hipError_t e = hipMalloc(&d_occ, size_t(n));
+    if (e != hipSuccess) { (void)hipFree(d_rays); return set_error(WR_ERR_OUT_OF_MEMORY, hipGetErrorString(e)); }
+    e = hipMemcpy(d_rays, rays, sizeof(float) * 8 * size_t(n), hipMemcpyHostToDevice);
+    if (e == hipSuccess) rc = trace_impl(s, d_rays, n, nullptr, d_occ, true, nullptr);
+    if (e == hipSuccess && rc == WR_OK) e = hipStreamSynchronize(s->stream);
+    if (e == hipSuccess && rc == WR_OK) e = hipMemcpy(occluded, d_occ, size_t(n), hipMemcpyDeviceToHost);
+    (void)hipFree(d_rays);
+    (void)hipFree(d_occ);
+    if (rc) return rc;
+    if (e != hipSuccess) return set_error(WR_ERR_DEVICE, hipGetErrorString(e));
+    return WR_OK;
+}
+

@@ -1,0 +1,211 @@
+// wr_traverse.h -- BVH4 traversal with a per-lane LDS stack and the
+// watertight ray/triangle test (Woop, Benthin, Wald, JCGT 2013), FP32 with an
+// FP64 fallback on edge hits.
+//
+// Semantics follow rtcIntersect1 / rtcOccluded1 as the reference calls them
+// (src/accel/embreeaccel.h:336-422): double-sided triangles, hits accepted
+// for t in (tmin, tmax], closest hit for extension rays, any hit for shadow
+// rays.  Node boxes are tested conservatively (far distance scaled up by
+// 1 + 2^-21) so that no watertight hit is lost to rounding in the box test.
+#pragma once
+#include "wr_device.h"
+
+namespace wr {
+
+constexpr uint32_t kInvalid = 0xFFFFFFFFu;
+
+struct RaySetup {
+    float3 o, d;
+    float tmin;
+    float ix, iy, iz;          // 1/d with zero components replaced by +-1e-20
+    int kx, ky, kz;            // watertight axis permutation
+    float Sx, Sy, Sz;
+    bool nx, ny, nz;           // direction sign per axis (picks near/far slab)
+};
+
+__device__ __forceinline__ float pick(float3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+
+__device__ __forceinline__ RaySetup setup_ray(float3 o, float3 d, float tmin) {
+    RaySetup r;
+    r.o = o; r.d = d; r.tmin = tmin;
+    auto safe = [](float x) { return fabsf(x) < 1e-20f ? copysignf(1e-20f, x) : x; };
+    r.ix = 1.0f / safe(d.x); r.iy = 1.0f / safe(d.y); r.iz = 1.0f / safe(d.z);
+    r.nx = r.ix < 0.f; r.ny = r.iy < 0.f; r.nz = r.iz < 0.f;
+    float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+    int kz = (ax >= ay) ? ((ax >= az) ? 0 : 2) : ((ay >= az) ? 1 : 2);
+    int kx = kz == 2 ? 0 : kz + 1;
+    int ky = kx == 2 ? 0 : kx + 1;
+    if (pick(d, kz) < 0.f) { int t = kx; kx = ky; ky = t; }
+    r.kx = kx; r.ky = ky; r.kz = kz;
+    float dz = pick(d, kz);
+    r.Sx = pick(d, kx) / dz; r.Sy = pick(d, ky) / dz; r.Sz = 1.0f / dz;
+    return r;
+}
+
+// FP64 recomputation of the watertight test (edge and vertex hits)
+__device__ __noinline__ bool tri_test_fp64(const RaySetup &r, float3 a, float3 b, float3 c, double tmin, double tmax,
+                                           double &t_out) {
+    double o[3] = {r.o.x, r.o.y, r.o.z};
+    double A[3] = {a.x - o[0], a.y - o[1], a.z - o[2]};
+    double B[3] = {b.x - o[0], b.y - o[1], b.z - o[2]};
+    double C[3] = {c.x - o[0], c.y - o[1], c.z - o[2]};
+    double d[3] = {r.d.x, r.d.y, r.d.z};
+    double Sx = d[r.kx] / d[r.kz], Sy = d[r.ky] / d[r.kz], Sz = 1.0 / d[r.kz];
+    double Ax = A[r.kx] - Sx * A[r.kz], Ay = A[r.ky] - Sy * A[r.kz];
+    double Bx = B[r.kx] - Sx * B[r.kz], By = B[r.ky] - Sy * B[r.kz];
+    double Cx = C[r.kx] - Sx * C[r.kz], Cy = C[r.ky] - Sy * C[r.kz];
+    double U = Cx * By - Cy * Bx, V = Ax * Cy - Ay * Cx, W = Bx * Ay - By * Ax;
+    if ((U < 0 || V < 0 || W < 0) && (U > 0 || V > 0 || W > 0)) return false;
+    double det = U + V + W;
+    if (det == 0.0) return false;
+    double T = Sz * (U * A[r.kz] + V * B[r.kz] + W * C[r.kz]);
+    double t = T / det;
+    if (!(t > tmin && t <= tmax)) return false;
+    t_out = t;
+    return true;
+}
+
+// Watertight FP32 test; returns true and t for a hit in (tmin, tmax].
+__device__ __forceinline__ bool tri_test(const RaySetup &r, float4 va, float4 vb, float4 vc, float tmax, float &t_out) {
+    const float3 A = xyz(va) - r.o, B = xyz(vb) - r.o, C = xyz(vc) - r.o;
+    const float Akz = pick(A, r.kz), Bkz = pick(B, r.kz), Ckz = pick(C, r.kz);
+    const float Ax = fmaf(-r.Sx, Akz, pick(A, r.kx)), Ay = fmaf(-r.Sy, Akz, pick(A, r.ky));
+    const float Bx = fmaf(-r.Sx, Bkz, pick(B, r.kx)), By = fmaf(-r.Sy, Bkz, pick(B, r.ky));
+    const float Cx = fmaf(-r.Sx, Ckz, pick(C, r.kx)), Cy = fmaf(-r.Sy, Ckz, pick(C, r.ky));
+    float U = Cx * By - Cy * Bx;
+    float V = Ax * Cy - Ay * Cx;
+    float W = Bx * Ay - By * Ax;
+    if (U == 0.f || V == 0.f || W == 0.f) {
+        double t;
+        if (!tri_test_fp64(r, xyz(va), xyz(vb), xyz(vc), r.tmin, tmax, t)) return false;
+        t_out = float(t);
+        return true;
+    }
+    if ((U < 0.f || V < 0.f || W < 0.f) && (U > 0.f || V > 0.f || W > 0.f)) return false;
+    const float det = U + V + W;
+    if (det == 0.f) return false;
+    const float T = r.Sz * (U * Akz + V * Bkz + W * Ckz);
+    const float t = T / det;
+    if (!(t > r.tmin && t <= tmax)) return false;
+    t_out = t;
+    return true;
+}
+
+// Per-lane traversal stack of (node ref, entry distance).  The first
+// kLdsStackDepth entries live in LDS at [k * kBlock + lane] (bank-conflict
+// free); deeper entries spill to a per-lane slice of global memory.  The BVH
+// builder guarantees at most kStackDepth live entries.
+template <int kBlock>
+struct LaneStack {
+    uint32_t *node;
+    float *tnear;
+    uint2 *spill;
+    int sp;
+    __device__ __forceinline__ void push(uint32_t ref, float t) {
+        if (sp < kLdsStackDepth) {
+            node[sp * kBlock] = ref;
+            tnear[sp * kBlock] = t;
+        } else {
+            spill[sp - kLdsStackDepth] = make_uint2(ref, __float_as_uint(t));
+        }
+        sp++;
+    }
+    __device__ __forceinline__ void top(uint32_t &ref, float &t) const {
+        if (sp < kLdsStackDepth) {
+            ref = node[sp * kBlock];
+            t = tnear[sp * kBlock];
+        } else {
+            const uint2 e = spill[sp - kLdsStackDepth];
+            ref = e.x;
+            t = __uint_as_float(e.y);
+        }
+    }
+};
+
+struct TraversalCounters {
+    uint32_t nodes = 0, tris = 0;
+};
+
+// Closest hit (ANY = false) or any hit (ANY = true).  Returns the leaf-order
+// triangle index or kInvalid; t_best is updated in place.
+template <bool ANY, bool COUNT, int kBlock>
+__device__ __forceinline__ uint32_t traverse(const DScene &s, const RaySetup &r, float &t_best, LaneStack<kBlock> &st,
+                                             TraversalCounters &cnt) {
+    uint32_t hit_tri = kInvalid;
+    uint32_t cur = 0;  // root (internal node 0)
+    st.sp = 0;
+    const float ox = r.o.x, oy = r.o.y, oz = r.o.z;
+    for (;;) {
+        if (!(cur & kLeafBit)) {
+            const float4 *np = reinterpret_cast<const float4 *>(s.nodes + cur);
+            const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
+            const uint4 ch = reinterpret_cast<const uint4 *>(np)[6];
+            if (COUNT) cnt.nodes++;
+            const float4 nx_ = r.nx ? hx : lx, fx_ = r.nx ? lx : hx;
+            const float4 ny_ = r.ny ? hy : ly, fy_ = r.ny ? ly : hy;
+            const float4 nz_ = r.nz ? hz : lz, fz_ = r.nz ? lz : hz;
+            float tn[4], tf[4];
+            tn[0] = fmaxf(fmaxf((nx_.x - ox) * r.ix, (ny_.x - oy) * r.iy), fmaxf((nz_.x - oz) * r.iz, r.tmin));
+            tn[1] = fmaxf(fmaxf((nx_.y - ox) * r.ix, (ny_.y - oy) * r.iy), fmaxf((nz_.y - oz) * r.iz, r.tmin));
+            tn[2] = fmaxf(fmaxf((nx_.z - ox) * r.ix, (ny_.z - oy) * r.iy), fmaxf((nz_.z - oz) * r.iz, r.tmin));
+            tn[3] = fmaxf(fmaxf((nx_.w - ox) * r.ix, (ny_.w - oy) * r.iy), fmaxf((nz_.w - oz) * r.iz, r.tmin));
+            tf[0] = fminf(fminf((fx_.x - ox) * r.ix, (fy_.x - oy) * r.iy), (fz_.x - oz) * r.iz);
+            tf[1] = fminf(fminf((fx_.y - ox) * r.ix, (fy_.y - oy) * r.iy), (fz_.y - oz) * r.iz);
+            tf[2] = fminf(fminf((fx_.z - ox) * r.ix, (fy_.z - oy) * r.iy), (fz_.z - oz) * r.iz);
+            tf[3] = fminf(fminf((fx_.w - ox) * r.ix, (fy_.w - oy) * r.iy), (fz_.w - oz) * r.iz);
+            uint32_t c[4] = {ch.x, ch.y, ch.z, ch.w};
+            float key[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const bool h = c[i] != kEmptyChild && tn[i] <= fminf(tf[i] * 1.00000048f, t_best);
+                key[i] = h ? tn[i] : INFINITY;
+            }
+            if (!ANY) {
+                // sort (key, child) ascending: 5-comparator network
+#define WR_CSWAP(a, b)                                                          \
+    if (key[b] < key[a]) {                                                      \
+        float tk = key[a]; key[a] = key[b]; key[b] = tk;                        \
+        uint32_t tc = c[a]; c[a] = c[b]; c[b] = tc;                             \
+    }
+                WR_CSWAP(0, 1) WR_CSWAP(2, 3) WR_CSWAP(0, 2) WR_CSWAP(1, 3) WR_CSWAP(1, 2)
+#undef WR_CSWAP
+            }
+            // push hits farthest first; continue with the nearest
+            uint32_t next = kInvalid;
+            float next_t = 0.f;
+#pragma unroll
+            for (int i = 3; i >= 0; i--) {
+                if (key[i] != INFINITY) {
+                    if (next != kInvalid) st.push(next, next_t);
+                    next = c[i];
+                    next_t = key[i];
+                }
+            }
+            if (next != kInvalid) { cur = next; continue; }
+        } else {
+            const uint32_t first = cur & 0x0FFFFFFFu;
+            const uint32_t count = ((cur >> 28) & 0x7u) + 1u;
+            if (COUNT) cnt.tris += count;
+            for (uint32_t k = 0; k < count; k++) {
+                const float4 *tp = s.tris + 3 * size_t(first + k);
+                float t;
+                if (tri_test(r, tp[0], tp[1], tp[2], t_best, t)) {
+                    t_best = t;
+                    hit_tri = first + k;
+                    if (ANY) return hit_tri;
+                }
+            }
+        }
+        // pop, skipping entries farther than the current hit
+        for (;;) {
+            if (st.sp == 0) return hit_tri;
+            st.sp--;
+            uint32_t ref;
+            float tn;
+            st.top(ref, tn);
+            if (tn <= t_best) { cur = ref; break; }
+        }
+    }
+}
+
+}  // namespace wr
