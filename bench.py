@@ -41,33 +41,52 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU time of the baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-count", action="store_true", help="skip the traversal-counting pass")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first copy 1 GiB device-to-device three times (known-size kernel for byte counters)")
     return ap.parse_args()
 
 
-def cpu_baseline(sc, spp_frame, N, target_s):
-    """The oracle (FP64 restatement of the reference path) on the host cores,
-    over full-width row bands at 1 spp, grown until ~target_s of CPU work."""
+def calibrate_copy(nbytes=1 << 30, reps=3):
+    """Device-to-device copies of a known size through the HIP runtime the
+    library links, so rocprofv3 byte counters can be scaled (FETCH_SIZE and
+    WRITE_SIZE of the copy kernel should each read nbytes)."""
+    import ctypes as C
+    import wurstrenderer_amd as wr
+    wr.lib()
+    path = next(line.split()[-1] for line in open("/proc/self/maps") if "libamdhip64.so" in line)
+    hip = C.CDLL(path)
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    hip.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
+    hip.hipFree.argtypes = [C.c_void_p]
+    src, dst = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(src), nbytes) == 0 and hip.hipMalloc(C.byref(dst), nbytes) == 0
+    assert hip.hipMemset(src, 1, nbytes) == 0
+    for _ in range(reps):
+        assert hip.hipMemcpy(dst, src, nbytes, 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    print(f"# calibration: {reps} x {nbytes} B device-to-device copies", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(sc, N, target_s):
+    """The oracle (FP64 restatement of the reference path) on the host cores:
+    whole frames at 1 spp, successive sample indices, until ~target_s of work."""
     import oracle
     threads = min(16, os.cpu_count() or 1)
     osc = oracle.OracleScene(sc)
-    W, H = sc.resolution
-    rows, total_rays, total_s, total_paths = 2, 0, 0.0, 0
-    y0 = H // 2 - 1
-    while True:
-        y1 = min(H, y0 + rows)
-        _, _, st = osc.render(1, num_path_vertices=N, seed=7, threads=threads, window=(0, y0, W, y1))
+    total_rays, total_s, total_paths, k = 0, 0.0, 0, 0
+    while total_s < target_s:
+        _, _, st = osc.render(1, num_path_vertices=N, seed=7, first_sample=k, threads=threads)
         total_rays += st.extend_rays + st.shadow_rays
         total_s += st.seconds
         total_paths += st.paths
-        if total_s >= target_s or y1 >= H:
-            break
-        per_row = st.seconds / (y1 - y0)
-        rows = max(1, min(H - y1, int((target_s - total_s) / max(per_row, 1e-6))))
-        y0 = y1
-    return {"value": total_rays / total_s / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "sample": f"oracle FP64, {total_paths} paths (full-width row band around the image centre, 1 spp, "
-                      f"N={N}) of the same scene, {total_s:.1f} s",
-            "samples_per_s": total_paths / total_s}
+        k += 1
+    return {"value": round(total_rays / total_s / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": f"oracle FP64 restatement, {k} full frame(s) at 1 spp = {total_paths} paths of the same "
+                      f"scene and depth, {total_s:.1f} s on {threads} threads",
+            "samples_per_s": round(total_paths / total_s, 1)}
 
 
 def main():
@@ -79,6 +98,8 @@ def main():
     import wurstrenderer_amd as wr
     from wurstrenderer_amd import scenes
 
+    if a.calibrate_copy:
+        calibrate_copy()
     N = a.depth + 2
     t0 = time.time()
     sc = scenes.icospheres(2, resolution=(a.width, a.height))
@@ -131,7 +152,7 @@ def main():
                 "bytes_per_ray": round(EXT_RAY_BYTES + nodes_per_ray * NODE_BYTES + tris_per_ray * TRI_BYTES, 1)}
     cpu = None
     if not a.no_cpu_baseline:
-        cpu = cpu_baseline(sc, a.spp, N, a.cpu_seconds)
+        cpu = cpu_baseline(sc, N, a.cpu_seconds)
     out = {
         "metric": "Mrays/s (plain GPU FP32 wavefront NEE+MIS, 1.31M-triangle scene)",
         "value": round(mrays, 2),

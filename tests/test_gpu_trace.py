@@ -91,7 +91,7 @@ def test_any_hit_soup_and_cornell_shadow_rays(gpu_device):
     oocc, amb = osc.trace_any(rays, brute=True)
     ok = amb == 0
     assert (occ[ok] == oocc[ok]).all()
-    assert 0.2 < oocc[ok].mean() < 0.9
+    assert 0.05 < oocc[ok].mean() < 0.9
     assert amb.mean() < 0.01
     # NEE-like rays in the Cornell box: surface points to points on the light
     sc = scenes.cornell(1)
@@ -151,22 +151,52 @@ def test_watertight_closed_icosphere_far_from_origin(gpu_device):
     assert (g2 == 0).all()
 
 
-def test_device_resident_entry_point_matches_host_one(gpu_device):
+def _hip():
+    """The HIP runtime the library links (already loaded in this process; found
+    through the process's own mappings so no second runtime is loaded)."""
     import ctypes as C
-    import torch
+    wr.lib()
+    path = next(line.split()[-1] for line in open("/proc/self/maps") if "libamdhip64.so" in line)
+    hip = C.CDLL(path)
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipFree.argtypes = [C.c_void_p]
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    hip.hipDeviceSynchronize.argtypes = []
+    return hip
+
+
+def test_device_resident_entry_point_matches_host_one(gpu_device):
+    """wr_trace_*_device on device buffers (allocated with the library's own HIP
+    runtime) gives the same hits as the host-buffer entry points."""
+    import ctypes as C
+    hip = _hip()
+    H2D, D2H = 1, 2
     sc = scenes.triangle_soup(2000, seed=13)
     gs = wr.GpuScene(gpu_device, sc)
     rays = scenes.random_rays(sc, 50000, seed=3)
     g, p, t = gs.trace_closest(rays)
-    d_rays = torch.from_numpy(rays).cuda()
-    d_hits = torch.zeros((len(rays), 2), dtype=torch.int32, device="cuda")
-    ms = C.c_float()
-    wr._check(wr.lib().wr_trace_closest_device(gs.h, d_rays.data_ptr(), len(rays), d_hits.data_ptr(), C.byref(ms)))
-    hits = d_hits.cpu().numpy().view(np.uint32)
-    t_dev = hits[:, 1].view(np.float32)
-    np.testing.assert_array_equal(np.isinf(t_dev), np.isinf(t))
-    np.testing.assert_array_equal(t_dev[np.isfinite(t)], t[np.isfinite(t)])
-    assert ms.value > 0
-    d_occ = torch.zeros(len(rays), dtype=torch.uint8, device="cuda")
-    wr._check(wr.lib().wr_trace_any_device(gs.h, d_rays.data_ptr(), len(rays), d_occ.data_ptr(), None))
-    np.testing.assert_array_equal(d_occ.cpu().numpy(), gs.trace_any(rays))
+    n = len(rays)
+    d_rays, d_hits, d_occ = C.c_void_p(), C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(d_rays), rays.nbytes) == 0
+    assert hip.hipMalloc(C.byref(d_hits), 8 * n) == 0
+    assert hip.hipMalloc(C.byref(d_occ), n) == 0
+    try:
+        assert hip.hipMemcpy(d_rays, rays.ctypes.data, rays.nbytes, H2D) == 0
+        ms = C.c_float()
+        wr._check(wr.lib().wr_trace_closest_device(gs.h, d_rays, n, d_hits, C.byref(ms)))
+        hits = np.zeros((n, 2), np.uint32)
+        assert hip.hipDeviceSynchronize() == 0
+        assert hip.hipMemcpy(hits.ctypes.data, d_hits, 8 * n, D2H) == 0
+        t_dev = hits[:, 1].view(np.float32)
+        np.testing.assert_array_equal(np.isinf(t_dev), np.isinf(t))
+        np.testing.assert_array_equal(t_dev[np.isfinite(t)], t[np.isfinite(t)])
+        assert ((hits[:, 0] == INV) == (g == INV)).all()
+        assert ms.value > 0
+        wr._check(wr.lib().wr_trace_any_device(gs.h, d_rays, n, d_occ, None))
+        occ = np.zeros(n, np.uint8)
+        assert hip.hipDeviceSynchronize() == 0
+        assert hip.hipMemcpy(occ.ctypes.data, d_occ, n, D2H) == 0
+        np.testing.assert_array_equal(occ, gs.trace_any(rays))
+    finally:
+        for ptr in (d_rays, d_hits, d_occ):
+            hip.hipFree(ptr)

@@ -20,7 +20,7 @@ constexpr uint32_t kEmptyChild = 0xFFFFFFFFu;
 constexpr int kMaxLeafTris = 4;
 constexpr uint32_t kMaxTriangles = 1u << 28;  // leaf refs hold 28-bit triangle offsets
 constexpr int kStackDepth = 64;     // traversal stack entries per ray (worst case the builder allows)
-constexpr int kLdsStackDepth = 16;  // entries kept in LDS; deeper entries spill to a per-lane global region
+constexpr int kLdsStackDepth = 8;   // entries kept in LDS (4 KB per wave); deeper entries spill to a per-lane global region
 constexpr int kSpillDepth = kStackDepth - kLdsStackDepth;
 constexpr int kMaxTraversalBlocksPerCU = 32;  // bound on traversal grid size (sizes the spill region)
 

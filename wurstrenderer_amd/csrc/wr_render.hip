@@ -2,21 +2,26 @@
 //
 // Reference: PathNeeRenderer (src/renderer/pathnextevent.h:15-196) and the
 // random walk of GeneralSubpathSampler (src/pathsampler/generalsubpathsampler.h:13-160).
-// The recursive per-sample loop of the reference becomes, per batch of paths:
+// The reference's recursive per-sample loop over tiles becomes a wavefront
+// over a fixed pool of paths, one iteration per path vertex:
 //
-//   generate    camera ray + light vertex + light->camera (t = 1) connection
-//   repeat N-1 times:
-//     extend    closest hit for the active queue               (BVH4 traversal)
-//     shade     emitter hit with MIS, NEE with MIS, BSDF sample  (SoA path state)
-//     shadow    any hit for NEE and t = 1 rays; add visible contributions
-//     rotate    next queue becomes current (device-side counters)
-//   accumulate  per-pixel FP64 sums of the batch's path radiance
+//   regen     start the next unclaimed samples behind the continuing paths:
+//             light vertex, camera ray, light->camera (t = 1) connection
+//   extend    closest hit for every active path                (BVH4 traversal)
+//   shade     emitter hit with MIS, NEE with MIS, BSDF sample; continuing paths
+//             are compacted into the next state, a finished path stores its
+//             radiance under its sample index
+//   shadow    any hit for NEE and t = 1 rays; add visible contributions
+//   rotate    the next state becomes current (device-side counters)
+//   (per pass) accumulate: per-pixel FP64 sums of the samples' radiance
 //
-// Path state lives in HBM as SoA arrays indexed by path slot; queues hold slot
-// indices and are compacted with one atomic per wave (ballot + mbcnt).  Every
-// kernel reads its work count from device memory, so a batch needs no host
-// round trip.  Radiance is accumulated per path in FP32 and summed into the
-// FP64 film per pixel; t = 1 splats go to the FP64 splat film with atomics.
+// Refilling the pool as paths end keeps every kernel's queue full until the
+// last samples drain.  Path state is stored in queue order, so every kernel
+// reads and writes it contiguously; compaction is one atomic per wave (ballot
+// + mbcnt).  Every kernel reads its work count from device memory, and the
+// host polls the counters only every few iterations.  Radiance is gathered in
+// FP32 per path and summed per pixel in FP64 in sample order (reproducible);
+// t = 1 splats go to the FP64 splat film with atomics.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -49,65 +54,136 @@ static int set_error(int code, const std::string &msg) {
     } while (0)
 
 // ============================================================== kernel arguments
-constexpr uint32_t kSplatFlag = 0x80000000u;
 constexpr int kBlock = 64;  // one wavefront per block; LDS stack indexed by lane
 
+// Shadow-ray targets: where a visible connection's contribution goes.
+constexpr uint32_t kTargetMask = 0x3FFFFFFFu;
+constexpr uint32_t kToNextPath = 0u << 30;   // radiance of a continuing path (next-state index)
+constexpr uint32_t kToSample = 1u << 30;     // per-sample radiance of a finished path
+constexpr uint32_t kToSplat = 2u << 30;      // light->camera splat (pixel of the FP64 splat film)
+
+// Path state in queue order (SoA, double buffered): entry i of the current
+// state is the i-th active path.  Shade writes continuing paths compacted into
+// the next state; regen appends new samples behind them.  Every kernel walks
+// the state in order, so all path-state traffic is coalesced.
+struct PathState {
+    float4 *ray_o, *ray_d;  // ray to extend
+    float4 *thr;            // rgb: throughput of the next vertex; w: pdfW of the ray direction
+    float4 *rad;            // rgb: radiance gathered so far (FP32)
+    uint4 *meta;            // x: RNG key, y: sample index in the pass, z: bounce of the next vertex
+};
+
 struct PathBuffers {
-    float4 *ray_o, *ray_d;    // origin / direction of the ray to extend
-    float4 *thr;              // rgb: path throughput of the next vertex; w: pdfW of that direction
-    float4 *rad;              // rgb: accumulated radiance (FP32)
-    uint32_t *key;            // RNG key of the path
-    uint2 *hit;               // leaf-order triangle (or kInvalid), t bits
-    uint32_t *queue[2];       // active slot queues (ping-pong)
+    PathState cur, next;
+    uint2 *hit;                  // per current entry: leaf-order triangle (or kInvalid), t bits
+    float4 *sample_rad;          // per sample of the pass: final radiance (FP32)
     float4 *sh_o, *sh_d, *sh_c;  // shadow rays: o.xyz+tmax, d.xyz+target, contribution
-    uint32_t *counters;       // [0] active, [1] next, [2] shadow
-    unsigned long long *stats;  // [0] extend rays [1] shadow rays [2] ext nodes [3] ext tris [4] shd nodes [5] shd tris
+    uint32_t *counters;          // see Counter
+    unsigned long long *stats;   // [0] extend rays [1] shadow rays [2] ext nodes [3] ext tris [4] shd nodes [5] shd tris
     double *film_base, *film_splat;
 };
 
+// Each counter sits on its own 256-byte line: atomics on one line serialise.
+constexpr int kCounterStride = 64;
+enum Counter : int {
+    kQ = 0 * kCounterStride,         // continuing paths at the head of the current state
+    kFresh = 1 * kCounterStride,     // new samples started behind them this iteration
+    kQn = 2 * kCounterStride,        // paths continuing into the next state
+    kS = 3 * kCounterStride,         // shadow rays
+    kFetchExt = 4 * kCounterStride,  // work counter of the extend kernel
+    kFetchShd = 5 * kCounterStride,  // work counter of the shadow kernel
+    kClaimed = 6 * kCounterStride,   // samples started so far in the pass
+    kBase = 7 * kCounterStride,      // first sample index started this iteration
+    kNumCounterLines = 8
+};
+
 struct RenderArgs {
-    uint32_t n_paths;         // paths in this batch = spp_batch * W * H
+    uint32_t total;           // samples of this pass = spp * W * H
     uint32_t wh;
-    uint32_t first_sample;    // sample index of slot 0
+    uint32_t first_sample;    // sample index of the pass's first spp
     uint32_t seed;
+    uint32_t capacity;        // path pool size
     int N;                    // num_path_vertices
     int use_mis;
 };
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
-// Wave-aggregated queue append: one atomic per wave.  All lanes must call it.
+__device__ __forceinline__ uint32_t mbcnt64(unsigned long long mask) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+// Wave-aggregated append: one atomic per wave.  All lanes must call it.
 __device__ __forceinline__ uint32_t wave_append(uint32_t *counter, bool pred) {
     const unsigned long long mask = __ballot(pred);
     if (mask == 0) return kInvalid;
-    const uint32_t lane = lane_id();
     const uint32_t leader = __ffsll((long long)mask) - 1;
     uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+    if (lane_id() == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
     base = __shfl(base, leader);
-    const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-    return pred ? base + below : kInvalid;
+    return pred ? base + mbcnt64(mask) : kInvalid;
+}
+
+// Block-aggregated append for kShadeBlock-thread blocks: one atomic per block.
+// All threads of the block must call it (it synchronises the block).
+constexpr int kShadeBlock = 256;
+constexpr int kWavesPerShadeBlock = kShadeBlock / 64;
+__device__ __forceinline__ uint32_t block_append(uint32_t *counter, bool pred, uint32_t *lds /* kWavesPerShadeBlock+1 */) {
+    const unsigned long long mask = __ballot(pred);
+    const uint32_t wave = threadIdx.x / 64;
+    if (lane_id() == 0) lds[wave] = (uint32_t)__popcll(mask);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t total = 0;
+        for (int w = 0; w < kWavesPerShadeBlock; w++) {
+            const uint32_t k = lds[w];
+            lds[w] = total;
+            total += k;
+        }
+        lds[kWavesPerShadeBlock] = total ? atomicAdd(counter, total) : 0u;
+    }
+    __syncthreads();
+    const uint32_t pos = lds[kWavesPerShadeBlock] + lds[wave] + mbcnt64(mask);
+    __syncthreads();  // lds is reused by the next call
+    return pred ? pos : kInvalid;
 }
 
 __device__ __forceinline__ void wave_add_stat(unsigned long long *dst, uint32_t v) {
-    // sum over the wave, one atomic
     uint32_t s = v;
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
     if (lane_id() == 0 && s) atomicAdd(dst, (unsigned long long)s);
 }
 
-// ============================================================== small control kernels
-__global__ void k_reset(PathBuffers pb, uint32_t n_active) {
-    pb.counters[0] = n_active;
-    pb.counters[1] = 0;
-    pb.counters[2] = 0;
+__device__ __forceinline__ uint32_t active_count(const uint32_t *c) { return c[kQ] + c[kFresh]; }
+
+// ============================================================== control kernels
+// Number of new samples the next iteration starts: the free room of the pool
+// or what is left of the pass.  Written here so regen needs no atomics.
+__device__ __forceinline__ void plan_fresh(uint32_t *c, const RenderArgs &a) {
+    const uint32_t head = c[kQ];
+    const uint32_t left = a.total - min(c[kClaimed], a.total);
+    const uint32_t fresh = min(a.capacity - head, left);
+    c[kBase] = c[kClaimed];
+    c[kFresh] = fresh;
+    c[kClaimed] += fresh;
 }
-__global__ void k_rotate(PathBuffers pb) {
-    pb.stats[0] += pb.counters[0];
-    pb.stats[1] += pb.counters[2];
-    pb.counters[0] = pb.counters[1];
-    pb.counters[1] = 0;
-    pb.counters[2] = 0;
+__global__ void k_init(PathBuffers pb, RenderArgs a) {
+    if (threadIdx.x < kNumCounterLines) pb.counters[threadIdx.x * kCounterStride] = 0u;
+    __syncthreads();
+    if (threadIdx.x == 0) plan_fresh(pb.counters, a);
+}
+// End of an iteration: the next state becomes current, work counters restart,
+// and the next refill is planned.
+__global__ void k_rotate(PathBuffers pb, RenderArgs a) {
+    uint32_t *c = pb.counters;
+    pb.stats[0] += active_count(c);
+    pb.stats[1] += c[kS];
+    c[kQ] = c[kQn];
+    c[kQn] = 0;
+    c[kS] = 0;
+    c[kFetchExt] = 0;
+    c[kFetchShd] = 0;
+    plan_fresh(c, a);
 }
 __global__ void k_fill_u32(uint32_t *p, uint32_t v, size_t n) {
     for (size_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) p[i] = v;
@@ -116,19 +192,28 @@ __global__ void k_fill_f64(double *p, size_t n) {
     for (size_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) p[i] = 0.0;
 }
 
-// ============================================================== generate
-// createLightSubpath + createCameraSubpath up to the first ray, including the
-// t = 1 light->camera connection (pathnextevent.h:53-99; thinlens.h:121-166).
-__global__ __launch_bounds__(kBlock) void k_generate(DScene s, PathBuffers pb, RenderArgs a) {
+// ============================================================== regenerate
+// Fills the pool behind the continuing paths with the next unclaimed samples:
+// createLightSubpath + createCameraSubpath up to the first ray, with the t = 1
+// light->camera connection (pathnextevent.h:53-99; thinlens.h:121-166).
+__global__ __launch_bounds__(kShadeBlock) void k_regen(DScene s, PathBuffers pb, RenderArgs a) {
+    __shared__ uint32_t agg[kWavesPerShadeBlock + 1];
     const DCamera &cam = s.cam;
-    for (uint32_t base = blockIdx.x * kBlock; base < a.n_paths; base += gridDim.x * kBlock) {
-        const uint32_t slot = base + lane_id();
-        const bool valid = slot < a.n_paths;
+    const uint32_t head = pb.counters[kQ];
+    const uint32_t fresh_n = pb.counters[kFresh];
+    const uint32_t sbase = pb.counters[kBase];
+    const PathState st = pb.cur;
+    for (uint32_t base = blockIdx.x * kShadeBlock; base < fresh_n; base += gridDim.x * kShadeBlock) {
+        const uint32_t i = base + threadIdx.x;
+        const bool fresh = i < fresh_n;
+        const uint32_t sidx = sbase + i;
         bool splat = false;
         float4 so = make_float4(0, 0, 0, 0), sd = so, sc = so;
-        if (valid) {
-            const uint32_t pixel = slot % a.wh;
-            const uint32_t sample = a.first_sample + slot / a.wh;
+        float4 ro = so, rdir = so, thr = so;
+        uint4 meta = make_uint4(0, 0, 0, 0);
+        if (fresh) {
+            const uint32_t pixel = sidx % a.wh;
+            const uint32_t sample = a.first_sample + sidx / a.wh;
             const uint32_t px = pixel % uint32_t(cam.W), py = pixel / uint32_t(cam.W);
             const uint32_t key = path_key(a.seed, pixel, sample);
             const LightSample l = sample_light(s, u01(key, kDimLight), u01(key, kDimLight + 1));
@@ -161,7 +246,7 @@ __global__ __launch_bounds__(kBlock) void k_generate(DScene s, PathBuffers pb, R
                             mis = l.pdfA / (l.pdfA + camPdfA);
                         }
                         const float k = mis * we1 * G * kInvPi;
-                        const uint32_t target = kSplatFlag | (uint32_t(ry) * uint32_t(cam.W) + uint32_t(rx));
+                        const uint32_t target = kToSplat | (uint32_t(ry) * uint32_t(cam.W) + uint32_t(rx));
                         so = make_float4(l.pos.x, l.pos.y, l.pos.z, len - kRayEps);
                         sd = make_float4(nuv.x, nuv.y, nuv.z, __uint_as_float(target));
                         sc = make_float4(k * l.contrib.x, k * l.contrib.y, k * l.contrib.z, 0.f);
@@ -177,40 +262,111 @@ __global__ __launch_bounds__(kBlock) void k_generate(DScene s, PathBuffers pb, R
             const float3 dl = normalize3(focal - lens);
             const float pdfW = cam.per_pixel_norm / (dl.z * dl.z * dl.z);
             const float3 dir = cam_to_world(cam, dl);
-            pb.ray_o[slot] = make_float4(cpos.x, cpos.y, cpos.z, 0.f);
-            pb.ray_d[slot] = make_float4(dir.x, dir.y, dir.z, 0.f);
-            pb.thr[slot] = make_float4(1.f, 1.f, 1.f, a.use_mis ? pdfW : 0.f);
-            pb.rad[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
-            pb.key[slot] = key;
-            pb.queue[0][slot] = slot;
+            ro = make_float4(cpos.x, cpos.y, cpos.z, 0.f);
+            rdir = make_float4(dir.x, dir.y, dir.z, 0.f);
+            thr = make_float4(1.f, 1.f, 1.f, a.use_mis ? pdfW : 0.f);
+            meta = make_uint4(key, sidx, 0u, 0u);
         }
-        const uint32_t idx = wave_append(&pb.counters[2], splat);
+        if (fresh) {
+            const uint32_t e = head + i;
+            st.ray_o[e] = ro;
+            st.ray_d[e] = rdir;
+            st.thr[e] = thr;
+            st.rad[e] = make_float4(0.f, 0.f, 0.f, 0.f);
+            st.meta[e] = meta;
+        }
+        const uint32_t si = block_append(&pb.counters[kS], splat, agg);
         if (splat) {
-            pb.sh_o[idx] = so;
-            pb.sh_d[idx] = sd;
-            pb.sh_c[idx] = sc;
+            pb.sh_o[si] = so;
+            pb.sh_d[si] = sd;
+            pb.sh_c[si] = sc;
         }
     }
 }
 
 // ============================================================== extend
+// Persistent traversal: a wave claims kFetchChunk consecutive entries per
+// atomic and refills its lanes from that local range as soon as kRefillLanes
+// of them are idle, so a wave never waits for its slowest ray and atomics stay
+// rare.  Waves beyond the work exit at once.  (Chunk 128 / threshold 24 were
+// the fastest of a sweep over 64..1024 and 16..64 on the 1.31M-triangle scene.)
+#ifndef WR_REFILL_LANES
+#define WR_REFILL_LANES 24
+#endif
+constexpr int kRefillLanes = WR_REFILL_LANES;
+
+#ifdef WR_TRAV_WAVES
+#define WR_TRAV_ATTR __attribute__((amdgpu_waves_per_eu(WR_TRAV_WAVES, WR_TRAV_WAVES)))
+#else
+#define WR_TRAV_ATTR
+#endif
+
+// Work distribution of the persistent traversal kernels: a wave claims
+// kFetchChunk consecutive entries per atomic and hands them to its idle lanes
+// from a wave-local range, so atomics stay rare.
+#ifndef WR_FETCH_CHUNK
+#define WR_FETCH_CHUNK 128
+#endif
+constexpr uint32_t kFetchChunk = WR_FETCH_CHUNK;
+struct WaveRange {
+    uint32_t lo = 0, hi = 0;  // wave-uniform
+};
+// Returns the entry for this lane (>= n when it got none).  A new chunk is
+// claimed only when the local range is empty; `exhausted` (wave-uniform) is
+// set when the queue has no chunk left.  All lanes of the wave must call it.
+__device__ __forceinline__ uint32_t refill(uint32_t *fetch_counter, uint32_t n, bool idle, unsigned long long idle_mask,
+                                           WaveRange &wr, bool &exhausted) {
+    if (wr.lo == wr.hi) {
+        uint32_t base = 0;
+        if (lane_id() == 0) base = atomicAdd(fetch_counter, kFetchChunk);
+        base = __shfl(base, 0);
+        if (base >= n) {
+            exhausted = true;
+            return n;
+        }
+        wr.lo = base;
+        wr.hi = min(base + kFetchChunk, n);
+    }
+    const uint32_t k = (uint32_t)__popcll(idle_mask);
+    const uint32_t avail = wr.hi - wr.lo;
+    const uint32_t r = mbcnt64(idle_mask);
+    const uint32_t e = (idle && r < avail) ? wr.lo + r : n;
+    wr.lo += min(k, avail);
+    return e;
+}
+
 template <bool COUNT>
-__global__ __launch_bounds__(kBlock) void k_extend(DScene s, PathBuffers pb, const uint32_t *__restrict__ queue, uint2 *spill) {
+__global__ __launch_bounds__(kBlock) WR_TRAV_ATTR void k_extend(DScene s, PathBuffers pb, uint2 *spill) {
+    const uint32_t n = active_count(pb.counters);
+    if (blockIdx.x * kBlock >= n) return;
     __shared__ uint32_t st_node[kLdsStackDepth * kBlock];
     __shared__ float st_t[kLdsStackDepth * kBlock];
     LaneStack<kBlock> st{st_node + lane_id(), st_t + lane_id(),
                          spill + size_t(blockIdx.x * kBlock + lane_id()) * kSpillDepth, 0};
-    const uint32_t n = pb.counters[0];
     TraversalCounters cnt;
-    for (uint32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
-        const uint32_t i = base + lane_id();
-        if (i < n) {
-            const uint32_t p = queue[i];
-            const float4 o = pb.ray_o[p], d = pb.ray_d[p];
-            const RaySetup r = setup_ray(xyz(o), xyz(d), kRayEps);
-            float t = INFINITY;
-            const uint32_t tri = traverse<false, COUNT, kBlock>(s, r, t, st, cnt);
-            pb.hit[p] = make_uint2(tri, __float_as_uint(t));
+    TravState ts;
+    uint32_t p = 0;
+    WaveRange wr;
+    bool active = false, exhausted = false;
+    for (;;) {
+        const unsigned long long idle = __ballot(!active);
+        if (!exhausted && __popcll(idle) >= kRefillLanes) {
+            const uint32_t i = refill(&pb.counters[kFetchExt], n, !active, idle, wr, exhausted);
+            if (!active && i < n) {
+                p = i;
+                const float4 o = pb.cur.ray_o[p], d = pb.cur.ray_d[p];
+                trav_begin(ts, xyz(o), xyz(d), kRayEps, INFINITY);
+                st.sp = 0;
+                active = true;
+            }
+        }
+        if (__ballot(active) == 0) {
+            if (exhausted) break;
+            continue;
+        }
+        if (active && trav_step<false, COUNT, kBlock>(s, ts, st, cnt)) {
+            pb.hit[p] = make_uint2(ts.hit, __float_as_uint(ts.t));
+            active = false;
         }
     }
     if (COUNT) {
@@ -221,49 +377,52 @@ __global__ __launch_bounds__(kBlock) void k_extend(DScene s, PathBuffers pb, con
 
 // ============================================================== shade
 // One vertex of randomWalk with the PathNeeRenderer callback
-// (generalsubpathsampler.h:34-130, pathnextevent.h:100-149).
-__global__ __launch_bounds__(kBlock) void k_shade(DScene s, PathBuffers pb, RenderArgs a, const uint32_t *__restrict__ queue,
-                                                  uint32_t *__restrict__ next_queue, int bounce) {
-    const uint32_t n = pb.counters[0];
-    const int cNum = bounce + 2;  // camera is vertex 1
-    for (uint32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
-        const uint32_t i = base + lane_id();
+// (generalsubpathsampler.h:34-130, pathnextevent.h:100-149).  Continuing
+// paths are compacted into the next state; a finished path stores its
+// radiance in its sample's slot of the pass.
+__global__ __launch_bounds__(kShadeBlock) void k_shade(DScene s, PathBuffers pb, RenderArgs a) {
+    __shared__ uint32_t agg[kWavesPerShadeBlock + 1];
+    const uint32_t n = active_count(pb.counters);
+    const PathState cs = pb.cur, ns = pb.next;
+    for (uint32_t base = blockIdx.x * kShadeBlock; base < n; base += gridDim.x * kShadeBlock) {
+        const uint32_t i = base + threadIdx.x;
         bool push_shadow = false, cont = false;
-        uint32_t p = 0;
+        const bool valid = i < n;
         float4 so = make_float4(0, 0, 0, 0), sd = so, sc = so;
-        float4 new_o = so, new_d = so, new_thr = so;
-        if (i < n) {
-            p = queue[i];
-            const uint2 h = pb.hit[p];
+        float4 new_o = so, new_d = so, new_thr = so, rad = so;
+        uint4 meta = make_uint4(0, 0, 0, 0);
+        if (valid) {
+            meta = cs.meta[i];
+            rad = cs.rad[i];
+            const uint32_t bounce = meta.z;
+            const int cNum = int(bounce) + 2;  // camera is vertex 1
+            const uint2 h = pb.hit[i];
             if (h.x != kInvalid) {
                 const float t = __uint_as_float(h.y);
-                const float3 ro = xyz(pb.ray_o[p]), rd = xyz(pb.ray_d[p]);
+                const float3 ro = xyz(cs.ray_o[i]), rd = xyz(cs.ray_d[i]);
                 const float4 v0 = s.tris[3 * size_t(h.x) + 0];
                 const float4 v1 = s.tris[3 * size_t(h.x) + 1];
                 const float4 v2 = s.tris[3 * size_t(h.x) + 2];
                 const float3 pos = ro + rd * t;
                 const float3 ng = normalize3(cross3(xyz(v1) - xyz(v0), xyz(v2) - xyz(v0)));
                 const int2 gi = s.geom_info[__float_as_uint(v1.w)];
-                const float4 thr = pb.thr[p];
+                const float4 thr = cs.thr[i];
                 const float3 T = xyz(thr);
-                const uint32_t key = pb.key[p];
+                const uint32_t key = meta.x;
                 if (gi.y >= 0) {
                     // emitter hit: MIS against light sampling (pathnextevent.h:128-146)
                     const float cos_l = -dot3(ng, rd);
                     if (cos_l >= 0.f) {
+                        const DLight &L = s.lights[gi.y];
                         float mis = 0.f;
                         if (a.use_mis) {
                             const float cPdfA = thr.w * fabsf(cos_l) / (t * t);
-                            const DLight &L = s.lights[gi.y];
                             const float lightPdfA = L.pdf_select * L.inv_area;
                             mis = cPdfA / (lightPdfA + cPdfA);
                         }
-                        const DLight &L = s.lights[gi.y];
-                        float4 rr = pb.rad[p];
-                        rr.x += mis * T.x * L.L[0];
-                        rr.y += mis * T.y * L.L[1];
-                        rr.z += mis * T.z * L.L[2];
-                        pb.rad[p] = rr;
+                        rad.x += mis * T.x * L.L[0];
+                        rad.y += mis * T.y * L.L[1];
+                        rad.z += mis * T.z * L.L[2];
                     }
                 } else if (cNum != a.N) {
                     const DMat m = s.mats[gi.x];
@@ -290,14 +449,14 @@ __global__ __launch_bounds__(kBlock) void k_shade(DScene s, PathBuffers pb, Rend
                             const float3 c = T * fs * l.contrib * k;
                             if (!is_black(c)) {
                                 so = make_float4(pos.x, pos.y, pos.z, len - kRayEps);
-                                sd = make_float4(nuv.x, nuv.y, nuv.z, __uint_as_float(p));
+                                sd = make_float4(nuv.x, nuv.y, nuv.z, 0.f);
                                 sc = make_float4(c.x, c.y, c.z, 0.f);
                                 push_shadow = true;
                             }
                         }
                     }
                     // BSDF sampling (generalsubpathsampler.h:81-130); here cNum < N
-                    const uint32_t dim = kDimBounce0 + kDimsPerBounce * uint32_t(bounce) + kBdBsdf;
+                    const uint32_t dim = kDimBounce0 + kDimsPerBounce * bounce + kBdBsdf;
                     float3 wo;
                     float pdf = 0.f;
                     const float3 w = bsdf_sample(m, wi, u01(key, dim), u01(key, dim + 1), wo, pdf);
@@ -311,50 +470,76 @@ __global__ __launch_bounds__(kBlock) void k_shade(DScene s, PathBuffers pb, Rend
                 }
             }
         }
-        const uint32_t si = wave_append(&pb.counters[2], push_shadow);
+        const uint32_t qi = block_append(&pb.counters[kQn], cont, agg);
+        if (cont) {
+            ns.ray_o[qi] = new_o;
+            ns.ray_d[qi] = new_d;
+            ns.thr[qi] = new_thr;
+            ns.rad[qi] = rad;
+            ns.meta[qi] = make_uint4(meta.x, meta.y, meta.z + 1, 0u);
+        } else if (valid) {
+            pb.sample_rad[meta.y] = rad;
+        }
+        const uint32_t target = cont ? (kToNextPath | qi) : (kToSample | meta.y);
+        sd.w = __uint_as_float(target);
+        const uint32_t si = block_append(&pb.counters[kS], push_shadow, agg);
         if (push_shadow) {
             pb.sh_o[si] = so;
             pb.sh_d[si] = sd;
             pb.sh_c[si] = sc;
         }
-        const uint32_t qi = wave_append(&pb.counters[1], cont);
-        if (cont) {
-            next_queue[qi] = p;
-            pb.ray_o[p] = new_o;
-            pb.ray_d[p] = new_d;
-            pb.thr[p] = new_thr;
-        }
     }
 }
 
 // ============================================================== shadow
+// Any-hit traversal of the shadow queue; a visible connection adds its
+// contribution to the radiance of its path (continuing or finished), or to the
+// FP64 splat film for light->camera (t = 1) connections.
 template <bool COUNT>
-__global__ __launch_bounds__(kBlock) void k_shadow(DScene s, PathBuffers pb, uint2 *spill) {
+__global__ __launch_bounds__(kBlock) WR_TRAV_ATTR void k_shadow(DScene s, PathBuffers pb, uint2 *spill) {
+    const uint32_t n = pb.counters[kS];
+    if (blockIdx.x * kBlock >= n) return;
     __shared__ uint32_t st_node[kLdsStackDepth * kBlock];
     __shared__ float st_t[kLdsStackDepth * kBlock];
     LaneStack<kBlock> st{st_node + lane_id(), st_t + lane_id(),
                          spill + size_t(blockIdx.x * kBlock + lane_id()) * kSpillDepth, 0};
-    const uint32_t n = pb.counters[2];
     TraversalCounters cnt;
-    for (uint32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
-        const uint32_t i = base + lane_id();
-        if (i < n) {
-            const float4 o = pb.sh_o[i], d = pb.sh_d[i];
-            const RaySetup r = setup_ray(xyz(o), xyz(d), kRayEps);
-            float t = o.w;
-            const uint32_t tri = traverse<true, COUNT, kBlock>(s, r, t, st, cnt);
-            if (tri == kInvalid) {
+    TravState ts;
+    uint32_t i = 0;
+    WaveRange wr;
+    bool active = false, exhausted = false;
+    for (;;) {
+        const unsigned long long idle = __ballot(!active);
+        if (!exhausted && __popcll(idle) >= kRefillLanes) {
+            const uint32_t j = refill(&pb.counters[kFetchShd], n, !active, idle, wr, exhausted);
+            if (!active && j < n) {
+                i = j;
+                const float4 o = pb.sh_o[i], d = pb.sh_d[i];
+                trav_begin(ts, xyz(o), xyz(d), kRayEps, o.w);
+                st.sp = 0;
+                active = true;
+            }
+        }
+        if (__ballot(active) == 0) {
+            if (exhausted) break;
+            continue;
+        }
+        if (active && trav_step<true, COUNT, kBlock>(s, ts, st, cnt)) {
+            active = false;
+            if (ts.hit == kInvalid) {
                 const float4 c = pb.sh_c[i];
-                const uint32_t target = __float_as_uint(d.w);
-                if (target & kSplatFlag) {
-                    double *dst = pb.film_splat + 3 * size_t(target & ~kSplatFlag);
+                const uint32_t target = __float_as_uint(pb.sh_d[i].w);
+                const uint32_t idx = target & kTargetMask;
+                if ((target & ~kTargetMask) == kToSplat) {
+                    double *dst = pb.film_splat + 3 * size_t(idx);
                     atomicAdd(dst + 0, double(c.x));
                     atomicAdd(dst + 1, double(c.y));
                     atomicAdd(dst + 2, double(c.z));
                 } else {
-                    float4 rr = pb.rad[target];
+                    float4 *r = (target & ~kTargetMask) == kToSample ? pb.sample_rad + idx : pb.next.rad + idx;
+                    float4 rr = *r;
                     rr.x += c.x; rr.y += c.y; rr.z += c.z;
-                    pb.rad[target] = rr;
+                    *r = rr;
                 }
             }
         }
@@ -366,12 +551,14 @@ __global__ __launch_bounds__(kBlock) void k_shadow(DScene s, PathBuffers pb, uin
 }
 
 // ============================================================== accumulate
-// Film::addSample for the batch (film.h:42-48): pixel-owned FP64 sums.
-__global__ void k_accumulate(PathBuffers pb, uint32_t wh, uint32_t spp_batch) {
+// Film::addSample for a pass (film.h:42-48): the samples of pixel p are
+// p, p + wh, p + 2 wh, ...; their FP32 radiance is summed in FP64, in sample
+// order, so the film is bit-reproducible.
+__global__ void k_accumulate(PathBuffers pb, uint32_t wh, uint32_t spp) {
     for (uint32_t px = blockIdx.x * blockDim.x + threadIdx.x; px < wh; px += gridDim.x * blockDim.x) {
         double r = 0.0, g = 0.0, b = 0.0;
-        for (uint32_t k = 0; k < spp_batch; k++) {
-            const float4 v = pb.rad[size_t(k) * wh + px];
+        for (uint32_t k = 0; k < spp; k++) {
+            const float4 v = pb.sample_rad[size_t(k) * wh + px];
             r += v.x; g += v.y; b += v.z;
         }
         double *dst = pb.film_base + 3 * size_t(px);
@@ -405,6 +592,8 @@ __global__ __launch_bounds__(kBlock) void k_trace(DScene s, const float4 *__rest
 struct wr_device {
     int ordinal = 0;
     int n_cu = 256;
+    // resident blocks per CU of each kernel (HIP occupancy API), used to size persistent grids
+    int occ_extend = 8, occ_shadow = 8, occ_shade = 8, occ_generate = 8, occ_trace = 8;
 };
 
 struct wr_scene {
@@ -426,8 +615,9 @@ struct wr_scene {
     int W = 0, H = 0;
     // path buffers (lazily sized)
     PathBuffers pb{};
-    uint32_t capacity = 0;
+    uint32_t capacity = 0, sample_capacity = 0;
     void *pool = nullptr;
+    uint32_t *h_counters = nullptr;  // pinned host copy of the pool counters
     hipEvent_t ev[2] = {nullptr, nullptr};
 };
 
@@ -459,6 +649,16 @@ int wr_device_create(int ordinal, wr_device **out) {
     wr_device *d = new wr_device;
     d->ordinal = ordinal;
     d->n_cu = prop.multiProcessorCount;
+    auto occ = [](const void *fn, int block, int &out_blocks) {
+        int nb = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, block, 0) == hipSuccess && nb > 0) out_blocks = nb;
+        out_blocks = std::min(out_blocks, kMaxTraversalBlocksPerCU);
+    };
+    occ(reinterpret_cast<const void *>(&k_extend<false>), kBlock, d->occ_extend);
+    occ(reinterpret_cast<const void *>(&k_shadow<false>), kBlock, d->occ_shadow);
+    occ(reinterpret_cast<const void *>(&k_shade), kShadeBlock, d->occ_shade);
+    occ(reinterpret_cast<const void *>(&k_regen), kShadeBlock, d->occ_generate);
+    occ(reinterpret_cast<const void *>(&k_trace<false>), kBlock, d->occ_trace);
     *out = d;
     return WR_OK;
 }
@@ -477,6 +677,7 @@ void wr_scene_destroy(wr_scene *s) {
     (void)hipFree(s->d_lights);
     (void)hipFree(s->d_spill);
     (void)hipFree(s->pool);
+    if (s->h_counters) (void)hipHostFree(s->h_counters);
     for (auto &e : s->ev)
         if (e) (void)hipEventDestroy(e);
     if (s->stream) (void)hipStreamDestroy(s->stream);
@@ -781,20 +982,30 @@ int wr_film_read(wr_film *f, double *base, double *splat) {
     return WR_OK;
 }
 
-// Path-state pool: per slot 16*4 (ray_o, ray_d, thr, rad) + 4 key + 8 hit + 8 queues
-// + 2 shadow entries * 48.
-static int ensure_capacity(wr_scene *s, uint32_t cap) {
-    if (s->capacity >= cap) return WR_OK;
+// Path pool: two queue-ordered states of 16*5 bytes per path (ray_o, ray_d,
+// thr, rad, meta), the hit records (8 B), two shadow entries (2 * 48 B), and
+// the per-sample radiance of a pass (16 B per sample).
+static int ensure_capacity(wr_scene *s, uint32_t cap, uint32_t samples) {
+    if (s->capacity >= cap && s->sample_capacity >= samples) return WR_OK;
+    cap = std::max(cap, s->capacity);
+    samples = std::max(samples, s->sample_capacity);
     if (s->pool) {
         WR_HIP(hipStreamSynchronize(s->stream));
         WR_HIP(hipFree(s->pool));
         s->pool = nullptr;
         s->capacity = 0;
+        s->sample_capacity = 0;
     }
     auto al = [](size_t b) { return (b + 255) / 256 * 256; };
     const size_t c = cap;
-    const size_t sizes[] = {al(16 * c), al(16 * c), al(16 * c), al(16 * c), al(4 * c), al(8 * c),
-                            al(4 * c),  al(4 * c),  al(32 * c), al(32 * c), al(32 * c), al(64), al(64)};
+    std::vector<size_t> sizes;
+    for (int k = 0; k < 2; k++)
+        for (int j = 0; j < 5; j++) sizes.push_back(al(16 * c));
+    sizes.push_back(al(8 * c));              // hit
+    sizes.push_back(al(16 * size_t(samples)));  // per-sample radiance
+    for (int j = 0; j < 3; j++) sizes.push_back(al(32 * c));  // shadow o, d, contribution (2 entries per path)
+    sizes.push_back(al(4 * kCounterStride * kNumCounterLines));  // counters, one line each
+    sizes.push_back(al(64));                 // stats
     size_t total = 0;
     for (size_t b : sizes) total += b;
     char *p = nullptr;
@@ -802,21 +1013,28 @@ static int ensure_capacity(wr_scene *s, uint32_t cap) {
     s->pool = p;
     PathBuffers &pb = s->pb;
     size_t off = 0;
-    auto take = [&](size_t i) { char *q = p + off; off += sizes[i]; return q; };
-    pb.ray_o = reinterpret_cast<float4 *>(take(0));
-    pb.ray_d = reinterpret_cast<float4 *>(take(1));
-    pb.thr = reinterpret_cast<float4 *>(take(2));
-    pb.rad = reinterpret_cast<float4 *>(take(3));
-    pb.key = reinterpret_cast<uint32_t *>(take(4));
-    pb.hit = reinterpret_cast<uint2 *>(take(5));
-    pb.queue[0] = reinterpret_cast<uint32_t *>(take(6));
-    pb.queue[1] = reinterpret_cast<uint32_t *>(take(7));
-    pb.sh_o = reinterpret_cast<float4 *>(take(8));
-    pb.sh_d = reinterpret_cast<float4 *>(take(9));
-    pb.sh_c = reinterpret_cast<float4 *>(take(10));
-    pb.counters = reinterpret_cast<uint32_t *>(take(11));
-    pb.stats = reinterpret_cast<unsigned long long *>(take(12));
+    int idx = 0;
+    auto take = [&]() { char *q = p + off; off += sizes[idx++]; return q; };
+    PathState *sts[2] = {&pb.cur, &pb.next};
+    for (PathState *st : sts) {
+        st->ray_o = reinterpret_cast<float4 *>(take());
+        st->ray_d = reinterpret_cast<float4 *>(take());
+        st->thr = reinterpret_cast<float4 *>(take());
+        st->rad = reinterpret_cast<float4 *>(take());
+        st->meta = reinterpret_cast<uint4 *>(take());
+    }
+    pb.hit = reinterpret_cast<uint2 *>(take());
+    pb.sample_rad = reinterpret_cast<float4 *>(take());
+    pb.sh_o = reinterpret_cast<float4 *>(take());
+    pb.sh_d = reinterpret_cast<float4 *>(take());
+    pb.sh_c = reinterpret_cast<float4 *>(take());
+    pb.counters = reinterpret_cast<uint32_t *>(take());
+    pb.stats = reinterpret_cast<unsigned long long *>(take());
     s->capacity = cap;
+    s->sample_capacity = samples;
+    if (!s->h_counters)
+        WR_HIP(hipHostMalloc(reinterpret_cast<void **>(&s->h_counters), sizeof(uint32_t) * kCounterStride * kNumCounterLines,
+                             hipHostMallocDefault));
     return WR_OK;
 }
 
@@ -828,6 +1046,10 @@ static int grid_for(const wr_scene *s, uint32_t n, int per_cu) {
 static int trav_grid(const wr_scene *s, uint32_t n, int per_cu) {
     return std::min(grid_for(s, n, std::min(per_cu, kMaxTraversalBlocksPerCU)), s->max_trav_blocks);
 }
+
+// Samples of one pass: bounds the per-sample radiance buffer to 2 GiB, enough
+// for a 1920x1080 frame at 64 spp, so such a frame drains the pool once.
+constexpr uint64_t kMaxSamplesPerPass = uint64_t(1) << 27;
 
 int wr_render_nee(wr_scene *s, wr_film *f, const wr_render_params *prm, wr_render_stats *stats) {
     if (!s || !f || !prm) return set_error(WR_ERR_INVALID_ARGUMENT, "null argument");
@@ -843,11 +1065,16 @@ int wr_render_nee(wr_scene *s, wr_film *f, const wr_render_params *prm, wr_rende
         if (stats) *stats = st;
         return WR_OK;
     }
-    uint64_t max_paths = prm->max_paths_in_flight > 0 ? uint64_t(prm->max_paths_in_flight) : (uint64_t(1) << 22);
-    max_paths = std::max<uint64_t>(max_paths, wh);
-    if (max_paths > (uint64_t(1) << 30)) return set_error(WR_ERR_UNSUPPORTED, "too many paths in flight");
-    const uint32_t spp_per_batch = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(max_paths / wh, uint64_t(prm->spp))));
-    rc = ensure_capacity(s, spp_per_batch * wh);
+    const uint32_t spp_pass =
+        uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(kMaxSamplesPerPass / wh, uint64_t(prm->spp))));
+    const uint32_t samples_pass = spp_pass * wh;
+    // default pool: 32M paths (about 5.6 GB of state): large launches hide the
+    // traversal latency and shrink the per-launch tail
+    uint64_t pool = prm->max_paths_in_flight > 0 ? uint64_t(prm->max_paths_in_flight) : (uint64_t(1) << 25);
+    pool = std::min<uint64_t>(pool, samples_pass);
+    if (pool > (uint64_t(1) << 30)) return set_error(WR_ERR_UNSUPPORTED, "too many paths in flight");
+    const uint32_t P = uint32_t(pool);
+    rc = ensure_capacity(s, P, samples_pass);
     if (rc) return rc;
     PathBuffers pb = s->pb;
     pb.film_base = f->d_base;
@@ -868,38 +1095,59 @@ int wr_render_nee(wr_scene *s, wr_film *f, const wr_render_params *prm, wr_rende
     };
     auto begin = [&](int kind) { if (prof) marks.push_back({kind, ev_record()}); };
     auto end = [&]() { if (prof) ev_record(); };
-    const int g_trav = 0;  // per-CU multiplier chosen below
-    (void)g_trav;
-    for (uint32_t done = 0; done < uint32_t(prm->spp); done += spp_per_batch) {
-        const uint32_t k = std::min<uint32_t>(spp_per_batch, uint32_t(prm->spp) - done);
+    const int g_regen = std::max(1, std::min<int>((P + kShadeBlock - 1) / kShadeBlock, s->dev->n_cu * s->dev->occ_generate));
+    const int g_shade = std::max(1, std::min<int>((P + kShadeBlock - 1) / kShadeBlock, s->dev->n_cu * s->dev->occ_shade));
+    const int g_ext = trav_grid(s, P, s->dev->occ_extend);
+    const int g_shd = trav_grid(s, 2 * P, s->dev->occ_shadow);
+    for (uint32_t done = 0; done < uint32_t(prm->spp); done += spp_pass) {
+        const uint32_t k = std::min<uint32_t>(spp_pass, uint32_t(prm->spp) - done);
         RenderArgs a;
-        a.n_paths = k * wh;
+        a.total = k * wh;
         a.wh = wh;
         a.first_sample = uint32_t(prm->first_sample) + done;
         a.seed = prm->seed;
+        a.capacity = P;
         a.N = N;
         a.use_mis = prm->use_mis ? 1 : 0;
-        k_reset<<<1, 1, 0, s->stream>>>(pb, a.n_paths);
-        begin(3);
-        k_generate<<<grid_for(s, a.n_paths, 16), kBlock, 0, s->stream>>>(s->ds, pb, a);
-        end();
-        for (int b = 0; b < N - 1; b++) {
-            uint32_t *qc = pb.queue[b & 1], *qn = pb.queue[(b + 1) & 1];
+        k_init<<<1, 64, 0, s->stream>>>(pb, a);
+        WR_HIP(hipGetLastError());
+        // Each iteration refills the pool, extends every active path by one
+        // vertex and resolves its shadow rays.  The host polls the counters
+        // every few iterations; iterations past the end find empty queues.
+        const int kPoll = 4;
+        // every path ends within N-1 vertices, and each iteration that is not
+        // draining starts new samples in all free entries
+        const uint64_t max_iters = (uint64_t(a.total) / P + 2) * uint64_t(N + 1) + 64;
+        for (int it = 0;; it++) {
+            if (uint64_t(it) > max_iters) return set_error(WR_ERR_DEVICE, "path pool did not drain");
+            begin(3);
+            k_regen<<<g_regen, kShadeBlock, 0, s->stream>>>(s->ds, pb, a);
+            end();
             begin(0);
-            if (count) k_extend<true><<<trav_grid(s, a.n_paths, 8), kBlock, 0, s->stream>>>(s->ds, pb, qc, s->d_spill);
-            else k_extend<false><<<trav_grid(s, a.n_paths, 8), kBlock, 0, s->stream>>>(s->ds, pb, qc, s->d_spill);
+            if (count) k_extend<true><<<g_ext, kBlock, 0, s->stream>>>(s->ds, pb, s->d_spill);
+            else k_extend<false><<<g_ext, kBlock, 0, s->stream>>>(s->ds, pb, s->d_spill);
             end();
             begin(2);
-            k_shade<<<grid_for(s, a.n_paths, 16), kBlock, 0, s->stream>>>(s->ds, pb, a, qc, qn, b);
+            k_shade<<<g_shade, kShadeBlock, 0, s->stream>>>(s->ds, pb, a);
             end();
             begin(1);
-            if (count) k_shadow<true><<<trav_grid(s, 2 * a.n_paths, 8), kBlock, 0, s->stream>>>(s->ds, pb, s->d_spill);
-            else k_shadow<false><<<trav_grid(s, 2 * a.n_paths, 8), kBlock, 0, s->stream>>>(s->ds, pb, s->d_spill);
+            if (count) k_shadow<true><<<g_shd, kBlock, 0, s->stream>>>(s->ds, pb, s->d_spill);
+            else k_shadow<false><<<g_shd, kBlock, 0, s->stream>>>(s->ds, pb, s->d_spill);
             end();
-            k_rotate<<<1, 1, 0, s->stream>>>(pb);
+            k_rotate<<<1, 1, 0, s->stream>>>(pb, a);
+            WR_HIP(hipGetLastError());
+            std::swap(pb.cur, pb.next);
+            if (it % kPoll == kPoll - 1) {
+                WR_HIP(hipMemcpyAsync(s->h_counters, pb.counters, sizeof(uint32_t) * kCounterStride * kNumCounterLines,
+                                      hipMemcpyDeviceToHost, s->stream));
+                WR_HIP(hipStreamSynchronize(s->stream));
+                const uint32_t *c = s->h_counters;
+                if (c[kQ] == 0 && c[kFresh] == 0) break;
+            }
         }
         begin(4);
-        k_accumulate<<<grid_for(s, wh, 16), 256, 0, s->stream>>>(pb, wh, k);
+        k_accumulate<<<std::max<uint32_t>(1, std::min<uint32_t>((wh + 255) / 256, uint32_t(s->dev->n_cu) * 8)), 256, 0,
+                       s->stream>>>(pb, wh, k);
         end();
         WR_HIP(hipGetLastError());
     }
@@ -936,7 +1184,7 @@ static int trace_impl(wr_scene *s, const float *d_rays, uint32_t n, void *d_hits
     int rc = set_device(s->dev);
     if (rc) return rc;
     if (n == 0) { if (ms) *ms = 0.f; return WR_OK; }
-    const int grid = trav_grid(s, n, 8);
+    const int grid = trav_grid(s, n, s->dev->occ_trace);
     if (ms) WR_HIP(hipEventRecord(s->ev[0], s->stream));
     if (any) k_trace<true><<<grid, kBlock, 0, s->stream>>>(s->ds, reinterpret_cast<const float4 *>(d_rays), n, nullptr,
                                                            d_occ, s->d_spill);

@@ -42,24 +42,27 @@ __device__ __forceinline__ RaySetup setup_ray(float3 o, float3 d, float tmin) {
     return r;
 }
 
-// FP64 recomputation of the watertight test (edge and vertex hits)
-__device__ __noinline__ bool tri_test_fp64(const RaySetup &r, float3 a, float3 b, float3 c, double tmin, double tmax,
-                                           double &t_out) {
-    double o[3] = {r.o.x, r.o.y, r.o.z};
-    double A[3] = {a.x - o[0], a.y - o[1], a.z - o[2]};
-    double B[3] = {b.x - o[0], b.y - o[1], b.z - o[2]};
-    double C[3] = {c.x - o[0], c.y - o[1], c.z - o[2]};
-    double d[3] = {r.d.x, r.d.y, r.d.z};
-    double Sx = d[r.kx] / d[r.kz], Sy = d[r.ky] / d[r.kz], Sz = 1.0 / d[r.kz];
-    double Ax = A[r.kx] - Sx * A[r.kz], Ay = A[r.ky] - Sy * A[r.kz];
-    double Bx = B[r.kx] - Sx * B[r.kz], By = B[r.ky] - Sy * B[r.kz];
-    double Cx = C[r.kx] - Sx * C[r.kz], Cy = C[r.ky] - Sy * C[r.kz];
-    double U = Cx * By - Cy * Bx, V = Ax * Cy - Ay * Cx, W = Bx * Ay - By * Ax;
+// FP64 recomputation of the watertight test (edge and vertex hits, where an
+// FP32 edge function is exactly zero).
+__device__ __forceinline__ double pickd(double x, double y, double z, int k) { return k == 0 ? x : (k == 1 ? y : z); }
+__device__ __forceinline__ bool tri_test_fp64(const RaySetup &r, float3 a, float3 b, float3 c, double tmin, double tmax,
+                                              double &t_out) {
+    const double ox = r.o.x, oy = r.o.y, oz = r.o.z;
+    const double ax = a.x - ox, ay = a.y - oy, az = a.z - oz;
+    const double bx = b.x - ox, by = b.y - oy, bz = b.z - oz;
+    const double cx = c.x - ox, cy = c.y - oy, cz = c.z - oz;
+    const double dkx = pickd(r.d.x, r.d.y, r.d.z, r.kx), dky = pickd(r.d.x, r.d.y, r.d.z, r.ky);
+    const double dkz = pickd(r.d.x, r.d.y, r.d.z, r.kz);
+    const double Sx = dkx / dkz, Sy = dky / dkz, Sz = 1.0 / dkz;
+    const double Akz = pickd(ax, ay, az, r.kz), Bkz = pickd(bx, by, bz, r.kz), Ckz = pickd(cx, cy, cz, r.kz);
+    const double Ax = pickd(ax, ay, az, r.kx) - Sx * Akz, Ay = pickd(ax, ay, az, r.ky) - Sy * Akz;
+    const double Bx = pickd(bx, by, bz, r.kx) - Sx * Bkz, By = pickd(bx, by, bz, r.ky) - Sy * Bkz;
+    const double Cx = pickd(cx, cy, cz, r.kx) - Sx * Ckz, Cy = pickd(cx, cy, cz, r.ky) - Sy * Ckz;
+    const double U = Cx * By - Cy * Bx, V = Ax * Cy - Ay * Cx, W = Bx * Ay - By * Ax;
     if ((U < 0 || V < 0 || W < 0) && (U > 0 || V > 0 || W > 0)) return false;
-    double det = U + V + W;
+    const double det = U + V + W;
     if (det == 0.0) return false;
-    double T = Sz * (U * A[r.kz] + V * B[r.kz] + W * C[r.kz]);
-    double t = T / det;
+    const double t = Sz * (U * Akz + V * Bkz + W * Ckz) / det;
     if (!(t > tmin && t <= tmax)) return false;
     t_out = t;
     return true;
@@ -126,86 +129,115 @@ struct TraversalCounters {
     uint32_t nodes = 0, tris = 0;
 };
 
-// Closest hit (ANY = false) or any hit (ANY = true).  Returns the leaf-order
-// triangle index or kInvalid; t_best is updated in place.
+// Resumable traversal state of one lane.
+struct TravState {
+    RaySetup r;
+    float t;          // current far limit (closest hit so far, or the shadow ray's tmax)
+    uint32_t hit;     // leaf-order triangle or kInvalid
+    uint32_t cur;     // node reference to process next
+};
+
+__device__ __forceinline__ void trav_begin(TravState &ts, float3 o, float3 d, float tmin, float tmax) {
+    ts.r = setup_ray(o, d, tmin);
+    ts.t = tmax;
+    ts.hit = kInvalid;
+    ts.cur = 0;  // root (internal node 0)
+}
+
+// Processes one node (internal: box tests and pushes; leaf: triangle tests),
+// then pops if needed.  Returns true when the lane's ray is finished.
+// Closest hit (ANY = false) visits children near-to-far and culls stack entries
+// farther than the current hit; any hit (ANY = true) stops at the first hit.
 template <bool ANY, bool COUNT, int kBlock>
-__device__ __forceinline__ uint32_t traverse(const DScene &s, const RaySetup &r, float &t_best, LaneStack<kBlock> &st,
-                                             TraversalCounters &cnt) {
-    uint32_t hit_tri = kInvalid;
-    uint32_t cur = 0;  // root (internal node 0)
-    st.sp = 0;
-    const float ox = r.o.x, oy = r.o.y, oz = r.o.z;
-    for (;;) {
-        if (!(cur & kLeafBit)) {
-            const float4 *np = reinterpret_cast<const float4 *>(s.nodes + cur);
-            const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
-            const uint4 ch = reinterpret_cast<const uint4 *>(np)[6];
-            if (COUNT) cnt.nodes++;
-            const float4 nx_ = r.nx ? hx : lx, fx_ = r.nx ? lx : hx;
-            const float4 ny_ = r.ny ? hy : ly, fy_ = r.ny ? ly : hy;
-            const float4 nz_ = r.nz ? hz : lz, fz_ = r.nz ? lz : hz;
-            float tn[4], tf[4];
-            tn[0] = fmaxf(fmaxf((nx_.x - ox) * r.ix, (ny_.x - oy) * r.iy), fmaxf((nz_.x - oz) * r.iz, r.tmin));
-            tn[1] = fmaxf(fmaxf((nx_.y - ox) * r.ix, (ny_.y - oy) * r.iy), fmaxf((nz_.y - oz) * r.iz, r.tmin));
-            tn[2] = fmaxf(fmaxf((nx_.z - ox) * r.ix, (ny_.z - oy) * r.iy), fmaxf((nz_.z - oz) * r.iz, r.tmin));
-            tn[3] = fmaxf(fmaxf((nx_.w - ox) * r.ix, (ny_.w - oy) * r.iy), fmaxf((nz_.w - oz) * r.iz, r.tmin));
-            tf[0] = fminf(fminf((fx_.x - ox) * r.ix, (fy_.x - oy) * r.iy), (fz_.x - oz) * r.iz);
-            tf[1] = fminf(fminf((fx_.y - ox) * r.ix, (fy_.y - oy) * r.iy), (fz_.y - oz) * r.iz);
-            tf[2] = fminf(fminf((fx_.z - ox) * r.ix, (fy_.z - oy) * r.iy), (fz_.z - oz) * r.iz);
-            tf[3] = fminf(fminf((fx_.w - ox) * r.ix, (fy_.w - oy) * r.iy), (fz_.w - oz) * r.iz);
-            uint32_t c[4] = {ch.x, ch.y, ch.z, ch.w};
-            float key[4];
+__device__ __forceinline__ bool trav_step(const DScene &s, TravState &ts, LaneStack<kBlock> &st, TraversalCounters &cnt) {
+    const RaySetup &r = ts.r;
+    const uint32_t cur = ts.cur;
+    if (!(cur & kLeafBit)) {
+        const float4 *np = reinterpret_cast<const float4 *>(s.nodes + cur);
+        const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
+        const uint4 ch = reinterpret_cast<const uint4 *>(np)[6];
+        if (COUNT) cnt.nodes++;
+        const float ox = r.o.x, oy = r.o.y, oz = r.o.z;
+        const float4 nx_ = r.nx ? hx : lx, fx_ = r.nx ? lx : hx;
+        const float4 ny_ = r.ny ? hy : ly, fy_ = r.ny ? ly : hy;
+        const float4 nz_ = r.nz ? hz : lz, fz_ = r.nz ? lz : hz;
+        float tn[4], tf[4];
+        tn[0] = fmaxf(fmaxf((nx_.x - ox) * r.ix, (ny_.x - oy) * r.iy), fmaxf((nz_.x - oz) * r.iz, r.tmin));
+        tn[1] = fmaxf(fmaxf((nx_.y - ox) * r.ix, (ny_.y - oy) * r.iy), fmaxf((nz_.y - oz) * r.iz, r.tmin));
+        tn[2] = fmaxf(fmaxf((nx_.z - ox) * r.ix, (ny_.z - oy) * r.iy), fmaxf((nz_.z - oz) * r.iz, r.tmin));
+        tn[3] = fmaxf(fmaxf((nx_.w - ox) * r.ix, (ny_.w - oy) * r.iy), fmaxf((nz_.w - oz) * r.iz, r.tmin));
+        tf[0] = fminf(fminf((fx_.x - ox) * r.ix, (fy_.x - oy) * r.iy), (fz_.x - oz) * r.iz);
+        tf[1] = fminf(fminf((fx_.y - ox) * r.ix, (fy_.y - oy) * r.iy), (fz_.y - oz) * r.iz);
+        tf[2] = fminf(fminf((fx_.z - ox) * r.ix, (fy_.z - oy) * r.iy), (fz_.z - oz) * r.iz);
+        tf[3] = fminf(fminf((fx_.w - ox) * r.ix, (fy_.w - oy) * r.iy), (fz_.w - oz) * r.iz);
+        uint32_t c[4] = {ch.x, ch.y, ch.z, ch.w};
+        float key[4];
 #pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const bool h = c[i] != kEmptyChild && tn[i] <= fminf(tf[i] * 1.00000048f, t_best);
-                key[i] = h ? tn[i] : INFINITY;
-            }
-            if (!ANY) {
-                // sort (key, child) ascending: 5-comparator network
+        for (int i = 0; i < 4; i++) {
+            const bool h = c[i] != kEmptyChild && tn[i] <= fminf(tf[i] * 1.00000048f, ts.t);
+            key[i] = h ? tn[i] : INFINITY;
+        }
+        if (!ANY) {
+            // sort (key, child) ascending: 5-comparator network
 #define WR_CSWAP(a, b)                                                          \
     if (key[b] < key[a]) {                                                      \
         float tk = key[a]; key[a] = key[b]; key[b] = tk;                        \
         uint32_t tc = c[a]; c[a] = c[b]; c[b] = tc;                             \
     }
-                WR_CSWAP(0, 1) WR_CSWAP(2, 3) WR_CSWAP(0, 2) WR_CSWAP(1, 3) WR_CSWAP(1, 2)
+            WR_CSWAP(0, 1) WR_CSWAP(2, 3) WR_CSWAP(0, 2) WR_CSWAP(1, 3) WR_CSWAP(1, 2)
 #undef WR_CSWAP
-            }
-            // push hits farthest first; continue with the nearest
-            uint32_t next = kInvalid;
-            float next_t = 0.f;
+        }
+        // push hits farthest first; continue with the nearest
+        uint32_t next = kInvalid;
+        float next_t = 0.f;
 #pragma unroll
-            for (int i = 3; i >= 0; i--) {
-                if (key[i] != INFINITY) {
-                    if (next != kInvalid) st.push(next, next_t);
-                    next = c[i];
-                    next_t = key[i];
-                }
-            }
-            if (next != kInvalid) { cur = next; continue; }
-        } else {
-            const uint32_t first = cur & 0x0FFFFFFFu;
-            const uint32_t count = ((cur >> 28) & 0x7u) + 1u;
-            if (COUNT) cnt.tris += count;
-            for (uint32_t k = 0; k < count; k++) {
-                const float4 *tp = s.tris + 3 * size_t(first + k);
-                float t;
-                if (tri_test(r, tp[0], tp[1], tp[2], t_best, t)) {
-                    t_best = t;
-                    hit_tri = first + k;
-                    if (ANY) return hit_tri;
-                }
+        for (int i = 3; i >= 0; i--) {
+            if (key[i] != INFINITY) {
+                if (next != kInvalid) st.push(next, next_t);
+                next = c[i];
+                next_t = key[i];
             }
         }
-        // pop, skipping entries farther than the current hit
-        for (;;) {
-            if (st.sp == 0) return hit_tri;
-            st.sp--;
-            uint32_t ref;
-            float tn;
-            st.top(ref, tn);
-            if (tn <= t_best) { cur = ref; break; }
+        if (next != kInvalid) { ts.cur = next; return false; }
+    } else {
+        const uint32_t first = cur & 0x0FFFFFFFu;
+        const uint32_t count = ((cur >> 28) & 0x7u) + 1u;
+        if (COUNT) cnt.tris += count;
+        for (uint32_t k = 0; k < count; k++) {
+            const float4 *tp = s.tris + 3 * size_t(first + k);
+            float t;
+            if (tri_test(r, tp[0], tp[1], tp[2], ts.t, t)) {
+                ts.t = t;
+                ts.hit = first + k;
+                if (ANY) return true;
+            }
         }
     }
+    // pop, skipping entries farther than the current hit
+    for (;;) {
+        if (st.sp == 0) return true;
+        st.sp--;
+        uint32_t ref;
+        float tn;
+        st.top(ref, tn);
+        if (tn <= ts.t) { ts.cur = ref; return false; }
+    }
+}
+
+// Whole traversal of one ray (used where rays are not refilled per lane).
+template <bool ANY, bool COUNT, int kBlock>
+__device__ __forceinline__ uint32_t traverse(const DScene &s, const RaySetup &r, float &t_best, LaneStack<kBlock> &st,
+                                             TraversalCounters &cnt) {
+    TravState ts;
+    ts.r = r;
+    ts.t = t_best;
+    ts.hit = kInvalid;
+    ts.cur = 0;
+    st.sp = 0;
+    while (!trav_step<ANY, COUNT, kBlock>(s, ts, st, cnt)) {
+    }
+    t_best = ts.t;
+    return ts.hit;
 }
 
 }  // namespace wr
