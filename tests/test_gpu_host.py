@@ -1,0 +1,47 @@
+"""The command line renders a scene JSON on the GPU through PathNeeRenderer and
+writes the reference's PFM; the image equals the Python API's render of the
+same scene and seed, and matches the oracle on the same random numbers."""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+import wurstrenderer_amd as wr
+from wurstrenderer_amd import scenes
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "wurstrenderer_amd", "wurst_amd")
+
+
+def read_pfm(path):
+    with open(path, "rb") as f:
+        assert f.readline().strip() == b"PF"
+        w, h = (int(x) for x in f.readline().split())
+        assert float(f.readline()) < 0  # little endian
+        data = np.frombuffer(f.read(), dtype="<f4").reshape(h, w, 3)
+    return data[::-1]  # rows are stored bottom-up (floatimage.h:453-457)
+
+
+def test_cli_render_equals_api_and_oracle(tmp_path, gpu_device):
+    sc = scenes.cornell(1, resolution=(64, 48))
+    scenes.write_obj(sc, str(tmp_path), "mesh")
+    j = scenes.scene_json(sc, "mesh.obj", [{"integrator": "nextevent_path_tracer", "output": "out.pfm",
+                                            "num_spp": 8, "num_path_vertices": 10, "seed": 3}])
+    (tmp_path / "scene.json").write_text(json.dumps(j))
+    r = subprocess.run([EXE, "--simple2parse", str(tmp_path / "scene.json")], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    stats = json.loads(r.stdout.strip().splitlines()[-1])
+    assert stats["paths"] == 64 * 48 * 8
+    img = read_pfm(tmp_path / "out.pfm")
+    gs = wr.GpuScene(gpu_device, sc)
+    film = wr.Film(gs)
+    wr.render_nee(gs, film, 8, num_path_vertices=10, seed=3)
+    api = film.image().astype(np.float32)
+    np.testing.assert_allclose(img, api, rtol=1e-6, atol=1e-7)
+    ref, _ = oracle.OracleScene(sc).image(8, num_path_vertices=10, seed=3)
+    other, _ = oracle.OracleScene(sc).image(8, num_path_vertices=10, seed=4)
+    assert oracle.rel_mse(img, ref) * 30 < oracle.rel_mse(other, ref)

@@ -1,0 +1,133 @@
+"""The C++ host (wurstrenderer_amd/wurst_amd): scene JSON + OBJ/MTL loading with
+the reference's material mapping, and its error behaviour.  No GPU: --check
+parses and validates, --dump writes the flattened scene."""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from wurstrenderer_amd import scenes
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "wurstrenderer_amd", "wurst_amd")
+NEE = {"integrator": "nextevent_path_tracer", "output": "out.pfm", "num_spp": 4}
+
+
+def run(args, cwd=None):
+    return subprocess.run([EXE] + args, capture_output=True, text=True, cwd=cwd, timeout=120)
+
+
+def write(tmp_path, sc, renders=(NEE,), extra=None):
+    scenes.write_obj(sc, str(tmp_path), "mesh")
+    j = scenes.scene_json(sc, "mesh.obj", list(renders))
+    if extra:
+        j.update(extra)
+    p = tmp_path / "scene.json"
+    p.write_text(json.dumps(j))
+    return p
+
+
+def dump(tmp_path, sc):
+    p = write(tmp_path, sc)
+    d = tmp_path / "dump"
+    d.mkdir()
+    r = run(["--check", "--dump", str(d), "--simple2parse", str(p)])
+    assert r.returncode == 0, r.stderr
+    rd = lambda n, t: np.fromfile(d / n, dtype=t)
+    return dict(verts=rd("verts.f32", np.float32).reshape(-1, 3), tris=rd("tris.u32", np.uint32).reshape(-1, 3),
+                tri_geom=rd("tri_geom.u32", np.uint32), geom_material=rd("geom_material.i32", np.int32),
+                geom_light=rd("geom_light.i32", np.int32), materials=rd("materials.bin", scenes.MATERIAL_DTYPE),
+                lights=rd("lights.bin", scenes.LIGHT_DTYPE), camera=rd("camera.bin", scenes.CAMERA_DTYPE)[0])
+
+
+def _mixed_scene():
+    sc = scenes.icospheres(3, n_spheres=6, level=2, resolution=(40, 30))
+    mats = sc.materials.copy()
+    mats[0] = (scenes.MIXED, 1, (0.3, 0.4, 0.5), (0.25, 0.25, 0.25), 0.0, 60.0)
+    sc.materials = mats
+    return sc
+
+
+@pytest.mark.parametrize("maker", ["cornell", "icospheres", "mixed"])
+def test_obj_roundtrip_matches_generated_scene(tmp_path, maker):
+    sc = {"cornell": lambda: scenes.cornell(1, resolution=(64, 48)),
+          "icospheres": lambda: scenes.icospheres(2, n_spheres=8, level=2, resolution=(40, 30)),
+          "mixed": _mixed_scene}[maker]()
+    d = dump(tmp_path, sc)
+    np.testing.assert_array_equal(d["tri_geom"], sc.tri_geom)
+    np.testing.assert_array_equal(d["verts"][d["tris"]], sc.verts[sc.tris])  # same triangles, same order
+    np.testing.assert_array_equal(d["geom_light"] >= 0, sc.geom_light >= 0)
+    for g in range(len(sc.geom_material)):
+        if sc.geom_light[g] >= 0:
+            np.testing.assert_array_equal(d["lights"][d["geom_light"][g]]["L"], sc.lights[sc.geom_light[g]]["L"])
+            assert d["lights"][d["geom_light"][g]]["geom"] == g
+        else:
+            a, b = d["materials"][d["geom_material"][g]], sc.materials[sc.geom_material[g]]
+            assert a["type"] == b["type"] and a["twosided"] == 1
+            if b["type"] != scenes.PHONG:
+                np.testing.assert_array_equal(a["rho_d"], b["rho_d"])
+            if b["type"] != scenes.LAMBERT:
+                np.testing.assert_array_equal(a["rho_s"], b["rho_s"])
+                assert a["exponent"] == b["exponent"]  # Ns / 4 (assimpio.h:310-315)
+    assert d["camera"].tobytes() == sc.camera.tobytes()
+
+
+def test_mitsuba_srgb_converts_reflectances(tmp_path):
+    sc = scenes.cornell(1, resolution=(16, 16))
+    p = write(tmp_path, sc)
+    j = json.loads(p.read_text())
+    j["meshes"][0]["mitsuba_srgb"] = True
+    p.write_text(json.dumps(j))
+    d = tmp_path / "dump"
+    d.mkdir()
+    assert run(["--check", "--dump", str(d), "--simple2parse", str(p)]).returncode == 0
+    mats = np.fromfile(d / "materials.bin", dtype=scenes.MATERIAL_DTYPE)
+    srgb = lambda v: v / 12.92 if v <= 0.04045 else ((v + 0.055) / 1.055) ** 2.4  # spectrum.h:13-17
+    got = mats[1]["rho_d"]  # the ceiling (gray 0.75)
+    np.testing.assert_allclose(got, [srgb(np.float64(np.float32(0.75)))] * 3, rtol=1e-6)
+
+
+@pytest.mark.parametrize("mutate,msg", [
+    (lambda j: j["render"][0].update(integrator="nonsense"), "unknown integrator"),
+    (lambda j: j["render"][0].update(integrator="bidirectional_path_tracer"), "not part of this build"),
+    (lambda j: j["camera"].update(radius=0), "focal distance"),
+    (lambda j: j["meshes"][0].update(path="missing.obj"), "cannot open file"),
+    (lambda j: j.update(envmap="sky.hdr"), "envmap"),
+    (lambda j: j["camera"].update(fovy=200), "fov"),
+    (lambda j: j["camera"].update(resolution=[4]), "can't parse resolution"),
+    (lambda j: j.update(medium={"type": "fog"}), "unknown medium"),
+])
+def test_errors_are_reported(tmp_path, mutate, msg):
+    p = write(tmp_path, scenes.cornell(1, resolution=(16, 16)))
+    j = json.loads(p.read_text())
+    mutate(j)
+    p.write_text(json.dumps(j))
+    r = run(["--check", "--simple2parse", str(p)])
+    assert r.returncode == 1 and msg in r.stderr, r.stderr
+
+
+def test_malformed_json_and_emitter_with_exponent(tmp_path):
+    r = run(["--check", "--jsimple2parse", '{"camera": [1, 2,'])
+    assert r.returncode == 1 and "json" in r.stderr
+    sc = scenes.cornell(1, resolution=(16, 16))
+    p = write(tmp_path, sc)
+    mtl = (tmp_path / "mesh.mtl").read_text().replace("Ke 17 17 17", "Ke 17 17 17\nNs 10")
+    (tmp_path / "mesh.mtl").write_text(mtl)
+    r = run(["--check", "--simple2parse", str(p)])
+    assert r.returncode == 1 and "cone-emission" in r.stderr
+
+
+def test_no_lights_is_an_error(tmp_path):
+    sc = scenes.cornell(1, resolution=(16, 16))
+    p = write(tmp_path, sc)
+    mtl = (tmp_path / "mesh.mtl").read_text().replace("Ke 17 17 17", "Ke 0 0 0\nKd 0.5 0.5 0.5")
+    (tmp_path / "mesh.mtl").write_text(mtl)
+    r = run(["--check", "--simple2parse", str(p)])
+    assert r.returncode == 1 and "no lights" in r.stderr
+
+
+def test_default_txt_is_created_when_missing(tmp_path):
+    r = run([], cwd=str(tmp_path))
+    assert r.returncode == 0 and (tmp_path / "default.txt").exists()

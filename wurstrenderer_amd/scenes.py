@@ -291,3 +291,53 @@ def random_rays(scene: Scene, n: int, seed: int = 0, inside: bool = False) -> np
     rays[:, 4:7] = d
     rays[:, 7] = np.inf
     return rays
+
+
+def write_obj(scene: Scene, directory: str, name: str = "scene") -> str:
+    """Writes the scene as OBJ + MTL in the form the reference's loader maps
+    (one object and material per geometry; Kd -> Lambert, Ks/Ns -> Phong with
+    exponent Ns/4, Ke -> emitter) and returns the OBJ path.  Floats are written
+    with 9 significant digits, so FP32 values round-trip exactly."""
+    import os
+    os.makedirs(directory, exist_ok=True)
+    obj_path = os.path.join(directory, name + ".obj")
+    mtl_lines, obj_lines = [], [f"mtllib {name}.mtl"]
+    f9 = lambda x: format(float(x), ".9g")
+    for m_i, m in enumerate(scene.materials):
+        t = int(m["type"])
+        if t == OREN_NAYAR or not int(m["twosided"]):
+            raise ValueError("OBJ/MTL has no encoding for Oren-Nayar or one-sided materials")
+        kd = m["rho_d"] if t in (LAMBERT, MIXED) else np.zeros(3)
+        ks = m["rho_s"] if t in (PHONG, MIXED) else np.zeros(3)
+        ns = 4.0 * float(m["exponent"]) if t in (PHONG, MIXED) else 0.0
+        mtl_lines += [f"newmtl m{m_i}", "Kd " + " ".join(f9(x) for x in kd), "Ks " + " ".join(f9(x) for x in ks),
+                      f"Ns {f9(ns)}", "Ke 0 0 0"]
+    for l_i, l in enumerate(scene.lights):
+        mtl_lines += [f"newmtl e{l_i}", "Kd 0 0 0", "Ks 0 0 0", "Ns 0", "Ke " + " ".join(f9(x) for x in l["L"])]
+    for v in scene.verts:
+        obj_lines.append("v " + " ".join(f9(x) for x in v))
+    for g in range(len(scene.geom_material)):
+        mat = f"m{scene.geom_material[g]}" if scene.geom_light[g] < 0 else f"e{scene.geom_light[g]}"
+        obj_lines += [f"o geom{g}", f"usemtl {mat}"]
+        for t in scene.tris[scene.tri_geom == g]:
+            obj_lines.append("f " + " ".join(str(int(i) + 1) for i in t))
+    with open(os.path.join(directory, name + ".mtl"), "w") as f:
+        f.write("\n".join(mtl_lines) + "\n")
+    with open(obj_path, "w") as f:
+        f.write("\n".join(obj_lines) + "\n")
+    return obj_path
+
+
+def scene_json(scene: Scene, obj_name: str, renders: list[dict]) -> dict:
+    """The Simple2 scene JSON (src/sceneio/simple2parser.h:102-223 keys) for a
+    scene written by write_obj."""
+    cam = scene.camera
+    j = {"camera": {"pos": [float(x) for x in cam["pos"]], "lookat": [float(x) for x in cam["lookat"]],
+                    "up": [float(x) for x in cam["up"]], "resolution": [int(cam["width"]), int(cam["height"])],
+                    "fovy": float(cam["fovy_deg"])},
+         "meshes": [{"path": obj_name}], "render": renders}
+    if float(cam["aperture_radius"]) > 0:
+        if float(cam["aperture_radius"]) != float(cam["focal_distance"]):
+            raise ValueError("the JSON key \"radius\" sets both the aperture and the focal distance")
+        j["camera"]["radius"] = float(cam["aperture_radius"])
+    return j
