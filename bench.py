@@ -41,6 +41,12 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU time of the baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-count", action="store_true", help="skip the traversal-counting pass")
+    ap.add_argument("--no-ttr", action="store_true", help="skip the guided-vs-plain time-to-relMSE measurement")
+    ap.add_argument("--ttr-width", type=int, default=480)
+    ap.add_argument("--ttr-height", type=int, default=270)
+    ap.add_argument("--ttr-target", type=float, default=1e-3)
+    ap.add_argument("--ttr-ref-spp", type=int, default=16384)
+    ap.add_argument("--ttr-runs", type=int, default=3)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first copy 1 GiB device-to-device three times (known-size kernel for byte counters)")
     return ap.parse_args()
@@ -68,6 +74,91 @@ def calibrate_copy(nbytes=1 << 30, reps=3):
     hip.hipFree(src)
     hip.hipFree(dst)
     print(f"# calibration: {reps} x {nbytes} B device-to-device copies", file=sys.stderr, flush=True)
+
+
+def rel_mse(img, ref, eps=1e-2):
+    """mean over pixels and channels of (I - R)^2 / (R^2 + eps)"""
+    import numpy as np
+    return float(np.mean((img - ref) ** 2 / (ref * ref + eps)))
+
+
+def progressive(render, film, ref, target, first_spp, max_spp):
+    """Progressive spp doubling; wall time of the render calls until relMSE <= target."""
+    t_total, spp, step, curve = 0.0, 0, first_spp, []
+    while spp < max_spp:
+        t = time.perf_counter()
+        render(film, step)
+        t_total += time.perf_counter() - t
+        spp += step
+        e = rel_mse(film.image(), ref)
+        curve.append([spp, round(t_total, 4), e])
+        if e <= target:
+            return t_total, spp, curve
+        step = spp
+    return None, spp, curve
+
+
+def time_to_relmse(wr, scenes, dev, a):
+    """Guided vs plain GPU NEE+MIS on the gap-lit room (1,003,550 triangles, light
+    L = 500 hidden behind a 0.2 x 0.2 gap), max depth 16: wall time until the
+    progressively doubled image reaches relMSE <= target against a plain
+    reference at ttr_ref_spp; median of ttr_runs runs, each from an untrained
+    network.  Guided: 4-spp passes with 4 Adam steps of 2^18 records after each
+    until 64 spp, then the trained network without further training."""
+    import numpy as np
+    N = 18
+    sc = scenes.guide_room(4, resolution=(a.ttr_width, a.ttr_height))
+    gs = wr.GpuScene(dev, sc)
+    t = time.perf_counter()
+    half = a.ttr_ref_spp // 2
+    fa, fb = wr.Film(gs), wr.Film(gs)
+    wr.render_nee(gs, fa, half, num_path_vertices=N, seed=9001)
+    wr.render_nee(gs, fb, half, num_path_vertices=N, seed=9002)
+    ia, ib = fa.image(), fb.image()
+    ref = 0.5 * (ia + ib)
+    ref_s = time.perf_counter() - t
+    floor = rel_mse(ia, ib) / 4.0  # relMSE of the mean of two halves against the truth, approximately
+    print(f"# ttr reference: {a.ttr_ref_spp} spp in {ref_s:.1f}s, noise floor ~{floor:.2e}", file=sys.stderr, flush=True)
+    max_spp = a.ttr_ref_spp // 2
+
+    def plain(seed):
+        return lambda film, spp: wr.render_nee(gs, film, spp, num_path_vertices=N, seed=seed)
+
+    def guided(seed):
+        g = wr.Guide(gs, batch_size=1 << 18, record_capacity=1 << 20, seed=seed)
+        state = {"calls": 0}
+
+        def render(film, spp):
+            if film.spp < 64:
+                wr.render_guided(gs, film, g, spp, num_path_vertices=N, seed=seed, guide_fraction=0.5,
+                                 spp_per_pass=4, train_steps=4, warmup_passes=1 if state["calls"] == 0 else 0)
+            else:
+                wr.render_guided(gs, film, g, spp, num_path_vertices=N, seed=seed, guide_fraction=0.5,
+                                 spp_per_pass=spp, train_steps=0, warmup_passes=0)
+            state["calls"] += 1
+        return render
+
+    res = {}
+    for name, make, first in (("plain", plain, 16), ("guided", guided, 16)):
+        times, curves = [], []
+        for r in range(a.ttr_runs):
+            film = wr.Film(gs)
+            tt, spp, curve = progressive(make(100 + r), film, ref, a.ttr_target, first, max_spp)
+            times.append(tt)
+            curves.append(curve)
+            print(f"# ttr {name} run {r}: {tt if tt is None else round(tt, 3)} s at {spp} spp; "
+                  f"last relMSE {curve[-1][2]:.3e}", file=sys.stderr, flush=True)
+        ok = [x for x in times if x is not None]
+        res[name] = {"seconds_median": float(np.median(ok)) if len(ok) == len(times) else None,
+                     "seconds_runs": times, "curve_run0": curves[0]}
+    sp = None
+    if res["plain"]["seconds_median"] and res["guided"]["seconds_median"]:
+        sp = res["plain"]["seconds_median"] / res["guided"]["seconds_median"]
+    return {"target_relmse": a.ttr_target, "workload": f"guide_room seed 4 (1,003,550 tris, gap 0.2, L 500), "
+                                                       f"{a.ttr_width}x{a.ttr_height}, max depth 16",
+            "reference": {"spp": a.ttr_ref_spp, "estimator": "plain", "noise_floor_relmse": floor,
+                          "seconds": round(ref_s, 2)},
+            "plain": res["plain"], "guided": res["guided"], "speedup_guided_vs_plain": sp}
 
 
 def cpu_baseline(sc, N, target_s):
@@ -150,6 +241,9 @@ def main():
                 "avg_launch_ms": round(avg_ms, 4), "launches": ext_launches,
                 "nodes_per_ray": round(nodes_per_ray, 2), "tris_per_ray": round(tris_per_ray, 2),
                 "bytes_per_ray": round(EXT_RAY_BYTES + nodes_per_ray * NODE_BYTES + tris_per_ray * TRI_BYTES, 1)}
+    ttr = None
+    if not a.no_ttr:
+        ttr = time_to_relmse(wr, scenes, dev, a)
     cpu = None
     if not a.no_cpu_baseline:
         cpu = cpu_baseline(sc, N, a.cpu_seconds)
@@ -176,6 +270,7 @@ def main():
                       ("extend_ms", "shadow_ms", "shade_ms", "generate_ms", "accumulate_ms")},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "time_to_relmse": ttr,
     }
     print(json.dumps(out))
 

@@ -174,6 +174,72 @@ int wr_trace_any(wr_scene *scene, const float *rays, uint32_t n, uint8_t *occlud
 int wr_trace_closest_device(wr_scene *scene, const float *d_rays, uint32_t n, void *d_hits, float *ms);
 int wr_trace_any_device(wr_scene *scene, const float *d_rays, uint32_t n, uint8_t *d_occluded, float *ms);
 
+/* ------------------------------------------------------------------ guiding
+ * Learned path guiding.  No reference counterpart: the reference has no
+ * guiding; PathNeeRenderer samples the BSDF only (generalsubpathsampler.h:81-130).
+ * The network (one-blob 32 -> 4 x 64 ReLU -> 40) outputs 8 normalised
+ * spherical-Gaussian (von Mises-Fisher) lobes: for lobe k the raw outputs
+ * 5k..5k+4 are (weight logit, axis xyz, sharpness), with weights softmax(logits),
+ * axes normalised and sharpness softplus(raw) clamped to [0.05, 2000].  Its
+ * density, lambda / (2 pi (1 - e^{-2 lambda})) exp(lambda (mu.w - 1)) per lobe, is
+ * mixed with BSDF sampling by one-sample MIS, so the estimator stays unbiased
+ * for any weights.  Parameters are FP32 (layer l: W_l [in][out] row-major, then
+ * b_l); the GPU computes in BF16 with FP32 accumulation. */
+#define WR_GUIDE_NUM_PARAMS 17192
+#define WR_GUIDE_NUM_OUTPUTS 40
+
+typedef struct wr_guide wr_guide;
+
+typedef struct {
+    float learning_rate;       /* Adam step size (1e-2) */
+    float beta1, beta2, epsilon; /* Adam moments (0.9, 0.999, 1e-8) */
+    uint32_t batch_size;       /* records per training step (2^18) */
+    uint32_t record_capacity;  /* training records kept per pass (2^20) */
+    uint32_t seed;             /* weight initialisation */
+} wr_guide_config;
+
+typedef struct {
+    float guide_fraction;      /* probability of sampling the network at a vertex (0.5) */
+    uint32_t spp_per_pass;     /* samples per pixel between training rounds */
+    uint32_t train_steps;      /* Adam steps after each pass (0: render only) */
+    uint32_t warmup_passes;    /* passes that sample the BSDF only (and collect); the network
+                                  is sampled from pass warmup_passes on, trained or not */
+} wr_guided_params;
+
+typedef struct {
+    uint64_t records;          /* training records collected */
+    uint64_t train_steps;      /* Adam steps taken */
+    double train_ms;           /* HIP-event time of finalise + training */
+    double last_loss;          /* normalised loss of the last step */
+} wr_guide_stats;
+
+void wr_guide_config_default(wr_guide_config *cfg);
+/* A network bound to a scene (its bounds fix the position encoding) and that
+ * scene's stream. */
+int wr_guide_create(wr_scene *scene, const wr_guide_config *cfg, wr_guide **out);
+void wr_guide_destroy(wr_guide *guide);
+/* Host copies of the FP32 parameters; set also resets the Adam state. */
+int wr_guide_get_params(wr_guide *guide, float *params);
+int wr_guide_set_params(wr_guide *guide, const float *params);
+/* Network outputs for n host records (pos, nrm, wo: n*3 floats; nrm faces the
+ * incoming ray, wo points to the previous vertex): out n*40 raw outputs. */
+int wr_guide_forward(wr_guide *guide, const float *pos, const float *nrm, const float *wo, uint32_t n, float *out);
+/* Gradient of the loss (1/n) sum_r -weight_r log q_r(wi_r) over n host records
+ * (wi: n*3 sampled directions, weight: n); grad has WR_GUIDE_NUM_PARAMS floats. */
+int wr_guide_gradient(wr_guide *guide, const float *pos, const float *nrm, const float *wo, const float *wi,
+                      const float *weight, uint32_t n, float *grad, double *loss);
+/* `steps` Adam steps on n host records (the whole set per step). */
+int wr_guide_train(wr_guide *guide, const float *pos, const float *nrm, const float *wo, const float *wi,
+                   const float *weight, uint32_t n, uint32_t steps, double *loss);
+/* The mixture of n raw output rows (n*40): pdf at dirs (n*3), and one sample
+ * per row from u (n*3 uniforms) with its pdf. */
+int wr_guide_mixture(wr_guide *guide, const float *raw, const float *dirs, const float *u, uint32_t n, float *pdf_dirs,
+                     float *samples, float *sample_pdf);
+/* wr_render_nee with guided sampling: progressive passes of spp_per_pass;
+ * after each pass the network trains on records of that pass. */
+int wr_render_guided(wr_scene *scene, wr_film *film, wr_guide *guide, const wr_render_params *params,
+                     const wr_guided_params *gparams, wr_render_stats *stats, wr_guide_stats *gstats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,218 @@
+"""GPU path-guiding network against the FP64 oracle (oracle/guide.py), and the
+guided renderer against the plain one.
+
+Bars, written per test:
+  - forward: GPU (BF16 MFMA, FP32 accumulation) vs the oracle with the same BF16
+    rounding points: relative L2 error < 2e-3, max abs error < 3e-2;
+  - gradient: relative L2 error < 1e-2 per layer and cosine > 0.9999 overall
+    (the oracle's own gradient is pinned by finite differences);
+  - Adam: parameters after 3 steps within 1e-4 of the oracle's where the
+    gradient is not tiny (|g| > 1e-3 max|g|);
+  - mixture: pdf rtol 1e-4; samples from the same uniforms within 2e-3 away
+    from lobe-selection boundaries;
+  - guided render: unbiased for any network -- the image mean within 4 sigma
+    of the plain GPU render, which matches the oracle elsewhere;
+  - gain: after training on the gap-lit room, guided relMSE at equal spp
+    below the plain renderer's."""
+import numpy as np
+import pytest
+
+from oracle import guide as G
+import wurstrenderer_amd as wr
+from wurstrenderer_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def scene_box(sc):
+    v = sc.verts.astype(np.float32)
+    return np.concatenate([v.min(0), v.max(0)]).astype(np.float64)
+
+
+def records(rng, n, box):
+    lo, hi = box[:3], box[3:]
+    pos = (lo + (hi - lo) * rng.random((n, 3))).astype(np.float32)
+
+    def unit(k):
+        v = rng.normal(size=(k, 3))
+        return (v / np.linalg.norm(v, axis=1, keepdims=True)).astype(np.float32)
+
+    return pos, unit(n), unit(n), unit(n), (rng.random(n) + 0.1).astype(np.float32)
+
+
+def params(rng, scale_out=0.5):
+    layers = []
+    for l, (k, n) in enumerate(G.SHAPES):
+        lim = np.sqrt(6.0 / k) * (scale_out if l == G.N_LAYERS - 1 else 1.0)
+        layers.append((rng.uniform(-lim, lim, (k, n)), rng.uniform(-0.1, 0.1, n)))
+    return G.pack(layers).astype(np.float32)
+
+
+@pytest.fixture(scope="module")
+def guide_scene(gpu_device):
+    sc = scenes.cornell(1, resolution=(64, 64))
+    return sc, wr.GpuScene(gpu_device, sc)
+
+
+def test_forward_matches_oracle(guide_scene):
+    sc, gs = guide_scene
+    print("guide forward: 1000 records", flush=True)
+    g = wr.Guide(gs)
+    rng = np.random.default_rng(10)
+    p = params(rng)
+    g.set_params(p)
+    np.testing.assert_array_equal(g.get_params(), p)
+    box = scene_box(sc)
+    pos, nrm, wo, _, _ = records(rng, 1000, box)
+    out = g.forward(pos, nrm, wo)
+    ref, _ = G.forward(p, pos, nrm, wo, box, bf=True)
+    fp64, _ = G.forward(p, pos, nrm, wo, box)
+    err = np.linalg.norm(out - ref) / np.linalg.norm(ref)
+    err64 = np.linalg.norm(out - fp64) / np.linalg.norm(fp64)
+    print(f"  rel err vs bf16 oracle {err:.2e}, vs fp64 oracle {err64:.2e}, max abs {np.abs(out - ref).max():.2e}")
+    assert err < 2e-3 and np.abs(out - ref).max() < 3e-2
+    # the GPU is closer to the BF16 emulation than to FP64 (it really rounds there)
+    assert err < err64
+
+
+def test_gradient_matches_oracle(guide_scene):
+    sc, gs = guide_scene
+    print("guide gradient: 333 records (ragged last tile)", flush=True)
+    g = wr.Guide(gs)
+    rng = np.random.default_rng(11)
+    p = params(rng, scale_out=1.0)
+    g.set_params(p)
+    box = scene_box(sc)
+    pos, nrm, wo, wi, w = records(rng, 333, box)
+    loss, grad = g.gradient(pos, nrm, wo, wi, w)
+    rloss, rgrad = G.loss_and_grad(p, pos, nrm, wo, wi, w, box, bf=True)
+    print(f"  loss gpu {loss:.6f} oracle {rloss:.6f}")
+    assert abs(loss - rloss) < 1e-3 * abs(rloss)
+    o = 0
+    for l, (k, n) in enumerate(G.SHAPES):
+        for name, sz in (("W", k * n), ("b", n)):
+            a, b = grad[o:o + sz], rgrad[o:o + sz]
+            e = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+            print(f"  layer {l} {name}: rel err {e:.2e}")
+            assert e < 1e-2, (l, name, e)
+            o += sz
+    cos = grad @ rgrad / (np.linalg.norm(grad) * np.linalg.norm(rgrad))
+    assert cos > 0.9999
+
+
+def test_adam_steps_match_oracle(guide_scene):
+    sc, gs = guide_scene
+    print("guide adam: 3 steps on 256 records", flush=True)
+    g = wr.Guide(gs, learning_rate=1e-2)
+    rng = np.random.default_rng(12)
+    p = params(rng)
+    g.set_params(p)
+    box = scene_box(sc)
+    pos, nrm, wo, wi, w = records(rng, 256, box)
+    g.train(pos, nrm, wo, wi, w, steps=3)
+    got = g.get_params().astype(np.float64)
+    q, m, v = p.astype(np.float64), np.zeros(G.NUM_PARAMS), np.zeros(G.NUM_PARAMS)
+    big = np.zeros(G.NUM_PARAMS, bool)
+    for t in (1, 2, 3):
+        _, gr = G.loss_and_grad(q.astype(np.float32), pos, nrm, wo, wi, w, box, bf=True)
+        big |= np.abs(gr) > 1e-3 * np.abs(gr).max()
+        q, m, v = G.adam_step(q, m, v, gr, t, lr=1e-2)
+    moved = np.abs(got - p) > 0
+    assert moved.mean() > 0.5
+    err = np.abs(got - q)[big]
+    print(f"  max |param - oracle| over {big.sum()} params: {err.max():.2e}")
+    assert err.max() < 1e-4 * 3 + 1e-5
+
+
+def test_mixture_matches_oracle(guide_scene):
+    _, gs = guide_scene
+    print("guide mixture: 4096 rows", flush=True)
+    g = wr.Guide(gs)
+    rng = np.random.default_rng(13)
+    n = 4096
+    raw = rng.normal(size=(n, 40)).astype(np.float32)
+    raw[:, 4::5] = rng.uniform(-2, 6, size=(n, 8))  # sharpness 0.13 .. 6
+    raw[::7, 4::5] = rng.uniform(20, 200, size=(len(raw[::7]), 8))  # and very sharp lobes
+    d = rng.normal(size=(n, 3))
+    d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    u = rng.random((n, 3)).astype(np.float32)
+    pd, smp, sp = g.mixture(raw, d, u)
+    np.testing.assert_allclose(pd, G.pdf(raw, d), rtol=1e-4, atol=1e-30)
+    ref, _ = G.sample(raw, u)
+    logw, _, _ = G.decode(raw)
+    c = np.cumsum(np.exp(logw), axis=1)
+    far = np.abs(u[:, :1] - c[:, :-1]).min(axis=1) > 1e-4
+    err = np.linalg.norm(smp - ref, axis=1)
+    print(f"  sample dir err: median {np.median(err):.1e}, max away from boundaries {err[far].max():.1e}")
+    assert far.mean() > 0.95 and err[far].max() < 2e-3
+    np.testing.assert_allclose(sp, G.pdf(raw, smp.astype(np.float64)), rtol=2e-3)
+
+
+def _render(gs, spp, N, seed, guide=None, **kw):
+    film = wr.Film(gs)
+    if guide is None:
+        st = wr.render_nee(gs, film, spp, num_path_vertices=N, seed=seed)
+        return film.image(), st, None
+    st, gst = wr.render_guided(gs, film, guide, spp, num_path_vertices=N, seed=seed, **kw)
+    return film.image(), st, gst
+
+
+def test_guided_render_is_unbiased_with_any_network(gpu_device):
+    """A random network mixed in with alpha = 0.5 changes the noise, not the mean."""
+    sc = scenes.cornell(1, resolution=(48, 48))
+    gs = wr.GpuScene(gpu_device, sc)
+    print("guided render, random network: cornell 48x48", flush=True)
+    g = wr.Guide(gs, seed=5)
+    spp = 1024
+    plain, _, _ = _render(gs, spp, 10, seed=21)
+    guided, st, gst = _render(gs, spp, 10, seed=22, guide=g, guide_fraction=0.5, spp_per_pass=spp, train_steps=0,
+                              warmup_passes=0)
+    assert gst.records == 0 and gst.train_steps == 0
+    assert np.isfinite(guided).all() and st.extend_rays > 0
+    # per-pixel noise of the two estimators from two more independent renders
+    plain2, _, _ = _render(gs, spp, 10, seed=23)
+    guided2, _, _ = _render(gs, spp, 10, seed=24, guide=g, guide_fraction=0.5, spp_per_pass=spp, train_steps=0,
+                            warmup_passes=0)
+    npx = guided.shape[0] * guided.shape[1]
+    sig = np.sqrt(((guided - guided2) ** 2).mean((0, 1)) / 2 / npx + ((plain - plain2) ** 2).mean((0, 1)) / 2 / npx)
+    diff = guided.mean((0, 1)) - plain.mean((0, 1))
+    print(f"  mean plain {plain.mean((0, 1))} guided {guided.mean((0, 1))} sigma {sig}")
+    assert np.all(np.abs(diff) < 4 * sig + 1e-6), (diff, sig)
+
+
+def test_guided_render_trains_and_stays_unbiased(gpu_device):
+    sc = scenes.guide_room(4, n_spheres=6, level=3, resolution=(64, 48))
+    gs = wr.GpuScene(gpu_device, sc)
+    print("guided render with training: guide room 64x48", flush=True)
+    g = wr.Guide(gs, batch_size=1 << 14, record_capacity=1 << 16)
+    before = g.get_params()
+    img, st, gst = _render(gs, 256, 18, seed=31, guide=g, guide_fraction=0.5, spp_per_pass=32, train_steps=8,
+                           warmup_passes=1)
+    print(f"  records {gst.records} steps {gst.train_steps} loss {gst.last_loss:.4f} train_ms {gst.train_ms:.1f}")
+    assert gst.records > 1000 and gst.train_steps == 8 * 8 and np.isfinite(gst.last_loss)
+    assert np.isfinite(img).all()
+    assert np.abs(g.get_params() - before).max() > 0
+    ref, _, _ = _render(gs, 2048, 18, seed=32)
+    plain, _, _ = _render(gs, 256, 18, seed=33)
+    plain2, _, _ = _render(gs, 256, 18, seed=34)
+    npx = img.shape[0] * img.shape[1]
+    sig = np.sqrt(2 * ((plain - plain2) ** 2).mean((0, 1)) / 2 / npx)
+    print(f"  mean ref {ref.mean((0, 1))} guided {img.mean((0, 1))} sigma {sig}")
+    assert np.all(np.abs(img.mean((0, 1)) - ref.mean((0, 1))) < 5 * sig + 1e-6)
+
+
+def test_trained_guiding_beats_plain_at_equal_spp(gpu_device):
+    sc = scenes.guide_room(4, n_spheres=12, level=3, resolution=(96, 64))
+    gs = wr.GpuScene(gpu_device, sc)
+    print("guided vs plain at equal spp: guide room 96x64", flush=True)
+    ref, _, _ = _render(gs, 16384, 18, seed=41)
+    g = wr.Guide(gs, batch_size=1 << 15, record_capacity=1 << 17)
+    # train, then render with the trained network
+    _render(gs, 256, 18, seed=42, guide=g, guide_fraction=0.5, spp_per_pass=16, train_steps=8, warmup_passes=1)
+    guided, _, _ = _render(gs, 256, 18, seed=43, guide=g, guide_fraction=0.5, spp_per_pass=256, train_steps=0,
+                           warmup_passes=0)
+    plain, _, _ = _render(gs, 256, 18, seed=44)
+    import oracle
+    e_g, e_p = oracle.rel_mse(guided, ref), oracle.rel_mse(plain, ref)
+    print(f"  relMSE at 256 spp: plain {e_p:.4e} guided {e_g:.4e} (gain {e_p / e_g:.2f}x)")
+    assert e_g < e_p

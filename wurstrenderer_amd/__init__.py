@@ -60,6 +60,27 @@ class RenderStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class GuideConfig(C.Structure):
+    _fields_ = [("learning_rate", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("epsilon", C.c_float),
+                ("batch_size", C.c_uint32), ("record_capacity", C.c_uint32), ("seed", C.c_uint32)]
+
+
+class GuidedParams(C.Structure):
+    _fields_ = [("guide_fraction", C.c_float), ("spp_per_pass", C.c_uint32), ("train_steps", C.c_uint32),
+                ("warmup_passes", C.c_uint32)]
+
+
+class GuideStats(C.Structure):
+    _fields_ = [("records", C.c_uint64), ("train_steps", C.c_uint64), ("train_ms", C.c_double),
+                ("last_loss", C.c_double)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+GUIDE_NUM_PARAMS = 17192
+GUIDE_NUM_OUTPUTS = 40
+
 # name -> (restype, argtypes); every symbol declared in include/wurst_amd.h
 P = C.c_void_p
 EXPORTS = {
@@ -83,6 +104,17 @@ EXPORTS = {
     "wr_trace_any": (C.c_int, [P, P, C.c_uint32, P]),
     "wr_trace_closest_device": (C.c_int, [P, P, C.c_uint32, P, C.POINTER(C.c_float)]),
     "wr_trace_any_device": (C.c_int, [P, P, C.c_uint32, P, C.POINTER(C.c_float)]),
+    "wr_guide_config_default": (None, [C.POINTER(GuideConfig)]),
+    "wr_guide_create": (C.c_int, [P, C.POINTER(GuideConfig), C.POINTER(P)]),
+    "wr_guide_destroy": (None, [P]),
+    "wr_guide_get_params": (C.c_int, [P, P]),
+    "wr_guide_set_params": (C.c_int, [P, P]),
+    "wr_guide_forward": (C.c_int, [P, P, P, P, C.c_uint32, P]),
+    "wr_guide_gradient": (C.c_int, [P, P, P, P, P, P, C.c_uint32, P, C.POINTER(C.c_double)]),
+    "wr_guide_train": (C.c_int, [P, P, P, P, P, P, C.c_uint32, C.c_uint32, C.POINTER(C.c_double)]),
+    "wr_guide_mixture": (C.c_int, [P, P, P, P, C.c_uint32, P, P, P]),
+    "wr_render_guided": (C.c_int, [P, P, P, C.POINTER(RenderParams), C.POINTER(GuidedParams), C.POINTER(RenderStats),
+                                   C.POINTER(GuideStats)]),
 }
 
 _lib = None
@@ -241,3 +273,95 @@ def render_nee(scene: GpuScene, film: Film, spp: int, num_path_vertices: int = 1
     _check(lib().wr_render_nee(scene.h, film.h, C.byref(p), C.byref(st)))
     film.spp += spp
     return st
+
+
+def _f32(a, cols=None):
+    a = np.ascontiguousarray(np.asarray(a, dtype=np.float32))
+    if cols is not None and (a.ndim != 2 or a.shape[1] != cols):
+        raise ValueError(f"expected an (n, {cols}) array, got {a.shape}")
+    return a
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Guide:
+    """The path-guiding network bound to a scene (wr_guide_* of the C-ABI)."""
+
+    def __init__(self, scene: GpuScene, learning_rate: float = 1e-2, batch_size: int = 1 << 18,
+                 record_capacity: int = 1 << 20, seed: int = 0):
+        cfg = GuideConfig()
+        lib().wr_guide_config_default(C.byref(cfg))
+        cfg.learning_rate = learning_rate
+        cfg.batch_size = batch_size
+        cfg.record_capacity = record_capacity
+        cfg.seed = seed
+        h = C.c_void_p()
+        _check(lib().wr_guide_create(scene.h, C.byref(cfg), C.byref(h)))
+        self.h = h
+        self.scene = scene
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().wr_guide_destroy(self.h)
+            self.h = None
+
+    def get_params(self) -> np.ndarray:
+        p = np.empty(GUIDE_NUM_PARAMS, dtype=np.float32)
+        _check(lib().wr_guide_get_params(self.h, _ptr(p)))
+        return p
+
+    def set_params(self, params) -> None:
+        p = _f32(params).ravel()
+        if p.size != GUIDE_NUM_PARAMS:
+            raise ValueError("wrong parameter count")
+        _check(lib().wr_guide_set_params(self.h, _ptr(p)))
+
+    def forward(self, pos, nrm, wo) -> np.ndarray:
+        pos, nrm, wo = _f32(pos, 3), _f32(nrm, 3), _f32(wo, 3)
+        out = np.empty((len(pos), GUIDE_NUM_OUTPUTS), dtype=np.float32)
+        _check(lib().wr_guide_forward(self.h, _ptr(pos), _ptr(nrm), _ptr(wo), len(pos), _ptr(out)))
+        return out
+
+    def gradient(self, pos, nrm, wo, wi, weight):
+        pos, nrm, wo, wi = _f32(pos, 3), _f32(nrm, 3), _f32(wo, 3), _f32(wi, 3)
+        w = _f32(weight).ravel()
+        g = np.empty(GUIDE_NUM_PARAMS, dtype=np.float32)
+        loss = C.c_double()
+        _check(lib().wr_guide_gradient(self.h, _ptr(pos), _ptr(nrm), _ptr(wo), _ptr(wi), _ptr(w), len(pos), _ptr(g),
+                                       C.byref(loss)))
+        return loss.value, g
+
+    def train(self, pos, nrm, wo, wi, weight, steps: int) -> float:
+        pos, nrm, wo, wi = _f32(pos, 3), _f32(nrm, 3), _f32(wo, 3), _f32(wi, 3)
+        w = _f32(weight).ravel()
+        loss = C.c_double()
+        _check(lib().wr_guide_train(self.h, _ptr(pos), _ptr(nrm), _ptr(wo), _ptr(wi), _ptr(w), len(pos), steps,
+                                    C.byref(loss)))
+        return loss.value
+
+    def mixture(self, raw, dirs, u):
+        raw, dirs, u = _f32(raw, GUIDE_NUM_OUTPUTS), _f32(dirs, 3), _f32(u, 3)
+        n = len(raw)
+        pd = np.empty(n, dtype=np.float32)
+        smp = np.empty((n, 3), dtype=np.float32)
+        sp = np.empty(n, dtype=np.float32)
+        _check(lib().wr_guide_mixture(self.h, _ptr(raw), _ptr(dirs), _ptr(u), n, _ptr(pd), _ptr(smp), _ptr(sp)))
+        return pd, smp, sp
+
+
+def render_guided(scene: GpuScene, film: Film, guide: Guide, spp: int, num_path_vertices: int = 10, seed: int = 0,
+                  guide_fraction: float = 0.5, spp_per_pass: int = 4, train_steps: int = 4, warmup_passes: int = 1,
+                  use_mis: bool = True, first_sample: int | None = None, max_paths_in_flight: int = 0,
+                  profile: bool = False):
+    """wr_render_guided: progressive passes of spp_per_pass, training between passes."""
+    if first_sample is None:
+        first_sample = film.spp
+    p = RenderParams(spp, num_path_vertices, seed, int(bool(use_mis)), first_sample, max_paths_in_flight,
+                     int(bool(profile)), 0)
+    gp = GuidedParams(guide_fraction, spp_per_pass, train_steps, warmup_passes)
+    st, gs = RenderStats(), GuideStats()
+    _check(lib().wr_render_guided(scene.h, film.h, guide.h, C.byref(p), C.byref(gp), C.byref(st), C.byref(gs)))
+    film.spp += spp
+    return st, gs

@@ -1,0 +1,368 @@
+// wr_guide.h -- path-guiding network on the device: one-blob encoding, the
+// von Mises-Fisher (normalised spherical-Gaussian) mixture, and the wave-level
+// BF16 MFMA forward pass.
+//
+// No reference counterpart: the reference has no learned guiding.  Its only SG
+// convention is SgLobe::eval = a exp((1 - w.mu) lambda)
+// (src/sphericalrepresentation/sphericalgaussian.h:10-21), which peaks at
+// w = -mu; this file uses the normalised lobe
+//   c(w) = lambda / (2 pi (1 - exp(-2 lambda))) exp(lambda (mu.w - 1)).
+//
+// Network: 32 inputs -> 64 -> 64 -> 64 -> 64 (ReLU) -> 40 raw outputs,
+// 8 lobes x (weight logit, axis xyz, sharpness).  Mixture weights are
+// softmax(logits), axes are normalised, sharpness = softplus(raw) clamped to
+// [kLambdaMin, kLambdaMax].  Weights and activations are BF16, products
+// accumulate in FP32 (v_mfma_f32_32x32x16_bf16), master weights are FP32.
+//
+// Register-resident layout.  The network runs transposed, Y^T = W^T X^T, so a
+// wave's 64 records are the MFMA columns.  A layer's output D tile (lane
+// (p, h) = (l % 32, l / 32) holds rows drow(v, h) = 8 (v / 4) + 4 h + v % 4 of
+// column p in register v) feeds the next layer's B operand without leaving
+// registers: B element e of k-step ks in lane half h is taken to be input
+// feature kperm(ks, h, e) = 16 ks + 8 (e / 4) + 4 h + e % 4, which is exactly
+// where the previous layer left it, and the weights are packed with the same
+// permutation of the contraction index.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <hip/hip_bf16.h>
+
+#include "wr_device.h"
+
+namespace wr {
+namespace guide {
+
+constexpr int kIn = 32, kHid = 64, kOut = 40, kLayers = 5, kLobes = 8;
+constexpr float kLambdaMin = 0.05f, kLambdaMax = 2000.f;
+
+// Layer l: weight W_l [K_l][N_l] (input-major), then bias b_l [N_l]; the FP32
+// parameter vector stores W0, b0, W1, b1, ...
+__host__ __device__ constexpr int layer_in(int l) { return l == 0 ? kIn : kHid; }
+__host__ __device__ constexpr int layer_out(int l) { return l == kLayers - 1 ? kOut : kHid; }
+// (closed forms: no recursion, which a runtime l would turn into a device call)
+__host__ __device__ constexpr int w_offset(int l) {
+    return l == 0 ? 0
+                  : (kIn * kHid + kHid) + (l - 1) * (kHid * kHid + kHid) -
+                        (l == kLayers ? (kHid * kHid + kHid) - (kHid * kOut + kOut) : 0);
+}
+__host__ __device__ constexpr int b_offset(int l) { return w_offset(l) + layer_in(l) * layer_out(l); }
+constexpr int kNumParams = w_offset(kLayers);
+static_assert(kNumParams == 17192, "parameter count");
+static_assert(w_offset(1) == 32 * 64 + 64 && w_offset(4) == w_offset(3) + 64 * 64 + 64, "layer offsets");
+
+// Packed BF16 operands, 64 lanes x 8 values per (tile, k-step):
+//   forward  F_l[mt][ks]: lane (r, h), e -> W_l[kperm(ks, h, e)][32 mt + r]  (0 past N_l)
+//            mt < 2 (outputs padded to 64), ks < K_l / 16
+//   backward B_l[mt][ks]: lane (r, h), e -> W_l[32 mt + r][kperm(ks, h, e)]  (0 past N_l)
+//            mt < K_l / 32, ks < 4 (outputs padded to 64); layers 1..4
+__host__ __device__ constexpr int kperm(int ks, int h, int e) { return 16 * ks + 8 * (e >> 2) + 4 * h + (e & 3); }
+__host__ __device__ constexpr int fwd_offset(int l) { return l == 0 ? 0 : 2 * (kIn / 16) * 64 + (l - 1) * 2 * (kHid / 16) * 64; }
+constexpr int kFwdOps = fwd_offset(kLayers);
+__host__ __device__ constexpr int bwd_offset(int l) { return l <= 1 ? 0 : (l - 1) * (kHid / 32) * 4 * 64; }
+constexpr int kBwdOps = bwd_offset(kLayers);
+// Row of D register v in lane half h.
+__host__ __device__ constexpr int drow(int v, int h) { return 8 * (v >> 2) + 4 * h + (v & 3); }
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ f32x16 mfma(bf16x8 a, bf16x8 b, f32x16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ __bf16 bf(float x) { return __float2bfloat16(x); }
+__device__ __forceinline__ float fb(__bf16 x) { return __bfloat162float(x); }
+
+// Device view of a network.
+struct Net {
+    const bf16x8 *fwd;    // kFwdOps packed forward operands
+    const bf16x8 *bwd;    // kBwdOps packed backward operands
+    const float *params;  // FP32 master parameters (biases are read from here)
+    float box[6];         // scene bounds for the position encoding (lo xyz, hi xyz)
+};
+
+// Training records written by the renderer, one per recorded path vertex.
+struct Records {
+    float4 *pos;       // xyz: vertex position; w: sample index in the pass (bits)
+    float4 *nrm;       // xyz: normal facing the incoming ray
+    float4 *wo;        // xyz: direction to the previous vertex
+    float4 *wi;        // xyz: sampled direction; w: training weight (set after the pass)
+    float4 *thr;       // xyz: path throughput before the vertex
+    float4 *rad;       // xyz: path radiance when the next vertex is shaded
+    uint32_t *count;   // records appended (may exceed capacity; the excess is dropped)
+    uint32_t capacity;
+};
+
+// Decoded lobes of the current path state: 10 float4 per entry, SoA
+// (lobes[j * stride + i], j < 10), written by the inference kernel.
+constexpr int kLobeVecs = kOut / 4;
+
+// What the shading kernel needs for guided sampling.
+struct Shade {
+    const float4 *lobes;
+    uint32_t lobe_stride;
+    Records rec;
+    float alpha;        // one-sample MIS probability of sampling the network (0: BSDF only)
+    float record_prob;  // fraction of paths whose vertices are recorded
+    int collect;        // record training data
+};
+
+// ---------------------------------------------------------------- encoding
+// Spherical coordinates of a unit vector mapped to [0,1]^2.
+__device__ __forceinline__ void sph01(float3 d, float &a, float &b) {
+    a = acosf(fminf(fmaxf(d.y, -1.f), 1.f)) * kInvPi;
+    b = atan2f(d.z, d.x) * (0.5f * kInvPi) + 0.5f;
+}
+// The 7 encoded scalars of a record, all in [0,1]: position in the scene box
+// (3), normal (2), direction to the previous vertex (2).
+struct Enc { float v[7]; };
+__device__ __forceinline__ Enc encode(float3 pos, float3 n, float3 wo, const float box[6]) {
+    Enc e;
+    e.v[0] = fminf(fmaxf((pos.x - box[0]) / fmaxf(box[3] - box[0], 1e-20f), 0.f), 1.f);
+    e.v[1] = fminf(fmaxf((pos.y - box[1]) / fmaxf(box[4] - box[1], 1e-20f), 0.f), 1.f);
+    e.v[2] = fminf(fmaxf((pos.z - box[2]) / fmaxf(box[5] - box[2], 1e-20f), 0.f), 1.f);
+    sph01(n, e.v[3], e.v[4]);
+    sph01(wo, e.v[5], e.v[6]);
+    return e;
+}
+// One-blob feature f of 32 (Mueller et al. 2019, "Neural Importance
+// Sampling"): a Gaussian kernel one bin wide over k bins; position 3 x 6 bins,
+// normal 2 x 4, direction 2 x 3.
+__device__ __forceinline__ float feature(const Enc &e, int f) {
+    // selects rather than an indexed load: f is not always a compile-time constant
+    const float x = f < 6 ? e.v[0] : f < 12 ? e.v[1] : f < 18 ? e.v[2] : f < 22 ? e.v[3] : f < 26 ? e.v[4]
+                  : f < 29 ? e.v[5] : e.v[6];
+    const int k = f < 18 ? 6 : f < 26 ? 4 : 3;
+    const int j = f < 18 ? f % 6 : f < 26 ? (f - 18) % 4 : (f - 26) % 3;
+    const float d = (x - (float(j) + 0.5f) / float(k)) * float(k);
+    return expf(-0.5f * d * d);
+}
+
+// ---------------------------------------------------------------- mixture
+// Decoded lobes live in a row of kStageStride floats (LDS or registers):
+// lobe k at 5k: (log w_k + log norm_k, mu_k xyz, lambda_k), with
+// norm = lambda / (2 pi (1 - exp(-2 lambda))).
+constexpr int kStageStride = 41;  // odd: the 64 rows of a wave fall in distinct banks
+__device__ __forceinline__ float softplus(float x) { return x > 15.f ? x : log1pf(expf(x)); }
+__device__ __forceinline__ float log_vmf_norm(float lam) { return logf(lam) - logf(2.f * kPi) - logf(-expm1f(-2.f * lam)); }
+
+// Raw outputs -> decoded lobes, in place.
+__device__ __forceinline__ void decode(float *o) {
+    float mx = -INFINITY;
+#pragma unroll
+    for (int k = 0; k < kLobes; k++) mx = fmaxf(mx, o[5 * k]);
+    float s = 0.f;
+#pragma unroll
+    for (int k = 0; k < kLobes; k++) s += expf(o[5 * k] - mx);
+    const float lse = mx + logf(s);
+#pragma unroll
+    for (int k = 0; k < kLobes; k++) {
+        const float3 a = f3(o[5 * k + 1], o[5 * k + 2], o[5 * k + 3]);
+        const float n = sqrtf(dot3(a, a));
+        const float3 mu = n > 1e-12f ? a * (1.f / n) : f3(0.f, 1.f, 0.f);
+        const float lam = fminf(fmaxf(softplus(o[5 * k + 4]), kLambdaMin), kLambdaMax);
+        o[5 * k] = o[5 * k] - lse + log_vmf_norm(lam);
+        o[5 * k + 1] = mu.x;
+        o[5 * k + 2] = mu.y;
+        o[5 * k + 3] = mu.z;
+        o[5 * k + 4] = lam;
+    }
+}
+__device__ __forceinline__ float pdf(const float *L, float3 w) {
+    float p = 0.f;
+#pragma unroll
+    for (int k = 0; k < kLobes; k++)
+        p += expf(L[5 * k] + L[5 * k + 4] * (L[5 * k + 1] * w.x + L[5 * k + 2] * w.y + L[5 * k + 3] * w.z - 1.f));
+    return p;
+}
+// Lobe by weight (u0), then the vMF around its axis (Wood 1994, on the 2-sphere):
+// cos = 1 + log(1 - u1 + u1 exp(-2 lambda)) / lambda, azimuth 2 pi u2.
+// The lobe is picked with selects, so L may live in registers.
+__device__ __forceinline__ float3 sample(const float *L, float u0, float u1, float u2) {
+    float3 mu = f3(L[5 * (kLobes - 1) + 1], L[5 * (kLobes - 1) + 2], L[5 * (kLobes - 1) + 3]);
+    float lam = L[5 * (kLobes - 1) + 4];
+    float c = 0.f;
+    bool found = false;
+#pragma unroll
+    for (int k = 0; k < kLobes - 1; k++) {
+        c += expf(L[5 * k] - log_vmf_norm(L[5 * k + 4]));
+        const bool take = !found && u0 < c;
+        mu = take ? f3(L[5 * k + 1], L[5 * k + 2], L[5 * k + 3]) : mu;
+        lam = take ? L[5 * k + 4] : lam;
+        found = found || take;
+    }
+    const float ct = fminf(fmaxf(1.f + logf(1.f - u1 + u1 * expf(-2.f * lam)) / lam, -1.f), 1.f);
+    const float st = sqrtf(fmaxf(0.f, 1.f - ct * ct));
+    float s, co;
+    sincospif(2.f * u2, &s, &co);
+    return normalize3(to_world(make_frame(mu), f3(st * co, ct, st * s)));
+}
+
+// ---------------------------------------------------------------- forward pass
+// Input operands of layer 0 for the wave's 64 records: record r = 32 nt + p is
+// encoded in lane r; lane (p, h) builds its 16 features of records p and 32 + p.
+__device__ __forceinline__ void input_operands(const Enc &mine, bf16x8 x[2][2] /* [ks][nt] */) {
+    const int lane = __lane_id(), p = lane & 31, h = lane >> 5;
+#pragma unroll
+    for (int nt = 0; nt < 2; nt++) {
+        Enc e;
+#pragma unroll
+        for (int j = 0; j < 7; j++) e.v[j] = __shfl(mine.v[j], 32 * nt + p);
+#pragma unroll
+        for (int ks = 0; ks < 2; ks++)
+#pragma unroll
+            for (int i = 0; i < 8; i++) x[ks][nt][i] = bf(feature(e, kperm(ks, h, i)));
+    }
+}
+
+// Output tile mt (32 outputs) of a layer for the wave's 64 records:
+// acc[nt] = b + (W_l^T X^T)[32 mt .. 32 mt + 31][32 nt .. 32 nt + 31]; the
+// accumulators start at the bias (N: outputs of the layer).
+template <int K>
+__device__ __forceinline__ void tile_fwd(const bf16x8 *ops, const float *bias, int N, const bf16x8 (*x)[2], int mt,
+                                         f32x16 acc[2]) {
+    const int lane = __lane_id(), h = lane >> 5;
+#pragma unroll
+    for (int v = 0; v < 16; v++) {
+        const int o = 32 * mt + drow(v, h);
+        acc[0][v] = o < N ? bias[o] : 0.f;
+    }
+    acc[1] = acc[0];
+#pragma unroll
+    for (int ks = 0; ks < K / 16; ks++) {
+        const bf16x8 a = ops[(mt * (K / 16) + ks) * 64 + lane];
+        acc[0] = mfma(a, x[ks][0], acc[0]);
+        acc[1] = mfma(a, x[ks][1], acc[1]);
+    }
+}
+
+// ReLU of output tile mt into operands 2 mt, 2 mt + 1 of the next layer:
+// xn[ks][nt] element e = relu(acc[nt][8 (ks % 2) + e]).
+__device__ __forceinline__ void tile_to_operands(const f32x16 acc[2], int mt, bf16x8 xn[4][2]) {
+#pragma unroll
+    for (int kk = 0; kk < 2; kk++)
+#pragma unroll
+        for (int e = 0; e < 8; e++)
+#pragma unroll
+            for (int nt = 0; nt < 2; nt++) xn[2 * mt + kk][nt][e] = bf(fmaxf(acc[nt][8 * kk + e], 0.f));
+}
+
+__device__ __forceinline__ void wave_sync_lds() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Forward pass of the wave's 64 records (record r is the one encoded in lane
+// r).  Raw outputs of record r land in stage[r * kStageStride + o], o < 40.
+// keep(l, ks, nt, x) sees every layer's input operands (for the backward
+// pass).  Must be called by all 64 lanes of the wave.
+struct NoKeep {
+    __device__ void operator()(int, int, int, const bf16x8 &) const {}
+};
+template <typename Keep = NoKeep>
+__device__ __forceinline__ void forward(const Net &net, const Enc &mine, float *stage, const Keep &keep = Keep()) {
+    const int lane = __lane_id(), p = lane & 31, h = lane >> 5;
+    bf16x8 x[4][2];
+    {
+        bf16x8 x0[2][2];
+        input_operands(mine, x0);
+#pragma unroll
+        for (int ks = 0; ks < 2; ks++)
+#pragma unroll
+            for (int nt = 0; nt < 2; nt++) keep(0, ks, nt, x0[ks][nt]);
+#pragma unroll
+        for (int mt = 0; mt < 2; mt++) {
+            f32x16 acc[2];
+            tile_fwd<kIn>(net.fwd + fwd_offset(0), net.params + b_offset(0), kHid, x0, mt, acc);
+            tile_to_operands(acc, mt, x);
+        }
+    }
+    // hidden layers 1..3 (same shape): rolled, so one layer's operands are live at a time
+#pragma unroll 1
+    for (int l = 1; l < kLayers - 1; l++) {
+#pragma unroll
+        for (int ks = 0; ks < 4; ks++)
+#pragma unroll
+            for (int nt = 0; nt < 2; nt++) keep(l, ks, nt, x[ks][nt]);
+        bf16x8 xn[4][2];
+#pragma unroll
+        for (int mt = 0; mt < 2; mt++) {
+            f32x16 acc[2];
+            tile_fwd<kHid>(net.fwd + fwd_offset(l), net.params + b_offset(l), kHid, x, mt, acc);
+            tile_to_operands(acc, mt, xn);
+        }
+#pragma unroll
+        for (int ks = 0; ks < 4; ks++)
+#pragma unroll
+            for (int nt = 0; nt < 2; nt++) x[ks][nt] = xn[ks][nt];
+    }
+#pragma unroll
+    for (int ks = 0; ks < 4; ks++)
+#pragma unroll
+        for (int nt = 0; nt < 2; nt++) keep(kLayers - 1, ks, nt, x[ks][nt]);
+#pragma unroll
+    for (int mt = 0; mt < 2; mt++) {
+        f32x16 acc[2];
+        tile_fwd<kHid>(net.fwd + fwd_offset(kLayers - 1), net.params + b_offset(kLayers - 1), kOut, x, mt, acc);
+#pragma unroll
+        for (int v = 0; v < 16; v++) {
+            const int o = 32 * mt + drow(v, h);
+            if (o < kOut) {
+#pragma unroll
+                for (int nt = 0; nt < 2; nt++) stage[(32 * nt + p) * kStageStride + o] = acc[nt][v];
+            }
+        }
+    }
+    wave_sync_lds();
+}
+
+// ---------------------------------------------------------------- loss gradient
+// One record: loss = -w log q(wi).  Writes dloss/dout[40] to g (which may be
+// o itself) and returns the loss.  Computed in log space: lobes underflow far
+// from their axis.
+__device__ __forceinline__ float loss_grad(const float *o, float3 wi, float w, float *g) {
+    float mx = -INFINITY;
+    for (int k = 0; k < kLobes; k++) mx = fmaxf(mx, o[5 * k]);
+    float s = 0.f;
+    for (int k = 0; k < kLobes; k++) s += expf(o[5 * k] - mx);
+    const float lse = mx + logf(s);
+    float lc[kLobes], lam[kLobes], nrm[kLobes], dl[kLobes], cs[kLobes];
+    float3 mu[kLobes];
+    float lq_max = -INFINITY;
+    for (int k = 0; k < kLobes; k++) {
+        const float3 a = f3(o[5 * k + 1], o[5 * k + 2], o[5 * k + 3]);
+        nrm[k] = sqrtf(dot3(a, a));
+        mu[k] = nrm[k] > 1e-12f ? a * (1.f / nrm[k]) : f3(0.f, 1.f, 0.f);
+        const float raw = o[5 * k + 4];
+        const float sp = softplus(raw);
+        lam[k] = fminf(fmaxf(sp, kLambdaMin), kLambdaMax);
+        dl[k] = (sp > kLambdaMin && sp < kLambdaMax) ? 1.f / (1.f + expf(-raw)) : 0.f;  // dlambda/draw
+        cs[k] = dot3(mu[k], wi);
+        lc[k] = (o[5 * k] - lse) + log_vmf_norm(lam[k]) + lam[k] * (cs[k] - 1.f);
+        lq_max = fmaxf(lq_max, lc[k]);
+    }
+    float sq = 0.f;
+    for (int k = 0; k < kLobes; k++) sq += expf(lc[k] - lq_max);
+    const float logq = lq_max + logf(sq);
+    float wk[kLobes];
+    for (int k = 0; k < kLobes; k++) wk[k] = expf(o[5 * k] - lse);
+    for (int k = 0; k < kLobes; k++) {
+        const float gam = expf(lc[k] - logq);  // responsibility of lobe k
+        g[5 * k] = -w * (gam - wk[k]);
+        // d log c / d lambda = 1/lambda - 2 / (exp(2 lambda) - 1) + (mu.wi - 1)
+        const float dlogc = 1.f / lam[k] - 2.f / expm1f(2.f * lam[k]) + (cs[k] - 1.f);
+        g[5 * k + 4] = -w * gam * dlogc * dl[k];
+        // d log c / d mu = lambda wi, through mu = a / |a|
+        float3 ga = f3(0.f, 0.f, 0.f);
+        if (nrm[k] > 1e-12f) {
+            const float3 gm = wi * (gam * lam[k]);
+            ga = (gm - mu[k] * dot3(mu[k], gm)) * (1.f / nrm[k]);
+        }
+        g[5 * k + 1] = -w * ga.x;
+        g[5 * k + 2] = -w * ga.y;
+        g[5 * k + 3] = -w * ga.z;
+    }
+    return -w * logq;
+}
+
+}  // namespace guide
+}  // namespace wr

@@ -1,0 +1,708 @@
+// wr_guide.hip -- training and test kernels of the path-guiding network, and
+// its C-ABI (no reference counterpart; the reference has no guiding).
+//
+//   pack      FP32 master weights -> BF16 MFMA operands (forward and backward)
+//   train     per wave of 64 records: forward (MFMA), loss gradient, backward
+//             data (MFMA) and weight gradients (MFMA over the records), summed
+//             per block in LDS and written as one partial per block
+//   adam      sums the block partials and applies Adam to the FP32 master
+//   finalize  training weight of each record from its path's final radiance
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "wr_guide.h"
+#include "wr_host.h"
+
+using namespace wr;
+using namespace wr::guide;
+
+namespace {
+
+constexpr int kXt = 72;                  // LDS row stride (bf16) of the transposed tiles
+constexpr int kMaxTrainBlocks = 256;
+constexpr uint32_t kTestBlock = 64;
+
+// ---------------------------------------------------------------- pack
+__global__ void k_pack(const float *params, bf16x8 *fwd, bf16x8 *bwd) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < kFwdOps) {
+        int l = 0;
+        while (l + 1 < kLayers && j >= fwd_offset(l + 1)) l++;
+        const int K = layer_in(l), N = layer_out(l), nks = K / 16;
+        const int jj = j - fwd_offset(l);
+        const int mt = jj / (nks * 64), ks = (jj / 64) % nks, lane = jj % 64;
+        const int o = 32 * mt + (lane & 31), h = lane >> 5;
+        bf16x8 v;
+        for (int e = 0; e < 8; e++) v[e] = bf(o < N ? params[w_offset(l) + kperm(ks, h, e) * N + o] : 0.f);
+        fwd[j] = v;
+    } else if (j < kFwdOps + kBwdOps) {
+        const int jb = j - kFwdOps;
+        int l = 1;
+        while (l + 1 < kLayers && jb >= bwd_offset(l + 1)) l++;
+        const int N = layer_out(l);
+        const int jj = jb - bwd_offset(l);
+        const int mt = jj / (4 * 64), ks = (jj / 64) % 4, lane = jj % 64;
+        const int i = 32 * mt + (lane & 31), h = lane >> 5;
+        bf16x8 v;
+        for (int e = 0; e < 8; e++) {
+            const int o = kperm(ks, h, e);
+            v[e] = bf(o < N ? params[w_offset(l) + i * N + o] : 0.f);
+        }
+        bwd[jb] = v;
+    }
+}
+
+// ---------------------------------------------------------------- train
+struct TrainArgs {
+    const float4 *pos, *nrm, *wo, *wi;  // records; wi.w = weight
+    uint32_t n;                         // records available
+    uint32_t start, batch;              // this step uses records (start + j) % n, j < batch
+    float wscale;                       // loss = sum_j wscale * w_j * (-log q_j)
+    float *partials;                    // [gridDim.x][kNumParams]
+    float *loss;                        // [gridDim.x]
+};
+
+struct KeepXt {
+    __bf16 *xt;  // [kLayers][64][kXt]
+    __device__ void operator()(int l, int ks, int nt, const bf16x8 &x) const {
+        const int lane = __lane_id(), p = lane & 31, h = lane >> 5;
+        __bf16 *t = xt + l * 64 * kXt;
+#pragma unroll
+        for (int e = 0; e < 8; e++) t[kperm(ks, h, e) * kXt + 32 * nt + p] = x[e];
+    }
+};
+
+__global__ __launch_bounds__(64) void k_train(Net net, TrainArgs a) {
+    __shared__ __attribute__((aligned(16))) __bf16 xt[kLayers * 64 * kXt];
+    __shared__ __attribute__((aligned(16))) __bf16 dyt[64 * kXt];
+    __shared__ float stage[64 * kStageStride];
+    __shared__ float gacc[kNumParams];
+    const int lane = threadIdx.x, p = lane & 31, h = lane >> 5;
+    for (int j = lane; j < kNumParams; j += 64) gacc[j] = 0.f;
+    float loss = 0.f;
+    const uint32_t tiles = (a.batch + 63) / 64;
+    for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+        const uint32_t jr = tile * 64 + lane;
+        const bool valid = jr < a.batch;
+        Enc enc;
+        float3 wi = f3(0.f, 1.f, 0.f);
+        float w = 0.f;
+        if (valid) {
+            const uint32_t r = (a.start + jr) % a.n;
+            enc = encode(xyz(a.pos[r]), xyz(a.nrm[r]), xyz(a.wo[r]), net.box);
+            const float4 d = a.wi[r];
+            wi = xyz(d);
+            w = d.w * a.wscale;
+        } else {
+            for (int k = 0; k < 7; k++) enc.v[k] = 0.f;
+        }
+        wave_sync_lds();
+        forward(net, enc, stage, KeepXt{xt});
+        // loss gradient of this lane's record, written over its outputs
+        {
+            float *row = stage + lane * kStageStride;
+            const float l = loss_grad(row, wi, w, row);
+            if (valid) loss += l;
+            else
+                for (int k = 0; k < kOut; k++) row[k] = 0.f;
+        }
+        wave_sync_lds();
+        f32x16 dacc[2][2];
+#pragma unroll
+        for (int mt = 0; mt < 2; mt++)
+#pragma unroll
+            for (int nt = 0; nt < 2; nt++)
+#pragma unroll
+                for (int v = 0; v < 16; v++) {
+                    const int o = 32 * mt + drow(v, h);
+                    dacc[mt][nt][v] = o < kOut ? stage[(32 * nt + p) * kStageStride + o] : 0.f;
+                }
+#pragma unroll 1
+        for (int l = kLayers - 1; l >= 0; l--) {
+            const int K = layer_in(l), N = layer_out(l);
+            // dY_l^T as BF16, feature-major (the weight-gradient operand)
+#pragma unroll
+            for (int mt = 0; mt < 2; mt++)
+#pragma unroll
+                for (int nt = 0; nt < 2; nt++)
+#pragma unroll
+                    for (int v = 0; v < 16; v++) dyt[(32 * mt + drow(v, h)) * kXt + 32 * nt + p] = bf(dacc[mt][nt][v]);
+            wave_sync_lds();
+            // bias gradient: row sums of dY^T
+            if (lane < N) {
+                float s = 0.f;
+                for (int r = 0; r < 64; r++) s += fb(dyt[lane * kXt + r]);
+                gacc[b_offset(l) + lane] += s;
+            }
+            // weight gradient dW[i][o] = sum_r X[r][i] dY[r][o]: MFMA over the records
+            const __bf16 *x = xt + l * 64 * kXt;
+#pragma unroll
+            for (int mt = 0; mt < 2; mt++)
+#pragma unroll
+                for (int nt = 0; nt < 2; nt++) {
+                    if (mt >= K / 32) continue;
+                    f32x16 acc = f32x16{};
+#pragma unroll
+                    for (int ks = 0; ks < 4; ks++) {
+                        const bf16x8 av = *reinterpret_cast<const bf16x8 *>(x + (32 * mt + p) * kXt + 16 * ks + 8 * h);
+                        const bf16x8 bv = *reinterpret_cast<const bf16x8 *>(dyt + (32 * nt + p) * kXt + 16 * ks + 8 * h);
+                        acc = mfma(av, bv, acc);
+                    }
+                    const int o = 32 * nt + p;
+                    if (o < N) {
+#pragma unroll
+                        for (int v = 0; v < 16; v++) gacc[w_offset(l) + (32 * mt + drow(v, h)) * N + o] += acc[v];
+                    }
+                }
+            if (l > 0) {
+                // backward data: dX^T = W_l dY^T, operands straight from dacc (same
+                // BF16 values as dyt), then the ReLU mask of the layer's input
+                bf16x8 b[4][2];
+#pragma unroll
+                for (int ks = 0; ks < 4; ks++)
+#pragma unroll
+                    for (int nt = 0; nt < 2; nt++)
+#pragma unroll
+                        for (int e = 0; e < 8; e++) b[ks][nt][e] = bf(dacc[ks >> 1][nt][8 * (ks & 1) + e]);
+                const bf16x8 *ops = net.bwd + bwd_offset(l);
+#pragma unroll
+                for (int mt = 0; mt < 2; mt++) {
+                    f32x16 d0 = f32x16{}, d1 = f32x16{};
+#pragma unroll
+                    for (int ks = 0; ks < 4; ks++) {
+                        const bf16x8 av = ops[(mt * 4 + ks) * 64 + lane];
+                        d0 = mfma(av, b[ks][0], d0);
+                        d1 = mfma(av, b[ks][1], d1);
+                    }
+#pragma unroll
+                    for (int v = 0; v < 16; v++) {
+                        const int i = 32 * mt + drow(v, h);
+                        d0[v] = fb(x[i * kXt + p]) > 0.f ? d0[v] : 0.f;
+                        d1[v] = fb(x[i * kXt + 32 + p]) > 0.f ? d1[v] : 0.f;
+                    }
+                    dacc[mt][0] = d0;
+                    dacc[mt][1] = d1;
+                }
+            }
+            wave_sync_lds();
+        }
+    }
+    wave_sync_lds();
+    float *dst = a.partials + size_t(blockIdx.x) * kNumParams;
+    for (int j = lane; j < kNumParams; j += 64) dst[j] = gacc[j];
+    for (int off = 32; off > 0; off >>= 1) loss += __shfl_xor(loss, off);
+    if (lane == 0) a.loss[blockIdx.x] = loss;
+}
+
+// ---------------------------------------------------------------- adam
+struct AdamArgs {
+    float *params, *m, *v;
+    const float *partials;
+    int blocks;
+    float lr_t;  // lr * sqrt(1 - beta2^t) / (1 - beta1^t)
+    float beta1, beta2, eps_t;  // eps * sqrt(1 - beta2^t)
+    float *grad_out;            // non-null: store the summed gradient, no update
+};
+__global__ void k_adam(AdamArgs a) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= kNumParams) return;
+    float g = 0.f;
+    for (int b = 0; b < a.blocks; b++) g += a.partials[size_t(b) * kNumParams + j];
+    if (a.grad_out) {
+        a.grad_out[j] = g;
+        return;
+    }
+    const float m = a.beta1 * a.m[j] + (1.f - a.beta1) * g;
+    const float v = a.beta2 * a.v[j] + (1.f - a.beta2) * g * g;
+    a.m[j] = m;
+    a.v[j] = v;
+    a.params[j] -= a.lr_t * m / (sqrtf(v) + a.eps_t);
+}
+
+// ---------------------------------------------------------------- finalize
+// weight = mean over channels of (final radiance - radiance before the
+// successor vertex) / throughput before the vertex = f cos L_i / p(wi).
+__global__ void k_finalize(Records rec, uint32_t n, const float4 *sample_rad, double *wsum) {
+    float s = 0.f;
+    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+        const uint32_t sidx = __float_as_uint(rec.pos[r].w);
+        const float4 fin = sample_rad[sidx];
+        const float4 base = rec.rad[r];
+        const float4 t = rec.thr[r];
+        float w = 0.f;
+        w += t.x > 0.f ? fmaxf(fin.x - base.x, 0.f) / t.x : 0.f;
+        w += t.y > 0.f ? fmaxf(fin.y - base.y, 0.f) / t.y : 0.f;
+        w += t.z > 0.f ? fmaxf(fin.z - base.z, 0.f) / t.z : 0.f;
+        w *= 1.f / 3.f;
+        if (!(w < 3.0e38f)) w = 0.f;
+        rec.wi[r].w = w;
+        s += w;
+    }
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if (__lane_id() == 0 && s != 0.f) atomicAdd(wsum, double(s));
+}
+
+// ---------------------------------------------------------------- tests
+__global__ __launch_bounds__(kTestBlock) void k_forward_test(Net net, const float *pos, const float *nrm, const float *wo,
+                                                             uint32_t n, float *out) {
+    __shared__ float stage[64 * kStageStride];
+    const uint32_t r = blockIdx.x * 64 + threadIdx.x;
+    Enc e;
+    if (r < n)
+        e = encode(f3(pos[3 * r], pos[3 * r + 1], pos[3 * r + 2]), f3(nrm[3 * r], nrm[3 * r + 1], nrm[3 * r + 2]),
+                   f3(wo[3 * r], wo[3 * r + 1], wo[3 * r + 2]), net.box);
+    else
+        for (int k = 0; k < 7; k++) e.v[k] = 0.f;
+    forward(net, e, stage);
+    if (r < n)
+        for (int k = 0; k < kOut; k++) out[size_t(r) * kOut + k] = stage[threadIdx.x * kStageStride + k];
+}
+
+__global__ __launch_bounds__(kTestBlock) void k_mixture_test(const float *raw, const float *dirs, const float *u, uint32_t n,
+                                                             float *pdf_dirs, float *samples, float *sample_pdf) {
+    __shared__ float stage[64 * kStageStride];
+    const uint32_t r = blockIdx.x * 64 + threadIdx.x;
+    if (r >= n) return;
+    float *L = stage + threadIdx.x * kStageStride;
+    for (int k = 0; k < kOut; k++) L[k] = raw[size_t(r) * kOut + k];
+    decode(L);
+    pdf_dirs[r] = pdf(L, f3(dirs[3 * r], dirs[3 * r + 1], dirs[3 * r + 2]));
+    const float3 s = sample(L, u[3 * r], u[3 * r + 1], u[3 * r + 2]);
+    samples[3 * r] = s.x;
+    samples[3 * r + 1] = s.y;
+    samples[3 * r + 2] = s.z;
+    sample_pdf[r] = pdf(L, s);
+}
+
+}  // namespace
+
+// ============================================================== host
+struct wr_guide {
+    wr_scene *scene = nullptr;
+    wr_guide_config cfg{};
+    float *d_params = nullptr, *d_m = nullptr, *d_v = nullptr;
+    bf16x8 *d_fwd = nullptr, *d_bwd = nullptr;
+    float *d_partials = nullptr, *d_loss = nullptr;
+    double *d_wsum = nullptr;
+    Records rec{};
+    void *rec_pool = nullptr;
+    float4 *d_lobes = nullptr;  // decoded lobes of the path pool (kLobeVecs float4 per entry, SoA)
+    uint32_t lobe_capacity = 0;
+    uint64_t step = 0;
+    int train_blocks = 0;
+    double last_loss = 0.0;
+    // adaptive record selection: recorded vertices per selected path
+    double verts_per_path = 4.0;
+};
+
+namespace {
+
+// splitmix64, for the weight initialisation
+struct HostRng {
+    uint64_t s;
+    uint64_t next() {
+        uint64_t z = (s += 0x9e3779b97f4a7c15ull);
+        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+        z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+        return z ^ (z >> 31);
+    }
+    double uniform() { return double(next() >> 11) * 0x1p-53; }
+};
+
+// He-uniform hidden layers; a small output layer whose biases start the
+// mixture as 8 equal lobes of sharpness 2 along the cube diagonals.
+void init_params(uint32_t seed, std::vector<float> &p) {
+    p.assign(kNumParams, 0.f);
+    HostRng rng{0x243f6a8885a308d3ull ^ seed};
+    for (int l = 0; l < kLayers; l++) {
+        const int K = layer_in(l), N = layer_out(l);
+        const double lim = std::sqrt(6.0 / K) * (l == kLayers - 1 ? 0.1 : 1.0);
+        for (int j = 0; j < K * N; j++) p[w_offset(l) + j] = float((2.0 * rng.uniform() - 1.0) * lim);
+    }
+    float *b = p.data() + b_offset(kLayers - 1);
+    for (int k = 0; k < kLobes; k++) {
+        b[5 * k + 0] = 0.f;
+        b[5 * k + 1] = (k & 1) ? 1.f : -1.f;
+        b[5 * k + 2] = (k & 2) ? 1.f : -1.f;
+        b[5 * k + 3] = (k & 4) ? 1.f : -1.f;
+        b[5 * k + 4] = float(std::log(std::expm1(2.0)));  // softplus^-1(2)
+    }
+}
+
+Net net_view(const wr_guide *g) {
+    Net n;
+    n.fwd = g->d_fwd;
+    n.bwd = g->d_bwd;
+    n.params = g->d_params;
+    for (int k = 0; k < 6; k++) n.box[k] = g->scene->box[k];
+    return n;
+}
+
+int pack(wr_guide *g) {
+    const int n = kFwdOps + kBwdOps;
+    k_pack<<<(n + 255) / 256, 256, 0, g->scene->stream>>>(g->d_params, g->d_fwd, g->d_bwd);
+    WR_HIP(hipGetLastError());
+    return WR_OK;
+}
+
+int reset_adam(wr_guide *g) {
+    WR_HIP(hipMemsetAsync(g->d_m, 0, sizeof(float) * kNumParams, g->scene->stream));
+    WR_HIP(hipMemsetAsync(g->d_v, 0, sizeof(float) * kNumParams, g->scene->stream));
+    g->step = 0;
+    return WR_OK;
+}
+
+// One training step (or, with grad_out, the gradient only) over `batch`
+// records starting at `start` of the n records in recs.
+int train_step(wr_guide *g, const float4 *pos, const float4 *nrm, const float4 *wo, const float4 *wi, uint32_t n,
+               uint32_t start, uint32_t batch, float wscale, float *grad_out, double *loss) {
+    hipStream_t st = g->scene->stream;
+    TrainArgs a;
+    a.pos = pos; a.nrm = nrm; a.wo = wo; a.wi = wi;
+    a.n = n; a.start = start; a.batch = batch; a.wscale = wscale;
+    a.partials = g->d_partials;
+    a.loss = g->d_loss;
+    const int tiles = int((batch + 63) / 64);
+    const int blocks = std::max(1, std::min(tiles, g->train_blocks));
+    k_train<<<blocks, 64, 0, st>>>(net_view(g), a);
+    WR_HIP(hipGetLastError());
+    AdamArgs ad;
+    ad.params = g->d_params; ad.m = g->d_m; ad.v = g->d_v;
+    ad.partials = g->d_partials;
+    ad.blocks = blocks;
+    ad.grad_out = grad_out;
+    if (!grad_out) {
+        g->step++;
+        const double t = double(g->step);
+        const double c1 = 1.0 - std::pow(double(g->cfg.beta1), t), c2 = 1.0 - std::pow(double(g->cfg.beta2), t);
+        ad.lr_t = float(g->cfg.learning_rate * std::sqrt(c2) / c1);
+        ad.eps_t = float(g->cfg.epsilon * std::sqrt(c2));
+    } else {
+        ad.lr_t = 0.f;
+        ad.eps_t = 1.f;
+    }
+    ad.beta1 = g->cfg.beta1;
+    ad.beta2 = g->cfg.beta2;
+    k_adam<<<(kNumParams + 255) / 256, 256, 0, st>>>(ad);
+    WR_HIP(hipGetLastError());
+    if (!grad_out) {
+        int rc = pack(g);
+        if (rc) return rc;
+    }
+    if (loss) {
+        std::vector<float> l(blocks);
+        WR_HIP(hipMemcpyAsync(l.data(), g->d_loss, sizeof(float) * blocks, hipMemcpyDeviceToHost, st));
+        WR_HIP(hipStreamSynchronize(st));
+        double s = 0.0;
+        for (float x : l) s += x;
+        *loss = s;
+    }
+    return WR_OK;
+}
+
+// Host records -> temporary device arrays (float4 each).
+struct DeviceRecords {
+    float4 *buf = nullptr;
+    float4 *pos = nullptr, *nrm = nullptr, *wo = nullptr, *wi = nullptr;
+    ~DeviceRecords() { if (buf) (void)hipFree(buf); }
+    int upload(hipStream_t st, const float *p, const float *nr, const float *o, const float *i, const float *w, uint32_t n) {
+        std::vector<float4> h(4 * size_t(n));
+        for (uint32_t r = 0; r < n; r++) {
+            h[r] = make_float4(p[3 * r], p[3 * r + 1], p[3 * r + 2], 0.f);
+            h[n + r] = make_float4(nr[3 * r], nr[3 * r + 1], nr[3 * r + 2], 0.f);
+            h[2 * size_t(n) + r] = make_float4(o[3 * r], o[3 * r + 1], o[3 * r + 2], 0.f);
+            h[3 * size_t(n) + r] = make_float4(i[3 * r], i[3 * r + 1], i[3 * r + 2], w[r]);
+        }
+        WR_HIP(hipMalloc(reinterpret_cast<void **>(&buf), sizeof(float4) * h.size()));
+        WR_HIP(hipMemcpyAsync(buf, h.data(), sizeof(float4) * h.size(), hipMemcpyHostToDevice, st));
+        WR_HIP(hipStreamSynchronize(st));
+        pos = buf; nrm = buf + n; wo = buf + 2 * size_t(n); wi = buf + 3 * size_t(n);
+        return WR_OK;
+    }
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------- internal interface for the renderer
+int guide_shade_view(wr_guide *g, float alpha, float record_prob, bool collect, guide::Shade *out) {
+    out->lobes = g->d_lobes;
+    out->lobe_stride = g->lobe_capacity;
+    out->rec = g->rec;
+    out->alpha = alpha;
+    out->record_prob = record_prob;
+    out->collect = collect ? 1 : 0;
+    return WR_OK;
+}
+
+int guide_begin_pass(wr_guide *g, uint64_t samples, float *record_prob) {
+    WR_HIP(hipMemsetAsync(g->rec.count, 0, sizeof(uint32_t), g->scene->stream));
+    const double want = 0.9 * double(g->rec.capacity);
+    const double expect = double(samples) * g->verts_per_path;
+    *record_prob = float(std::min(1.0, want / std::max(expect, 1.0)));
+    return WR_OK;
+}
+
+// After a pass: weights from the final radiance, then `steps` Adam steps.
+int guide_end_pass(wr_guide *g, const float4 *sample_rad, uint64_t samples, float record_prob, uint32_t steps,
+                   wr_guide_stats *gs) {
+    hipStream_t st = g->scene->stream;
+    uint32_t count = 0;
+    WR_HIP(hipMemcpyAsync(&count, g->rec.count, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    WR_HIP(hipStreamSynchronize(st));
+    const double selected = double(samples) * double(record_prob);
+    if (selected >= 1.0 && count > 0) g->verts_per_path = std::max(1.0, double(count) / selected);
+    const uint32_t n = std::min(count, g->rec.capacity);
+    gs->records += n;
+    if (n == 0 || steps == 0) return WR_OK;
+    hipEvent_t e0, e1;
+    WR_HIP(hipEventCreate(&e0));
+    WR_HIP(hipEventCreate(&e1));
+    WR_HIP(hipEventRecord(e0, st));
+    WR_HIP(hipMemsetAsync(g->d_wsum, 0, sizeof(double), st));
+    k_finalize<<<std::min<uint32_t>((n + 255) / 256, 4096), 256, 0, st>>>(g->rec, n, sample_rad, g->d_wsum);
+    WR_HIP(hipGetLastError());
+    double wsum = 0.0;
+    WR_HIP(hipMemcpyAsync(&wsum, g->d_wsum, sizeof(double), hipMemcpyDeviceToHost, st));
+    WR_HIP(hipStreamSynchronize(st));
+    if (wsum > 0.0) {
+        const uint32_t batch = std::min(n, g->cfg.batch_size);
+        // mean weight 1 over the pass's records; loss averaged over the batch
+        const float wscale = float(double(n) / wsum / double(batch));
+        for (uint32_t k = 0; k < steps; k++) {
+            const uint32_t start = uint32_t((uint64_t(k) * batch) % n);
+            double loss = 0.0;
+            int rc = train_step(g, g->rec.pos, g->rec.nrm, g->rec.wo, g->rec.wi, n, start, batch, wscale, nullptr,
+                                k + 1 == steps ? &loss : nullptr);
+            if (rc) return rc;
+            gs->train_steps++;
+            if (k + 1 == steps) gs->last_loss = loss;
+        }
+    }
+    WR_HIP(hipEventRecord(e1, st));
+    WR_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    gs->train_ms += ms;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return WR_OK;
+}
+
+guide::Net guide_net(const wr_guide *g) { return net_view(g); }
+
+int guide_reserve_lobes(wr_guide *g, uint32_t capacity) {
+    if (g->lobe_capacity >= capacity) return WR_OK;
+    WR_HIP(hipStreamSynchronize(g->scene->stream));
+    if (g->d_lobes) WR_HIP(hipFree(g->d_lobes));
+    g->d_lobes = nullptr;
+    g->lobe_capacity = 0;
+    WR_HIP(hipMalloc(reinterpret_cast<void **>(&g->d_lobes), sizeof(float4) * kLobeVecs * size_t(capacity)));
+    g->lobe_capacity = capacity;
+    return WR_OK;
+}
+wr_scene *guide_scene(const wr_guide *g) { return g->scene; }
+
+// ---------------------------------------------------------------- C-ABI
+void wr_guide_config_default(wr_guide_config *c) {
+    if (!c) return;
+    c->learning_rate = 1e-2f;
+    c->beta1 = 0.9f;
+    c->beta2 = 0.999f;
+    c->epsilon = 1e-8f;
+    c->batch_size = 1u << 18;
+    c->record_capacity = 1u << 20;
+    c->seed = 0;
+}
+
+void wr_guide_destroy(wr_guide *g) {
+    if (!g) return;
+    (void)hipSetDevice(g->scene->dev->ordinal);
+    (void)hipStreamSynchronize(g->scene->stream);
+    for (void *p : {(void *)g->d_params, (void *)g->d_m, (void *)g->d_v, (void *)g->d_fwd, (void *)g->d_bwd,
+                    (void *)g->d_partials, (void *)g->d_loss, (void *)g->d_wsum, g->rec_pool, (void *)g->d_lobes})
+        if (p) (void)hipFree(p);
+    delete g;
+}
+
+static int guide_create_impl(wr_guide *g) {
+    int rc = set_device(g->scene->dev);
+    if (rc) return rc;
+    int ndev_blocks = g->scene->dev->n_cu;
+    g->train_blocks = std::min(kMaxTrainBlocks, ndev_blocks);
+    WR_HIP(hipMalloc(reinterpret_cast<void **>(&g->d_params), sizeof(float) * kNumParams));
+    WR_HIP(hipMalloc(reinterpret_cast<void **>(&g->d_m), sizeof(float) * kNumParams));
+    WR_HIP(hipMalloc(reinterpret_cast<void **>(&g->d_v), sizeof(float) * kNumParams));
+    WR_HIP(hipMalloc(reinterpret_cast<void **>(&g->d_fwd), sizeof(bf16x8) * kFwdOps));
+    WR_HIP(hipMalloc(reinterpret_cast<void **>(&g->d_bwd), sizeof(bf16x8) * kBwdOps));
+    WR_HIP(hipMalloc(reinterpret_cast<void **>(&g->d_partials), sizeof(float) * kNumParams * size_t(g->train_blocks)));
+    WR_HIP(hipMalloc(reinterpret_cast<void **>(&g->d_loss), sizeof(float) * g->train_blocks));
+    WR_HIP(hipMalloc(reinterpret_cast<void **>(&g->d_wsum), sizeof(double)));
+    const size_t cap = g->cfg.record_capacity;
+    WR_HIP(hipMalloc(&g->rec_pool, sizeof(float4) * 6 * cap + 256));
+    float4 *b = reinterpret_cast<float4 *>(g->rec_pool);
+    g->rec.pos = b;
+    g->rec.nrm = b + cap;
+    g->rec.wo = b + 2 * cap;
+    g->rec.wi = b + 3 * cap;
+    g->rec.thr = b + 4 * cap;
+    g->rec.rad = b + 5 * cap;
+    g->rec.count = reinterpret_cast<uint32_t *>(b + 6 * cap);
+    g->rec.capacity = uint32_t(cap);
+    std::vector<float> p;
+    init_params(g->cfg.seed, p);
+    return wr_guide_set_params(g, p.data());
+}
+
+int wr_guide_create(wr_scene *s, const wr_guide_config *cfg, wr_guide **out) {
+    if (!s || !out) return set_error(WR_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    wr_guide_config c;
+    wr_guide_config_default(&c);
+    if (cfg) c = *cfg;
+    if (c.batch_size == 0 || c.record_capacity == 0 || c.record_capacity > (1u << 26))
+        return set_error(WR_ERR_INVALID_ARGUMENT, "batch_size must be > 0 and record_capacity in [1, 2^26]");
+    if (!(c.learning_rate > 0.f) || !(c.beta1 >= 0.f && c.beta1 < 1.f) || !(c.beta2 >= 0.f && c.beta2 < 1.f) ||
+        !(c.epsilon > 0.f))
+        return set_error(WR_ERR_INVALID_ARGUMENT, "bad Adam hyper-parameters");
+    wr_guide *g = new wr_guide;
+    g->scene = s;
+    g->cfg = c;
+    int rc = guide_create_impl(g);
+    if (rc) {
+        std::string msg = wr::error_string();
+        wr_guide_destroy(g);
+        wr::error_string() = msg;
+        return rc;
+    }
+    *out = g;
+    return WR_OK;
+}
+
+int wr_guide_get_params(wr_guide *g, float *params) {
+    if (!g || !params) return set_error(WR_ERR_INVALID_ARGUMENT, "null argument");
+    int rc = set_device(g->scene->dev);
+    if (rc) return rc;
+    WR_HIP(hipMemcpyAsync(params, g->d_params, sizeof(float) * kNumParams, hipMemcpyDeviceToHost, g->scene->stream));
+    WR_HIP(hipStreamSynchronize(g->scene->stream));
+    return WR_OK;
+}
+
+int wr_guide_set_params(wr_guide *g, const float *params) {
+    if (!g || !params) return set_error(WR_ERR_INVALID_ARGUMENT, "null argument");
+    int rc = set_device(g->scene->dev);
+    if (rc) return rc;
+    for (int j = 0; j < kNumParams; j++)
+        if (!std::isfinite(params[j])) return set_error(WR_ERR_INVALID_ARGUMENT, "non-finite parameter");
+    WR_HIP(hipMemcpyAsync(g->d_params, params, sizeof(float) * kNumParams, hipMemcpyHostToDevice, g->scene->stream));
+    rc = reset_adam(g);
+    if (!rc) rc = pack(g);
+    if (rc) return rc;
+    WR_HIP(hipStreamSynchronize(g->scene->stream));
+    return WR_OK;
+}
+
+static int check_records(const float *pos, const float *nrm, const float *wo, uint32_t n) {
+    if (n > 0 && (!pos || !nrm || !wo)) return set_error(WR_ERR_INVALID_ARGUMENT, "null record arrays");
+    if (n > (1u << 26)) return set_error(WR_ERR_UNSUPPORTED, "too many records (limit 2^26)");
+    return WR_OK;
+}
+
+int wr_guide_forward(wr_guide *g, const float *pos, const float *nrm, const float *wo, uint32_t n, float *out) {
+    if (!g || (n && !out)) return set_error(WR_ERR_INVALID_ARGUMENT, "null argument");
+    int rc = check_records(pos, nrm, wo, n);
+    if (!rc) rc = set_device(g->scene->dev);
+    if (rc || n == 0) return rc;
+    hipStream_t st = g->scene->stream;
+    float *d = nullptr;
+    const size_t in_bytes = sizeof(float) * 9 * size_t(n), out_bytes = sizeof(float) * kOut * size_t(n);
+    WR_HIP(hipMalloc(reinterpret_cast<void **>(&d), in_bytes + out_bytes));
+    float *dp = d, *dn = d + 3 * size_t(n), *dw = d + 6 * size_t(n), *dout = d + 9 * size_t(n);
+    hipError_t e = hipMemcpyAsync(dp, pos, in_bytes / 3, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(dn, nrm, in_bytes / 3, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(dw, wo, in_bytes / 3, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) {
+        k_forward_test<<<(n + 63) / 64, kTestBlock, 0, st>>>(net_view(g), dp, dn, dw, n, dout);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(out, dout, out_bytes, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipFree(d);
+    WR_HIP(e);
+    return WR_OK;
+}
+
+int wr_guide_gradient(wr_guide *g, const float *pos, const float *nrm, const float *wo, const float *wi,
+                      const float *weight, uint32_t n, float *grad, double *loss) {
+    if (!g || !grad || !wi || !weight) return set_error(WR_ERR_INVALID_ARGUMENT, "null argument");
+    int rc = check_records(pos, nrm, wo, n);
+    if (!rc) rc = set_device(g->scene->dev);
+    if (rc) return rc;
+    if (n == 0) return set_error(WR_ERR_INVALID_ARGUMENT, "no records");
+    DeviceRecords dr;
+    rc = dr.upload(g->scene->stream, pos, nrm, wo, wi, weight, n);
+    if (rc) return rc;
+    float *dg = nullptr;
+    WR_HIP(hipMalloc(reinterpret_cast<void **>(&dg), sizeof(float) * kNumParams));
+    rc = train_step(g, dr.pos, dr.nrm, dr.wo, dr.wi, n, 0, n, 1.f / float(n), dg, loss);
+    hipError_t e = hipSuccess;
+    if (!rc) {
+        e = hipMemcpyAsync(grad, dg, sizeof(float) * kNumParams, hipMemcpyDeviceToHost, g->scene->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(g->scene->stream);
+    }
+    (void)hipFree(dg);
+    if (rc) return rc;
+    WR_HIP(e);
+    return WR_OK;
+}
+
+int wr_guide_train(wr_guide *g, const float *pos, const float *nrm, const float *wo, const float *wi,
+                   const float *weight, uint32_t n, uint32_t steps, double *loss) {
+    if (!g || !wi || !weight) return set_error(WR_ERR_INVALID_ARGUMENT, "null argument");
+    int rc = check_records(pos, nrm, wo, n);
+    if (!rc) rc = set_device(g->scene->dev);
+    if (rc) return rc;
+    if (n == 0) return set_error(WR_ERR_INVALID_ARGUMENT, "no records");
+    DeviceRecords dr;
+    rc = dr.upload(g->scene->stream, pos, nrm, wo, wi, weight, n);
+    if (rc) return rc;
+    for (uint32_t k = 0; k < steps; k++) {
+        rc = train_step(g, dr.pos, dr.nrm, dr.wo, dr.wi, n, 0, n, 1.f / float(n), nullptr,
+                        (k + 1 == steps) ? loss : nullptr);
+        if (rc) return rc;
+    }
+    WR_HIP(hipStreamSynchronize(g->scene->stream));
+    return WR_OK;
+}
+
+int wr_guide_mixture(wr_guide *g, const float *raw, const float *dirs, const float *u, uint32_t n, float *pdf_dirs,
+                     float *samples, float *sample_pdf) {
+    if (!g || (n && (!raw || !dirs || !u || !pdf_dirs || !samples || !sample_pdf)))
+        return set_error(WR_ERR_INVALID_ARGUMENT, "null argument");
+    if (n > (1u << 26)) return set_error(WR_ERR_UNSUPPORTED, "too many rows (limit 2^26)");
+    int rc = set_device(g->scene->dev);
+    if (rc || n == 0) return rc;
+    hipStream_t st = g->scene->stream;
+    float *d = nullptr;
+    const size_t N = n;
+    WR_HIP(hipMalloc(reinterpret_cast<void **>(&d), sizeof(float) * N * (kOut + 3 + 3 + 1 + 3 + 1)));
+    float *draw = d, *ddir = d + kOut * N, *du = ddir + 3 * N, *dpd = du + 3 * N, *ds = dpd + N, *dsp = ds + 3 * N;
+    hipError_t e = hipMemcpyAsync(draw, raw, sizeof(float) * kOut * N, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(ddir, dirs, sizeof(float) * 3 * N, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(du, u, sizeof(float) * 3 * N, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) {
+        k_mixture_test<<<(n + 63) / 64, kTestBlock, 0, st>>>(draw, ddir, du, n, dpd, ds, dsp);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(pdf_dirs, dpd, sizeof(float) * N, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(samples, ds, sizeof(float) * 3 * N, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(sample_pdf, dsp, sizeof(float) * N, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipFree(d);
+    WR_HIP(e);
+    return WR_OK;
+}

@@ -1,0 +1,149 @@
+// wr_host.h -- host-side state shared by the library's translation units:
+// error reporting, the path-pool layout and the device, scene and film handles.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/wurst_amd.h"
+#include "wr_device.h"
+#include "wr_layout.h"
+
+namespace wr {
+// Thread-local message behind wr_last_error().
+inline std::string &error_string() {
+    static thread_local std::string s;
+    return s;
+}
+}  // namespace wr
+
+using namespace wr;
+
+// ============================================================== errors
+static inline int set_error(int code, const std::string &msg) {
+    wr::error_string() = msg;
+    return code;
+}
+#define WR_HIP(expr)                                                                                     \
+    do {                                                                                                 \
+        hipError_t e_ = (expr);                                                                          \
+        if (e_ != hipSuccess)                                                                            \
+            return set_error(e_ == hipErrorOutOfMemory ? WR_ERR_OUT_OF_MEMORY : WR_ERR_DEVICE,           \
+                             std::string(#expr) + ": " + hipGetErrorString(e_));                         \
+    } while (0)
+
+// ============================================================== kernel arguments
+constexpr int kBlock = 64;  // one wavefront per block; LDS stack indexed by lane
+
+// Shadow-ray targets: where a visible connection's contribution goes.
+constexpr uint32_t kTargetMask = 0x3FFFFFFFu;
+constexpr uint32_t kToNextPath = 0u << 30;   // radiance of a continuing path (next-state index)
+constexpr uint32_t kToSample = 1u << 30;     // per-sample radiance of a finished path
+constexpr uint32_t kToSplat = 2u << 30;      // light->camera splat (pixel of the FP64 splat film)
+
+// Path state in queue order (SoA, double buffered): entry i of the current
+// state is the i-th active path.  Shade writes continuing paths compacted into
+// the next state; regen appends new samples behind them.  Every kernel walks
+// the state in order, so all path-state traffic is coalesced.
+struct PathState {
+    float4 *ray_o, *ray_d;  // ray to extend
+    float4 *thr;            // rgb: throughput of the next vertex; w: pdfW of the ray direction
+    float4 *rad;            // rgb: radiance gathered so far (FP32)
+    uint4 *meta;            // x: RNG key, y: sample index in the pass, z: bounce of the next vertex,
+                            // w: 1 + training record of the previous vertex (0: none)
+};
+
+struct PathBuffers {
+    PathState cur, next;
+    uint2 *hit;                  // per current entry: leaf-order triangle (or kInvalid), t bits
+    float4 *sample_rad;          // per sample of the pass: final radiance (FP32)
+    float4 *sh_o, *sh_d, *sh_c;  // shadow rays: o.xyz+tmax, d.xyz+target, contribution
+    uint32_t *counters;          // see Counter
+    unsigned long long *stats;   // [0] extend rays [1] shadow rays [2] ext nodes [3] ext tris [4] shd nodes [5] shd tris
+    double *film_base, *film_splat;
+};
+
+// Each counter sits on its own 256-byte line: atomics on one line serialise.
+constexpr int kCounterStride = 64;
+enum Counter : int {
+    kQ = 0 * kCounterStride,         // continuing paths at the head of the current state
+    kFresh = 1 * kCounterStride,     // new samples started behind them this iteration
+    kQn = 2 * kCounterStride,        // paths continuing into the next state
+    kS = 3 * kCounterStride,         // shadow rays
+    kFetchExt = 4 * kCounterStride,  // work counter of the extend kernel
+    kFetchShd = 5 * kCounterStride,  // work counter of the shadow kernel
+    kClaimed = 6 * kCounterStride,   // samples started so far in the pass
+    kBase = 7 * kCounterStride,      // first sample index started this iteration
+    kNumCounterLines = 8
+};
+
+struct RenderArgs {
+    uint32_t total;           // samples of this pass = spp * W * H
+    uint32_t wh;
+    uint32_t first_sample;    // sample index of the pass's first spp
+    uint32_t seed;
+    uint32_t capacity;        // path pool size
+    int N;                    // num_path_vertices
+    int use_mis;
+};
+
+// ============================================================== host objects
+struct wr_device {
+    int ordinal = 0;
+    int n_cu = 256;
+    // resident blocks per CU of each kernel (HIP occupancy API), used to size persistent grids
+    int occ_extend = 8, occ_shadow = 8, occ_shade = 8, occ_generate = 8, occ_trace = 8, occ_shade_guided = 4, occ_infer = 8;
+};
+
+struct wr_scene {
+    wr_device *dev = nullptr;
+    hipStream_t stream = nullptr;
+    DScene ds{};
+    // device allocations
+    Node4 *d_nodes = nullptr;
+    float4 *d_tris = nullptr, *d_ltris = nullptr;
+    uint2 *d_spill = nullptr;   // traversal stack spill: kSpillDepth entries per lane of the largest grid
+    int max_trav_blocks = 0;
+    int2 *d_geom = nullptr;
+    DMat *d_mats = nullptr;
+    DLight *d_lights = nullptr;
+    uint64_t n_nodes = 0, n_tris = 0, device_bytes = 0;
+    uint32_t max_stack = 0;
+    // host side: triangle index -> (geom, prim)
+    std::vector<uint32_t> tri_geom, geom_first, leaf_to_orig;
+    int W = 0, H = 0;
+    float box[6] = {0, 0, 0, 0, 0, 0};  // bounds of the vertices (lo xyz, hi xyz)
+    // path buffers (lazily sized)
+    PathBuffers pb{};
+    uint32_t capacity = 0, sample_capacity = 0;
+    void *pool = nullptr;
+    uint32_t *h_counters = nullptr;  // pinned host copy of the pool counters
+    hipEvent_t ev[2] = {nullptr, nullptr};
+};
+
+struct wr_film {
+    wr_scene *scene = nullptr;
+    double *d_base = nullptr, *d_splat = nullptr;
+    int W = 0, H = 0;
+};
+
+static inline int set_device(wr_device *dev) {
+    WR_HIP(hipSetDevice(dev->ordinal));
+    return WR_OK;
+}
+
+// ============================================================== guiding (wr_guide.hip)
+#include "wr_guide.h"
+wr::guide::Net guide_net(const wr_guide *g);
+// Device view for the guided shading kernel.
+int guide_shade_view(wr_guide *g, float alpha, float record_prob, bool collect, wr::guide::Shade *out);
+// Clears the record buffer and picks the fraction of paths to record so that
+// about 90% of the capacity fills.
+int guide_begin_pass(wr_guide *g, uint64_t samples, float *record_prob);
+// Training weights from the pass's final per-sample radiance, then `steps` Adam steps.
+int guide_end_pass(wr_guide *g, const float4 *sample_rad, uint64_t samples, float record_prob, uint32_t steps,
+                   wr_guide_stats *gs);
+// Sizes the decoded-lobe buffer for a path pool of `capacity` entries.
+int guide_reserve_lobes(wr_guide *g, uint32_t capacity);
+wr_scene *guide_scene(const wr_guide *g);

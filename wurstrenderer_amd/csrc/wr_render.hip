@@ -36,76 +36,11 @@
 #include "wr_device.h"
 #include "wr_layout.h"
 #include "wr_traverse.h"
+#include "wr_guide.h"
 
 using namespace wr;
 
-// ============================================================== errors
-static thread_local std::string g_error;
-static int set_error(int code, const std::string &msg) {
-    g_error = msg;
-    return code;
-}
-#define WR_HIP(expr)                                                                                     \
-    do {                                                                                                 \
-        hipError_t e_ = (expr);                                                                          \
-        if (e_ != hipSuccess)                                                                            \
-            return set_error(e_ == hipErrorOutOfMemory ? WR_ERR_OUT_OF_MEMORY : WR_ERR_DEVICE,           \
-                             std::string(#expr) + ": " + hipGetErrorString(e_));                         \
-    } while (0)
-
-// ============================================================== kernel arguments
-constexpr int kBlock = 64;  // one wavefront per block; LDS stack indexed by lane
-
-// Shadow-ray targets: where a visible connection's contribution goes.
-constexpr uint32_t kTargetMask = 0x3FFFFFFFu;
-constexpr uint32_t kToNextPath = 0u << 30;   // radiance of a continuing path (next-state index)
-constexpr uint32_t kToSample = 1u << 30;     // per-sample radiance of a finished path
-constexpr uint32_t kToSplat = 2u << 30;      // light->camera splat (pixel of the FP64 splat film)
-
-// Path state in queue order (SoA, double buffered): entry i of the current
-// state is the i-th active path.  Shade writes continuing paths compacted into
-// the next state; regen appends new samples behind them.  Every kernel walks
-// the state in order, so all path-state traffic is coalesced.
-struct PathState {
-    float4 *ray_o, *ray_d;  // ray to extend
-    float4 *thr;            // rgb: throughput of the next vertex; w: pdfW of the ray direction
-    float4 *rad;            // rgb: radiance gathered so far (FP32)
-    uint4 *meta;            // x: RNG key, y: sample index in the pass, z: bounce of the next vertex
-};
-
-struct PathBuffers {
-    PathState cur, next;
-    uint2 *hit;                  // per current entry: leaf-order triangle (or kInvalid), t bits
-    float4 *sample_rad;          // per sample of the pass: final radiance (FP32)
-    float4 *sh_o, *sh_d, *sh_c;  // shadow rays: o.xyz+tmax, d.xyz+target, contribution
-    uint32_t *counters;          // see Counter
-    unsigned long long *stats;   // [0] extend rays [1] shadow rays [2] ext nodes [3] ext tris [4] shd nodes [5] shd tris
-    double *film_base, *film_splat;
-};
-
-// Each counter sits on its own 256-byte line: atomics on one line serialise.
-constexpr int kCounterStride = 64;
-enum Counter : int {
-    kQ = 0 * kCounterStride,         // continuing paths at the head of the current state
-    kFresh = 1 * kCounterStride,     // new samples started behind them this iteration
-    kQn = 2 * kCounterStride,        // paths continuing into the next state
-    kS = 3 * kCounterStride,         // shadow rays
-    kFetchExt = 4 * kCounterStride,  // work counter of the extend kernel
-    kFetchShd = 5 * kCounterStride,  // work counter of the shadow kernel
-    kClaimed = 6 * kCounterStride,   // samples started so far in the pass
-    kBase = 7 * kCounterStride,      // first sample index started this iteration
-    kNumCounterLines = 8
-};
-
-struct RenderArgs {
-    uint32_t total;           // samples of this pass = spp * W * H
-    uint32_t wh;
-    uint32_t first_sample;    // sample index of the pass's first spp
-    uint32_t seed;
-    uint32_t capacity;        // path pool size
-    int N;                    // num_path_vertices
-    int use_mis;
-};
+#include "wr_host.h"
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
@@ -491,6 +426,229 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(DScene s, PathBuffers pb,
     }
 }
 
+// ============================================================== guided shade
+// k_shade with learned guiding: the direction at a surface vertex is drawn by
+// one-sample MIS from the network's lobe mixture (probability alpha) or the
+// BSDF, weighted by f cos / (alpha q + (1 - alpha) p_bsdf); the same combined
+// density enters the NEE and emitter-hit MIS weights, so the estimator stays
+// unbiased for any network.  The lobes come from k_guide_infer.  Selected
+// paths append a training record at every surface vertex; the record is
+// closed when the path's next vertex is shaded (radiance snapshot).
+__device__ __forceinline__ float guide_alpha(const DMat &m, float frac) {
+    // sharp Phong lobes are sampled well by the BSDF itself
+    return m.type == kPhong ? frac * fminf(1.f, fmaxf(0.1f, 30.f / m.n)) : frac;
+}
+
+// Network inference for the current state: the decoded lobes of every surface
+// vertex that k_shade_guided will shade (not an emitter, not the last vertex),
+// one forward pass per wave of 64 entries on MFMA.
+__global__ __launch_bounds__(kBlock) void k_guide_infer(const float4 *__restrict__ tris, const int2 *__restrict__ geom_info,
+                                                        PathState cs, const uint2 *__restrict__ hit,
+                                                        const uint32_t *__restrict__ counters, int N, guide::Net net,
+                                                        float4 *lobes, uint32_t stride) {
+    __shared__ float stage[64 * guide::kStageStride];
+    const uint32_t n = active_count(counters);
+    for (uint32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
+        const uint32_t i = base + threadIdx.x;
+        bool need = false;
+        float3 pos = f3(0.f, 0.f, 0.f), nf = f3(0.f, 1.f, 0.f), rd = f3(0.f, 0.f, 1.f);
+        if (i < n) {
+            const uint2 h = hit[i];
+            if (h.x != kInvalid && int(cs.meta[i].z) + 2 != N) {
+                const float4 v0 = tris[3 * size_t(h.x) + 0];
+                const float4 v1 = tris[3 * size_t(h.x) + 1];
+                const float4 v2 = tris[3 * size_t(h.x) + 2];
+                if (geom_info[__float_as_uint(v1.w)].y < 0) {
+                    need = true;
+                    rd = xyz(cs.ray_d[i]);
+                    pos = xyz(cs.ray_o[i]) + rd * __uint_as_float(h.y);
+                    const float3 ng = normalize3(cross3(xyz(v1) - xyz(v0), xyz(v2) - xyz(v0)));
+                    nf = dot3(ng, rd) > 0.f ? -ng : ng;
+                }
+            }
+        }
+        if (__ballot(need)) {
+            guide::forward(net, guide::encode(pos, nf, -rd, net.box), stage);
+            if (need) {
+                float *L = stage + threadIdx.x * guide::kStageStride;
+                guide::decode(L);
+#pragma unroll
+                for (int j = 0; j < guide::kLobeVecs; j++)
+                    lobes[size_t(j) * stride + i] = make_float4(L[4 * j], L[4 * j + 1], L[4 * j + 2], L[4 * j + 3]);
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(kShadeBlock) void k_shade_guided(DScene s, PathBuffers pb, RenderArgs a, guide::Shade g) {
+    __shared__ uint32_t agg[kWavesPerShadeBlock + 1];
+    __shared__ float lobes[kShadeBlock * guide::kStageStride];
+    float *lob = lobes + threadIdx.x * guide::kStageStride;
+    const uint32_t n = active_count(pb.counters);
+    const PathState cs = pb.cur, ns = pb.next;
+    const float3 zero = f3(0.f, 0.f, 0.f);
+    for (uint32_t base = blockIdx.x * kShadeBlock; base < n; base += gridDim.x * kShadeBlock) {
+        const uint32_t i = base + threadIdx.x;
+        const bool valid = i < n;
+        // ---- the vertex
+        uint4 meta = make_uint4(0, 0, 0, 0);
+        uint2 h = make_uint2(kInvalid, 0u);
+        float3 pos = zero, ng = f3(0.f, 1.f, 0.f), rd = f3(0.f, 0.f, 1.f);
+        int2 gi = make_int2(0, -1);
+        if (valid) {
+            meta = cs.meta[i];
+            h = pb.hit[i];
+            if (h.x != kInvalid) {
+                rd = xyz(cs.ray_d[i]);
+                pos = xyz(cs.ray_o[i]) + rd * __uint_as_float(h.y);
+                const float4 v0 = s.tris[3 * size_t(h.x) + 0];
+                const float4 v1 = s.tris[3 * size_t(h.x) + 1];
+                const float4 v2 = s.tris[3 * size_t(h.x) + 2];
+                ng = normalize3(cross3(xyz(v1) - xyz(v0), xyz(v2) - xyz(v0)));
+                gi = s.geom_info[__float_as_uint(v1.w)];
+            }
+        }
+        const bool surf = valid && h.x != kInvalid && gi.y < 0 && int(meta.z) + 2 != a.N;
+        const float3 nf = dot3(ng, rd) > 0.f ? -ng : ng;  // normal facing the incoming ray
+        const bool need = surf && g.alpha > 0.f;
+        // this vertex's lobes, staged in LDS (a 40-float register array would live in scratch)
+        if (need) {
+#pragma unroll
+            for (int j = 0; j < guide::kLobeVecs; j++) {
+                const float4 q = g.lobes[size_t(j) * g.lobe_stride + i];
+                lob[4 * j] = q.x; lob[4 * j + 1] = q.y; lob[4 * j + 2] = q.z; lob[4 * j + 3] = q.w;
+            }
+        }
+        // ---- shading (as k_shade, with the combined density)
+        bool push_shadow = false, cont = false;
+        float4 so = make_float4(0, 0, 0, 0), sd = so, sc = so;
+        float4 new_o = so, new_d = so, new_thr = so, rad = so;
+        float3 wdir = zero, T = zero;
+        const uint32_t key = meta.x, bounce = meta.z;
+        if (valid) {
+            rad = cs.rad[i];
+            // radiance before this vertex closes the record of the previous one
+            if (meta.w) g.rec.rad[meta.w - 1] = rad;
+        }
+        if (valid && h.x != kInvalid) {
+            const float4 thr = cs.thr[i];
+            T = xyz(thr);
+            if (gi.y >= 0) {
+                // emitter hit: MIS against light sampling (pathnextevent.h:128-146)
+                const float t = __uint_as_float(h.y);
+                const float cos_l = -dot3(ng, rd);
+                if (cos_l >= 0.f) {
+                    const DLight &L = s.lights[gi.y];
+                    float mis = 0.f;
+                    if (a.use_mis) {
+                        const float cPdfA = thr.w * fabsf(cos_l) / (t * t);
+                        const float lightPdfA = L.pdf_select * L.inv_area;
+                        mis = cPdfA / (lightPdfA + cPdfA);
+                    }
+                    rad.x += mis * T.x * L.L[0];
+                    rad.y += mis * T.y * L.L[1];
+                    rad.z += mis * T.z * L.L[2];
+                }
+            }
+        }
+        if (surf) {
+            const DMat m = s.mats[gi.x];
+            const Frame f = make_frame(ng);
+            const float3 wi = to_local(f, -rd);
+            const float alpha = need ? guide_alpha(m, g.alpha) : 0.f;
+            // NEE with MIS against the combined density (pathnextevent.h:106-127)
+            const LightSample l = sample_light(s, u01(key, kDimLight), u01(key, kDimLight + 1));
+            const float3 uv = l.pos - pos;
+            const float len2 = dot3(uv, uv);
+            const float len = sqrtf(len2);
+            const float3 nuv = uv * (1.0f / len);
+            const float cos_lv = -dot3(l.n, nuv);
+            const float G = fabsf(dot3(ng, nuv)) * fabsf(cos_lv) / len2;
+            if (G != 0.f && cos_lv >= 0.f) {
+                const float3 wo = to_local(f, nuv);
+                const float3 fs = bsdf_eval(m, wi, wo);
+                if (!is_black(fs)) {
+                    float mis = 1.f;
+                    if (a.use_mis) {
+                        float p = bsdf_pdf(m, wi, wo);
+                        if (need) p = alpha * guide::pdf(lob, nuv) + (1.f - alpha) * p;
+                        const float bsdfPdfA = p * fabsf(cos_lv) / len2;
+                        mis = l.pdfA / (l.pdfA + bsdfPdfA);
+                    }
+                    const float k = mis * G * kInvPi;
+                    const float3 c = T * fs * l.contrib * k;
+                    if (!is_black(c)) {
+                        so = make_float4(pos.x, pos.y, pos.z, len - kRayEps);
+                        sd = make_float4(nuv.x, nuv.y, nuv.z, 0.f);
+                        sc = make_float4(c.x, c.y, c.z, 0.f);
+                        push_shadow = true;
+                    }
+                }
+            }
+            // direction: one-sample MIS of network and BSDF
+            const uint32_t dim = kDimBounce0 + kDimsPerBounce * bounce + kBdBsdf;
+            const float u0 = u01(key, dim), u1 = u01(key, dim + 1);
+            float3 wo = zero, w = zero;
+            float pdf = 0.f;
+            const float us = u01(key, dim + 2);
+            if (need && us < alpha) {
+                const float3 d = guide::sample(lob, us / alpha, u0, u1);
+                wo = to_local(f, d);
+                const float3 fs = bsdf_eval(m, wi, wo);
+                pdf = alpha * guide::pdf(lob, d) + (1.f - alpha) * bsdf_pdf(m, wi, wo);
+                if (pdf > 0.f && !is_black(fs)) w = fs * (fabsf(wo.y) / pdf);
+            } else {
+                w = bsdf_sample(m, wi, u0, u1, wo, pdf);
+                if (need && !is_black(w)) {
+                    const float pbs = pdf;
+                    pdf = alpha * guide::pdf(lob, to_world(f, wo)) + (1.f - alpha) * pbs;
+                    w = w * (pbs / pdf);
+                }
+            }
+            if (!is_black(w) && pdf > 0.f) {
+                const float3 nd = to_world(f, wo);
+                wdir = nd;
+                new_o = make_float4(pos.x, pos.y, pos.z, 0.f);
+                new_d = make_float4(nd.x, nd.y, nd.z, 0.f);
+                new_thr = make_float4(w.x * T.x, w.y * T.y, w.z * T.z, a.use_mis ? pdf : 0.f);
+                cont = true;
+            }
+        }
+        uint32_t rec_id = 0;
+        if (g.collect) {
+            const bool want = cont && float(mix32(key ^ 0x68bc21ebu) >> 8) * 0x1p-24f < g.record_prob;
+            const uint32_t r = block_append(g.rec.count, want, agg);
+            if (want && r < g.rec.capacity) {
+                g.rec.pos[r] = make_float4(pos.x, pos.y, pos.z, __uint_as_float(meta.y));
+                g.rec.nrm[r] = make_float4(nf.x, nf.y, nf.z, 0.f);
+                g.rec.wo[r] = make_float4(-rd.x, -rd.y, -rd.z, 0.f);
+                g.rec.wi[r] = make_float4(wdir.x, wdir.y, wdir.z, 0.f);
+                g.rec.thr[r] = make_float4(T.x, T.y, T.z, 0.f);
+                g.rec.rad[r] = make_float4(0.f, 0.f, 0.f, 0.f);
+                rec_id = r + 1;
+            }
+        }
+        const uint32_t qi = block_append(&pb.counters[kQn], cont, agg);
+        if (cont) {
+            ns.ray_o[qi] = new_o;
+            ns.ray_d[qi] = new_d;
+            ns.thr[qi] = new_thr;
+            ns.rad[qi] = rad;
+            ns.meta[qi] = make_uint4(meta.x, meta.y, meta.z + 1, rec_id);
+        } else if (valid) {
+            pb.sample_rad[meta.y] = rad;
+        }
+        const uint32_t target = cont ? (kToNextPath | qi) : (kToSample | meta.y);
+        sd.w = __uint_as_float(target);
+        const uint32_t si = block_append(&pb.counters[kS], push_shadow, agg);
+        if (push_shadow) {
+            pb.sh_o[si] = so;
+            pb.sh_d[si] = sd;
+            pb.sh_c[si] = sc;
+        }
+    }
+}
+
 // ============================================================== shadow
 // Any-hit traversal of the shadow queue; a visible connection adds its
 // contribution to the radiance of its path (continuing or finished), or to the
@@ -588,53 +746,9 @@ __global__ __launch_bounds__(kBlock) void k_trace(DScene s, const float4 *__rest
     }
 }
 
-// ============================================================== host objects
-struct wr_device {
-    int ordinal = 0;
-    int n_cu = 256;
-    // resident blocks per CU of each kernel (HIP occupancy API), used to size persistent grids
-    int occ_extend = 8, occ_shadow = 8, occ_shade = 8, occ_generate = 8, occ_trace = 8;
-};
-
-struct wr_scene {
-    wr_device *dev = nullptr;
-    hipStream_t stream = nullptr;
-    DScene ds{};
-    // device allocations
-    Node4 *d_nodes = nullptr;
-    float4 *d_tris = nullptr, *d_ltris = nullptr;
-    uint2 *d_spill = nullptr;   // traversal stack spill: kSpillDepth entries per lane of the largest grid
-    int max_trav_blocks = 0;
-    int2 *d_geom = nullptr;
-    DMat *d_mats = nullptr;
-    DLight *d_lights = nullptr;
-    uint64_t n_nodes = 0, n_tris = 0, device_bytes = 0;
-    uint32_t max_stack = 0;
-    // host side: triangle index -> (geom, prim)
-    std::vector<uint32_t> tri_geom, geom_first, leaf_to_orig;
-    int W = 0, H = 0;
-    // path buffers (lazily sized)
-    PathBuffers pb{};
-    uint32_t capacity = 0, sample_capacity = 0;
-    void *pool = nullptr;
-    uint32_t *h_counters = nullptr;  // pinned host copy of the pool counters
-    hipEvent_t ev[2] = {nullptr, nullptr};
-};
-
-struct wr_film {
-    wr_scene *scene = nullptr;
-    double *d_base = nullptr, *d_splat = nullptr;
-    int W = 0, H = 0;
-};
-
-static int set_device(wr_device *dev) {
-    WR_HIP(hipSetDevice(dev->ordinal));
-    return WR_OK;
-}
-
 // The entry points below inherit C linkage from their declarations in wurst_amd.h.
 
-const char *wr_last_error(void) { return g_error.c_str(); }
+const char *wr_last_error(void) { return wr::error_string().c_str(); }
 const char *wr_version(void) { return "wurst_amd 0.1 (gfx950, BVH4-128B, wavefront NEE+MIS)"; }
 
 int wr_device_create(int ordinal, wr_device **out) {
@@ -657,6 +771,8 @@ int wr_device_create(int ordinal, wr_device **out) {
     occ(reinterpret_cast<const void *>(&k_extend<false>), kBlock, d->occ_extend);
     occ(reinterpret_cast<const void *>(&k_shadow<false>), kBlock, d->occ_shadow);
     occ(reinterpret_cast<const void *>(&k_shade), kShadeBlock, d->occ_shade);
+    occ(reinterpret_cast<const void *>(&k_shade_guided), kShadeBlock, d->occ_shade_guided);
+    occ(reinterpret_cast<const void *>(&k_guide_infer), kBlock, d->occ_infer);
     occ(reinterpret_cast<const void *>(&k_regen), kShadeBlock, d->occ_generate);
     occ(reinterpret_cast<const void *>(&k_trace<false>), kBlock, d->occ_trace);
     *out = d;
@@ -753,6 +869,12 @@ static int scene_create_impl(wr_device *dev, const wr_scene_desc *d, wr_scene *s
     WR_HIP(hipEventCreate(&s->ev[1]));
     s->W = d->camera.width;
     s->H = d->camera.height;
+    for (int k = 0; k < 3; k++) { s->box[k] = INFINITY; s->box[3 + k] = -INFINITY; }
+    for (uint32_t v = 0; v < d->n_verts; v++)
+        for (int k = 0; k < 3; k++) {
+            s->box[k] = std::min(s->box[k], d->verts[3 * size_t(v) + k]);
+            s->box[3 + k] = std::max(s->box[3 + k], d->verts[3 * size_t(v) + k]);
+        }
     // host tables
     s->tri_geom.assign(d->tri_geom, d->tri_geom + d->n_tris);
     s->geom_first.assign(d->n_geoms, kInvalid);
@@ -911,9 +1033,9 @@ int wr_scene_create(wr_device *dev, const wr_scene_desc *desc, wr_scene **out) {
     wr_scene *s = new wr_scene;
     rc = scene_create_impl(dev, desc, s);
     if (rc) {
-        std::string msg = g_error;
+        std::string msg = wr::error_string();
         wr_scene_destroy(s);
-        g_error = msg;
+        wr::error_string() = msg;
         return rc;
     }
     *out = s;
@@ -1051,10 +1173,17 @@ static int trav_grid(const wr_scene *s, uint32_t n, int per_cu) {
 // for a 1920x1080 frame at 64 spp, so such a frame drains the pool once.
 constexpr uint64_t kMaxSamplesPerPass = uint64_t(1) << 27;
 
-int wr_render_nee(wr_scene *s, wr_film *f, const wr_render_params *prm, wr_render_stats *stats) {
+// The frame loop of wr_render_nee and wr_render_guided (g non-null).
+static int render_impl(wr_scene *s, wr_film *f, const wr_render_params *prm, wr_guide *g, const wr_guided_params *gp,
+                       wr_render_stats *stats, wr_guide_stats *gstats) {
     if (!s || !f || !prm) return set_error(WR_ERR_INVALID_ARGUMENT, "null argument");
     if (f->scene != s) return set_error(WR_ERR_INVALID_ARGUMENT, "film belongs to another scene");
     if (prm->spp < 0) return set_error(WR_ERR_INVALID_ARGUMENT, "spp must be >= 0");
+    if (g && !gp) return set_error(WR_ERR_INVALID_ARGUMENT, "null guided parameters");
+    if (g && guide_scene(g) != s) return set_error(WR_ERR_INVALID_ARGUMENT, "guide belongs to another scene");
+    if (g && !(gp->guide_fraction >= 0.f && gp->guide_fraction < 1.f))
+        return set_error(WR_ERR_INVALID_ARGUMENT, "guide_fraction must be in [0, 1)");
+    wr_guide_stats gst{};
     const auto t0 = std::chrono::steady_clock::now();
     wr_render_stats st{};
     int rc = set_device(s->dev);
@@ -1063,10 +1192,12 @@ int wr_render_nee(wr_scene *s, wr_film *f, const wr_render_params *prm, wr_rende
     const uint32_t wh = uint32_t(s->W) * uint32_t(s->H);
     if (N <= 1 || prm->spp == 0) {  // pathnextevent.h:163
         if (stats) *stats = st;
+        if (gstats) *gstats = gst;
         return WR_OK;
     }
-    const uint32_t spp_pass =
-        uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(kMaxSamplesPerPass / wh, uint64_t(prm->spp))));
+    uint64_t spp_cap = std::min<uint64_t>(kMaxSamplesPerPass / wh, uint64_t(prm->spp));
+    if (g && gp->spp_per_pass > 0) spp_cap = std::min<uint64_t>(spp_cap, gp->spp_per_pass);
+    const uint32_t spp_pass = uint32_t(std::max<uint64_t>(1, spp_cap));
     const uint32_t samples_pass = spp_pass * wh;
     // default pool: 32M paths (about 5.6 GB of state): large launches hide the
     // traversal latency and shrink the per-launch tail
@@ -1076,6 +1207,10 @@ int wr_render_nee(wr_scene *s, wr_film *f, const wr_render_params *prm, wr_rende
     const uint32_t P = uint32_t(pool);
     rc = ensure_capacity(s, P, samples_pass);
     if (rc) return rc;
+    if (g) {
+        rc = guide_reserve_lobes(g, P);
+        if (rc) return rc;
+    }
     PathBuffers pb = s->pb;
     pb.film_base = f->d_base;
     pb.film_splat = f->d_splat;
@@ -1096,7 +1231,11 @@ int wr_render_nee(wr_scene *s, wr_film *f, const wr_render_params *prm, wr_rende
     auto begin = [&](int kind) { if (prof) marks.push_back({kind, ev_record()}); };
     auto end = [&]() { if (prof) ev_record(); };
     const int g_regen = std::max(1, std::min<int>((P + kShadeBlock - 1) / kShadeBlock, s->dev->n_cu * s->dev->occ_generate));
-    const int g_shade = std::max(1, std::min<int>((P + kShadeBlock - 1) / kShadeBlock, s->dev->n_cu * s->dev->occ_shade));
+    const int g_shade = std::max(
+        1, std::min<int>((P + kShadeBlock - 1) / kShadeBlock, s->dev->n_cu * (g ? s->dev->occ_shade_guided : s->dev->occ_shade)));
+    int pass = 0;
+    const int g_infer = std::max(1, std::min<int>((P + kBlock - 1) / kBlock, s->dev->n_cu * s->dev->occ_infer));
+    const guide::Net net = g ? guide_net(g) : guide::Net{};
     const int g_ext = trav_grid(s, P, s->dev->occ_extend);
     const int g_shd = trav_grid(s, 2 * P, s->dev->occ_shadow);
     for (uint32_t done = 0; done < uint32_t(prm->spp); done += spp_pass) {
@@ -1109,6 +1248,18 @@ int wr_render_nee(wr_scene *s, wr_film *f, const wr_render_params *prm, wr_rende
         a.capacity = P;
         a.N = N;
         a.use_mis = prm->use_mis ? 1 : 0;
+        guide::Shade gs{};
+        float record_prob = 0.f;
+        const bool collect = g && gp->train_steps > 0;
+        if (g) {
+            if (collect) {
+                rc = guide_begin_pass(g, a.total, &record_prob);
+                if (rc) return rc;
+            }
+            const bool sample_net = pass >= int(gp->warmup_passes);
+            rc = guide_shade_view(g, sample_net ? gp->guide_fraction : 0.f, record_prob, collect, &gs);
+            if (rc) return rc;
+        }
         k_init<<<1, 64, 0, s->stream>>>(pb, a);
         WR_HIP(hipGetLastError());
         // Each iteration refills the pool, extends every active path by one
@@ -1128,7 +1279,11 @@ int wr_render_nee(wr_scene *s, wr_film *f, const wr_render_params *prm, wr_rende
             else k_extend<false><<<g_ext, kBlock, 0, s->stream>>>(s->ds, pb, s->d_spill);
             end();
             begin(2);
-            k_shade<<<g_shade, kShadeBlock, 0, s->stream>>>(s->ds, pb, a);
+            if (g && gs.alpha > 0.f)
+                k_guide_infer<<<g_infer, kBlock, 0, s->stream>>>(s->ds.tris, s->ds.geom_info, pb.cur, pb.hit, pb.counters, N,
+                                                                   net, const_cast<float4 *>(gs.lobes), gs.lobe_stride);
+            if (g) k_shade_guided<<<g_shade, kShadeBlock, 0, s->stream>>>(s->ds, pb, a, gs);
+            else k_shade<<<g_shade, kShadeBlock, 0, s->stream>>>(s->ds, pb, a);
             end();
             begin(1);
             if (count) k_shadow<true><<<g_shd, kBlock, 0, s->stream>>>(s->ds, pb, s->d_spill);
@@ -1150,6 +1305,11 @@ int wr_render_nee(wr_scene *s, wr_film *f, const wr_render_params *prm, wr_rende
                        s->stream>>>(pb, wh, k);
         end();
         WR_HIP(hipGetLastError());
+        if (collect) {
+            rc = guide_end_pass(g, pb.sample_rad, a.total, record_prob, gp->train_steps, &gst);
+            if (rc) return rc;
+        }
+        pass++;
     }
     unsigned long long hs[6] = {0};
     WR_HIP(hipMemcpyAsync(hs, pb.stats, sizeof hs, hipMemcpyDeviceToHost, s->stream));
@@ -1177,7 +1337,18 @@ int wr_render_nee(wr_scene *s, wr_film *f, const wr_render_params *prm, wr_rende
     for (hipEvent_t e : evs) (void)hipEventDestroy(e);
     st.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (stats) *stats = st;
+    if (gstats) *gstats = gst;
     return WR_OK;
+}
+
+int wr_render_nee(wr_scene *s, wr_film *f, const wr_render_params *prm, wr_render_stats *stats) {
+    return render_impl(s, f, prm, nullptr, nullptr, stats, nullptr);
+}
+
+int wr_render_guided(wr_scene *s, wr_film *f, wr_guide *g, const wr_render_params *prm, const wr_guided_params *gp,
+                     wr_render_stats *stats, wr_guide_stats *gstats) {
+    if (!g) return set_error(WR_ERR_INVALID_ARGUMENT, "guide is null");
+    return render_impl(s, f, prm, g, gp, stats, gstats);
 }
 
 static int trace_impl(wr_scene *s, const float *d_rays, uint32_t n, void *d_hits, uint8_t *d_occ, bool any, float *ms) {

@@ -214,6 +214,59 @@ def icospheres(seed: int = 2, n_spheres: int = 64, level: int = 5, resolution=(1
                    name=f"icospheres_s{seed}_n{n_spheres}_l{level}")
 
 
+def guide_room(seed: int = 4, n_spheres: int = 49, level: int = 5, resolution=(1920, 1080), gap: float = 0.2,
+               L: float = 500.0) -> Scene:
+    """A closed 4 x 2.5 x 4 room lit only through a gap x gap hole in the ceiling
+    by a hidden gap-sized quad light of radiance L inside a closed attic above
+    it; the room is filled with seeded icosphere clutter (49 level-5 spheres:
+    1,003,520 triangles).  Almost every surface sees the light only indirectly,
+    through the bright patch the gap throws on the floor: the hard case that
+    guiding addresses.  Walls and clutter are two-sided; 70% of the clutter is
+    Lambert, 30% modified Phong."""
+    rng = np.random.default_rng(seed)
+    b = SceneBuilder()
+    grey = b.material(LAMBERT, rho_d=(0.7, 0.7, 0.7))
+    red = b.material(LAMBERT, rho_d=(0.63, 0.065, 0.05))
+    green = b.material(LAMBERT, rho_d=(0.14, 0.45, 0.09))
+    X, Y = 2.0, 2.5
+    b.mesh(*quad((0, 0, 0), (0, 0, X), (X, 0, 0)), material=grey)            # floor
+    b.mesh(*quad((0, Y / 2, -X), (X, 0, 0), (0, Y / 2, 0)), material=grey)   # back
+    b.mesh(*quad((0, Y / 2, X), (X, 0, 0), (0, Y / 2, 0)), material=grey)    # front (behind the camera)
+    b.mesh(*quad((-X, Y / 2, 0), (0, 0, X), (0, Y / 2, 0)), material=red)    # left
+    b.mesh(*quad((X, Y / 2, 0), (0, 0, X), (0, Y / 2, 0)), material=green)   # right
+    h = 0.5 * gap
+    w = 0.5 * (X - h)
+    # ceiling around the hole [-h, h]^2
+    b.mesh(*quad((0, Y, h + w), (X, 0, 0), (0, 0, w)), material=grey)
+    b.mesh(*quad((0, Y, -h - w), (X, 0, 0), (0, 0, w)), material=grey)
+    b.mesh(*quad((-h - w, Y, 0), (w, 0, 0), (0, 0, h)), material=grey)
+    b.mesh(*quad((h + w, Y, 0), (w, 0, 0), (0, 0, h)), material=grey)
+    # closed attic above the hole, light near its top facing down
+    A, top = 3 * h, Y + 0.4
+    mid = 0.5 * (Y + top)
+    b.mesh(*quad((0, top, 0), (A, 0, 0), (0, 0, A)), material=grey)
+    for sx in (-A, A):
+        b.mesh(*quad((sx, mid, 0), (0, 0, A), (0, 0.2, 0)), material=grey)
+    for sz in (-A, A):
+        b.mesh(*quad((0, mid, sz), (A, 0, 0), (0, 0.2, 0)), material=grey)
+    sv, sf = icosphere(level)
+    centres = np.column_stack([rng.uniform(-1.7, 1.7, n_spheres), rng.uniform(0.3, 2.0, n_spheres),
+                               rng.uniform(-1.7, 1.7, n_spheres)])
+    radii = rng.uniform(0.12, 0.3, n_spheres)
+    lambert = rng.random(n_spheres) < 0.7
+    albedo = rng.uniform(0.2, 0.8, size=(n_spheres, 3))
+    expo = np.exp(rng.uniform(math.log(10), math.log(200), size=n_spheres))
+    for i in range(n_spheres):
+        if lambert[i]:
+            m = b.material(LAMBERT, rho_d=tuple(albedo[i]))
+        else:
+            m = b.material(PHONG, rho_s=(0.6, 0.6, 0.6), exponent=float(expo[i]))
+        b.mesh(sv * radii[i] + centres[i], sf, material=m)
+    b.mesh(*quad((0, top - 0.02, 0), (h, 0, 0), (0, 0, h)), light_L=(L, L, L))
+    return b.build(dict(pos=(0, 1.3, 1.95), lookat=(0, 0.9, -1.0), up=(0, 1, 0), fovy=65.0, resolution=resolution),
+                   name=f"guide_room_s{seed}_n{n_spheres}_l{level}")
+
+
 def plane_under_light(resolution=(32, 32), light_half=0.5, height=1.0, rho=0.8, L=10.0, plane_half=4.0) -> Scene:
     """Lambert plane y=0 under a square emitter at y=height facing down; the
     camera looks straight down from below the emitter.  Direct lighting of
