@@ -147,6 +147,20 @@ void wr_scene_destroy(wr_scene *scene);
 int wr_scene_info(const wr_scene *scene, uint64_t *n_nodes, uint64_t *n_tris, uint32_t *max_stack,
                   uint64_t *device_bytes);
 
+/* Homogeneous medium filling the scene (HomogeneousMedium, src/medium/homogeneous.h:11-66;
+ * JSON "medium" {"type": "homogeneous", "sigma_a", "sigma_s"}, simple2parser.h:163-167),
+ * with a Henyey-Greenstein phase function of asymmetry g in (-1, 1) (g = 0 is the
+ * reference's IsotropicPhaseFunction, src/phasefunction/isotropic.h).  Free flights
+ * are sampled on one colour channel with the channel-averaged density, as the
+ * reference does (homogeneous.h:36-50); NEE shadow rays are weighted by the
+ * transmittance.  A null medium (or all-zero coefficients) is vacuum. */
+typedef struct {
+    float sigma_a[3];
+    float sigma_s[3];
+    float g;
+} wr_medium;
+int wr_scene_set_medium(wr_scene *scene, const wr_medium *medium);
+
 /* FP64 film in device memory: a base buffer (own-pixel samples) and a splat
  * buffer (light->camera connections).  Replaces Film's base and atomic buffers
  * (src/common/film.h:35-68). */

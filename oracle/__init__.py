@@ -88,6 +88,10 @@ def lib():
         L.orc_camera_ray.argtypes = [P, C.c_int, C.c_int, D3, D3, D3, D3, D3, D3]
         L.orc_camera_we1.argtypes = [P, D3, D3, D3]
         L.orc_camera_we1.restype = C.c_double
+        L.orc_scene_set_medium.argtypes = [P, D3, D3, C.c_double]
+        L.orc_hg_pdf.argtypes = [C.c_double, C.c_double]
+        L.orc_hg_pdf.restype = C.c_double
+        L.orc_hg_sample.argtypes = [C.c_double, D3, D3, D3]
         _lib = L
     return _lib
 
@@ -130,6 +134,8 @@ class OracleScene:
             raise RuntimeError("oracle: " + L.orc_last_error().decode())
         self.h = h
         self.W, self.H = sc.resolution
+        if getattr(sc, "medium", None):
+            self.set_medium(sc.medium["sigma_a"], sc.medium["sigma_s"], sc.medium["g"])
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -177,6 +183,12 @@ class OracleScene:
             return occ, amb
         lib().orc_trace_any(self.h, rays.ctypes.data, n, occ.ctypes.data)
         return occ
+
+    def set_medium(self, sigma_a=(0, 0, 0), sigma_s=(0, 0, 0), g=0.0):
+        """Homogeneous medium filling the scene, HG phase (zeros remove it)."""
+        a, s = _darr(sigma_a), _darr(sigma_s)
+        if lib().orc_scene_set_medium(self.h, _dp(a), _dp(s), float(g)) != 0:
+            raise RuntimeError(lib().orc_last_error().decode())
 
     def light_sample(self, u):
         u = _darr(u, 2)
@@ -249,6 +261,16 @@ def bsdf_sample(m: OrcMaterial, wi, u):
     wo, w, pdf = np.zeros(3), np.zeros(3), np.zeros(1)
     lib().orc_bsdf_sample(C.byref(m), _dp(wi), _dp(u), _dp(wo), _dp(w), _dp(pdf))
     return wo, w, float(pdf[0])
+
+
+def hg_pdf(g, cos_theta) -> float:
+    return lib().orc_hg_pdf(float(g), float(cos_theta))
+
+
+def hg_sample(g, fwd, u):
+    f, uu, out = _darr(fwd), _darr(u, 2), np.zeros(3)
+    lib().orc_hg_sample(float(g), _dp(f), _dp(uu), _dp(out))
+    return out
 
 
 def rng_uniform(seed, pixel, sample, dim) -> float:

@@ -375,6 +375,9 @@ struct orc_scene {
     v3 cam_origin; frame3 cam_frame;
     int W, H;
     double cam_d, per_film_norm, per_pixel_norm, lens_area, aperture, focal;
+    /* homogeneous medium filling the scene (homogeneous.h:11-66), HG phase */
+    int has_medium;
+    double sig_t[3], sig_s[3], hg_g;
 };
 
 typedef struct { double c[3]; uint32_t tri; } bref;
@@ -817,6 +820,66 @@ static uint32_t closest(const orc_scene *s, v3 o, v3 d, double *t, counters *cn)
     return tri;
 }
 
+/* ------------------------------------------------------------------ medium */
+/* HomogeneousMedium (homogeneous.h:11-66): transmittance exp(-sigma_t d) per
+ * channel; free-flight distances sampled on one channel chosen uniformly,
+ * with the channel-averaged density (homogeneous.h:36-50): sigma_t Tr when the
+ * flight ends in the medium, Tr when it passes the segment. */
+static inline rgb tr_rgb(const orc_scene *s, double d) {
+    return RGB(exp(-s->sig_t[0] * d), exp(-s->sig_t[1] * d), exp(-s->sig_t[2] * d));
+}
+static inline double pass_pdf(const orc_scene *s, double d) {
+    if (!s->has_medium) return 1.0;
+    rgb t = tr_rgb(s, d);
+    return (t.r + t.g + t.b) / 3.0;
+}
+static inline double scatter_pdf(const orc_scene *s, double d) {
+    rgb t = tr_rgb(s, d);
+    return (s->sig_t[0] * t.r + s->sig_t[1] * t.g + s->sig_t[2] * t.b) / 3.0;
+}
+/* Henyey-Greenstein (1941): p(cos) = (1 - g^2) / (4 pi (1 + g^2 - 2 g cos)^{3/2}), cos
+ * between the propagation direction and the scattered direction (g > 0: forward).
+ * The reference's HenyeyGreenstein cannot serve (henyengreenstein.h:21-51 throws in
+ * evalPdfW and never sets the sampling pdf); this is the textbook function. */
+static double hg_pdf(double g, double c) {
+    double d = 1.0 + g * g - 2.0 * g * c;
+    return (1.0 - g * g) / (4.0 * PI * d * sqrt(d));
+}
+/* inversion of the HG CDF: cos = (1 + g^2 - s^2) / (2 g), s = (1 - g^2) / (1 - g + 2 g u) */
+static v3 hg_sample(double g, v3 fwd, double u0, double u1) {
+    double c;
+    if (fabs(g) < 1e-3) c = 1.0 - 2.0 * u0;
+    else {
+        double q = (1.0 - g * g) / (1.0 - g + 2.0 * g * u0);
+        c = (1.0 + g * g - q * q) / (2.0 * g);
+    }
+    if (c > 1.0) c = 1.0;
+    if (c < -1.0) c = -1.0;
+    double st = sqrt(fmax(0.0, 1.0 - c * c));
+    double phi = 2.0 * PI * u1;
+    frame3 f = make_frame(fwd);
+    return vnorm(to_world(&f, V(st * cos(phi), c, st * sin(phi))));
+}
+int orc_scene_set_medium(orc_scene *s, const double sigma_a[3], const double sigma_s[3], double g) {
+    if (!s) return fail("null scene");
+    if (!(g > -1.0 && g < 1.0)) return fail("HG g must be in (-1, 1)");
+    int any = 0;
+    for (int c = 0; c < 3; c++) {
+        if (!(sigma_a[c] >= 0.0) || !(sigma_s[c] >= 0.0)) return fail("negative medium coefficient");
+        s->sig_s[c] = sigma_s[c];
+        s->sig_t[c] = sigma_a[c] + sigma_s[c];
+        any |= s->sig_t[c] > 0.0;
+    }
+    s->hg_g = g;
+    s->has_medium = any;
+    return 0;
+}
+double orc_hg_pdf(double g, double cos_theta) { return hg_pdf(g, cos_theta); }
+void orc_hg_sample(double g, const double fwd[3], const double u[2], double out[3]) {
+    v3 d = hg_sample(g, V(fwd[0], fwd[1], fwd[2]), u[0], u[1]);
+    out[0] = d.x; out[1] = d.y; out[2] = d.z;
+}
+
 /* ------------------------------------------------------------------ the estimator */
 /* pathnextevent.h:41-150 with the random walk of generalsubpathsampler.h:13-160.
  * base[3] receives splats[0].mContrib; sp_px, sp_py and sp[3] receive splats[1]. */
@@ -855,11 +918,14 @@ static void eval_contribution(const orc_scene *s, int N, int use_mis, uint32_t k
                             v3 dl = to_local(&s->cam_frame, wcam);
                             double c = dl.z;
                             double pdfW = s->per_pixel_norm / (c * c * c);
-                            double camPdfA = pdfW * fabs(vdot(l.n, vmul(nuv, -1.0))) / len2;
+                            double camPdfA = pdfW * fabs(vdot(l.n, vmul(nuv, -1.0))) / len2 * pass_pdf(s, len);
                             mis = l.pdfA / (l.pdfA + camPdfA);
                         }
                         double k = mis * we1 * G * le1;
-                        sp[0] = k * l.contrib.r; sp[1] = k * l.contrib.g; sp[2] = k * l.contrib.b;
+                        /* with a medium the connection is attenuated (the reference omits this: the
+                         * camera vertex has no medium, generalsubpathsampler.h:302) */
+                        rgb tr = s->has_medium ? tr_rgb(s, len) : RGB(1, 1, 1);
+                        sp[0] = k * l.contrib.r * tr.r; sp[1] = k * l.contrib.g * tr.g; sp[2] = k * l.contrib.b * tr.b;
                         *sp_px = (int)raster[0]; *sp_py = (int)raster[1];
                         *sp_ok = 1;
                     }
@@ -879,25 +945,84 @@ static void eval_contribution(const orc_scene *s, int N, int use_mis, uint32_t k
     int cNum = 1;
     int count = 0;
     for (int bounce = 0;; bounce++) {
-        double t;
+        double t = INFINITY;
         uint32_t tri = closest(s, ro, rd, &t, cn);
-        if (tri == UINT32_MAX) break; /* no envmap: callback(nullptr) -> stop */
-        /* hit vertex (embreeaccel.h:398-420) */
+        /* free flight through the medium (generalsubpathsampler.h:40-53, homogeneous.h:36-50):
+         * the 2 medium draws of the bounce pick a channel and a distance */
+        int in_medium = 0;
+        rgb seg = RGB(1, 1, 1);
+        double seg_pdf = 1.0;
+        if (s->has_medium) {
+            uint32_t dm = DIM_BOUNCE0 + DIMS_PER_BOUNCE * (uint32_t)bounce + BD_MEDIUM;
+            double um = u01(key, dm), uc = u01(key, dm + 1);
+            int ch = (int)(uc * 3.0);
+            if (ch > 2) ch = 2;
+            double tf = s->sig_t[ch] > 0.0 ? -log(1.0 - um) / s->sig_t[ch] : INFINITY;
+            if (tf < (tri == UINT32_MAX ? INFINITY : t)) {
+                in_medium = 1;
+                t = tf;
+                seg_pdf = scatter_pdf(s, tf);
+                seg = cscale(cmul(rgb3(s->sig_s), tr_rgb(s, tf)), 1.0 / seg_pdf);
+            } else if (tri != UINT32_MAX) {
+                seg_pdf = pass_pdf(s, t);
+                seg = cscale(tr_rgb(s, t), 1.0 / seg_pdf);
+            }
+        }
+        if (!in_medium && tri == UINT32_MAX) break; /* no envmap: callback(nullptr) -> stop */
         v3 p = vadd(ro, vmul(rd, t));
+        rgb path = cmul(cmul(prev_contrib, prev_path), seg);
+        cNum++;
+        if (in_medium) {
+            /* medium vertex: NEE through the phase function (no cosine at this vertex) */
+            if (cNum == N) break;
+            v3 uv = vsub(l.pos, p);
+            double len2 = vlen2(uv);
+            double len = sqrt(len2);
+            v3 nuv = vmul(uv, 1.0 / len);
+            double G = fabs(vdot(l.n, nuv)) / len2;
+            if (G != 0.0) {
+                int occ = occluded(s, p, nuv, RAY_EPS, len - RAY_EPS, cn);
+                if (!occ) {
+                    double ph = hg_pdf(s->hg_g, vdot(rd, nuv));
+                    v3 wl = to_local(&l.f, vnorm(vsub(p, l.pos)));
+                    double le1 = (wl.y < 0.0) ? 0.0 : INV_PI;
+                    if (le1 != 0.0) {
+                        double mis = 1.0;
+                        if (use_mis) {
+                            double phPdfA = ph * fabs(vdot(l.n, nuv)) / len2 * pass_pdf(s, len);
+                            mis = lPdfA / (lPdfA + phPdfA);
+                        }
+                        rgb c = cmul(cmul(path, l.contrib), tr_rgb(s, len));
+                        double k = mis * le1 * G * ph;
+                        base[0] += k * c.r; base[1] += k * c.g; base[2] += k * c.b;
+                    }
+                }
+            }
+            if (++count == N - 1) break;
+            /* phase sampling: HG is sampled exactly, weight 1 */
+            uint32_t dim = DIM_BOUNCE0 + DIMS_PER_BOUNCE * (uint32_t)bounce + BD_BSDF;
+            v3 nd = hg_sample(s->hg_g, rd, u01(key, dim), u01(key, dim + 1));
+            prevPdfW = use_mis ? hg_pdf(s->hg_g, vdot(rd, nd)) : 0.0;
+            prev_contrib = RGB(1, 1, 1);
+            prev_path = path;
+            prev_pos = p;
+            ro = p;
+            rd = nd;
+            continue;
+        }
+        /* hit vertex (embreeaccel.h:398-420) */
         v3 p0 = tri_v(s, tri, 0), p1 = tri_v(s, tri, 1), p2 = tri_v(s, tri, 2);
         v3 ng = vnorm(vcross(vsub(p1, p0), vsub(p2, p0)));
         frame3 f = make_frame(ng);
         uint32_t g = s->tri_geom[tri];
-        rgb path = cmul(prev_contrib, prev_path);
         /* area pdf of the sampling that produced this vertex (path.h:47-60) */
         double cPdfA = prevPdfW;
         if (use_mis) {
             v3 uv = vsub(p, prev_pos);
             double len2 = vlen2(uv);
             v3 nuv = vmul(uv, 1.0 / sqrt(len2));
-            cPdfA *= fabs(vdot(ng, nuv)) / len2;
+            cPdfA *= fabs(vdot(ng, nuv)) / len2 * seg_pdf;
         }
-        cNum++;
         int is_light = s->geom_light[g] >= 0;
         if (!is_light && cNum != N) {
             /* NEE (pathnextevent.h:106-127) */
@@ -919,10 +1044,11 @@ static void eval_contribution(const orc_scene *s, int N, int use_mis, uint32_t k
                         if (le1 != 0.0) {
                             double mis = 1.0;
                             if (use_mis) {
-                                double bsdfPdfA = bsdf_pdf(b, wi, wo) * fabs(vdot(l.n, nuv)) / len2;
+                                double bsdfPdfA = bsdf_pdf(b, wi, wo) * fabs(vdot(l.n, nuv)) / len2 * pass_pdf(s, len);
                                 mis = lPdfA / (lPdfA + bsdfPdfA);
                             }
                             rgb c = cmul(cmul(path, fs), l.contrib);
+                            if (s->has_medium) c = cmul(c, tr_rgb(s, len));
                             double k = mis * le1 * G;
                             base[0] += k * c.r; base[1] += k * c.g; base[2] += k * c.b;
                         }

@@ -25,6 +25,12 @@
  *                    direct illumination of a plane by a polygonal luminaire
  *                    (Lambert's formula), directly-visible emitter radiance,
  *                    furnace-style energy bound.
+ *   medium, HG     : HG normalisation and mean cosine g (quadrature / Monte
+ *                    Carlo), free-flight mean 1/sigma_t; renders: a purely
+ *                    absorbing medium seen against an emitting enclosure gives
+ *                    L exp(-sigma_t d) per pixel, a purely scattering one in an
+ *                    emitting enclosure gives L (radiative equilibrium), MIS
+ *                    on/off agreement in a scattering medium.
  * The full integrator of the reference cannot be built here (its accelerator,
  * Embree 3.5.2, needs CMake-generated headers), so there are no reference film
  * fixtures; render_nee is pinned by the closed forms above.
@@ -101,6 +107,14 @@ const char *orc_last_error(void);
  * image = base / spp_total + splat / (spp_total * W * H)   (pathnextevent.h:154-157) */
 int orc_render_nee(const orc_scene *s, const orc_render_params *p,
                    double *film_base, double *film_splat, orc_stats *stats);
+
+/* A homogeneous medium filling the whole scene (src/medium/homogeneous.h:11-66,
+ * JSON "medium" {"sigma_a", "sigma_s"}, simple2parser.h:163-167) with a
+ * Henyey-Greenstein phase function of asymmetry g (0 = isotropic,
+ * src/phasefunction/isotropic.h).  All-zero coefficients remove it. */
+int orc_scene_set_medium(orc_scene *s, const double sigma_a[3], const double sigma_s[3], double g);
+double orc_hg_pdf(double g, double cos_theta);
+void orc_hg_sample(double g, const double fwd[3], const double u[2], double out[3]);
 
 /* Rays are 8 floats: o.xyz, tmin, d.xyz, tmax.  Hit interval is (tmin, tmax]. */
 int orc_trace_closest(const orc_scene *s, const float *rays, uint32_t n,

@@ -6,8 +6,8 @@ Bars, written per test:
     rounding points: relative L2 error < 2e-3, max abs error < 3e-2;
   - gradient: relative L2 error < 1e-2 per layer and cosine > 0.9999 overall
     (the oracle's own gradient is pinned by finite differences);
-  - Adam: parameters after 3 steps within 1e-4 of the oracle's where the
-    gradient is not tiny (|g| > 1e-3 max|g|);
+  - Adam: parameters after each of 3 steps within 2e-6 of the oracle's
+    bias-corrected update fed with the GPU's gradient;
   - mixture: pdf rtol 1e-4; samples from the same uniforms within 2e-3 away
     from lobe-selection boundaries;
   - guided render: unbiased for any network -- the image mean within 4 sigma
@@ -101,6 +101,10 @@ def test_gradient_matches_oracle(guide_scene):
 
 
 def test_adam_steps_match_oracle(guide_scene):
+    """The Adam kernel against the bias-corrected update of the oracle, fed with
+    the GPU's own gradient at each step (the gradient is pinned separately;
+    Adam's per-element normalisation would amplify BF16 noise in elements with
+    tiny gradients into +-lr differences)."""
     sc, gs = guide_scene
     print("guide adam: 3 steps on 256 records", flush=True)
     g = wr.Guide(gs, learning_rate=1e-2)
@@ -109,19 +113,16 @@ def test_adam_steps_match_oracle(guide_scene):
     g.set_params(p)
     box = scene_box(sc)
     pos, nrm, wo, wi, w = records(rng, 256, box)
-    g.train(pos, nrm, wo, wi, w, steps=3)
-    got = g.get_params().astype(np.float64)
     q, m, v = p.astype(np.float64), np.zeros(G.NUM_PARAMS), np.zeros(G.NUM_PARAMS)
-    big = np.zeros(G.NUM_PARAMS, bool)
     for t in (1, 2, 3):
-        _, gr = G.loss_and_grad(q.astype(np.float32), pos, nrm, wo, wi, w, box, bf=True)
-        big |= np.abs(gr) > 1e-3 * np.abs(gr).max()
-        q, m, v = G.adam_step(q, m, v, gr, t, lr=1e-2)
-    moved = np.abs(got - p) > 0
-    assert moved.mean() > 0.5
-    err = np.abs(got - q)[big]
-    print(f"  max |param - oracle| over {big.sum()} params: {err.max():.2e}")
-    assert err.max() < 1e-4 * 3 + 1e-5
+        _, grad = g.gradient(pos, nrm, wo, wi, w)
+        g.train(pos, nrm, wo, wi, w, steps=1)
+        q, m, v = G.adam_step(q, m, v, grad.astype(np.float64), t, lr=1e-2)
+        got = g.get_params().astype(np.float64)
+        err = np.abs(got - q).max()
+        print(f"  step {t}: max |param - oracle| {err:.2e}, max step {np.abs(got - p).max():.2e}")
+        assert err < 2e-6
+    assert (np.abs(g.get_params() - p) > 0).mean() > 0.5
 
 
 def test_mixture_matches_oracle(guide_scene):

@@ -98,6 +98,10 @@ def test_mitsuba_srgb_converts_reflectances(tmp_path):
     (lambda j: j["camera"].update(fovy=200), "fov"),
     (lambda j: j["camera"].update(resolution=[4]), "can't parse resolution"),
     (lambda j: j.update(medium={"type": "fog"}), "unknown medium"),
+    (lambda j: j.update(medium={"type": "homogeneous", "sigma_a": [0.1, 0.1], "sigma_s": 0.4}), "can't parse sigma_a"),
+    (lambda j: j.update(medium={"type": "homogeneous", "sigma_a": -1, "sigma_s": 0.4}), "must be >= 0"),
+    (lambda j: j.update(medium={"type": "homogeneous", "sigma_a": 0.1, "sigma_s": 0.4, "g": 1.0}), "(-1, 1)"),
+    (lambda j: j["render"][0].update(guiding={"guide_fraction": 1.5}), "guide_fraction"),
 ])
 def test_errors_are_reported(tmp_path, mutate, msg):
     p = write(tmp_path, scenes.cornell(1, resolution=(16, 16)))
@@ -131,3 +135,17 @@ def test_no_lights_is_an_error(tmp_path):
 def test_default_txt_is_created_when_missing(tmp_path):
     r = run([], cwd=str(tmp_path))
     assert r.returncode == 0 and (tmp_path / "default.txt").exists()
+
+
+def test_medium_and_guiding_keys_are_accepted(tmp_path):
+    sc = scenes.cornell(1, resolution=(16, 16)).with_medium(sigma_a=(0.05, 0.05, 0.05), sigma_s=0.4, g=0.7)
+    p = write(tmp_path, sc, renders=(dict(NEE, guiding={"guide_fraction": 0.5, "train_steps": 2}),))
+    j = json.loads(p.read_text())
+    assert j["medium"] == {"type": "homogeneous", "sigma_a": [0.05] * 3, "sigma_s": [0.4] * 3, "g": 0.7}
+    r = run(["--check", "--simple2parse", str(p)])
+    assert r.returncode == 0, r.stderr
+    # a single number broadcasts (Vec3FromJson, json.h:9-15); "none" is no medium
+    for med in ({"type": "homogeneous", "sigma_a": 0.1, "sigma_s": 0.2}, {"type": "none"}):
+        j["medium"] = med
+        p.write_text(json.dumps(j))
+        assert run(["--check", "--simple2parse", str(p)]).returncode == 0

@@ -8,9 +8,11 @@ raise.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
 import subprocess
+import weakref
 
 import numpy as np
 
@@ -60,6 +62,10 @@ class RenderStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class Medium(C.Structure):
+    _fields_ = [("sigma_a", C.c_float * 3), ("sigma_s", C.c_float * 3), ("g", C.c_float)]
+
+
 class GuideConfig(C.Structure):
     _fields_ = [("learning_rate", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("epsilon", C.c_float),
                 ("batch_size", C.c_uint32), ("record_capacity", C.c_uint32), ("seed", C.c_uint32)]
@@ -95,6 +101,7 @@ EXPORTS = {
     "wr_scene_destroy": (None, [P]),
     "wr_scene_info": (C.c_int, [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint32),
                                 C.POINTER(C.c_uint64)]),
+    "wr_scene_set_medium": (C.c_int, [P, C.POINTER(Medium)]),
     "wr_film_create": (C.c_int, [P, C.POINTER(P)]),
     "wr_film_destroy": (None, [P]),
     "wr_film_clear": (C.c_int, [P]),
@@ -141,6 +148,19 @@ def lib():
     return _lib
 
 
+# Live handles, released at interpreter exit in dependency order (guides and
+# films before their scene, scenes before their device) while the HIP runtime
+# is still intact, rather than in whatever order finalisation reaches them.
+_LIVE = {k: weakref.WeakSet() for k in ("guide", "film", "scene", "device")}
+
+
+@atexit.register
+def _release_all():
+    for kind in ("guide", "film", "scene", "device"):
+        for obj in list(_LIVE[kind]):
+            obj.close()
+
+
 def _check(rc: int):
     if rc != WR_OK:
         raise WrError(rc, lib().wr_last_error().decode())
@@ -151,11 +171,16 @@ class Device:
         h = C.c_void_p()
         _check(lib().wr_device_create(ordinal, C.byref(h)))
         self.h = h
+        _LIVE["device"].add(self)
 
-    def __del__(self):
+    def close(self):
+        """Releases the handle (idempotent)."""
         if getattr(self, "h", None):
             lib().wr_device_destroy(self.h)
             self.h = None
+
+    def __del__(self):
+        self.close()
 
 
 def scene_desc(sc: scenes.Scene):
@@ -196,14 +221,29 @@ class GpuScene:
         h = C.c_void_p()
         _check(lib().wr_scene_create(device.h, C.byref(d), C.byref(h)))
         self.h = h
+        _LIVE["scene"].add(self)
         self.device = device
         self.W, self.H = sc.resolution
         del keep
+        if getattr(sc, "medium", None):
+            self.set_medium(sc.medium["sigma_a"], sc.medium["sigma_s"], sc.medium["g"])
 
-    def __del__(self):
+    def set_medium(self, sigma_a=(0, 0, 0), sigma_s=(0, 0, 0), g=0.0):
+        """Homogeneous medium filling the scene with an HG phase function (zeros: vacuum)."""
+        m = Medium()
+        m.sigma_a[:] = [float(x) for x in sigma_a]
+        m.sigma_s[:] = [float(x) for x in sigma_s]
+        m.g = float(g)
+        _check(lib().wr_scene_set_medium(self.h, C.byref(m)))
+
+    def close(self):
+        """Releases the handle (idempotent)."""
         if getattr(self, "h", None):
             lib().wr_scene_destroy(self.h)
             self.h = None
+
+    def __del__(self):
+        self.close()
 
     def info(self) -> dict:
         nn, nt, nb = C.c_uint64(), C.c_uint64(), C.c_uint64()
@@ -235,13 +275,18 @@ class Film:
         h = C.c_void_p()
         _check(lib().wr_film_create(scene.h, C.byref(h)))
         self.h = h
+        _LIVE["film"].add(self)
         self.scene = scene
         self.spp = 0
 
-    def __del__(self):
+    def close(self):
+        """Releases the handle (idempotent)."""
         if getattr(self, "h", None):
             lib().wr_film_destroy(self.h)
             self.h = None
+
+    def __del__(self):
+        self.close()
 
     def clear(self):
         _check(lib().wr_film_clear(self.h))
@@ -300,12 +345,17 @@ class Guide:
         h = C.c_void_p()
         _check(lib().wr_guide_create(scene.h, C.byref(cfg), C.byref(h)))
         self.h = h
+        _LIVE["guide"].add(self)
         self.scene = scene
 
-    def __del__(self):
+    def close(self):
+        """Releases the handle (idempotent)."""
         if getattr(self, "h", None):
             lib().wr_guide_destroy(self.h)
             self.h = None
+
+    def __del__(self):
+        self.close()
 
     def get_params(self) -> np.ndarray:
         p = np.empty(GUIDE_NUM_PARAMS, dtype=np.float32)

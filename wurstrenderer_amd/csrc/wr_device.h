@@ -45,6 +45,14 @@ struct DCamera {
     float aperture, focal;
 };
 
+// Homogeneous medium filling the scene (homogeneous.h:11-66) with a
+// Henyey-Greenstein phase function; on == 0: vacuum.
+struct DMedium {
+    float sig_t[3], sig_s[3];
+    float g;
+    int on;
+};
+
 struct DScene {
     const Node4 *nodes;
     const float4 *tris;       // 3 float4 per leaf-order triangle
@@ -54,6 +62,7 @@ struct DScene {
     const float4 *ltris;      // 3 float4 per emitter triangle
     uint32_t n_lights;
     DCamera cam;
+    DMedium med;
 };
 
 // ---------------------------------------------------------------- float3 helpers
@@ -89,7 +98,7 @@ __device__ __forceinline__ uint32_t path_key(uint32_t seed, uint32_t pixel, uint
 __device__ __forceinline__ float u01(uint32_t key, uint32_t dim) {
     return float(mix32(key + dim * 0x9e3779b9u + 0x632be5abu) >> 8) * 0x1p-24f;
 }
-enum : uint32_t { kDimLight = 0, kDimLens = 2, kDimPixel = 4, kDimBounce0 = 6, kDimsPerBounce = 6, kBdBsdf = 2 };
+enum : uint32_t { kDimLight = 0, kDimLens = 2, kDimPixel = 4, kDimBounce0 = 6, kDimsPerBounce = 6, kBdMedium = 0, kBdBsdf = 2 };
 
 // ---------------------------------------------------------------- frames and warps
 struct Frame { float3 x, y, z; };
@@ -294,6 +303,53 @@ __device__ __forceinline__ LightSample sample_light(const DScene &s, float u0, f
     ls.contrib = f3(L.L[0] * k, L.L[1] * k, L.L[2] * k);
     ls.pdfA = pdfL * L.inv_area;
     return ls;
+}
+
+// ---------------------------------------------------------------- medium (homogeneous.h:11-66)
+__device__ __forceinline__ float3 transmittance(const DMedium &m, float d) {
+    return f3(expf(-m.sig_t[0] * d), expf(-m.sig_t[1] * d), expf(-m.sig_t[2] * d));
+}
+// Channel-averaged free-flight densities (homogeneous.h:36-50): passing a
+// segment of length d, and ending the flight at distance d.
+__device__ __forceinline__ float pass_pdf(const DMedium &m, float d) {
+    const float3 t = transmittance(m, d);
+    return (t.x + t.y + t.z) * (1.f / 3.f);
+}
+__device__ __forceinline__ float scatter_pdf(const DMedium &m, float d) {
+    const float3 t = transmittance(m, d);
+    return (m.sig_t[0] * t.x + m.sig_t[1] * t.y + m.sig_t[2] * t.z) * (1.f / 3.f);
+}
+// Free flight along a segment whose surface (or infinity) is t_surf away: the
+// bounce's two medium draws pick a channel and a distance.  Returns true and
+// the distance when the flight ends in the medium.
+__device__ __forceinline__ bool free_flight(const DMedium &m, uint32_t key, uint32_t bounce, float t_surf, float &t) {
+    const uint32_t dm = kDimBounce0 + kDimsPerBounce * bounce + kBdMedium;
+    const float um = u01(key, dm), uc = u01(key, dm + 1);
+    const int ch = min(int(uc * 3.f), 2);
+    const float st = ch == 0 ? m.sig_t[0] : (ch == 1 ? m.sig_t[1] : m.sig_t[2]);
+    const float tf = st > 0.f ? -logf(1.f - um) / st : INFINITY;
+    t = tf;
+    return tf < t_surf;
+}
+// Henyey-Greenstein (1941): cos between the propagation direction and the
+// scattered direction; g > 0 scatters forward.
+__device__ __forceinline__ float hg_pdf(float g, float c) {
+    const float d = 1.f + g * g - 2.f * g * c;
+    return (1.f - g * g) / (4.f * kPi * d * sqrtf(d));
+}
+// inversion of the HG CDF: cos = (1 + g^2 - s^2) / (2 g), s = (1 - g^2) / (1 - g + 2 g u)
+__device__ __forceinline__ float3 hg_sample(float g, float3 fwd, float u0, float u1) {
+    float c;
+    if (fabsf(g) < 1e-3f) c = 1.f - 2.f * u0;
+    else {
+        const float q = (1.f - g * g) / (1.f - g + 2.f * g * u0);
+        c = (1.f + g * g - q * q) / (2.f * g);
+    }
+    c = fminf(fmaxf(c, -1.f), 1.f);
+    const float st = sqrtf(fmaxf(0.f, 1.f - c * c));
+    float s, co;
+    sincospif(2.f * u1, &s, &co);
+    return normalize3(to_world(make_frame(fwd), f3(st * co, c, st * s)));
 }
 
 // ---------------------------------------------------------------- camera (thinlens.h, camera.h)

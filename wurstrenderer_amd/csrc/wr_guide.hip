@@ -133,11 +133,16 @@ __global__ __launch_bounds__(64) void k_train(Net net, TrainArgs a) {
 #pragma unroll
                     for (int v = 0; v < 16; v++) dyt[(32 * mt + drow(v, h)) * kXt + 32 * nt + p] = bf(dacc[mt][nt][v]);
             wave_sync_lds();
-            // bias gradient: row sums of dY^T
+            // bias gradient: row sums of dY^T (16-byte reads, independent partial sums)
             if (lane < N) {
-                float s = 0.f;
-                for (int r = 0; r < 64; r++) s += fb(dyt[lane * kXt + r]);
-                gacc[b_offset(l) + lane] += s;
+                float s[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int r = 0; r < 64; r += 8) {
+                    const bf16x8 v = *reinterpret_cast<const bf16x8 *>(dyt + lane * kXt + r);
+#pragma unroll
+                    for (int e = 0; e < 8; e++) s[e & 3] += fb(v[e]);
+                }
+                gacc[b_offset(l) + lane] += (s[0] + s[1]) + (s[2] + s[3]);
             }
             // weight gradient dW[i][o] = sum_r X[r][i] dY[r][o]: MFMA over the records
             const __bf16 *x = xt + l * 64 * kXt;

@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <map>
 #include <string>
 #include <vector>
 
@@ -93,7 +94,8 @@ struct wr_device {
     int ordinal = 0;
     int n_cu = 256;
     // resident blocks per CU of each kernel (HIP occupancy API), used to size persistent grids
-    int occ_extend = 8, occ_shadow = 8, occ_shade = 8, occ_generate = 8, occ_trace = 8, occ_shade_guided = 4, occ_infer = 8;
+    int occ_extend = 8, occ_shadow = 8, occ_shade = 8, occ_generate = 8, occ_trace = 8;
+    std::map<const void *, int> occ_cache;  // other kernels, filled on first use
 };
 
 struct wr_scene {

@@ -177,14 +177,17 @@ __global__ __launch_bounds__(kShadeBlock) void k_regen(DScene s, PathBuffers pb,
                         float mis = 1.f;
                         if (a.use_mis) {
                             const float c = cam_to_local(cam, wcam).z;
-                            const float camPdfA = cam.per_pixel_norm / (c * c * c) * fabsf(cos_l) / len2;
+                            float camPdfA = cam.per_pixel_norm / (c * c * c) * fabsf(cos_l) / len2;
+                            if (s.med.on) camPdfA *= pass_pdf(s.med, len);
                             mis = l.pdfA / (l.pdfA + camPdfA);
                         }
                         const float k = mis * we1 * G * kInvPi;
+                        float3 c = l.contrib * k;
+                        if (s.med.on) c = c * transmittance(s.med, len);  // attenuated connection
                         const uint32_t target = kToSplat | (uint32_t(ry) * uint32_t(cam.W) + uint32_t(rx));
                         so = make_float4(l.pos.x, l.pos.y, l.pos.z, len - kRayEps);
                         sd = make_float4(nuv.x, nuv.y, nuv.z, __uint_as_float(target));
-                        sc = make_float4(k * l.contrib.x, k * l.contrib.y, k * l.contrib.z, 0.f);
+                        sc = make_float4(c.x, c.y, c.z, 0.f);
                         splat = true;
                     }
                 }
@@ -426,24 +429,35 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(DScene s, PathBuffers pb,
     }
 }
 
-// ============================================================== guided shade
-// k_shade with learned guiding: the direction at a surface vertex is drawn by
-// one-sample MIS from the network's lobe mixture (probability alpha) or the
-// BSDF, weighted by f cos / (alpha q + (1 - alpha) p_bsdf); the same combined
-// density enters the NEE and emitter-hit MIS weights, so the estimator stays
-// unbiased for any network.  The lobes come from k_guide_infer.  Selected
-// paths append a training record at every surface vertex; the record is
-// closed when the path's next vertex is shaded (radiance snapshot).
+// ============================================================== extended shade
+// k_shade with learned guiding and/or a homogeneous medium.
+//
+// Guiding: the direction at a vertex is drawn by one-sample MIS from the
+// network's lobe mixture (probability alpha) or the BSDF / phase function,
+// weighted by f / (alpha q + (1 - alpha) p); the same combined density enters
+// the NEE and emitter-hit MIS weights, so the estimator stays unbiased for any
+// network.  The lobes come from k_guide_infer.  Selected paths append a
+// training record at every scattering vertex; the record is closed when the
+// path's next vertex is shaded (radiance snapshot).
+//
+// Medium (homogeneous.h:11-66, generalsubpathsampler.h:40-53,131-154): each
+// segment first samples a free flight on one colour channel; a flight that ends
+// before the surface makes a medium vertex (throughput sigma_s Tr / pdf, NEE
+// through the HG phase function, phase sampling), otherwise the surface is
+// reached with throughput Tr / P(pass).  Shadow contributions carry the
+// transmittance of their segment, and the MIS densities the pass probability.
 __device__ __forceinline__ float guide_alpha(const DMat &m, float frac) {
     // sharp Phong lobes are sampled well by the BSDF itself
     return m.type == kPhong ? frac * fminf(1.f, fmaxf(0.1f, 30.f / m.n)) : frac;
 }
 
-// Network inference for the current state: the decoded lobes of every surface
-// vertex that k_shade_guided will shade (not an emitter, not the last vertex),
-// one forward pass per wave of 64 entries on MFMA.
-__global__ __launch_bounds__(kBlock) void k_guide_infer(const float4 *__restrict__ tris, const int2 *__restrict__ geom_info,
-                                                        PathState cs, const uint2 *__restrict__ hit,
+// Network inference for the current state: the decoded lobes of every vertex
+// that k_shade_ext samples a direction at -- a surface that is not an emitter
+// or the last vertex, or (with a medium) a scattering event before the last
+// vertex -- one forward pass per wave of 64 entries on MFMA.  The free-flight
+// decision repeats k_shade_ext's (same random numbers).
+template <bool MEDIUM>
+__global__ __launch_bounds__(kBlock) void k_guide_infer(DScene s, PathState cs, const uint2 *__restrict__ hit,
                                                         const uint32_t *__restrict__ counters, int N, guide::Net net,
                                                         float4 *lobes, uint32_t stride) {
     __shared__ float stage[64 * guide::kStageStride];
@@ -454,16 +468,27 @@ __global__ __launch_bounds__(kBlock) void k_guide_infer(const float4 *__restrict
         float3 pos = f3(0.f, 0.f, 0.f), nf = f3(0.f, 1.f, 0.f), rd = f3(0.f, 0.f, 1.f);
         if (i < n) {
             const uint2 h = hit[i];
-            if (h.x != kInvalid && int(cs.meta[i].z) + 2 != N) {
-                const float4 v0 = tris[3 * size_t(h.x) + 0];
-                const float4 v1 = tris[3 * size_t(h.x) + 1];
-                const float4 v2 = tris[3 * size_t(h.x) + 2];
-                if (geom_info[__float_as_uint(v1.w)].y < 0) {
+            const uint4 meta = cs.meta[i];
+            if (int(meta.z) + 2 != N) {
+                float tf = 0.f;
+                const bool scatter =
+                    MEDIUM && free_flight(s.med, meta.x, meta.z, h.x != kInvalid ? __uint_as_float(h.y) : INFINITY, tf);
+                if (scatter) {
                     need = true;
                     rd = xyz(cs.ray_d[i]);
-                    pos = xyz(cs.ray_o[i]) + rd * __uint_as_float(h.y);
-                    const float3 ng = normalize3(cross3(xyz(v1) - xyz(v0), xyz(v2) - xyz(v0)));
-                    nf = dot3(ng, rd) > 0.f ? -ng : ng;
+                    pos = xyz(cs.ray_o[i]) + rd * tf;
+                    nf = -rd;  // no surface: the incoming direction stands in for the normal
+                } else if (h.x != kInvalid) {
+                    const float4 v0 = s.tris[3 * size_t(h.x) + 0];
+                    const float4 v1 = s.tris[3 * size_t(h.x) + 1];
+                    const float4 v2 = s.tris[3 * size_t(h.x) + 2];
+                    if (s.geom_info[__float_as_uint(v1.w)].y < 0) {
+                        need = true;
+                        rd = xyz(cs.ray_d[i]);
+                        pos = xyz(cs.ray_o[i]) + rd * __uint_as_float(h.y);
+                        const float3 ng = normalize3(cross3(xyz(v1) - xyz(v0), xyz(v2) - xyz(v0)));
+                        nf = dot3(ng, rd) > 0.f ? -ng : ng;
+                    }
                 }
             }
         }
@@ -480,10 +505,12 @@ __global__ __launch_bounds__(kBlock) void k_guide_infer(const float4 *__restrict
     }
 }
 
-__global__ __launch_bounds__(kShadeBlock) void k_shade_guided(DScene s, PathBuffers pb, RenderArgs a, guide::Shade g) {
+template <bool GUIDED, bool MEDIUM>
+__global__ __launch_bounds__(kShadeBlock) void k_shade_ext(DScene s, PathBuffers pb, RenderArgs a, guide::Shade g) {
     __shared__ uint32_t agg[kWavesPerShadeBlock + 1];
-    __shared__ float lobes[kShadeBlock * guide::kStageStride];
-    float *lob = lobes + threadIdx.x * guide::kStageStride;
+    // this thread's lobes, staged in LDS (a 40-float register array would live in scratch)
+    __shared__ float lobes[GUIDED ? kShadeBlock * guide::kStageStride : 1];
+    float *lob = lobes + (GUIDED ? threadIdx.x * guide::kStageStride : 0);
     const uint32_t n = active_count(pb.counters);
     const PathState cs = pb.cur, ns = pb.next;
     const float3 zero = f3(0.f, 0.f, 0.f);
@@ -495,12 +522,22 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade_guided(DScene s, PathBuff
         uint2 h = make_uint2(kInvalid, 0u);
         float3 pos = zero, ng = f3(0.f, 1.f, 0.f), rd = f3(0.f, 0.f, 1.f);
         int2 gi = make_int2(0, -1);
+        bool in_medium = false;
+        float seg_t = 0.f;
         if (valid) {
             meta = cs.meta[i];
             h = pb.hit[i];
-            if (h.x != kInvalid) {
-                rd = xyz(cs.ray_d[i]);
-                pos = xyz(cs.ray_o[i]) + rd * __uint_as_float(h.y);
+            rd = xyz(cs.ray_d[i]);
+            seg_t = h.x != kInvalid ? __uint_as_float(h.y) : INFINITY;
+            if (MEDIUM) {
+                float tf = 0.f;
+                in_medium = free_flight(s.med, meta.x, meta.z, seg_t, tf);
+                if (in_medium) seg_t = tf;
+            }
+            if (in_medium) {
+                pos = xyz(cs.ray_o[i]) + rd * seg_t;
+            } else if (h.x != kInvalid) {
+                pos = xyz(cs.ray_o[i]) + rd * seg_t;
                 const float4 v0 = s.tris[3 * size_t(h.x) + 0];
                 const float4 v1 = s.tris[3 * size_t(h.x) + 1];
                 const float4 v2 = s.tris[3 * size_t(h.x) + 2];
@@ -508,75 +545,98 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade_guided(DScene s, PathBuff
                 gi = s.geom_info[__float_as_uint(v1.w)];
             }
         }
-        const bool surf = valid && h.x != kInvalid && gi.y < 0 && int(meta.z) + 2 != a.N;
-        const float3 nf = dot3(ng, rd) > 0.f ? -ng : ng;  // normal facing the incoming ray
-        const bool need = surf && g.alpha > 0.f;
-        // this vertex's lobes, staged in LDS (a 40-float register array would live in scratch)
-        if (need) {
+        const bool hit_vertex = valid && !in_medium && h.x != kInvalid;
+        const bool last = int(meta.z) + 2 == a.N;
+        const bool surf = hit_vertex && gi.y < 0 && !last;
+        const bool med = valid && in_medium && !last;
+        const float3 nf = in_medium ? -rd : (dot3(ng, rd) > 0.f ? -ng : ng);  // faces the incoming ray
+        const bool need = GUIDED && (surf || med) && g.alpha > 0.f;
+        if (GUIDED && need) {
 #pragma unroll
             for (int j = 0; j < guide::kLobeVecs; j++) {
                 const float4 q = g.lobes[size_t(j) * g.lobe_stride + i];
                 lob[4 * j] = q.x; lob[4 * j + 1] = q.y; lob[4 * j + 2] = q.z; lob[4 * j + 3] = q.w;
             }
         }
-        // ---- shading (as k_shade, with the combined density)
+        // ---- shading
         bool push_shadow = false, cont = false;
         float4 so = make_float4(0, 0, 0, 0), sd = so, sc = so;
         float4 new_o = so, new_d = so, new_thr = so, rad = so;
         float3 wdir = zero, T = zero;
         const uint32_t key = meta.x, bounce = meta.z;
+        float prev_pdfw = 0.f;
         if (valid) {
             rad = cs.rad[i];
             // radiance before this vertex closes the record of the previous one
-            if (meta.w) g.rec.rad[meta.w - 1] = rad;
-        }
-        if (valid && h.x != kInvalid) {
+            if (GUIDED && meta.w) g.rec.rad[meta.w - 1] = rad;
             const float4 thr = cs.thr[i];
+            prev_pdfw = thr.w;
             T = xyz(thr);
-            if (gi.y >= 0) {
-                // emitter hit: MIS against light sampling (pathnextevent.h:128-146)
-                const float t = __uint_as_float(h.y);
-                const float cos_l = -dot3(ng, rd);
-                if (cos_l >= 0.f) {
-                    const DLight &L = s.lights[gi.y];
-                    float mis = 0.f;
-                    if (a.use_mis) {
-                        const float cPdfA = thr.w * fabsf(cos_l) / (t * t);
-                        const float lightPdfA = L.pdf_select * L.inv_area;
-                        mis = cPdfA / (lightPdfA + cPdfA);
-                    }
-                    rad.x += mis * T.x * L.L[0];
-                    rad.y += mis * T.y * L.L[1];
-                    rad.z += mis * T.z * L.L[2];
-                }
+            if (MEDIUM && (in_medium || h.x != kInvalid)) {
+                // free-flight weight of the segment
+                const float3 tr = transmittance(s.med, seg_t);
+                if (in_medium) T = T * f3(s.med.sig_s[0], s.med.sig_s[1], s.med.sig_s[2]) * tr * (1.f / scatter_pdf(s.med, seg_t));
+                else T = T * tr * (1.f / pass_pdf(s.med, seg_t));
             }
         }
-        if (surf) {
-            const DMat m = s.mats[gi.x];
-            const Frame f = make_frame(ng);
-            const float3 wi = to_local(f, -rd);
-            const float alpha = need ? guide_alpha(m, g.alpha) : 0.f;
-            // NEE with MIS against the combined density (pathnextevent.h:106-127)
+        if (hit_vertex && gi.y >= 0) {
+            // emitter hit: MIS against light sampling (pathnextevent.h:128-146)
+            const float cos_l = -dot3(ng, rd);
+            if (cos_l >= 0.f) {
+                const DLight &L = s.lights[gi.y];
+                float mis = 0.f;
+                if (a.use_mis) {
+                    float cPdfA = prev_pdfw * fabsf(cos_l) / (seg_t * seg_t);
+                    if (MEDIUM) cPdfA *= pass_pdf(s.med, seg_t);
+                    const float lightPdfA = L.pdf_select * L.inv_area;
+                    mis = cPdfA / (lightPdfA + cPdfA);
+                }
+                rad.x += mis * T.x * L.L[0];
+                rad.y += mis * T.y * L.L[1];
+                rad.z += mis * T.z * L.L[2];
+            }
+        }
+        if (surf || med) {
+            DMat m{};
+            Frame f{};
+            float3 wi = zero;
+            if (surf) {
+                m = s.mats[gi.x];
+                f = make_frame(ng);
+                wi = to_local(f, -rd);
+            }
+            const float alpha = need ? (surf ? guide_alpha(m, g.alpha) : g.alpha) : 0.f;
+            // NEE with MIS against the density that samples directions here (pathnextevent.h:106-127)
             const LightSample l = sample_light(s, u01(key, kDimLight), u01(key, kDimLight + 1));
             const float3 uv = l.pos - pos;
             const float len2 = dot3(uv, uv);
             const float len = sqrtf(len2);
             const float3 nuv = uv * (1.0f / len);
             const float cos_lv = -dot3(l.n, nuv);
-            const float G = fabsf(dot3(ng, nuv)) * fabsf(cos_lv) / len2;
+            const float G = (med ? 1.f : fabsf(dot3(ng, nuv))) * fabsf(cos_lv) / len2;
             if (G != 0.f && cos_lv >= 0.f) {
-                const float3 wo = to_local(f, nuv);
-                const float3 fs = bsdf_eval(m, wi, wo);
+                float3 fs;
+                float p;
+                if (med) {
+                    const float ph = hg_pdf(s.med.g, dot3(rd, nuv));
+                    fs = f3(ph, ph, ph);
+                    p = ph;
+                } else {
+                    const float3 wo = to_local(f, nuv);
+                    fs = bsdf_eval(m, wi, wo);
+                    p = bsdf_pdf(m, wi, wo);
+                }
                 if (!is_black(fs)) {
                     float mis = 1.f;
                     if (a.use_mis) {
-                        float p = bsdf_pdf(m, wi, wo);
-                        if (need) p = alpha * guide::pdf(lob, nuv) + (1.f - alpha) * p;
-                        const float bsdfPdfA = p * fabsf(cos_lv) / len2;
-                        mis = l.pdfA / (l.pdfA + bsdfPdfA);
+                        if (GUIDED && need) p = alpha * guide::pdf(lob, nuv) + (1.f - alpha) * p;
+                        float dirPdfA = p * fabsf(cos_lv) / len2;
+                        if (MEDIUM) dirPdfA *= pass_pdf(s.med, len);
+                        mis = l.pdfA / (l.pdfA + dirPdfA);
                     }
                     const float k = mis * G * kInvPi;
-                    const float3 c = T * fs * l.contrib * k;
+                    float3 c = T * fs * l.contrib * k;
+                    if (MEDIUM) c = c * transmittance(s.med, len);
                     if (!is_black(c)) {
                         so = make_float4(pos.x, pos.y, pos.z, len - kRayEps);
                         sd = make_float4(nuv.x, nuv.y, nuv.z, 0.f);
@@ -585,28 +645,43 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade_guided(DScene s, PathBuff
                     }
                 }
             }
-            // direction: one-sample MIS of network and BSDF
+            // direction: BSDF or phase function, one-sample MIS with the network when guided
             const uint32_t dim = kDimBounce0 + kDimsPerBounce * bounce + kBdBsdf;
             const float u0 = u01(key, dim), u1 = u01(key, dim + 1);
-            float3 wo = zero, w = zero;
+            float3 nd = zero, w = zero;
             float pdf = 0.f;
-            const float us = u01(key, dim + 2);
-            if (need && us < alpha) {
-                const float3 d = guide::sample(lob, us / alpha, u0, u1);
-                wo = to_local(f, d);
-                const float3 fs = bsdf_eval(m, wi, wo);
-                pdf = alpha * guide::pdf(lob, d) + (1.f - alpha) * bsdf_pdf(m, wi, wo);
-                if (pdf > 0.f && !is_black(fs)) w = fs * (fabsf(wo.y) / pdf);
+            const float us = GUIDED ? u01(key, dim + 2) : 1.f;
+            if (GUIDED && need && us < alpha) {
+                nd = guide::sample(lob, us / alpha, u0, u1);
+                float3 fv;
+                float pd;
+                if (med) {
+                    pd = hg_pdf(s.med.g, dot3(rd, nd));
+                    fv = f3(pd, pd, pd);
+                } else {
+                    const float3 wo = to_local(f, nd);
+                    fv = bsdf_eval(m, wi, wo) * fabsf(wo.y);
+                    pd = bsdf_pdf(m, wi, wo);
+                }
+                pdf = alpha * guide::pdf(lob, nd) + (1.f - alpha) * pd;
+                if (pdf > 0.f && !is_black(fv)) w = fv * (1.f / pdf);
             } else {
-                w = bsdf_sample(m, wi, u0, u1, wo, pdf);
-                if (need && !is_black(w)) {
+                if (med) {
+                    nd = hg_sample(s.med.g, rd, u0, u1);
+                    pdf = hg_pdf(s.med.g, dot3(rd, nd));
+                    w = f3(1.f, 1.f, 1.f);
+                } else {
+                    float3 wo = zero;
+                    w = bsdf_sample(m, wi, u0, u1, wo, pdf);
+                    nd = to_world(f, wo);
+                }
+                if (GUIDED && need && !is_black(w)) {
                     const float pbs = pdf;
-                    pdf = alpha * guide::pdf(lob, to_world(f, wo)) + (1.f - alpha) * pbs;
+                    pdf = alpha * guide::pdf(lob, nd) + (1.f - alpha) * pbs;
                     w = w * (pbs / pdf);
                 }
             }
             if (!is_black(w) && pdf > 0.f) {
-                const float3 nd = to_world(f, wo);
                 wdir = nd;
                 new_o = make_float4(pos.x, pos.y, pos.z, 0.f);
                 new_d = make_float4(nd.x, nd.y, nd.z, 0.f);
@@ -615,7 +690,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade_guided(DScene s, PathBuff
             }
         }
         uint32_t rec_id = 0;
-        if (g.collect) {
+        if (GUIDED && g.collect) {
             const bool want = cont && float(mix32(key ^ 0x68bc21ebu) >> 8) * 0x1p-24f < g.record_prob;
             const uint32_t r = block_append(g.rec.count, want, agg);
             if (want && r < g.rec.capacity) {
@@ -746,6 +821,17 @@ __global__ __launch_bounds__(kBlock) void k_trace(DScene s, const float4 *__rest
     }
 }
 
+// Resident blocks per CU of a kernel (HIP occupancy API), cached per device.
+static int occupancy(wr_device *d, const void *fn, int block) {
+    auto it = d->occ_cache.find(fn);
+    if (it != d->occ_cache.end()) return it->second;
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, block, 0) != hipSuccess || nb <= 0) nb = 1;
+    nb = std::min(nb, kMaxTraversalBlocksPerCU);
+    d->occ_cache[fn] = nb;
+    return nb;
+}
+
 // The entry points below inherit C linkage from their declarations in wurst_amd.h.
 
 const char *wr_last_error(void) { return wr::error_string().c_str(); }
@@ -771,8 +857,7 @@ int wr_device_create(int ordinal, wr_device **out) {
     occ(reinterpret_cast<const void *>(&k_extend<false>), kBlock, d->occ_extend);
     occ(reinterpret_cast<const void *>(&k_shadow<false>), kBlock, d->occ_shadow);
     occ(reinterpret_cast<const void *>(&k_shade), kShadeBlock, d->occ_shade);
-    occ(reinterpret_cast<const void *>(&k_shade_guided), kShadeBlock, d->occ_shade_guided);
-    occ(reinterpret_cast<const void *>(&k_guide_infer), kBlock, d->occ_infer);
+
     occ(reinterpret_cast<const void *>(&k_regen), kShadeBlock, d->occ_generate);
     occ(reinterpret_cast<const void *>(&k_trace<false>), kBlock, d->occ_trace);
     *out = d;
@@ -1051,6 +1136,24 @@ int wr_scene_info(const wr_scene *s, uint64_t *n_nodes, uint64_t *n_tris, uint32
     return WR_OK;
 }
 
+int wr_scene_set_medium(wr_scene *s, const wr_medium *m) {
+    if (!s) return set_error(WR_ERR_INVALID_ARGUMENT, "scene is null");
+    DMedium dm{};
+    if (m) {
+        if (!(m->g > -1.f && m->g < 1.f)) return set_error(WR_ERR_INVALID_ARGUMENT, "HG g must be in (-1, 1)");
+        for (int c = 0; c < 3; c++) {
+            if (!(m->sigma_a[c] >= 0.f) || !(m->sigma_s[c] >= 0.f) || !std::isfinite(m->sigma_a[c] + m->sigma_s[c]))
+                return set_error(WR_ERR_INVALID_ARGUMENT, "medium coefficients must be finite and >= 0");
+            dm.sig_s[c] = m->sigma_s[c];
+            dm.sig_t[c] = m->sigma_a[c] + m->sigma_s[c];
+            dm.on |= dm.sig_t[c] > 0.f;
+        }
+        dm.g = m->g;
+    }
+    s->ds.med = dm;
+    return WR_OK;
+}
+
 int wr_film_create(wr_scene *s, wr_film **out) {
     if (!s || !out) return set_error(WR_ERR_INVALID_ARGUMENT, "null argument");
     *out = nullptr;
@@ -1231,10 +1334,18 @@ static int render_impl(wr_scene *s, wr_film *f, const wr_render_params *prm, wr_
     auto begin = [&](int kind) { if (prof) marks.push_back({kind, ev_record()}); };
     auto end = [&]() { if (prof) ev_record(); };
     const int g_regen = std::max(1, std::min<int>((P + kShadeBlock - 1) / kShadeBlock, s->dev->n_cu * s->dev->occ_generate));
-    const int g_shade = std::max(
-        1, std::min<int>((P + kShadeBlock - 1) / kShadeBlock, s->dev->n_cu * (g ? s->dev->occ_shade_guided : s->dev->occ_shade)));
+    // shading kernel: plain, or extended for guiding and/or a medium
+    const bool medium = s->ds.med.on != 0;
+    const void *shade_fn = reinterpret_cast<const void *>(&k_shade);
+    if (g && medium) shade_fn = reinterpret_cast<const void *>(&k_shade_ext<true, true>);
+    else if (g) shade_fn = reinterpret_cast<const void *>(&k_shade_ext<true, false>);
+    else if (medium) shade_fn = reinterpret_cast<const void *>(&k_shade_ext<false, true>);
+    const int g_shade = std::max(1, std::min<int>((P + kShadeBlock - 1) / kShadeBlock,
+                                                  s->dev->n_cu * occupancy(s->dev, shade_fn, kShadeBlock)));
     int pass = 0;
-    const int g_infer = std::max(1, std::min<int>((P + kBlock - 1) / kBlock, s->dev->n_cu * s->dev->occ_infer));
+    const void *infer_fn = medium ? reinterpret_cast<const void *>(&k_guide_infer<true>)
+                                  : reinterpret_cast<const void *>(&k_guide_infer<false>);
+    const int g_infer = std::max(1, std::min<int>((P + kBlock - 1) / kBlock, s->dev->n_cu * occupancy(s->dev, infer_fn, kBlock)));
     const guide::Net net = g ? guide_net(g) : guide::Net{};
     const int g_ext = trav_grid(s, P, s->dev->occ_extend);
     const int g_shd = trav_grid(s, 2 * P, s->dev->occ_shadow);
@@ -1279,10 +1390,18 @@ static int render_impl(wr_scene *s, wr_film *f, const wr_render_params *prm, wr_
             else k_extend<false><<<g_ext, kBlock, 0, s->stream>>>(s->ds, pb, s->d_spill);
             end();
             begin(2);
-            if (g && gs.alpha > 0.f)
-                k_guide_infer<<<g_infer, kBlock, 0, s->stream>>>(s->ds.tris, s->ds.geom_info, pb.cur, pb.hit, pb.counters, N,
-                                                                   net, const_cast<float4 *>(gs.lobes), gs.lobe_stride);
-            if (g) k_shade_guided<<<g_shade, kShadeBlock, 0, s->stream>>>(s->ds, pb, a, gs);
+            if (g && gs.alpha > 0.f) {
+                float4 *lob = const_cast<float4 *>(gs.lobes);
+                if (medium)
+                    k_guide_infer<true><<<g_infer, kBlock, 0, s->stream>>>(s->ds, pb.cur, pb.hit, pb.counters, N, net, lob,
+                                                                           gs.lobe_stride);
+                else
+                    k_guide_infer<false><<<g_infer, kBlock, 0, s->stream>>>(s->ds, pb.cur, pb.hit, pb.counters, N, net, lob,
+                                                                            gs.lobe_stride);
+            }
+            if (g && medium) k_shade_ext<true, true><<<g_shade, kShadeBlock, 0, s->stream>>>(s->ds, pb, a, gs);
+            else if (g) k_shade_ext<true, false><<<g_shade, kShadeBlock, 0, s->stream>>>(s->ds, pb, a, gs);
+            else if (medium) k_shade_ext<false, true><<<g_shade, kShadeBlock, 0, s->stream>>>(s->ds, pb, a, gs);
             else k_shade<<<g_shade, kShadeBlock, 0, s->stream>>>(s->ds, pb, a);
             end();
             begin(1);

@@ -243,6 +243,23 @@ std::shared_ptr<PathNeeRenderer> PathNeeRenderer::FromSimple2Json(const std::sha
                                                int(json.number("num_spp", 16)), ctx);
     r->seed = uint32_t(json.number("seed", 0));
     r->use_mis = json.boolean("use_mis", true);
+    // "guiding": {"guide_fraction", "spp_per_pass", "train_steps", "warmup_passes",
+    //             "learning_rate", "batch_size", "record_capacity", "seed"} (all optional)
+    if (const wj::Value *gj = json.find("guiding")) {
+        if (gj->type != wj::Value::Object) throw std::runtime_error("\"guiding\" must be an object");
+        r->guided = gj->boolean("enabled", true);
+        wr_guide_config_default(&r->guide_config);
+        r->guide_config.learning_rate = float(gj->number("learning_rate", r->guide_config.learning_rate));
+        r->guide_config.batch_size = uint32_t(gj->number("batch_size", r->guide_config.batch_size));
+        r->guide_config.record_capacity = uint32_t(gj->number("record_capacity", r->guide_config.record_capacity));
+        r->guide_config.seed = uint32_t(gj->number("seed", 0));
+        r->guide_params.guide_fraction = float(gj->number("guide_fraction", 0.5));
+        r->guide_params.spp_per_pass = uint32_t(gj->number("spp_per_pass", 4));
+        r->guide_params.train_steps = uint32_t(gj->number("train_steps", 4));
+        r->guide_params.warmup_passes = uint32_t(gj->number("warmup_passes", 1));
+        if (!(r->guide_params.guide_fraction >= 0.f && r->guide_params.guide_fraction < 1.f))
+            throw std::runtime_error("guide_fraction must be in [0, 1)");
+    }
     return r;
 }
 
@@ -260,18 +277,22 @@ void PathNeeRenderer::render() {
     const wr_scene_desc d = scene->desc(camera->desc);
     wr_scene *sc = nullptr;
     wr_film *f = nullptr;
+    wr_guide *g = nullptr;
     check(wr_scene_create(ctx->get(), &d, &sc));
-    int rc = wr_film_create(sc, &f);
+    int rc = camera->has_medium ? wr_scene_set_medium(sc, &camera->medium) : WR_OK;
+    if (rc == WR_OK) rc = wr_film_create(sc, &f);
+    if (rc == WR_OK && guided) rc = wr_guide_create(sc, &guide_config, &g);
     if (rc == WR_OK) {
         wr_render_params p{};
         p.spp = num_spp;
         p.num_path_vertices = num_path_vertices;
         p.seed = seed;
         p.use_mis = use_mis ? 1 : 0;
-        rc = wr_render_nee(sc, f, &p, &stats);
+        rc = guided ? wr_render_guided(sc, f, g, &p, &guide_params, &stats, &guide_stats) : wr_render_nee(sc, f, &p, &stats);
     }
     if (rc == WR_OK) rc = wr_film_read(f, film.base.data(), film.splat.data());
     const std::string err = rc == WR_OK ? "" : wr_last_error();
+    if (g) wr_guide_destroy(g);
     if (f) wr_film_destroy(f);
     wr_scene_destroy(sc);
     if (rc != WR_OK) throw std::runtime_error(err);
@@ -292,6 +313,23 @@ const char *kOtherIntegrators[] = {"bidirectional_path_tracer", "photon_map", "p
                                    "visualize", "diffuse_prt", "glossy_prt2002"};
 
 }  // namespace
+
+bool ParseMedium(const wj::Value &json, wr_medium *out) {
+    const std::string type = json.string("type", "");
+    if (type == "none") return false;
+    if (type != "homogeneous") throw std::runtime_error("unknown medium");
+    const auto a = json.vec3("sigma_a"), s = json.vec3("sigma_s");
+    for (int c = 0; c < 3; c++) {
+        out->sigma_a[c] = float(a[c]);
+        out->sigma_s[c] = float(s[c]);
+    }
+    out->g = float(json.number("g", 0.0));
+    if (!(out->g > -1.f && out->g < 1.f)) throw std::runtime_error("medium \"g\" must be in (-1, 1)");
+    for (int c = 0; c < 3; c++)
+        if (!(out->sigma_a[c] >= 0.f && out->sigma_s[c] >= 0.f))
+            throw std::runtime_error("medium coefficients must be >= 0");
+    return true;
+}
 
 void Simple2Parser::Parse(const std::string &json_path, const ParseOptions &opt) {
     std::ifstream is(json_path);
@@ -315,11 +353,7 @@ void Simple2Parser::Parse(const wj::Value &j, const std::string &root_dir, const
     }
     if (!j.has("render")) return;
     auto camera = ThinlensCamera::FromSimple2ParseJson(j.at("camera"));
-    if (const wj::Value *med = j.find("medium")) {
-        const std::string type = med->string("type", "");
-        if (type != "none") throw std::runtime_error(type == "homogeneous" ? "homogeneous media are not enabled in this build"
-                                                                          : "unknown medium");
-    }
+    if (const wj::Value *med = j.find("medium")) camera->has_medium = ParseMedium(*med, &camera->medium);
     const wr_scene_desc d = scene->desc(camera->desc);
     if (wr_scene_validate(&d) != WR_OK) throw std::runtime_error(wr_last_error());
     if (!opt.dump_dir.empty()) {
@@ -355,10 +389,11 @@ void Simple2Parser::Parse(const wj::Value &j, const std::string &root_dir, const
         if (!output.empty()) camera->film->save_pfm(root_dir + "/" + output);
         if (auto *nee = dynamic_cast<PathNeeRenderer *>(renderer.get())) {
             std::printf("{\"integrator\": \"%s\", \"output\": \"%s\", \"spp\": %d, \"paths\": %llu, \"extend_rays\": %llu, "
-                        "\"shadow_rays\": %llu, \"ms\": %.3f}\n",
+                        "\"shadow_rays\": %llu, \"ms\": %.3f, \"guided\": %s, \"train_steps\": %llu}\n",
                         name.c_str(), output.c_str(), nee->num_spp, (unsigned long long)nee->stats.paths,
                         (unsigned long long)nee->stats.extend_rays, (unsigned long long)nee->stats.shadow_rays,
-                        nee->stats.wall_ms);
+                        nee->stats.wall_ms, nee->guided ? "true" : "false",
+                        (unsigned long long)nee->guide_stats.train_steps);
         }
     }
 }

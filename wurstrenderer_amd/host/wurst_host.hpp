@@ -31,6 +31,9 @@ struct Film {
 struct ThinlensCamera {
     wr_camera desc{};
     std::shared_ptr<Film> film;
+    // the scene's medium hangs off the camera, as in the reference (simple2parser.h:163-167)
+    bool has_medium = false;
+    wr_medium medium{};
     static std::shared_ptr<ThinlensCamera> FromSimple2ParseJson(const wj::Value &json);
 };
 
@@ -80,13 +83,24 @@ struct PathNeeRenderer : Renderer {
     int num_path_vertices, num_spp;
     uint32_t seed = 0;
     bool use_mis = true;
+    // learned guiding: the optional "guiding" object of the render entry (no reference counterpart)
+    bool guided = false;
+    wr_guided_params guide_params{};
+    wr_guide_config guide_config{};
     wr_render_stats stats{};
+    wr_guide_stats guide_stats{};
 };
 
 struct ParseOptions {
     bool check_only = false;     // parse and validate; no GPU
     std::string dump_dir;        // write the flattened scene arrays here
 };
+
+// Simple2Parser::ParseMedium (simple2parser.h:70-82): "homogeneous" with
+// RgbFromJson "sigma_a", "sigma_s" (homogeneous.h:11-20), plus an optional HG
+// asymmetry "g" (default 0: the reference's isotropic phase); "none"; anything
+// else throws "unknown medium".
+bool ParseMedium(const wj::Value &json, wr_medium *out);
 
 // Simple2Parser::Parse (src/sceneio/simple2parser.h:84-223).  Throws
 // std::runtime_error on malformed input or unsupported features.

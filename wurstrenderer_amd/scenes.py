@@ -39,6 +39,8 @@ class Scene:
     lights: np.ndarray         # LIGHT_DTYPE
     camera: np.ndarray         # CAMERA_DTYPE, shape ()
     name: str = "scene"
+    # homogeneous medium filling the scene: {"sigma_a": rgb, "sigma_s": rgb, "g": HG asymmetry} or None
+    medium: dict | None = None
 
     @property
     def n_tris(self) -> int:
@@ -47,6 +49,11 @@ class Scene:
     @property
     def resolution(self) -> tuple[int, int]:
         return int(self.camera["width"]), int(self.camera["height"])
+
+    def with_medium(self, sigma_a=(0.05, 0.05, 0.05), sigma_s=(0.4, 0.4, 0.4), g=0.7) -> "Scene":
+        """The scene filled with a homogeneous medium (a number broadcasts to RGB)."""
+        rgb = lambda v: tuple(float(x) for x in np.broadcast_to(np.asarray(v, np.float64), (3,)))
+        return dataclasses.replace(self, medium={"sigma_a": rgb(sigma_a), "sigma_s": rgb(sigma_s), "g": float(g)})
 
     def with_camera(self, **kw) -> "Scene":
         cam = self.camera.copy()
@@ -267,6 +274,28 @@ def guide_room(seed: int = 4, n_spheres: int = 49, level: int = 5, resolution=(1
                    name=f"guide_room_s{seed}_n{n_spheres}_l{level}")
 
 
+def emitting_box(resolution=(24, 24), L=1.0, half=1.0) -> Scene:
+    """Camera at the centre of a closed cube whose six faces emit L inward (no
+    reflecting surface): with a purely absorbing medium a pixel sees
+    L exp(-sigma_t d), d the distance to the wall; with a purely scattering one
+    the radiance is L everywhere (radiative equilibrium)."""
+    b = SceneBuilder()
+    v, t = box((0, 0, 0), (2 * half,) * 3, 0.0)
+    b.mesh(v, t[:, ::-1].copy(), light_L=(L, L, L))  # inward-wound: Ng points into the cube
+    return b.build(dict(pos=(0.0, 0.0, 0.0), lookat=(0.0, 0.0, -1.0), up=(0, 1, 0), fovy=70.0,
+                        resolution=resolution), name="emitting_box")
+
+
+def box_exit_distance(o, d, half=1.0):
+    """Distance from points o (inside) along unit directions d to the cube [-half, half]^3."""
+    t = np.full(len(d), np.inf)
+    for k in range(3):
+        with np.errstate(divide="ignore", invalid="ignore"):
+            tk = np.where(d[:, k] > 0, (half - o[:, k]) / d[:, k], (-half - o[:, k]) / d[:, k])
+        t = np.minimum(t, np.where(d[:, k] != 0, tk, np.inf))
+    return t
+
+
 def plane_under_light(resolution=(32, 32), light_half=0.5, height=1.0, rho=0.8, L=10.0, plane_half=4.0) -> Scene:
     """Lambert plane y=0 under a square emitter at y=height facing down; the
     camera looks straight down from below the emitter.  Direct lighting of
@@ -393,4 +422,7 @@ def scene_json(scene: Scene, obj_name: str, renders: list[dict]) -> dict:
         if float(cam["aperture_radius"]) != float(cam["focal_distance"]):
             raise ValueError("the JSON key \"radius\" sets both the aperture and the focal distance")
         j["camera"]["radius"] = float(cam["aperture_radius"])
+    if scene.medium:
+        j["medium"] = {"type": "homogeneous", "sigma_a": list(scene.medium["sigma_a"]),
+                       "sigma_s": list(scene.medium["sigma_s"]), "g": scene.medium["g"]}
     return j
