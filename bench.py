@@ -45,8 +45,9 @@ def parse():
     ap.add_argument("--ttr-width", type=int, default=480)
     ap.add_argument("--ttr-height", type=int, default=270)
     ap.add_argument("--ttr-target", type=float, default=1e-3)
-    ap.add_argument("--ttr-ref-spp", type=int, default=16384)
+    ap.add_argument("--ttr-ref-spp", type=int, default=32768)
     ap.add_argument("--ttr-runs", type=int, default=3)
+    ap.add_argument("--ttr-train-spp", type=int, default=256)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first copy 1 GiB device-to-device three times (known-size kernel for byte counters)")
     return ap.parse_args()
@@ -83,7 +84,10 @@ def rel_mse(img, ref, eps=1e-2):
 
 
 def progressive(render, film, ref, target, first_spp, max_spp):
-    """Progressive spp doubling; wall time of the render calls until relMSE <= target."""
+    """Progressive spp doubling: wall time of the render calls until the first
+    checkpoint with relMSE <= target, and the crossing time interpolated between
+    the last two checkpoints (relMSE ~ 1/time there)."""
+    import math
     t_total, spp, step, curve = 0.0, 0, first_spp, []
     while spp < max_spp:
         t = time.perf_counter()
@@ -93,9 +97,14 @@ def progressive(render, film, ref, target, first_spp, max_spp):
         e = rel_mse(film.image(), ref)
         curve.append([spp, round(t_total, 4), e])
         if e <= target:
-            return t_total, spp, curve
+            t_cross = t_total
+            if len(curve) > 1:
+                (_, t0, e0), (_, t1, e1) = curve[-2], curve[-1]
+                if e0 > e1 > 0:
+                    t_cross = math.exp(math.log(t0) + (math.log(t1) - math.log(t0)) * math.log(e0 / target) / math.log(e0 / e1))
+            return t_total, t_cross, spp, curve
         step = spp
-    return None, spp, curve
+    return None, None, spp, curve
 
 
 def time_to_relmse(wr, scenes, dev, a):
@@ -119,20 +128,25 @@ def time_to_relmse(wr, scenes, dev, a):
     ref_s = time.perf_counter() - t
     floor = rel_mse(ia, ib) / 4.0  # relMSE of the mean of two halves against the truth, approximately
     print(f"# ttr reference: {a.ttr_ref_spp} spp in {ref_s:.1f}s, noise floor ~{floor:.2e}", file=sys.stderr, flush=True)
-    max_spp = a.ttr_ref_spp // 2
+    max_spp = a.ttr_ref_spp
 
     def plain(seed):
         return lambda film, spp: wr.render_nee(gs, film, spp, num_path_vertices=N, seed=seed)
 
     def guided(seed):
-        g = wr.Guide(gs, batch_size=1 << 18, record_capacity=1 << 20, seed=seed)
+        g = wr.Guide(gs, learning_rate=1e-2, batch_size=1 << 18, record_capacity=1 << 20, seed=seed)
         state = {"calls": 0}
 
         def render(film, spp):
-            if film.spp < 64:
-                wr.render_guided(gs, film, g, spp, num_path_vertices=N, seed=seed, guide_fraction=0.5,
-                                 spp_per_pass=4, train_steps=4, warmup_passes=1 if state["calls"] == 0 else 0)
-            else:
+            # online training: 8-spp passes, 32 Adam steps after each, for the first
+            # TRAIN_SPP samples per pixel; then the trained network alone
+            left = max(0, a.ttr_train_spp - film.spp)
+            if left:
+                k = min(spp, left)
+                wr.render_guided(gs, film, g, k, num_path_vertices=N, seed=seed, guide_fraction=0.5,
+                                 spp_per_pass=8, train_steps=32, warmup_passes=1 if state["calls"] == 0 else 0)
+                spp -= k
+            if spp:
                 wr.render_guided(gs, film, g, spp, num_path_vertices=N, seed=seed, guide_fraction=0.5,
                                  spp_per_pass=spp, train_steps=0, warmup_passes=0)
             state["calls"] += 1
@@ -140,25 +154,34 @@ def time_to_relmse(wr, scenes, dev, a):
 
     res = {}
     for name, make, first in (("plain", plain, 16), ("guided", guided, 16)):
-        times, curves = [], []
+        times, crosses, curves = [], [], []
         for r in range(a.ttr_runs):
             film = wr.Film(gs)
-            tt, spp, curve = progressive(make(100 + r), film, ref, a.ttr_target, first, max_spp)
+            tt, tc, spp, curve = progressive(make(100 + r), film, ref, a.ttr_target, first, max_spp)
             times.append(tt)
+            crosses.append(tc)
             curves.append(curve)
-            print(f"# ttr {name} run {r}: {tt if tt is None else round(tt, 3)} s at {spp} spp; "
-                  f"last relMSE {curve[-1][2]:.3e}", file=sys.stderr, flush=True)
+            print(f"# ttr {name} run {r}: {tt if tt is None else round(tt, 3)} s at {spp} spp "
+                  f"(interpolated crossing {tc if tc is None else round(tc, 3)} s); last relMSE {curve[-1][2]:.3e}",
+                  file=sys.stderr, flush=True)
         ok = [x for x in times if x is not None]
+        okc = [x for x in crosses if x is not None]
         res[name] = {"seconds_median": float(np.median(ok)) if len(ok) == len(times) else None,
-                     "seconds_runs": times, "curve_run0": curves[0]}
-    sp = None
+                     "crossing_seconds_median": float(np.median(okc)) if len(okc) == len(crosses) else None,
+                     "seconds_runs": times, "crossing_seconds_runs": crosses, "curve_run0": curves[0]}
+    sp = spc = None
     if res["plain"]["seconds_median"] and res["guided"]["seconds_median"]:
         sp = res["plain"]["seconds_median"] / res["guided"]["seconds_median"]
+    if res["plain"]["crossing_seconds_median"] and res["guided"]["crossing_seconds_median"]:
+        spc = res["plain"]["crossing_seconds_median"] / res["guided"]["crossing_seconds_median"]
     return {"target_relmse": a.ttr_target, "workload": f"guide_room seed 4 (1,003,550 tris, gap 0.2, L 500), "
                                                        f"{a.ttr_width}x{a.ttr_height}, max depth 16",
             "reference": {"spp": a.ttr_ref_spp, "estimator": "plain", "noise_floor_relmse": floor,
                           "seconds": round(ref_s, 2)},
-            "plain": res["plain"], "guided": res["guided"], "speedup_guided_vs_plain": sp}
+            "guided_schedule": {"train_spp": a.ttr_train_spp, "spp_per_pass": 8, "adam_steps_per_pass": 32,
+                                "batch": 1 << 18, "lr": 1e-2, "guide_fraction": 0.5},
+            "plain": res["plain"], "guided": res["guided"], "speedup_guided_vs_plain": sp,
+            "speedup_guided_vs_plain_interpolated": spc}
 
 
 def cpu_baseline(sc, N, target_s):

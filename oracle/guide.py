@@ -60,11 +60,17 @@ def pack(layers):
 
 
 # ------------------------------------------------------------------ encoding
-def sph01(d):
+def oct01(d):
+    """Octahedral map of unit vectors to [0,1]^2 (Engelhardt and Dachsbacher 2008), y up:
+    u = x / |d|_1, v = z / |d|_1, the lower half folded over the diagonals."""
     d = np.asarray(d, dtype=np.float64)
-    a = np.arccos(np.clip(d[:, 1], -1.0, 1.0)) / np.pi
-    b = np.arctan2(d[:, 2], d[:, 0]) / (2 * np.pi) + 0.5
-    return a, b
+    l1 = np.abs(d).sum(axis=1)
+    u, v = d[:, 0] / l1, d[:, 2] / l1
+    sgn = lambda t: np.where(t >= 0, 1.0, -1.0)
+    low = d[:, 1] < 0
+    uu = np.where(low, (1 - np.abs(v)) * sgn(u), u)
+    vv = np.where(low, (1 - np.abs(u)) * sgn(v), v)
+    return np.clip(0.5 * uu + 0.5, 0, 1), np.clip(0.5 * vv + 0.5, 0, 1)
 
 
 def encode(pos, nrm, wo, box):
@@ -72,8 +78,8 @@ def encode(pos, nrm, wo, box):
     pos = np.asarray(pos, dtype=np.float64)
     lo, hi = np.asarray(box[:3], dtype=np.float64), np.asarray(box[3:], dtype=np.float64)
     p01 = np.clip((pos - lo) / np.maximum(hi - lo, 1e-20), 0.0, 1.0)
-    na, nb = sph01(nrm)
-    oa, ob = sph01(wo)
+    na, nb = oct01(nrm)
+    oa, ob = oct01(wo)
     return np.column_stack([p01, na, nb, oa, ob])
 
 

@@ -64,19 +64,35 @@ __host__ __device__ constexpr int drow(int v, int h) { return 8 * (v >> 2) + 4 *
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+// An MFMA operand: 8 BF16 values packed two per 32-bit word (element e in word
+// e / 2, low half first), so each operand is exactly 4 registers.
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ f32x16 mfma(bf16x8 a, bf16x8 b, f32x16 c) {
-    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+__device__ __forceinline__ f32x16 mfma(u32x4 a, u32x4 b, f32x16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
 }
 __device__ __forceinline__ __bf16 bf(float x) { return __float2bfloat16(x); }
 __device__ __forceinline__ float fb(__bf16 x) { return __bfloat162float(x); }
+// FP32 -> BF16 bits, round to nearest even (finite inputs)
+__device__ __forceinline__ uint32_t bf_bits(float x) {
+    const uint32_t u = __float_as_uint(x);
+    return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
+}
+__device__ __forceinline__ uint32_t pack2(float lo, float hi) { return bf_bits(lo) | (bf_bits(hi) << 16); }
+__device__ __forceinline__ float unpack_lo(uint32_t w) { return __uint_as_float(w << 16); }
+__device__ __forceinline__ float unpack_hi(uint32_t w) { return __uint_as_float(w & 0xFFFF0000u); }
+// element e of an operand as FP32
+__device__ __forceinline__ float op_elem(const u32x4 &x, int e) { return (e & 1) ? unpack_hi(x[e >> 1]) : unpack_lo(x[e >> 1]); }
 
 // Device view of a network.
+// Biases of all layers packed at 64 per layer (the last has 40).
+constexpr int kBiasPacked = 64 * (kLayers - 1) + kOut;
+
 struct Net {
-    const bf16x8 *fwd;    // kFwdOps packed forward operands
-    const bf16x8 *bwd;    // kBwdOps packed backward operands
-    const float *params;  // FP32 master parameters (biases are read from here)
-    float box[6];         // scene bounds for the position encoding (lo xyz, hi xyz)
+    const u32x4 *fwd;     // kFwdOps packed forward operands
+    const u32x4 *bwd;     // kBwdOps packed backward operands
+    const float *bias;    // kBiasPacked FP32 biases, layer l at 64 l
+    float box[6];         // position encoding: scene bounds lo xyz, then 1 / extent xyz
 };
 
 // Training records written by the renderer, one per recorded path vertex.
@@ -91,36 +107,56 @@ struct Records {
     uint32_t capacity;
 };
 
-// Decoded lobes of the current path state: 10 float4 per entry, SoA
-// (lobes[j * stride + i], j < 10), written by the inference kernel.
-constexpr int kLobeVecs = kOut / 4;
+// What the inference kernel leaves for the shading kernel, per entry of the
+// path state: the direction it drew (from the network with probability alpha,
+// else from the BSDF or phase function) with the network's density there, and
+// the network's density in the direction of the vertex's light sample.
+struct Draw {
+    float4 *dir;   // xyz: sampled direction, w: q(direction)
+    float *q_nee;  // q(direction to the light sample)
+};
 
 // What the shading kernel needs for guided sampling.
 struct Shade {
-    const float4 *lobes;
-    uint32_t lobe_stride;
+    Draw draw;
     Records rec;
     float alpha;        // one-sample MIS probability of sampling the network (0: BSDF only)
     float record_prob;  // fraction of paths whose vertices are recorded
     int collect;        // record training data
 };
 
+// Hardware exp2 / log2 (v_exp_f32 / v_log_f32, about 1 ulp) for the network's
+// encoding and mixture: the features are rounded to BF16 anyway, and the
+// mixture's pdf and sampler read the same decoded values.
+__device__ __forceinline__ float fexp(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504088896341f); }
+__device__ __forceinline__ float flog(float x) { return __builtin_amdgcn_logf(x) * 0.69314718055994531f; }
+
 // ---------------------------------------------------------------- encoding
-// Spherical coordinates of a unit vector mapped to [0,1]^2.
-__device__ __forceinline__ void sph01(float3 d, float &a, float &b) {
-    a = acosf(fminf(fmaxf(d.y, -1.f), 1.f)) * kInvPi;
-    b = atan2f(d.z, d.x) * (0.5f * kInvPi) + 0.5f;
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+// Octahedral map of a unit vector to [0,1]^2 (Engelhardt and Dachsbacher 2008),
+// y up: project onto |x| + |y| + |z| = 1, fold the lower half over the diagonals.
+__device__ __forceinline__ void oct01(float3 d, float &a, float &b) {
+    const float r = frcp(fabsf(d.x) + fabsf(d.y) + fabsf(d.z));
+    float u = d.x * r, v = d.z * r;
+    if (d.y < 0.f) {
+        const float uu = (1.f - fabsf(v)) * (u >= 0.f ? 1.f : -1.f);
+        v = (1.f - fabsf(u)) * (v >= 0.f ? 1.f : -1.f);
+        u = uu;
+    }
+    a = fminf(fmaxf(0.5f * u + 0.5f, 0.f), 1.f);
+    b = fminf(fmaxf(0.5f * v + 0.5f, 0.f), 1.f);
 }
 // The 7 encoded scalars of a record, all in [0,1]: position in the scene box
-// (3), normal (2), direction to the previous vertex (2).
+// (3), normal (2), direction to the previous vertex (2).  box: lo xyz, then
+// 1 / extent xyz.
 struct Enc { float v[7]; };
 __device__ __forceinline__ Enc encode(float3 pos, float3 n, float3 wo, const float box[6]) {
     Enc e;
-    e.v[0] = fminf(fmaxf((pos.x - box[0]) / fmaxf(box[3] - box[0], 1e-20f), 0.f), 1.f);
-    e.v[1] = fminf(fmaxf((pos.y - box[1]) / fmaxf(box[4] - box[1], 1e-20f), 0.f), 1.f);
-    e.v[2] = fminf(fmaxf((pos.z - box[2]) / fmaxf(box[5] - box[2], 1e-20f), 0.f), 1.f);
-    sph01(n, e.v[3], e.v[4]);
-    sph01(wo, e.v[5], e.v[6]);
+    e.v[0] = fminf(fmaxf((pos.x - box[0]) * box[3], 0.f), 1.f);
+    e.v[1] = fminf(fmaxf((pos.y - box[1]) * box[4], 0.f), 1.f);
+    e.v[2] = fminf(fmaxf((pos.z - box[2]) * box[5], 0.f), 1.f);
+    oct01(n, e.v[3], e.v[4]);
+    oct01(wo, e.v[5], e.v[6]);
     return e;
 }
 // One-blob feature f of 32 (Mueller et al. 2019, "Neural Importance
@@ -133,7 +169,7 @@ __device__ __forceinline__ float feature(const Enc &e, int f) {
     const int k = f < 18 ? 6 : f < 26 ? 4 : 3;
     const int j = f < 18 ? f % 6 : f < 26 ? (f - 18) % 4 : (f - 26) % 3;
     const float d = (x - (float(j) + 0.5f) / float(k)) * float(k);
-    return expf(-0.5f * d * d);
+    return fexp(-0.5f * d * d);
 }
 
 // ---------------------------------------------------------------- mixture
@@ -143,23 +179,33 @@ __device__ __forceinline__ float feature(const Enc &e, int f) {
 constexpr int kStageStride = 41;  // odd: the 64 rows of a wave fall in distinct banks
 __device__ __forceinline__ float softplus(float x) { return x > 15.f ? x : log1pf(expf(x)); }
 __device__ __forceinline__ float log_vmf_norm(float lam) { return logf(lam) - logf(2.f * kPi) - logf(-expm1f(-2.f * lam)); }
+// fast forms for rendering (lambda >= kLambdaMin, so 1 - e^{-2 lambda} >= 0.095)
+__device__ __forceinline__ float softplus_fast(float x) { return x > 15.f ? x : flog(1.f + fexp(x)); }
+__device__ __forceinline__ float log_vmf_norm_fast(float lam) {
+    return flog(lam * (1.f / (2.f * kPi)) * frcp(1.f - fexp(-2.f * lam)));
+}
 
-// Raw outputs -> decoded lobes, in place.
-__device__ __forceinline__ void decode(float *o) {
+// Raw outputs -> decoded lobes, in place; w (if given) receives the mixture
+// weights, which the sampler picks lobes by.
+__device__ __forceinline__ void decode(float *o, float *w = nullptr) {
     float mx = -INFINITY;
 #pragma unroll
     for (int k = 0; k < kLobes; k++) mx = fmaxf(mx, o[5 * k]);
-    float s = 0.f;
-#pragma unroll
-    for (int k = 0; k < kLobes; k++) s += expf(o[5 * k] - mx);
-    const float lse = mx + logf(s);
+    float e[kLobes], s = 0.f;
 #pragma unroll
     for (int k = 0; k < kLobes; k++) {
+        e[k] = fexp(o[5 * k] - mx);
+        s += e[k];
+    }
+    const float lse = mx + flog(s), rs = frcp(s);
+#pragma unroll
+    for (int k = 0; k < kLobes; k++) {
+        if (w) w[k] = e[k] * rs;
         const float3 a = f3(o[5 * k + 1], o[5 * k + 2], o[5 * k + 3]);
-        const float n = sqrtf(dot3(a, a));
-        const float3 mu = n > 1e-12f ? a * (1.f / n) : f3(0.f, 1.f, 0.f);
-        const float lam = fminf(fmaxf(softplus(o[5 * k + 4]), kLambdaMin), kLambdaMax);
-        o[5 * k] = o[5 * k] - lse + log_vmf_norm(lam);
+        const float n2 = dot3(a, a);
+        const float3 mu = n2 > 1e-24f ? a * __builtin_amdgcn_rsqf(n2) : f3(0.f, 1.f, 0.f);
+        const float lam = fminf(fmaxf(softplus_fast(o[5 * k + 4]), kLambdaMin), kLambdaMax);
+        o[5 * k] = o[5 * k] - lse + log_vmf_norm_fast(lam);
         o[5 * k + 1] = mu.x;
         o[5 * k + 2] = mu.y;
         o[5 * k + 3] = mu.z;
@@ -170,26 +216,26 @@ __device__ __forceinline__ float pdf(const float *L, float3 w) {
     float p = 0.f;
 #pragma unroll
     for (int k = 0; k < kLobes; k++)
-        p += expf(L[5 * k] + L[5 * k + 4] * (L[5 * k + 1] * w.x + L[5 * k + 2] * w.y + L[5 * k + 3] * w.z - 1.f));
+        p += fexp(L[5 * k] + L[5 * k + 4] * (L[5 * k + 1] * w.x + L[5 * k + 2] * w.y + L[5 * k + 3] * w.z - 1.f));
     return p;
 }
-// Lobe by weight (u0), then the vMF around its axis (Wood 1994, on the 2-sphere):
-// cos = 1 + log(1 - u1 + u1 exp(-2 lambda)) / lambda, azimuth 2 pi u2.
-// The lobe is picked with selects, so L may live in registers.
-__device__ __forceinline__ float3 sample(const float *L, float u0, float u1, float u2) {
+// Lobe by weight w (u0), then the vMF around its axis (Wood 1994, on the
+// 2-sphere): cos = 1 + log(1 - u1 + u1 exp(-2 lambda)) / lambda, azimuth 2 pi u2.
+// The lobe is picked with selects, so L and w may live in registers.
+__device__ __forceinline__ float3 sample(const float *L, const float *w, float u0, float u1, float u2) {
     float3 mu = f3(L[5 * (kLobes - 1) + 1], L[5 * (kLobes - 1) + 2], L[5 * (kLobes - 1) + 3]);
     float lam = L[5 * (kLobes - 1) + 4];
     float c = 0.f;
     bool found = false;
 #pragma unroll
     for (int k = 0; k < kLobes - 1; k++) {
-        c += expf(L[5 * k] - log_vmf_norm(L[5 * k + 4]));
+        c += w[k];
         const bool take = !found && u0 < c;
         mu = take ? f3(L[5 * k + 1], L[5 * k + 2], L[5 * k + 3]) : mu;
         lam = take ? L[5 * k + 4] : lam;
         found = found || take;
     }
-    const float ct = fminf(fmaxf(1.f + logf(1.f - u1 + u1 * expf(-2.f * lam)) / lam, -1.f), 1.f);
+    const float ct = fminf(fmaxf(1.f + flog(1.f - u1 + u1 * fexp(-2.f * lam)) * frcp(lam), -1.f), 1.f);
     const float st = sqrtf(fmaxf(0.f, 1.f - ct * ct));
     float s, co;
     sincospif(2.f * u2, &s, &co);
@@ -199,7 +245,7 @@ __device__ __forceinline__ float3 sample(const float *L, float u0, float u1, flo
 // ---------------------------------------------------------------- forward pass
 // Input operands of layer 0 for the wave's 64 records: record r = 32 nt + p is
 // encoded in lane r; lane (p, h) builds its 16 features of records p and 32 + p.
-__device__ __forceinline__ void input_operands(const Enc &mine, bf16x8 x[2][2] /* [ks][nt] */) {
+__device__ __forceinline__ void input_operands(const Enc &mine, u32x4 x[2][2] /* [ks][nt] */) {
     const int lane = __lane_id(), p = lane & 31, h = lane >> 5;
 #pragma unroll
     for (int nt = 0; nt < 2; nt++) {
@@ -209,7 +255,8 @@ __device__ __forceinline__ void input_operands(const Enc &mine, bf16x8 x[2][2] /
 #pragma unroll
         for (int ks = 0; ks < 2; ks++)
 #pragma unroll
-            for (int i = 0; i < 8; i++) x[ks][nt][i] = bf(feature(e, kperm(ks, h, i)));
+            for (int i = 0; i < 4; i++)
+                x[ks][nt][i] = pack2(feature(e, kperm(ks, h, 2 * i)), feature(e, kperm(ks, h, 2 * i + 1)));
     }
 }
 
@@ -217,7 +264,7 @@ __device__ __forceinline__ void input_operands(const Enc &mine, bf16x8 x[2][2] /
 // acc[nt] = b + (W_l^T X^T)[32 mt .. 32 mt + 31][32 nt .. 32 nt + 31]; the
 // accumulators start at the bias (N: outputs of the layer).
 template <int K>
-__device__ __forceinline__ void tile_fwd(const bf16x8 *ops, const float *bias, int N, const bf16x8 (*x)[2], int mt,
+__device__ __forceinline__ void tile_fwd(const u32x4 *ops, const float *bias, int N, const u32x4 (*x)[2], int mt,
                                          f32x16 acc[2]) {
     const int lane = __lane_id(), h = lane >> 5;
 #pragma unroll
@@ -228,7 +275,7 @@ __device__ __forceinline__ void tile_fwd(const bf16x8 *ops, const float *bias, i
     acc[1] = acc[0];
 #pragma unroll
     for (int ks = 0; ks < K / 16; ks++) {
-        const bf16x8 a = ops[(mt * (K / 16) + ks) * 64 + lane];
+        const u32x4 a = ops[(mt * (K / 16) + ks) * 64 + lane];
         acc[0] = mfma(a, x[ks][0], acc[0]);
         acc[1] = mfma(a, x[ks][1], acc[1]);
     }
@@ -236,13 +283,15 @@ __device__ __forceinline__ void tile_fwd(const bf16x8 *ops, const float *bias, i
 
 // ReLU of output tile mt into operands 2 mt, 2 mt + 1 of the next layer:
 // xn[ks][nt] element e = relu(acc[nt][8 (ks % 2) + e]).
-__device__ __forceinline__ void tile_to_operands(const f32x16 acc[2], int mt, bf16x8 xn[4][2]) {
+__device__ __forceinline__ void tile_to_operands(const f32x16 acc[2], int mt, u32x4 xn[4][2]) {
 #pragma unroll
     for (int kk = 0; kk < 2; kk++)
 #pragma unroll
-        for (int e = 0; e < 8; e++)
+        for (int i = 0; i < 4; i++)
 #pragma unroll
-            for (int nt = 0; nt < 2; nt++) xn[2 * mt + kk][nt][e] = bf(fmaxf(acc[nt][8 * kk + e], 0.f));
+            for (int nt = 0; nt < 2; nt++)
+                xn[2 * mt + kk][nt][i] =
+                    pack2(fmaxf(acc[nt][8 * kk + 2 * i], 0.f), fmaxf(acc[nt][8 * kk + 2 * i + 1], 0.f));
 }
 
 __device__ __forceinline__ void wave_sync_lds() {
@@ -253,41 +302,28 @@ __device__ __forceinline__ void wave_sync_lds() {
 
 // Forward pass of the wave's 64 records (record r is the one encoded in lane
 // r).  Raw outputs of record r land in stage[r * kStageStride + o], o < 40.
-// keep(l, ks, nt, x) sees every layer's input operands (for the backward
-// pass).  Must be called by all 64 lanes of the wave.
-struct NoKeep {
-    __device__ void operator()(int, int, int, const bf16x8 &) const {}
-};
-template <typename Keep = NoKeep>
-__device__ __forceinline__ void forward(const Net &net, const Enc &mine, float *stage, const Keep &keep = Keep()) {
+// Must be called by all 64 lanes of the wave.
+__device__ __forceinline__ void forward(const Net &net, const Enc &mine, float *stage) {
     const int lane = __lane_id(), p = lane & 31, h = lane >> 5;
-    bf16x8 x[4][2];
+    u32x4 x[4][2];
     {
-        bf16x8 x0[2][2];
+        u32x4 x0[2][2];
         input_operands(mine, x0);
-#pragma unroll
-        for (int ks = 0; ks < 2; ks++)
-#pragma unroll
-            for (int nt = 0; nt < 2; nt++) keep(0, ks, nt, x0[ks][nt]);
 #pragma unroll
         for (int mt = 0; mt < 2; mt++) {
             f32x16 acc[2];
-            tile_fwd<kIn>(net.fwd + fwd_offset(0), net.params + b_offset(0), kHid, x0, mt, acc);
+            tile_fwd<kIn>(net.fwd + fwd_offset(0), net.bias, kHid, x0, mt, acc);
             tile_to_operands(acc, mt, x);
         }
     }
     // hidden layers 1..3 (same shape): rolled, so one layer's operands are live at a time
 #pragma unroll 1
     for (int l = 1; l < kLayers - 1; l++) {
-#pragma unroll
-        for (int ks = 0; ks < 4; ks++)
-#pragma unroll
-            for (int nt = 0; nt < 2; nt++) keep(l, ks, nt, x[ks][nt]);
-        bf16x8 xn[4][2];
+        u32x4 xn[4][2];
 #pragma unroll
         for (int mt = 0; mt < 2; mt++) {
             f32x16 acc[2];
-            tile_fwd<kHid>(net.fwd + fwd_offset(l), net.params + b_offset(l), kHid, x, mt, acc);
+            tile_fwd<kHid>(net.fwd + fwd_offset(l), net.bias + 64 * l, kHid, x, mt, acc);
             tile_to_operands(acc, mt, xn);
         }
 #pragma unroll
@@ -296,13 +332,9 @@ __device__ __forceinline__ void forward(const Net &net, const Enc &mine, float *
             for (int nt = 0; nt < 2; nt++) x[ks][nt] = xn[ks][nt];
     }
 #pragma unroll
-    for (int ks = 0; ks < 4; ks++)
-#pragma unroll
-        for (int nt = 0; nt < 2; nt++) keep(kLayers - 1, ks, nt, x[ks][nt]);
-#pragma unroll
     for (int mt = 0; mt < 2; mt++) {
         f32x16 acc[2];
-        tile_fwd<kHid>(net.fwd + fwd_offset(kLayers - 1), net.params + b_offset(kLayers - 1), kOut, x, mt, acc);
+        tile_fwd<kHid>(net.fwd + fwd_offset(kLayers - 1), net.bias + 64 * (kLayers - 1), kOut, x, mt, acc);
 #pragma unroll
         for (int v = 0; v < 16; v++) {
             const int o = 32 * mt + drow(v, h);

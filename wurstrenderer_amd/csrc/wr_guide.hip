@@ -28,8 +28,9 @@ constexpr int kMaxTrainBlocks = 256;
 constexpr uint32_t kTestBlock = 64;
 
 // ---------------------------------------------------------------- pack
-__global__ void k_pack(const float *params, bf16x8 *fwd, bf16x8 *bwd) {
+__global__ void k_pack(const float *params, u32x4 *fwd, u32x4 *bwd, float *bias) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < kBiasPacked) bias[j] = params[b_offset(j / 64) + j % 64];
     if (j < kFwdOps) {
         int l = 0;
         while (l + 1 < kLayers && j >= fwd_offset(l + 1)) l++;
@@ -37,8 +38,11 @@ __global__ void k_pack(const float *params, bf16x8 *fwd, bf16x8 *bwd) {
         const int jj = j - fwd_offset(l);
         const int mt = jj / (nks * 64), ks = (jj / 64) % nks, lane = jj % 64;
         const int o = 32 * mt + (lane & 31), h = lane >> 5;
-        bf16x8 v;
-        for (int e = 0; e < 8; e++) v[e] = bf(o < N ? params[w_offset(l) + kperm(ks, h, e) * N + o] : 0.f);
+        u32x4 v;
+        for (int i = 0; i < 4; i++)
+            v[i] = o < N ? pack2(params[w_offset(l) + kperm(ks, h, 2 * i) * N + o],
+                                 params[w_offset(l) + kperm(ks, h, 2 * i + 1) * N + o])
+                         : 0u;
         fwd[j] = v;
     } else if (j < kFwdOps + kBwdOps) {
         const int jb = j - kFwdOps;
@@ -48,10 +52,10 @@ __global__ void k_pack(const float *params, bf16x8 *fwd, bf16x8 *bwd) {
         const int jj = jb - bwd_offset(l);
         const int mt = jj / (4 * 64), ks = (jj / 64) % 4, lane = jj % 64;
         const int i = 32 * mt + (lane & 31), h = lane >> 5;
-        bf16x8 v;
-        for (int e = 0; e < 8; e++) {
-            const int o = kperm(ks, h, e);
-            v[e] = bf(o < N ? params[w_offset(l) + i * N + o] : 0.f);
+        u32x4 v;
+        for (int k = 0; k < 4; k++) {
+            const int o0 = kperm(ks, h, 2 * k), o1 = kperm(ks, h, 2 * k + 1);
+            v[k] = pack2(o0 < N ? params[w_offset(l) + i * N + o0] : 0.f, o1 < N ? params[w_offset(l) + i * N + o1] : 0.f);
         }
         bwd[jb] = v;
     }
@@ -67,26 +71,127 @@ struct TrainArgs {
     float *loss;                        // [gridDim.x]
 };
 
-struct KeepXt {
-    __bf16 *xt;  // [kLayers][64][kXt]
-    __device__ void operator()(int l, int ks, int nt, const bf16x8 &x) const {
-        const int lane = __lane_id(), p = lane & 31, h = lane >> 5;
-        __bf16 *t = xt + l * 64 * kXt;
-#pragma unroll
-        for (int e = 0; e < 8; e++) t[kperm(ks, h, e) * kXt + 32 * nt + p] = x[e];
-    }
+// Four waves per block (one per SIMD of a CU), each on its own tiles of 64
+// records, sharing one LDS gradient accumulator.  A wave keeps every layer's
+// input operands in registers (it has a SIMD's whole register file), so LDS
+// holds only the current layer's transposed tiles, which the loss stage
+// aliases: 4 x 18 KiB + 67 KiB of accumulator.
+constexpr int kTrainWaves = 4;
+struct TrainLds {
+    union {
+        struct {
+            uint16_t xt[64 * kXt];   // X_l^T of the current layer (BF16 bits): feature-major, records contiguous
+            uint16_t dyt[64 * kXt];  // dY_l^T, likewise
+        } t;
+        float stage[64 * kStageStride];  // raw outputs, then d loss / d outputs
+    };
 };
 
-__global__ __launch_bounds__(64) void k_train(Net net, TrainArgs a) {
-    __shared__ __attribute__((aligned(16))) __bf16 xt[kLayers * 64 * kXt];
-    __shared__ __attribute__((aligned(16))) __bf16 dyt[64 * kXt];
-    __shared__ float stage[64 * kStageStride];
+// X^T of one layer's input operands into LDS (record-contiguous rows).
+template <int KS>
+__device__ __forceinline__ void store_xt(uint16_t *xt, const u32x4 (*x)[2]) {
+    const int lane = __lane_id(), p = lane & 31, h = lane >> 5;
+#pragma unroll
+    for (int ks = 0; ks < KS; ks++)
+#pragma unroll
+        for (int nt = 0; nt < 2; nt++)
+#pragma unroll
+            for (int e = 0; e < 8; e++)
+                xt[kperm(ks, h, e) * kXt + 32 * nt + p] = uint16_t(x[ks][nt][e >> 1] >> (16 * (e & 1)));
+}
+
+// Backward through layer l of one tile: dY_l in dacc (D layout), the layer's
+// input operands in x.  Adds dW_l and db_l into gacc and, for l > 0, leaves
+// dY_{l-1} (ReLU mask of the input applied) in dacc.
+template <int l>
+__device__ __forceinline__ void backward_layer(const Net &net, TrainLds &L, float *gacc, const u32x4 (*x)[2],
+                                               f32x16 dacc[2][2]) {
+    constexpr int K = layer_in(l), N = layer_out(l);
+    const int lane = __lane_id(), p = lane & 31, h = lane >> 5;
+#pragma unroll
+    for (int mt = 0; mt < 2; mt++)
+#pragma unroll
+        for (int nt = 0; nt < 2; nt++)
+#pragma unroll
+            for (int v = 0; v < 16; v++) L.t.dyt[(32 * mt + drow(v, h)) * kXt + 32 * nt + p] = uint16_t(bf_bits(dacc[mt][nt][v]));
+    store_xt<K / 16>(L.t.xt, x);
+    wave_sync_lds();
+    // bias gradient: row sums of dY^T
+    if (lane < N) {
+        float s[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int r = 0; r < 64; r += 8) {
+            const u32x4 v = *reinterpret_cast<const u32x4 *>(L.t.dyt + lane * kXt + r);
+#pragma unroll
+            for (int e = 0; e < 8; e++) s[e & 3] += op_elem(v, e);
+        }
+        atomicAdd(&gacc[b_offset(l) + lane], (s[0] + s[1]) + (s[2] + s[3]));
+    }
+    // weight gradient dW[i][o] = sum_r X[r][i] dY[r][o]: MFMA over the records
+#pragma unroll
+    for (int mt = 0; mt < K / 32; mt++)
+#pragma unroll
+        for (int nt = 0; nt < 2; nt++) {
+            if (32 * nt >= N) continue;
+            f32x16 acc = f32x16{};
+#pragma unroll
+            for (int ks = 0; ks < 4; ks++) {
+                const u32x4 av = *reinterpret_cast<const u32x4 *>(L.t.xt + (32 * mt + p) * kXt + 16 * ks + 8 * h);
+                const u32x4 bv = *reinterpret_cast<const u32x4 *>(L.t.dyt + (32 * nt + p) * kXt + 16 * ks + 8 * h);
+                acc = mfma(av, bv, acc);
+            }
+            const int o = 32 * nt + p;
+            if (o < N) {
+#pragma unroll
+                for (int v = 0; v < 16; v++) atomicAdd(&gacc[w_offset(l) + (32 * mt + drow(v, h)) * N + o], acc[v]);
+            }
+        }
+    if constexpr (l > 0) {
+        // backward data: dX^T = W_l dY^T, operands straight from dacc (the same BF16
+        // values as dyt); the input's ReLU mask sits in the same lane and slot
+        u32x4 b[4][2];
+#pragma unroll
+        for (int ks = 0; ks < 4; ks++)
+#pragma unroll
+            for (int nt = 0; nt < 2; nt++)
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+                    b[ks][nt][i] = pack2(dacc[ks >> 1][nt][8 * (ks & 1) + 2 * i], dacc[ks >> 1][nt][8 * (ks & 1) + 2 * i + 1]);
+        const u32x4 *ops = net.bwd + bwd_offset(l);
+#pragma unroll
+        for (int mt = 0; mt < 2; mt++) {
+            f32x16 d0 = f32x16{}, d1 = f32x16{};
+#pragma unroll
+            for (int ks = 0; ks < 4; ks++) {
+                const u32x4 av = ops[(mt * 4 + ks) * 64 + lane];
+                d0 = mfma(av, b[ks][0], d0);
+                d1 = mfma(av, b[ks][1], d1);
+            }
+#pragma unroll
+            for (int v = 0; v < 16; v++) {
+                // ReLU outputs are >= 0: the mask is "BF16 bits non-zero"
+                const int w = (v & 7) >> 1, sh = 16 * (v & 1);
+                d0[v] = ((x[2 * mt + (v >> 3)][0][w] >> sh) & 0xFFFFu) ? d0[v] : 0.f;
+                d1[v] = ((x[2 * mt + (v >> 3)][1][w] >> sh) & 0xFFFFu) ? d1[v] : 0.f;
+            }
+            dacc[mt][0] = d0;
+            dacc[mt][1] = d1;
+        }
+    }
+    wave_sync_lds();
+}
+
+__global__ __launch_bounds__(64 * kTrainWaves) void k_train(Net net, TrainArgs a) {
+    __shared__ __attribute__((aligned(16))) TrainLds wl[kTrainWaves];
     __shared__ float gacc[kNumParams];
-    const int lane = threadIdx.x, p = lane & 31, h = lane >> 5;
-    for (int j = lane; j < kNumParams; j += 64) gacc[j] = 0.f;
+    __shared__ float wave_loss[kTrainWaves];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, p = lane & 31, h = lane >> 5;
+    for (int j = threadIdx.x; j < kNumParams; j += blockDim.x) gacc[j] = 0.f;
+    __syncthreads();
+    TrainLds &L = wl[wave];
     float loss = 0.f;
     const uint32_t tiles = (a.batch + 63) / 64;
-    for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+    for (uint32_t tile = blockIdx.x * kTrainWaves + wave; tile < tiles; tile += gridDim.x * kTrainWaves) {
         const uint32_t jr = tile * 64 + lane;
         const bool valid = jr < a.batch;
         Enc enc;
@@ -99,13 +204,53 @@ __global__ __launch_bounds__(64) void k_train(Net net, TrainArgs a) {
             wi = xyz(d);
             w = d.w * a.wscale;
         } else {
+#pragma unroll
             for (int k = 0; k < 7; k++) enc.v[k] = 0.f;
         }
+        // forward, keeping every layer's input operands
+        u32x4 x0[2][2], x1[4][2], x2[4][2], x3[4][2], x4[4][2];
+        input_operands(enc, x0);
+#pragma unroll
+        for (int mt = 0; mt < 2; mt++) {
+            f32x16 acc[2];
+            tile_fwd<kIn>(net.fwd + fwd_offset(0), net.bias + 64 * 0, kHid, x0, mt, acc);
+            tile_to_operands(acc, mt, x1);
+        }
+#pragma unroll
+        for (int mt = 0; mt < 2; mt++) {
+            f32x16 acc[2];
+            tile_fwd<kHid>(net.fwd + fwd_offset(1), net.bias + 64 * 1, kHid, x1, mt, acc);
+            tile_to_operands(acc, mt, x2);
+        }
+#pragma unroll
+        for (int mt = 0; mt < 2; mt++) {
+            f32x16 acc[2];
+            tile_fwd<kHid>(net.fwd + fwd_offset(2), net.bias + 64 * 2, kHid, x2, mt, acc);
+            tile_to_operands(acc, mt, x3);
+        }
+#pragma unroll
+        for (int mt = 0; mt < 2; mt++) {
+            f32x16 acc[2];
+            tile_fwd<kHid>(net.fwd + fwd_offset(3), net.bias + 64 * 3, kHid, x3, mt, acc);
+            tile_to_operands(acc, mt, x4);
+        }
+#pragma unroll
+        for (int mt = 0; mt < 2; mt++) {
+            f32x16 acc[2];
+            tile_fwd<kHid>(net.fwd + fwd_offset(4), net.bias + 64 * 4, kOut, x4, mt, acc);
+#pragma unroll
+            for (int v = 0; v < 16; v++) {
+                const int o = 32 * mt + drow(v, h);
+                if (o < kOut) {
+                    L.stage[p * kStageStride + o] = acc[0][v];
+                    L.stage[(32 + p) * kStageStride + o] = acc[1][v];
+                }
+            }
+        }
         wave_sync_lds();
-        forward(net, enc, stage, KeepXt{xt});
         // loss gradient of this lane's record, written over its outputs
         {
-            float *row = stage + lane * kStageStride;
+            float *row = L.stage + lane * kStageStride;
             const float l = loss_grad(row, wi, w, row);
             if (valid) loss += l;
             else
@@ -120,88 +265,25 @@ __global__ __launch_bounds__(64) void k_train(Net net, TrainArgs a) {
 #pragma unroll
                 for (int v = 0; v < 16; v++) {
                     const int o = 32 * mt + drow(v, h);
-                    dacc[mt][nt][v] = o < kOut ? stage[(32 * nt + p) * kStageStride + o] : 0.f;
+                    dacc[mt][nt][v] = o < kOut ? L.stage[(32 * nt + p) * kStageStride + o] : 0.f;
                 }
-#pragma unroll 1
-        for (int l = kLayers - 1; l >= 0; l--) {
-            const int K = layer_in(l), N = layer_out(l);
-            // dY_l^T as BF16, feature-major (the weight-gradient operand)
-#pragma unroll
-            for (int mt = 0; mt < 2; mt++)
-#pragma unroll
-                for (int nt = 0; nt < 2; nt++)
-#pragma unroll
-                    for (int v = 0; v < 16; v++) dyt[(32 * mt + drow(v, h)) * kXt + 32 * nt + p] = bf(dacc[mt][nt][v]);
-            wave_sync_lds();
-            // bias gradient: row sums of dY^T (16-byte reads, independent partial sums)
-            if (lane < N) {
-                float s[4] = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-                for (int r = 0; r < 64; r += 8) {
-                    const bf16x8 v = *reinterpret_cast<const bf16x8 *>(dyt + lane * kXt + r);
-#pragma unroll
-                    for (int e = 0; e < 8; e++) s[e & 3] += fb(v[e]);
-                }
-                gacc[b_offset(l) + lane] += (s[0] + s[1]) + (s[2] + s[3]);
-            }
-            // weight gradient dW[i][o] = sum_r X[r][i] dY[r][o]: MFMA over the records
-            const __bf16 *x = xt + l * 64 * kXt;
-#pragma unroll
-            for (int mt = 0; mt < 2; mt++)
-#pragma unroll
-                for (int nt = 0; nt < 2; nt++) {
-                    if (mt >= K / 32) continue;
-                    f32x16 acc = f32x16{};
-#pragma unroll
-                    for (int ks = 0; ks < 4; ks++) {
-                        const bf16x8 av = *reinterpret_cast<const bf16x8 *>(x + (32 * mt + p) * kXt + 16 * ks + 8 * h);
-                        const bf16x8 bv = *reinterpret_cast<const bf16x8 *>(dyt + (32 * nt + p) * kXt + 16 * ks + 8 * h);
-                        acc = mfma(av, bv, acc);
-                    }
-                    const int o = 32 * nt + p;
-                    if (o < N) {
-#pragma unroll
-                        for (int v = 0; v < 16; v++) gacc[w_offset(l) + (32 * mt + drow(v, h)) * N + o] += acc[v];
-                    }
-                }
-            if (l > 0) {
-                // backward data: dX^T = W_l dY^T, operands straight from dacc (same
-                // BF16 values as dyt), then the ReLU mask of the layer's input
-                bf16x8 b[4][2];
-#pragma unroll
-                for (int ks = 0; ks < 4; ks++)
-#pragma unroll
-                    for (int nt = 0; nt < 2; nt++)
-#pragma unroll
-                        for (int e = 0; e < 8; e++) b[ks][nt][e] = bf(dacc[ks >> 1][nt][8 * (ks & 1) + e]);
-                const bf16x8 *ops = net.bwd + bwd_offset(l);
-#pragma unroll
-                for (int mt = 0; mt < 2; mt++) {
-                    f32x16 d0 = f32x16{}, d1 = f32x16{};
-#pragma unroll
-                    for (int ks = 0; ks < 4; ks++) {
-                        const bf16x8 av = ops[(mt * 4 + ks) * 64 + lane];
-                        d0 = mfma(av, b[ks][0], d0);
-                        d1 = mfma(av, b[ks][1], d1);
-                    }
-#pragma unroll
-                    for (int v = 0; v < 16; v++) {
-                        const int i = 32 * mt + drow(v, h);
-                        d0[v] = fb(x[i * kXt + p]) > 0.f ? d0[v] : 0.f;
-                        d1[v] = fb(x[i * kXt + 32 + p]) > 0.f ? d1[v] : 0.f;
-                    }
-                    dacc[mt][0] = d0;
-                    dacc[mt][1] = d1;
-                }
-            }
-            wave_sync_lds();
-        }
+        wave_sync_lds();  // the stage is read before the transposed tiles overwrite it
+        backward_layer<4>(net, L, gacc, x4, dacc);
+        backward_layer<3>(net, L, gacc, x3, dacc);
+        backward_layer<2>(net, L, gacc, x2, dacc);
+        backward_layer<1>(net, L, gacc, x1, dacc);
+        backward_layer<0>(net, L, gacc, x0, dacc);
     }
-    wave_sync_lds();
-    float *dst = a.partials + size_t(blockIdx.x) * kNumParams;
-    for (int j = lane; j < kNumParams; j += 64) dst[j] = gacc[j];
     for (int off = 32; off > 0; off >>= 1) loss += __shfl_xor(loss, off);
-    if (lane == 0) a.loss[blockIdx.x] = loss;
+    if (lane == 0) wave_loss[wave] = loss;
+    __syncthreads();
+    float *dst = a.partials + size_t(blockIdx.x) * kNumParams;
+    for (int j = threadIdx.x; j < kNumParams; j += blockDim.x) dst[j] = gacc[j];
+    if (threadIdx.x == 0) {
+        float s = 0.f;
+        for (int k = 0; k < kTrainWaves; k++) s += wave_loss[k];
+        a.loss[blockIdx.x] = s;
+    }
 }
 
 // ---------------------------------------------------------------- adam
@@ -213,11 +295,20 @@ struct AdamArgs {
     float beta1, beta2, eps_t;  // eps * sqrt(1 - beta2^t)
     float *grad_out;            // non-null: store the summed gradient, no update
 };
-__global__ void k_adam(AdamArgs a) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= kNumParams) return;
-    float g = 0.f;
-    for (int b = 0; b < a.blocks; b++) g += a.partials[size_t(b) * kNumParams + j];
+// 64 parameters per block: 4 threads per parameter sum interleaved block
+// partials (coalesced over the parameters), then combine in LDS.
+constexpr int kAdamParams = 64, kAdamBlock = 256;
+__global__ __launch_bounds__(kAdamBlock) void k_adam(AdamArgs a) {
+    __shared__ float part[kAdamBlock];
+    const int pl = threadIdx.x % kAdamParams, q = threadIdx.x / kAdamParams;
+    const int j = blockIdx.x * kAdamParams + pl;
+    float s = 0.f;
+    if (j < kNumParams)
+        for (int b = q; b < a.blocks; b += kAdamBlock / kAdamParams) s += a.partials[size_t(b) * kNumParams + j];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    if (q != 0 || j >= kNumParams) return;
+    const float g = (part[pl] + part[pl + kAdamParams]) + (part[pl + 2 * kAdamParams] + part[pl + 3 * kAdamParams]);
     if (a.grad_out) {
         a.grad_out[j] = g;
         return;
@@ -275,9 +366,10 @@ __global__ __launch_bounds__(kTestBlock) void k_mixture_test(const float *raw, c
     if (r >= n) return;
     float *L = stage + threadIdx.x * kStageStride;
     for (int k = 0; k < kOut; k++) L[k] = raw[size_t(r) * kOut + k];
-    decode(L);
+    float w[kLobes];
+    decode(L, w);
     pdf_dirs[r] = pdf(L, f3(dirs[3 * r], dirs[3 * r + 1], dirs[3 * r + 2]));
-    const float3 s = sample(L, u[3 * r], u[3 * r + 1], u[3 * r + 2]);
+    const float3 s = sample(L, w, u[3 * r], u[3 * r + 1], u[3 * r + 2]);
     samples[3 * r] = s.x;
     samples[3 * r + 1] = s.y;
     samples[3 * r + 2] = s.z;
@@ -291,13 +383,15 @@ struct wr_guide {
     wr_scene *scene = nullptr;
     wr_guide_config cfg{};
     float *d_params = nullptr, *d_m = nullptr, *d_v = nullptr;
-    bf16x8 *d_fwd = nullptr, *d_bwd = nullptr;
+    u32x4 *d_fwd = nullptr, *d_bwd = nullptr;
+    float *d_bias = nullptr;  // packed biases (kBiasPacked)
     float *d_partials = nullptr, *d_loss = nullptr;
     double *d_wsum = nullptr;
     Records rec{};
     void *rec_pool = nullptr;
-    float4 *d_lobes = nullptr;  // decoded lobes of the path pool (kLobeVecs float4 per entry, SoA)
-    uint32_t lobe_capacity = 0;
+    float4 *d_draw_dir = nullptr;  // per path-pool entry: the inference kernel's direction and q there
+    float *d_draw_qnee = nullptr;  // and q toward the light sample (same allocation)
+    uint32_t draw_capacity = 0;
     uint64_t step = 0;
     int train_blocks = 0;
     double last_loss = 0.0;
@@ -343,14 +437,17 @@ Net net_view(const wr_guide *g) {
     Net n;
     n.fwd = g->d_fwd;
     n.bwd = g->d_bwd;
-    n.params = g->d_params;
-    for (int k = 0; k < 6; k++) n.box[k] = g->scene->box[k];
+    n.bias = g->d_bias;
+    for (int k = 0; k < 3; k++) {
+        n.box[k] = g->scene->box[k];
+        n.box[3 + k] = float(1.0 / std::max(double(g->scene->box[3 + k]) - double(g->scene->box[k]), 1e-20));
+    }
     return n;
 }
 
 int pack(wr_guide *g) {
     const int n = kFwdOps + kBwdOps;
-    k_pack<<<(n + 255) / 256, 256, 0, g->scene->stream>>>(g->d_params, g->d_fwd, g->d_bwd);
+    k_pack<<<(n + 255) / 256, 256, 0, g->scene->stream>>>(g->d_params, g->d_fwd, g->d_bwd, g->d_bias);
     WR_HIP(hipGetLastError());
     return WR_OK;
 }
@@ -373,8 +470,8 @@ int train_step(wr_guide *g, const float4 *pos, const float4 *nrm, const float4 *
     a.partials = g->d_partials;
     a.loss = g->d_loss;
     const int tiles = int((batch + 63) / 64);
-    const int blocks = std::max(1, std::min(tiles, g->train_blocks));
-    k_train<<<blocks, 64, 0, st>>>(net_view(g), a);
+    const int blocks = std::max(1, std::min((tiles + kTrainWaves - 1) / kTrainWaves, g->train_blocks));
+    k_train<<<blocks, 64 * kTrainWaves, 0, st>>>(net_view(g), a);
     WR_HIP(hipGetLastError());
     AdamArgs ad;
     ad.params = g->d_params; ad.m = g->d_m; ad.v = g->d_v;
@@ -393,7 +490,7 @@ int train_step(wr_guide *g, const float4 *pos, const float4 *nrm, const float4 *
     }
     ad.beta1 = g->cfg.beta1;
     ad.beta2 = g->cfg.beta2;
-    k_adam<<<(kNumParams + 255) / 256, 256, 0, st>>>(ad);
+    k_adam<<<(kNumParams + kAdamParams - 1) / kAdamParams, kAdamBlock, 0, st>>>(ad);
     WR_HIP(hipGetLastError());
     if (!grad_out) {
         int rc = pack(g);
@@ -435,8 +532,8 @@ struct DeviceRecords {
 
 // ---------------------------------------------------------------- internal interface for the renderer
 int guide_shade_view(wr_guide *g, float alpha, float record_prob, bool collect, guide::Shade *out) {
-    out->lobes = g->d_lobes;
-    out->lobe_stride = g->lobe_capacity;
+    out->draw.dir = g->d_draw_dir;
+    out->draw.q_nee = g->d_draw_qnee;
     out->rec = g->rec;
     out->alpha = alpha;
     out->record_prob = record_prob;
@@ -500,14 +597,16 @@ int guide_end_pass(wr_guide *g, const float4 *sample_rad, uint64_t samples, floa
 
 guide::Net guide_net(const wr_guide *g) { return net_view(g); }
 
-int guide_reserve_lobes(wr_guide *g, uint32_t capacity) {
-    if (g->lobe_capacity >= capacity) return WR_OK;
+int guide_reserve_draws(wr_guide *g, uint32_t capacity) {
+    if (g->draw_capacity >= capacity) return WR_OK;
     WR_HIP(hipStreamSynchronize(g->scene->stream));
-    if (g->d_lobes) WR_HIP(hipFree(g->d_lobes));
-    g->d_lobes = nullptr;
-    g->lobe_capacity = 0;
-    WR_HIP(hipMalloc(reinterpret_cast<void **>(&g->d_lobes), sizeof(float4) * kLobeVecs * size_t(capacity)));
-    g->lobe_capacity = capacity;
+    if (g->d_draw_dir) WR_HIP(hipFree(g->d_draw_dir));
+    g->d_draw_dir = nullptr;
+    g->d_draw_qnee = nullptr;
+    g->draw_capacity = 0;
+    WR_HIP(hipMalloc(reinterpret_cast<void **>(&g->d_draw_dir), (sizeof(float4) + sizeof(float)) * size_t(capacity)));
+    g->d_draw_qnee = reinterpret_cast<float *>(g->d_draw_dir + capacity);
+    g->draw_capacity = capacity;
     return WR_OK;
 }
 wr_scene *guide_scene(const wr_guide *g) { return g->scene; }
@@ -528,8 +627,8 @@ void wr_guide_destroy(wr_guide *g) {
     if (!g) return;
     (void)hipSetDevice(g->scene->dev->ordinal);
     (void)hipStreamSynchronize(g->scene->stream);
-    for (void *p : {(void *)g->d_params, (void *)g->d_m, (void *)g->d_v, (void *)g->d_fwd, (void *)g->d_bwd,
-                    (void *)g->d_partials, (void *)g->d_loss, (void *)g->d_wsum, g->rec_pool, (void *)g->d_lobes})
+    for (void *p : {(void *)g->d_params, (void *)g->d_m, (void *)g->d_v, (void *)g->d_fwd, (void *)g->d_bwd, (void *)g->d_bias,
+                    (void *)g->d_partials, (void *)g->d_loss, (void *)g->d_wsum, g->rec_pool, (void *)g->d_draw_dir})
         if (p) (void)hipFree(p);
     delete g;
 }
@@ -542,8 +641,9 @@ static int guide_create_impl(wr_guide *g) {
     WR_HIP(hipMalloc(reinterpret_cast<void **>(&g->d_params), sizeof(float) * kNumParams));
     WR_HIP(hipMalloc(reinterpret_cast<void **>(&g->d_m), sizeof(float) * kNumParams));
     WR_HIP(hipMalloc(reinterpret_cast<void **>(&g->d_v), sizeof(float) * kNumParams));
-    WR_HIP(hipMalloc(reinterpret_cast<void **>(&g->d_fwd), sizeof(bf16x8) * kFwdOps));
-    WR_HIP(hipMalloc(reinterpret_cast<void **>(&g->d_bwd), sizeof(bf16x8) * kBwdOps));
+    WR_HIP(hipMalloc(reinterpret_cast<void **>(&g->d_fwd), sizeof(u32x4) * kFwdOps));
+    WR_HIP(hipMalloc(reinterpret_cast<void **>(&g->d_bwd), sizeof(u32x4) * kBwdOps));
+    WR_HIP(hipMalloc(reinterpret_cast<void **>(&g->d_bias), sizeof(float) * kBiasPacked));
     WR_HIP(hipMalloc(reinterpret_cast<void **>(&g->d_partials), sizeof(float) * kNumParams * size_t(g->train_blocks)));
     WR_HIP(hipMalloc(reinterpret_cast<void **>(&g->d_loss), sizeof(float) * g->train_blocks));
     WR_HIP(hipMalloc(reinterpret_cast<void **>(&g->d_wsum), sizeof(double)));

@@ -146,6 +146,6 @@ int guide_begin_pass(wr_guide *g, uint64_t samples, float *record_prob);
 // Training weights from the pass's final per-sample radiance, then `steps` Adam steps.
 int guide_end_pass(wr_guide *g, const float4 *sample_rad, uint64_t samples, float record_prob, uint32_t steps,
                    wr_guide_stats *gs);
-// Sizes the decoded-lobe buffer for a path pool of `capacity` entries.
-int guide_reserve_lobes(wr_guide *g, uint32_t capacity);
+// Sizes the per-entry draws of the inference kernel for a path pool of `capacity` entries.
+int guide_reserve_draws(wr_guide *g, uint32_t capacity);
 wr_scene *guide_scene(const wr_guide *g);

@@ -433,12 +433,12 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(DScene s, PathBuffers pb,
 // k_shade with learned guiding and/or a homogeneous medium.
 //
 // Guiding: the direction at a vertex is drawn by one-sample MIS from the
-// network's lobe mixture (probability alpha) or the BSDF / phase function,
+// network's lobe mixture (probability alpha) or the BSDF / phase function, and
 // weighted by f / (alpha q + (1 - alpha) p); the same combined density enters
 // the NEE and emitter-hit MIS weights, so the estimator stays unbiased for any
-// network.  The lobes come from k_guide_infer.  Selected paths append a
-// training record at every scattering vertex; the record is closed when the
-// path's next vertex is shaded (radiance snapshot).
+// network.  k_guide_infer draws the direction and evaluates q.  Selected paths
+// append a training record at every scattering vertex; the record is closed
+// when the path's next vertex is shaded (radiance snapshot).
 //
 // Medium (homogeneous.h:11-66, generalsubpathsampler.h:40-53,131-154): each
 // segment first samples a free flight on one colour channel; a flight that ends
@@ -451,29 +451,49 @@ __device__ __forceinline__ float guide_alpha(const DMat &m, float frac) {
     return m.type == kPhong ? frac * fminf(1.f, fmaxf(0.1f, 30.f / m.n)) : frac;
 }
 
-// Network inference for the current state: the decoded lobes of every vertex
-// that k_shade_ext samples a direction at -- a surface that is not an emitter
-// or the last vertex, or (with a medium) a scattering event before the last
-// vertex -- one forward pass per wave of 64 entries on MFMA.  The free-flight
-// decision repeats k_shade_ext's (same random numbers).
+// Network inference for the current state, for every vertex that k_shade_ext
+// samples a direction at -- a surface that is not an emitter or the last
+// vertex, or (with a medium) a scattering event before the last vertex -- one
+// forward pass per wave of 64 entries on MFMA.  With the lobes on chip it also
+// evaluates what shading needs of them: it draws the vertex's direction (the
+// network with probability alpha, else the BSDF or phase function, from the
+// same random numbers k_shade would use) and returns q at that direction and
+// at the direction of the vertex's light sample.  The free-flight decision
+// repeats k_shade_ext's.
+#ifndef WR_INFER_WAVES
+#define WR_INFER_WAVES 2
+#endif
+constexpr int kInferBlock = 256;  // 4 waves share one LDS copy of the weights
 template <bool MEDIUM>
-__global__ __launch_bounds__(kBlock) void k_guide_infer(DScene s, PathState cs, const uint2 *__restrict__ hit,
-                                                        const uint32_t *__restrict__ counters, int N, guide::Net net,
-                                                        float4 *lobes, uint32_t stride) {
-    __shared__ float stage[64 * guide::kStageStride];
+__global__ __launch_bounds__(kInferBlock) __attribute__((amdgpu_waves_per_eu(WR_INFER_WAVES, WR_INFER_WAVES))) void k_guide_infer(
+    DScene s, PathState cs, const uint2 *__restrict__ hit, const uint32_t *__restrict__ counters, int N, guide::Net gnet,
+    float frac, guide::Draw out) {
+    __shared__ guide::u32x4 w_ops[guide::kFwdOps];
+    __shared__ float w_bias[guide::kBiasPacked];
+    __shared__ float stage_all[kInferBlock * guide::kStageStride];
     const uint32_t n = active_count(counters);
-    for (uint32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
+    if (blockIdx.x * kInferBlock >= n) return;
+    for (int j = threadIdx.x; j < guide::kFwdOps; j += kInferBlock) w_ops[j] = gnet.fwd[j];
+    for (int j = threadIdx.x; j < guide::kBiasPacked; j += kInferBlock) w_bias[j] = gnet.bias[j];
+    __syncthreads();
+    guide::Net net = gnet;
+    net.fwd = w_ops;
+    net.bias = w_bias;
+    float *stage = stage_all + (threadIdx.x & ~63u) * guide::kStageStride;
+    const uint32_t lane = threadIdx.x & 63u;
+    for (uint32_t base = blockIdx.x * kInferBlock; base < n; base += gridDim.x * kInferBlock) {
         const uint32_t i = base + threadIdx.x;
-        bool need = false;
-        float3 pos = f3(0.f, 0.f, 0.f), nf = f3(0.f, 1.f, 0.f), rd = f3(0.f, 0.f, 1.f);
+        bool need = false, med = false;
+        float3 pos = f3(0.f, 0.f, 0.f), ng = f3(0.f, 1.f, 0.f), nf = ng, rd = f3(0.f, 0.f, 1.f);
+        int mat = 0;
+        uint4 meta = make_uint4(0, 0, 0, 0);
         if (i < n) {
             const uint2 h = hit[i];
-            const uint4 meta = cs.meta[i];
+            meta = cs.meta[i];
             if (int(meta.z) + 2 != N) {
                 float tf = 0.f;
-                const bool scatter =
-                    MEDIUM && free_flight(s.med, meta.x, meta.z, h.x != kInvalid ? __uint_as_float(h.y) : INFINITY, tf);
-                if (scatter) {
+                med = MEDIUM && free_flight(s.med, meta.x, meta.z, h.x != kInvalid ? __uint_as_float(h.y) : INFINITY, tf);
+                if (med) {
                     need = true;
                     rd = xyz(cs.ray_d[i]);
                     pos = xyz(cs.ray_o[i]) + rd * tf;
@@ -482,11 +502,13 @@ __global__ __launch_bounds__(kBlock) void k_guide_infer(DScene s, PathState cs, 
                     const float4 v0 = s.tris[3 * size_t(h.x) + 0];
                     const float4 v1 = s.tris[3 * size_t(h.x) + 1];
                     const float4 v2 = s.tris[3 * size_t(h.x) + 2];
-                    if (s.geom_info[__float_as_uint(v1.w)].y < 0) {
+                    const int2 gi = s.geom_info[__float_as_uint(v1.w)];
+                    if (gi.y < 0) {
                         need = true;
+                        mat = gi.x;
                         rd = xyz(cs.ray_d[i]);
                         pos = xyz(cs.ray_o[i]) + rd * __uint_as_float(h.y);
-                        const float3 ng = normalize3(cross3(xyz(v1) - xyz(v0), xyz(v2) - xyz(v0)));
+                        ng = normalize3(cross3(xyz(v1) - xyz(v0), xyz(v2) - xyz(v0)));
                         nf = dot3(ng, rd) > 0.f ? -ng : ng;
                     }
                 }
@@ -495,11 +517,34 @@ __global__ __launch_bounds__(kBlock) void k_guide_infer(DScene s, PathState cs, 
         if (__ballot(need)) {
             guide::forward(net, guide::encode(pos, nf, -rd, net.box), stage);
             if (need) {
-                float *L = stage + threadIdx.x * guide::kStageStride;
-                guide::decode(L);
-#pragma unroll
-                for (int j = 0; j < guide::kLobeVecs; j++)
-                    lobes[size_t(j) * stride + i] = make_float4(L[4 * j], L[4 * j + 1], L[4 * j + 2], L[4 * j + 3]);
+                float *L = stage + lane * guide::kStageStride;
+                float wl[guide::kLobes];
+                guide::decode(L, wl);
+                const uint32_t key = meta.x;
+                const LightSample l = sample_light(s, u01(key, kDimLight), u01(key, kDimLight + 1));
+                const float q_nee = guide::pdf(L, normalize3(l.pos - pos));
+                const uint32_t dim = kDimBounce0 + kDimsPerBounce * meta.z + kBdBsdf;
+                const float u0 = u01(key, dim), u1 = u01(key, dim + 1), us = u01(key, dim + 2);
+                float alpha = frac;
+                DMat m{};
+                if (!med) {
+                    m = s.mats[mat];
+                    alpha = guide_alpha(m, frac);
+                }
+                float3 d;
+                if (us < alpha) {
+                    d = guide::sample(L, wl, us * guide::frcp(alpha), u0, u1);
+                } else if (med) {
+                    d = hg_sample(s.med.g, rd, u0, u1);
+                } else {
+                    const Frame f = make_frame(ng);
+                    float3 wo = f3(0.f, 0.f, 0.f);
+                    float pb = 0.f;
+                    bsdf_sample(m, to_local(f, -rd), u0, u1, wo, pb);
+                    d = normalize3(to_world(f, wo));
+                }
+                out.dir[i] = make_float4(d.x, d.y, d.z, guide::pdf(L, d));
+                out.q_nee[i] = q_nee;
             }
         }
     }
@@ -508,9 +553,6 @@ __global__ __launch_bounds__(kBlock) void k_guide_infer(DScene s, PathState cs, 
 template <bool GUIDED, bool MEDIUM>
 __global__ __launch_bounds__(kShadeBlock) void k_shade_ext(DScene s, PathBuffers pb, RenderArgs a, guide::Shade g) {
     __shared__ uint32_t agg[kWavesPerShadeBlock + 1];
-    // this thread's lobes, staged in LDS (a 40-float register array would live in scratch)
-    __shared__ float lobes[GUIDED ? kShadeBlock * guide::kStageStride : 1];
-    float *lob = lobes + (GUIDED ? threadIdx.x * guide::kStageStride : 0);
     const uint32_t n = active_count(pb.counters);
     const PathState cs = pb.cur, ns = pb.next;
     const float3 zero = f3(0.f, 0.f, 0.f);
@@ -551,12 +593,11 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade_ext(DScene s, PathBuffers
         const bool med = valid && in_medium && !last;
         const float3 nf = in_medium ? -rd : (dot3(ng, rd) > 0.f ? -ng : ng);  // faces the incoming ray
         const bool need = GUIDED && (surf || med) && g.alpha > 0.f;
+        float4 gdir = make_float4(0.f, 0.f, 0.f, 0.f);  // k_guide_infer's direction and q there
+        float q_nee = 0.f;
         if (GUIDED && need) {
-#pragma unroll
-            for (int j = 0; j < guide::kLobeVecs; j++) {
-                const float4 q = g.lobes[size_t(j) * g.lobe_stride + i];
-                lob[4 * j] = q.x; lob[4 * j + 1] = q.y; lob[4 * j + 2] = q.z; lob[4 * j + 3] = q.w;
-            }
+            gdir = g.draw.dir[i];
+            q_nee = g.draw.q_nee[i];
         }
         // ---- shading
         bool push_shadow = false, cont = false;
@@ -629,7 +670,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade_ext(DScene s, PathBuffers
                 if (!is_black(fs)) {
                     float mis = 1.f;
                     if (a.use_mis) {
-                        if (GUIDED && need) p = alpha * guide::pdf(lob, nuv) + (1.f - alpha) * p;
+                        if (GUIDED && need) p = alpha * q_nee + (1.f - alpha) * p;
                         float dirPdfA = p * fabsf(cos_lv) / len2;
                         if (MEDIUM) dirPdfA *= pass_pdf(s.med, len);
                         mis = l.pdfA / (l.pdfA + dirPdfA);
@@ -650,9 +691,10 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade_ext(DScene s, PathBuffers
             const float u0 = u01(key, dim), u1 = u01(key, dim + 1);
             float3 nd = zero, w = zero;
             float pdf = 0.f;
-            const float us = GUIDED ? u01(key, dim + 2) : 1.f;
-            if (GUIDED && need && us < alpha) {
-                nd = guide::sample(lob, us / alpha, u0, u1);
+            if (GUIDED && need) {
+                // the direction k_guide_infer drew from the mixture alpha q + (1 - alpha) p;
+                // whichever component drew it, its weight is f cos / (alpha q + (1 - alpha) p)
+                nd = xyz(gdir);
                 float3 fv;
                 float pd;
                 if (med) {
@@ -663,23 +705,16 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade_ext(DScene s, PathBuffers
                     fv = bsdf_eval(m, wi, wo) * fabsf(wo.y);
                     pd = bsdf_pdf(m, wi, wo);
                 }
-                pdf = alpha * guide::pdf(lob, nd) + (1.f - alpha) * pd;
+                pdf = alpha * gdir.w + (1.f - alpha) * pd;
                 if (pdf > 0.f && !is_black(fv)) w = fv * (1.f / pdf);
+            } else if (med) {
+                nd = hg_sample(s.med.g, rd, u0, u1);
+                pdf = hg_pdf(s.med.g, dot3(rd, nd));
+                w = f3(1.f, 1.f, 1.f);
             } else {
-                if (med) {
-                    nd = hg_sample(s.med.g, rd, u0, u1);
-                    pdf = hg_pdf(s.med.g, dot3(rd, nd));
-                    w = f3(1.f, 1.f, 1.f);
-                } else {
-                    float3 wo = zero;
-                    w = bsdf_sample(m, wi, u0, u1, wo, pdf);
-                    nd = to_world(f, wo);
-                }
-                if (GUIDED && need && !is_black(w)) {
-                    const float pbs = pdf;
-                    pdf = alpha * guide::pdf(lob, nd) + (1.f - alpha) * pbs;
-                    w = w * (pbs / pdf);
-                }
+                float3 wo = zero;
+                w = bsdf_sample(m, wi, u0, u1, wo, pdf);
+                nd = to_world(f, wo);
             }
             if (!is_black(w) && pdf > 0.f) {
                 wdir = nd;
@@ -1311,7 +1346,7 @@ static int render_impl(wr_scene *s, wr_film *f, const wr_render_params *prm, wr_
     rc = ensure_capacity(s, P, samples_pass);
     if (rc) return rc;
     if (g) {
-        rc = guide_reserve_lobes(g, P);
+        rc = guide_reserve_draws(g, P);
         if (rc) return rc;
     }
     PathBuffers pb = s->pb;
@@ -1345,7 +1380,8 @@ static int render_impl(wr_scene *s, wr_film *f, const wr_render_params *prm, wr_
     int pass = 0;
     const void *infer_fn = medium ? reinterpret_cast<const void *>(&k_guide_infer<true>)
                                   : reinterpret_cast<const void *>(&k_guide_infer<false>);
-    const int g_infer = std::max(1, std::min<int>((P + kBlock - 1) / kBlock, s->dev->n_cu * occupancy(s->dev, infer_fn, kBlock)));
+    const int g_infer = std::max(1, std::min<int>((P + kInferBlock - 1) / kInferBlock,
+                                                  s->dev->n_cu * occupancy(s->dev, infer_fn, kInferBlock)));
     const guide::Net net = g ? guide_net(g) : guide::Net{};
     const int g_ext = trav_grid(s, P, s->dev->occ_extend);
     const int g_shd = trav_grid(s, 2 * P, s->dev->occ_shadow);
@@ -1391,13 +1427,12 @@ static int render_impl(wr_scene *s, wr_film *f, const wr_render_params *prm, wr_
             end();
             begin(2);
             if (g && gs.alpha > 0.f) {
-                float4 *lob = const_cast<float4 *>(gs.lobes);
                 if (medium)
-                    k_guide_infer<true><<<g_infer, kBlock, 0, s->stream>>>(s->ds, pb.cur, pb.hit, pb.counters, N, net, lob,
-                                                                           gs.lobe_stride);
+                    k_guide_infer<true><<<g_infer, kInferBlock, 0, s->stream>>>(s->ds, pb.cur, pb.hit, pb.counters, N, net,
+                                                                           gs.alpha, gs.draw);
                 else
-                    k_guide_infer<false><<<g_infer, kBlock, 0, s->stream>>>(s->ds, pb.cur, pb.hit, pb.counters, N, net, lob,
-                                                                            gs.lobe_stride);
+                    k_guide_infer<false><<<g_infer, kInferBlock, 0, s->stream>>>(s->ds, pb.cur, pb.hit, pb.counters, N, net,
+                                                                            gs.alpha, gs.draw);
             }
             if (g && medium) k_shade_ext<true, true><<<g_shade, kShadeBlock, 0, s->stream>>>(s->ds, pb, a, gs);
             else if (g) k_shade_ext<true, false><<<g_shade, kShadeBlock, 0, s->stream>>>(s->ds, pb, a, gs);
