@@ -267,10 +267,11 @@ template <int K>
 __device__ __forceinline__ void tile_fwd(const u32x4 *ops, const float *bias, int N, const u32x4 (*x)[2], int mt,
                                          f32x16 acc[2]) {
     const int lane = __lane_id(), h = lane >> 5;
+    const float *bb = bias + 4 * h;  // per-lane base, compile-time offsets
 #pragma unroll
     for (int v = 0; v < 16; v++) {
-        const int o = 32 * mt + drow(v, h);
-        acc[0][v] = o < N ? bias[o] : 0.f;
+        const int o0 = 32 * mt + drow(v, 0);
+        acc[0][v] = o0 + 4 * h < N ? bb[o0] : 0.f;
     }
     acc[1] = acc[0];
 #pragma unroll
@@ -331,16 +332,17 @@ __device__ __forceinline__ void forward(const Net &net, const Enc &mine, float *
 #pragma unroll
             for (int nt = 0; nt < 2; nt++) x[ks][nt] = xn[ks][nt];
     }
+    float *sb = stage + p * kStageStride + 4 * h;
 #pragma unroll
     for (int mt = 0; mt < 2; mt++) {
         f32x16 acc[2];
         tile_fwd<kHid>(net.fwd + fwd_offset(kLayers - 1), net.bias + 64 * (kLayers - 1), kOut, x, mt, acc);
 #pragma unroll
         for (int v = 0; v < 16; v++) {
-            const int o = 32 * mt + drow(v, h);
-            if (o < kOut) {
+            const int o0 = 32 * mt + drow(v, 0);
+            if (o0 + 4 * h < kOut) {
 #pragma unroll
-                for (int nt = 0; nt < 2; nt++) stage[(32 * nt + p) * kStageStride + o] = acc[nt][v];
+                for (int nt = 0; nt < 2; nt++) sb[32 * nt * kStageStride + o0] = acc[nt][v];
             }
         }
     }

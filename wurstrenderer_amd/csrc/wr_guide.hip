@@ -88,32 +88,32 @@ struct TrainLds {
 };
 
 // X^T of one layer's input operands into LDS (record-contiguous rows).
+// (Addresses are a per-lane base plus compile-time offsets, so they fold into
+// the LDS instructions' immediates instead of occupying registers.)
 template <int KS>
 __device__ __forceinline__ void store_xt(uint16_t *xt, const u32x4 (*x)[2]) {
     const int lane = __lane_id(), p = lane & 31, h = lane >> 5;
+    uint16_t *base = xt + 4 * h * kXt + p;
 #pragma unroll
     for (int ks = 0; ks < KS; ks++)
 #pragma unroll
         for (int nt = 0; nt < 2; nt++)
 #pragma unroll
             for (int e = 0; e < 8; e++)
-                xt[kperm(ks, h, e) * kXt + 32 * nt + p] = uint16_t(x[ks][nt][e >> 1] >> (16 * (e & 1)));
+                base[kperm(ks, 0, e) * kXt + 32 * nt] = uint16_t(x[ks][nt][e >> 1] >> (16 * (e & 1)));
 }
 
-// Backward through layer l of one tile: dY_l in dacc (D layout), the layer's
-// input operands in x.  Adds dW_l and db_l into gacc and, for l > 0, leaves
-// dY_{l-1} (ReLU mask of the input applied) in dacc.
+// Backward through layer l of one tile.  dY_l arrives as packed BF16 operands
+// (dy[ks][nt] element e = dY of output kperm(ks, h, e) of record 32 nt + p, the
+// values every product below uses); the layer's input operands are in x.
+// Adds dW_l and db_l into gacc and, for l > 0, leaves dY_{l-1} (the input's
+// ReLU mask applied) packed in dy.
 template <int l>
 __device__ __forceinline__ void backward_layer(const Net &net, TrainLds &L, float *gacc, const u32x4 (*x)[2],
-                                               f32x16 dacc[2][2]) {
+                                               u32x4 dy[4][2]) {
     constexpr int K = layer_in(l), N = layer_out(l);
     const int lane = __lane_id(), p = lane & 31, h = lane >> 5;
-#pragma unroll
-    for (int mt = 0; mt < 2; mt++)
-#pragma unroll
-        for (int nt = 0; nt < 2; nt++)
-#pragma unroll
-            for (int v = 0; v < 16; v++) L.t.dyt[(32 * mt + drow(v, h)) * kXt + 32 * nt + p] = uint16_t(bf_bits(dacc[mt][nt][v]));
+    store_xt<4>(L.t.dyt, dy);
     store_xt<K / 16>(L.t.xt, x);
     wave_sync_lds();
     // bias gradient: row sums of dY^T
@@ -142,43 +142,54 @@ __device__ __forceinline__ void backward_layer(const Net &net, TrainLds &L, floa
             }
             const int o = 32 * nt + p;
             if (o < N) {
+                float *gb = gacc + w_offset(l) + 4 * h * N + p;
 #pragma unroll
-                for (int v = 0; v < 16; v++) atomicAdd(&gacc[w_offset(l) + (32 * mt + drow(v, h)) * N + o], acc[v]);
+                for (int v = 0; v < 16; v++) atomicAdd(&gb[(32 * mt + drow(v, 0)) * N + 32 * nt], acc[v]);
             }
         }
     if constexpr (l > 0) {
-        // backward data: dX^T = W_l dY^T, operands straight from dacc (the same BF16
-        // values as dyt); the input's ReLU mask sits in the same lane and slot
-        u32x4 b[4][2];
-#pragma unroll
-        for (int ks = 0; ks < 4; ks++)
-#pragma unroll
-            for (int nt = 0; nt < 2; nt++)
-#pragma unroll
-                for (int i = 0; i < 4; i++)
-                    b[ks][nt][i] = pack2(dacc[ks >> 1][nt][8 * (ks & 1) + 2 * i], dacc[ks >> 1][nt][8 * (ks & 1) + 2 * i + 1]);
+        // backward data: dX^T = W_l dY^T; the input's ReLU mask sits in the same lane
+        // and slot as the result, which is packed into dy for the layer below
         const u32x4 *ops = net.bwd + bwd_offset(l);
+        u32x4 nd[4][2];
 #pragma unroll
         for (int mt = 0; mt < 2; mt++) {
             f32x16 d0 = f32x16{}, d1 = f32x16{};
 #pragma unroll
             for (int ks = 0; ks < 4; ks++) {
                 const u32x4 av = ops[(mt * 4 + ks) * 64 + lane];
-                d0 = mfma(av, b[ks][0], d0);
-                d1 = mfma(av, b[ks][1], d1);
+                d0 = mfma(av, dy[ks][0], d0);
+                d1 = mfma(av, dy[ks][1], d1);
             }
 #pragma unroll
-            for (int v = 0; v < 16; v++) {
+            for (int v = 0; v < 16; v += 2) {
                 // ReLU outputs are >= 0: the mask is "BF16 bits non-zero"
-                const int w = (v & 7) >> 1, sh = 16 * (v & 1);
-                d0[v] = ((x[2 * mt + (v >> 3)][0][w] >> sh) & 0xFFFFu) ? d0[v] : 0.f;
-                d1[v] = ((x[2 * mt + (v >> 3)][1][w] >> sh) & 0xFFFFu) ? d1[v] : 0.f;
+                const u32x4 &m0 = x[2 * mt + (v >> 3)][0], &m1 = x[2 * mt + (v >> 3)][1];
+                const int w = (v & 7) >> 1;
+                const float a0 = (m0[w] & 0xFFFFu) ? d0[v] : 0.f, b0 = (m0[w] >> 16) ? d0[v + 1] : 0.f;
+                const float a1 = (m1[w] & 0xFFFFu) ? d1[v] : 0.f, b1 = (m1[w] >> 16) ? d1[v + 1] : 0.f;
+                nd[2 * mt + (v >> 3)][0][w] = pack2(a0, b0);
+                nd[2 * mt + (v >> 3)][1][w] = pack2(a1, b1);
             }
-            dacc[mt][0] = d0;
-            dacc[mt][1] = d1;
         }
+#pragma unroll
+        for (int ks = 0; ks < 4; ks++)
+#pragma unroll
+            for (int nt = 0; nt < 2; nt++) dy[ks][nt] = nd[ks][nt];
     }
     wave_sync_lds();
+}
+
+// One hidden layer of the wave's 64 records: input operands x (K inputs) ->
+// output operands xn (bias + ReLU, BF16).
+template <int K>
+__device__ __forceinline__ void layer_to(const Net &net, int l, const u32x4 (*x)[2], u32x4 xn[4][2]) {
+#pragma unroll
+    for (int mt = 0; mt < 2; mt++) {
+        f32x16 acc[2];
+        tile_fwd<K>(net.fwd + fwd_offset(l), net.bias + 64 * l, kHid, x, mt, acc);
+        tile_to_operands(acc, mt, xn);
+    }
 }
 
 __global__ __launch_bounds__(64 * kTrainWaves) void k_train(Net net, TrainArgs a) {
@@ -207,43 +218,31 @@ __global__ __launch_bounds__(64 * kTrainWaves) void k_train(Net net, TrainArgs a
 #pragma unroll
             for (int k = 0; k < 7; k++) enc.v[k] = 0.f;
         }
-        // forward, keeping every layer's input operands
-        u32x4 x0[2][2], x1[4][2], x2[4][2], x3[4][2], x4[4][2];
+        // forward; the inputs of layers 0, 2 and 4 stay in registers (checkpoints),
+        // those of layers 1 and 3 are recomputed during the backward pass
+        u32x4 x0[2][2], x2[4][2], x4[4][2];
         input_operands(enc, x0);
-#pragma unroll
-        for (int mt = 0; mt < 2; mt++) {
-            f32x16 acc[2];
-            tile_fwd<kIn>(net.fwd + fwd_offset(0), net.bias + 64 * 0, kHid, x0, mt, acc);
-            tile_to_operands(acc, mt, x1);
+        {
+            u32x4 x1[4][2];
+            layer_to<kIn>(net, 0, x0, x1);
+            layer_to<kHid>(net, 1, x1, x2);
         }
-#pragma unroll
-        for (int mt = 0; mt < 2; mt++) {
-            f32x16 acc[2];
-            tile_fwd<kHid>(net.fwd + fwd_offset(1), net.bias + 64 * 1, kHid, x1, mt, acc);
-            tile_to_operands(acc, mt, x2);
-        }
-#pragma unroll
-        for (int mt = 0; mt < 2; mt++) {
-            f32x16 acc[2];
-            tile_fwd<kHid>(net.fwd + fwd_offset(2), net.bias + 64 * 2, kHid, x2, mt, acc);
-            tile_to_operands(acc, mt, x3);
-        }
-#pragma unroll
-        for (int mt = 0; mt < 2; mt++) {
-            f32x16 acc[2];
-            tile_fwd<kHid>(net.fwd + fwd_offset(3), net.bias + 64 * 3, kHid, x3, mt, acc);
-            tile_to_operands(acc, mt, x4);
+        {
+            u32x4 x3[4][2];
+            layer_to<kHid>(net, 2, x2, x3);
+            layer_to<kHid>(net, 3, x3, x4);
         }
 #pragma unroll
         for (int mt = 0; mt < 2; mt++) {
             f32x16 acc[2];
             tile_fwd<kHid>(net.fwd + fwd_offset(4), net.bias + 64 * 4, kOut, x4, mt, acc);
+            float *sb = L.stage + p * kStageStride + 4 * h;
 #pragma unroll
             for (int v = 0; v < 16; v++) {
-                const int o = 32 * mt + drow(v, h);
-                if (o < kOut) {
-                    L.stage[p * kStageStride + o] = acc[0][v];
-                    L.stage[(32 + p) * kStageStride + o] = acc[1][v];
+                const int o0 = 32 * mt + drow(v, 0);
+                if (o0 + 4 * h < kOut) {
+                    sb[o0] = acc[0][v];
+                    sb[32 * kStageStride + o0] = acc[1][v];
                 }
             }
         }
@@ -257,21 +256,36 @@ __global__ __launch_bounds__(64 * kTrainWaves) void k_train(Net net, TrainArgs a
                 for (int k = 0; k < kOut; k++) row[k] = 0.f;
         }
         wave_sync_lds();
-        f32x16 dacc[2][2];
+        // dY_4 as packed operands: element e of dy[ks][nt] is output kperm(ks, h, e) of record 32 nt + p
+        u32x4 dacc[4][2];
+        {
+            const float *sb = L.stage + p * kStageStride + 4 * h;
 #pragma unroll
-        for (int mt = 0; mt < 2; mt++)
+            for (int ks = 0; ks < 4; ks++)
 #pragma unroll
-            for (int nt = 0; nt < 2; nt++)
+                for (int nt = 0; nt < 2; nt++)
 #pragma unroll
-                for (int v = 0; v < 16; v++) {
-                    const int o = 32 * mt + drow(v, h);
-                    dacc[mt][nt][v] = o < kOut ? L.stage[(32 * nt + p) * kStageStride + o] : 0.f;
-                }
+                    for (int i = 0; i < 4; i++) {
+                        // output kperm(ks, h, e) = kperm(ks, 0, e) + 4 h; outputs >= 40 are padding
+                        const int o0 = kperm(ks, 0, 2 * i), o1 = kperm(ks, 0, 2 * i + 1);
+                        const float v0 = o0 + 4 * h < kOut ? sb[32 * nt * kStageStride + o0] : 0.f;
+                        const float v1 = o1 + 4 * h < kOut ? sb[32 * nt * kStageStride + o1] : 0.f;
+                        dacc[ks][nt][i] = pack2(v0, v1);
+                    }
+        }
         wave_sync_lds();  // the stage is read before the transposed tiles overwrite it
         backward_layer<4>(net, L, gacc, x4, dacc);
-        backward_layer<3>(net, L, gacc, x3, dacc);
+        {
+            u32x4 x3[4][2];
+            layer_to<kHid>(net, 2, x2, x3);
+            backward_layer<3>(net, L, gacc, x3, dacc);
+        }
         backward_layer<2>(net, L, gacc, x2, dacc);
-        backward_layer<1>(net, L, gacc, x1, dacc);
+        {
+            u32x4 x1[4][2];
+            layer_to<kIn>(net, 0, x0, x1);
+            backward_layer<1>(net, L, gacc, x1, dacc);
+        }
         backward_layer<0>(net, L, gacc, x0, dacc);
     }
     for (int off = 32; off > 0; off >>= 1) loss += __shfl_xor(loss, off);
