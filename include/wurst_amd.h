@@ -98,6 +98,11 @@ typedef struct {
     int32_t max_paths_in_flight; /* 0 = library default */
     int32_t profile;             /* 1: time each kernel class with HIP events */
     int32_t count_traversal;     /* 1: count visited nodes / tested triangles */
+    /* GPU-only scheduling (no reference counterpart; the CPU path has none):
+     * 0 = library default; wr_autotune picks them per workload */
+    int32_t use_graph;           /* 1: replay the per-iteration launches as a captured hipGraph */
+    uint32_t fetch_chunk;        /* traversal work items claimed per atomic (default 128) */
+    uint32_t refill_lanes;       /* idle lanes that trigger a wave's refill (default 24, max 64) */
 } wr_render_params;
 
 typedef struct {
@@ -187,6 +192,35 @@ int wr_trace_any(wr_scene *scene, const float *rays, uint32_t n, uint8_t *occlud
  * measured with HIP events on that stream. */
 int wr_trace_closest_device(wr_scene *scene, const float *d_rays, uint32_t n, void *d_hits, float *ms);
 int wr_trace_any_device(wr_scene *scene, const float *d_rays, uint32_t n, uint8_t *d_occluded, float *ms);
+
+/* ------------------------------------------------------------------ batches
+ * Several independent renders (scene, film, spp, depth, seed) in one path pool:
+ * every iteration is one launch of each wavefront kernel for all jobs, which
+ * keeps the GPU full when single jobs are too small to.  No reference
+ * counterpart (the reference renders one scene per process).  Each job's film
+ * receives exactly what wr_render_nee with the same parameters would add
+ * (same random numbers per pixel and sample).  All scenes must live on the
+ * same device; the batch runs on the first job's scene stream and path pool,
+ * so calls touching those scenes must be serialised with it.  From `params`
+ * only max_paths_in_flight, profile, use_graph, fetch_chunk and refill_lanes
+ * are used. */
+typedef struct {
+    wr_scene *scene;
+    wr_film *film;               /* a film of `scene` */
+    int32_t spp;
+    int32_t num_path_vertices;   /* 2..255 */
+    uint32_t seed;
+    int32_t use_mis;
+    int32_t first_sample;
+} wr_batch_job;
+int wr_render_batch(const wr_batch_job *jobs, uint32_t n_jobs, const wr_render_params *params, wr_render_stats *stats);
+
+/* Picks the path-pool size, traversal work chunk, refill threshold and graph
+ * replay for a batch by timing short trial renders (1 spp per job, into
+ * scratch films) of each candidate; writes the fastest into *best (other
+ * fields copied from *base) and its rays per second into *mrays. */
+int wr_autotune(const wr_batch_job *jobs, uint32_t n_jobs, const wr_render_params *base, wr_render_params *best,
+                double *mrays);
 
 /* ------------------------------------------------------------------ guiding
  * Learned path guiding.  No reference counterpart: the reference has no

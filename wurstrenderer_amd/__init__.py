@@ -47,7 +47,16 @@ class _SceneDesc(C.Structure):
 class RenderParams(C.Structure):
     _fields_ = [("spp", C.c_int32), ("num_path_vertices", C.c_int32), ("seed", C.c_uint32),
                 ("use_mis", C.c_int32), ("first_sample", C.c_int32), ("max_paths_in_flight", C.c_int32),
-                ("profile", C.c_int32), ("count_traversal", C.c_int32)]
+                ("profile", C.c_int32), ("count_traversal", C.c_int32), ("use_graph", C.c_int32),
+                ("fetch_chunk", C.c_uint32), ("refill_lanes", C.c_uint32)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class BatchJob(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("film", C.c_void_p), ("spp", C.c_int32), ("num_path_vertices", C.c_int32),
+                ("seed", C.c_uint32), ("use_mis", C.c_int32), ("first_sample", C.c_int32)]
 
 
 class RenderStats(C.Structure):
@@ -111,6 +120,9 @@ EXPORTS = {
     "wr_trace_any": (C.c_int, [P, P, C.c_uint32, P]),
     "wr_trace_closest_device": (C.c_int, [P, P, C.c_uint32, P, C.POINTER(C.c_float)]),
     "wr_trace_any_device": (C.c_int, [P, P, C.c_uint32, P, C.POINTER(C.c_float)]),
+    "wr_render_batch": (C.c_int, [C.POINTER(BatchJob), C.c_uint32, C.POINTER(RenderParams), C.POINTER(RenderStats)]),
+    "wr_autotune": (C.c_int, [C.POINTER(BatchJob), C.c_uint32, C.POINTER(RenderParams), C.POINTER(RenderParams),
+                              C.POINTER(C.c_double)]),
     "wr_guide_config_default": (None, [C.POINTER(GuideConfig)]),
     "wr_guide_create": (C.c_int, [P, C.POINTER(GuideConfig), C.POINTER(P)]),
     "wr_guide_destroy": (None, [P]),
@@ -306,14 +318,27 @@ class Film:
         return base / self.spp + splat / (self.spp * W * H)
 
 
+def render_params(spp=0, num_path_vertices=10, seed=0, use_mis=True, first_sample=0, max_paths_in_flight=0,
+                  profile=False, count_traversal=False, use_graph=False, fetch_chunk=0, refill_lanes=0,
+                  tuning: RenderParams | None = None) -> RenderParams:
+    """wr_render_params; `tuning` (from autotune) supplies the scheduling fields."""
+    p = RenderParams(spp, num_path_vertices, seed, int(bool(use_mis)), first_sample, max_paths_in_flight,
+                     int(bool(profile)), int(bool(count_traversal)), int(bool(use_graph)), fetch_chunk, refill_lanes)
+    if tuning is not None:
+        for k in ("max_paths_in_flight", "use_graph", "fetch_chunk", "refill_lanes"):
+            setattr(p, k, getattr(tuning, k))
+    return p
+
+
 def render_nee(scene: GpuScene, film: Film, spp: int, num_path_vertices: int = 10, seed: int = 0,
                use_mis: bool = True, first_sample: int | None = None, max_paths_in_flight: int = 0,
-               profile: bool = False, count_traversal: bool = False) -> RenderStats:
+               profile: bool = False, count_traversal: bool = False, use_graph: bool = False,
+               fetch_chunk: int = 0, refill_lanes: int = 0, tuning: RenderParams | None = None) -> RenderStats:
     """PathNeeRenderer::render on the GPU: adds `spp` samples per pixel into `film`."""
     if first_sample is None:
         first_sample = film.spp
-    p = RenderParams(spp, num_path_vertices, seed, int(bool(use_mis)), first_sample, max_paths_in_flight,
-                     int(bool(profile)), int(bool(count_traversal)))
+    p = render_params(spp, num_path_vertices, seed, use_mis, first_sample, max_paths_in_flight, profile,
+                      count_traversal, use_graph, fetch_chunk, refill_lanes, tuning)
     st = RenderStats()
     _check(lib().wr_render_nee(scene.h, film.h, C.byref(p), C.byref(st)))
     film.spp += spp
@@ -415,3 +440,38 @@ def render_guided(scene: GpuScene, film: Film, guide: Guide, spp: int, num_path_
     _check(lib().wr_render_guided(scene.h, film.h, guide.h, C.byref(p), C.byref(gp), C.byref(st), C.byref(gs)))
     film.spp += spp
     return st, gs
+
+
+def _batch_array(jobs):
+    """jobs: dicts with scene, film, spp and optional num_path_vertices, seed, use_mis, first_sample."""
+    arr = (BatchJob * len(jobs))()
+    for i, j in enumerate(jobs):
+        film = j.get("film")  # autotune renders into scratch films and needs none
+        first = j.get("first_sample")
+        if first is None:
+            first = film.spp if film is not None else 0
+        arr[i] = BatchJob(j["scene"].h, film.h if film is not None else None, j["spp"], j.get("num_path_vertices", 10),
+                          j.get("seed", 0), int(bool(j.get("use_mis", True))), first)
+    return arr
+
+
+def render_batch(jobs, max_paths_in_flight: int = 0, profile: bool = False, count_traversal: bool = False,
+                 use_graph: bool = False, fetch_chunk: int = 0, refill_lanes: int = 0,
+                 tuning: RenderParams | None = None) -> RenderStats:
+    """wr_render_batch: every job's film gets what render_nee would add, all jobs in one path pool."""
+    arr = _batch_array(jobs)
+    p = render_params(max_paths_in_flight=max_paths_in_flight, profile=profile, count_traversal=count_traversal,
+                      use_graph=use_graph, fetch_chunk=fetch_chunk, refill_lanes=refill_lanes, tuning=tuning)
+    st = RenderStats()
+    _check(lib().wr_render_batch(arr, len(jobs), C.byref(p), C.byref(st)))
+    for j in jobs:
+        j["film"].spp += j["spp"]
+    return st
+
+
+def autotune(jobs) -> tuple[RenderParams, float]:
+    """wr_autotune: the fastest pool size / traversal chunk / refill threshold / graph replay for a batch."""
+    arr = _batch_array(jobs)
+    base, best, rate = render_params(), RenderParams(), C.c_double()
+    _check(lib().wr_autotune(arr, len(jobs), C.byref(base), C.byref(best), C.byref(rate)))
+    return best, rate.value

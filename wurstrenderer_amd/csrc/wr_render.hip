@@ -127,13 +127,49 @@ __global__ void k_fill_f64(double *p, size_t n) {
     for (size_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) p[i] = 0.0;
 }
 
+// ============================================================== batches
+// A batch renders several (scene, film, parameters) jobs through one path pool
+// and one launch of every kernel per iteration: the pass's samples are the
+// jobs' samples laid end to end (job-major), a path carries its job in the
+// high bits of meta.z (bounce in the low 8), and every per-path lookup reads
+// the job's scene.  Single-scene kernels (BATCH = false) take the scene as a
+// kernel argument instead, and their job is always 0.
+struct BatchJob {
+    DScene s;
+    uint32_t offset;        // first pass sample of the job
+    uint32_t count;         // samples of the job in this pass = k * wh
+    uint32_t wh, k;         // pixels, samples per pixel in this pass
+    uint32_t first_sample;  // sample index of the pass's first spp
+    uint32_t seed;
+    int N, use_mis;
+    double *film_base, *film_splat;
+};
+constexpr uint32_t kBounceBits = 8;
+constexpr uint32_t kBounceMask = (1u << kBounceBits) - 1;
+constexpr int kMaxBatchJobs = 1 << 16;
+// The job a pass sample belongs to: the last job whose offset is <= g.
+__device__ __forceinline__ uint32_t find_job(const BatchJob *jobs, uint32_t n_jobs, uint32_t g) {
+    uint32_t lo = 0, hi = n_jobs;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (jobs[mid].offset <= g) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+// Geometry of one lane's scene for traversal.
+struct Geo {
+    const Node4 *nodes;
+    const float4 *tris;
+};
+
 // ============================================================== regenerate
 // Fills the pool behind the continuing paths with the next unclaimed samples:
 // createLightSubpath + createCameraSubpath up to the first ray, with the t = 1
 // light->camera connection (pathnextevent.h:53-99; thinlens.h:121-166).
-__global__ __launch_bounds__(kShadeBlock) void k_regen(DScene s, PathBuffers pb, RenderArgs a) {
+template <bool BATCH>
+__global__ __launch_bounds__(kShadeBlock) void k_regen(DScene s1, const BatchJob *__restrict__ jobs, uint32_t n_jobs,
+                                                       PathBuffers pb, RenderArgs a) {
     __shared__ uint32_t agg[kWavesPerShadeBlock + 1];
-    const DCamera &cam = s.cam;
     const uint32_t head = pb.counters[kQ];
     const uint32_t fresh_n = pb.counters[kFresh];
     const uint32_t sbase = pb.counters[kBase];
@@ -146,11 +182,25 @@ __global__ __launch_bounds__(kShadeBlock) void k_regen(DScene s, PathBuffers pb,
         float4 so = make_float4(0, 0, 0, 0), sd = so, sc = so;
         float4 ro = so, rdir = so, thr = so;
         uint4 meta = make_uint4(0, 0, 0, 0);
+        uint32_t job = 0;
         if (fresh) {
-            const uint32_t pixel = sidx % a.wh;
-            const uint32_t sample = a.first_sample + sidx / a.wh;
+            uint32_t local = sidx, wh = a.wh, first = a.first_sample, seed = a.seed;
+            int use_mis = a.use_mis;
+            if (BATCH) {
+                job = find_job(jobs, n_jobs, sidx);
+                const BatchJob &J = jobs[job];
+                local = sidx - J.offset;
+                wh = J.wh;
+                first = J.first_sample;
+                seed = J.seed;
+                use_mis = J.use_mis;
+            }
+            const DScene &s = BATCH ? jobs[job].s : s1;
+            const DCamera &cam = s.cam;
+            const uint32_t pixel = local % wh;
+            const uint32_t sample = first + local / wh;
             const uint32_t px = pixel % uint32_t(cam.W), py = pixel / uint32_t(cam.W);
-            const uint32_t key = path_key(a.seed, pixel, sample);
+            const uint32_t key = path_key(seed, pixel, sample);
             const LightSample l = sample_light(s, u01(key, kDimLight), u01(key, kDimLight + 1));
             // camera vertex (sampleWe0ByPdf)
             float lx = 0.f, ly = 0.f;
@@ -175,7 +225,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_regen(DScene s, PathBuffers pb,
                     const float we1 = camera_we1(cam, lens, wcam, rx, ry);
                     if (we1 != 0.f) {
                         float mis = 1.f;
-                        if (a.use_mis) {
+                        if (use_mis) {
                             const float c = cam_to_local(cam, wcam).z;
                             float camPdfA = cam.per_pixel_norm / (c * c * c) * fabsf(cos_l) / len2;
                             if (s.med.on) camPdfA *= pass_pdf(s.med, len);
@@ -187,7 +237,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_regen(DScene s, PathBuffers pb,
                         const uint32_t target = kToSplat | (uint32_t(ry) * uint32_t(cam.W) + uint32_t(rx));
                         so = make_float4(l.pos.x, l.pos.y, l.pos.z, len - kRayEps);
                         sd = make_float4(nuv.x, nuv.y, nuv.z, __uint_as_float(target));
-                        sc = make_float4(c.x, c.y, c.z, 0.f);
+                        sc = make_float4(c.x, c.y, c.z, __uint_as_float(job));
                         splat = true;
                     }
                 }
@@ -202,8 +252,8 @@ __global__ __launch_bounds__(kShadeBlock) void k_regen(DScene s, PathBuffers pb,
             const float3 dir = cam_to_world(cam, dl);
             ro = make_float4(cpos.x, cpos.y, cpos.z, 0.f);
             rdir = make_float4(dir.x, dir.y, dir.z, 0.f);
-            thr = make_float4(1.f, 1.f, 1.f, a.use_mis ? pdfW : 0.f);
-            meta = make_uint4(key, sidx, 0u, 0u);
+            thr = make_float4(1.f, 1.f, 1.f, use_mis ? pdfW : 0.f);
+            meta = make_uint4(key, sidx, job << kBounceBits, 0u);
         }
         if (fresh) {
             const uint32_t e = head + i;
@@ -223,15 +273,22 @@ __global__ __launch_bounds__(kShadeBlock) void k_regen(DScene s, PathBuffers pb,
 }
 
 // ============================================================== extend
-// Persistent traversal: a wave claims kFetchChunk consecutive entries per
-// atomic and refills its lanes from that local range as soon as kRefillLanes
-// of them are idle, so a wave never waits for its slowest ray and atomics stay
-// rare.  Waves beyond the work exit at once.  (Chunk 128 / threshold 24 were
-// the fastest of a sweep over 64..1024 and 16..64 on the 1.31M-triangle scene.)
+// Persistent traversal: a wave claims `chunk` consecutive entries per atomic
+// and refills its lanes from that local range as soon as `refill` of them are
+// idle, so a wave never waits for its slowest ray and atomics stay rare.
+// Waves beyond the work exit at once.  Both are runtime parameters
+// (TravTune); 128 / 24 were the fastest of a sweep over 64..1024 and 16..64 on
+// the 1.31M-triangle scene, and wr_autotune searches them per workload.
 #ifndef WR_REFILL_LANES
 #define WR_REFILL_LANES 24
 #endif
-constexpr int kRefillLanes = WR_REFILL_LANES;
+#ifndef WR_FETCH_CHUNK
+#define WR_FETCH_CHUNK 128
+#endif
+struct TravTune {
+    uint32_t chunk = WR_FETCH_CHUNK;
+    uint32_t refill = WR_REFILL_LANES;
+};
 
 #ifdef WR_TRAV_WAVES
 #define WR_TRAV_ATTR __attribute__((amdgpu_waves_per_eu(WR_TRAV_WAVES, WR_TRAV_WAVES)))
@@ -239,13 +296,6 @@ constexpr int kRefillLanes = WR_REFILL_LANES;
 #define WR_TRAV_ATTR
 #endif
 
-// Work distribution of the persistent traversal kernels: a wave claims
-// kFetchChunk consecutive entries per atomic and hands them to its idle lanes
-// from a wave-local range, so atomics stay rare.
-#ifndef WR_FETCH_CHUNK
-#define WR_FETCH_CHUNK 128
-#endif
-constexpr uint32_t kFetchChunk = WR_FETCH_CHUNK;
 struct WaveRange {
     uint32_t lo = 0, hi = 0;  // wave-uniform
 };
@@ -253,17 +303,17 @@ struct WaveRange {
 // claimed only when the local range is empty; `exhausted` (wave-uniform) is
 // set when the queue has no chunk left.  All lanes of the wave must call it.
 __device__ __forceinline__ uint32_t refill(uint32_t *fetch_counter, uint32_t n, bool idle, unsigned long long idle_mask,
-                                           WaveRange &wr, bool &exhausted) {
+                                           WaveRange &wr, bool &exhausted, uint32_t chunk) {
     if (wr.lo == wr.hi) {
         uint32_t base = 0;
-        if (lane_id() == 0) base = atomicAdd(fetch_counter, kFetchChunk);
+        if (lane_id() == 0) base = atomicAdd(fetch_counter, chunk);
         base = __shfl(base, 0);
         if (base >= n) {
             exhausted = true;
             return n;
         }
         wr.lo = base;
-        wr.hi = min(base + kFetchChunk, n);
+        wr.hi = min(base + chunk, n);
     }
     const uint32_t k = (uint32_t)__popcll(idle_mask);
     const uint32_t avail = wr.hi - wr.lo;
@@ -273,8 +323,9 @@ __device__ __forceinline__ uint32_t refill(uint32_t *fetch_counter, uint32_t n, 
     return e;
 }
 
-template <bool COUNT>
-__global__ __launch_bounds__(kBlock) WR_TRAV_ATTR void k_extend(DScene s, PathBuffers pb, uint2 *spill) {
+template <bool COUNT, bool BATCH>
+__global__ __launch_bounds__(kBlock) WR_TRAV_ATTR void k_extend(DScene s1, const BatchJob *__restrict__ jobs,
+                                                                PathBuffers pb, uint2 *spill, TravTune tune) {
     const uint32_t n = active_count(pb.counters);
     if (blockIdx.x * kBlock >= n) return;
     __shared__ uint32_t st_node[kLdsStackDepth * kBlock];
@@ -285,14 +336,19 @@ __global__ __launch_bounds__(kBlock) WR_TRAV_ATTR void k_extend(DScene s, PathBu
     TravState ts;
     uint32_t p = 0;
     WaveRange wr;
+    Geo geo{s1.nodes, s1.tris};
     bool active = false, exhausted = false;
     for (;;) {
         const unsigned long long idle = __ballot(!active);
-        if (!exhausted && __popcll(idle) >= kRefillLanes) {
-            const uint32_t i = refill(&pb.counters[kFetchExt], n, !active, idle, wr, exhausted);
+        if (!exhausted && __popcll(idle) >= tune.refill) {
+            const uint32_t i = refill(&pb.counters[kFetchExt], n, !active, idle, wr, exhausted, tune.chunk);
             if (!active && i < n) {
                 p = i;
                 const float4 o = pb.cur.ray_o[p], d = pb.cur.ray_d[p];
+                if (BATCH) {
+                    const DScene &sj = jobs[pb.cur.meta[p].z >> kBounceBits].s;
+                    geo = Geo{sj.nodes, sj.tris};
+                }
                 trav_begin(ts, xyz(o), xyz(d), kRayEps, INFINITY);
                 st.sp = 0;
                 active = true;
@@ -302,7 +358,8 @@ __global__ __launch_bounds__(kBlock) WR_TRAV_ATTR void k_extend(DScene s, PathBu
             if (exhausted) break;
             continue;
         }
-        if (active && trav_step<false, COUNT, kBlock>(s, ts, st, cnt)) {
+        if (active && (BATCH ? trav_step<false, COUNT, kBlock>(geo, ts, st, cnt)
+                             : trav_step<false, COUNT, kBlock>(s1, ts, st, cnt))) {
             pb.hit[p] = make_uint2(ts.hit, __float_as_uint(ts.t));
             active = false;
         }
@@ -318,7 +375,9 @@ __global__ __launch_bounds__(kBlock) WR_TRAV_ATTR void k_extend(DScene s, PathBu
 // (generalsubpathsampler.h:34-130, pathnextevent.h:100-149).  Continuing
 // paths are compacted into the next state; a finished path stores its
 // radiance in its sample's slot of the pass.
-__global__ __launch_bounds__(kShadeBlock) void k_shade(DScene s, PathBuffers pb, RenderArgs a) {
+template <bool BATCH>
+__global__ __launch_bounds__(kShadeBlock) void k_shade(DScene s1, const BatchJob *__restrict__ jobs, PathBuffers pb,
+                                                       RenderArgs a) {
     __shared__ uint32_t agg[kWavesPerShadeBlock + 1];
     const uint32_t n = active_count(pb.counters);
     const PathState cs = pb.cur, ns = pb.next;
@@ -329,10 +388,15 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(DScene s, PathBuffers pb,
         float4 so = make_float4(0, 0, 0, 0), sd = so, sc = so;
         float4 new_o = so, new_d = so, new_thr = so, rad = so;
         uint4 meta = make_uint4(0, 0, 0, 0);
+        uint32_t job = 0;
         if (valid) {
             meta = cs.meta[i];
             rad = cs.rad[i];
-            const uint32_t bounce = meta.z;
+            job = meta.z >> kBounceBits;
+            const DScene &s = BATCH ? jobs[job].s : s1;
+            const int N = BATCH ? jobs[job].N : a.N;
+            const int use_mis = BATCH ? jobs[job].use_mis : a.use_mis;
+            const uint32_t bounce = meta.z & kBounceMask;
             const int cNum = int(bounce) + 2;  // camera is vertex 1
             const uint2 h = pb.hit[i];
             if (h.x != kInvalid) {
@@ -353,7 +417,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(DScene s, PathBuffers pb,
                     if (cos_l >= 0.f) {
                         const DLight &L = s.lights[gi.y];
                         float mis = 0.f;
-                        if (a.use_mis) {
+                        if (use_mis) {
                             const float cPdfA = thr.w * fabsf(cos_l) / (t * t);
                             const float lightPdfA = L.pdf_select * L.inv_area;
                             mis = cPdfA / (lightPdfA + cPdfA);
@@ -362,7 +426,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(DScene s, PathBuffers pb,
                         rad.y += mis * T.y * L.L[1];
                         rad.z += mis * T.z * L.L[2];
                     }
-                } else if (cNum != a.N) {
+                } else if (cNum != N) {
                     const DMat m = s.mats[gi.x];
                     const Frame f = make_frame(ng);
                     const float3 wi = to_local(f, -rd);
@@ -379,7 +443,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(DScene s, PathBuffers pb,
                         const float3 fs = bsdf_eval(m, wi, wo);
                         if (!is_black(fs)) {
                             float mis = 1.f;
-                            if (a.use_mis) {
+                            if (use_mis) {
                                 const float bsdfPdfA = bsdf_pdf(m, wi, wo) * fabsf(cos_lv) / len2;
                                 mis = l.pdfA / (l.pdfA + bsdfPdfA);
                             }
@@ -402,7 +466,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(DScene s, PathBuffers pb,
                         const float3 nd = to_world(f, wo);
                         new_o = make_float4(pos.x, pos.y, pos.z, 0.f);
                         new_d = make_float4(nd.x, nd.y, nd.z, 0.f);
-                        new_thr = make_float4(w.x * T.x, w.y * T.y, w.z * T.z, a.use_mis ? pdf : 0.f);
+                        new_thr = make_float4(w.x * T.x, w.y * T.y, w.z * T.z, use_mis ? pdf : 0.f);
                         cont = true;
                     }
                 }
@@ -420,6 +484,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(DScene s, PathBuffers pb,
         }
         const uint32_t target = cont ? (kToNextPath | qi) : (kToSample | meta.y);
         sd.w = __uint_as_float(target);
+        sc.w = __uint_as_float(job);
         const uint32_t si = block_append(&pb.counters[kS], push_shadow, agg);
         if (push_shadow) {
             pb.sh_o[si] = so;
@@ -763,8 +828,9 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade_ext(DScene s, PathBuffers
 // Any-hit traversal of the shadow queue; a visible connection adds its
 // contribution to the radiance of its path (continuing or finished), or to the
 // FP64 splat film for light->camera (t = 1) connections.
-template <bool COUNT>
-__global__ __launch_bounds__(kBlock) WR_TRAV_ATTR void k_shadow(DScene s, PathBuffers pb, uint2 *spill) {
+template <bool COUNT, bool BATCH>
+__global__ __launch_bounds__(kBlock) WR_TRAV_ATTR void k_shadow(DScene s1, const BatchJob *__restrict__ jobs,
+                                                                PathBuffers pb, uint2 *spill, TravTune tune) {
     const uint32_t n = pb.counters[kS];
     if (blockIdx.x * kBlock >= n) return;
     __shared__ uint32_t st_node[kLdsStackDepth * kBlock];
@@ -775,14 +841,19 @@ __global__ __launch_bounds__(kBlock) WR_TRAV_ATTR void k_shadow(DScene s, PathBu
     TravState ts;
     uint32_t i = 0;
     WaveRange wr;
+    Geo geo{s1.nodes, s1.tris};
     bool active = false, exhausted = false;
     for (;;) {
         const unsigned long long idle = __ballot(!active);
-        if (!exhausted && __popcll(idle) >= kRefillLanes) {
-            const uint32_t j = refill(&pb.counters[kFetchShd], n, !active, idle, wr, exhausted);
+        if (!exhausted && __popcll(idle) >= tune.refill) {
+            const uint32_t j = refill(&pb.counters[kFetchShd], n, !active, idle, wr, exhausted, tune.chunk);
             if (!active && j < n) {
                 i = j;
                 const float4 o = pb.sh_o[i], d = pb.sh_d[i];
+                if (BATCH) {
+                    const DScene &sj = jobs[__float_as_uint(pb.sh_c[i].w)].s;
+                    geo = Geo{sj.nodes, sj.tris};
+                }
                 trav_begin(ts, xyz(o), xyz(d), kRayEps, o.w);
                 st.sp = 0;
                 active = true;
@@ -792,14 +863,15 @@ __global__ __launch_bounds__(kBlock) WR_TRAV_ATTR void k_shadow(DScene s, PathBu
             if (exhausted) break;
             continue;
         }
-        if (active && trav_step<true, COUNT, kBlock>(s, ts, st, cnt)) {
+        if (active && (BATCH ? trav_step<true, COUNT, kBlock>(geo, ts, st, cnt)
+                             : trav_step<true, COUNT, kBlock>(s1, ts, st, cnt))) {
             active = false;
             if (ts.hit == kInvalid) {
                 const float4 c = pb.sh_c[i];
                 const uint32_t target = __float_as_uint(pb.sh_d[i].w);
                 const uint32_t idx = target & kTargetMask;
                 if ((target & ~kTargetMask) == kToSplat) {
-                    double *dst = pb.film_splat + 3 * size_t(idx);
+                    double *dst = (BATCH ? jobs[__float_as_uint(c.w)].film_splat : pb.film_splat) + 3 * size_t(idx);
                     atomicAdd(dst + 0, double(c.x));
                     atomicAdd(dst + 1, double(c.y));
                     atomicAdd(dst + 2, double(c.z));
@@ -822,14 +894,14 @@ __global__ __launch_bounds__(kBlock) WR_TRAV_ATTR void k_shadow(DScene s, PathBu
 // Film::addSample for a pass (film.h:42-48): the samples of pixel p are
 // p, p + wh, p + 2 wh, ...; their FP32 radiance is summed in FP64, in sample
 // order, so the film is bit-reproducible.
-__global__ void k_accumulate(PathBuffers pb, uint32_t wh, uint32_t spp) {
+__global__ void k_accumulate(const float4 *__restrict__ sample_rad, double *film_base, uint32_t wh, uint32_t spp) {
     for (uint32_t px = blockIdx.x * blockDim.x + threadIdx.x; px < wh; px += gridDim.x * blockDim.x) {
         double r = 0.0, g = 0.0, b = 0.0;
         for (uint32_t k = 0; k < spp; k++) {
-            const float4 v = pb.sample_rad[size_t(k) * wh + px];
+            const float4 v = sample_rad[size_t(k) * wh + px];
             r += v.x; g += v.y; b += v.z;
         }
-        double *dst = pb.film_base + 3 * size_t(px);
+        double *dst = film_base + 3 * size_t(px);
         dst[0] += r; dst[1] += g; dst[2] += b;
     }
 }
@@ -889,11 +961,11 @@ int wr_device_create(int ordinal, wr_device **out) {
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, block, 0) == hipSuccess && nb > 0) out_blocks = nb;
         out_blocks = std::min(out_blocks, kMaxTraversalBlocksPerCU);
     };
-    occ(reinterpret_cast<const void *>(&k_extend<false>), kBlock, d->occ_extend);
-    occ(reinterpret_cast<const void *>(&k_shadow<false>), kBlock, d->occ_shadow);
-    occ(reinterpret_cast<const void *>(&k_shade), kShadeBlock, d->occ_shade);
+    occ(reinterpret_cast<const void *>(&k_extend<false, false>), kBlock, d->occ_extend);
+    occ(reinterpret_cast<const void *>(&k_shadow<false, false>), kBlock, d->occ_shadow);
+    occ(reinterpret_cast<const void *>(&k_shade<false>), kShadeBlock, d->occ_shade);
 
-    occ(reinterpret_cast<const void *>(&k_regen), kShadeBlock, d->occ_generate);
+    occ(reinterpret_cast<const void *>(&k_regen<false>), kShadeBlock, d->occ_generate);
     occ(reinterpret_cast<const void *>(&k_trace<false>), kBlock, d->occ_trace);
     *out = d;
     return WR_OK;
@@ -1311,6 +1383,70 @@ static int trav_grid(const wr_scene *s, uint32_t n, int per_cu) {
 // for a 1920x1080 frame at 64 spp, so such a frame drains the pool once.
 constexpr uint64_t kMaxSamplesPerPass = uint64_t(1) << 27;
 
+// Runs wavefront iterations until the pool drains.  launch(pb) issues one
+// iteration (refill, extend, shade, shadow, rotate) for the state pb; the host
+// polls the device counters every kPoll iterations.  With a graph, the two
+// iterations of a state round trip (cur -> next -> cur) are captured once and
+// replayed, so a pass costs one graph launch per two iterations.  Every
+// kernel reads its work counts from device memory, which is what makes the
+// replays valid; nothing in an iteration is a memset.
+template <class Launch>
+static int drain(wr_scene *s, PathBuffers &pb, uint64_t max_iters, bool use_graph, Launch &&launch) {
+    constexpr int kPoll = 4;
+    auto poll = [&](bool &done) -> int {
+        WR_HIP(hipMemcpyAsync(s->h_counters, pb.counters, sizeof(uint32_t) * kCounterStride * kNumCounterLines,
+                              hipMemcpyDeviceToHost, s->stream));
+        WR_HIP(hipStreamSynchronize(s->stream));
+        done = s->h_counters[kQ] == 0 && s->h_counters[kFresh] == 0;
+        return WR_OK;
+    };
+    if (!use_graph) {
+        for (uint64_t it = 0;; it++) {
+            if (it > max_iters) return set_error(WR_ERR_DEVICE, "path pool did not drain");
+            launch(pb);
+            WR_HIP(hipGetLastError());
+            std::swap(pb.cur, pb.next);
+            if (it % kPoll == kPoll - 1) {
+                bool done = false;
+                int rc = poll(done);
+                if (rc) return rc;
+                if (done) return WR_OK;
+            }
+        }
+    }
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    WR_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+    PathBuffers swapped = pb;
+    std::swap(swapped.cur, swapped.next);
+    launch(pb);
+    launch(swapped);
+    const hipError_t launch_err = hipGetLastError();
+    WR_HIP(hipStreamEndCapture(s->stream, &graph));
+    int rc = WR_OK;
+    if (launch_err != hipSuccess) rc = set_error(WR_ERR_DEVICE, std::string("launch in graph capture: ") + hipGetErrorString(launch_err));
+    if (!rc && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess)
+        rc = set_error(WR_ERR_DEVICE, "hipGraphInstantiate failed");
+    for (uint64_t it = 0; !rc; it += 2) {
+        if (it > max_iters) {
+            rc = set_error(WR_ERR_DEVICE, "path pool did not drain");
+            break;
+        }
+        if (hipGraphLaunch(exec, s->stream) != hipSuccess) {
+            rc = set_error(WR_ERR_DEVICE, "hipGraphLaunch failed");
+            break;
+        }
+        if (it % kPoll == kPoll - 2) {
+            bool done = false;
+            rc = poll(done);
+            if (done) break;
+        }
+    }
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;  // an even number of iterations: pb is the current state again
+}
+
 // The frame loop of wr_render_nee and wr_render_guided (g non-null).
 static int render_impl(wr_scene *s, wr_film *f, const wr_render_params *prm, wr_guide *g, const wr_guided_params *gp,
                        wr_render_stats *stats, wr_guide_stats *gstats) {
@@ -1356,6 +1492,11 @@ static int render_impl(wr_scene *s, wr_film *f, const wr_render_params *prm, wr_
     WR_HIP(hipGetLastError());
     const bool prof = prm->profile != 0;
     const bool count = prm->count_traversal != 0;
+    // per-kernel profiling records events between launches, which a graph cannot hold
+    const bool use_graph = prm->use_graph != 0 && !prof;
+    TravTune tune;
+    if (prm->fetch_chunk) tune.chunk = prm->fetch_chunk;
+    if (prm->refill_lanes) tune.refill = std::min<uint32_t>(prm->refill_lanes, 64);
     std::vector<hipEvent_t> evs;
     struct Mark { int kind; size_t e0; };
     std::vector<Mark> marks;
@@ -1371,7 +1512,7 @@ static int render_impl(wr_scene *s, wr_film *f, const wr_render_params *prm, wr_
     const int g_regen = std::max(1, std::min<int>((P + kShadeBlock - 1) / kShadeBlock, s->dev->n_cu * s->dev->occ_generate));
     // shading kernel: plain, or extended for guiding and/or a medium
     const bool medium = s->ds.med.on != 0;
-    const void *shade_fn = reinterpret_cast<const void *>(&k_shade);
+    const void *shade_fn = reinterpret_cast<const void *>(&k_shade<false>);
     if (g && medium) shade_fn = reinterpret_cast<const void *>(&k_shade_ext<true, true>);
     else if (g) shade_fn = reinterpret_cast<const void *>(&k_shade_ext<true, false>);
     else if (medium) shade_fn = reinterpret_cast<const void *>(&k_shade_ext<false, true>);
@@ -1410,53 +1551,43 @@ static int render_impl(wr_scene *s, wr_film *f, const wr_render_params *prm, wr_
         k_init<<<1, 64, 0, s->stream>>>(pb, a);
         WR_HIP(hipGetLastError());
         // Each iteration refills the pool, extends every active path by one
-        // vertex and resolves its shadow rays.  The host polls the counters
-        // every few iterations; iterations past the end find empty queues.
-        const int kPoll = 4;
-        // every path ends within N-1 vertices, and each iteration that is not
-        // draining starts new samples in all free entries
+        // vertex and resolves its shadow rays; iterations past the end find
+        // empty queues.  Every path ends within N-1 vertices, and each
+        // iteration that is not draining starts new samples in all free entries.
         const uint64_t max_iters = (uint64_t(a.total) / P + 2) * uint64_t(N + 1) + 64;
-        for (int it = 0;; it++) {
-            if (uint64_t(it) > max_iters) return set_error(WR_ERR_DEVICE, "path pool did not drain");
+        auto iteration = [&](const PathBuffers &q) {
             begin(3);
-            k_regen<<<g_regen, kShadeBlock, 0, s->stream>>>(s->ds, pb, a);
+            k_regen<false><<<g_regen, kShadeBlock, 0, s->stream>>>(s->ds, nullptr, 0, q, a);
             end();
             begin(0);
-            if (count) k_extend<true><<<g_ext, kBlock, 0, s->stream>>>(s->ds, pb, s->d_spill);
-            else k_extend<false><<<g_ext, kBlock, 0, s->stream>>>(s->ds, pb, s->d_spill);
+            if (count) k_extend<true, false><<<g_ext, kBlock, 0, s->stream>>>(s->ds, nullptr, q, s->d_spill, tune);
+            else k_extend<false, false><<<g_ext, kBlock, 0, s->stream>>>(s->ds, nullptr, q, s->d_spill, tune);
             end();
             begin(2);
             if (g && gs.alpha > 0.f) {
                 if (medium)
-                    k_guide_infer<true><<<g_infer, kInferBlock, 0, s->stream>>>(s->ds, pb.cur, pb.hit, pb.counters, N, net,
-                                                                           gs.alpha, gs.draw);
+                    k_guide_infer<true><<<g_infer, kInferBlock, 0, s->stream>>>(s->ds, q.cur, q.hit, q.counters, N, net,
+                                                                                gs.alpha, gs.draw);
                 else
-                    k_guide_infer<false><<<g_infer, kInferBlock, 0, s->stream>>>(s->ds, pb.cur, pb.hit, pb.counters, N, net,
-                                                                            gs.alpha, gs.draw);
+                    k_guide_infer<false><<<g_infer, kInferBlock, 0, s->stream>>>(s->ds, q.cur, q.hit, q.counters, N, net,
+                                                                                 gs.alpha, gs.draw);
             }
-            if (g && medium) k_shade_ext<true, true><<<g_shade, kShadeBlock, 0, s->stream>>>(s->ds, pb, a, gs);
-            else if (g) k_shade_ext<true, false><<<g_shade, kShadeBlock, 0, s->stream>>>(s->ds, pb, a, gs);
-            else if (medium) k_shade_ext<false, true><<<g_shade, kShadeBlock, 0, s->stream>>>(s->ds, pb, a, gs);
-            else k_shade<<<g_shade, kShadeBlock, 0, s->stream>>>(s->ds, pb, a);
+            if (g && medium) k_shade_ext<true, true><<<g_shade, kShadeBlock, 0, s->stream>>>(s->ds, q, a, gs);
+            else if (g) k_shade_ext<true, false><<<g_shade, kShadeBlock, 0, s->stream>>>(s->ds, q, a, gs);
+            else if (medium) k_shade_ext<false, true><<<g_shade, kShadeBlock, 0, s->stream>>>(s->ds, q, a, gs);
+            else k_shade<false><<<g_shade, kShadeBlock, 0, s->stream>>>(s->ds, nullptr, q, a);
             end();
             begin(1);
-            if (count) k_shadow<true><<<g_shd, kBlock, 0, s->stream>>>(s->ds, pb, s->d_spill);
-            else k_shadow<false><<<g_shd, kBlock, 0, s->stream>>>(s->ds, pb, s->d_spill);
+            if (count) k_shadow<true, false><<<g_shd, kBlock, 0, s->stream>>>(s->ds, nullptr, q, s->d_spill, tune);
+            else k_shadow<false, false><<<g_shd, kBlock, 0, s->stream>>>(s->ds, nullptr, q, s->d_spill, tune);
             end();
-            k_rotate<<<1, 1, 0, s->stream>>>(pb, a);
-            WR_HIP(hipGetLastError());
-            std::swap(pb.cur, pb.next);
-            if (it % kPoll == kPoll - 1) {
-                WR_HIP(hipMemcpyAsync(s->h_counters, pb.counters, sizeof(uint32_t) * kCounterStride * kNumCounterLines,
-                                      hipMemcpyDeviceToHost, s->stream));
-                WR_HIP(hipStreamSynchronize(s->stream));
-                const uint32_t *c = s->h_counters;
-                if (c[kQ] == 0 && c[kFresh] == 0) break;
-            }
-        }
+            k_rotate<<<1, 1, 0, s->stream>>>(q, a);
+        };
+        rc = drain(s, pb, max_iters, use_graph, iteration);
+        if (rc) return rc;
         begin(4);
         k_accumulate<<<std::max<uint32_t>(1, std::min<uint32_t>((wh + 255) / 256, uint32_t(s->dev->n_cu) * 8)), 256, 0,
-                       s->stream>>>(pb, wh, k);
+                       s->stream>>>(pb.sample_rad, pb.film_base, wh, k);
         end();
         WR_HIP(hipGetLastError());
         if (collect) {
@@ -1503,6 +1634,238 @@ int wr_render_guided(wr_scene *s, wr_film *f, wr_guide *g, const wr_render_param
                      wr_render_stats *stats, wr_guide_stats *gstats) {
     if (!g) return set_error(WR_ERR_INVALID_ARGUMENT, "guide is null");
     return render_impl(s, f, prm, g, gp, stats, gstats);
+}
+
+// ============================================================== batches (host)
+int wr_render_batch(const wr_batch_job *jobs, uint32_t n_jobs, const wr_render_params *prm, wr_render_stats *stats) {
+    if (!jobs || !prm) return set_error(WR_ERR_INVALID_ARGUMENT, "null argument");
+    if (n_jobs == 0) return set_error(WR_ERR_INVALID_ARGUMENT, "empty batch");
+    if (n_jobs > uint32_t(kMaxBatchJobs)) return set_error(WR_ERR_UNSUPPORTED, "too many jobs in a batch");
+    wr_scene *s0 = jobs[0].scene;
+    for (uint32_t j = 0; j < n_jobs; j++) {
+        const wr_batch_job &J = jobs[j];
+        if (!J.scene || !J.film) return set_error(WR_ERR_INVALID_ARGUMENT, "batch job without scene or film");
+        if (J.film->scene != J.scene) return set_error(WR_ERR_INVALID_ARGUMENT, "batch job film belongs to another scene");
+        if (J.scene->dev != s0->dev) return set_error(WR_ERR_INVALID_ARGUMENT, "batch scenes on different devices");
+        if (J.spp < 0 || J.first_sample < 0) return set_error(WR_ERR_INVALID_ARGUMENT, "negative spp or first_sample");
+        if (J.num_path_vertices > int(kBounceMask)) return set_error(WR_ERR_UNSUPPORTED, "num_path_vertices above 255");
+        if (uint64_t(J.scene->W) * J.scene->H > kMaxSamplesPerPass)
+            return set_error(WR_ERR_UNSUPPORTED, "resolution too large for a batch pass");
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    wr_render_stats st{};
+    int rc = set_device(s0->dev);
+    if (rc) return rc;
+    std::vector<uint32_t> done(n_jobs, 0);
+    uint64_t paths = 0, max_pass = 0;
+    int maxN = 2;
+    for (uint32_t j = 0; j < n_jobs; j++) {
+        const uint64_t wh = uint64_t(jobs[j].scene->W) * jobs[j].scene->H;
+        if (jobs[j].num_path_vertices > 1) {
+            paths += wh * uint64_t(jobs[j].spp);
+            maxN = std::max(maxN, jobs[j].num_path_vertices);
+        }
+        max_pass += wh * uint64_t(jobs[j].spp);
+    }
+    if (paths == 0) {
+        if (stats) *stats = st;
+        return WR_OK;
+    }
+    max_pass = std::min<uint64_t>(max_pass, kMaxSamplesPerPass);
+    uint64_t pool = prm->max_paths_in_flight > 0 ? uint64_t(prm->max_paths_in_flight) : (uint64_t(1) << 25);
+    pool = std::min<uint64_t>(pool, max_pass);
+    if (pool > (uint64_t(1) << 30)) return set_error(WR_ERR_UNSUPPORTED, "too many paths in flight");
+    const uint32_t P = uint32_t(pool);
+    rc = ensure_capacity(s0, P, uint32_t(max_pass));
+    if (rc) return rc;
+    PathBuffers pb = s0->pb;
+    pb.film_base = nullptr;  // per job, from the table
+    pb.film_splat = nullptr;
+    BatchJob *d_tab = nullptr;
+    WR_HIP(hipMalloc(reinterpret_cast<void **>(&d_tab), sizeof(BatchJob) * n_jobs));
+    struct Free { BatchJob *p; ~Free() { if (p) (void)hipFree(p); } } free_tab{d_tab};
+    k_fill_u32<<<1, 64, 0, s0->stream>>>(reinterpret_cast<uint32_t *>(pb.stats), 0u, 12);
+    WR_HIP(hipGetLastError());
+    const bool prof = prm->profile != 0;
+    const bool count = prm->count_traversal != 0;
+    const bool use_graph = prm->use_graph != 0 && !prof;
+    TravTune tune;
+    if (prm->fetch_chunk) tune.chunk = prm->fetch_chunk;
+    if (prm->refill_lanes) tune.refill = std::min<uint32_t>(prm->refill_lanes, 64);
+    std::vector<hipEvent_t> evs;
+    struct Mark { int kind; size_t e0; };
+    std::vector<Mark> marks;
+    auto ev_record = [&]() -> size_t {
+        hipEvent_t e;
+        (void)hipEventCreate(&e);
+        (void)hipEventRecord(e, s0->stream);
+        evs.push_back(e);
+        return evs.size() - 1;
+    };
+    auto begin = [&](int kind) { if (prof) marks.push_back({kind, ev_record()}); };
+    auto end = [&]() { if (prof) ev_record(); };
+    wr_device *dev = s0->dev;
+    auto grid = [&](const void *fn, int block) {
+        return std::max(1, std::min<int>((P + block - 1) / block, dev->n_cu * occupancy(dev, fn, block)));
+    };
+    const int g_regen = grid(reinterpret_cast<const void *>(&k_regen<true>), kShadeBlock);
+    const int g_shade = grid(reinterpret_cast<const void *>(&k_shade<true>), kShadeBlock);
+    const int g_ext = trav_grid(s0, P, occupancy(dev, reinterpret_cast<const void *>(&k_extend<false, true>), kBlock));
+    const int g_shd = trav_grid(s0, 2 * P, occupancy(dev, reinterpret_cast<const void *>(&k_shadow<false, true>), kBlock));
+    std::vector<BatchJob> tab;
+    for (;;) {
+        // plan the pass: whole spp planes of each unfinished job, job-major, within the pass budget
+        tab.clear();
+        std::vector<uint32_t> tab_job;
+        uint64_t budget = kMaxSamplesPerPass, offset = 0;
+        for (uint32_t j = 0; j < n_jobs; j++) {
+            const wr_batch_job &J = jobs[j];
+            if (J.num_path_vertices <= 1 || done[j] >= uint32_t(J.spp)) continue;
+            const uint32_t wh = uint32_t(J.scene->W) * uint32_t(J.scene->H);
+            const uint32_t k = uint32_t(std::min<uint64_t>(uint64_t(J.spp) - done[j], budget / wh));
+            if (k == 0) continue;
+            BatchJob b{};
+            b.s = J.scene->ds;
+            b.offset = uint32_t(offset);
+            b.count = k * wh;
+            b.wh = wh;
+            b.k = k;
+            b.first_sample = uint32_t(J.first_sample) + done[j];
+            b.seed = J.seed;
+            b.N = J.num_path_vertices;
+            b.use_mis = J.use_mis ? 1 : 0;
+            b.film_base = J.film->d_base;
+            b.film_splat = J.film->d_splat;
+            tab.push_back(b);
+            tab_job.push_back(j);
+            offset += b.count;
+            budget -= b.count;
+        }
+        if (tab.empty()) break;
+        WR_HIP(hipMemcpyAsync(d_tab, tab.data(), sizeof(BatchJob) * tab.size(), hipMemcpyHostToDevice, s0->stream));
+        RenderArgs a{};
+        a.total = uint32_t(offset);
+        a.capacity = P;
+        a.N = maxN;
+        const uint32_t nt = uint32_t(tab.size());
+        k_init<<<1, 64, 0, s0->stream>>>(pb, a);
+        WR_HIP(hipGetLastError());
+        const uint64_t max_iters = (uint64_t(a.total) / P + 2) * uint64_t(maxN + 1) + 64;
+        auto iteration = [&](const PathBuffers &q) {
+            begin(3);
+            k_regen<true><<<g_regen, kShadeBlock, 0, s0->stream>>>(s0->ds, d_tab, nt, q, a);
+            end();
+            begin(0);
+            if (count) k_extend<true, true><<<g_ext, kBlock, 0, s0->stream>>>(s0->ds, d_tab, q, s0->d_spill, tune);
+            else k_extend<false, true><<<g_ext, kBlock, 0, s0->stream>>>(s0->ds, d_tab, q, s0->d_spill, tune);
+            end();
+            begin(2);
+            k_shade<true><<<g_shade, kShadeBlock, 0, s0->stream>>>(s0->ds, d_tab, q, a);
+            end();
+            begin(1);
+            if (count) k_shadow<true, true><<<g_shd, kBlock, 0, s0->stream>>>(s0->ds, d_tab, q, s0->d_spill, tune);
+            else k_shadow<false, true><<<g_shd, kBlock, 0, s0->stream>>>(s0->ds, d_tab, q, s0->d_spill, tune);
+            end();
+            k_rotate<<<1, 1, 0, s0->stream>>>(q, a);
+        };
+        rc = drain(s0, pb, max_iters, use_graph, iteration);
+        if (rc) return rc;
+        begin(4);
+        for (uint32_t t = 0; t < nt; t++) {
+            const BatchJob &b = tab[t];
+            k_accumulate<<<std::max<uint32_t>(1, std::min<uint32_t>((b.wh + 255) / 256, uint32_t(dev->n_cu) * 8)), 256, 0,
+                           s0->stream>>>(pb.sample_rad + b.offset, b.film_base, b.wh, b.k);
+            done[tab_job[t]] += b.k;
+        }
+        end();
+        WR_HIP(hipGetLastError());
+    }
+    unsigned long long hs[6] = {0};
+    WR_HIP(hipMemcpyAsync(hs, pb.stats, sizeof hs, hipMemcpyDeviceToHost, s0->stream));
+    WR_HIP(hipStreamSynchronize(s0->stream));
+    st.paths = paths;
+    st.extend_rays = hs[0];
+    st.shadow_rays = hs[1];
+    st.extend_nodes = hs[2];
+    st.extend_tris = hs[3];
+    st.shadow_nodes = hs[4];
+    st.shadow_tris = hs[5];
+    for (const Mark &m : marks) {
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, evs[m.e0], evs[m.e0 + 1]);
+        switch (m.kind) {
+        case 0: st.extend_ms += ms; st.extend_launches++; break;
+        case 1: st.shadow_ms += ms; st.shadow_launches++; break;
+        case 2: st.shade_ms += ms; break;
+        case 3: st.generate_ms += ms; break;
+        default: st.accumulate_ms += ms; break;
+        }
+    }
+    for (hipEvent_t e : evs) (void)hipEventDestroy(e);
+    st.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (stats) *stats = st;
+    return WR_OK;
+}
+
+int wr_autotune(const wr_batch_job *jobs, uint32_t n_jobs, const wr_render_params *base, wr_render_params *best,
+                double *mrays) {
+    if (!jobs || !base || !best) return set_error(WR_ERR_INVALID_ARGUMENT, "null argument");
+    if (n_jobs == 0) return set_error(WR_ERR_INVALID_ARGUMENT, "empty batch");
+    // trial jobs: 1 spp each into scratch films
+    std::vector<wr_batch_job> trial(jobs, jobs + n_jobs);
+    std::vector<wr_film *> films;
+    int rc = WR_OK;
+    for (auto &t : trial) {
+        if (!t.scene) { rc = set_error(WR_ERR_INVALID_ARGUMENT, "batch job without scene"); break; }
+        wr_film *f = nullptr;
+        rc = wr_film_create(t.scene, &f);
+        if (rc) break;
+        films.push_back(f);
+        t.film = f;
+        t.spp = std::min(t.spp, 1);
+    }
+    struct Candidate { uint32_t pool, chunk, refill; int graph; };
+    std::vector<Candidate> cands;
+    for (uint32_t pool : {1u << 22, 1u << 23, 1u << 24, 1u << 25})
+        for (auto ck : {std::make_pair(64u, 16u), std::make_pair(128u, 24u), std::make_pair(256u, 24u), std::make_pair(128u, 40u)})
+            for (int graph : {0, 1}) cands.push_back({pool, ck.first, ck.second, graph});
+    double best_rate = -1.0;
+    wr_render_params bp = *base;
+    if (!rc) {
+        wr_render_params warm = *base;
+        warm.profile = 0;
+        warm.count_traversal = 0;
+        wr_render_stats ws;
+        rc = wr_render_batch(trial.data(), n_jobs, &warm, &ws);  // allocations, first launches
+    }
+    for (size_t c = 0; !rc && c < cands.size(); c++) {
+        wr_render_params p = *base;
+        p.profile = 0;
+        p.count_traversal = 0;
+        p.max_paths_in_flight = int32_t(cands[c].pool);
+        p.fetch_chunk = cands[c].chunk;
+        p.refill_lanes = cands[c].refill;
+        p.use_graph = cands[c].graph;
+        wr_render_stats st;
+        rc = wr_render_batch(trial.data(), n_jobs, &p, &st);
+        if (rc) break;
+        const double rate = double(st.extend_rays + st.shadow_rays) / (st.wall_ms * 1e-3) / 1e6;
+        if (rate > best_rate) {
+            best_rate = rate;
+            bp = p;
+        }
+    }
+    const std::string err = rc ? wr::error_string() : std::string();
+    for (wr_film *f : films) wr_film_destroy(f);
+    if (rc) {
+        wr::error_string() = err;
+        return rc;
+    }
+    bp.profile = base->profile;
+    bp.count_traversal = base->count_traversal;
+    *best = bp;
+    if (mrays) *mrays = best_rate;
+    return WR_OK;
 }
 
 static int trace_impl(wr_scene *s, const float *d_rays, uint32_t n, void *d_hits, uint8_t *d_occ, bool any, float *ms) {

@@ -148,8 +148,9 @@ __device__ __forceinline__ void trav_begin(TravState &ts, float3 o, float3 d, fl
 // then pops if needed.  Returns true when the lane's ray is finished.
 // Closest hit (ANY = false) visits children near-to-far and culls stack entries
 // farther than the current hit; any hit (ANY = true) stops at the first hit.
-template <bool ANY, bool COUNT, int kBlock>
-__device__ __forceinline__ bool trav_step(const DScene &s, TravState &ts, LaneStack<kBlock> &st, TraversalCounters &cnt) {
+// G is any scene handle with `nodes` and `tris` (DScene, or a per-lane Geo).
+template <bool ANY, bool COUNT, int kBlock, class G>
+__device__ __forceinline__ bool trav_step(const G &s, TravState &ts, LaneStack<kBlock> &st, TraversalCounters &cnt) {
     const RaySetup &r = ts.r;
     const uint32_t cur = ts.cur;
     if (!(cur & kLeafBit)) {
@@ -225,8 +226,8 @@ __device__ __forceinline__ bool trav_step(const DScene &s, TravState &ts, LaneSt
 }
 
 // Whole traversal of one ray (used where rays are not refilled per lane).
-template <bool ANY, bool COUNT, int kBlock>
-__device__ __forceinline__ uint32_t traverse(const DScene &s, const RaySetup &r, float &t_best, LaneStack<kBlock> &st,
+template <bool ANY, bool COUNT, int kBlock, class G>
+__device__ __forceinline__ uint32_t traverse(const G &s, const RaySetup &r, float &t_best, LaneStack<kBlock> &st,
                                              TraversalCounters &cnt) {
     TravState ts;
     ts.r = r;

@@ -274,6 +274,59 @@ def guide_room(seed: int = 4, n_spheres: int = 49, level: int = 5, resolution=(1
                    name=f"guide_room_s{seed}_n{n_spheres}_l{level}")
 
 
+def cornell_clutter(seed: int, n_tris: int, resolution=(256, 256)) -> Scene:
+    """The Cornell-style box of `cornell` filled with seeded icosphere clutter
+    of about n_tris triangles (the batched-workload generator): level-5, 4 or 3
+    spheres depending on the budget, Lambert with random albedo."""
+    base = cornell(seed, resolution)
+    rng = np.random.default_rng(seed)
+    level = 5 if n_tris >= 200_000 else (4 if n_tris >= 40_000 else 3)
+    sv, sf = icosphere(level)
+    per = len(sf)
+    count = max(1, int(round(n_tris / per)))
+    b = SceneBuilder()
+    # rebuild: the box first (its geometries and materials), then the clutter
+    for g in range(len(base.geom_material)):
+        sel = base.tri_geom == g
+        tris = base.tris[sel]
+        used, inv = np.unique(tris.ravel(), return_inverse=True)
+        verts = base.verts[used]
+        if base.geom_light[g] >= 0:
+            b.mesh(verts, inv.reshape(-1, 3), light_L=tuple(base.lights[base.geom_light[g]]["L"]))
+        else:
+            m = base.materials[base.geom_material[g]]
+            b.mesh(verts, inv.reshape(-1, 3), material=b.material(int(m["type"]), rho_d=tuple(m["rho_d"]),
+                                                                   rho_s=tuple(m["rho_s"]), sigma=float(m["sigma"]),
+                                                                   exponent=float(m["exponent"]),
+                                                                   twosided=bool(m["twosided"])))
+    r = 0.25 * (1.0 / count) ** (1.0 / 3.0) + 0.02
+    centres = rng.uniform(-0.85, 0.85, size=(count, 3))
+    radii = rng.uniform(0.4 * r, r, size=count)
+    albedo = rng.uniform(0.2, 0.8, size=(count, 3))
+    for i in range(count):
+        b.mesh(sv * radii[i] + centres[i], sf, material=b.material(LAMBERT, rho_d=tuple(albedo[i])))
+    cam = base.camera
+    return b.build(dict(pos=tuple(cam["pos"]), lookat=tuple(cam["lookat"]), up=tuple(cam["up"]),
+                        fovy=float(cam["fovy_deg"]), resolution=resolution),
+                   name=f"cornell_clutter_s{seed}_t{n_tris}")
+
+
+def batch_workload(seed: int = 3, n_scenes: int = 32, tri_range=(10_000, 2_000_000),
+                   resolutions=(128, 256, 512, 1024), spp_range=(16, 256), depth_range=(8, 16)):
+    """The batched workload: scene i uses seed 1000 + i; triangle counts
+    log-uniform in tri_range, square resolutions from `resolutions`, spp and
+    max depth uniform integers.  Returns [(scene, spp, num_path_vertices)]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n_scenes):
+        t = int(math.exp(rng.uniform(math.log(tri_range[0]), math.log(tri_range[1]))))
+        res = int(rng.choice(resolutions))
+        spp = int(rng.integers(spp_range[0], spp_range[1] + 1))
+        depth = int(rng.integers(depth_range[0], depth_range[1] + 1))
+        out.append((cornell_clutter(1000 + i, t, (res, res)), spp, depth + 2))
+    return out
+
+
 def emitting_box(resolution=(24, 24), L=1.0, half=1.0) -> Scene:
     """Camera at the centre of a closed cube whose six faces emit L inward (no
     reflecting surface): with a purely absorbing medium a pixel sees
