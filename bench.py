@@ -48,6 +48,8 @@ def parse():
     ap.add_argument("--ttr-ref-spp", type=int, default=32768)
     ap.add_argument("--ttr-runs", type=int, default=3)
     ap.add_argument("--ttr-train-spp", type=int, default=256)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-scene measurement")
+    ap.add_argument("--batch-scenes", type=int, default=32)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first copy 1 GiB device-to-device three times (known-size kernel for byte counters)")
     return ap.parse_args()
@@ -184,6 +186,57 @@ def time_to_relmse(wr, scenes, dev, a):
             "speedup_guided_vs_plain_interpolated": spc}
 
 
+def batched(wr, scenes, dev, a):
+    """The batched workload: 32 seeded Cornell-box scenes with icosphere clutter
+    (10k..2M triangles, log-uniform), resolutions 128^2..1024^2, 16..256 spp, max
+    depth 8..16, rendered as one batch (one launch of each kernel per iteration,
+    scheduling from the autotuner), then one scene at a time for comparison.
+    Per-scene relMSE is batched vs individual film (same random numbers)."""
+    import numpy as np
+    t0 = time.perf_counter()
+    work = scenes.batch_workload(seed=3, n_scenes=a.batch_scenes)
+    gsc = [wr.GpuScene(dev, sc) for sc, _, _ in work]
+    build_s = time.perf_counter() - t0
+    tris = sum(sc.n_tris for sc, _, _ in work)
+    jobs = [dict(scene=g, spp=spp, num_path_vertices=N, seed=1000 + i)
+            for i, (g, (sc, spp, N)) in enumerate(zip(gsc, work))]
+    t0 = time.perf_counter()
+    tuning, trial_rate = wr.autotune(jobs)
+    tune_s = time.perf_counter() - t0
+    for j in jobs:
+        j["film"] = wr.Film(j["scene"])
+    t0 = time.perf_counter()
+    st = wr.render_batch(jobs, tuning=tuning)
+    t_batch = time.perf_counter() - t0
+    rays_b = st.extend_rays + st.shadow_rays
+    print(f"# batch: {len(jobs)} scenes, {tris} tris, {st.paths} paths in {t_batch:.2f}s "
+          f"({rays_b / t_batch / 1e6:.0f} Mrays/s); build {build_s:.1f}s, autotune {tune_s:.1f}s", file=sys.stderr,
+          flush=True)
+    t_ind, rays_i, rel = 0.0, 0, []
+    for j in jobs:
+        f = wr.Film(j["scene"])
+        t0 = time.perf_counter()
+        s1 = wr.render_nee(j["scene"], f, j["spp"], num_path_vertices=j["num_path_vertices"], seed=j["seed"])
+        t_ind += time.perf_counter() - t0
+        rays_i += s1.extend_rays + s1.shadow_rays
+        rel.append(rel_mse(j["film"].image(), f.image()))
+        f.close()
+        j["film"].close()
+        j["scene"].close()
+    print(f"# individually: {t_ind:.2f}s ({rays_i / t_ind / 1e6:.0f} Mrays/s); max per-scene relMSE(batch, single) "
+          f"{max(rel):.2e}", file=sys.stderr, flush=True)
+    return {"workload": f"{len(jobs)} seeded Cornell boxes with icosphere clutter (scene i seed 1000+i), "
+                        f"{tris} triangles, 128^2..1024^2, 16..256 spp, max depth 8..16",
+            "paths": st.paths, "rays": rays_b, "seconds": round(t_batch, 3),
+            "mrays_per_s": round(rays_b / t_batch / 1e6, 1),
+            "individual_seconds": round(t_ind, 3), "individual_mrays_per_s": round(rays_i / t_ind / 1e6, 1),
+            "speedup_vs_individual": round(t_ind / t_batch, 3),
+            "autotuned": {"max_paths_in_flight": tuning.max_paths_in_flight, "fetch_chunk": tuning.fetch_chunk,
+                          "refill_lanes": tuning.refill_lanes, "use_graph": bool(tuning.use_graph),
+                          "seconds": round(tune_s, 2)},
+            "per_scene_relmse_vs_individual": [float(f"{x:.3e}") for x in rel]}
+
+
 def cpu_baseline(sc, N, target_s):
     """The oracle (FP64 restatement of the reference path) on the host cores:
     whole frames at 1 spp, successive sample indices, until ~target_s of work."""
@@ -264,6 +317,9 @@ def main():
                 "avg_launch_ms": round(avg_ms, 4), "launches": ext_launches,
                 "nodes_per_ray": round(nodes_per_ray, 2), "tris_per_ray": round(tris_per_ray, 2),
                 "bytes_per_ray": round(EXT_RAY_BYTES + nodes_per_ray * NODE_BYTES + tris_per_ray * TRI_BYTES, 1)}
+    bat = None
+    if not a.no_batch:
+        bat = batched(wr, scenes, dev, a)
     ttr = None
     if not a.no_ttr:
         ttr = time_to_relmse(wr, scenes, dev, a)
@@ -294,6 +350,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "time_to_relmse": ttr,
+        "batched": bat,
     }
     print(json.dumps(out))
 
