@@ -142,12 +142,47 @@ int wr_scene_validate(const wr_scene_desc *desc);
  * traversal stack).  Host only; no GPU needed. */
 int wr_bvh_selftest(const wr_scene_desc *desc, uint64_t *n_nodes, uint32_t *max_stack, uint32_t *depth);
 
-/* Uploads geometry, materials, lights and camera and builds the BVH.
+/* Uploads geometry, materials, lights and camera and builds the BVH on the GPU.
  * Replaces EmbreeAccel::addGeometry + commit (embreeaccel.h:274-315,466-469),
  * GeneralSubpathSampler's light CdfTable (generalsubpathsampler.h:162-183) and
  * ThinlensCamera construction (thinlens.h:40-55). */
 int wr_scene_create(wr_device *dev, const wr_scene_desc *desc, wr_scene **out);
 void wr_scene_destroy(wr_scene *scene);
+/* BVH builders.  WR_BVH_GPU (default): PLOC clustering (Meister and Bittner
+ * 2018) of Morton-ordered triangles on the GPU with SAH leaf selection,
+ * collapsed to the 4-wide node layout on the GPU; deterministic.  WR_BVH_HOST:
+ * binned SAH on the host, the same layout.  Both stand where rtcCommitScene
+ * builds Embree's BVH8 (embreeaccel.h:466-469); any valid BVH gives the same
+ * closest hit, so the choice changes build and traversal time only. */
+#define WR_BVH_GPU 0
+#define WR_BVH_HOST 1
+typedef struct {
+    int32_t builder;             /* WR_BVH_GPU or WR_BVH_HOST */
+    int32_t ploc_radius;         /* GPU builder: clusters searched on each side (0 = default 16, max 64) */
+} wr_build_params;
+
+/* wr_scene_create with an explicit builder (params NULL = defaults). */
+int wr_scene_create_ex(wr_device *dev, const wr_scene_desc *desc, const wr_build_params *params, wr_scene **out);
+
+typedef struct {
+    int32_t builder;
+    uint32_t depth;              /* node levels */
+    uint32_t max_stack;          /* worst-case traversal stack entries */
+    uint32_t iterations;         /* clustering iterations (GPU builder) */
+    uint64_t n_nodes;
+    double build_ms;             /* wall time of the build, including the mesh upload */
+    double sah_cost;             /* expected node + triangle tests of a ray through the root box:
+                                  * sum over nodes of area/area(root) + sum over leaves of
+                                  * count * area/area(root) */
+    uint64_t hash;               /* FNV-1a over the nodes and the leaf-order triangles */
+} wr_bvh_stats;
+
+/* Diagnostic (not on the render path): downloads the BVH, checks the invariants
+ * of wr_bvh_selftest on it against `desc` (the description the scene was created
+ * from) and fills the statistics.  No reference counterpart (Embree's build is
+ * internal). */
+int wr_scene_bvh_check(wr_scene *scene, const wr_scene_desc *desc, wr_bvh_stats *stats);
+
 /* BVH statistics: nodes, leaf triangles, max stack depth, bytes of nodes+tris. */
 int wr_scene_info(const wr_scene *scene, uint64_t *n_nodes, uint64_t *n_tris, uint32_t *max_stack,
                   uint64_t *device_bytes);

@@ -79,6 +79,22 @@ def test_zero_area_light_is_rejected_at_scene_creation_checks():
         oracle.OracleScene(sc)
 
 
+def test_non_finite_vertices_are_rejected():
+    sc = scenes.cornell(1, resolution=(8, 8))
+    sc.verts = sc.verts.copy()
+    sc.verts[5, 1] = np.nan
+    _expect_invalid(sc, "non-finite vertex")
+    sc.verts[5, 1] = np.inf
+    _expect_invalid(sc, "non-finite vertex")
+
+
+def test_gpu_build_entry_points_exported():
+    names = set(declared_functions())
+    assert {"wr_scene_create_ex", "wr_scene_bvh_check"} <= names
+    assert wr.BVH_GPU == 0 and wr.BVH_HOST == 1
+    assert C.sizeof(wr.BvhStats) == 48 and C.sizeof(wr.BuildParams) == 8
+
+
 @pytest.mark.parametrize("maker", ["cornell", "soup", "icospheres_small", "plane"])
 def test_bvh_invariants(maker):
     sc = {

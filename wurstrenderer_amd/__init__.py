@@ -96,6 +96,22 @@ class GuideStats(C.Structure):
 GUIDE_NUM_PARAMS = 17192
 GUIDE_NUM_OUTPUTS = 40
 
+BVH_GPU, BVH_HOST = 0, 1
+BUILDERS = {"gpu": BVH_GPU, "host": BVH_HOST}
+
+
+class BuildParams(C.Structure):
+    _fields_ = [("builder", C.c_int32), ("ploc_radius", C.c_int32)]
+
+
+class BvhStats(C.Structure):
+    _fields_ = [("builder", C.c_int32), ("depth", C.c_uint32), ("max_stack", C.c_uint32), ("iterations", C.c_uint32),
+                ("n_nodes", C.c_uint64), ("build_ms", C.c_double), ("sah_cost", C.c_double), ("hash", C.c_uint64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 # name -> (restype, argtypes); every symbol declared in include/wurst_amd.h
 P = C.c_void_p
 EXPORTS = {
@@ -107,6 +123,8 @@ EXPORTS = {
     "wr_bvh_selftest": (C.c_int, [C.POINTER(_SceneDesc), C.POINTER(C.c_uint64), C.POINTER(C.c_uint32),
                                   C.POINTER(C.c_uint32)]),
     "wr_scene_create": (C.c_int, [P, C.POINTER(_SceneDesc), C.POINTER(P)]),
+    "wr_scene_create_ex": (C.c_int, [P, C.POINTER(_SceneDesc), C.POINTER(BuildParams), C.POINTER(P)]),
+    "wr_scene_bvh_check": (C.c_int, [P, C.POINTER(_SceneDesc), C.POINTER(BvhStats)]),
     "wr_scene_destroy": (None, [P]),
     "wr_scene_info": (C.c_int, [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint32),
                                 C.POINTER(C.c_uint64)]),
@@ -228,14 +246,20 @@ def bvh_selftest(sc: scenes.Scene) -> dict:
 class GpuScene:
     """A scene resident in HBM (BVH, triangles, materials, lights, camera)."""
 
-    def __init__(self, device: Device, sc: scenes.Scene):
+    def __init__(self, device: Device, sc: scenes.Scene, builder: str = "gpu", ploc_radius: int = 0):
+        """builder: "gpu" (PLOC on the GPU, the default of wr_scene_create) or "host"
+        (binned SAH); ploc_radius: the GPU builder's search window (0 = default)."""
+        if builder not in BUILDERS:
+            raise ValueError(f"builder must be one of {sorted(BUILDERS)}")
         d, keep = scene_desc(sc)
         h = C.c_void_p()
-        _check(lib().wr_scene_create(device.h, C.byref(d), C.byref(h)))
+        bp = BuildParams(BUILDERS[builder], int(ploc_radius))
+        _check(lib().wr_scene_create_ex(device.h, C.byref(d), C.byref(bp), C.byref(h)))
         self.h = h
         _LIVE["scene"].add(self)
         self.device = device
         self.W, self.H = sc.resolution
+        self.builder = builder
         del keep
         if getattr(sc, "medium", None):
             self.set_medium(sc.medium["sigma_a"], sc.medium["sigma_s"], sc.medium["g"])
@@ -262,6 +286,15 @@ class GpuScene:
         ms = C.c_uint32()
         _check(lib().wr_scene_info(self.h, C.byref(nn), C.byref(nt), C.byref(ms), C.byref(nb)))
         return dict(nodes=nn.value, tris=nt.value, max_stack=ms.value, device_bytes=nb.value)
+
+    def bvh_check(self, sc: scenes.Scene) -> dict:
+        """Downloads the BVH, checks its invariants against `sc` (the scene this
+        was built from; raises WrError on a violation) and returns its statistics."""
+        d, keep = scene_desc(sc)
+        st = BvhStats()
+        _check(lib().wr_scene_bvh_check(self.h, C.byref(d), C.byref(st)))
+        del keep
+        return st.as_dict()
 
     def trace_closest(self, rays: np.ndarray):
         rays = np.ascontiguousarray(rays, np.float32)

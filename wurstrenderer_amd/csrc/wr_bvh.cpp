@@ -179,6 +179,9 @@ struct Collapser {
             kids[pick] = l;
             kids[k++] = l + 1;
         }
+        // slot order = decreasing area: any-hit traversal visits the slots in
+        // order, and a larger box is likelier to hold an occluder
+        std::stable_sort(kids, kids + k, [&](uint32_t a, uint32_t b) { return area(b2[a]) > area(b2[b]); });
         const uint32_t me = uint32_t(out.size());
         out.emplace_back();
         Node4 nd;

@@ -112,6 +112,10 @@ struct wr_scene {
     DLight *d_lights = nullptr;
     uint64_t n_nodes = 0, n_tris = 0, device_bytes = 0;
     uint32_t max_stack = 0;
+    int builder = WR_BVH_GPU;           // which builder made the BVH
+    int ploc_radius = 0;                // its clustering window (GPU builder; 0 = default)
+    uint32_t bvh_depth = 0, bvh_iterations = 0;
+    double build_ms = 0.0;
     // host side: triangle index -> (geom, prim)
     std::vector<uint32_t> tri_geom, geom_first, leaf_to_orig;
     int W = 0, H = 0;
@@ -134,6 +138,23 @@ static inline int set_device(wr_device *dev) {
     WR_HIP(hipSetDevice(dev->ordinal));
     return WR_OK;
 }
+
+// ============================================================== GPU BVH build (wr_bvh_gpu.hip)
+namespace wr {
+struct GpuBvhResult {
+    Node4 *nodes = nullptr;     // device, n_nodes, breadth-first (root = 0, children after parents)
+    TriRecord *tris = nullptr;  // device, leaf order
+    uint32_t n_nodes = 0, max_stack = 0, depth = 0, iterations = 0;
+    double ms = 0.0;            // wall time of the build
+};
+// Builds from a mesh already in device memory (verts [3n_v], tris [3n], tri_geom [n],
+// indices validated) with a clustering window of `radius` clusters on each side
+// (0: default, at most 64); leaf_to_orig (host, n entries) receives the input
+// triangle of each leaf-order record.  On error the caller frees whatever
+// out->nodes / out->tris hold.
+int build_bvh4_gpu(const float *d_verts, const uint32_t *d_tris, const uint32_t *d_tri_geom, uint32_t n,
+                   hipStream_t stream, int radius, uint32_t *leaf_to_orig, GpuBvhResult *out);
+}  // namespace wr
 
 // ============================================================== guiding (wr_guide.hip)
 #include "wr_guide.h"

@@ -998,6 +998,9 @@ static int validate_desc(const wr_scene_desc *d) {
         return set_error(WR_ERR_INVALID_ARGUMENT, "null geometry arrays");
     if (d->n_tris >= kMaxTriangles) return set_error(WR_ERR_UNSUPPORTED, "too many triangles (limit 2^28)");
     if (d->n_lights == 0) return set_error(WR_ERR_INVALID_ARGUMENT, "scene has no lights");
+    if (d->n_verts > 0 && !d->verts) return set_error(WR_ERR_INVALID_ARGUMENT, "null vertex array");
+    for (size_t i = 0; i < 3 * size_t(d->n_verts); i++)
+        if (!std::isfinite(d->verts[i])) return set_error(WR_ERR_INVALID_ARGUMENT, "non-finite vertex coordinate");
     if (d->camera.width <= 0 || d->camera.height <= 0) return set_error(WR_ERR_INVALID_ARGUMENT, "bad resolution");
     if ((uint64_t)d->camera.width * d->camera.height >= (1ull << 31))
         return set_error(WR_ERR_UNSUPPORTED, "resolution too large");
@@ -1052,7 +1055,53 @@ static int upload(T **dst, const void *src, size_t bytes, uint64_t &total) {
     return WR_OK;
 }
 
-static int scene_create_impl(wr_device *dev, const wr_scene_desc *d, wr_scene *s) {
+// Host SAH build, uploaded.
+static int bvh_host(wr_scene *s, const wr_scene_desc *d) {
+    const auto t0 = std::chrono::steady_clock::now();
+    BvhBuildResult bvh;
+    if (!build_bvh4(d->verts, d->n_verts, d->tris, d->n_tris, d->tri_geom, &bvh))
+        return set_error(WR_ERR_OUT_OF_MEMORY, "host allocation failed during BVH build");
+    s->leaf_to_orig.resize(d->n_tris);
+    for (uint32_t i = 0; i < d->n_tris; i++) s->leaf_to_orig[i] = bvh.tris[i].orig;
+    s->n_nodes = bvh.n_nodes;
+    s->max_stack = bvh.max_stack;
+    s->bvh_depth = bvh.depth;
+    int rc = upload(&s->d_nodes, bvh.nodes, sizeof(Node4) * bvh.n_nodes, s->device_bytes);
+    if (!rc) rc = upload(&s->d_tris, bvh.tris, sizeof(TriRecord) * bvh.n_tris, s->device_bytes);
+    free_bvh(&bvh);
+    s->build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return rc;
+}
+
+// GPU PLOC build from a temporary device copy of the mesh.
+static int bvh_gpu(wr_scene *s, const wr_scene_desc *d) {
+    const auto t0 = std::chrono::steady_clock::now();
+    const uint32_t n = d->n_tris;
+    float *dv = nullptr;
+    uint32_t *dt = nullptr, *dg = nullptr;
+    uint64_t mesh_bytes = 0;
+    GpuBvhResult g;
+    int rc = upload(&dv, d->verts, sizeof(float) * 3 * size_t(d->n_verts), mesh_bytes);
+    if (!rc) rc = upload(&dt, d->tris, sizeof(uint32_t) * 3 * size_t(n), mesh_bytes);
+    if (!rc) rc = upload(&dg, d->tri_geom, sizeof(uint32_t) * size_t(n), mesh_bytes);
+    s->leaf_to_orig.resize(n);
+    if (!rc) rc = build_bvh4_gpu(dv, dt, dg, n, s->stream, s->ploc_radius, s->leaf_to_orig.data(), &g);
+    (void)hipFree(dv);
+    (void)hipFree(dt);
+    (void)hipFree(dg);
+    s->d_nodes = g.nodes;
+    s->d_tris = reinterpret_cast<float4 *>(g.tris);
+    if (rc) return rc;
+    s->device_bytes += sizeof(Node4) * uint64_t(g.n_nodes) + sizeof(TriRecord) * uint64_t(n);
+    s->n_nodes = g.n_nodes;
+    s->max_stack = g.max_stack;
+    s->bvh_depth = g.depth;
+    s->bvh_iterations = g.iterations;
+    s->build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return WR_OK;
+}
+
+static int scene_create_impl(wr_device *dev, const wr_scene_desc *d, int builder, int radius, wr_scene *s) {
     s->dev = dev;
     int rc = set_device(dev);
     if (rc) return rc;
@@ -1073,32 +1122,18 @@ static int scene_create_impl(wr_device *dev, const wr_scene_desc *d, wr_scene *s
     for (uint32_t t = 0; t < d->n_tris; t++)
         if (s->geom_first[d->tri_geom[t]] == kInvalid) s->geom_first[d->tri_geom[t]] = t;
     // BVH
-    BvhBuildResult bvh;
-    if (!build_bvh4(d->verts, d->n_verts, d->tris, d->n_tris, d->tri_geom, &bvh))
-        return set_error(WR_ERR_OUT_OF_MEMORY, "host allocation failed during BVH build");
-    if (bvh.max_stack > (uint32_t)kStackDepth) {
-        free_bvh(&bvh);
-        return set_error(WR_ERR_UNSUPPORTED, "BVH deeper than the traversal stack");
-    }
+    s->builder = builder;
+    s->ploc_radius = radius;
+    s->n_tris = d->n_tris;
+    rc = builder == WR_BVH_HOST ? bvh_host(s, d) : bvh_gpu(s, d);
+    if (rc) return rc;
+    if (s->max_stack > (uint32_t)kStackDepth) return set_error(WR_ERR_UNSUPPORTED, "BVH deeper than the traversal stack");
     s->max_trav_blocks = dev->n_cu * kMaxTraversalBlocksPerCU;
     {
         const size_t spill_bytes = sizeof(uint2) * size_t(s->max_trav_blocks) * kBlock * kSpillDepth;
-        hipError_t e = hipMalloc(&s->d_spill, spill_bytes);
-        if (e != hipSuccess) {
-            free_bvh(&bvh);
-            return set_error(WR_ERR_OUT_OF_MEMORY, "traversal stack spill allocation failed");
-        }
+        WR_HIP(hipMalloc(&s->d_spill, spill_bytes));
         s->device_bytes += spill_bytes;
     }
-    s->leaf_to_orig.resize(d->n_tris);
-    for (uint32_t i = 0; i < d->n_tris; i++) s->leaf_to_orig[i] = bvh.tris[i].orig;
-    s->n_nodes = bvh.n_nodes;
-    s->n_tris = d->n_tris;
-    s->max_stack = bvh.max_stack;
-    rc = upload(&s->d_nodes, bvh.nodes, sizeof(Node4) * bvh.n_nodes, s->device_bytes);
-    if (!rc) rc = upload(&s->d_tris, bvh.tris, sizeof(TriRecord) * bvh.n_tris, s->device_bytes);
-    free_bvh(&bvh);
-    if (rc) return rc;
     // geometry info
     std::vector<int2> gi(d->n_geoms);
     for (uint32_t g = 0; g < d->n_geoms; g++) gi[g] = make_int2(d->geom_material[g], d->geom_light[g]);
@@ -1218,12 +1253,20 @@ static int scene_create_impl(wr_device *dev, const wr_scene_desc *d, wr_scene *s
 }
 
 int wr_scene_create(wr_device *dev, const wr_scene_desc *desc, wr_scene **out) {
+    return wr_scene_create_ex(dev, desc, nullptr, out);
+}
+
+int wr_scene_create_ex(wr_device *dev, const wr_scene_desc *desc, const wr_build_params *params, wr_scene **out) {
     if (!dev || !out) return set_error(WR_ERR_INVALID_ARGUMENT, "null argument");
     *out = nullptr;
+    const int builder = params ? params->builder : WR_BVH_GPU;
+    const int radius = params ? params->ploc_radius : 0;
+    if (builder != WR_BVH_GPU && builder != WR_BVH_HOST) return set_error(WR_ERR_INVALID_ARGUMENT, "unknown BVH builder");
+    if (radius < 0 || radius > 64) return set_error(WR_ERR_INVALID_ARGUMENT, "ploc_radius must be in [0, 64]");
     int rc = validate_desc(desc);
     if (rc) return rc;
     wr_scene *s = new wr_scene;
-    rc = scene_create_impl(dev, desc, s);
+    rc = scene_create_impl(dev, desc, builder, radius, s);
     if (rc) {
         std::string msg = wr::error_string();
         wr_scene_destroy(s);
@@ -1240,6 +1283,70 @@ int wr_scene_info(const wr_scene *s, uint64_t *n_nodes, uint64_t *n_tris, uint32
     if (n_tris) *n_tris = s->n_tris;
     if (max_stack) *max_stack = s->max_stack;
     if (bytes) *bytes = s->device_bytes;
+    return WR_OK;
+}
+
+int wr_scene_bvh_check(wr_scene *s, const wr_scene_desc *d, wr_bvh_stats *st) {
+    if (!s || !d || !st) return set_error(WR_ERR_INVALID_ARGUMENT, "null argument");
+    if (d->n_tris != s->n_tris) return set_error(WR_ERR_INVALID_ARGUMENT, "desc is not the scene's description");
+    int rc = set_device(s->dev);
+    if (rc) return rc;
+    BvhBuildResult r;
+    r.n_nodes = uint32_t(s->n_nodes);
+    r.n_tris = uint32_t(s->n_tris);
+    r.max_stack = s->max_stack;
+    r.depth = s->bvh_depth;
+    std::vector<Node4> nodes(s->n_nodes);
+    std::vector<TriRecord> tris(s->n_tris);
+    WR_HIP(hipStreamSynchronize(s->stream));
+    WR_HIP(hipMemcpy(nodes.data(), s->d_nodes, sizeof(Node4) * nodes.size(), hipMemcpyDeviceToHost));
+    if (!tris.empty())
+        WR_HIP(hipMemcpy(tris.data(), s->d_tris, sizeof(TriRecord) * tris.size(), hipMemcpyDeviceToHost));
+    r.nodes = nodes.data();
+    r.tris = tris.data();
+    const std::string err = verify_bvh4(r, d->verts, d->tris, d->n_tris);
+    if (!err.empty()) return set_error(WR_ERR_DEVICE, "BVH invariant violated: " + err);
+    for (uint32_t i = 0; i < s->n_tris; i++)
+        if (tris[i].geom != d->tri_geom[tris[i].orig] || tris[i].orig != s->leaf_to_orig[i])
+            return set_error(WR_ERR_DEVICE, "BVH invariant violated: leaf triangle geometry id or order map");
+    // SAH cost relative to the root box
+    auto area = [](const Node4 &n, int c) {
+        const double dx = double(n.hi_x[c]) - n.lo_x[c], dy = double(n.hi_y[c]) - n.lo_y[c],
+                     dz = double(n.hi_z[c]) - n.lo_z[c];
+        return (dx < 0 || dy < 0 || dz < 0) ? 0.0 : dx * dy + dy * dz + dz * dx;
+    };
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int c = 0; c < 4; c++)
+        if (nodes[0].child[c] != kEmptyChild) {
+            lo[0] = std::min(lo[0], double(nodes[0].lo_x[c])); hi[0] = std::max(hi[0], double(nodes[0].hi_x[c]));
+            lo[1] = std::min(lo[1], double(nodes[0].lo_y[c])); hi[1] = std::max(hi[1], double(nodes[0].hi_y[c]));
+            lo[2] = std::min(lo[2], double(nodes[0].lo_z[c])); hi[2] = std::max(hi[2], double(nodes[0].hi_z[c]));
+        }
+    const double dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+    const double aroot = (s->n_tris == 0) ? 0.0 : dx * dy + dy * dz + dz * dx;
+    double cost = 1.0;
+    if (aroot > 0.0)
+        for (const Node4 &n : nodes)
+            for (int c = 0; c < 4; c++) {
+                if (n.child[c] == kEmptyChild) continue;
+                cost += area(n, c) / aroot * ((n.child[c] & kLeafBit) ? double(n.count[c]) : 1.0);
+            }
+    // FNV-1a over nodes then triangles
+    uint64_t h = 1469598103934665603ull;
+    auto mix = [&h](const void *p, size_t bytes) {
+        const uint8_t *b = static_cast<const uint8_t *>(p);
+        for (size_t i = 0; i < bytes; i++) h = (h ^ b[i]) * 1099511628211ull;
+    };
+    mix(nodes.data(), sizeof(Node4) * nodes.size());
+    mix(tris.data(), sizeof(TriRecord) * tris.size());
+    st->builder = s->builder;
+    st->depth = s->bvh_depth;
+    st->max_stack = s->max_stack;
+    st->iterations = s->bvh_iterations;
+    st->n_nodes = s->n_nodes;
+    st->build_ms = s->build_ms;
+    st->sah_cost = cost;
+    st->hash = h;
     return WR_OK;
 }
 
