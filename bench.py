@@ -50,6 +50,12 @@ def parse():
     ap.add_argument("--ttr-train-spp", type=int, default=256)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-scene measurement")
     ap.add_argument("--batch-scenes", type=int, default=32)
+    ap.add_argument("--no-medium", action="store_true", help="skip the medium BF16/FP8 inference measurement")
+    ap.add_argument("--med-width", type=int, default=1280)
+    ap.add_argument("--med-height", type=int, default=720)
+    ap.add_argument("--med-spp", type=int, default=512)
+    ap.add_argument("--med-ref-spp", type=int, default=2048)
+    ap.add_argument("--med-train-spp", type=int, default=128)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first copy 1 GiB device-to-device three times (known-size kernel for byte counters)")
     return ap.parse_args()
@@ -184,6 +190,66 @@ def time_to_relmse(wr, scenes, dev, a):
                                 "batch": 1 << 18, "lr": 1e-2, "guide_fraction": 0.5},
             "plain": res["plain"], "guided": res["guided"], "speedup_guided_vs_plain": sp,
             "speedup_guided_vs_plain_interpolated": spc}
+
+
+def medium_fp8(wr, scenes, dev, a):
+    """The gap-lit room filled with a homogeneous medium (sigma_s 0.4, sigma_a 0.05
+    RGB-equal, HG g 0.7), max depth 32, med_spp samples per pixel: plain NEE+MIS,
+    and guided with BF16 and with FP8 (e4m3) network inference -- each guided run
+    trains its own network online (BF16) for the first med_train_spp samples
+    (8-spp passes, 32 Adam steps after each) and renders the rest with it.
+    relMSE against a plain render at med_ref_spp (two halves, whose difference
+    estimates the reference's own noise floor)."""
+    N = 34
+    sc = scenes.guide_room(4, resolution=(a.med_width, a.med_height)).with_medium(sigma_a=0.05, sigma_s=0.4, g=0.7)
+    gs = wr.GpuScene(dev, sc)
+    t = time.perf_counter()
+    half = a.med_ref_spp // 2
+    fa, fb = wr.Film(gs), wr.Film(gs)
+    wr.render_nee(gs, fa, half, num_path_vertices=N, seed=7001)
+    wr.render_nee(gs, fb, half, num_path_vertices=N, seed=7002)
+    ia, ib = fa.image(), fb.image()
+    ref = 0.5 * (ia + ib)
+    ref_s = time.perf_counter() - t
+    floor = rel_mse(ia, ib) / 4.0
+    print(f"# medium reference: {a.med_ref_spp} spp in {ref_s:.1f}s, noise floor ~{floor:.2e}", file=sys.stderr,
+          flush=True)
+    res = {}
+    film = wr.Film(gs)
+    t = time.perf_counter()
+    st = wr.render_nee(gs, film, a.med_spp, num_path_vertices=N, seed=7100)
+    sec = time.perf_counter() - t
+    res["plain"] = {"seconds": round(sec, 3), "relmse": rel_mse(film.image(), ref),
+                    "mrays_per_s": round((st.extend_rays + st.shadow_rays) / sec / 1e6, 1)}
+    for name, fp8 in (("guided_bf16", False), ("guided_fp8", True)):
+        g = wr.Guide(gs, learning_rate=1e-2, batch_size=1 << 18, record_capacity=1 << 20, seed=7)
+        film = wr.Film(gs)
+        t = time.perf_counter()
+        k = min(a.med_train_spp, a.med_spp)
+        s1, _ = wr.render_guided(gs, film, g, k, num_path_vertices=N, seed=7200, guide_fraction=0.5, spp_per_pass=8,
+                                 train_steps=32, warmup_passes=1, fp8_inference=fp8)
+        rays = s1.extend_rays + s1.shadow_rays
+        if a.med_spp > k:
+            s2, _ = wr.render_guided(gs, film, g, a.med_spp - k, num_path_vertices=N, seed=7200, guide_fraction=0.5,
+                                     spp_per_pass=a.med_spp - k, train_steps=0, warmup_passes=0, fp8_inference=fp8)
+            rays += s2.extend_rays + s2.shadow_rays
+        sec = time.perf_counter() - t
+        res[name] = {"seconds": round(sec, 3), "relmse": rel_mse(film.image(), ref),
+                     "mrays_per_s": round(rays / sec / 1e6, 1)}
+        g.close()
+        film.close()
+    for k, v in res.items():
+        print(f"# medium {k}: {v['seconds']} s, relMSE {v['relmse']:.4e}, {v['mrays_per_s']} Mrays/s", file=sys.stderr,
+              flush=True)
+    gs.close()
+    return {"workload": f"guide_room seed 4 (1,003,550 tris, gap 0.2, L 500) in a homogeneous medium "
+                        f"(sigma_s 0.4, sigma_a 0.05, HG g 0.7), {a.med_width}x{a.med_height}, {a.med_spp} spp, "
+                        f"max depth 32",
+            "reference": {"spp": a.med_ref_spp, "estimator": "plain GPU (pinned to the FP64 oracle in tests)",
+                          "noise_floor_relmse": floor, "seconds": round(ref_s, 2)},
+            "guided_schedule": {"train_spp": a.med_train_spp, "spp_per_pass": 8, "adam_steps_per_pass": 32,
+                                "training": "bf16"},
+            **res}
 
 
 def batched(wr, scenes, dev, a):
@@ -323,6 +389,9 @@ def main():
     ttr = None
     if not a.no_ttr:
         ttr = time_to_relmse(wr, scenes, dev, a)
+    med = None
+    if not a.no_medium:
+        med = medium_fp8(wr, scenes, dev, a)
     cpu = None
     if not a.no_cpu_baseline:
         cpu = cpu_baseline(sc, N, a.cpu_seconds)
@@ -351,6 +420,7 @@ def main():
         "cpu_baseline": cpu,
         "time_to_relmse": ttr,
         "batched": bat,
+        "medium_bf16_fp8": med,
     }
     print(json.dumps(out))
 

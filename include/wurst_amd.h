@@ -147,6 +147,7 @@ int wr_bvh_selftest(const wr_scene_desc *desc, uint64_t *n_nodes, uint32_t *max_
  * GeneralSubpathSampler's light CdfTable (generalsubpathsampler.h:162-183) and
  * ThinlensCamera construction (thinlens.h:40-55). */
 int wr_scene_create(wr_device *dev, const wr_scene_desc *desc, wr_scene **out);
+/* Destroy a scene's films and guides first: they refer to it. */
 void wr_scene_destroy(wr_scene *scene);
 /* BVH builders.  WR_BVH_GPU (default): PLOC clustering (Meister and Bittner
  * 2018) of Morton-ordered triangles on the GPU with SAH leaf selection,
@@ -267,7 +268,8 @@ int wr_autotune(const wr_batch_job *jobs, uint32_t n_jobs, const wr_render_param
  * density, lambda / (2 pi (1 - e^{-2 lambda})) exp(lambda (mu.w - 1)) per lobe, is
  * mixed with BSDF sampling by one-sample MIS, so the estimator stays unbiased
  * for any weights.  Parameters are FP32 (layer l: W_l [in][out] row-major, then
- * b_l); the GPU computes in BF16 with FP32 accumulation. */
+ * b_l); the GPU trains in BF16 with FP32 accumulation and infers in BF16 or,
+ * optionally, e4m3 (FP8). */
 #define WR_GUIDE_NUM_PARAMS 17192
 #define WR_GUIDE_NUM_OUTPUTS 40
 
@@ -287,6 +289,8 @@ typedef struct {
     uint32_t train_steps;      /* Adam steps after each pass (0: render only) */
     uint32_t warmup_passes;    /* passes that sample the BSDF only (and collect); the network
                                   is sampled from pass warmup_passes on, trained or not */
+    int32_t fp8_inference;     /* 1: the renderer's network inference runs the e4m3 variant
+                                  (wr_guide_forward_fp8); training stays BF16 */
 } wr_guided_params;
 
 typedef struct {
@@ -307,6 +311,11 @@ int wr_guide_set_params(wr_guide *guide, const float *params);
 /* Network outputs for n host records (pos, nrm, wo: n*3 floats; nrm faces the
  * incoming ray, wo points to the previous vertex): out n*40 raw outputs. */
 int wr_guide_forward(wr_guide *guide, const float *pos, const float *nrm, const float *wo, uint32_t n, float *out);
+/* The same through the FP8 inference variant: e4m3 weights with a power-of-two
+ * scale per layer (largest |weight| scaled into (64, 128]) and e4m3 activations
+ * with a power-of-two scale per layer and record (its largest input to the
+ * layer likewise), FP32 accumulation and FP32 biases. */
+int wr_guide_forward_fp8(wr_guide *guide, const float *pos, const float *nrm, const float *wo, uint32_t n, float *out);
 /* Gradient of the loss (1/n) sum_r -weight_r log q_r(wi_r) over n host records
  * (wi: n*3 sampled directions, weight: n); grad has WR_GUIDE_NUM_PARAMS floats. */
 int wr_guide_gradient(wr_guide *guide, const float *pos, const float *nrm, const float *wo, const float *wi,

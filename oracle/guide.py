@@ -18,10 +18,17 @@ Pinning (tests/test_oracle_guide.py):
   - sample(): sample means of test functions match quadrature of pdf();
   - features(): one-blob values against the Gaussian-kernel definition at
     bin centres and edges;
-  - adam_step(): against the update written out for a scalar by hand.
+  - adam_step(): against the update written out for a scalar by hand;
+  - e4m3(): hand-computed values of the OCP E4M3 format (ties to even,
+    subnormals, saturation, 126 positive finite values);
+  - pow2_exponent(): the (64, 128] range it promises, on hand and random values;
+  - forward8(): with the quantiser's mantissa widened to 23 bits it reduces to
+    forward() (pins the per-layer and per-record scale bookkeeping), and at 3
+    bits it is a bounded perturbation of it.
 With bf16=True every rounding point of the GPU (weights, features, hidden
 activations, output gradients) is rounded to BF16 (round to nearest even), so
-the GPU's FP32-accumulated results agree to accumulation order.
+the GPU's FP32-accumulated results agree to accumulation order.  forward8()
+rounds at the GPU's FP8 points the same way.
 """
 from __future__ import annotations
 
@@ -236,3 +243,52 @@ def adam_step(params, m, v, g, t, lr=1e-2, beta1=0.9, beta2=0.999, eps=1e-8):
     mh = m / (1 - beta1 ** t)
     vh = v / (1 - beta2 ** t)
     return params - lr * mh / (np.sqrt(vh) + eps), m, v
+
+
+# ------------------------------------------------------------------ FP8 (e4m3) inference
+E4M3_MAX = 448.0
+
+
+def e4m3(x, mbits=3):
+    """Round to the OCP E4M3 grid (bias 7: normals down to 2^-6, subnormals in
+    steps of 2^-9, largest finite 448), nearest even, saturating.  mbits widens
+    the mantissa (same exponent range) for the pinning test."""
+    x = np.asarray(x, dtype=np.float64)
+    a = np.minimum(np.abs(x), E4M3_MAX if mbits == 3 else np.inf)
+    _, E = np.frexp(a)                    # a = f 2^E, f in [0.5, 1)
+    e = np.maximum(E - 1, -6)             # binade; below 2^-6 the subnormal spacing
+    step = np.ldexp(1.0, e - mbits)
+    q = np.round(a / step) * step         # np.round: half to even
+    if mbits == 3:
+        q = np.minimum(q, E4M3_MAX)
+    return np.where(a == 0, 0.0, np.copysign(q, x))
+
+
+def pow2_exponent(m):
+    """k = ceil(log2 m) - 7 from the FP32 bits of m (so m 2^-k is in (64, 128]),
+    0 for m <= 0, clamped to [-100, 100]."""
+    m = np.asarray(m, dtype=np.float32)
+    u = m.view(np.uint32).astype(np.int64)
+    e = ((u >> 23) & 0xFF) - 127 + ((u & 0x7FFFFF) != 0)
+    k = np.clip(e - 7, -100, 100)
+    return np.where(m > 0, k, 0)
+
+
+def forward8(params, pos, nrm, wo, box, mbits=3):
+    """Raw outputs (n, 40) of the FP8 inference variant: e4m3 weights with a
+    power-of-two scale per layer (from the layer's largest |weight|), e4m3
+    inputs with a power-of-two scale per layer and record (from the record's
+    largest input to that layer), exact accumulation, FP32 biases."""
+    layers = unpack(np.asarray(params, dtype=np.float32).astype(np.float64))
+    x = features(encode(pos, nrm, wo, box)).astype(np.float32).astype(np.float64)
+    for l, (w, b) in enumerate(layers):
+        kw = int(pow2_exponent(np.float32(np.abs(w).max())))
+        w8 = e4m3(np.ldexp(w, -kw), mbits)
+        kx = pow2_exponent(x.astype(np.float32).max(axis=1))       # per record
+        x8 = e4m3(np.ldexp(x, -kx[:, None]), mbits)
+        y = (x8 @ w8) * np.ldexp(1.0, kw + kx)[:, None] + b
+        if l < N_LAYERS - 1:
+            x = np.maximum(y, 0.0).astype(np.float32).astype(np.float64)
+        else:
+            return y
+

@@ -13,7 +13,22 @@ Bars, written per test:
   - guided render: unbiased for any network -- the image mean within 4 sigma
     of the plain GPU render, which matches the oracle elsewhere;
   - gain: after training on the gap-lit room, guided relMSE at equal spp
-    below the plain renderer's."""
+    below the plain renderer's;
+  - FP8 MFMA: v_mfma_f32_32x32x16_fp8_fp8 does not sum its exact e4m3
+    products exactly: measured through a pass-through network, its error stays
+    within 2^-13 of the sum of the products' magnitudes (about 14 significant
+    bits; typical 3e-6);
+  - FP8 forward, exact: on a signed generalised permutation network (one
+    weight per neuron, so every MFMA sum is one exact product) with biases,
+    the GPU equals the oracle bit for bit through all five layers -- per-layer
+    weight scales, per-record activation scales, ReLU, operand layout;
+  - FP8 forward, dense: with the accumulator inexact, hidden values near an
+    e4m3 rounding midpoint may round the other way; 95% of records within
+    1e-4 of the oracle and the relative L2 error under a fifth of the FP8
+    quantisation error itself (FP8 oracle vs FP64);
+  - FP8 render: unbiased for any network (as above); after BF16 training in a
+    scattering medium, FP8 inference's relMSE within 1.3x of BF16 inference's
+    at equal spp."""
 import numpy as np
 import pytest
 
@@ -73,6 +88,93 @@ def test_forward_matches_oracle(guide_scene):
     assert err < 2e-3 and np.abs(out - ref).max() < 3e-2
     # the GPU is closer to the BF16 emulation than to FP64 (it really rounds there)
     assert err < err64
+
+
+FP8_ACC_REL = 2.0 ** -13
+
+
+def _identity_layers(last):
+    layers = []
+    for l, (k, m) in enumerate(G.SHAPES):
+        w = np.zeros((k, m))
+        if l < G.N_LAYERS - 1:
+            w[np.arange(min(k, m)), np.arange(min(k, m))] = 1.0
+        else:
+            w[:32] = last
+        layers.append((w, np.zeros(m)))
+    return G.pack(layers).astype(np.float32)
+
+
+def test_fp8_mfma_accumulation_error_bound(guide_scene):
+    """Layers 0-3 pass the e4m3 features through unchanged (exactly: identity
+    weights are 128 I in e4m3), so the last layer's FP32 output, divided by its
+    power-of-two scales, is the MFMA's raw sum of known exact products."""
+    sc, gs = guide_scene
+    print("fp8 mfma dot-product error: 2000 records x 40 outputs", flush=True)
+    g = wr.Guide(gs)
+    rng = np.random.default_rng(15)
+    box = scene_box(sc)
+    pos, nrm, wo, _, _ = records(rng, 2000, box)
+    f = G.features(G.encode(pos, nrm, wo, box)).astype(np.float32).astype(np.float64)
+    kx = G.pow2_exponent(f.astype(np.float32).max(axis=1))
+    x8 = G.e4m3(np.ldexp(f, -kx[:, None]))
+    worst = 0.0
+    for w in (rng.uniform(-1, 1, (32, 40)), rng.uniform(0, 1, (32, 40))):
+        p = _identity_layers(w)
+        g.set_params(p)
+        out = g.forward(pos, nrm, wo, fp8=True).astype(np.float64)
+        np.testing.assert_array_equal(G.forward8(_identity_layers(np.eye(32, 40)), pos, nrm, wo, box)[:, :32],
+                                      G.e4m3(np.ldexp(f, -kx[:, None])) * np.ldexp(1.0, kx)[:, None])
+        kw = int(G.pow2_exponent(np.float32(np.abs(w).max())))
+        w8 = G.e4m3(np.ldexp(w, -kw))
+        acc = out / np.ldexp(1.0, kw + kx)[:, None]
+        rel = np.abs(acc - x8 @ w8) / np.maximum(np.abs(x8) @ np.abs(w8), 1e-30)
+        print(f"  |sum error| / sum |products|: max {rel.max():.2e} median {np.median(rel):.2e}")
+        worst = max(worst, rel.max())
+    assert worst <= FP8_ACC_REL
+
+
+def test_fp8_forward_exact_on_permutation_network(guide_scene):
+    sc, gs = guide_scene
+    print("guide fp8 forward, signed permutation network: 1000 records", flush=True)
+    g = wr.Guide(gs)
+    rng = np.random.default_rng(16)
+    vals = np.array([0.5, 0.75, 1.0, 1.25, 1.5])
+    layers = []
+    for k, n in G.SHAPES:
+        w = np.zeros((k, n))
+        w[rng.integers(0, k, n), np.arange(n)] = rng.choice(vals, n) * rng.choice([-1.0, 1.0, 1.0, 1.0], n)
+        layers.append((w, rng.choice([0.0, 0.125, -0.25, 0.5], n)))
+    p = G.pack(layers).astype(np.float32)
+    g.set_params(p)
+    box = scene_box(sc)
+    pos, nrm, wo, _, _ = records(rng, 1000, box)
+    out = g.forward(pos, nrm, wo, fp8=True)
+    ref = G.forward8(p, pos, nrm, wo, box)
+    print(f"  max |gpu - oracle| {np.abs(out - ref).max():.2e}, outputs nonzero {(ref != 0).mean():.2f}")
+    assert (ref != 0).mean() > 0.3
+    np.testing.assert_array_equal(out, ref.astype(np.float32))
+
+
+def test_fp8_forward_matches_oracle(guide_scene):
+    sc, gs = guide_scene
+    print("guide fp8 forward, dense network: 1000 records", flush=True)
+    g = wr.Guide(gs)
+    rng = np.random.default_rng(14)
+    p = params(rng)
+    g.set_params(p)
+    box = scene_box(sc)
+    pos, nrm, wo, _, _ = records(rng, 1000, box)
+    out = g.forward(pos, nrm, wo, fp8=True).astype(np.float64)
+    ref8 = G.forward8(p, pos, nrm, wo, box)
+    fp64, _ = G.forward(p, pos, nrm, wo, box)
+    err = np.linalg.norm(out - ref8) / np.linalg.norm(ref8)
+    quant = np.linalg.norm(ref8 - fp64) / np.linalg.norm(fp64)
+    rec = np.abs(out - ref8).max(axis=1)
+    print(f"  rel err vs fp8 oracle {err:.2e}; fp8 quantisation error {quant:.2e}; "
+          f"records off by > 1e-4: {(rec > 1e-4).sum()}")
+    assert (rec <= 1e-4).mean() >= 0.95
+    assert err < 0.2 * quant
 
 
 def test_gradient_matches_oracle(guide_scene):
@@ -158,22 +260,23 @@ def _render(gs, spp, N, seed, guide=None, **kw):
     return film.image(), st, gst
 
 
-def test_guided_render_is_unbiased_with_any_network(gpu_device):
+@pytest.mark.parametrize("fp8", [False, True], ids=["bf16", "fp8"])
+def test_guided_render_is_unbiased_with_any_network(gpu_device, fp8):
     """A random network mixed in with alpha = 0.5 changes the noise, not the mean."""
     sc = scenes.cornell(1, resolution=(48, 48))
     gs = wr.GpuScene(gpu_device, sc)
-    print("guided render, random network: cornell 48x48", flush=True)
+    print(f"guided render, random network ({'fp8' if fp8 else 'bf16'}): cornell 48x48", flush=True)
     g = wr.Guide(gs, seed=5)
     spp = 1024
     plain, _, _ = _render(gs, spp, 10, seed=21)
     guided, st, gst = _render(gs, spp, 10, seed=22, guide=g, guide_fraction=0.5, spp_per_pass=spp, train_steps=0,
-                              warmup_passes=0)
+                              warmup_passes=0, fp8_inference=fp8)
     assert gst.records == 0 and gst.train_steps == 0
     assert np.isfinite(guided).all() and st.extend_rays > 0
     # per-pixel noise of the two estimators from two more independent renders
     plain2, _, _ = _render(gs, spp, 10, seed=23)
     guided2, _, _ = _render(gs, spp, 10, seed=24, guide=g, guide_fraction=0.5, spp_per_pass=spp, train_steps=0,
-                            warmup_passes=0)
+                            warmup_passes=0, fp8_inference=fp8)
     npx = guided.shape[0] * guided.shape[1]
     sig = np.sqrt(((guided - guided2) ** 2).mean((0, 1)) / 2 / npx + ((plain - plain2) ** 2).mean((0, 1)) / 2 / npx)
     diff = guided.mean((0, 1)) - plain.mean((0, 1))
@@ -217,3 +320,30 @@ def test_trained_guiding_beats_plain_at_equal_spp(gpu_device):
     e_g, e_p = oracle.rel_mse(guided, ref), oracle.rel_mse(plain, ref)
     print(f"  relMSE at 256 spp: plain {e_p:.4e} guided {e_g:.4e} (gain {e_p / e_g:.2f}x)")
     assert e_g < e_p
+
+
+def test_fp8_inference_in_medium_matches_bf16(gpu_device):
+    """Config-5 shape at small size: clutter in a scattering medium (sigma_s 0.4,
+    sigma_a 0.05, HG g 0.7), network trained in BF16, then rendered with BF16
+    and with FP8 inference: both unbiased against a plain reference, FP8 noise
+    close to BF16's."""
+    sc = scenes.cornell_clutter(5, 20_000, resolution=(64, 48)).with_medium(sigma_a=0.05, sigma_s=0.4, g=0.7)
+    gs = wr.GpuScene(gpu_device, sc)
+    print("fp8 vs bf16 inference in a medium: clutter 20k 64x48", flush=True)
+    ref, _, _ = _render(gs, 8192, 34, seed=51)
+    g = wr.Guide(gs, batch_size=1 << 15, record_capacity=1 << 17)
+    _render(gs, 256, 34, seed=52, guide=g, guide_fraction=0.5, spp_per_pass=16, train_steps=8, warmup_passes=1)
+    import oracle
+    res = {}
+    for fp8, seeds in ((False, (53, 54)), (True, (55, 56))):
+        imgs = [_render(gs, 256, 34, seed=sd, guide=g, guide_fraction=0.5, spp_per_pass=256, train_steps=0,
+                        warmup_passes=0, fp8_inference=fp8)[0] for sd in seeds]
+        res[fp8] = imgs
+    e_bf, e_f8 = oracle.rel_mse(res[False][0], ref), oracle.rel_mse(res[True][0], ref)
+    print(f"  relMSE at 256 spp vs 8192-spp plain: bf16 {e_bf:.4e} fp8 {e_f8:.4e}")
+    assert e_f8 < 1.3 * e_bf
+    npx = ref.shape[0] * ref.shape[1]
+    for fp8 in (False, True):
+        a, b = res[fp8]
+        sig = np.sqrt(((a - b) ** 2).mean((0, 1)) / 2 / npx)
+        assert np.all(np.abs(a.mean((0, 1)) - ref.mean((0, 1))) < 5 * sig + 1e-6), (fp8, a.mean((0, 1)), ref.mean((0, 1)))

@@ -142,3 +142,20 @@ def test_degenerate_requests(gpu_device):
     assert not b.any() and not s.any()
     with pytest.raises(wr.WrError):
         wr.render_nee(gs, film, -1)
+
+
+def test_closing_a_scene_closes_its_films_and_guides(gpu_device):
+    """The library needs films and guides destroyed before their scene; the
+    binding closes them first, so dropping them afterwards is harmless."""
+    sc = scenes.cornell(1, resolution=(16, 16))
+    gs = wr.GpuScene(gpu_device, sc)
+    film = wr.Film(gs)
+    g = wr.Guide(gs)
+    wr.render_nee(gs, film, 2, num_path_vertices=6, seed=1)
+    gs.close()
+    assert film.h is None and g.h is None
+    del film, g
+    other = wr.GpuScene(gpu_device, sc)
+    f2 = wr.Film(other)
+    wr.render_nee(other, f2, 2, num_path_vertices=6, seed=1)
+    assert np.isfinite(f2.image()).all()

@@ -182,3 +182,49 @@ def test_bf16_forward_close_to_fp64():
     b, _ = G.forward(params, pos, nrm, wo, box, bf=True)
     rel = np.linalg.norm(a - b) / np.linalg.norm(a)
     assert 1e-5 < rel < 3e-2
+
+
+def test_e4m3_known_values():
+    q = G.e4m3
+    assert q(1.0) == 1.0 and q(-2.5) == -2.5 and q(448.0) == 448.0 and q(0.0) == 0.0
+    assert q(1.0625) == 1.0            # tie between 1 and 1.125 -> even mantissa (1.0)
+    assert q(1.1875) == 1.25           # tie between 1.125 and 1.25 -> even (1.25)
+    assert q(1.19) == 1.25 and q(1.18) == 1.125
+    assert q(500.0) == 448.0 and q(-1e6) == -448.0   # saturation
+    assert q(2.0 ** -9) == 2.0 ** -9   # smallest subnormal
+    assert q(0.75 * 2.0 ** -9) == 2.0 ** -9 and q(0.5 * 2.0 ** -9) == 0.0  # tie to even (0)
+    assert q(1.5 * 2.0 ** -9) == 2.0 * 2.0 ** -9     # tie between 1 and 2 steps -> 2
+    assert q(8.5 * 2.0 ** -9) == 2.0 ** -6           # tie at the normal boundary -> 8 steps
+    assert q(240.0) == 240.0 and q(232.0) == 224.0   # 232: tie between 224 and 240 -> 224 (even)
+    # codes 0x01..0x7E (0x7F is NaN): 7 subnormals, 14 binades of 8, and 7 in [256, 448]
+    grid = np.exp(np.linspace(np.log(2.0 ** -10), np.log(460.0), 400_000))
+    vals = np.unique(q(grid))
+    assert len(vals[vals > 0]) == 126 and vals.max() == 448.0
+
+
+def test_pow2_exponent_range():
+    assert G.pow2_exponent(1.0) == -7 and G.pow2_exponent(1.5) == -6 and G.pow2_exponent(128.0) == 0
+    assert G.pow2_exponent(0.0) == 0 and G.pow2_exponent(129.0) == 1
+    m = np.exp(np.random.default_rng(6).uniform(-40, 40, 10000)).astype(np.float32)
+    k = G.pow2_exponent(m)
+    s = m.astype(np.float64) * 2.0 ** -k.astype(np.float64)
+    assert np.all((s > 64) & (s <= 128))
+
+
+def test_fp8_forward_reduces_to_fp64_and_is_a_bounded_perturbation():
+    rng = np.random.default_rng(7)
+    box = (-1, -1, -1, 1, 1, 1)
+    pos, nrm, wo, _, _ = random_records(rng, 200, box)
+    params = random_params(rng).astype(np.float32)
+    ref, _ = G.forward(params.astype(np.float64), pos, nrm, wo, box)
+    wide = G.forward8(params, pos, nrm, wo, box, mbits=23)
+    # 23-bit mantissa: only the FP32 rounding of features / hidden values remains
+    assert np.abs(wide - ref).max() <= 1e-5 * max(1.0, np.abs(ref).max())
+    out8 = G.forward8(params, pos, nrm, wo, box)
+    bf, _ = G.forward(params.astype(np.float64), pos, nrm, wo, box, bf=True)
+    e8 = np.linalg.norm(out8 - ref) / np.linalg.norm(ref)
+    ebf = np.linalg.norm(bf - ref) / np.linalg.norm(ref)
+    # 3 mantissa bits against BF16's 8: about 2^5 times the error, and still small
+    assert 4 * ebf < e8 < 0.15, (e8, ebf)
+    # a record's outputs do not depend on the other records
+    np.testing.assert_array_equal(G.forward8(params, pos[:1], nrm[:1], wo[:1], box), out8[:1])

@@ -82,7 +82,7 @@ class GuideConfig(C.Structure):
 
 class GuidedParams(C.Structure):
     _fields_ = [("guide_fraction", C.c_float), ("spp_per_pass", C.c_uint32), ("train_steps", C.c_uint32),
-                ("warmup_passes", C.c_uint32)]
+                ("warmup_passes", C.c_uint32), ("fp8_inference", C.c_int32)]
 
 
 class GuideStats(C.Structure):
@@ -147,6 +147,7 @@ EXPORTS = {
     "wr_guide_get_params": (C.c_int, [P, P]),
     "wr_guide_set_params": (C.c_int, [P, P]),
     "wr_guide_forward": (C.c_int, [P, P, P, P, C.c_uint32, P]),
+    "wr_guide_forward_fp8": (C.c_int, [P, P, P, P, C.c_uint32, P]),
     "wr_guide_gradient": (C.c_int, [P, P, P, P, P, P, C.c_uint32, P, C.POINTER(C.c_double)]),
     "wr_guide_train": (C.c_int, [P, P, P, P, P, P, C.c_uint32, C.c_uint32, C.POINTER(C.c_double)]),
     "wr_guide_mixture": (C.c_int, [P, P, P, P, C.c_uint32, P, P, P]),
@@ -273,8 +274,13 @@ class GpuScene:
         _check(lib().wr_scene_set_medium(self.h, C.byref(m)))
 
     def close(self):
-        """Releases the handle (idempotent)."""
+        """Releases the handle (idempotent), after the guides and films of this
+        scene, which the library requires to go first."""
         if getattr(self, "h", None):
+            for kind in ("guide", "film"):
+                for obj in list(_LIVE[kind]):
+                    if getattr(obj, "scene", None) is self:
+                        obj.close()
             lib().wr_scene_destroy(self.h)
             self.h = None
 
@@ -426,10 +432,12 @@ class Guide:
             raise ValueError("wrong parameter count")
         _check(lib().wr_guide_set_params(self.h, _ptr(p)))
 
-    def forward(self, pos, nrm, wo) -> np.ndarray:
+    def forward(self, pos, nrm, wo, fp8: bool = False) -> np.ndarray:
+        """Raw outputs (n, 40); fp8 selects the e4m3 inference variant."""
         pos, nrm, wo = _f32(pos, 3), _f32(nrm, 3), _f32(wo, 3)
         out = np.empty((len(pos), GUIDE_NUM_OUTPUTS), dtype=np.float32)
-        _check(lib().wr_guide_forward(self.h, _ptr(pos), _ptr(nrm), _ptr(wo), len(pos), _ptr(out)))
+        fn = lib().wr_guide_forward_fp8 if fp8 else lib().wr_guide_forward
+        _check(fn(self.h, _ptr(pos), _ptr(nrm), _ptr(wo), len(pos), _ptr(out)))
         return out
 
     def gradient(self, pos, nrm, wo, wi, weight):
@@ -462,13 +470,14 @@ class Guide:
 def render_guided(scene: GpuScene, film: Film, guide: Guide, spp: int, num_path_vertices: int = 10, seed: int = 0,
                   guide_fraction: float = 0.5, spp_per_pass: int = 4, train_steps: int = 4, warmup_passes: int = 1,
                   use_mis: bool = True, first_sample: int | None = None, max_paths_in_flight: int = 0,
-                  profile: bool = False):
-    """wr_render_guided: progressive passes of spp_per_pass, training between passes."""
+                  profile: bool = False, fp8_inference: bool = False):
+    """wr_render_guided: progressive passes of spp_per_pass, training between passes;
+    fp8_inference runs the renderer's network inference in e4m3."""
     if first_sample is None:
         first_sample = film.spp
     p = RenderParams(spp, num_path_vertices, seed, int(bool(use_mis)), first_sample, max_paths_in_flight,
                      int(bool(profile)), 0)
-    gp = GuidedParams(guide_fraction, spp_per_pass, train_steps, warmup_passes)
+    gp = GuidedParams(guide_fraction, spp_per_pass, train_steps, warmup_passes, int(bool(fp8_inference)))
     st, gs = RenderStats(), GuideStats()
     _check(lib().wr_render_guided(scene.h, film.h, guide.h, C.byref(p), C.byref(gp), C.byref(st), C.byref(gs)))
     film.spp += spp

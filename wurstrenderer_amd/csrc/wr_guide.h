@@ -14,6 +14,13 @@
 // [kLambdaMin, kLambdaMax].  Weights and activations are BF16, products
 // accumulate in FP32 (v_mfma_f32_32x32x16_bf16), master weights are FP32.
 //
+// FP8 inference variant: weights and activations in e4m3 with FP32
+// accumulation (v_mfma_f32_32x32x16_fp8_fp8), per-layer power-of-two weight
+// scales and, per layer, a power-of-two activation scale for each record (from
+// that record's own largest input), so a record's outputs never depend on the
+// other records of its wave.  Scaled values stay within (64, 128] at the
+// maximum, inside the normal range of e4m3.
+//
 // Register-resident layout.  The network runs transposed, Y^T = W^T X^T, so a
 // wave's 64 records are the MFMA columns.  A layer's output D tile (lane
 // (p, h) = (l % 32, l / 32) holds rows drow(v, h) = 8 (v / 4) + 4 h + v % 4 of
@@ -67,6 +74,9 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 // An MFMA operand: 8 BF16 values packed two per 32-bit word (element e in word
 // e / 2, low half first), so each operand is exactly 4 registers.
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+// An FP8 operand: the same 8 elements as a BF16 operand, one e4m3 byte each
+// (element e in byte e), 2 registers.
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ f32x16 mfma(u32x4 a, u32x4 b, f32x16 c) {
     return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
@@ -92,6 +102,8 @@ struct Net {
     const u32x4 *fwd;     // kFwdOps packed forward operands
     const u32x4 *bwd;     // kBwdOps packed backward operands
     const float *bias;    // kBiasPacked FP32 biases, layer l at 64 l
+    const u32x2 *fwd8;    // kFwdOps packed e4m3 forward operands (weights / wscale[l])
+    const float *wscale;  // kLayers power-of-two weight scales
     float box[6];         // position encoding: scene bounds lo xyz, then 1 / extent xyz
 };
 
@@ -346,6 +358,163 @@ __device__ __forceinline__ void forward(const Net &net, const Enc &mine, float *
             }
         }
     }
+    wave_sync_lds();
+}
+
+// ---------------------------------------------------------------- FP8 forward pass
+__device__ __forceinline__ f32x16 mfma8(u32x2 a, u32x2 b, f32x16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_fp8_fp8(__builtin_bit_cast(long, a), __builtin_bit_cast(long, b), c, 0, 0,
+                                                      0);
+}
+// Four FP32 values (|x| <= 128) -> four e4m3 bytes, round to nearest even.
+__device__ __forceinline__ uint32_t pack4_fp8(float a, float b, float c, float d) {
+    return uint32_t(__builtin_amdgcn_cvt_pk_fp8_f32(c, d, __builtin_amdgcn_cvt_pk_fp8_f32(a, b, 0, false), true));
+}
+// Power-of-two scale of a set with largest magnitude m: k = ceil(log2 m) - 7
+// (from the FP32 bits, exactly), so m 2^-k lies in (64, 128]; k = 0 for m = 0.
+// Returns 2^k; the quantiser multiplies by 2^-k (exact).
+__device__ __forceinline__ int pow2_exponent(float m) {
+    const uint32_t u = __float_as_uint(m);
+    if (!(m > 0.f)) return 0;
+    const int e = int(u >> 23) - 127 + ((u & 0x7FFFFFu) ? 1 : 0);
+    const int k = e - 7;
+    return k < -100 ? -100 : (k > 100 ? 100 : k);
+}
+__device__ __forceinline__ float pow2f(int k) { return __uint_as_float(uint32_t(127 + k) << 23); }
+
+// Quantises 8 values (the elements of one operand) scaled by inv.
+__device__ __forceinline__ u32x2 quant8(const float *v, float inv) {
+    u32x2 r;
+    r[0] = pack4_fp8(v[0] * inv, v[1] * inv, v[2] * inv, v[3] * inv);
+    r[1] = pack4_fp8(v[4] * inv, v[5] * inv, v[6] * inv, v[7] * inv);
+    return r;
+}
+
+// One layer on e4m3 operands: acc[mt][nt] = W8 X8 over K inputs, both output tiles.
+template <int K>
+__device__ __forceinline__ void layer8(const u32x2 *ops, const u32x2 (*x)[2], f32x16 acc[2][2]) {
+    const int lane = __lane_id();
+#pragma unroll
+    for (int mt = 0; mt < 2; mt++) {
+#pragma unroll
+        for (int nt = 0; nt < 2; nt++)
+#pragma unroll
+            for (int v = 0; v < 16; v++) acc[mt][nt][v] = 0.f;
+#pragma unroll
+        for (int ks = 0; ks < K / 16; ks++) {
+            const u32x2 a = ops[(mt * (K / 16) + ks) * 64 + lane];
+            acc[mt][0] = mfma8(a, x[ks][0], acc[mt][0]);
+            acc[mt][1] = mfma8(a, x[ks][1], acc[mt][1]);
+        }
+    }
+}
+
+// Dequantised pre-activation of output tile mt: acc * (wscale * xscale[nt]) + bias.
+// Hidden layers then take relu, find each record's largest input to the next
+// layer (its 64 values sit in the lane's two tiles and in lane p ^ 32) and
+// quantise with that record's scale.
+__device__ __forceinline__ void hidden_to_operands8(f32x16 acc[2][2], const float *bias, float ws, const float xs[2],
+                                                    u32x2 xn[4][2], float xsn[2]) {
+    const int lane = __lane_id(), h = lane >> 5;
+    const float *bb = bias + 4 * h;
+#pragma unroll
+    for (int nt = 0; nt < 2; nt++) {
+        const float sc = ws * xs[nt];
+        float m = 0.f;
+#pragma unroll
+        for (int mt = 0; mt < 2; mt++)
+#pragma unroll
+            for (int v = 0; v < 16; v++) {
+                const float y = fmaxf(fmaf(acc[mt][nt][v], sc, bb[32 * mt + drow(v, 0)]), 0.f);
+                acc[mt][nt][v] = y;
+                m = fmaxf(m, y);
+            }
+        m = fmaxf(m, __shfl_xor(m, 32));
+        const int k = pow2_exponent(m);
+        const float inv = pow2f(-k);
+        xsn[nt] = pow2f(k);
+#pragma unroll
+        for (int mt = 0; mt < 2; mt++)
+#pragma unroll
+            for (int kk = 0; kk < 2; kk++) {
+                float v8[8];
+#pragma unroll
+                for (int e = 0; e < 8; e++) v8[e] = acc[mt][nt][8 * kk + e];
+                xn[2 * mt + kk][nt] = quant8(v8, inv);
+            }
+    }
+}
+
+// forward() on e4m3 weights and activations: same inputs, outputs and staging.
+__device__ __forceinline__ void forward8(const Net &net, const Enc &mine, float *stage) {
+    const int lane = __lane_id(), p = lane & 31, h = lane >> 5;
+    u32x2 x[4][2];
+    float xs[2];
+    {
+        // layer-0 inputs: the one-blob features of records p and 32 + p
+#pragma unroll
+        for (int nt = 0; nt < 2; nt++) {
+            Enc e;
+#pragma unroll
+            for (int j = 0; j < 7; j++) e.v[j] = __shfl(mine.v[j], 32 * nt + p);
+            float f[16];
+            float m = 0.f;
+#pragma unroll
+            for (int ks = 0; ks < 2; ks++)
+#pragma unroll
+                for (int i = 0; i < 8; i++) {
+                    f[8 * ks + i] = feature(e, kperm(ks, h, i));
+                    m = fmaxf(m, f[8 * ks + i]);
+                }
+            m = fmaxf(m, __shfl_xor(m, 32));
+            const int k = pow2_exponent(m);
+            xs[nt] = pow2f(k);
+            const float inv = pow2f(-k);
+            x[0][nt] = quant8(f, inv);
+            x[1][nt] = quant8(f + 8, inv);
+        }
+        f32x16 acc[2][2];
+        layer8<kIn>(net.fwd8 + fwd_offset(0), x, acc);
+        float xsn[2];
+        u32x2 xn[4][2];
+        hidden_to_operands8(acc, net.bias, net.wscale[0], xs, xn, xsn);
+#pragma unroll
+        for (int ks = 0; ks < 4; ks++)
+#pragma unroll
+            for (int nt = 0; nt < 2; nt++) x[ks][nt] = xn[ks][nt];
+        xs[0] = xsn[0];
+        xs[1] = xsn[1];
+    }
+#pragma unroll 1
+    for (int l = 1; l < kLayers - 1; l++) {
+        f32x16 acc[2][2];
+        layer8<kHid>(net.fwd8 + fwd_offset(l), x, acc);
+        float xsn[2];
+        u32x2 xn[4][2];
+        hidden_to_operands8(acc, net.bias + 64 * l, net.wscale[l], xs, xn, xsn);
+#pragma unroll
+        for (int ks = 0; ks < 4; ks++)
+#pragma unroll
+            for (int nt = 0; nt < 2; nt++) x[ks][nt] = xn[ks][nt];
+        xs[0] = xsn[0];
+        xs[1] = xsn[1];
+    }
+    f32x16 acc[2][2];
+    layer8<kHid>(net.fwd8 + fwd_offset(kLayers - 1), x, acc);
+    const float ws = net.wscale[kLayers - 1];
+    const float *bb = net.bias + 64 * (kLayers - 1) + 4 * h;
+    float *sb = stage + p * kStageStride + 4 * h;
+#pragma unroll
+    for (int mt = 0; mt < 2; mt++)
+#pragma unroll
+        for (int v = 0; v < 16; v++) {
+            const int o0 = 32 * mt + drow(v, 0);
+            if (o0 + 4 * h < kOut) {
+#pragma unroll
+                for (int nt = 0; nt < 2; nt++)
+                    sb[32 * nt * kStageStride + o0] = fmaf(acc[mt][nt][v], ws * xs[nt], bb[o0]);
+            }
+        }
     wave_sync_lds();
 }
 

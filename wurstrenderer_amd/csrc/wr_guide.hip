@@ -61,6 +61,36 @@ __global__ void k_pack(const float *params, u32x4 *fwd, u32x4 *bwd, float *bias)
     }
 }
 
+// FP32 master weights -> e4m3 forward operands, one block per layer: the
+// layer's largest |weight| sets its power-of-two scale (wscale[l] = 2^k, the
+// operands hold W / 2^k, at most 128 in magnitude).
+constexpr int kPack8Block = 256;
+__global__ __launch_bounds__(kPack8Block) void k_pack8(const float *params, u32x2 *fwd8, float *wscale) {
+    __shared__ float red[kPack8Block];
+    const int l = blockIdx.x;
+    const int K = layer_in(l), N = layer_out(l), nks = K / 16;
+    const float *W = params + w_offset(l);
+    float m = 0.f;
+    for (int j = threadIdx.x; j < K * N; j += kPack8Block) m = fmaxf(m, fabsf(W[j]));
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = kPack8Block / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    const int k = pow2_exponent(red[0]);
+    const float inv = pow2f(-k);
+    if (threadIdx.x == 0) wscale[l] = pow2f(k);
+    const int n_ops = fwd_offset(l + 1) - fwd_offset(l);
+    for (int jj = threadIdx.x; jj < n_ops; jj += kPack8Block) {
+        const int mt = jj / (nks * 64), ks = (jj / 64) % nks, lane = jj % 64;
+        const int o = 32 * mt + (lane & 31), h = lane >> 5;
+        float v[8];
+        for (int e = 0; e < 8; e++) v[e] = o < N ? W[kperm(ks, h, e) * N + o] : 0.f;
+        fwd8[fwd_offset(l) + jj] = quant8(v, inv);
+    }
+}
+
 // ---------------------------------------------------------------- train
 struct TrainArgs {
     const float4 *pos, *nrm, *wo, *wi;  // records; wi.w = weight
@@ -358,6 +388,7 @@ __global__ void k_finalize(Records rec, uint32_t n, const float4 *sample_rad, do
 }
 
 // ---------------------------------------------------------------- tests
+template <bool FP8>
 __global__ __launch_bounds__(kTestBlock) void k_forward_test(Net net, const float *pos, const float *nrm, const float *wo,
                                                              uint32_t n, float *out) {
     __shared__ float stage[64 * kStageStride];
@@ -368,7 +399,8 @@ __global__ __launch_bounds__(kTestBlock) void k_forward_test(Net net, const floa
                    f3(wo[3 * r], wo[3 * r + 1], wo[3 * r + 2]), net.box);
     else
         for (int k = 0; k < 7; k++) e.v[k] = 0.f;
-    forward(net, e, stage);
+    if (FP8) forward8(net, e, stage);
+    else forward(net, e, stage);
     if (r < n)
         for (int k = 0; k < kOut; k++) out[size_t(r) * kOut + k] = stage[threadIdx.x * kStageStride + k];
 }
@@ -399,6 +431,8 @@ struct wr_guide {
     float *d_params = nullptr, *d_m = nullptr, *d_v = nullptr;
     u32x4 *d_fwd = nullptr, *d_bwd = nullptr;
     float *d_bias = nullptr;  // packed biases (kBiasPacked)
+    u32x2 *d_fwd8 = nullptr;  // e4m3 forward operands (inference variant)
+    float *d_wscale = nullptr;  // their per-layer scales
     float *d_partials = nullptr, *d_loss = nullptr;
     double *d_wsum = nullptr;
     Records rec{};
@@ -452,6 +486,8 @@ Net net_view(const wr_guide *g) {
     n.fwd = g->d_fwd;
     n.bwd = g->d_bwd;
     n.bias = g->d_bias;
+    n.fwd8 = g->d_fwd8;
+    n.wscale = g->d_wscale;
     for (int k = 0; k < 3; k++) {
         n.box[k] = g->scene->box[k];
         n.box[3 + k] = float(1.0 / std::max(double(g->scene->box[3 + k]) - double(g->scene->box[k]), 1e-20));
@@ -462,6 +498,7 @@ Net net_view(const wr_guide *g) {
 int pack(wr_guide *g) {
     const int n = kFwdOps + kBwdOps;
     k_pack<<<(n + 255) / 256, 256, 0, g->scene->stream>>>(g->d_params, g->d_fwd, g->d_bwd, g->d_bias);
+    k_pack8<<<kLayers, kPack8Block, 0, g->scene->stream>>>(g->d_params, g->d_fwd8, g->d_wscale);
     WR_HIP(hipGetLastError());
     return WR_OK;
 }
@@ -642,7 +679,8 @@ void wr_guide_destroy(wr_guide *g) {
     (void)hipSetDevice(g->scene->dev->ordinal);
     (void)hipStreamSynchronize(g->scene->stream);
     for (void *p : {(void *)g->d_params, (void *)g->d_m, (void *)g->d_v, (void *)g->d_fwd, (void *)g->d_bwd, (void *)g->d_bias,
-                    (void *)g->d_partials, (void *)g->d_loss, (void *)g->d_wsum, g->rec_pool, (void *)g->d_draw_dir})
+                    (void *)g->d_fwd8, (void *)g->d_wscale, (void *)g->d_partials, (void *)g->d_loss, (void *)g->d_wsum,
+                    g->rec_pool, (void *)g->d_draw_dir})
         if (p) (void)hipFree(p);
     delete g;
 }
@@ -658,6 +696,8 @@ static int guide_create_impl(wr_guide *g) {
     WR_HIP(hipMalloc(reinterpret_cast<void **>(&g->d_fwd), sizeof(u32x4) * kFwdOps));
     WR_HIP(hipMalloc(reinterpret_cast<void **>(&g->d_bwd), sizeof(u32x4) * kBwdOps));
     WR_HIP(hipMalloc(reinterpret_cast<void **>(&g->d_bias), sizeof(float) * kBiasPacked));
+    WR_HIP(hipMalloc(reinterpret_cast<void **>(&g->d_fwd8), sizeof(u32x2) * kFwdOps));
+    WR_HIP(hipMalloc(reinterpret_cast<void **>(&g->d_wscale), sizeof(float) * kLayers));
     WR_HIP(hipMalloc(reinterpret_cast<void **>(&g->d_partials), sizeof(float) * kNumParams * size_t(g->train_blocks)));
     WR_HIP(hipMalloc(reinterpret_cast<void **>(&g->d_loss), sizeof(float) * g->train_blocks));
     WR_HIP(hipMalloc(reinterpret_cast<void **>(&g->d_wsum), sizeof(double)));
@@ -731,7 +771,8 @@ static int check_records(const float *pos, const float *nrm, const float *wo, ui
     return WR_OK;
 }
 
-int wr_guide_forward(wr_guide *g, const float *pos, const float *nrm, const float *wo, uint32_t n, float *out) {
+static int forward_impl(wr_guide *g, const float *pos, const float *nrm, const float *wo, uint32_t n, float *out,
+                        bool fp8) {
     if (!g || (n && !out)) return set_error(WR_ERR_INVALID_ARGUMENT, "null argument");
     int rc = check_records(pos, nrm, wo, n);
     if (!rc) rc = set_device(g->scene->dev);
@@ -745,7 +786,8 @@ int wr_guide_forward(wr_guide *g, const float *pos, const float *nrm, const floa
     if (e == hipSuccess) e = hipMemcpyAsync(dn, nrm, in_bytes / 3, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemcpyAsync(dw, wo, in_bytes / 3, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) {
-        k_forward_test<<<(n + 63) / 64, kTestBlock, 0, st>>>(net_view(g), dp, dn, dw, n, dout);
+        if (fp8) k_forward_test<true><<<(n + 63) / 64, kTestBlock, 0, st>>>(net_view(g), dp, dn, dw, n, dout);
+        else k_forward_test<false><<<(n + 63) / 64, kTestBlock, 0, st>>>(net_view(g), dp, dn, dw, n, dout);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpyAsync(out, dout, out_bytes, hipMemcpyDeviceToHost, st);
@@ -753,6 +795,14 @@ int wr_guide_forward(wr_guide *g, const float *pos, const float *nrm, const floa
     (void)hipFree(d);
     WR_HIP(e);
     return WR_OK;
+}
+
+int wr_guide_forward(wr_guide *g, const float *pos, const float *nrm, const float *wo, uint32_t n, float *out) {
+    return forward_impl(g, pos, nrm, wo, n, out, false);
+}
+
+int wr_guide_forward_fp8(wr_guide *g, const float *pos, const float *nrm, const float *wo, uint32_t n, float *out) {
+    return forward_impl(g, pos, nrm, wo, n, out, true);
 }
 
 int wr_guide_gradient(wr_guide *g, const float *pos, const float *nrm, const float *wo, const float *wi,

@@ -30,6 +30,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/wurst_amd.h"
@@ -529,20 +530,32 @@ __device__ __forceinline__ float guide_alpha(const DMat &m, float frac) {
 #define WR_INFER_WAVES 2
 #endif
 constexpr int kInferBlock = 256;  // 4 waves share one LDS copy of the weights
-template <bool MEDIUM>
+// FP8: the e4m3 variant of the network (guide::forward8).
+template <bool MEDIUM, bool FP8>
 __global__ __launch_bounds__(kInferBlock) __attribute__((amdgpu_waves_per_eu(WR_INFER_WAVES, WR_INFER_WAVES))) void k_guide_infer(
     DScene s, PathState cs, const uint2 *__restrict__ hit, const uint32_t *__restrict__ counters, int N, guide::Net gnet,
     float frac, guide::Draw out) {
-    __shared__ guide::u32x4 w_ops[guide::kFwdOps];
+    using OpT = typename std::conditional<FP8, guide::u32x2, guide::u32x4>::type;
+    __shared__ OpT w_ops[guide::kFwdOps];
     __shared__ float w_bias[guide::kBiasPacked];
+    __shared__ float w_scale[guide::kLayers];
     __shared__ float stage_all[kInferBlock * guide::kStageStride];
     const uint32_t n = active_count(counters);
     if (blockIdx.x * kInferBlock >= n) return;
-    for (int j = threadIdx.x; j < guide::kFwdOps; j += kInferBlock) w_ops[j] = gnet.fwd[j];
+    const OpT *src;
+    if constexpr (FP8) src = gnet.fwd8;
+    else src = gnet.fwd;
+    for (int j = threadIdx.x; j < guide::kFwdOps; j += kInferBlock) w_ops[j] = src[j];
     for (int j = threadIdx.x; j < guide::kBiasPacked; j += kInferBlock) w_bias[j] = gnet.bias[j];
+    if (FP8 && threadIdx.x < guide::kLayers) w_scale[threadIdx.x] = gnet.wscale[threadIdx.x];
     __syncthreads();
     guide::Net net = gnet;
-    net.fwd = w_ops;
+    if constexpr (FP8) {
+        net.fwd8 = w_ops;
+        net.wscale = w_scale;
+    } else {
+        net.fwd = w_ops;
+    }
     net.bias = w_bias;
     float *stage = stage_all + (threadIdx.x & ~63u) * guide::kStageStride;
     const uint32_t lane = threadIdx.x & 63u;
@@ -580,7 +593,8 @@ __global__ __launch_bounds__(kInferBlock) __attribute__((amdgpu_waves_per_eu(WR_
             }
         }
         if (__ballot(need)) {
-            guide::forward(net, guide::encode(pos, nf, -rd, net.box), stage);
+            if constexpr (FP8) guide::forward8(net, guide::encode(pos, nf, -rd, net.box), stage);
+            else guide::forward(net, guide::encode(pos, nf, -rd, net.box), stage);
             if (need) {
                 float *L = stage + lane * guide::kStageStride;
                 float wl[guide::kLobes];
@@ -1626,8 +1640,11 @@ static int render_impl(wr_scene *s, wr_film *f, const wr_render_params *prm, wr_
     const int g_shade = std::max(1, std::min<int>((P + kShadeBlock - 1) / kShadeBlock,
                                                   s->dev->n_cu * occupancy(s->dev, shade_fn, kShadeBlock)));
     int pass = 0;
-    const void *infer_fn = medium ? reinterpret_cast<const void *>(&k_guide_infer<true>)
-                                  : reinterpret_cast<const void *>(&k_guide_infer<false>);
+    const bool fp8 = g && gp->fp8_inference != 0;
+    const void *infer_fn = medium ? (fp8 ? reinterpret_cast<const void *>(&k_guide_infer<true, true>)
+                                         : reinterpret_cast<const void *>(&k_guide_infer<true, false>))
+                                  : (fp8 ? reinterpret_cast<const void *>(&k_guide_infer<false, true>)
+                                         : reinterpret_cast<const void *>(&k_guide_infer<false, false>));
     const int g_infer = std::max(1, std::min<int>((P + kInferBlock - 1) / kInferBlock,
                                                   s->dev->n_cu * occupancy(s->dev, infer_fn, kInferBlock)));
     const guide::Net net = g ? guide_net(g) : guide::Net{};
@@ -1672,12 +1689,18 @@ static int render_impl(wr_scene *s, wr_film *f, const wr_render_params *prm, wr_
             end();
             begin(2);
             if (g && gs.alpha > 0.f) {
-                if (medium)
-                    k_guide_infer<true><<<g_infer, kInferBlock, 0, s->stream>>>(s->ds, q.cur, q.hit, q.counters, N, net,
-                                                                                gs.alpha, gs.draw);
+                if (medium && fp8)
+                    k_guide_infer<true, true><<<g_infer, kInferBlock, 0, s->stream>>>(s->ds, q.cur, q.hit, q.counters, N,
+                                                                                      net, gs.alpha, gs.draw);
+                else if (medium)
+                    k_guide_infer<true, false><<<g_infer, kInferBlock, 0, s->stream>>>(s->ds, q.cur, q.hit, q.counters, N,
+                                                                                       net, gs.alpha, gs.draw);
+                else if (fp8)
+                    k_guide_infer<false, true><<<g_infer, kInferBlock, 0, s->stream>>>(s->ds, q.cur, q.hit, q.counters, N,
+                                                                                       net, gs.alpha, gs.draw);
                 else
-                    k_guide_infer<false><<<g_infer, kInferBlock, 0, s->stream>>>(s->ds, q.cur, q.hit, q.counters, N, net,
-                                                                                 gs.alpha, gs.draw);
+                    k_guide_infer<false, false><<<g_infer, kInferBlock, 0, s->stream>>>(s->ds, q.cur, q.hit, q.counters, N,
+                                                                                        net, gs.alpha, gs.draw);
             }
             if (g && medium) k_shade_ext<true, true><<<g_shade, kShadeBlock, 0, s->stream>>>(s->ds, q, a, gs);
             else if (g) k_shade_ext<true, false><<<g_shade, kShadeBlock, 0, s->stream>>>(s->ds, q, a, gs);
