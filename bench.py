@@ -47,7 +47,7 @@ def parse():
     ap.add_argument("--ttr-target", type=float, default=1e-3)
     ap.add_argument("--ttr-ref-spp", type=int, default=32768)
     ap.add_argument("--ttr-runs", type=int, default=3)
-    ap.add_argument("--ttr-train-spp", type=int, default=256)
+    ap.add_argument("--ttr-train-spp", type=int, default=128)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-scene measurement")
     ap.add_argument("--batch-scenes", type=int, default=32)
     ap.add_argument("--no-medium", action="store_true", help="skip the medium BF16/FP8 inference measurement")
@@ -120,8 +120,8 @@ def time_to_relmse(wr, scenes, dev, a):
     L = 500 hidden behind a 0.2 x 0.2 gap), max depth 16: wall time until the
     progressively doubled image reaches relMSE <= target against a plain
     reference at ttr_ref_spp; median of ttr_runs runs, each from an untrained
-    network.  Guided: 4-spp passes with 4 Adam steps of 2^18 records after each
-    until 64 spp, then the trained network without further training."""
+    network.  Guided: 8-spp passes with 32 Adam steps of 2^18 records after each
+    until ttr_train_spp, then the trained network without further training."""
     import numpy as np
     N = 18
     sc = scenes.guide_room(4, resolution=(a.ttr_width, a.ttr_height))
@@ -342,6 +342,18 @@ def main():
     film = wr.Film(gs)
     print(f"# scene {sc.name}: {sc.n_tris} tris, {info['nodes']} nodes, build+upload {time.time() - t0:.1f}s",
           file=sys.stderr, flush=True)
+    # the BVH: GPU build (the default) against the host SAH build, invariants checked on both
+    bvh = {}
+    for name, g_ in (("gpu_ploc", gs), ("gpu_ploc_warm", None), ("host_sah", None)):
+        if g_ is None:
+            g_ = wr.GpuScene(dev, sc, builder="host" if name == "host_sah" else "gpu")
+        st = g_.bvh_check(sc)  # also checks the tree's invariants
+        bvh[name] = {"build_ms": round(st["build_ms"], 1), "nodes": st["n_nodes"], "depth": st["depth"],
+                     "sah_cost": round(st["sah_cost"], 3)}
+        if g_ is not gs:
+            g_.close()
+    bvh["note"] = "gpu_ploc: first build in the process (includes loading the build kernels); build_ms includes the mesh upload"
+    print(f"# bvh: gpu {bvh['gpu_ploc']} host {bvh['host_sah']}", file=sys.stderr, flush=True)
     for i in range(a.warmup):
         wr.render_nee(gs, film, a.spp, num_path_vertices=N, seed=1)
     film.clear()
@@ -421,6 +433,7 @@ def main():
         "time_to_relmse": ttr,
         "batched": bat,
         "medium_bf16_fp8": med,
+        "bvh_build": bvh,
     }
     print(json.dumps(out))
 
