@@ -45,7 +45,7 @@ def parse():
     ap.add_argument("--ttr-width", type=int, default=480)
     ap.add_argument("--ttr-height", type=int, default=270)
     ap.add_argument("--ttr-target", type=float, default=1e-3)
-    ap.add_argument("--ttr-ref-spp", type=int, default=32768)
+    ap.add_argument("--ttr-ref-spp", type=int, default=65536)
     ap.add_argument("--ttr-runs", type=int, default=3)
     ap.add_argument("--ttr-train-spp", type=int, default=128)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-scene measurement")

@@ -347,3 +347,22 @@ def test_fp8_inference_in_medium_matches_bf16(gpu_device):
         a, b = res[fp8]
         sig = np.sqrt(((a - b) ** 2).mean((0, 1)) / 2 / npx)
         assert np.all(np.abs(a.mean((0, 1)) - ref.mean((0, 1))) < 5 * sig + 1e-6), (fp8, a.mean((0, 1)), ref.mean((0, 1)))
+
+
+def test_guided_graph_replay_gives_the_same_image(gpu_device):
+    """Guided passes (inference, extended shading) replayed from a captured
+    hipGraph follow the same random numbers as launched one by one: the same
+    image up to FP64 summation order.  (With training the records arrive in
+    atomic order, so trained networks differ between any two runs.)"""
+    sc = scenes.guide_room(4, n_spheres=6, level=3, resolution=(48, 32))
+    gs = wr.GpuScene(gpu_device, sc)
+    print("guided render, graph vs launches: guide room 48x32", flush=True)
+    g = wr.Guide(gs, seed=9)
+    imgs = []
+    for graph in (False, True):
+        img, _, _ = _render(gs, 64, 12, seed=61, guide=g, guide_fraction=0.5, spp_per_pass=16, train_steps=0,
+                            warmup_passes=0, use_graph=graph)
+        imgs.append(img)
+    import oracle
+    print(f"  relMSE(graph, launches) {oracle.rel_mse(imgs[1], imgs[0]):.2e}")
+    assert oracle.rel_mse(imgs[1], imgs[0]) < 1e-20

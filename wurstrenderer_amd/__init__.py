@@ -470,13 +470,14 @@ class Guide:
 def render_guided(scene: GpuScene, film: Film, guide: Guide, spp: int, num_path_vertices: int = 10, seed: int = 0,
                   guide_fraction: float = 0.5, spp_per_pass: int = 4, train_steps: int = 4, warmup_passes: int = 1,
                   use_mis: bool = True, first_sample: int | None = None, max_paths_in_flight: int = 0,
-                  profile: bool = False, fp8_inference: bool = False):
+                  profile: bool = False, fp8_inference: bool = False, use_graph: bool = False):
     """wr_render_guided: progressive passes of spp_per_pass, training between passes;
-    fp8_inference runs the renderer's network inference in e4m3."""
+    fp8_inference runs the renderer's network inference in e4m3; use_graph replays
+    each pass's iterations as a captured hipGraph."""
     if first_sample is None:
         first_sample = film.spp
     p = RenderParams(spp, num_path_vertices, seed, int(bool(use_mis)), first_sample, max_paths_in_flight,
-                     int(bool(profile)), 0)
+                     int(bool(profile)), 0, int(bool(use_graph)))
     gp = GuidedParams(guide_fraction, spp_per_pass, train_steps, warmup_passes, int(bool(fp8_inference)))
     st, gs = RenderStats(), GuideStats()
     _check(lib().wr_render_guided(scene.h, film.h, guide.h, C.byref(p), C.byref(gp), C.byref(st), C.byref(gs)))

@@ -83,12 +83,14 @@ __device__ __forceinline__ f32x16 mfma(u32x4 a, u32x4 b, f32x16 c) {
 }
 __device__ __forceinline__ __bf16 bf(float x) { return __float2bfloat16(x); }
 __device__ __forceinline__ float fb(__bf16 x) { return __bfloat162float(x); }
-// FP32 -> BF16 bits, round to nearest even (finite inputs)
-__device__ __forceinline__ uint32_t bf_bits(float x) {
-    const uint32_t u = __float_as_uint(x);
-    return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+// Two FP32 -> two BF16 (round to nearest even) in one word, lo in the low half:
+// one v_cvt_pk_bf16_f32.
+__device__ __forceinline__ uint32_t pack2(float lo, float hi) {
+    const f32x2 v = {lo, hi};
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2));
 }
-__device__ __forceinline__ uint32_t pack2(float lo, float hi) { return bf_bits(lo) | (bf_bits(hi) << 16); }
 __device__ __forceinline__ float unpack_lo(uint32_t w) { return __uint_as_float(w << 16); }
 __device__ __forceinline__ float unpack_hi(uint32_t w) { return __uint_as_float(w & 0xFFFF0000u); }
 // element e of an operand as FP32
@@ -174,13 +176,20 @@ __device__ __forceinline__ Enc encode(float3 pos, float3 n, float3 wo, const flo
 // One-blob feature f of 32 (Mueller et al. 2019, "Neural Importance
 // Sampling"): a Gaussian kernel one bin wide over k bins; position 3 x 6 bins,
 // normal 2 x 4, direction 2 x 3.
-__device__ __forceinline__ float feature(const Enc &e, int f) {
-    // selects rather than an indexed load: f is not always a compile-time constant
-    const float x = f < 6 ? e.v[0] : f < 12 ? e.v[1] : f < 18 ? e.v[2] : f < 22 ? e.v[3] : f < 26 ? e.v[4]
-                  : f < 29 ? e.v[5] : e.v[6];
-    const int k = f < 18 ? 6 : f < 26 ? 4 : 3;
-    const int j = f < 18 ? f % 6 : f < 26 ? (f - 18) % 4 : (f - 26) % 3;
-    const float d = (x - (float(j) + 0.5f) / float(k)) * float(k);
+__host__ __device__ constexpr int blob_dim(int f) { return f < 6 ? 0 : f < 12 ? 1 : f < 18 ? 2 : f < 22 ? 3 : f < 26 ? 4 : f < 29 ? 5 : 6; }
+__host__ __device__ constexpr int blob_bins(int f) { return f < 18 ? 6 : f < 26 ? 4 : 3; }
+__host__ __device__ constexpr float blob_centre(int f) {
+    return (float(f < 18 ? f % 6 : f < 26 ? (f - 18) % 4 : (f - 26) % 3) + 0.5f) / float(blob_bins(f));
+}
+// Feature f0 (lane half h = 0) or f0 + 4 (h = 1) of the record, f0 a
+// compile-time constant (the features of lane half h are kperm(ks, h, e) =
+// kperm(ks, 0, e) + 4 h), so only three selects depend on h.
+__device__ __forceinline__ float feature_h(const Enc &e, int f0, int h) {
+    const int f1 = f0 + 4;
+    const float x = h ? e.v[blob_dim(f1)] : e.v[blob_dim(f0)];
+    const float c = h ? blob_centre(f1) : blob_centre(f0);
+    const float k = h ? float(blob_bins(f1)) : float(blob_bins(f0));
+    const float d = (x - c) * k;
     return fexp(-0.5f * d * d);
 }
 
@@ -268,7 +277,7 @@ __device__ __forceinline__ void input_operands(const Enc &mine, u32x4 x[2][2] /*
         for (int ks = 0; ks < 2; ks++)
 #pragma unroll
             for (int i = 0; i < 4; i++)
-                x[ks][nt][i] = pack2(feature(e, kperm(ks, h, 2 * i)), feature(e, kperm(ks, h, 2 * i + 1)));
+                x[ks][nt][i] = pack2(feature_h(e, kperm(ks, 0, 2 * i), h), feature_h(e, kperm(ks, 0, 2 * i + 1), h));
     }
 }
 
@@ -463,7 +472,7 @@ __device__ __forceinline__ void forward8(const Net &net, const Enc &mine, float 
             for (int ks = 0; ks < 2; ks++)
 #pragma unroll
                 for (int i = 0; i < 8; i++) {
-                    f[8 * ks + i] = feature(e, kperm(ks, h, i));
+                    f[8 * ks + i] = feature_h(e, kperm(ks, 0, i), h);
                     m = fmaxf(m, f[8 * ks + i]);
                 }
             m = fmaxf(m, __shfl_xor(m, 32));

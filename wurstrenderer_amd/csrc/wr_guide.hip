@@ -91,6 +91,13 @@ __global__ __launch_bounds__(kPack8Block) void k_pack8(const float *params, u32x
     }
 }
 
+__global__ void k_fill_u32_guide(uint32_t *p, uint32_t n) {
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0u;
+}
+__global__ void k_fill_f64_guide(double *p, uint32_t n) {
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0.0;
+}
+
 // ---------------------------------------------------------------- train
 struct TrainArgs {
     const float4 *pos, *nrm, *wo, *wi;  // records; wi.w = weight
@@ -435,8 +442,23 @@ struct wr_guide {
     float *d_wscale = nullptr;  // their per-layer scales
     float *d_partials = nullptr, *d_loss = nullptr;
     double *d_wsum = nullptr;
-    Records rec{};
+    Records recs[2]{};             // renders: pass q writes set q % 2 while set (q - 1) % 2 trains
     void *rec_pool = nullptr;
+    // Training of a render pass runs on tstream; the host goes on enqueueing the
+    // next pass, whose inference waits for it on the GPU.  Inference reads its
+    // own copies of the packed operands: set k holds the network after the
+    // training of the last pass that wrote records set k.
+    hipStream_t tstream = nullptr;
+    u32x4 *d_ifwd[2] = {nullptr, nullptr};
+    float *d_ibias[2] = {nullptr, nullptr};
+    u32x2 *d_ifwd8[2] = {nullptr, nullptr};
+    float *d_iwscale[2] = {nullptr, nullptr};
+    void *infer_pool = nullptr;
+    hipEvent_t ev_trained[2] = {nullptr, nullptr};
+    bool pending[2] = {false, false};
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> train_marks;  // per trained pass: start, end (tstream)
+    int last_blocks = 0;          // blocks of the last render training step (its loss partials)
+    uint32_t last_batch = 0;
     float4 *d_draw_dir = nullptr;  // per path-pool entry: the inference kernel's direction and q there
     float *d_draw_qnee = nullptr;  // and q toward the light sample (same allocation)
     uint32_t draw_capacity = 0;
@@ -495,10 +517,10 @@ Net net_view(const wr_guide *g) {
     return n;
 }
 
-int pack(wr_guide *g) {
+int pack(wr_guide *g, hipStream_t st) {
     const int n = kFwdOps + kBwdOps;
-    k_pack<<<(n + 255) / 256, 256, 0, g->scene->stream>>>(g->d_params, g->d_fwd, g->d_bwd, g->d_bias);
-    k_pack8<<<kLayers, kPack8Block, 0, g->scene->stream>>>(g->d_params, g->d_fwd8, g->d_wscale);
+    k_pack<<<(n + 255) / 256, 256, 0, st>>>(g->d_params, g->d_fwd, g->d_bwd, g->d_bias);
+    k_pack8<<<kLayers, kPack8Block, 0, st>>>(g->d_params, g->d_fwd8, g->d_wscale);
     WR_HIP(hipGetLastError());
     return WR_OK;
 }
@@ -513,8 +535,7 @@ int reset_adam(wr_guide *g) {
 // One training step (or, with grad_out, the gradient only) over `batch`
 // records starting at `start` of the n records in recs.
 int train_step(wr_guide *g, const float4 *pos, const float4 *nrm, const float4 *wo, const float4 *wi, uint32_t n,
-               uint32_t start, uint32_t batch, float wscale, float *grad_out, double *loss) {
-    hipStream_t st = g->scene->stream;
+               uint32_t start, uint32_t batch, float wscale, float *grad_out, double *loss, hipStream_t st) {
     TrainArgs a;
     a.pos = pos; a.nrm = nrm; a.wo = wo; a.wi = wi;
     a.n = n; a.start = start; a.batch = batch; a.wscale = wscale;
@@ -524,6 +545,7 @@ int train_step(wr_guide *g, const float4 *pos, const float4 *nrm, const float4 *
     const int blocks = std::max(1, std::min((tiles + kTrainWaves - 1) / kTrainWaves, g->train_blocks));
     k_train<<<blocks, 64 * kTrainWaves, 0, st>>>(net_view(g), a);
     WR_HIP(hipGetLastError());
+    g->last_blocks = blocks;
     AdamArgs ad;
     ad.params = g->d_params; ad.m = g->d_m; ad.v = g->d_v;
     ad.partials = g->d_partials;
@@ -544,7 +566,7 @@ int train_step(wr_guide *g, const float4 *pos, const float4 *nrm, const float4 *
     k_adam<<<(kNumParams + kAdamParams - 1) / kAdamParams, kAdamBlock, 0, st>>>(ad);
     WR_HIP(hipGetLastError());
     if (!grad_out) {
-        int rc = pack(g);
+        int rc = pack(g, st);
         if (rc) return rc;
     }
     if (loss) {
@@ -582,67 +604,129 @@ struct DeviceRecords {
 }  // namespace
 
 // ---------------------------------------------------------------- internal interface for the renderer
-int guide_shade_view(wr_guide *g, float alpha, float record_prob, bool collect, guide::Shade *out) {
+// A render call: guide_render_begin, then per pass q (record set k = q % 2)
+// guide_pass_net (the network trained on pass q - 1), guide_begin_pass,
+// guide_shade_view, the pass, guide_end_pass (finalise on the render stream,
+// train on tstream; the host does not wait), and guide_render_end (waits for
+// the last training).
+int guide_render_begin(wr_guide *g) {
+    hipStream_t st = g->scene->stream;
+    WR_HIP(hipStreamSynchronize(g->tstream));
+    for (int k = 0; k < 2; k++) {
+        WR_HIP(hipMemcpyAsync(g->d_ifwd[k], g->d_fwd, sizeof(u32x4) * kFwdOps, hipMemcpyDeviceToDevice, st));
+        WR_HIP(hipMemcpyAsync(g->d_ibias[k], g->d_bias, sizeof(float) * kBiasPacked, hipMemcpyDeviceToDevice, st));
+        WR_HIP(hipMemcpyAsync(g->d_ifwd8[k], g->d_fwd8, sizeof(u32x2) * kFwdOps, hipMemcpyDeviceToDevice, st));
+        WR_HIP(hipMemcpyAsync(g->d_iwscale[k], g->d_wscale, sizeof(float) * kLayers, hipMemcpyDeviceToDevice, st));
+        g->pending[k] = false;
+    }
+    g->train_marks.clear();
+    return WR_OK;
+}
+
+// Pass q (records set = q % 2) renders with the network trained on pass q - 1
+// (inference copy set ^ 1): the render stream waits for that training on the
+// GPU, which also orders it after the training of pass q - 2 that read this
+// pass's record set.  (Rendering with the network of pass q - 2 instead, so
+// that training overlaps the next pass, measured worse: the stale network cost
+// more variance than the overlap saved time.)
+int guide_pass_net(wr_guide *g, int set, guide::Net *net) {
+    const int prev = set ^ 1;
+    if (g->pending[prev]) WR_HIP(hipStreamWaitEvent(g->scene->stream, g->ev_trained[prev], 0));
+    g->pending[prev] = false;
+    *net = net_view(g);
+    net->fwd = g->d_ifwd[prev];
+    net->bias = g->d_ibias[prev];
+    net->fwd8 = g->d_ifwd8[prev];
+    net->wscale = g->d_iwscale[prev];
+    net->bwd = nullptr;
+    return WR_OK;
+}
+
+int guide_shade_view(wr_guide *g, int set, float alpha, float record_prob, bool collect, guide::Shade *out) {
     out->draw.dir = g->d_draw_dir;
     out->draw.q_nee = g->d_draw_qnee;
-    out->rec = g->rec;
+    out->rec = g->recs[set];
     out->alpha = alpha;
     out->record_prob = record_prob;
     out->collect = collect ? 1 : 0;
     return WR_OK;
 }
 
-int guide_begin_pass(wr_guide *g, uint64_t samples, float *record_prob) {
-    WR_HIP(hipMemsetAsync(g->rec.count, 0, sizeof(uint32_t), g->scene->stream));
-    const double want = 0.9 * double(g->rec.capacity);
+int guide_begin_pass(wr_guide *g, int set, uint64_t samples, float *record_prob) {
+    k_fill_u32_guide<<<1, 64, 0, g->scene->stream>>>(g->recs[set].count, 1);
+    WR_HIP(hipGetLastError());
+    const double want = 0.9 * double(g->recs[set].capacity);
     const double expect = double(samples) * g->verts_per_path;
     *record_prob = float(std::min(1.0, want / std::max(expect, 1.0)));
     return WR_OK;
 }
 
-// After a pass: weights from the final radiance, then `steps` Adam steps.
-int guide_end_pass(wr_guide *g, const float4 *sample_rad, uint64_t samples, float record_prob, uint32_t steps,
+// After a pass: weights from the final radiance (render stream), then `steps`
+// Adam steps on tstream, which also leave the trained network in inference set
+// `set`.
+int guide_end_pass(wr_guide *g, int set, const float4 *sample_rad, uint64_t samples, float record_prob, uint32_t steps,
                    wr_guide_stats *gs) {
-    hipStream_t st = g->scene->stream;
+    hipStream_t st = g->scene->stream, ts = g->tstream;
+    const Records &rec = g->recs[set];
     uint32_t count = 0;
-    WR_HIP(hipMemcpyAsync(&count, g->rec.count, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    WR_HIP(hipMemcpyAsync(&count, rec.count, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     WR_HIP(hipStreamSynchronize(st));
     const double selected = double(samples) * double(record_prob);
     if (selected >= 1.0 && count > 0) g->verts_per_path = std::max(1.0, double(count) / selected);
-    const uint32_t n = std::min(count, g->rec.capacity);
+    const uint32_t n = std::min(count, rec.capacity);
     gs->records += n;
     if (n == 0 || steps == 0) return WR_OK;
-    hipEvent_t e0, e1;
-    WR_HIP(hipEventCreate(&e0));
-    WR_HIP(hipEventCreate(&e1));
-    WR_HIP(hipEventRecord(e0, st));
-    WR_HIP(hipMemsetAsync(g->d_wsum, 0, sizeof(double), st));
-    k_finalize<<<std::min<uint32_t>((n + 255) / 256, 4096), 256, 0, st>>>(g->rec, n, sample_rad, g->d_wsum);
+    k_fill_f64_guide<<<1, 64, 0, st>>>(g->d_wsum, 1);
+    k_finalize<<<std::min<uint32_t>((n + 255) / 256, 4096), 256, 0, st>>>(rec, n, sample_rad, g->d_wsum);
     WR_HIP(hipGetLastError());
     double wsum = 0.0;
     WR_HIP(hipMemcpyAsync(&wsum, g->d_wsum, sizeof(double), hipMemcpyDeviceToHost, st));
-    WR_HIP(hipStreamSynchronize(st));
-    if (wsum > 0.0) {
-        const uint32_t batch = std::min(n, g->cfg.batch_size);
-        // mean weight 1 over the pass's records; loss averaged over the batch
-        const float wscale = float(double(n) / wsum / double(batch));
-        for (uint32_t k = 0; k < steps; k++) {
-            const uint32_t start = uint32_t((uint64_t(k) * batch) % n);
-            double loss = 0.0;
-            int rc = train_step(g, g->rec.pos, g->rec.nrm, g->rec.wo, g->rec.wi, n, start, batch, wscale, nullptr,
-                                k + 1 == steps ? &loss : nullptr);
-            if (rc) return rc;
-            gs->train_steps++;
-            if (k + 1 == steps) gs->last_loss = loss;
-        }
+    WR_HIP(hipStreamSynchronize(st));  // the records are final
+    if (!(wsum > 0.0)) return WR_OK;
+    hipEvent_t t0, t1;
+    WR_HIP(hipEventCreate(&t0));
+    WR_HIP(hipEventCreate(&t1));
+    g->train_marks.emplace_back(t0, t1);
+    WR_HIP(hipEventRecord(t0, ts));
+    const uint32_t batch = std::min(n, g->cfg.batch_size);
+    // mean weight 1 over the pass's records; loss averaged over the batch
+    const float wscale = float(double(n) / wsum / double(batch));
+    for (uint32_t k = 0; k < steps; k++) {
+        const uint32_t start = uint32_t((uint64_t(k) * batch) % n);
+        int rc = train_step(g, rec.pos, rec.nrm, rec.wo, rec.wi, n, start, batch, wscale, nullptr, nullptr, ts);
+        if (rc) return rc;
+        gs->train_steps++;
     }
-    WR_HIP(hipEventRecord(e1, st));
-    WR_HIP(hipEventSynchronize(e1));
-    float ms = 0.f;
-    (void)hipEventElapsedTime(&ms, e0, e1);
-    gs->train_ms += ms;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    g->last_batch = batch;
+    WR_HIP(hipMemcpyAsync(g->d_ifwd[set], g->d_fwd, sizeof(u32x4) * kFwdOps, hipMemcpyDeviceToDevice, ts));
+    WR_HIP(hipMemcpyAsync(g->d_ibias[set], g->d_bias, sizeof(float) * kBiasPacked, hipMemcpyDeviceToDevice, ts));
+    WR_HIP(hipMemcpyAsync(g->d_ifwd8[set], g->d_fwd8, sizeof(u32x2) * kFwdOps, hipMemcpyDeviceToDevice, ts));
+    WR_HIP(hipMemcpyAsync(g->d_iwscale[set], g->d_wscale, sizeof(float) * kLayers, hipMemcpyDeviceToDevice, ts));
+    WR_HIP(hipEventRecord(t1, ts));
+    WR_HIP(hipEventRecord(g->ev_trained[set], ts));
+    g->pending[set] = true;
+    return WR_OK;
+}
+
+// Waits for the last training, then the loss of the last step and the
+// training time.
+int guide_render_end(wr_guide *g, wr_guide_stats *gs) {
+    WR_HIP(hipStreamSynchronize(g->tstream));
+    for (auto &m : g->train_marks) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, m.first, m.second) == hipSuccess) gs->train_ms += ms;
+        (void)hipEventDestroy(m.first);
+        (void)hipEventDestroy(m.second);
+    }
+    if (!g->train_marks.empty() && g->last_blocks > 0) {
+        std::vector<float> l(g->last_blocks);
+        WR_HIP(hipMemcpy(l.data(), g->d_loss, sizeof(float) * l.size(), hipMemcpyDeviceToHost));
+        double sum = 0.0;
+        for (float x : l) sum += x;
+        gs->last_loss = sum;
+    }
+    g->train_marks.clear();
+    g->pending[0] = g->pending[1] = false;
     return WR_OK;
 }
 
@@ -678,9 +762,18 @@ void wr_guide_destroy(wr_guide *g) {
     if (!g) return;
     (void)hipSetDevice(g->scene->dev->ordinal);
     (void)hipStreamSynchronize(g->scene->stream);
+    if (g->tstream) (void)hipStreamSynchronize(g->tstream);
+    for (auto &m : g->train_marks) {
+        (void)hipEventDestroy(m.first);
+        (void)hipEventDestroy(m.second);
+    }
+    for (int k = 0; k < 2; k++) {
+        if (g->ev_trained[k]) (void)hipEventDestroy(g->ev_trained[k]);
+    }
+    if (g->tstream) (void)hipStreamDestroy(g->tstream);
     for (void *p : {(void *)g->d_params, (void *)g->d_m, (void *)g->d_v, (void *)g->d_fwd, (void *)g->d_bwd, (void *)g->d_bias,
                     (void *)g->d_fwd8, (void *)g->d_wscale, (void *)g->d_partials, (void *)g->d_loss, (void *)g->d_wsum,
-                    g->rec_pool, (void *)g->d_draw_dir})
+                    g->rec_pool, g->infer_pool, (void *)g->d_draw_dir})
         if (p) (void)hipFree(p);
     delete g;
 }
@@ -702,16 +795,33 @@ static int guide_create_impl(wr_guide *g) {
     WR_HIP(hipMalloc(reinterpret_cast<void **>(&g->d_loss), sizeof(float) * g->train_blocks));
     WR_HIP(hipMalloc(reinterpret_cast<void **>(&g->d_wsum), sizeof(double)));
     const size_t cap = g->cfg.record_capacity;
-    WR_HIP(hipMalloc(&g->rec_pool, sizeof(float4) * 6 * cap + 256));
-    float4 *b = reinterpret_cast<float4 *>(g->rec_pool);
-    g->rec.pos = b;
-    g->rec.nrm = b + cap;
-    g->rec.wo = b + 2 * cap;
-    g->rec.wi = b + 3 * cap;
-    g->rec.thr = b + 4 * cap;
-    g->rec.rad = b + 5 * cap;
-    g->rec.count = reinterpret_cast<uint32_t *>(b + 6 * cap);
-    g->rec.capacity = uint32_t(cap);
+    WR_HIP(hipMalloc(&g->rec_pool, 2 * (sizeof(float4) * 6 * cap + 256)));
+    for (int k = 0; k < 2; k++) {
+        float4 *b = reinterpret_cast<float4 *>(static_cast<char *>(g->rec_pool) + k * (sizeof(float4) * 6 * cap + 256));
+        Records &r = g->recs[k];
+        r.pos = b;
+        r.nrm = b + cap;
+        r.wo = b + 2 * cap;
+        r.wi = b + 3 * cap;
+        r.thr = b + 4 * cap;
+        r.rad = b + 5 * cap;
+        r.count = reinterpret_cast<uint32_t *>(b + 6 * cap);
+        r.capacity = uint32_t(cap);
+    }
+    // inference copies of the packed operands, two sets, 256-byte aligned pieces
+    const size_t fwd_b = (sizeof(u32x4) * kFwdOps + 255) / 256 * 256, bias_b = (sizeof(float) * kBiasPacked + 255) / 256 * 256;
+    const size_t fwd8_b = (sizeof(u32x2) * kFwdOps + 255) / 256 * 256, ws_b = 256;
+    const size_t set_b = fwd_b + bias_b + fwd8_b + ws_b;
+    WR_HIP(hipMalloc(&g->infer_pool, 2 * set_b));
+    for (int k = 0; k < 2; k++) {
+        char *b = static_cast<char *>(g->infer_pool) + k * set_b;
+        g->d_ifwd[k] = reinterpret_cast<u32x4 *>(b);
+        g->d_ibias[k] = reinterpret_cast<float *>(b + fwd_b);
+        g->d_ifwd8[k] = reinterpret_cast<u32x2 *>(b + fwd_b + bias_b);
+        g->d_iwscale[k] = reinterpret_cast<float *>(b + fwd_b + bias_b + fwd8_b);
+        WR_HIP(hipEventCreateWithFlags(&g->ev_trained[k], hipEventDisableTiming));
+    }
+    WR_HIP(hipStreamCreateWithFlags(&g->tstream, hipStreamNonBlocking));
     std::vector<float> p;
     init_params(g->cfg.seed, p);
     return wr_guide_set_params(g, p.data());
@@ -759,7 +869,7 @@ int wr_guide_set_params(wr_guide *g, const float *params) {
         if (!std::isfinite(params[j])) return set_error(WR_ERR_INVALID_ARGUMENT, "non-finite parameter");
     WR_HIP(hipMemcpyAsync(g->d_params, params, sizeof(float) * kNumParams, hipMemcpyHostToDevice, g->scene->stream));
     rc = reset_adam(g);
-    if (!rc) rc = pack(g);
+    if (!rc) rc = pack(g, g->scene->stream);
     if (rc) return rc;
     WR_HIP(hipStreamSynchronize(g->scene->stream));
     return WR_OK;
@@ -817,7 +927,7 @@ int wr_guide_gradient(wr_guide *g, const float *pos, const float *nrm, const flo
     if (rc) return rc;
     float *dg = nullptr;
     WR_HIP(hipMalloc(reinterpret_cast<void **>(&dg), sizeof(float) * kNumParams));
-    rc = train_step(g, dr.pos, dr.nrm, dr.wo, dr.wi, n, 0, n, 1.f / float(n), dg, loss);
+    rc = train_step(g, dr.pos, dr.nrm, dr.wo, dr.wi, n, 0, n, 1.f / float(n), dg, loss, g->scene->stream);
     hipError_t e = hipSuccess;
     if (!rc) {
         e = hipMemcpyAsync(grad, dg, sizeof(float) * kNumParams, hipMemcpyDeviceToHost, g->scene->stream);
@@ -841,7 +951,7 @@ int wr_guide_train(wr_guide *g, const float *pos, const float *nrm, const float 
     if (rc) return rc;
     for (uint32_t k = 0; k < steps; k++) {
         rc = train_step(g, dr.pos, dr.nrm, dr.wo, dr.wi, n, 0, n, 1.f / float(n), nullptr,
-                        (k + 1 == steps) ? loss : nullptr);
+                        (k + 1 == steps) ? loss : nullptr, g->scene->stream);
         if (rc) return rc;
     }
     WR_HIP(hipStreamSynchronize(g->scene->stream));

@@ -159,14 +159,22 @@ int build_bvh4_gpu(const float *d_verts, const uint32_t *d_tris, const uint32_t 
 // ============================================================== guiding (wr_guide.hip)
 #include "wr_guide.h"
 wr::guide::Net guide_net(const wr_guide *g);
-// Device view for the guided shading kernel.
-int guide_shade_view(wr_guide *g, float alpha, float record_prob, bool collect, wr::guide::Shade *out);
-// Clears the record buffer and picks the fraction of paths to record so that
+// A render call: guide_render_begin; per pass (record set k = pass % 2):
+// guide_pass_net (the network inference reads: the one trained on the previous
+// pass; the render stream waits for that training on the GPU),
+// guide_begin_pass, guide_shade_view, the pass, guide_end_pass (training
+// weights from the pass's final per-sample radiance, then `steps` Adam steps
+// enqueued on the guide's own stream); guide_render_end waits for the last
+// training.
+int guide_render_begin(wr_guide *g);
+int guide_pass_net(wr_guide *g, int set, wr::guide::Net *net);
+int guide_shade_view(wr_guide *g, int set, float alpha, float record_prob, bool collect, wr::guide::Shade *out);
+// Clears the record set and picks the fraction of paths to record so that
 // about 90% of the capacity fills.
-int guide_begin_pass(wr_guide *g, uint64_t samples, float *record_prob);
-// Training weights from the pass's final per-sample radiance, then `steps` Adam steps.
-int guide_end_pass(wr_guide *g, const float4 *sample_rad, uint64_t samples, float record_prob, uint32_t steps,
+int guide_begin_pass(wr_guide *g, int set, uint64_t samples, float *record_prob);
+int guide_end_pass(wr_guide *g, int set, const float4 *sample_rad, uint64_t samples, float record_prob, uint32_t steps,
                    wr_guide_stats *gs);
+int guide_render_end(wr_guide *g, wr_guide_stats *gs);
 // Sizes the per-entry draws of the inference kernel for a path pool of `capacity` entries.
 int guide_reserve_draws(wr_guide *g, uint32_t capacity);
 wr_scene *guide_scene(const wr_guide *g);

@@ -601,7 +601,7 @@ __global__ __launch_bounds__(kInferBlock) __attribute__((amdgpu_waves_per_eu(WR_
                 guide::decode(L, wl);
                 const uint32_t key = meta.x;
                 const LightSample l = sample_light(s, u01(key, kDimLight), u01(key, kDimLight + 1));
-                const float q_nee = guide::pdf(L, normalize3(l.pos - pos));
+                const float3 dl = normalize3(l.pos - pos);
                 const uint32_t dim = kDimBounce0 + kDimsPerBounce * meta.z + kBdBsdf;
                 const float u0 = u01(key, dim), u1 = u01(key, dim + 1), us = u01(key, dim + 2);
                 float alpha = frac;
@@ -610,9 +610,20 @@ __global__ __launch_bounds__(kInferBlock) __attribute__((amdgpu_waves_per_eu(WR_
                     m = s.mats[mat];
                     alpha = guide_alpha(m, frac);
                 }
+                // At a surface the mixture is folded onto the hemisphere of nf (the
+                // BSDFs reflect only): a draw below the surface is mirrored across
+                // the tangent plane, so the density is q(w) + q(R w) above and 0
+                // below -- normalised, and no network draw is lost to f = 0.
+                auto q = [&](float3 w) {
+                    if (med) return guide::pdf(L, w);
+                    const float c = dot3(w, nf);
+                    return c < 0.f ? 0.f : guide::pdf(L, w) + guide::pdf(L, w - nf * (2.f * c));
+                };
                 float3 d;
                 if (us < alpha) {
                     d = guide::sample(L, wl, us * guide::frcp(alpha), u0, u1);
+                    const float c = dot3(d, nf);
+                    if (!med && c < 0.f) d = d - nf * (2.f * c);
                 } else if (med) {
                     d = hg_sample(s.med.g, rd, u0, u1);
                 } else {
@@ -622,8 +633,8 @@ __global__ __launch_bounds__(kInferBlock) __attribute__((amdgpu_waves_per_eu(WR_
                     bsdf_sample(m, to_local(f, -rd), u0, u1, wo, pb);
                     d = normalize3(to_world(f, wo));
                 }
-                out.dir[i] = make_float4(d.x, d.y, d.z, guide::pdf(L, d));
-                out.q_nee[i] = q_nee;
+                out.dir[i] = make_float4(d.x, d.y, d.z, q(d));
+                out.q_nee[i] = q(dl);
             }
         }
     }
@@ -1604,6 +1615,7 @@ static int render_impl(wr_scene *s, wr_film *f, const wr_render_params *prm, wr_
     if (rc) return rc;
     if (g) {
         rc = guide_reserve_draws(g, P);
+        if (!rc) rc = guide_render_begin(g);
         if (rc) return rc;
     }
     PathBuffers pb = s->pb;
@@ -1647,7 +1659,7 @@ static int render_impl(wr_scene *s, wr_film *f, const wr_render_params *prm, wr_
                                          : reinterpret_cast<const void *>(&k_guide_infer<false, false>));
     const int g_infer = std::max(1, std::min<int>((P + kInferBlock - 1) / kInferBlock,
                                                   s->dev->n_cu * occupancy(s->dev, infer_fn, kInferBlock)));
-    const guide::Net net = g ? guide_net(g) : guide::Net{};
+    guide::Net net{};
     const int g_ext = trav_grid(s, P, s->dev->occ_extend);
     const int g_shd = trav_grid(s, 2 * P, s->dev->occ_shadow);
     for (uint32_t done = 0; done < uint32_t(prm->spp); done += spp_pass) {
@@ -1663,13 +1675,13 @@ static int render_impl(wr_scene *s, wr_film *f, const wr_render_params *prm, wr_
         guide::Shade gs{};
         float record_prob = 0.f;
         const bool collect = g && gp->train_steps > 0;
+        const int set = pass & 1;  // record set and inference copy of this pass
         if (g) {
-            if (collect) {
-                rc = guide_begin_pass(g, a.total, &record_prob);
-                if (rc) return rc;
-            }
+            rc = guide_pass_net(g, set, &net);
+            if (!rc && collect) rc = guide_begin_pass(g, set, a.total, &record_prob);
+            if (rc) return rc;
             const bool sample_net = pass >= int(gp->warmup_passes);
-            rc = guide_shade_view(g, sample_net ? gp->guide_fraction : 0.f, record_prob, collect, &gs);
+            rc = guide_shade_view(g, set, sample_net ? gp->guide_fraction : 0.f, record_prob, collect, &gs);
             if (rc) return rc;
         }
         k_init<<<1, 64, 0, s->stream>>>(pb, a);
@@ -1721,10 +1733,14 @@ static int render_impl(wr_scene *s, wr_film *f, const wr_render_params *prm, wr_
         end();
         WR_HIP(hipGetLastError());
         if (collect) {
-            rc = guide_end_pass(g, pb.sample_rad, a.total, record_prob, gp->train_steps, &gst);
+            rc = guide_end_pass(g, set, pb.sample_rad, a.total, record_prob, gp->train_steps, &gst);
             if (rc) return rc;
         }
         pass++;
+    }
+    if (g) {
+        rc = guide_render_end(g, &gst);
+        if (rc) return rc;
     }
     unsigned long long hs[6] = {0};
     WR_HIP(hipMemcpyAsync(hs, pb.stats, sizeof hs, hipMemcpyDeviceToHost, s->stream));
