@@ -404,8 +404,10 @@ __global__ __launch_bounds__(kBB) void k_collapse(CollapseArgs a) {
 
 inline unsigned grid_of(uint64_t n) { return unsigned((n + kBB - 1) / kBB); }
 
-// Device allocations of one build, released on every return path.
+// Device allocations of one build, released on every return path once the
+// build's stream has finished with them.
 struct Scratch {
+    hipStream_t stream = nullptr;
     std::vector<void *> ptrs;
     template <typename T>
     hipError_t alloc(T **p, size_t count) {
@@ -416,6 +418,7 @@ struct Scratch {
         return e;
     }
     ~Scratch() {
+        if (stream) (void)hipStreamSynchronize(stream);
         for (void *p : ptrs) (void)hipFree(p);
     }
 };
@@ -444,6 +447,7 @@ int build_bvh4_gpu(const float *d_verts, const uint32_t *d_tris, const uint32_t 
         return WR_OK;
     }
     Scratch sc;
+    sc.stream = stream;
     const size_t nn = 2 * size_t(n) - 1;
     float *partial, *cbox;
     uint64_t *keys, *keys2, *flags, *incl;
