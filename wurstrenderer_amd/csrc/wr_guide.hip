@@ -458,7 +458,6 @@ struct wr_guide {
     bool pending[2] = {false, false};
     std::vector<std::pair<hipEvent_t, hipEvent_t>> train_marks;  // per trained pass: start, end (tstream)
     int last_blocks = 0;          // blocks of the last render training step (its loss partials)
-    uint32_t last_batch = 0;
     float4 *d_draw_dir = nullptr;  // per path-pool entry: the inference kernel's direction and q there
     float *d_draw_qnee = nullptr;  // and q toward the light sample (same allocation)
     uint32_t draw_capacity = 0;
@@ -697,7 +696,6 @@ int guide_end_pass(wr_guide *g, int set, const float4 *sample_rad, uint64_t samp
         if (rc) return rc;
         gs->train_steps++;
     }
-    g->last_batch = batch;
     WR_HIP(hipMemcpyAsync(g->d_ifwd[set], g->d_fwd, sizeof(u32x4) * kFwdOps, hipMemcpyDeviceToDevice, ts));
     WR_HIP(hipMemcpyAsync(g->d_ibias[set], g->d_bias, sizeof(float) * kBiasPacked, hipMemcpyDeviceToDevice, ts));
     WR_HIP(hipMemcpyAsync(g->d_ifwd8[set], g->d_fwd8, sizeof(u32x2) * kFwdOps, hipMemcpyDeviceToDevice, ts));
@@ -730,7 +728,6 @@ int guide_render_end(wr_guide *g, wr_guide_stats *gs) {
     return WR_OK;
 }
 
-guide::Net guide_net(const wr_guide *g) { return net_view(g); }
 
 int guide_reserve_draws(wr_guide *g, uint32_t capacity) {
     if (g->draw_capacity >= capacity) return WR_OK;

@@ -158,7 +158,6 @@ int build_bvh4_gpu(const float *d_verts, const uint32_t *d_tris, const uint32_t 
 
 // ============================================================== guiding (wr_guide.hip)
 #include "wr_guide.h"
-wr::guide::Net guide_net(const wr_guide *g);
 // A render call: guide_render_begin; per pass (record set k = pass % 2):
 // guide_pass_net (the network inference reads: the one trained on the previous
 // pass; the render stream waits for that training on the GPU),
