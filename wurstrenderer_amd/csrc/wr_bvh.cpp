@@ -149,10 +149,22 @@ struct Builder {
     }
 };
 
+// Collapse of the binary tree to 4-wide nodes, choosing which binary nodes
+// become the children of each wide node by the SAH (Ylitie, Karras and Laine,
+// "Efficient Incoherent Ray Traversal on GPUs Through Compressed Wide BVHs",
+// HPG 2017, section 4.1): cost[n][i-1] is the least SAH cost of covering the
+// subtree of binary node n with at most i wide-node children, computed bottom
+// up; binary leaves stay whole, and an inner binary node of at most
+// kMaxLeafTris triangles may become one wide leaf.
 struct Collapser {
     const std::vector<B2> &b2;
     std::vector<Node4> out;
     uint32_t max_stack = 0, max_depth = 0;
+    std::vector<float> cost;       // 4 per binary node
+    std::vector<uint8_t> split;    // 4 per binary node: [0] 1 = one wide leaf; [i-1] (i > 1): 0 = as with i-1
+                                   // children, k = k children from the left subtree and i-k from the right
+    std::vector<uint8_t> top_split;  // per inner binary node: slots of its own wide node given to the left
+    std::vector<uint32_t> cnt, first;  // triangles under a binary node, first of them in leaf order
 
     explicit Collapser(const std::vector<B2> &b) : b2(b) {}
 
@@ -161,23 +173,73 @@ struct Collapser {
         return dx * dy + dy * dz + dz * dx;
     }
 
+    // children always have larger ids than their parent (allocated after it)
+    void optimise() {
+        const size_t n = b2.size();
+        cost.assign(4 * n, 0.f);
+        split.assign(4 * n, 0);
+        cnt.assign(n, 0);
+        first.assign(n, 0);
+        top_split.assign(n, 0);
+        for (size_t q = n; q-- > 0;) {
+            const B2 &b = b2[q];
+            const float A = area(b);
+            float *c = &cost[4 * q];
+            uint8_t *sp = &split[4 * q];
+            if (b.count > 0) {
+                cnt[q] = b.count;
+                first[q] = b.first;
+                for (int i = 0; i < 4; i++) c[i] = kTriangleCost * A * float(b.count);
+                sp[0] = 1;
+                continue;
+            }
+            const uint32_t l = b.left, r = l + 1;
+            cnt[q] = cnt[l] + cnt[r];
+            first[q] = first[l];  // the left subtree's triangles come first
+            float dist[5];
+            uint8_t dk[5];
+            for (int j = 2; j <= 4; j++) {  // k = 1 first: a valid split even if a cost is not finite
+                dist[j] = cost[4 * l] + cost[4 * r + j - 2];
+                dk[j] = 1;
+                for (int k = 2; k < j; k++) {
+                    const float v = cost[4 * l + k - 1] + cost[4 * r + j - k - 1];
+                    if (v < dist[j]) { dist[j] = v; dk[j] = uint8_t(k); }
+                }
+            }
+            const float inner = kTraversalCost * A + dist[4];
+            const float leaf = cnt[q] <= uint32_t(kMaxLeafTris) ? kTriangleCost * A * float(cnt[q]) : FLT_MAX;
+            c[0] = std::min(inner, leaf);
+            sp[0] = leaf <= inner ? 1 : 0;
+            for (int i = 2; i <= 4; i++) {
+                if (dist[i] < c[i - 2]) { c[i - 1] = dist[i]; sp[i - 1] = dk[i]; }
+                else { c[i - 1] = c[i - 2]; sp[i - 1] = 0; }
+            }
+            top_split[q] = dk[4];
+        }
+    }
+    // Appends the wide children that cover binary node m with at most i slots.
+    void expand(uint32_t m, int i, uint32_t *kids, int &k) const {
+        for (;;) {
+            const int s = i > 1 ? split[4 * size_t(m) + i - 1] : 0;
+            if (i == 1 || b2[m].count > 0) { kids[k++] = m; return; }
+            if (s == 0) { i--; continue; }
+            expand(b2[m].left, s, kids, k);
+            expand(b2[m].left + 1, i - s, kids, k);
+            return;
+        }
+    }
+    bool wide_leaf(uint32_t m) const { return split[4 * size_t(m)] != 0; }
+
     // stack_before: entries that may sit on the stack when this node is visited
-    uint32_t collapse(uint32_t root_children[], int nc, uint32_t stack_before, uint32_t depth) {
+    uint32_t collapse(uint32_t q, uint32_t stack_before, uint32_t depth) {
         uint32_t kids[4];
         int k = 0;
-        for (int i = 0; i < nc; i++) kids[k++] = root_children[i];
-        for (;;) {
-            if (k == 4) break;
-            int pick = -1;
-            float best = -1.f;
-            for (int i = 0; i < k; i++) {
-                const B2 &n = b2[kids[i]];
-                if (n.count == 0 && area(n) > best) { best = area(n); pick = i; }
-            }
-            if (pick < 0) break;
-            uint32_t l = b2[kids[pick]].left;
-            kids[pick] = l;
-            kids[k++] = l + 1;
+        if (b2[q].count > 0) {
+            kids[k++] = q;  // a root that is one leaf
+        } else {
+            const int s = top_split[q];
+            expand(b2[q].left, s, kids, k);
+            expand(b2[q].left + 1, 4 - s, kids, k);
         }
         // slot order = decreasing area: any-hit traversal visits the slots in
         // order, and a larger box is likelier to hold an occluder
@@ -199,12 +261,11 @@ struct Collapser {
             const B2 &c = b2[kids[i]];
             nd.lo_x[i] = c.lo[0]; nd.lo_y[i] = c.lo[1]; nd.lo_z[i] = c.lo[2];
             nd.hi_x[i] = c.hi[0]; nd.hi_y[i] = c.hi[1]; nd.hi_z[i] = c.hi[2];
-            if (c.count > 0) {
-                nd.child[i] = kLeafBit | ((c.count - 1u) << 28) | c.first;
-                nd.count[i] = c.count;
+            if (wide_leaf(kids[i])) {
+                nd.child[i] = kLeafBit | ((cnt[kids[i]] - 1u) << 28) | first[kids[i]];
+                nd.count[i] = cnt[kids[i]];
             } else {
-                uint32_t gk[2] = {c.left, c.left + 1};
-                nd.child[i] = collapse(gk, 2, stack_before + uint32_t(k) - 1, depth + 1);
+                nd.child[i] = collapse(kids[i], stack_before + uint32_t(k) - 1, depth + 1);
                 nd.count[i] = 0;
             }
         }
@@ -254,12 +315,9 @@ bool build_bvh4(const float *verts, uint32_t nv, const uint32_t *tris, uint32_t 
         b.build(root, 0, nt, 0);
         b.nodes.resize(b.n_nodes.load());
         Collapser c(b.nodes);
-        c.out.reserve(b.nodes.size() / 2 + 1);
-        uint32_t rc[2];
-        int nc;
-        if (b.nodes[root].count > 0) { rc[0] = root; nc = 1; }
-        else { rc[0] = b.nodes[root].left; rc[1] = rc[0] + 1; nc = 2; }
-        c.collapse(rc, nc, 0, 0);
+        c.out.reserve(b.nodes.size() / 3 + 1);
+        c.optimise();
+        c.collapse(root, 0, 0);
         nodes.swap(c.out);
         max_stack = c.max_stack;
         depth = c.max_depth;

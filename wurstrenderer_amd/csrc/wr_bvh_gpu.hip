@@ -3,7 +3,11 @@
 // Construction", IEEE TVCG 24(12), 2018) over Morton-ordered triangle boxes
 // gives a binary tree whose leaves are chosen by the SAH; it is then collapsed
 // top-down, one tree level per launch pair, into the 128-byte Node4 layout of
-// wr_layout.h that the traversal kernels read.
+// wr_layout.h that the traversal kernels read.  Which binary nodes become the
+// children of each 4-wide node is chosen by the SAH too (Ylitie, Karras and
+// Laine, "Efficient Incoherent Ray Traversal on GPUs Through Compressed Wide
+// BVHs", HPG 2017, section 4.1), by a table that each merge fills from its two
+// children's, the same rule as the host builder's collapse.
 //
 // Replaces rtcCommitScene's BVH8/Triangle4 SAH build (reference:
 // src/accel/embreeaccel.h:466-469; Embree kernels/common/scene.cpp:120-140).
@@ -18,10 +22,11 @@
 // equal areas (duplicate triangles) into random matchings of logarithmic depth
 // instead of a chain.
 //
-// Device memory of the binary tree: 2n-1 nodes of 40 B (box lo.xyz + SAH cost,
-// box hi.xyz + triangle count with the leaf mark, two child ids); nodes 0..n-1
-// are the triangles in Morton order, inner nodes follow in merge order, so a
-// child always has a smaller id than its parent.
+// Device memory of the binary tree: 2n-1 nodes of 60 B (box lo.xyz + SAH cost,
+// box hi.xyz + triangle count with the leaf mark, two child ids, the collapse
+// table: four costs and one word of decisions); nodes 0..n-1 are the triangles
+// in Morton order, inner nodes follow in merge order, so a child always has a
+// smaller id than its parent.
 #include <hip/hip_runtime.h>
 
 #include <rocprim/device/device_radix_sort.hpp>
@@ -43,6 +48,16 @@ constexpr float kCt = 1.0f, kCi = 1.0f;  // SAH costs of a box test and a triang
 constexpr uint32_t kLeafMark = 0x80000000u;  // in a node's count word: the subtree becomes one Node4 leaf
 constexpr int kBoundsBlocks = 1024;
 constexpr uint32_t kNone = 0xFFFFFFFFu;   // no node / unused child slot
+
+// Collapse table of binary node x: wc[x] = (c1, c2, c3, c4), ci = least SAH cost
+// of covering x's subtree with at most i children of a 4-wide node; wd[x] =
+// decisions: bit 0 = with one child, x is one wide leaf (else one wide inner
+// node); bits 2(i-1) .. 2(i-1)+1 for i = 2..4 = s: 0 = as with i-1 children,
+// else s children from the left subtree and i-s from the right; bits 8..9 =
+// the split of x's own four slots when x is a wide inner node.
+constexpr uint32_t kWideLeaf = 1u;
+__device__ __forceinline__ int wide_split(uint32_t w, int i) { return int(w >> (2 * (i - 1))) & 3; }
+__device__ __forceinline__ int wide_top(uint32_t w) { return int(w >> 8) & 3; }
 
 __device__ __forceinline__ float half_area(float3 lo, float3 hi) {
     const float dx = hi.x - lo.x, dy = hi.y - lo.y, dz = hi.z - lo.z;
@@ -143,15 +158,18 @@ __global__ __launch_bounds__(kBB) void k_morton(const float *verts, const uint32
 // Leaves: node j = the j-th triangle in Morton order.
 __global__ __launch_bounds__(kBB) void k_leaves(const float *verts, const uint32_t *tris, uint32_t n,
                                                 const uint32_t *order, float4 *nlo, float4 *nhi, uint2 *kids,
-                                                uint32_t *clusters) {
+                                                float4 *wc, uint32_t *wd, uint32_t *clusters) {
     const uint32_t j = blockIdx.x * kBB + threadIdx.x;
     if (j >= n) return;
     const uint32_t t = order[j];
     float3 lo, hi;
     tri_box(verts, tris, t, lo, hi);
-    nlo[j] = make_float4(lo.x, lo.y, lo.z, kCi * half_area(lo, hi));
+    const float c = kCi * half_area(lo, hi);
+    nlo[j] = make_float4(lo.x, lo.y, lo.z, c);
     nhi[j] = make_float4(hi.x, hi.y, hi.z, __uint_as_float(1u | kLeafMark));
     kids[j] = make_uint2(kNone, t);
+    wc[j] = make_float4(c, c, c, c);
+    wd[j] = kWideLeaf;
     clusters[j] = j;
 }
 
@@ -222,7 +240,8 @@ __global__ __launch_bounds__(kBB) void k_merge_flags(const uint32_t *nearest, ui
 // leaf decision, and compacts the surviving clusters in order.
 __global__ __launch_bounds__(kBB) void k_merge(const uint32_t *clusters, const uint32_t *nearest, uint32_t m,
                                                const uint64_t *flags, const uint64_t *incl, uint32_t node_base,
-                                               float4 *nlo, float4 *nhi, uint2 *kids, uint32_t *out) {
+                                               float4 *nlo, float4 *nhi, uint2 *kids, float4 *wc, uint32_t *wd,
+                                               uint32_t *out) {
     const uint32_t i = blockIdx.x * kBB + threadIdx.x;
     if (i >= m) return;
     const uint64_t f = flags[i];
@@ -246,6 +265,39 @@ __global__ __launch_bounds__(kBB) void k_merge(const uint32_t *clusters, const u
     nlo[id] = make_float4(lo.x, lo.y, lo.z, leaf ? leaf_cost : inner);
     nhi[id] = make_float4(hi.x, hi.y, hi.z, __uint_as_float(cnt | (leaf ? kLeafMark : 0u)));
     kids[id] = make_uint2(a, b);
+    // collapse table (a binary SAH leaf stays one wide leaf)
+    if (leaf) {
+        wc[id] = make_float4(leaf_cost, leaf_cost, leaf_cost, leaf_cost);
+        wd[id] = kWideLeaf;
+    } else {
+        const float4 ca = wc[a], cb = wc[b];
+        const float cl[4] = {ca.x, ca.y, ca.z, ca.w}, cr[4] = {cb.x, cb.y, cb.z, cb.w};
+        float dist[5];
+        uint32_t dk[5];
+#pragma unroll
+        for (int j = 2; j <= 4; j++) {  // k = 1 first: a valid split even if a cost is not finite
+            dist[j] = cl[0] + cr[j - 2];
+            dk[j] = 1;
+#pragma unroll
+            for (int k = 2; k < j; k++) {
+                const float v = cl[k - 1] + cr[j - k - 1];
+                if (v < dist[j]) { dist[j] = v; dk[j] = uint32_t(k); }
+            }
+        }
+        const float wide_inner = kCt * A + dist[4];
+        const float wide_leaf = cnt <= uint32_t(kMaxLeafTris) ? leaf_cost : INFINITY;
+        float c[4];
+        uint32_t w = wide_leaf <= wide_inner ? kWideLeaf : 0u;
+        c[0] = fminf(wide_inner, wide_leaf);
+#pragma unroll
+        for (int i = 2; i <= 4; i++) {
+            const bool take = dist[i] < c[i - 2];
+            c[i - 1] = take ? dist[i] : c[i - 2];
+            w |= (take ? dk[i] : 0u) << (2 * (i - 1));
+        }
+        wc[id] = make_float4(c[0], c[1], c[2], c[3]);
+        wd[id] = w | dk[4] << 8;
+    }
     out[pos] = id;
 }
 
@@ -253,47 +305,49 @@ __global__ __launch_bounds__(kBB) void k_merge(const uint32_t *clusters, const u
 __device__ __forceinline__ uint32_t count_of(const float4 *nhi, uint32_t x) {
     return __float_as_uint(nhi[x].w) & ~kLeafMark;
 }
-__device__ __forceinline__ bool is_leaf(const float4 *nhi, uint32_t x) {
-    return (__float_as_uint(nhi[x].w) & kLeafMark) != 0u;
-}
 
 // Queue entry e = (binary node, first leaf-order triangle of its subtree,
 // traversal stack entries that may be live when the Node4 is visited).
-// Picks up to four children: the node's two, then the largest-area inner child
-// opened until there are four (the host collapse's rule).
+// Picks the node's children by its collapse table, left to right.
 __global__ __launch_bounds__(kBB) void k_expand(const uint4 *queue, uint32_t qn, const float4 *nlo,
-                                                const float4 *nhi, const uint2 *kids, uint4 *ekid, uint4 *eoff,
-                                                uint32_t *einner) {
+                                                const float4 *nhi, const uint2 *kids, const uint32_t *wd,
+                                                uint4 *ekid, uint4 *eoff, uint32_t *einner) {
     const uint32_t e = blockIdx.x * kBB + threadIdx.x;
     if (e >= qn) return;
     const uint4 q = queue[e];
     uint32_t kid[4] = {kNone, kNone, kNone, kNone}, off[4] = {0, 0, 0, 0};
-    int k;
-    if (is_leaf(nhi, q.x)) {  // only the root can be a leaf here
+    int k = 0;
+    const uint32_t wq = wd[q.x];
+    if (wq & kWideLeaf) {  // only the root can be a leaf here
         kid[0] = q.x;
         off[0] = q.y;
         k = 1;
     } else {
+        // pending (subtree, slots): the slots of all pending entries and of the
+        // current one sum to at most 4, each at least 1, so at most 3 wait
+        uint32_t pn[4], pi[4];
         const uint2 c = kids[q.x];
-        kid[0] = c.x; off[0] = q.y;
-        kid[1] = c.y; off[1] = q.y + count_of(nhi, c.x);
-        k = 2;
-        while (k < 4) {
-            int pick = -1;
-            float best = -1.f;
-            for (int i = 0; i < k; i++) {
-                if (is_leaf(nhi, kid[i])) continue;
-                const float4 lo = nlo[kid[i]], hi = nhi[kid[i]];
-                const float a = half_area(make_float3(lo.x, lo.y, lo.z), make_float3(hi.x, hi.y, hi.z));
-                if (a > best) { best = a; pick = i; }
+        const uint32_t t = uint32_t(wide_top(wq));
+        pn[0] = c.y; pi[0] = 4u - t;
+        pn[1] = c.x; pi[1] = t;
+        int np = 2;
+        uint32_t o = q.y;
+        while (np > 0) {
+            np--;
+            uint32_t x = pn[np], i = pi[np];
+            for (;;) {
+                const uint32_t s = i > 1u ? uint32_t(wide_split(wd[x], int(i))) : 0u;
+                if (i <= 1u) {
+                    if (k < 4) { kid[k] = x; off[k] = o; k++; }
+                    o += count_of(nhi, x);
+                    break;
+                }
+                if (s == 0u || s >= i) { i--; continue; }
+                const uint2 g = kids[x];
+                if (np < 4) { pn[np] = g.y; pi[np] = i - s; np++; }
+                x = g.x;
+                i = s;
             }
-            if (pick < 0) break;
-            const uint2 g = kids[kid[pick]];
-            const uint32_t o = off[pick];
-            kid[pick] = g.x;
-            kid[k] = g.y;
-            off[k] = o + count_of(nhi, g.x);
-            k++;
         }
     }
     // slot order = decreasing area (stable): any-hit traversal visits the slots
@@ -310,7 +364,7 @@ __global__ __launch_bounds__(kBB) void k_expand(const uint4 *queue, uint32_t qn,
             const uint32_t to = off[j]; off[j] = off[j - 1]; off[j - 1] = to;
         }
     uint32_t inner = 0;
-    for (int i = 0; i < k; i++) inner += is_leaf(nhi, kid[i]) ? 0u : 1u;
+    for (int i = 0; i < k; i++) inner += (wd[kid[i]] & kWideLeaf) ? 0u : 1u;
     ekid[e] = make_uint4(kid[0], kid[1], kid[2], kid[3]);  // unused slots kNone
     eoff[e] = make_uint4(off[0], off[1], off[2], off[3]);
     einner[e] = inner;
@@ -323,6 +377,7 @@ struct CollapseArgs {
     const uint32_t *einner, *eincl;
     const float4 *nlo, *nhi;
     const uint2 *kids;
+    const uint32_t *wd;                  // collapse decisions
     uint32_t n;                          // triangles = binary leaves
     const float *verts;
     const uint32_t *tris, *tri_geom;
@@ -354,7 +409,7 @@ __global__ __launch_bounds__(kBB) void k_collapse(CollapseArgs a) {
         const float4 bl = a.nlo[x], bh = a.nhi[x];
         lo[0][i] = bl.x; lo[1][i] = bl.y; lo[2][i] = bl.z;
         hi[0][i] = bh.x; hi[1][i] = bh.y; hi[2][i] = bh.z;
-        if (is_leaf(a.nhi, x)) {
+        if (a.wd[x] & kWideLeaf) {
             const uint32_t cnt = count_of(a.nhi, x);
             child[i] = kLeafBit | ((cnt - 1u) << 28) | off[i];
             count[i] = cnt;
@@ -452,7 +507,8 @@ int build_bvh4_gpu(const float *d_verts, const uint32_t *d_tris, const uint32_t 
     float *partial, *cbox;
     uint64_t *keys, *keys2, *flags, *incl;
     uint32_t *vals, *order, *cl, *cl2, *nearest, *einner, *eincl, *max_stack, *leaf_orig;
-    float4 *nlo, *nhi;
+    float4 *nlo, *nhi, *wc;
+    uint32_t *wd;
     uint2 *kids;
     uint4 *q, *q2, *ekid, *eoff;
     Node4 *nodes_tmp;
@@ -466,6 +522,8 @@ int build_bvh4_gpu(const float *d_verts, const uint32_t *d_tris, const uint32_t 
     WR_HIP(sc.alloc(&nlo, nn));
     WR_HIP(sc.alloc(&nhi, nn));
     WR_HIP(sc.alloc(&kids, nn));
+    WR_HIP(sc.alloc(&wc, nn));
+    WR_HIP(sc.alloc(&wd, nn));
     WR_HIP(sc.alloc(&cl, n));
     WR_HIP(sc.alloc(&cl2, n));
     WR_HIP(sc.alloc(&nearest, n));
@@ -497,7 +555,7 @@ int build_bvh4_gpu(const float *d_verts, const uint32_t *d_tris, const uint32_t 
     WR_HIP(hipGetLastError());
     size_t bytes = sort_bytes;
     WR_HIP(rocprim::radix_sort_pairs(tmp, bytes, keys, keys2, vals, order, n, 0, 63, stream));
-    k_leaves<<<grid_of(n), kBB, 0, stream>>>(d_verts, d_tris, n, order, nlo, nhi, kids, cl);
+    k_leaves<<<grid_of(n), kBB, 0, stream>>>(d_verts, d_tris, n, order, nlo, nhi, kids, wc, wd, cl);
     WR_HIP(hipGetLastError());
 
     // PLOC
@@ -509,7 +567,8 @@ int build_bvh4_gpu(const float *d_verts, const uint32_t *d_tris, const uint32_t 
         WR_HIP(hipGetLastError());
         bytes = scan64_bytes;
         WR_HIP(rocprim::inclusive_scan(tmp, bytes, flags, incl, size_t(m), rocprim::plus<uint64_t>(), stream));
-        k_merge<<<grid_of(m), kBB, 0, stream>>>(cl, nearest, m, flags, incl, node_base, nlo, nhi, kids, cl2);
+        k_merge<<<grid_of(m), kBB, 0, stream>>>(cl, nearest, m, flags, incl, node_base, nlo, nhi, kids, wc, wd,
+                                                cl2);
         WR_HIP(hipGetLastError());
         WR_HIP(hipMemcpyAsync(&total, incl + (m - 1), sizeof total, hipMemcpyDeviceToHost, stream));
         WR_HIP(hipStreamSynchronize(stream));
@@ -534,12 +593,12 @@ int build_bvh4_gpu(const float *d_verts, const uint32_t *d_tris, const uint32_t 
     uint32_t qn = 1, level_base = 0, depth = 0;
     while (qn > 0) {
         const uint32_t next_base = level_base + qn;
-        k_expand<<<grid_of(qn), kBB, 0, stream>>>(q, qn, nlo, nhi, kids, ekid, eoff, einner);
+        k_expand<<<grid_of(qn), kBB, 0, stream>>>(q, qn, nlo, nhi, kids, wd, ekid, eoff, einner);
         WR_HIP(hipGetLastError());
         bytes = scan32_bytes;
         WR_HIP(rocprim::inclusive_scan(tmp, bytes, einner, eincl, size_t(qn), rocprim::plus<uint32_t>(), stream));
         uint32_t inner_total = 0;
-        CollapseArgs ca{q, qn, level_base, next_base, ekid, eoff, einner, eincl, nlo, nhi, kids, n,
+        CollapseArgs ca{q, qn, level_base, next_base, ekid, eoff, einner, eincl, nlo, nhi, kids, wd, n,
                         d_verts, d_tris, d_tri_geom, nodes_tmp, reinterpret_cast<float4 *>(res->tris), leaf_orig,
                         q2, max_stack};
         // this level's nodes end at next_base (checked by the previous level); its
