@@ -298,8 +298,9 @@ struct TravTune {
 #endif
 // The plain closest-hit kernel (k_extend_fast) fits 72 VGPRs without spills,
 // which admits 7 waves per SIMD instead of 6 (3% faster extension on the
-// 1.31M-triangle frame); the counting, batched and any-hit kernels would spill
-// at that size and keep the compiler's choice.
+// 1.31M-triangle frame); the counting and batched kernels would spill at that
+// size and keep the compiler's choice.  The plain any-hit kernel (k_shadow)
+// gets 72 VGPRs and 7 waves by itself: its stack holds node refs only.
 #ifndef WR_EXTEND_WAVES
 #define WR_EXTEND_WAVES 7
 #endif
@@ -877,10 +878,8 @@ __global__ __launch_bounds__(kBlock) WR_TRAV_ATTR void k_shadow(DScene s1, const
                                                                 PathBuffers pb, uint2 *spill, TravTune tune) {
     const uint32_t n = pb.counters[kS];
     if (blockIdx.x * kBlock >= n) return;
-    __shared__ uint32_t st_node[kLdsStackDepth * kBlock];
-    __shared__ float st_t[kLdsStackDepth * kBlock];
-    LaneStack<kBlock> st{st_node + lane_id(), st_t + lane_id(),
-                         spill + size_t(blockIdx.x * kBlock + lane_id()) * kSpillDepth, 0};
+    __shared__ uint32_t st_node[kLdsStackDepth * kBlock];  // any hit: node refs only
+    LaneStack<kBlock> st{st_node + lane_id(), nullptr, spill + size_t(blockIdx.x * kBlock + lane_id()) * kSpillDepth, 0};
     TraversalCounters cnt;
     TravState ts;
     uint32_t i = 0;

@@ -123,6 +123,15 @@ struct LaneStack {
             t = __uint_as_float(e.y);
         }
     }
+    // node refs only (any-hit traversal never culls a popped entry)
+    __device__ __forceinline__ void push_ref(uint32_t ref) {
+        if (sp < kLdsStackDepth) node[sp * kBlock] = ref;
+        else reinterpret_cast<uint32_t *>(spill + (sp - kLdsStackDepth))[0] = ref;
+        sp++;
+    }
+    __device__ __forceinline__ uint32_t top_ref() const {
+        return sp < kLdsStackDepth ? node[sp * kBlock] : reinterpret_cast<const uint32_t *>(spill + (sp - kLdsStackDepth))[0];
+    }
 };
 
 struct TraversalCounters {
@@ -194,7 +203,10 @@ __device__ __forceinline__ bool trav_step(const G &s, TravState &ts, LaneStack<k
 #pragma unroll
         for (int i = 3; i >= 0; i--) {
             if (key[i] != INFINITY) {
-                if (next != kInvalid) st.push(next, next_t);
+                if (next != kInvalid) {
+                    if (ANY) st.push_ref(next);
+                    else st.push(next, next_t);
+                }
                 next = c[i];
                 next_t = key[i];
             }
@@ -214,7 +226,14 @@ __device__ __forceinline__ bool trav_step(const G &s, TravState &ts, LaneStack<k
             }
         }
     }
-    // pop, skipping entries farther than the current hit
+    // pop, skipping entries farther than the current hit (any hit: the far
+    // limit has not moved since the push)
+    if (ANY) {
+        if (st.sp == 0) return true;
+        st.sp--;
+        ts.cur = st.top_ref();
+        return false;
+    }
     for (;;) {
         if (st.sp == 0) return true;
         st.sp--;
