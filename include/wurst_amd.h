@@ -216,6 +216,24 @@ int wr_film_read(wr_film *film, double *base, double *splat);
  * of the NEE + MIS estimator into `film`.  Blocking. */
 int wr_render_nee(wr_scene *scene, wr_film *film, const wr_render_params *params, wr_render_stats *stats);
 
+/* PrimitivePathRenderer::render (primitivepath.h:42-96; JSON "integrator":
+ * "primitive_path_tracer", simple2parser.h:184-187): spp samples per pixel of
+ * BSDF-sampled camera paths of up to num_path_vertices vertices that add the
+ * radiance of every emitter they hit with weight 1 -- no NEE, no light->camera
+ * connections (`use_mis` is ignored, the splat buffer stays 0).  The paths
+ * scatter in the scene's medium as wr_render_nee's do (the reference's sampler
+ * is volumetric here).  Blocking. */
+int wr_render_primitive(wr_scene *scene, wr_film *film, const wr_render_params *params, wr_render_stats *stats);
+
+/* VisualizeGeometryNormal::render (visualizegeomnormal.h:32-74; JSON
+ * "integrator": "visualize", simple2parser.h:196-199): per sample the camera
+ * ray's first hit adds (1, 1, 1) on an emitter and the unit geometric normal
+ * (v1 - v0) x (v2 - v0), unflipped, as RGB on any other surface; misses add 0;
+ * image = base / spp.  Only spp, seed, first_sample and profile are read; the
+ * medium is ignored (the reference's sampler is not volumetric here).
+ * Blocking. */
+int wr_render_visualize(wr_scene *scene, wr_film *film, const wr_render_params *params, wr_render_stats *stats);
+
 /* Closest hit for n host rays (EmbreeAccel::intersect -> rtcIntersect1,
  * embreeaccel.h:353-422).  Misses give geom = prim = WR_INVALID_ID, t = +inf. */
 int wr_trace_closest(wr_scene *scene, const float *rays, uint32_t n, uint32_t *geom, uint32_t *prim, float *t);

@@ -65,6 +65,8 @@ def lib():
         L.orc_scene_destroy.argtypes = [P]
         L.orc_last_error.restype = C.c_char_p
         L.orc_render_nee.argtypes = [P, C.POINTER(OrcRenderParams), P, P, C.POINTER(OrcStats)]
+        L.orc_render_primitive.argtypes = [P, C.POINTER(OrcRenderParams), P, P, C.POINTER(OrcStats)]
+        L.orc_render_visualize.argtypes = [P, C.POINTER(OrcRenderParams), P, P, C.POINTER(OrcStats)]
         L.orc_trace_closest.argtypes = [P, P, C.c_uint32, P, P, P]
         L.orc_trace_any.argtypes = [P, P, C.c_uint32, P]
         L.orc_trace_closest_brute.argtypes = [P, P, C.c_uint32, P, P, P, P, P]
@@ -142,15 +144,20 @@ class OracleScene:
             lib().orc_scene_destroy(self.h)
             self.h = None
 
-    def render(self, spp, num_path_vertices=10, seed=0, use_mis=True, first_sample=0, threads=0, window=None):
-        """Returns (base, splat, stats); image = base/spp + splat/(spp*W*H)."""
+    def render(self, spp, num_path_vertices=10, seed=0, use_mis=True, first_sample=0, threads=0, window=None,
+               integrator="nextevent_path_tracer"):
+        """Returns (base, splat, stats); image = base/spp + splat/(spp*W*H).  `integrator`
+        is the reference's JSON name: nextevent_path_tracer, primitive_path_tracer or
+        visualize."""
+        fn = {"nextevent_path_tracer": lib().orc_render_nee, "primitive_path_tracer": lib().orc_render_primitive,
+              "visualize": lib().orc_render_visualize}[integrator]
         base = np.zeros((self.H, self.W, 3), np.float64)
         splat = np.zeros((self.H, self.W, 3), np.float64)
         p = OrcRenderParams(spp, num_path_vertices, seed, int(bool(use_mis)), first_sample, threads, 0, 0, 0, 0)
         if window is not None:
             p.px0, p.py0, p.px1, p.py1 = window
         st = OrcStats()
-        if lib().orc_render_nee(self.h, C.byref(p), base.ctypes.data, splat.ctypes.data, C.byref(st)) != 0:
+        if fn(self.h, C.byref(p), base.ctypes.data, splat.ctypes.data, C.byref(st)) != 0:
             raise RuntimeError("oracle: " + lib().orc_last_error().decode())
         return base, splat, st
 

@@ -882,8 +882,11 @@ void orc_hg_sample(double g, const double fwd[3], const double u[2], double out[
 
 /* ------------------------------------------------------------------ the estimator */
 /* pathnextevent.h:41-150 with the random walk of generalsubpathsampler.h:13-160.
- * base[3] receives splats[0].mContrib; sp_px, sp_py and sp[3] receive splats[1]. */
-static void eval_contribution(const orc_scene *s, int N, int use_mis, uint32_t key, int px, int py,
+ * base[3] receives splats[0].mContrib; sp_px, sp_py and sp[3] receive splats[1].
+ * nee == 0: PrimitivePathRenderer::evalContribution (primitivepath.h:42-58) on
+ * the same walk -- no light->camera connection, no NEE, every emitter hit
+ * counts with weight 1 (its callback returns mPathContrib * Le). */
+static void eval_contribution(const orc_scene *s, int N, int use_mis, int nee, uint32_t key, int px, int py,
                               double base[3], int *sp_ok, int *sp_px, int *sp_py, double sp[3], counters *cn) {
     base[0] = base[1] = base[2] = 0.0;
     *sp_ok = 0;
@@ -896,7 +899,7 @@ static void eval_contribution(const orc_scene *s, int N, int use_mis, uint32_t k
     v3 cpos = camera_position(s, u01(key, DIM_LENS), u01(key, DIM_LENS + 1), &lens_local);
     v3 cn_ = s->cam_frame.z; /* camera geometry normal (thinlens.h:130) */
     /* t = 1: light -> camera connection (pathnextevent.h:75-99) */
-    {
+    if (nee) {
         v3 uv = vsub(cpos, l.pos);
         double len2 = vlen2(uv);
         v3 nuv = vmul(uv, 1.0 / sqrt(len2));
@@ -980,7 +983,7 @@ static void eval_contribution(const orc_scene *s, int N, int use_mis, uint32_t k
             double len = sqrt(len2);
             v3 nuv = vmul(uv, 1.0 / len);
             double G = fabs(vdot(l.n, nuv)) / len2;
-            if (G != 0.0) {
+            if (nee && G != 0.0) {
                 int occ = occluded(s, p, nuv, RAY_EPS, len - RAY_EPS, cn);
                 if (!occ) {
                     double ph = hg_pdf(s->hg_g, vdot(rd, nuv));
@@ -1031,7 +1034,7 @@ static void eval_contribution(const orc_scene *s, int N, int use_mis, uint32_t k
             double len = sqrt(len2);
             v3 nuv = vmul(uv, 1.0 / len);
             double G = fabs(vdot(ng, nuv)) * fabs(vdot(l.n, nuv)) / len2;
-            if (G != 0.0) {
+            if (nee && G != 0.0) {
                 int occ = occluded(s, p, nuv, RAY_EPS, len - RAY_EPS, cn);
                 if (!occ) {
                     const bsdf_t *b = &s->mats[s->geom_material[g]];
@@ -1058,7 +1061,7 @@ static void eval_contribution(const orc_scene *s, int N, int use_mis, uint32_t k
         } else if (is_light) {
             /* emitter hit (pathnextevent.h:128-146) */
             const light_t *L = &s->lights[s->geom_light[g]];
-            double mis = 0.0;
+            double mis = nee ? 0.0 : 1.0;
             if (use_mis) {
                 double lightPdfA = s->light_cdf.pdf[s->geom_light[g]] * L->inv_area;
                 mis = cPdfA / (lightPdfA + cPdfA);
@@ -1089,10 +1092,32 @@ static void eval_contribution(const orc_scene *s, int N, int use_mis, uint32_t k
     }
 }
 
+/* VisualizeGeometryNormal (visualizegeomnormal.h:45-66): the camera sample's
+ * first hit gives (1, 1, 1) on an emitter, else the unit geometric normal
+ * normalize((p1 - p0) x (p2 - p0)) (embreeaccel.h:418), unflipped; a miss 0. */
+static void visualize_sample(const orc_scene *s, uint32_t key, int px, int py, double out[3], counters *cn) {
+    out[0] = out[1] = out[2] = 0.0;
+    v3 lens_local;
+    v3 cpos = camera_position(s, u01(key, DIM_LENS), u01(key, DIM_LENS + 1), &lens_local);
+    double raster[2], pdfW;
+    v3 dir = camera_direction(s, lens_local, px, py, u01(key, DIM_PIXEL), u01(key, DIM_PIXEL + 1), raster, &pdfW);
+    double t = INFINITY;
+    uint32_t tri = closest(s, cpos, dir, &t, cn);
+    if (tri == UINT32_MAX) return;
+    if (s->geom_light[s->tri_geom[tri]] >= 0) {
+        out[0] = out[1] = out[2] = 1.0;
+        return;
+    }
+    v3 p0 = tri_v(s, tri, 0), p1 = tri_v(s, tri, 1), p2 = tri_v(s, tri, 2);
+    v3 ng = vnorm(vcross(vsub(p1, p0), vsub(p2, p0)));
+    out[0] = ng.x; out[1] = ng.y; out[2] = ng.z;
+}
+
 /* ------------------------------------------------------------------ render driver (pathnextevent.h:152-191) */
 typedef struct {
     const orc_scene *s;
     const orc_render_params *p;
+    int mode;  /* 0: PathNeeRenderer, 1: PrimitivePathRenderer, 2: VisualizeGeometryNormal */
     double *base, *splat;
     int tiles_x, tiles_y, x0, y0, x1, y1;
     volatile int next_tile;
@@ -1131,7 +1156,14 @@ static void *worker(void *arg) {
                     uint32_t key = path_key(p->seed, pixel, sample);
                     double b[3], sp[3];
                     int ok, sx, sy;
-                    eval_contribution(s, p->num_path_vertices, p->use_mis, key, px, py, b, &ok, &sx, &sy, sp, &cn);
+                    if (j->mode == 2) {
+                        visualize_sample(s, key, px, py, b, &cn);
+                        acc[0] += b[0]; acc[1] += b[1]; acc[2] += b[2];
+                        continue;
+                    }
+                    const int nee = j->mode == 0;
+                    eval_contribution(s, p->num_path_vertices, nee ? p->use_mis : 0, nee, key, px, py, b, &ok, &sx, &sy,
+                                      sp, &cn);
                     acc[0] += b[0]; acc[1] += b[1]; acc[2] += b[2];
                     if (ok) {
                         double *dst = j->splat + 3 * ((size_t)sy * s->W + sx);
@@ -1149,13 +1181,14 @@ static void *worker(void *arg) {
     pthread_mutex_unlock(&j->mu);
     return NULL;
 }
-int orc_render_nee(const orc_scene *s, const orc_render_params *p, double *base, double *splat, orc_stats *st) {
+static int render_mode(const orc_scene *s, const orc_render_params *p, int mode, double *base, double *splat,
+                       orc_stats *st) {
     if (!s || !p || !base || !splat) return fail("null argument");
-    if (p->num_path_vertices <= 1) { if (st) memset(st, 0, sizeof *st); return 0; } /* pathnextevent.h:163 */
+    if (mode != 2 && p->num_path_vertices <= 1) { if (st) memset(st, 0, sizeof *st); return 0; } /* pathnextevent.h:163 */
     if (p->spp < 0) return fail("negative spp");
     job_t j;
     memset(&j, 0, sizeof j);
-    j.s = s; j.p = p; j.base = base; j.splat = splat;
+    j.s = s; j.p = p; j.mode = mode; j.base = base; j.splat = splat;
     j.x0 = p->px0; j.y0 = p->py0; j.x1 = p->px1; j.y1 = p->py1;
     if (j.x1 == 0) { j.x0 = 0; j.y0 = 0; j.x1 = s->W; j.y1 = s->H; }
     if (j.x0 < 0 || j.y0 < 0 || j.x1 > s->W || j.y1 > s->H || j.x0 >= j.x1 || j.y0 >= j.y1) return fail("bad pixel window");
@@ -1179,6 +1212,15 @@ int orc_render_nee(const orc_scene *s, const orc_render_params *p, double *base,
         st->seconds = (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
     }
     return 0;
+}
+int orc_render_nee(const orc_scene *s, const orc_render_params *p, double *base, double *splat, orc_stats *st) {
+    return render_mode(s, p, 0, base, splat, st);
+}
+int orc_render_primitive(const orc_scene *s, const orc_render_params *p, double *base, double *splat, orc_stats *st) {
+    return render_mode(s, p, 1, base, splat, st);
+}
+int orc_render_visualize(const orc_scene *s, const orc_render_params *p, double *base, double *splat, orc_stats *st) {
+    return render_mode(s, p, 2, base, splat, st);
 }
 
 /* ------------------------------------------------------------------ trace API */

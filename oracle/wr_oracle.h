@@ -107,6 +107,16 @@ const char *orc_last_error(void);
  * image = base / spp_total + splat / (spp_total * W * H)   (pathnextevent.h:154-157) */
 int orc_render_nee(const orc_scene *s, const orc_render_params *p,
                    double *film_base, double *film_splat, orc_stats *stats);
+/* PrimitivePathRenderer (primitivepath.h:42-96): the same walk without NEE or
+ * light->camera connections, emitter hits at weight 1 (use_mis is ignored;
+ * film_splat is left unchanged). */
+int orc_render_primitive(const orc_scene *s, const orc_render_params *p,
+                         double *film_base, double *film_splat, orc_stats *stats);
+/* VisualizeGeometryNormal (visualizegeomnormal.h:32-74): per sample the primary
+ * hit's unit geometric normal, or (1, 1, 1) on an emitter; image = base / spp.
+ * num_path_vertices and use_mis are ignored; the medium is not traversed. */
+int orc_render_visualize(const orc_scene *s, const orc_render_params *p,
+                         double *film_base, double *film_splat, orc_stats *stats);
 
 /* A homogeneous medium filling the whole scene (src/medium/homogeneous.h:11-66,
  * JSON "medium" {"sigma_a", "sigma_s"}, simple2parser.h:163-167) with a

@@ -1,0 +1,99 @@
+"""Pins the oracle's two other integrators on the same walk by what the
+mathematics fixes.
+
+primitive_path_tracer (PrimitivePathRenderer, primitivepath.h:42-96): BSDF
+sampling only, emitters counted with weight 1.
+  - one bounce (N = 3) over a Lambert plane under a square emitter equals
+    Lambert's polygon formula (the same closed form that pins NEE);
+  - it converges to the NEE+MIS image (both are unbiased for the same integral);
+  - inside a closed box whose walls emit L inward the image is exactly L in
+    vacuum and L in expectation in a purely scattering medium (equilibrium).
+visualize (VisualizeGeometryNormal, visualizegeomnormal.h:32-74):
+  - a directly seen emitter gives exactly 1, its back side too, a miss 0;
+  - a plane gives exactly its unit normal, unflipped (the back side reads the
+    same normal as the front)."""
+import math
+
+import numpy as np
+
+import oracle
+from wurstrenderer_amd import scenes
+
+from test_oracle_math import _plane_reference
+
+
+def test_primitive_one_bounce_matches_lambert_polygon_formula():
+    rho, L = 0.8, 10.0
+    sc = scenes.plane_under_light(resolution=(24, 24), rho=rho, L=L)
+    ref = _plane_reference(sc, rho, L)
+    base, splat, st = oracle.OracleScene(sc).render(4096, num_path_vertices=3, seed=5,
+                                                    integrator="primitive_path_tracer")
+    img = base / 4096
+    assert np.abs(splat).max() == 0.0 and st.shadow_rays == 0
+    # BSDF-sampled hits of the emitter: ~3% noise per pixel at 4096 spp, 0.13% on the mean
+    assert abs(img[..., 0].mean() / ref.mean() - 1) < 0.01
+    assert oracle.rel_mse(img[..., 0], ref) < 3e-3
+    # the emitter weight counted twice, or a lost 1/pi in Le, fails by far
+    assert abs(2 * img[..., 0].mean() / ref.mean() - 1) > 0.5
+    assert abs(img[..., 0].mean() / math.pi / ref.mean() - 1) > 0.5
+
+
+def test_primitive_converges_to_nee():
+    sc = scenes.cornell(1, resolution=(24, 24))
+    osc = oracle.OracleScene(sc)
+    a, _ = osc.image(1024, num_path_vertices=10, seed=1)
+    base, _, _ = osc.render(2048, num_path_vertices=10, seed=2, integrator="primitive_path_tracer")
+    b = base / 2048
+    c, _ = osc.image(1024, num_path_vertices=10, seed=3)
+    noise = np.sqrt((((a - c) ** 2).mean((0, 1)) / a[..., 0].size))
+    print("means", a.mean((0, 1)), b.mean((0, 1)), "nee-seed noise", noise)
+    np.testing.assert_allclose(b.mean((0, 1)), a.mean((0, 1)), rtol=0.01)
+    # without NEE the camera-visible emitter and the lit walls still match
+    assert oracle.rel_mse(a, b) < 2e-2
+
+
+def test_primitive_emitting_box_exact_and_equilibrium():
+    sc = scenes.emitting_box(resolution=(16, 16), L=2.5)
+    osc = oracle.OracleScene(sc)
+    base, _, _ = osc.render(8, num_path_vertices=10, seed=1, integrator="primitive_path_tracer")
+    assert np.all(base / 8 == 2.5)
+    med = sc.with_medium(sigma_a=(0, 0, 0), sigma_s=(0.5, 0.5, 0.5), g=0.7)
+    base, _, st = oracle.OracleScene(med).render(256, num_path_vertices=64, seed=2,
+                                                 integrator="primitive_path_tracer")
+    img = base / 256
+    noise = img[..., 0].std() / math.sqrt(img[..., 0].size)
+    assert abs(img.mean() - 2.5) < 5 * noise + 5e-3, (img.mean(), noise)
+    assert st.shadow_rays == 0
+    # an absorbing medium darkens it
+    dark = sc.with_medium(sigma_a=(0.5, 0.5, 0.5), sigma_s=(0, 0, 0), g=0.0)
+    base, _, _ = oracle.OracleScene(dark).render(16, num_path_vertices=10, seed=3, integrator="primitive_path_tracer")
+    assert base.mean() / 16 < 2.5 * 0.9
+
+
+def _emitter_scene(cam_z):
+    b = scenes.SceneBuilder()
+    b.mesh(*scenes.quad((0, 0, 0), (1, 0, 0), (0, 1, 0)), light_L=(3.0, 2.0, 1.0))  # Ng = +z
+    return b.build(dict(pos=(0, 0, cam_z), lookat=(0, 0, 0), fovy=60.0, resolution=(16, 16)), "emitter")
+
+
+def test_visualize_emitter_and_miss():
+    for z in (3.0, -3.0):  # back side too: any hit on an emitter is a light vertex
+        base, splat, _ = oracle.OracleScene(_emitter_scene(z)).render(64, seed=4, integrator="visualize")
+        img = base / 64
+        assert np.all(img[5:11, 5:11] == 1.0)
+        assert np.all(img[0, 0] == 0.0)
+        assert np.abs(splat).max() == 0.0
+
+
+def test_visualize_plane_normal_unflipped():
+    b = scenes.SceneBuilder()
+    m = b.material(scenes.LAMBERT, rho_d=(0.5, 0.5, 0.5))
+    v, t = scenes.quad((0, 0, 0), (4, 0, 0), (0, 0, 4))  # in y = 0
+    b.mesh(v, t, material=m)
+    b.mesh(*scenes.quad((0, 5, 0), (0.1, 0, 0), (0, 0, 0.1)), light_L=(1, 1, 1))
+    n = np.cross(v[t[0][1]] - v[t[0][0]], v[t[0][2]] - v[t[0][0]])
+    n = n / np.linalg.norm(n)
+    for y in (2.0, -2.0):
+        sc = b.build(dict(pos=(0, y, 0.01), lookat=(0, 0, 0), fovy=40.0, resolution=(8, 8)), "plane")
+        base, _, _ = oracle.OracleScene(sc).render(4, seed=1, integrator="visualize")
+        np.testing.assert_allclose(base / 4, np.broadcast_to(n, (8, 8, 3)), rtol=0, atol=1e-15)

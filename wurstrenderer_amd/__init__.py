@@ -134,6 +134,8 @@ EXPORTS = {
     "wr_film_clear": (C.c_int, [P]),
     "wr_film_read": (C.c_int, [P, P, P]),
     "wr_render_nee": (C.c_int, [P, P, C.POINTER(RenderParams), C.POINTER(RenderStats)]),
+    "wr_render_primitive": (C.c_int, [P, P, C.POINTER(RenderParams), C.POINTER(RenderStats)]),
+    "wr_render_visualize": (C.c_int, [P, P, C.POINTER(RenderParams), C.POINTER(RenderStats)]),
     "wr_trace_closest": (C.c_int, [P, P, C.c_uint32, P, P, P]),
     "wr_trace_any": (C.c_int, [P, P, C.c_uint32, P]),
     "wr_trace_closest_device": (C.c_int, [P, P, C.c_uint32, P, C.POINTER(C.c_float)]),
@@ -380,6 +382,33 @@ def render_nee(scene: GpuScene, film: Film, spp: int, num_path_vertices: int = 1
                       count_traversal, use_graph, fetch_chunk, refill_lanes, tuning)
     st = RenderStats()
     _check(lib().wr_render_nee(scene.h, film.h, C.byref(p), C.byref(st)))
+    film.spp += spp
+    return st
+
+
+def render_primitive(scene: GpuScene, film: Film, spp: int, num_path_vertices: int = 10, seed: int = 0,
+                     first_sample: int | None = None, max_paths_in_flight: int = 0, profile: bool = False,
+                     count_traversal: bool = False) -> RenderStats:
+    """PrimitivePathRenderer::render on the GPU (primitivepath.h:42-96): BSDF-sampled
+    paths that count the emitters they hit, without NEE; adds `spp` samples per pixel."""
+    if first_sample is None:
+        first_sample = film.spp
+    p = render_params(spp, num_path_vertices, seed, False, first_sample, max_paths_in_flight, profile, count_traversal)
+    st = RenderStats()
+    _check(lib().wr_render_primitive(scene.h, film.h, C.byref(p), C.byref(st)))
+    film.spp += spp
+    return st
+
+
+def render_visualize(scene: GpuScene, film: Film, spp: int, seed: int = 0, first_sample: int | None = None,
+                     profile: bool = False) -> RenderStats:
+    """VisualizeGeometryNormal::render on the GPU (visualizegeomnormal.h:32-74): the
+    primary hit's unit geometric normal as RGB, (1, 1, 1) on emitters, averaged over spp."""
+    if first_sample is None:
+        first_sample = film.spp
+    p = render_params(spp, 2, seed, False, first_sample, 0, profile)
+    st = RenderStats()
+    _check(lib().wr_render_visualize(scene.h, film.h, C.byref(p), C.byref(st)))
     film.spp += spp
     return st
 

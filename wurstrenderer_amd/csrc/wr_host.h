@@ -87,6 +87,7 @@ struct RenderArgs {
     uint32_t capacity;        // path pool size
     int N;                    // num_path_vertices
     int use_mis;
+    int nee;                  // 1: PathNeeRenderer (NEE, t = 1 splats); 0: PrimitivePathRenderer (BSDF sampling only)
 };
 
 // ============================================================== host objects

@@ -163,10 +163,34 @@ struct Geo {
     const float4 *tris;
 };
 
+// Lens point of a camera sample (sampleWe0ByPdf, thinlens.h:121-139): the
+// pinhole, or a point of the aperture disk.
+__device__ __forceinline__ float3 lens_point(const DCamera &cam, uint32_t key) {
+    float lx = 0.f, ly = 0.f;
+    if (cam.aperture > 0.f) {
+        disk_from_square(u01(key, kDimLens), u01(key, kDimLens + 1), lx, ly);
+        lx *= cam.aperture; ly *= cam.aperture;
+    }
+    return f3(lx, ly, 0.f);
+}
+// Primary direction through the jittered pixel (samplePerPixelWe1CosByPdf,
+// thinlens.h:141-166) in world space, and its per-pixel pdfW.
+__device__ __forceinline__ float3 primary_dir(const DCamera &cam, uint32_t key, uint32_t px, uint32_t py, float3 lens,
+                                              float &pdfW) {
+    const float j0 = u01(key, kDimPixel), j1 = u01(key, kDimPixel + 1);
+    const float ns0 = (float(px) + j0) * cam.invW, ns1 = (float(py) + j1) * cam.invH;
+    const float3 film = f3((ns0 - 0.5f) * float(cam.W), (ns1 - 0.5f) * float(cam.H), -cam.d);
+    const float3 focal = film * (cam.focal / film.z);
+    const float3 dl = normalize3(focal - lens);
+    pdfW = cam.per_pixel_norm / (dl.z * dl.z * dl.z);
+    return cam_to_world(cam, dl);
+}
+
 // ============================================================== regenerate
 // Fills the pool behind the continuing paths with the next unclaimed samples:
 // createLightSubpath + createCameraSubpath up to the first ray, with the t = 1
-// light->camera connection (pathnextevent.h:53-99; thinlens.h:121-166).
+// light->camera connection (pathnextevent.h:53-99; thinlens.h:121-166), which
+// PrimitivePathRenderer (a.nee == 0) does not make.
 template <bool BATCH>
 __global__ __launch_bounds__(kShadeBlock) void k_regen(DScene s1, const BatchJob *__restrict__ jobs, uint32_t n_jobs,
                                                        PathBuffers pb, RenderArgs a) {
@@ -202,18 +226,13 @@ __global__ __launch_bounds__(kShadeBlock) void k_regen(DScene s1, const BatchJob
             const uint32_t sample = first + local / wh;
             const uint32_t px = pixel % uint32_t(cam.W), py = pixel / uint32_t(cam.W);
             const uint32_t key = path_key(seed, pixel, sample);
-            const LightSample l = sample_light(s, u01(key, kDimLight), u01(key, kDimLight + 1));
             // camera vertex (sampleWe0ByPdf)
-            float lx = 0.f, ly = 0.f;
-            if (cam.aperture > 0.f) {
-                disk_from_square(u01(key, kDimLens), u01(key, kDimLens + 1), lx, ly);
-                lx *= cam.aperture; ly *= cam.aperture;
-            }
-            const float3 lens = f3(lx, ly, 0.f);
+            const float3 lens = lens_point(cam, key);
             const float3 cpos = cam_to_world(cam, lens) + f3(cam.ox, cam.oy, cam.oz);
             const float3 cz = f3(cam.zx, cam.zy, cam.zz);
             // t = 1 connection
-            {
+            if (a.nee) {
+                const LightSample l = sample_light(s, u01(key, kDimLight), u01(key, kDimLight + 1));
                 const float3 uv = cpos - l.pos;
                 const float len2 = dot3(uv, uv);
                 const float len = sqrtf(len2);
@@ -244,13 +263,8 @@ __global__ __launch_bounds__(kShadeBlock) void k_regen(DScene s1, const BatchJob
                 }
             }
             // primary ray (samplePerPixelWe1CosByPdf)
-            const float j0 = u01(key, kDimPixel), j1 = u01(key, kDimPixel + 1);
-            const float ns0 = (float(px) + j0) * cam.invW, ns1 = (float(py) + j1) * cam.invH;
-            const float3 film = f3((ns0 - 0.5f) * float(cam.W), (ns1 - 0.5f) * float(cam.H), -cam.d);
-            const float3 focal = film * (cam.focal / film.z);
-            const float3 dl = normalize3(focal - lens);
-            const float pdfW = cam.per_pixel_norm / (dl.z * dl.z * dl.z);
-            const float3 dir = cam_to_world(cam, dl);
+            float pdfW;
+            const float3 dir = primary_dir(cam, key, px, py, lens, pdfW);
             ro = make_float4(cpos.x, cpos.y, cpos.z, 0.f);
             rdir = make_float4(dir.x, dir.y, dir.z, 0.f);
             thr = make_float4(1.f, 1.f, 1.f, use_mis ? pdfW : 0.f);
@@ -433,11 +447,12 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(DScene s1, const BatchJob
                 const float3 T = xyz(thr);
                 const uint32_t key = meta.x;
                 if (gi.y >= 0) {
-                    // emitter hit: MIS against light sampling (pathnextevent.h:128-146)
+                    // emitter hit: MIS against light sampling (pathnextevent.h:128-146);
+                    // without NEE it is the only estimate (weight 1, primitivepath.h:52-56)
                     const float cos_l = -dot3(ng, rd);
                     if (cos_l >= 0.f) {
                         const DLight &L = s.lights[gi.y];
-                        float mis = 0.f;
+                        float mis = a.nee ? 0.f : 1.f;
                         if (use_mis) {
                             const float cPdfA = thr.w * fabsf(cos_l) / (t * t);
                             const float lightPdfA = L.pdf_select * L.inv_area;
@@ -459,7 +474,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(DScene s1, const BatchJob
                     const float3 nuv = uv * (1.0f / len);
                     const float cos_lv = -dot3(l.n, nuv);
                     const float G = fabsf(dot3(ng, nuv)) * fabsf(cos_lv) / len2;
-                    if (G != 0.f && cos_lv >= 0.f) {
+                    if (a.nee && G != 0.f && cos_lv >= 0.f) {
                         const float3 wo = to_local(f, nuv);
                         const float3 fs = bsdf_eval(m, wi, wo);
                         if (!is_black(fs)) {
@@ -735,7 +750,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade_ext(DScene s, PathBuffers
             const float cos_l = -dot3(ng, rd);
             if (cos_l >= 0.f) {
                 const DLight &L = s.lights[gi.y];
-                float mis = 0.f;
+                float mis = a.nee ? 0.f : 1.f;
                 if (a.use_mis) {
                     float cPdfA = prev_pdfw * fabsf(cos_l) / (seg_t * seg_t);
                     if (MEDIUM) cPdfA *= pass_pdf(s.med, seg_t);
@@ -765,7 +780,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade_ext(DScene s, PathBuffers
             const float3 nuv = uv * (1.0f / len);
             const float cos_lv = -dot3(l.n, nuv);
             const float G = (med ? 1.f : fabsf(dot3(ng, nuv))) * fabsf(cos_lv) / len2;
-            if (G != 0.f && cos_lv >= 0.f) {
+            if (a.nee && G != 0.f && cos_lv >= 0.f) {
                 float3 fs;
                 float p;
                 if (med) {
@@ -968,6 +983,46 @@ __global__ __launch_bounds__(kBlock) void k_trace(DScene s, const float4 *__rest
             if (ANY) occ[i] = tri != kInvalid;
             else hits[i] = make_uint2(tri, __float_as_uint(tri == kInvalid ? INFINITY : t));
         }
+    }
+}
+
+// VisualizeGeometryNormal::render (visualizegeomnormal.h:32-74): per sample
+// the camera ray's first hit adds (1, 1, 1) on an emitter and the unit
+// geometric normal (v1 - v0) x (v2 - v0), unflipped, as RGB on any other
+// surface; a miss adds nothing.  The camera sample (lens point, pixel jitter)
+// is the path tracers' one.  One lane per pixel sums its samples in sample
+// order into the FP64 base film, so the film is bit-reproducible.
+__global__ __launch_bounds__(kBlock) void k_visualize(DScene s, uint32_t wh, uint32_t spp, uint32_t first_sample,
+                                                      uint32_t seed, double *film_base, uint2 *spill) {
+    __shared__ uint32_t st_node[kLdsStackDepth * kBlock];
+    __shared__ float st_t[kLdsStackDepth * kBlock];
+    LaneStack<kBlock> st{st_node + lane_id(), st_t + lane_id(),
+                         spill + size_t(blockIdx.x * kBlock + lane_id()) * kSpillDepth, 0};
+    TraversalCounters cnt;
+    const DCamera &cam = s.cam;
+    for (uint32_t pixel = blockIdx.x * kBlock + lane_id(); pixel < wh; pixel += gridDim.x * kBlock) {
+        const uint32_t px = pixel % uint32_t(cam.W), py = pixel / uint32_t(cam.W);
+        double r = 0.0, g = 0.0, b = 0.0;
+        for (uint32_t k = 0; k < spp; k++) {
+            const uint32_t key = path_key(seed, pixel, first_sample + k);
+            const float3 lens = lens_point(cam, key);
+            const float3 cpos = cam_to_world(cam, lens) + f3(cam.ox, cam.oy, cam.oz);
+            float pdfW;
+            const float3 dir = primary_dir(cam, key, px, py, lens, pdfW);
+            float t = INFINITY;
+            const uint32_t tri = traverse<false, false, kBlock>(s, setup_ray(cpos, dir, kRayEps), t, st, cnt);
+            if (tri == kInvalid) continue;
+            const float4 v0 = s.tris[3 * size_t(tri) + 0], v1 = s.tris[3 * size_t(tri) + 1],
+                         v2 = s.tris[3 * size_t(tri) + 2];
+            if (s.geom_info[__float_as_uint(v1.w)].y >= 0) {
+                r += 1.0; g += 1.0; b += 1.0;
+            } else {
+                const float3 ng = normalize3(cross3(xyz(v1) - xyz(v0), xyz(v2) - xyz(v0)));
+                r += ng.x; g += ng.y; b += ng.z;
+            }
+        }
+        double *dst = film_base + 3 * size_t(pixel);
+        dst[0] += r; dst[1] += g; dst[2] += b;
     }
 }
 
@@ -1597,9 +1652,10 @@ static int drain(wr_scene *s, PathBuffers &pb, uint64_t max_iters, bool use_grap
     return rc;  // an even number of iterations: pb is the current state again
 }
 
-// The frame loop of wr_render_nee and wr_render_guided (g non-null).
+// The frame loop of wr_render_nee, wr_render_guided (g non-null) and
+// wr_render_primitive (nee false).
 static int render_impl(wr_scene *s, wr_film *f, const wr_render_params *prm, wr_guide *g, const wr_guided_params *gp,
-                       wr_render_stats *stats, wr_guide_stats *gstats) {
+                       wr_render_stats *stats, wr_guide_stats *gstats, bool nee = true) {
     if (!s || !f || !prm) return set_error(WR_ERR_INVALID_ARGUMENT, "null argument");
     if (f->scene != s) return set_error(WR_ERR_INVALID_ARGUMENT, "film belongs to another scene");
     if (prm->spp < 0) return set_error(WR_ERR_INVALID_ARGUMENT, "spp must be >= 0");
@@ -1689,7 +1745,8 @@ static int render_impl(wr_scene *s, wr_film *f, const wr_render_params *prm, wr_
         a.seed = prm->seed;
         a.capacity = P;
         a.N = N;
-        a.use_mis = prm->use_mis ? 1 : 0;
+        a.use_mis = nee && prm->use_mis ? 1 : 0;
+        a.nee = nee ? 1 : 0;
         guide::Shade gs{};
         float record_prob = 0.f;
         const bool collect = g && gp->train_steps > 0;
@@ -1798,6 +1855,41 @@ int wr_render_guided(wr_scene *s, wr_film *f, wr_guide *g, const wr_render_param
                      wr_render_stats *stats, wr_guide_stats *gstats) {
     if (!g) return set_error(WR_ERR_INVALID_ARGUMENT, "guide is null");
     return render_impl(s, f, prm, g, gp, stats, gstats);
+}
+
+int wr_render_primitive(wr_scene *s, wr_film *f, const wr_render_params *prm, wr_render_stats *stats) {
+    return render_impl(s, f, prm, nullptr, nullptr, stats, nullptr, false);
+}
+
+int wr_render_visualize(wr_scene *s, wr_film *f, const wr_render_params *prm, wr_render_stats *stats) {
+    if (!s || !f || !prm) return set_error(WR_ERR_INVALID_ARGUMENT, "null argument");
+    if (f->scene != s) return set_error(WR_ERR_INVALID_ARGUMENT, "film belongs to another scene");
+    if (prm->spp < 0) return set_error(WR_ERR_INVALID_ARGUMENT, "spp must be >= 0");
+    const auto t0 = std::chrono::steady_clock::now();
+    int rc = set_device(s->dev);
+    if (rc) return rc;
+    wr_render_stats st{};
+    const uint32_t wh = uint32_t(s->W) * uint32_t(s->H);
+    if (prm->spp > 0) {
+        const int grid = trav_grid(s, wh, s->dev->occ_trace);
+        if (prm->profile) WR_HIP(hipEventRecord(s->ev[0], s->stream));
+        k_visualize<<<grid, kBlock, 0, s->stream>>>(s->ds, wh, uint32_t(prm->spp), uint32_t(prm->first_sample),
+                                                    prm->seed, f->d_base, s->d_spill);
+        WR_HIP(hipGetLastError());
+        if (prm->profile) WR_HIP(hipEventRecord(s->ev[1], s->stream));
+        WR_HIP(hipStreamSynchronize(s->stream));
+        if (prm->profile) {
+            float ms = 0.f;
+            WR_HIP(hipEventElapsedTime(&ms, s->ev[0], s->ev[1]));
+            st.extend_ms = ms;
+            st.extend_launches = 1;
+        }
+    }
+    st.paths = uint64_t(prm->spp) * wh;
+    st.extend_rays = st.paths;
+    st.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (stats) *stats = st;
+    return WR_OK;
 }
 
 // ============================================================== batches (host)
@@ -1911,6 +2003,7 @@ int wr_render_batch(const wr_batch_job *jobs, uint32_t n_jobs, const wr_render_p
         a.total = uint32_t(offset);
         a.capacity = P;
         a.N = maxN;
+        a.nee = 1;
         const uint32_t nt = uint32_t(tab.size());
         k_init<<<1, 64, 0, s0->stream>>>(pb, a);
         WR_HIP(hipGetLastError());

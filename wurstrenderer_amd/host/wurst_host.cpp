@@ -302,6 +302,79 @@ void PathNeeRenderer::render() {
     if (rc != WR_OK) throw std::runtime_error(err);
 }
 
+// A render of one entry point on a fresh GPU scene and film, read back into
+// the host film.
+template <typename Fn>
+static void render_on_gpu(const ThinlensCamera &camera, const SceneData &scene, RenderContext *ctx, Fn &&fn) {
+    const wr_scene_desc d = scene.desc(camera.desc);
+    wr_scene *sc = nullptr;
+    wr_film *f = nullptr;
+    check(wr_scene_create(ctx->get(), &d, &sc));
+    int rc = camera.has_medium ? wr_scene_set_medium(sc, &camera.medium) : WR_OK;
+    if (rc == WR_OK) rc = wr_film_create(sc, &f);
+    if (rc == WR_OK) rc = fn(sc, f);
+    if (rc == WR_OK) rc = wr_film_read(f, camera.film->base.data(), camera.film->splat.data());
+    const std::string err = rc == WR_OK ? "" : wr_last_error();
+    if (f) wr_film_destroy(f);
+    wr_scene_destroy(sc);
+    if (rc != WR_OK) throw std::runtime_error(err);
+}
+
+std::shared_ptr<PrimitivePathRenderer> PrimitivePathRenderer::FromSimple2Json(
+    const std::shared_ptr<ThinlensCamera> &camera, const std::shared_ptr<const SceneData> &scene,
+    const wj::Value &json, RenderContext *ctx) {
+    auto r = std::make_shared<PrimitivePathRenderer>();
+    r->camera = camera;
+    r->scene = scene;
+    r->ctx = ctx;
+    r->num_path_vertices = int(json.number("num_path_vertices", 10));
+    r->num_spp = int(json.number("num_spp", 16));
+    r->seed = uint32_t(json.number("seed", 0));
+    return r;
+}
+
+// render (primitivepath.h:60-94): scales 1/spp and 1/(spp W H)
+void PrimitivePathRenderer::render() {
+    Film &film = *camera->film;
+    film.base_scale = 1.0 / double(num_spp);
+    film.splat_scale = 1.0 / double(num_spp) / (double(film.W) * double(film.H));
+    if (num_path_vertices <= 1 || num_spp <= 0) return;
+    render_on_gpu(*camera, *scene, ctx, [&](wr_scene *sc, wr_film *f) {
+        wr_render_params p{};
+        p.spp = num_spp;
+        p.num_path_vertices = num_path_vertices;
+        p.seed = seed;
+        return wr_render_primitive(sc, f, &p, &stats);
+    });
+}
+
+std::shared_ptr<VisualizeGeometryNormal> VisualizeGeometryNormal::FromSimple2Json(
+    const std::shared_ptr<ThinlensCamera> &camera, const std::shared_ptr<const SceneData> &scene,
+    const wj::Value &json, RenderContext *ctx) {
+    auto r = std::make_shared<VisualizeGeometryNormal>();
+    r->camera = camera;
+    r->scene = scene;
+    r->ctx = ctx;
+    r->num_spp = int(json.number("num_spp", 16));
+    r->seed = uint32_t(json.number("seed", 0));
+    return r;
+}
+
+// render (visualizegeomnormal.h:32-74): the reference adds the per-pixel mean
+// at scale 1; the device sums the samples, so the host scales by 1/spp.
+void VisualizeGeometryNormal::render() {
+    Film &film = *camera->film;
+    film.base_scale = num_spp > 0 ? 1.0 / double(num_spp) : 1.0;
+    film.splat_scale = 1.0;
+    if (num_spp <= 0) return;
+    render_on_gpu(*camera, *scene, ctx, [&](wr_scene *sc, wr_film *f) {
+        wr_render_params p{};
+        p.spp = num_spp;
+        p.seed = seed;
+        return wr_render_visualize(sc, f, &p, &stats);
+    });
+}
+
 // ------------------------------------------------------------------ scene JSON
 namespace {
 
@@ -313,8 +386,8 @@ void dump_array(const std::string &dir, const char *name, const std::vector<T> &
 }
 
 // integrator names the reference's dispatch knows (simple2parser.h:176-207)
-const char *kOtherIntegrators[] = {"bidirectional_path_tracer", "photon_map", "primitive_path_tracer", "light_splatting",
-                                   "visualize", "diffuse_prt", "glossy_prt2002"};
+const char *kOtherIntegrators[] = {"bidirectional_path_tracer", "photon_map", "light_splatting", "diffuse_prt",
+                                   "glossy_prt2002"};
 
 }  // namespace
 
@@ -379,6 +452,10 @@ void Simple2Parser::Parse(const wj::Value &j, const std::string &root_dir, const
         std::shared_ptr<Renderer> renderer;
         if (name == "nextevent_path_tracer") {
             renderer = PathNeeRenderer::FromSimple2Json(camera, scene, rj, &ctx);
+        } else if (name == "primitive_path_tracer") {
+            renderer = PrimitivePathRenderer::FromSimple2Json(camera, scene, rj, &ctx);
+        } else if (name == "visualize") {
+            renderer = VisualizeGeometryNormal::FromSimple2Json(camera, scene, rj, &ctx);
         } else {
             for (const char *other : kOtherIntegrators)
                 if (name == other) throw std::runtime_error("integrator \"" + name + "\" is not part of this build");
@@ -398,6 +475,16 @@ void Simple2Parser::Parse(const wj::Value &j, const std::string &root_dir, const
                         (unsigned long long)nee->stats.extend_rays, (unsigned long long)nee->stats.shadow_rays,
                         nee->stats.wall_ms, nee->guided ? "true" : "false",
                         (unsigned long long)nee->guide_stats.train_steps);
+        } else {
+            const wr_render_stats *st = nullptr;
+            int spp = 0;
+            if (auto *pr = dynamic_cast<PrimitivePathRenderer *>(renderer.get())) { st = &pr->stats; spp = pr->num_spp; }
+            if (auto *vz = dynamic_cast<VisualizeGeometryNormal *>(renderer.get())) { st = &vz->stats; spp = vz->num_spp; }
+            if (st)
+                std::printf("{\"integrator\": \"%s\", \"output\": \"%s\", \"spp\": %d, \"paths\": %llu, "
+                            "\"extend_rays\": %llu, \"ms\": %.3f}\n",
+                            name.c_str(), output.c_str(), spp, (unsigned long long)st->paths,
+                            (unsigned long long)st->extend_rays, st->wall_ms);
         }
     }
 }

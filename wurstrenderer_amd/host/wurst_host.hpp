@@ -1,4 +1,5 @@
-// wurst_host.hpp -- C++ host of the MI355X `nextevent_path_tracer`.
+// wurst_host.hpp -- C++ host of the MI355X `nextevent_path_tracer` (and of
+// `primitive_path_tracer` and `visualize` on the same kernels).
 //
 // Mirrors the reference's scene-JSON entry point (Simple2Parser,
 // src/sceneio/simple2parser.h:84-223), its ThinlensCamera JSON reader
@@ -89,6 +90,40 @@ struct PathNeeRenderer : Renderer {
     wr_guide_config guide_config{};
     wr_render_stats stats{};
     wr_guide_stats guide_stats{};
+};
+
+// PrimitivePathRenderer (src/renderer/primitivepath.h:15-100) on the GPU:
+// BSDF-sampled paths that count the emitters they hit; JSON keys
+// "num_path_vertices" (10), "num_spp" (16), "seed".
+struct PrimitivePathRenderer : Renderer {
+    static std::shared_ptr<PrimitivePathRenderer> FromSimple2Json(const std::shared_ptr<ThinlensCamera> &camera,
+                                                                  const std::shared_ptr<const SceneData> &scene,
+                                                                  const wj::Value &json, RenderContext *ctx);
+    void render() override;
+
+    std::shared_ptr<ThinlensCamera> camera;
+    std::shared_ptr<const SceneData> scene;
+    RenderContext *ctx = nullptr;
+    int num_path_vertices = 10, num_spp = 16;
+    uint32_t seed = 0;
+    wr_render_stats stats{};
+};
+
+// VisualizeGeometryNormal (src/renderer/visualizegeomnormal.h:14-76) on the
+// GPU: the primary hit's geometric normal as RGB, 1 on emitters; JSON keys
+// "num_spp" (16), "seed".
+struct VisualizeGeometryNormal : Renderer {
+    static std::shared_ptr<VisualizeGeometryNormal> FromSimple2Json(const std::shared_ptr<ThinlensCamera> &camera,
+                                                                    const std::shared_ptr<const SceneData> &scene,
+                                                                    const wj::Value &json, RenderContext *ctx);
+    void render() override;
+
+    std::shared_ptr<ThinlensCamera> camera;
+    std::shared_ptr<const SceneData> scene;
+    RenderContext *ctx = nullptr;
+    int num_spp = 16;
+    uint32_t seed = 0;
+    wr_render_stats stats{};
 };
 
 struct ParseOptions {
