@@ -225,6 +225,16 @@ int wr_render_nee(wr_scene *scene, wr_film *film, const wr_render_params *params
  * is volumetric here).  Blocking. */
 int wr_render_primitive(wr_scene *scene, wr_film *film, const wr_render_params *params, wr_render_stats *stats);
 
+/* LightSplattingRenderer::render (lightsplatting.h:37-113; JSON "integrator":
+ * "light_splatting", simple2parser.h:188-191): spp * W * H light subpaths of up
+ * to num_path_vertices - 1 vertices (light vertex, then cosine emission and BSDF
+ * sampling; an emitter hit ends one); the light vertex and every surface vertex
+ * connect to the camera with weight 1 and add G We1 f T to the splat buffer at
+ * the camera raster: image = splat / (spp * W * H), the base buffer stays 0.
+ * `use_mis` is ignored; the medium is ignored (the reference's sampler here is
+ * not volumetric).  Blocking. */
+int wr_render_light(wr_scene *scene, wr_film *film, const wr_render_params *params, wr_render_stats *stats);
+
 /* VisualizeGeometryNormal::render (visualizegeomnormal.h:32-74; JSON
  * "integrator": "visualize", simple2parser.h:196-199): per sample the camera
  * ray's first hit adds (1, 1, 1) on an emitter and the unit geometric normal

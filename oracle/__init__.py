@@ -67,6 +67,7 @@ def lib():
         L.orc_render_nee.argtypes = [P, C.POINTER(OrcRenderParams), P, P, C.POINTER(OrcStats)]
         L.orc_render_primitive.argtypes = [P, C.POINTER(OrcRenderParams), P, P, C.POINTER(OrcStats)]
         L.orc_render_visualize.argtypes = [P, C.POINTER(OrcRenderParams), P, P, C.POINTER(OrcStats)]
+        L.orc_render_light.argtypes = [P, C.POINTER(OrcRenderParams), P, P, C.POINTER(OrcStats)]
         L.orc_trace_closest.argtypes = [P, P, C.c_uint32, P, P, P]
         L.orc_trace_any.argtypes = [P, P, C.c_uint32, P]
         L.orc_trace_closest_brute.argtypes = [P, P, C.c_uint32, P, P, P, P, P]
@@ -147,10 +148,10 @@ class OracleScene:
     def render(self, spp, num_path_vertices=10, seed=0, use_mis=True, first_sample=0, threads=0, window=None,
                integrator="nextevent_path_tracer"):
         """Returns (base, splat, stats); image = base/spp + splat/(spp*W*H).  `integrator`
-        is the reference's JSON name: nextevent_path_tracer, primitive_path_tracer or
-        visualize."""
+        is the reference's JSON name: nextevent_path_tracer, primitive_path_tracer,
+        light_splatting or visualize."""
         fn = {"nextevent_path_tracer": lib().orc_render_nee, "primitive_path_tracer": lib().orc_render_primitive,
-              "visualize": lib().orc_render_visualize}[integrator]
+              "light_splatting": lib().orc_render_light, "visualize": lib().orc_render_visualize}[integrator]
         base = np.zeros((self.H, self.W, 3), np.float64)
         splat = np.zeros((self.H, self.W, 3), np.float64)
         p = OrcRenderParams(spp, num_path_vertices, seed, int(bool(use_mis)), first_sample, threads, 0, 0, 0, 0)

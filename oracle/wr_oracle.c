@@ -1113,18 +1113,6 @@ static void visualize_sample(const orc_scene *s, uint32_t key, int px, int py, d
     out[0] = ng.x; out[1] = ng.y; out[2] = ng.z;
 }
 
-/* ------------------------------------------------------------------ render driver (pathnextevent.h:152-191) */
-typedef struct {
-    const orc_scene *s;
-    const orc_render_params *p;
-    int mode;  /* 0: PathNeeRenderer, 1: PrimitivePathRenderer, 2: VisualizeGeometryNormal */
-    double *base, *splat;
-    int tiles_x, tiles_y, x0, y0, x1, y1;
-    volatile int next_tile;
-    counters total;
-    pthread_mutex_t mu;
-} job_t;
-#define TILE 16
 static void atomic_add_double(double *addr, double v) {
     uint64_t *a = (uint64_t *)addr;
     uint64_t oldb = __atomic_load_n(a, __ATOMIC_RELAXED), newb;
@@ -1134,6 +1122,91 @@ static void atomic_add_double(double *addr, double v) {
         memcpy(&newb, &n, 8);
     } while (!__atomic_compare_exchange_n(a, &oldb, newb, 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED));
 }
+/* LightSplattingRenderer (lightsplatting.h:58-108) with createLightSubpath
+ * (generalsubpathsampler.h:185-268) and its non-volumetric randomWalk: a light
+ * subpath of up to N - 1 vertices; its light vertex and every surface vertex
+ * connect to the camera vertex with weight 1, adding
+ * c0.pathContrib (1) * We1 * G * visibility * scatter * pathContrib to the
+ * splat film at the camera raster.  An emitter hit ends the walk. */
+static void splat_add(const orc_scene *s, double *splat, const double raster[2], rgb c) {
+    double *dst = splat + 3 * ((size_t)(int)raster[1] * s->W + (size_t)(int)raster[0]);
+    atomic_add_double(dst + 0, c.r);
+    atomic_add_double(dst + 1, c.g);
+    atomic_add_double(dst + 2, c.b);
+}
+static void light_subpath(const orc_scene *s, int N, uint32_t key, double *splat, counters *cn) {
+    if (N < 2) return;
+    lvert l;
+    sample_light(s, u01(key, DIM_LIGHT), u01(key, DIM_LIGHT + 1), &l);
+    v3 lens_local;
+    v3 cpos = camera_position(s, u01(key, DIM_LENS), u01(key, DIM_LENS + 1), &lens_local);
+    v3 cz = s->cam_frame.z;
+    /* the light vertex (epCallbackFunc, lightsplatting.h:77-89) */
+    {
+        v3 uv = vsub(cpos, l.pos);
+        double len2 = vlen2(uv), len = sqrt(len2);
+        v3 nuv = vmul(uv, 1.0 / len);
+        double G = fabs(vdot(l.n, nuv)) * fabs(vdot(cz, nuv)) / len2;
+        double raster[2];
+        double we1 = G != 0.0 ? camera_we1(s, cpos, vnorm(vsub(l.pos, cpos)), raster) : 0.0;
+        v3 wl = to_local(&l.f, nuv);
+        double le1 = wl.y < 0.0 ? 0.0 : INV_PI;
+        if (we1 != 0.0 && le1 != 0.0 && !occluded(s, l.pos, nuv, RAY_EPS, len - RAY_EPS, cn))
+            splat_add(s, splat, raster, cscale(l.contrib, we1 * G * le1));
+    }
+    if (N - 1 == 1) return;
+    /* Le1: cosine-distributed direction in the light's frame, contribution 1 */
+    v3 ro = l.pos, rd = to_world(&l.f, cosine_hemisphere(u01(key, DIM_PIXEL), u01(key, DIM_PIXEL + 1)));
+    rgb T = l.contrib;
+    for (int bounce = 0;; bounce++) {
+        const int k = bounce + 2; /* vertex of the light subpath */
+        double t = INFINITY;
+        uint32_t tri = closest(s, ro, rd, &t, cn);
+        if (tri == UINT32_MAX) break;
+        uint32_t g = s->tri_geom[tri];
+        if (s->geom_light[g] >= 0) break; /* callback adds nothing; the walk ends at a light */
+        v3 p = vadd(ro, vmul(rd, t));
+        v3 p0 = tri_v(s, tri, 0), p1 = tri_v(s, tri, 1), p2 = tri_v(s, tri, 2);
+        v3 ng = vnorm(vcross(vsub(p1, p0), vsub(p2, p0)));
+        frame3 f = make_frame(ng);
+        const bsdf_t *b = &s->mats[s->geom_material[g]];
+        v3 wi = to_local(&f, vneg(rd));
+        /* connection to the camera (lightsplatting.h:91-105) */
+        v3 uv = vsub(cpos, p);
+        double len2 = vlen2(uv), len = sqrt(len2);
+        v3 nuv = vmul(uv, 1.0 / len);
+        double G = fabs(vdot(ng, nuv)) * fabs(vdot(cz, nuv)) / len2;
+        double raster[2];
+        double we1 = G != 0.0 ? camera_we1(s, cpos, vnorm(vsub(p, cpos)), raster) : 0.0;
+        if (we1 != 0.0) {
+            rgb fs = bsdf_eval(b, wi, to_local(&f, nuv));
+            if (!czero(fs) && !occluded(s, p, nuv, RAY_EPS, len - RAY_EPS, cn))
+                splat_add(s, splat, raster, cscale(cmul(T, fs), we1 * G));
+        }
+        if (k == N - 1) break;
+        uint32_t dim = DIM_BOUNCE0 + DIMS_PER_BOUNCE * (uint32_t)bounce + BD_BSDF;
+        v3 wo;
+        double pdf = 0.0;
+        rgb w = bsdf_sample(b, wi, u01(key, dim), u01(key, dim + 1), &wo, &pdf);
+        if (czero(w)) break;
+        T = cmul(T, w);
+        ro = p;
+        rd = to_world(&f, wo);
+    }
+}
+
+/* ------------------------------------------------------------------ render driver (pathnextevent.h:152-191) */
+typedef struct {
+    const orc_scene *s;
+    const orc_render_params *p;
+    int mode;  /* 0: PathNeeRenderer, 1: PrimitivePathRenderer, 2: VisualizeGeometryNormal, 3: LightSplattingRenderer */
+    double *base, *splat;
+    int tiles_x, tiles_y, x0, y0, x1, y1;
+    volatile int next_tile;
+    counters total;
+    pthread_mutex_t mu;
+} job_t;
+#define TILE 16
 static void *worker(void *arg) {
     job_t *j = (job_t *)arg;
     const orc_scene *s = j->s;
@@ -1159,6 +1232,10 @@ static void *worker(void *arg) {
                     if (j->mode == 2) {
                         visualize_sample(s, key, px, py, b, &cn);
                         acc[0] += b[0]; acc[1] += b[1]; acc[2] += b[2];
+                        continue;
+                    }
+                    if (j->mode == 3) {
+                        light_subpath(s, p->num_path_vertices, key, j->splat, &cn);
                         continue;
                     }
                     const int nee = j->mode == 0;
@@ -1221,6 +1298,9 @@ int orc_render_primitive(const orc_scene *s, const orc_render_params *p, double 
 }
 int orc_render_visualize(const orc_scene *s, const orc_render_params *p, double *base, double *splat, orc_stats *st) {
     return render_mode(s, p, 2, base, splat, st);
+}
+int orc_render_light(const orc_scene *s, const orc_render_params *p, double *base, double *splat, orc_stats *st) {
+    return render_mode(s, p, 3, base, splat, st);
 }
 
 /* ------------------------------------------------------------------ trace API */

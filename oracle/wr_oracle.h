@@ -117,6 +117,11 @@ int orc_render_primitive(const orc_scene *s, const orc_render_params *p,
  * num_path_vertices and use_mis are ignored; the medium is not traversed. */
 int orc_render_visualize(const orc_scene *s, const orc_render_params *p,
                          double *film_base, double *film_splat, orc_stats *stats);
+/* LightSplattingRenderer (lightsplatting.h:37-113): spp * W * H light subpaths
+ * connected to the camera; image = splat / (spp * W * H).  use_mis and the
+ * medium are ignored (the reference's sampler there is not volumetric). */
+int orc_render_light(const orc_scene *s, const orc_render_params *p,
+                     double *film_base, double *film_splat, orc_stats *stats);
 
 /* A homogeneous medium filling the whole scene (src/medium/homogeneous.h:11-66,
  * JSON "medium" {"sigma_a", "sigma_s"}, simple2parser.h:163-167) with a

@@ -48,22 +48,26 @@ def test_cli_render_equals_api_and_oracle(tmp_path, gpu_device):
 
 
 def test_cli_primitive_and_visualize_equal_api(tmp_path, gpu_device):
-    """One scene JSON with the three integrators of this build: each PFM equals
+    """One scene JSON with the other integrators of this build: each PFM equals
     the Python API's render of the same scene, integrator and seed."""
     sc = scenes.cornell(2, resolution=(48, 40))
     scenes.write_obj(sc, str(tmp_path), "mesh")
     j = scenes.scene_json(sc, "mesh.obj", [
         {"integrator": "primitive_path_tracer", "output": "prim.pfm", "num_spp": 8, "num_path_vertices": 10, "seed": 5},
+        {"integrator": "light_splatting", "output": "light.pfm", "num_spp": 4, "num_path_vertices": 8, "seed": 7},
         {"integrator": "visualize", "output": "vis.pfm", "num_spp": 4, "seed": 6}])
     (tmp_path / "scene.json").write_text(json.dumps(j))
     r = subprocess.run([EXE, "--simple2parse", str(tmp_path / "scene.json")], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
-    lines = [json.loads(x) for x in r.stdout.strip().splitlines()[-2:]]
-    assert [x["integrator"] for x in lines] == ["primitive_path_tracer", "visualize"]
-    assert lines[0]["paths"] == 48 * 40 * 8 and lines[1]["paths"] == 48 * 40 * 4
+    lines = [json.loads(x) for x in r.stdout.strip().splitlines()[-3:]]
+    assert [x["integrator"] for x in lines] == ["primitive_path_tracer", "light_splatting", "visualize"]
+    assert lines[0]["paths"] == 48 * 40 * 8 and lines[2]["paths"] == 48 * 40 * 4
     gs = wr.GpuScene(gpu_device, sc)
     fp, fv = wr.Film(gs), wr.Film(gs)
     wr.render_primitive(gs, fp, 8, num_path_vertices=10, seed=5)
     wr.render_visualize(gs, fv, 4, seed=6)
     np.testing.assert_allclose(read_pfm(tmp_path / "prim.pfm"), fp.image().astype(np.float32), rtol=1e-6, atol=1e-7)
     np.testing.assert_allclose(read_pfm(tmp_path / "vis.pfm"), fv.image().astype(np.float32), rtol=1e-6, atol=1e-7)
+    fl = wr.Film(gs)
+    wr.render_light(gs, fl, 4, num_path_vertices=8, seed=7)  # FP64 splat atomics: summation order only
+    np.testing.assert_allclose(read_pfm(tmp_path / "light.pfm"), fl.image().astype(np.float32), rtol=1e-5, atol=1e-6)

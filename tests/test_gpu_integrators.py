@@ -1,4 +1,4 @@
-"""GPU primitive_path_tracer and visualize against the FP64 oracle.
+"""GPU primitive_path_tracer, light_splatting and visualize against the FP64 oracle.
 
 Both sides draw the same counter-based random numbers per (pixel, sample,
 dimension), so at equal seeds the images follow the same paths and agree pixel
@@ -9,6 +9,11 @@ by pixel up to FP32-vs-FP64 arithmetic.  Bars, written per test:
     the emitter moves its pixel by L T / spp, and the seed-to-seed noise of the
     cornell mean is ~1.7%), extension-ray counts within 0.5%, no shadow rays and
     an empty splat film; inside a box of emitting walls the image is exactly L;
+  - light_splatting: relMSE(gpu, oracle) at the same seed at least 30x below
+    the seed noise, means within 3e-3, extension- and shadow-ray counts within
+    0.5%, an empty base film; GPU light tracing and GPU NEE+MIS agree (means
+    1%); a medium changes nothing (the reference's light sampler is not
+    volumetric);
   - visualize: per pixel |gpu - oracle| <= 1e-5 on at least 99% of the pixels
     (the rest are pixels where an FP32 rounding moves a sample's hit across a
     triangle edge), exact 1 on a seen emitter and 0 on misses."""
@@ -117,3 +122,60 @@ def test_integrator_argument_errors(gpu_device):
         wr.render_visualize(ga, fa, -1)
     assert wr.render_primitive(ga, fa, 4, num_path_vertices=1).paths == 0
     assert not fa.read()[0].any()
+
+
+# ----------------------------------------------------------------- light_splatting
+def _light(dev, sc, spp, N, seed, noise_factor, mean_rtol=3e-3):
+    print(f"light tracer matched render: {sc.name} {sc.resolution} spp={spp} N={N}", flush=True)
+    W, H = sc.resolution
+    gs = wr.GpuScene(dev, sc)
+    film = wr.Film(gs)
+    st = wr.render_light(gs, film, spp, num_path_vertices=N, seed=seed)
+    base, splat = film.read()
+    img = film.image()
+    osc = oracle.OracleScene(sc)
+    _, rs, ost = osc.render(spp, num_path_vertices=N, seed=seed, integrator="light_splatting")
+    _, os_, _ = osc.render(spp, num_path_vertices=N, seed=seed + 1000, integrator="light_splatting")
+    ref, other = rs / (spp * W * H), os_ / (spp * W * H)
+    noise, diff = oracle.rel_mse(other, ref), oracle.rel_mse(img, ref)
+    print(f"  relMSE(gpu, oracle)={diff:.3e}  noise={noise:.3e}  rays gpu={st.extend_rays}/{st.shadow_rays} "
+          f"oracle={ost.extend_rays}/{ost.shadow_rays}")
+    assert not base.any() and np.isfinite(img).all()
+    assert diff * noise_factor < noise, (diff, noise)
+    np.testing.assert_allclose(img.mean((0, 1)), ref.mean((0, 1)), rtol=mean_rtol)
+    assert abs(st.extend_rays / ost.extend_rays - 1) < 5e-3
+    assert abs(st.shadow_rays / ost.shadow_rays - 1) < 5e-3
+    return img, ref
+
+
+def test_light_tracer_cornell_matches_oracle(gpu_device):
+    _light(gpu_device, scenes.cornell(1, resolution=(64, 64)), 32, 10, seed=3, noise_factor=30.0)
+
+
+def test_light_tracer_all_materials_match_oracle(gpu_device):
+    _light(gpu_device, _all_materials_scene(), 32, 10, seed=5, noise_factor=30.0)
+
+
+def test_light_tracer_thin_lens_matches_oracle(gpu_device):
+    sc = scenes.cornell(1, resolution=(48, 48)).with_camera(aperture_radius=0.05, focal_distance=3.5)
+    _light(gpu_device, sc, 32, 8, seed=6, noise_factor=30.0)
+
+
+def test_light_tracer_and_nee_agree_on_the_gpu(gpu_device):
+    sc = scenes.cornell(1, resolution=(32, 32))
+    gs = wr.GpuScene(gpu_device, sc)
+    fa, fb = wr.Film(gs), wr.Film(gs)
+    wr.render_nee(gs, fa, 1024, num_path_vertices=10, seed=1)
+    wr.render_light(gs, fb, 1024, num_path_vertices=10, seed=2)
+    np.testing.assert_allclose(fb.image().mean((0, 1)), fa.image().mean((0, 1)), rtol=0.01)
+    assert oracle.rel_mse(fa.image(), fb.image()) < 5e-3
+
+
+def test_light_tracer_ignores_the_medium(gpu_device):
+    sc = scenes.cornell(1, resolution=(32, 32))
+    ga = wr.GpuScene(gpu_device, sc)
+    gb = wr.GpuScene(gpu_device, sc.with_medium(sigma_a=(0.3, 0.3, 0.3), sigma_s=(0.3, 0.3, 0.3), g=0.5))
+    fa, fb = wr.Film(ga), wr.Film(gb)
+    wr.render_light(ga, fa, 16, num_path_vertices=10, seed=4)
+    wr.render_light(gb, fb, 16, num_path_vertices=10, seed=4)
+    np.testing.assert_allclose(fb.read()[1], fa.read()[1], rtol=1e-9, atol=1e-12)  # FP64 atomics: order only

@@ -152,14 +152,16 @@ def test_medium_and_guiding_keys_are_accepted(tmp_path):
 
 
 def test_other_integrators_of_this_build_parse(tmp_path):
-    """primitive_path_tracer and visualize are dispatched by name as the
-    reference's parser does (simple2parser.h:184-199); the rest stay rejected."""
+    """primitive_path_tracer, light_splatting and visualize are dispatched by
+    name as the reference's parser does (simple2parser.h:184-199); the rest
+    stay rejected."""
     sc = scenes.cornell(1, resolution=(16, 16))
     p = write(tmp_path, sc, renders=({"integrator": "primitive_path_tracer", "num_spp": 2, "num_path_vertices": 5},
+                                     {"integrator": "light_splatting", "num_spp": 2, "num_path_vertices": 5},
                                      {"integrator": "visualize", "num_spp": 2}))
     r = run(["--check", "--simple2parse", str(p)])
     assert r.returncode == 0, r.stderr
-    for name in ("photon_map", "light_splatting", "diffuse_prt"):
+    for name in ("photon_map", "bidirectional_path_tracer", "diffuse_prt"):
         p = write(tmp_path, sc, renders=({"integrator": name},))
         r = run(["--check", "--simple2parse", str(p)])
         assert r.returncode != 0 and "not part of this build" in r.stderr

@@ -97,3 +97,45 @@ def test_visualize_plane_normal_unflipped():
         sc = b.build(dict(pos=(0, y, 0.01), lookat=(0, 0, 0), fovy=40.0, resolution=(8, 8)), "plane")
         base, _, _ = oracle.OracleScene(sc).render(4, seed=1, integrator="visualize")
         np.testing.assert_allclose(base / 4, np.broadcast_to(n, (8, 8, 3)), rtol=0, atol=1e-15)
+
+
+# ----------------------------------------------------------------- light_splatting
+def test_light_tracer_one_bounce_matches_lambert_polygon_formula():
+    """N = 3: light vertex (behind the camera: no film position) -> plane vertex
+    -> camera; the splatted image is the plane's direct lighting rho E / pi."""
+    rho, L = 0.8, 10.0
+    sc = scenes.plane_under_light(resolution=(24, 24), rho=rho, L=L)
+    ref = _plane_reference(sc, rho, L)
+    spp = 4096
+    base, splat, st = oracle.OracleScene(sc).render(spp, num_path_vertices=3, seed=5, integrator="light_splatting")
+    img = splat / (spp * 24 * 24)
+    assert np.abs(base).max() == 0.0
+    assert abs(img[..., 0].mean() / ref.mean() - 1) < 0.01
+    # noise-limited (most light paths land outside the view): 9e-3 at 1024 spp, 1/spp
+    assert oracle.rel_mse(img[..., 0], ref) < 5e-3
+    assert abs(img[..., 0].mean() * math.pi / ref.mean() - 1) > 0.5
+
+
+def test_light_tracer_sees_the_emitter():
+    """N = 2: only light vertices connect; inside the emitter's projection the
+    image is L in expectation, outside 0, and the back side shows nothing."""
+    spp = 256
+    sc = _emitter_scene(3.0)
+    _, splat, _ = oracle.OracleScene(sc).render(spp, num_path_vertices=2, seed=9, integrator="light_splatting")
+    img = splat / (spp * 16 * 16)
+    np.testing.assert_allclose(img[5:11, 5:11].mean((0, 1)), [3.0, 2.0, 1.0], rtol=0.02)
+    assert np.abs(img[0, 0]).max() == 0.0
+    _, splat, _ = oracle.OracleScene(_emitter_scene(-3.0)).render(16, num_path_vertices=2, seed=9,
+                                                                  integrator="light_splatting")
+    assert np.abs(splat).max() == 0.0
+
+
+def test_light_tracer_converges_to_nee():
+    sc = scenes.cornell(1, resolution=(24, 24))
+    osc = oracle.OracleScene(sc)
+    a, _ = osc.image(1024, num_path_vertices=10, seed=1)
+    _, splat, st = osc.render(1024, num_path_vertices=10, seed=2, integrator="light_splatting")
+    b = splat / (1024 * 24 * 24)
+    np.testing.assert_allclose(b.mean((0, 1)), a.mean((0, 1)), rtol=0.01)
+    assert oracle.rel_mse(a, b) < 5e-3
+    assert st.extend_rays > 0 and st.shadow_rays > 0

@@ -136,6 +136,7 @@ EXPORTS = {
     "wr_render_nee": (C.c_int, [P, P, C.POINTER(RenderParams), C.POINTER(RenderStats)]),
     "wr_render_primitive": (C.c_int, [P, P, C.POINTER(RenderParams), C.POINTER(RenderStats)]),
     "wr_render_visualize": (C.c_int, [P, P, C.POINTER(RenderParams), C.POINTER(RenderStats)]),
+    "wr_render_light": (C.c_int, [P, P, C.POINTER(RenderParams), C.POINTER(RenderStats)]),
     "wr_trace_closest": (C.c_int, [P, P, C.c_uint32, P, P, P]),
     "wr_trace_any": (C.c_int, [P, P, C.c_uint32, P]),
     "wr_trace_closest_device": (C.c_int, [P, P, C.c_uint32, P, C.POINTER(C.c_float)]),
@@ -396,6 +397,20 @@ def render_primitive(scene: GpuScene, film: Film, spp: int, num_path_vertices: i
     p = render_params(spp, num_path_vertices, seed, False, first_sample, max_paths_in_flight, profile, count_traversal)
     st = RenderStats()
     _check(lib().wr_render_primitive(scene.h, film.h, C.byref(p), C.byref(st)))
+    film.spp += spp
+    return st
+
+
+def render_light(scene: GpuScene, film: Film, spp: int, num_path_vertices: int = 10, seed: int = 0,
+                 first_sample: int | None = None, max_paths_in_flight: int = 0, profile: bool = False,
+                 count_traversal: bool = False) -> RenderStats:
+    """LightSplattingRenderer::render on the GPU (lightsplatting.h:37-113): spp*W*H light
+    subpaths whose vertices connect to the camera; film.image() = splat/(spp*W*H)."""
+    if first_sample is None:
+        first_sample = film.spp
+    p = render_params(spp, num_path_vertices, seed, False, first_sample, max_paths_in_flight, profile, count_traversal)
+    st = RenderStats()
+    _check(lib().wr_render_light(scene.h, film.h, C.byref(p), C.byref(st)))
     film.spp += spp
     return st
 

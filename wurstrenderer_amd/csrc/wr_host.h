@@ -88,6 +88,7 @@ struct RenderArgs {
     int N;                    // num_path_vertices
     int use_mis;
     int nee;                  // 1: PathNeeRenderer (NEE, t = 1 splats); 0: PrimitivePathRenderer (BSDF sampling only)
+    int light;                // 1: LightSplattingRenderer (light subpaths connected to the camera)
 };
 
 // ============================================================== host objects

@@ -190,7 +190,11 @@ __device__ __forceinline__ float3 primary_dir(const DCamera &cam, uint32_t key, 
 // Fills the pool behind the continuing paths with the next unclaimed samples:
 // createLightSubpath + createCameraSubpath up to the first ray, with the t = 1
 // light->camera connection (pathnextevent.h:53-99; thinlens.h:121-166), which
-// PrimitivePathRenderer (a.nee == 0) does not make.
+// PrimitivePathRenderer (a.nee == 0) does not make.  For LightSplattingRenderer
+// (a.light) an entry is a light subpath instead (lightsplatting.h:58-108,
+// generalsubpathsampler.h:185-268): its light vertex connects to the camera
+// with weight 1, and its first ray leaves the light in a cosine-distributed
+// direction (DiffuseAreaLight::sampleLe1CosByPdf, contribution 1).
 template <bool BATCH>
 __global__ __launch_bounds__(kShadeBlock) void k_regen(DScene s1, const BatchJob *__restrict__ jobs, uint32_t n_jobs,
                                                        PathBuffers pb, RenderArgs a) {
@@ -231,8 +235,8 @@ __global__ __launch_bounds__(kShadeBlock) void k_regen(DScene s1, const BatchJob
             const float3 cpos = cam_to_world(cam, lens) + f3(cam.ox, cam.oy, cam.oz);
             const float3 cz = f3(cam.zx, cam.zy, cam.zz);
             // t = 1 connection
-            if (a.nee) {
-                const LightSample l = sample_light(s, u01(key, kDimLight), u01(key, kDimLight + 1));
+            const LightSample l = sample_light(s, u01(key, kDimLight), u01(key, kDimLight + 1));
+            if (a.nee || a.light) {
                 const float3 uv = cpos - l.pos;
                 const float len2 = dot3(uv, uv);
                 const float len = sqrtf(len2);
@@ -262,12 +266,21 @@ __global__ __launch_bounds__(kShadeBlock) void k_regen(DScene s1, const BatchJob
                     }
                 }
             }
-            // primary ray (samplePerPixelWe1CosByPdf)
-            float pdfW;
-            const float3 dir = primary_dir(cam, key, px, py, lens, pdfW);
-            ro = make_float4(cpos.x, cpos.y, cpos.z, 0.f);
-            rdir = make_float4(dir.x, dir.y, dir.z, 0.f);
-            thr = make_float4(1.f, 1.f, 1.f, use_mis ? pdfW : 0.f);
+            if (a.light) {
+                // the light subpath's first ray (pdf not needed: no MIS)
+                const float3 dir =
+                    to_world(make_frame(l.n), cosine_hemisphere(u01(key, kDimPixel), u01(key, kDimPixel + 1)));
+                ro = make_float4(l.pos.x, l.pos.y, l.pos.z, 0.f);
+                rdir = make_float4(dir.x, dir.y, dir.z, 0.f);
+                thr = make_float4(l.contrib.x, l.contrib.y, l.contrib.z, 0.f);
+            } else {
+                // primary ray (samplePerPixelWe1CosByPdf)
+                float pdfW;
+                const float3 dir = primary_dir(cam, key, px, py, lens, pdfW);
+                ro = make_float4(cpos.x, cpos.y, cpos.z, 0.f);
+                rdir = make_float4(dir.x, dir.y, dir.z, 0.f);
+                thr = make_float4(1.f, 1.f, 1.f, use_mis ? pdfW : 0.f);
+            }
             meta = make_uint4(key, sidx, job << kBounceBits, 0u);
         }
         if (fresh) {
@@ -521,6 +534,104 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(DScene s1, const BatchJob
         const uint32_t target = cont ? (kToNextPath | qi) : (kToSample | meta.y);
         sd.w = __uint_as_float(target);
         sc.w = __uint_as_float(job);
+        const uint32_t si = block_append(&pb.counters[kS], push_shadow, agg);
+        if (push_shadow) {
+            pb.sh_o[si] = so;
+            pb.sh_d[si] = sd;
+            pb.sh_c[si] = sc;
+        }
+    }
+}
+
+// ============================================================== light subpaths
+// One vertex of a light subpath (LightSplattingRenderer, lightsplatting.h:91-105;
+// randomWalk, generalsubpathsampler.h:34-160).  Vertex k of the subpath (the
+// light vertex is 1; the first hit is bounce 0, vertex 2) exists for k <= N - 1.
+// A surface vertex connects to the camera: G We1 f T with f evaluated from the
+// previous vertex toward the camera, splatted at the camera raster through a
+// shadow ray; then, before vertex N - 1, the BSDF samples the next direction.
+// An emitter hit ends the subpath.  No medium (the reference's sampler here
+// is not volumetric).
+__global__ __launch_bounds__(kShadeBlock) void k_shade_light(DScene s, PathBuffers pb, RenderArgs a) {
+    __shared__ uint32_t agg[kWavesPerShadeBlock + 1];
+    const uint32_t n = active_count(pb.counters);
+    const PathState cs = pb.cur, ns = pb.next;
+    const DCamera &cam = s.cam;
+    for (uint32_t base = blockIdx.x * kShadeBlock; base < n; base += gridDim.x * kShadeBlock) {
+        const uint32_t i = base + threadIdx.x;
+        bool push_shadow = false, cont = false;
+        const bool valid = i < n;
+        float4 so = make_float4(0, 0, 0, 0), sd = so, sc = so;
+        float4 new_o = so, new_d = so, new_thr = so, rad = so;
+        uint4 meta = make_uint4(0, 0, 0, 0);
+        if (valid) {
+            meta = cs.meta[i];
+            rad = cs.rad[i];
+            const uint32_t bounce = meta.z & kBounceMask;
+            const int k = int(bounce) + 2;  // vertex of the light subpath
+            const uint2 h = pb.hit[i];
+            if (h.x != kInvalid && k <= a.N - 1) {
+                const float t = __uint_as_float(h.y);
+                const float3 ro = xyz(cs.ray_o[i]), rd = xyz(cs.ray_d[i]);
+                const float4 v0 = s.tris[3 * size_t(h.x) + 0];
+                const float4 v1 = s.tris[3 * size_t(h.x) + 1];
+                const float4 v2 = s.tris[3 * size_t(h.x) + 2];
+                const int2 gi = s.geom_info[__float_as_uint(v1.w)];
+                if (gi.y < 0) {
+                    const float3 pos = ro + rd * t;
+                    const float3 ng = normalize3(cross3(xyz(v1) - xyz(v0), xyz(v2) - xyz(v0)));
+                    const float3 T = xyz(cs.thr[i]);
+                    const uint32_t key = meta.x;
+                    const DMat m = s.mats[gi.x];
+                    const Frame f = make_frame(ng);
+                    const float3 wi = to_local(f, -rd);
+                    // connection to the camera vertex
+                    const float3 lens = lens_point(cam, key);
+                    const float3 cpos = cam_to_world(cam, lens) + f3(cam.ox, cam.oy, cam.oz);
+                    const float3 uv = cpos - pos;
+                    const float len2 = dot3(uv, uv);
+                    const float len = sqrtf(len2);
+                    const float3 nuv = uv * (1.0f / len);
+                    const float G = fabsf(dot3(ng, nuv)) * fabsf(dot3(f3(cam.zx, cam.zy, cam.zz), nuv)) / len2;
+                    float rx = 0.f, ry = 0.f;
+                    const float we1 = G != 0.f ? camera_we1(cam, lens, -nuv, rx, ry) : 0.f;
+                    if (we1 != 0.f) {
+                        const float3 fs = bsdf_eval(m, wi, to_local(f, nuv));
+                        const float3 c = T * fs * (G * we1);
+                        if (!is_black(c)) {
+                            const uint32_t target = kToSplat | (uint32_t(ry) * uint32_t(cam.W) + uint32_t(rx));
+                            so = make_float4(pos.x, pos.y, pos.z, len - kRayEps);
+                            sd = make_float4(nuv.x, nuv.y, nuv.z, __uint_as_float(target));
+                            sc = make_float4(c.x, c.y, c.z, 0.f);
+                            push_shadow = true;
+                        }
+                    }
+                    if (k < a.N - 1) {
+                        const uint32_t dim = kDimBounce0 + kDimsPerBounce * bounce + kBdBsdf;
+                        float3 wo;
+                        float pdf = 0.f;
+                        const float3 w = bsdf_sample(m, wi, u01(key, dim), u01(key, dim + 1), wo, pdf);
+                        if (!is_black(w)) {
+                            const float3 nd = to_world(f, wo);
+                            new_o = make_float4(pos.x, pos.y, pos.z, 0.f);
+                            new_d = make_float4(nd.x, nd.y, nd.z, 0.f);
+                            new_thr = make_float4(w.x * T.x, w.y * T.y, w.z * T.z, 0.f);
+                            cont = true;
+                        }
+                    }
+                }
+            }
+        }
+        const uint32_t qi = block_append(&pb.counters[kQn], cont, agg);
+        if (cont) {
+            ns.ray_o[qi] = new_o;
+            ns.ray_d[qi] = new_d;
+            ns.thr[qi] = new_thr;
+            ns.rad[qi] = rad;
+            ns.meta[qi] = make_uint4(meta.x, meta.y, meta.z + 1, 0u);
+        } else if (valid) {
+            pb.sample_rad[meta.y] = rad;
+        }
         const uint32_t si = block_append(&pb.counters[kS], push_shadow, agg);
         if (push_shadow) {
             pb.sh_o[si] = so;
@@ -1652,10 +1763,12 @@ static int drain(wr_scene *s, PathBuffers &pb, uint64_t max_iters, bool use_grap
     return rc;  // an even number of iterations: pb is the current state again
 }
 
-// The frame loop of wr_render_nee, wr_render_guided (g non-null) and
-// wr_render_primitive (nee false).
+// Estimators of the frame loop.
+enum Estimator { kEstNee, kEstPrimitive, kEstLight };
+// The frame loop of wr_render_nee, wr_render_guided (g non-null),
+// wr_render_primitive and wr_render_light.
 static int render_impl(wr_scene *s, wr_film *f, const wr_render_params *prm, wr_guide *g, const wr_guided_params *gp,
-                       wr_render_stats *stats, wr_guide_stats *gstats, bool nee = true) {
+                       wr_render_stats *stats, wr_guide_stats *gstats, Estimator est = kEstNee) {
     if (!s || !f || !prm) return set_error(WR_ERR_INVALID_ARGUMENT, "null argument");
     if (f->scene != s) return set_error(WR_ERR_INVALID_ARGUMENT, "film belongs to another scene");
     if (prm->spp < 0) return set_error(WR_ERR_INVALID_ARGUMENT, "spp must be >= 0");
@@ -1718,8 +1831,12 @@ static int render_impl(wr_scene *s, wr_film *f, const wr_render_params *prm, wr_
     auto end = [&]() { if (prof) ev_record(); };
     const int g_regen = std::max(1, std::min<int>((P + kShadeBlock - 1) / kShadeBlock, s->dev->n_cu * s->dev->occ_generate));
     // shading kernel: plain, or extended for guiding and/or a medium
-    const bool medium = s->ds.med.on != 0;
-    const void *shade_fn = reinterpret_cast<const void *>(&k_shade<false>);
+    // the light tracer's sampler is not volumetric (lightsplatting.h:34): it sees vacuum
+    DScene ds = s->ds;
+    if (est == kEstLight) ds.med.on = 0;
+    const bool medium = ds.med.on != 0;
+    const void *shade_fn = est == kEstLight ? reinterpret_cast<const void *>(&k_shade_light)
+                                            : reinterpret_cast<const void *>(&k_shade<false>);
     if (g && medium) shade_fn = reinterpret_cast<const void *>(&k_shade_ext<true, true>);
     else if (g) shade_fn = reinterpret_cast<const void *>(&k_shade_ext<true, false>);
     else if (medium) shade_fn = reinterpret_cast<const void *>(&k_shade_ext<false, true>);
@@ -1745,8 +1862,9 @@ static int render_impl(wr_scene *s, wr_film *f, const wr_render_params *prm, wr_
         a.seed = prm->seed;
         a.capacity = P;
         a.N = N;
-        a.use_mis = nee && prm->use_mis ? 1 : 0;
-        a.nee = nee ? 1 : 0;
+        a.use_mis = est == kEstNee && prm->use_mis ? 1 : 0;
+        a.nee = est == kEstNee ? 1 : 0;
+        a.light = est == kEstLight ? 1 : 0;
         guide::Shade gs{};
         float record_prob = 0.f;
         const bool collect = g && gp->train_steps > 0;
@@ -1768,35 +1886,36 @@ static int render_impl(wr_scene *s, wr_film *f, const wr_render_params *prm, wr_
         const uint64_t max_iters = (uint64_t(a.total) / P + 2) * uint64_t(N + 1) + 64;
         auto iteration = [&](const PathBuffers &q) {
             begin(3);
-            k_regen<false><<<g_regen, kShadeBlock, 0, s->stream>>>(s->ds, nullptr, 0, q, a);
+            k_regen<false><<<g_regen, kShadeBlock, 0, s->stream>>>(ds, nullptr, 0, q, a);
             end();
             begin(0);
-            if (count) k_extend<true, false><<<g_ext, kBlock, 0, s->stream>>>(s->ds, nullptr, q, s->d_spill, tune);
-            else k_extend_fast<<<g_ext, kBlock, 0, s->stream>>>(s->ds, q, s->d_spill, tune);
+            if (count) k_extend<true, false><<<g_ext, kBlock, 0, s->stream>>>(ds, nullptr, q, s->d_spill, tune);
+            else k_extend_fast<<<g_ext, kBlock, 0, s->stream>>>(ds, q, s->d_spill, tune);
             end();
             begin(2);
             if (g && gs.alpha > 0.f) {
                 if (medium && fp8)
-                    k_guide_infer<true, true><<<g_infer, kInferBlock, 0, s->stream>>>(s->ds, q.cur, q.hit, q.counters, N,
+                    k_guide_infer<true, true><<<g_infer, kInferBlock, 0, s->stream>>>(ds, q.cur, q.hit, q.counters, N,
                                                                                       net, gs.alpha, gs.draw);
                 else if (medium)
-                    k_guide_infer<true, false><<<g_infer, kInferBlock, 0, s->stream>>>(s->ds, q.cur, q.hit, q.counters, N,
+                    k_guide_infer<true, false><<<g_infer, kInferBlock, 0, s->stream>>>(ds, q.cur, q.hit, q.counters, N,
                                                                                        net, gs.alpha, gs.draw);
                 else if (fp8)
-                    k_guide_infer<false, true><<<g_infer, kInferBlock, 0, s->stream>>>(s->ds, q.cur, q.hit, q.counters, N,
+                    k_guide_infer<false, true><<<g_infer, kInferBlock, 0, s->stream>>>(ds, q.cur, q.hit, q.counters, N,
                                                                                        net, gs.alpha, gs.draw);
                 else
-                    k_guide_infer<false, false><<<g_infer, kInferBlock, 0, s->stream>>>(s->ds, q.cur, q.hit, q.counters, N,
+                    k_guide_infer<false, false><<<g_infer, kInferBlock, 0, s->stream>>>(ds, q.cur, q.hit, q.counters, N,
                                                                                         net, gs.alpha, gs.draw);
             }
-            if (g && medium) k_shade_ext<true, true><<<g_shade, kShadeBlock, 0, s->stream>>>(s->ds, q, a, gs);
-            else if (g) k_shade_ext<true, false><<<g_shade, kShadeBlock, 0, s->stream>>>(s->ds, q, a, gs);
-            else if (medium) k_shade_ext<false, true><<<g_shade, kShadeBlock, 0, s->stream>>>(s->ds, q, a, gs);
-            else k_shade<false><<<g_shade, kShadeBlock, 0, s->stream>>>(s->ds, nullptr, q, a);
+            if (g && medium) k_shade_ext<true, true><<<g_shade, kShadeBlock, 0, s->stream>>>(ds, q, a, gs);
+            else if (g) k_shade_ext<true, false><<<g_shade, kShadeBlock, 0, s->stream>>>(ds, q, a, gs);
+            else if (medium) k_shade_ext<false, true><<<g_shade, kShadeBlock, 0, s->stream>>>(ds, q, a, gs);
+            else if (est == kEstLight) k_shade_light<<<g_shade, kShadeBlock, 0, s->stream>>>(ds, q, a);
+            else k_shade<false><<<g_shade, kShadeBlock, 0, s->stream>>>(ds, nullptr, q, a);
             end();
             begin(1);
-            if (count) k_shadow<true, false><<<g_shd, kBlock, 0, s->stream>>>(s->ds, nullptr, q, s->d_spill, tune);
-            else k_shadow<false, false><<<g_shd, kBlock, 0, s->stream>>>(s->ds, nullptr, q, s->d_spill, tune);
+            if (count) k_shadow<true, false><<<g_shd, kBlock, 0, s->stream>>>(ds, nullptr, q, s->d_spill, tune);
+            else k_shadow<false, false><<<g_shd, kBlock, 0, s->stream>>>(ds, nullptr, q, s->d_spill, tune);
             end();
             k_rotate<<<1, 1, 0, s->stream>>>(q, a);
         };
@@ -1858,7 +1977,11 @@ int wr_render_guided(wr_scene *s, wr_film *f, wr_guide *g, const wr_render_param
 }
 
 int wr_render_primitive(wr_scene *s, wr_film *f, const wr_render_params *prm, wr_render_stats *stats) {
-    return render_impl(s, f, prm, nullptr, nullptr, stats, nullptr, false);
+    return render_impl(s, f, prm, nullptr, nullptr, stats, nullptr, kEstPrimitive);
+}
+
+int wr_render_light(wr_scene *s, wr_film *f, const wr_render_params *prm, wr_render_stats *stats) {
+    return render_impl(s, f, prm, nullptr, nullptr, stats, nullptr, kEstLight);
 }
 
 int wr_render_visualize(wr_scene *s, wr_film *f, const wr_render_params *prm, wr_render_stats *stats) {

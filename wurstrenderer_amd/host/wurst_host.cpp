@@ -348,6 +348,34 @@ void PrimitivePathRenderer::render() {
     });
 }
 
+std::shared_ptr<LightSplattingRenderer> LightSplattingRenderer::FromSimple2Json(
+    const std::shared_ptr<ThinlensCamera> &camera, const std::shared_ptr<const SceneData> &scene,
+    const wj::Value &json, RenderContext *ctx) {
+    auto r = std::make_shared<LightSplattingRenderer>();
+    r->camera = camera;
+    r->scene = scene;
+    r->ctx = ctx;
+    r->num_path_vertices = int(json.number("num_path_vertices", 10));
+    r->num_spp = int(json.number("num_spp", 16));
+    r->seed = uint32_t(json.number("seed", 0));
+    return r;
+}
+
+// render (lightsplatting.h:37-113): atomic buffer only, scale 1/(spp W H)
+void LightSplattingRenderer::render() {
+    Film &film = *camera->film;
+    film.base_scale = 0.0;
+    film.splat_scale = 1.0 / double(num_spp) / (double(film.W) * double(film.H));
+    if (num_path_vertices < 2 || num_spp <= 0) return;
+    render_on_gpu(*camera, *scene, ctx, [&](wr_scene *sc, wr_film *f) {
+        wr_render_params p{};
+        p.spp = num_spp;
+        p.num_path_vertices = num_path_vertices;
+        p.seed = seed;
+        return wr_render_light(sc, f, &p, &stats);
+    });
+}
+
 std::shared_ptr<VisualizeGeometryNormal> VisualizeGeometryNormal::FromSimple2Json(
     const std::shared_ptr<ThinlensCamera> &camera, const std::shared_ptr<const SceneData> &scene,
     const wj::Value &json, RenderContext *ctx) {
@@ -386,8 +414,7 @@ void dump_array(const std::string &dir, const char *name, const std::vector<T> &
 }
 
 // integrator names the reference's dispatch knows (simple2parser.h:176-207)
-const char *kOtherIntegrators[] = {"bidirectional_path_tracer", "photon_map", "light_splatting", "diffuse_prt",
-                                   "glossy_prt2002"};
+const char *kOtherIntegrators[] = {"bidirectional_path_tracer", "photon_map", "diffuse_prt", "glossy_prt2002"};
 
 }  // namespace
 
@@ -454,6 +481,8 @@ void Simple2Parser::Parse(const wj::Value &j, const std::string &root_dir, const
             renderer = PathNeeRenderer::FromSimple2Json(camera, scene, rj, &ctx);
         } else if (name == "primitive_path_tracer") {
             renderer = PrimitivePathRenderer::FromSimple2Json(camera, scene, rj, &ctx);
+        } else if (name == "light_splatting") {
+            renderer = LightSplattingRenderer::FromSimple2Json(camera, scene, rj, &ctx);
         } else if (name == "visualize") {
             renderer = VisualizeGeometryNormal::FromSimple2Json(camera, scene, rj, &ctx);
         } else {
@@ -480,6 +509,7 @@ void Simple2Parser::Parse(const wj::Value &j, const std::string &root_dir, const
             int spp = 0;
             if (auto *pr = dynamic_cast<PrimitivePathRenderer *>(renderer.get())) { st = &pr->stats; spp = pr->num_spp; }
             if (auto *vz = dynamic_cast<VisualizeGeometryNormal *>(renderer.get())) { st = &vz->stats; spp = vz->num_spp; }
+            if (auto *ls = dynamic_cast<LightSplattingRenderer *>(renderer.get())) { st = &ls->stats; spp = ls->num_spp; }
             if (st)
                 std::printf("{\"integrator\": \"%s\", \"output\": \"%s\", \"spp\": %d, \"paths\": %llu, "
                             "\"extend_rays\": %llu, \"ms\": %.3f}\n",

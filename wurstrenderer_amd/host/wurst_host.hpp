@@ -1,5 +1,6 @@
 // wurst_host.hpp -- C++ host of the MI355X `nextevent_path_tracer` (and of
-// `primitive_path_tracer` and `visualize` on the same kernels).
+// `primitive_path_tracer`, `light_splatting` and `visualize` on the same
+// kernels).
 //
 // Mirrors the reference's scene-JSON entry point (Simple2Parser,
 // src/sceneio/simple2parser.h:84-223), its ThinlensCamera JSON reader
@@ -99,6 +100,23 @@ struct PrimitivePathRenderer : Renderer {
     static std::shared_ptr<PrimitivePathRenderer> FromSimple2Json(const std::shared_ptr<ThinlensCamera> &camera,
                                                                   const std::shared_ptr<const SceneData> &scene,
                                                                   const wj::Value &json, RenderContext *ctx);
+    void render() override;
+
+    std::shared_ptr<ThinlensCamera> camera;
+    std::shared_ptr<const SceneData> scene;
+    RenderContext *ctx = nullptr;
+    int num_path_vertices = 10, num_spp = 16;
+    uint32_t seed = 0;
+    wr_render_stats stats{};
+};
+
+// LightSplattingRenderer (src/renderer/lightsplatting.h:12-116) on the GPU:
+// light subpaths connected to the camera; JSON keys "num_path_vertices" (10),
+// "num_spp" (16), "seed".
+struct LightSplattingRenderer : Renderer {
+    static std::shared_ptr<LightSplattingRenderer> FromSimple2Json(const std::shared_ptr<ThinlensCamera> &camera,
+                                                                   const std::shared_ptr<const SceneData> &scene,
+                                                                   const wj::Value &json, RenderContext *ctx);
     void render() override;
 
     std::shared_ptr<ThinlensCamera> camera;
