@@ -202,6 +202,19 @@ typedef struct {
 } wr_medium;
 int wr_scene_set_medium(wr_scene *scene, const wr_medium *medium);
 
+/* Envmap light (EnvmapLight over EquiRectRep: light/envmap.h:9-113,
+ * sphericalrepresentation/equirectangular.h; JSON "envmap": image path,
+ * simple2parser.h:110-119): an equirectangular RGB image rgb[height][width][3],
+ * width = 2 height, looked up bilinearly (repeat in u, clamp in v) along
+ * u = atan2(-z, -x) / 2pi + 1/2, v = acos(y) / pi (mapping.h:91-96), sampled by
+ * a texel CdfTable with weights sin(theta) |rgb|_2, and selected for NEE as the
+ * last light with CdfTable weight 1.23 (envmap.h:22-26).  Rays that leave the
+ * scene see it (Scene::intersect, scene.h:82-88).  wr_render_nee,
+ * wr_render_primitive, wr_render_visualize (a miss counts 1) and batches use it;
+ * wr_render_light, wr_render_guided and a medium reject it.  A null rgb removes
+ * it.  A scene may have no area light when it has an envmap. */
+int wr_scene_set_envmap(wr_scene *scene, const float *rgb, uint32_t width, uint32_t height);
+
 /* FP64 film in device memory: a base buffer (own-pixel samples) and a splat
  * buffer (light->camera connections).  Replaces Film's base and atomic buffers
  * (src/common/film.h:35-68). */
@@ -263,7 +276,8 @@ int wr_trace_any_device(wr_scene *scene, const float *d_rays, uint32_t n, uint8_
  * keeps the GPU full when single jobs are too small to.  No reference
  * counterpart (the reference renders one scene per process).  Each job's film
  * receives exactly what wr_render_nee with the same parameters would add
- * (same random numbers per pixel and sample).  All scenes must live on the
+ * (same random numbers per pixel and sample; when a job has an envmap, to FP32
+ * rounding).  Scenes with a medium are rejected.  All scenes must live on the
  * same device; the batch runs on the first job's scene stream and path pool,
  * so calls touching those scenes must be serialised with it.  From `params`
  * only max_paths_in_flight, profile, use_graph, fetch_chunk and refill_lanes

@@ -92,6 +92,9 @@ def lib():
         L.orc_camera_we1.argtypes = [P, D3, D3, D3]
         L.orc_camera_we1.restype = C.c_double
         L.orc_scene_set_medium.argtypes = [P, D3, D3, C.c_double]
+        L.orc_scene_set_envmap.argtypes = [P, P, C.c_int, C.c_int]
+        L.orc_env_eval.argtypes = [P, D3, D3, D3]
+        L.orc_env_sample_ref.argtypes = [P, D3, D3, D3, D3]
         L.orc_hg_pdf.argtypes = [C.c_double, C.c_double]
         L.orc_hg_pdf.restype = C.c_double
         L.orc_hg_sample.argtypes = [C.c_double, D3, D3, D3]
@@ -139,6 +142,8 @@ class OracleScene:
         self.W, self.H = sc.resolution
         if getattr(sc, "medium", None):
             self.set_medium(sc.medium["sigma_a"], sc.medium["sigma_s"], sc.medium["g"])
+        if getattr(sc, "envmap", None) is not None:
+            self.set_envmap(sc.envmap)
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -191,6 +196,31 @@ class OracleScene:
             return occ, amb
         lib().orc_trace_any(self.h, rays.ctypes.data, n, occ.ctypes.data)
         return occ
+
+    def set_envmap(self, rgb):
+        """Equirectangular envmap light, float (H, 2H, 3); None removes it."""
+        if rgb is None:
+            lib().orc_scene_set_envmap(self.h, None, 0, 0)
+            return
+        a = np.ascontiguousarray(rgb, np.float32)
+        self._env = a
+        if lib().orc_scene_set_envmap(self.h, a.ctypes.data, a.shape[1], a.shape[0]) != 0:
+            raise RuntimeError("oracle: " + lib().orc_last_error().decode())
+
+    def env_eval(self, d):
+        """EquiRectRep eval (bilinear) and pdfA at direction d."""
+        dd = np.ascontiguousarray(d, np.float64)
+        out = np.zeros(3)
+        pdf = np.zeros(1)
+        lib().orc_env_eval(self.h, _dp(dd), _dp(out), _dp(pdf))
+        return out, float(pdf[0])
+
+    def env_sample_ref(self, u):
+        """EquiRectRep::sample with the reference's use of the two numbers: (dir, pdf, value)."""
+        uu = np.ascontiguousarray(u, np.float64)
+        d, pdf, v = np.zeros(3), np.zeros(1), np.zeros(3)
+        lib().orc_env_sample_ref(self.h, _dp(uu), _dp(d), _dp(pdf), _dp(v))
+        return d, float(pdf[0]), v
 
     def set_medium(self, sigma_a=(0, 0, 0), sigma_s=(0, 0, 0), g=0.0):
         """Homogeneous medium filling the scene, HG phase (zeros remove it)."""

@@ -9,7 +9,9 @@
 //   CdfTable (src/common/cdftable.h), LambertDiffuseBsdf, OrenNayarBsdf, PhongBsdf,
 //   MixedBsdf, TwoSidedBsdf (src/bsdf/*.h), DiffuseAreaLight
 //   (src/light/diffusearealight.h) on TriangleMeshGeometry
-//   (src/geometry/trianglemesh.h), ThinlensCamera (src/camera/thinlens.h).
+//   (src/geometry/trianglemesh.h), ThinlensCamera (src/camera/thinlens.h),
+//   EquiRectRep (src/sphericalrepresentation/equirectangular.h: the envmap's
+//   bilinear lookup, pixel CDF, pdf and sampling; Fimage, src/common/floatimage.h).
 // The light-selection composition mirrors GeneralSubpathSampler::createLightSubpath
 // (src/pathsampler/generalsubpathsampler.h:203-250), which cannot be
 // instantiated without Embree.  Scene inputs (light meshes, cameras) are exactly
@@ -25,6 +27,7 @@
 #include "geometry/trianglemesh.h"
 #include "common/cdftable.h"
 #include "common/mapping.h"
+#include "sphericalrepresentation/equirectangular.h"
 
 #include <cstdio>
 #include <cstdint>
@@ -260,7 +263,50 @@ int main() {
                 printf("}");
             }
         }
-        printf("\n]\n");
+        printf("\n],\n");
+    }
+    // ---- equirectangular envmap (EquiRectRep with its pixel CDF, as EnvmapLight uses it)
+    printf("\"envmap\":{\n");
+    {
+        const int W = 8, H = 4;
+        std::vector<Spectrum> data(W * H);
+        for (auto &t : data) {  // FP32 values: the oracle and the GPU take FP32 images
+            const double r = float(4 * rnd()), g = float(4 * rnd()), b = float(4 * rnd());
+            t = Spectrum(r, g, b);
+        }
+        data[5] = Spectrum(0.0);  // a black texel is never sampled
+        Fimage<Spectrum> img(W, H, data);
+        EquiRectRep<Spectrum> rep(img, true);
+        printf("\"width\":%d,\"height\":%d,\"texels\":[", W, H);
+        for (int i = 0; i < W * H; i++) printf("%s%.17g,%.17g,%.17g", i ? "," : "", data[i][0], data[i][1], data[i][2]);
+        printf("],\n\"lookups\":[\n");
+        for (int i = 0; i < 64; i++) {
+            Vec3 d = rnd_dir();
+            if (i == 0) d = Vec3(0, 1, 0);   // pole: pdf 0
+            if (i == 1) d = Vec3(0, -1, 0);  // other pole
+            if (i == 2) d = Vec3(-1, 0, 0);  // u = 0.5
+            if (i == 3) d = Math::Normalize(Vec3(1, 0.3, -1e-6));  // just before the seam (u -> 1)
+            if (i == 4) d = Math::Normalize(Vec3(1, -0.2, 1e-6));  // just after it (u -> 0)
+            const Vec2 uv = Mapping::PanoramaFromWorld(d);
+            const Spectrum e = rep.eval(d);
+            printf("{"); pv("dir", d); pv2("uv", uv); pv("eval", Vec3(e[0], e[1], e[2])); pd("pdf", rep.pdfA(d), false);
+            printf("}%s\n", i == 63 ? "" : ",");
+        }
+        printf("],\n\"samples\":[\n");
+        for (int i = 0; i < 64; i++) {
+            const Vec2 u(rnd(), rnd());
+            Vec3 dir;
+            double pdf = 0;
+            const Spectrum v = rep.sample(&dir, &pdf, u);
+            printf("{"); pv2("u", u); pv("dir", dir); pd("pdf", pdf); pv("value", Vec3(v[0], v[1], v[2]), false);
+            printf("}%s\n", i == 63 ? "" : ",");
+        }
+        printf("],\n\"panorama\":[\n");
+        for (int i = 0; i < 32; i++) {
+            const Vec2 uv(rnd(), rnd());
+            printf("{"); pv2("uv", uv); pv("dir", Mapping::WorldFromPanorama(uv), false); printf("}%s\n", i == 31 ? "" : ",");
+        }
+        printf("]}\n");
     }
     printf("}\n");
     return 0;

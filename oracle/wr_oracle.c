@@ -85,6 +85,10 @@ double orc_rng_uniform(uint32_t seed, uint32_t pixel, uint32_t sample, uint32_t 
 }
 /* dimension layout per path */
 enum { DIM_LIGHT = 0, DIM_LENS = 2, DIM_PIXEL = 4, DIM_BOUNCE0 = 6, DIMS_PER_BOUNCE = 6 };
+/* position inside the envmap texel picked for a light sample (the reference
+ * re-uses the twice-remapped selection number there; fresh numbers keep the
+ * FP32 device's in-texel position at full resolution) */
+enum { DIM_ENV = 0x00fffff0 };
 enum { BD_MEDIUM = 0, BD_BSDF = 2 };
 
 /* ------------------------------------------------------------------ frames, mappings */
@@ -378,6 +382,12 @@ struct orc_scene {
     /* homogeneous medium filling the scene (homogeneous.h:11-66), HG phase */
     int has_medium;
     double sig_t[3], sig_s[3], hg_g;
+    /* envmap (EnvmapLight over EquiRectRep, light/envmap.h, equirectangular.h):
+     * texels [env_h][env_w] RGB, pixel CDF; last entry of light_cdf */
+    int env_on, env_w, env_h;
+    double *env_rgb;
+    cdf_table env_cdf;
+    double *light_w;  /* area-light CDF weights (|L|_2), for rebuilding light_cdf */
 };
 
 typedef struct { double c[3]; uint32_t tri; } bref;
@@ -624,7 +634,6 @@ static int bvh_any(const orc_scene *s, const fray *r) {
 int orc_scene_create(const orc_scene_desc *d, orc_scene **out) {
     *out = NULL;
     if (!d) return fail("null desc");
-    if (d->n_lights == 0) return fail("scene has no lights");
     if (d->camera.width <= 0 || d->camera.height <= 0) return fail("bad resolution");
     orc_scene *s = (orc_scene *)calloc(1, sizeof(orc_scene));
     s->n_verts = d->n_verts; s->n_tris = d->n_tris; s->n_geoms = d->n_geoms;
@@ -687,7 +696,7 @@ int orc_scene_create(const orc_scene_desc *d, orc_scene **out) {
         w[l] = sqrt(L->L.r * L->L.r + L->L.g * L->L.g + L->L.b * L->L.b);
     }
     cdf_build(&s->light_cdf, w, (int)d->n_lights);
-    free(w);
+    s->light_w = w;
     /* camera */
     const orc_camera *c = &d->camera;
     s->W = c->width; s->H = c->height;
@@ -715,18 +724,141 @@ void orc_scene_destroy(orc_scene *s) {
     if (s->lights) for (uint32_t l = 0; l < s->n_lights; l++) cdf_free(&s->lights[l].tri_cdf);
     free(s->lights);
     cdf_free(&s->light_cdf);
+    free(s->light_w);
+    free(s->env_rgb);
+    cdf_free(&s->env_cdf);
     free(s->nodes); free(s->tri_order);
     free(s);
 }
 
+/* ------------------------------------------------------------------ envmap (light/envmap.h, equirectangular.h) */
+#define ENV_CDF_WEIGHT 1.23 /* EnvmapLight::cdfWeight (envmap.h:22-26) */
+/* Mapping::PanoramaFromWorld / WorldFromPanorama (mapping.h:91-96,302-323) */
+static void panorama_from_world(v3 d, double *u, double *v) {
+    *u = atan2(-d.z, -d.x) * INV_PI * 0.5 + 0.5;
+    *v = acos(fmax(-1.0, fmin(1.0, d.y))) * INV_PI;
+}
+static v3 world_from_panorama(double u, double v) {
+    double th = u * PI * 2.0, ph = v * PI;
+    return V(cos(th) * sin(ph), cos(ph), sin(th) * sin(ph));
+}
+/* Fimage::evalTexel with S repeat, T clamp (floatimage.h:91-112) */
+static rgb env_texel(const orc_scene *s, int x, int y) {
+    x %= s->env_w; if (x < 0) x += s->env_w;
+    if (y < 0) y = 0;
+    if (y > s->env_h - 1) y = s->env_h - 1;
+    const double *t = s->env_rgb + 3 * ((size_t)y * s->env_w + x);
+    return RGB(t[0], t[1], t[2]);
+}
+/* EquiRectRep::eval = Fimage::evalBilinear (floatimage.h:120-133) */
+static rgb env_eval(const orc_scene *s, v3 d) {
+    double u, v;
+    panorama_from_world(d, &u, &v);
+    double sx = u * s->env_w - 0.5, sy = v * s->env_h - 0.5;
+    int x = (int)floor(sx), y = (int)floor(sy);
+    double dx1 = sx - x, dy1 = sy - y, dx2 = 1.0 - dx1, dy2 = 1.0 - dy1;
+    rgb r = cscale(env_texel(s, x, y), dx2 * dy2);
+    r = cadd(r, cscale(env_texel(s, x, y + 1), dx2 * dy1));
+    r = cadd(r, cscale(env_texel(s, x + 1, y), dx1 * dy2));
+    return cadd(r, cscale(env_texel(s, x + 1, y + 1), dx1 * dy1));
+}
+/* EquiRectRep::pdfA: solid-angle pdf of sampling direction d (equirectangular.h:64-71) */
+static double env_pdf(const orc_scene *s, v3 d) {
+    double u, v;
+    panorama_from_world(d, &u, &v);
+    int x = (int)(u * s->env_w), y = (int)(v * s->env_h);
+    if (x > s->env_w - 1) x = s->env_w - 1;
+    if (y > s->env_h - 1) y = s->env_h - 1;
+    double sine = sqrt(fmax(0.0, 1.0 - d.y * d.y));
+    if (sine <= SMALL_VALUE) return 0.0;
+    return s->env_cdf.pdf[(size_t)y * s->env_w + x] * s->env_w * s->env_h / sine * INV_PI * INV_PI * 0.5;
+}
+/* EquiRectRep::sample (equirectangular.h:85-117): texel by the CDF, then
+ * (ox, oy) inside it; returns eval / pdf and the pdf */
+static rgb env_sample(const orc_scene *s, double u_texel, double ox, double oy, v3 *dir, double *pdf) {
+    double pt;
+    int idx = cdf_sample(&s->env_cdf, u_texel, &pt, NULL);
+    int x = idx % s->env_w, y = idx / s->env_w;
+    *dir = world_from_panorama((x + ox) / s->env_w, (y + oy) / s->env_h);
+    double sine = sqrt(fmax(0.0, 1.0 - dir->y * dir->y));
+    *pdf = sine <= SMALL_VALUE ? 0.0 : pt * s->env_w * s->env_h / sine * INV_PI * INV_PI * 0.5;
+    return *pdf > 0.0 ? cscale(env_eval(s, *dir), 1.0 / *pdf) : BLACK;
+}
+static void rebuild_light_cdf(orc_scene *s) {
+    cdf_free(&s->light_cdf);
+    int n = (int)s->n_lights + (s->env_on ? 1 : 0);
+    double *w = (double *)malloc(sizeof(double) * (n ? n : 1));
+    for (uint32_t i = 0; i < s->n_lights; i++) w[i] = s->light_w[i];
+    if (s->env_on) w[n - 1] = ENV_CDF_WEIGHT;
+    cdf_build(&s->light_cdf, w, n);
+    free(w);
+}
+int orc_scene_set_envmap(orc_scene *s, const float *rgb, int w, int h) {
+    free(s->env_rgb); s->env_rgb = NULL;
+    cdf_free(&s->env_cdf); memset(&s->env_cdf, 0, sizeof s->env_cdf);
+    s->env_on = 0;
+    if (rgb && w > 0) {
+        if (h <= 0 || w != 2 * h) return fail("envmap has wrong size"); /* equirectangular.h:31 */
+        size_t n = (size_t)w * h;
+        s->env_rgb = (double *)malloc(sizeof(double) * 3 * n);
+        double *wt = (double *)malloc(sizeof(double) * n);
+        double sum = 0.0;
+        for (size_t i = 0; i < 3 * n; i++) {
+            if (!(rgb[i] >= 0.0f) || !isfinite(rgb[i])) { free(wt); return fail("envmap texels must be finite and >= 0"); }
+            s->env_rgb[i] = rgb[i];
+        }
+        for (int y = 0; y < h; y++) {
+            double sine = sin((y + 0.5) / h * PI);
+            for (int x = 0; x < w; x++) {
+                const double *t = s->env_rgb + 3 * ((size_t)y * w + x);
+                wt[(size_t)y * w + x] = sine * sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+                sum += wt[(size_t)y * w + x];
+            }
+        }
+        if (!(sum > 0.0)) { free(wt); return fail("envmap is black"); }
+        cdf_build(&s->env_cdf, wt, (int)n);
+        free(wt);
+        s->env_w = w; s->env_h = h; s->env_on = 1;
+    }
+    rebuild_light_cdf(s);
+    return 0;
+}
+void orc_env_eval(const orc_scene *s, const double d[3], double out[3], double *pdf) {
+    v3 v = V(d[0], d[1], d[2]);
+    rgb e = env_eval(s, v);
+    out[0] = e.r; out[1] = e.g; out[2] = e.b;
+    *pdf = env_pdf(s, v);
+}
+/* EquiRectRep::sample as the reference draws it: the in-texel u is the remapped selection number */
+void orc_env_sample_ref(const orc_scene *s, const double u[2], double dir[3], double *pdf, double value[3]) {
+    double r;
+    cdf_sample(&s->env_cdf, u[0], NULL, &r);
+    v3 d;
+    rgb v = env_sample(s, u[0], r, u[1], &d, pdf);
+    dir[0] = d.x; dir[1] = d.y; dir[2] = d.z;
+    value[0] = v.r; value[1] = v.g; value[2] = v.b;
+}
+
 /* ------------------------------------------------------------------ light + camera sampling */
-typedef struct { v3 pos, n; frame3 f; rgb contrib; double pdfA; int light; } lvert;
+/* env != 0: an envmap vertex; pos holds its direction (path.h:126-131) */
+typedef struct { v3 pos, n; frame3 f; rgb contrib; double pdfA; int light, env; } lvert;
 /* createLightSubpath with numVertices = 1 (generalsubpathsampler.h:185-255),
  * DiffuseAreaLight::sampleLe0ByPdf (diffusearealight.h:53-72),
  * TriangleMeshGeometry::sample (trianglemesh.h:131-159) */
-static void sample_light(const orc_scene *s, double u0, double u1, lvert *lv) {
+static void sample_light(const orc_scene *s, double u0, double u1, uint32_t key, lvert *lv) {
     double pdfLight, ur;
     int li = cdf_sample(&s->light_cdf, u0, &pdfLight, &ur);
+    lv->env = 0;
+    if (li == (int)s->n_lights) { /* EnvmapLight::sampleLe0ByPdf (envmap.h:61-77) */
+        double pdfW;
+        v3 d;
+        rgb v = env_sample(s, ur, u01(key, DIM_ENV), u01(key, DIM_ENV + 1), &d, &pdfW);
+        lv->pos = d; lv->n = V(0, 0, 0);
+        lv->contrib = cscale(v, 1.0 / pdfLight);
+        lv->pdfA = pdfLight * pdfW;
+        lv->light = li; lv->env = 1;
+        return;
+    }
     const light_t *L = &s->lights[li];
     double ur2;
     int ti = cdf_sample(&L->tri_cdf, ur, NULL, &ur2);
@@ -744,7 +876,7 @@ static void sample_light(const orc_scene *s, double u0, double u1, lvert *lv) {
 }
 int orc_light_sample(const orc_scene *s, const double u[2], double pos[3], double normal[3], double contrib[3], double *pdfA) {
     lvert lv;
-    sample_light(s, u[0], u[1], &lv);
+    sample_light(s, u[0], u[1], 0u, &lv);
     pos[0] = lv.pos.x; pos[1] = lv.pos.y; pos[2] = lv.pos.z;
     normal[0] = lv.n.x; normal[1] = lv.n.y; normal[2] = lv.n.z;
     contrib[0] = lv.contrib.r; contrib[1] = lv.contrib.g; contrib[2] = lv.contrib.b;
@@ -892,14 +1024,34 @@ static void eval_contribution(const orc_scene *s, int N, int use_mis, int nee, u
     *sp_ok = 0;
     /* light subpath: one vertex */
     lvert l;
-    sample_light(s, u01(key, DIM_LIGHT), u01(key, DIM_LIGHT + 1), &l);
+    memset(&l, 0, sizeof l);
+    if (s->light_cdf.n > 0) sample_light(s, u01(key, DIM_LIGHT), u01(key, DIM_LIGHT + 1), key, &l); /* primitive paths never use it */
     double lPdfA = use_mis ? l.pdfA : 1.0;
     /* camera vertex */
     v3 lens_local;
     v3 cpos = camera_position(s, u01(key, DIM_LENS), u01(key, DIM_LENS + 1), &lens_local);
     v3 cn_ = s->cam_frame.z; /* camera geometry normal (thinlens.h:130) */
     /* t = 1: light -> camera connection (pathnextevent.h:75-99) */
-    if (nee) {
+    if (nee && l.env) {
+        /* envmap vertex: the camera looks along its direction; G = |cos| at the
+         * camera only, Le1 = 1, visibility to infinity (path.h:64-76, scene.h:98-101) */
+        v3 d = l.pos;
+        double G = fabs(vdot(cn_, d));
+        double raster[2];
+        double we1 = G != 0.0 ? camera_we1(s, cpos, d, raster) : 0.0;
+        if (we1 != 0.0 && !occluded(s, cpos, d, RAY_EPS, INFINITY, cn)) {
+            double mis = 1.0;
+            if (use_mis) {
+                v3 dl = to_local(&s->cam_frame, d);
+                double camPdfA = s->per_pixel_norm / (dl.z * dl.z * dl.z);
+                mis = l.pdfA / (l.pdfA + camPdfA);
+            }
+            double k = mis * we1 * G;
+            sp[0] = k * l.contrib.r; sp[1] = k * l.contrib.g; sp[2] = k * l.contrib.b;
+            *sp_px = (int)raster[0]; *sp_py = (int)raster[1];
+            *sp_ok = 1;
+        }
+    } else if (nee) {
         v3 uv = vsub(cpos, l.pos);
         double len2 = vlen2(uv);
         v3 nuv = vmul(uv, 1.0 / sqrt(len2));
@@ -971,7 +1123,22 @@ static void eval_contribution(const orc_scene *s, int N, int use_mis, int nee, u
                 seg = cscale(tr_rgb(s, t), 1.0 / seg_pdf);
             }
         }
-        if (!in_medium && tri == UINT32_MAX) break; /* no envmap: callback(nullptr) -> stop */
+        if (!in_medium && tri == UINT32_MAX) {
+            /* the envmap vertex Scene::intersect returns on a miss (scene.h:82-88): an
+             * emitter hit (pathnextevent.h:128-146) with Le1 = 1, Le0 = the lookup, and
+             * area pdfs equal to the solid-angle ones (path.h:47-60) */
+            if (s->env_on) {
+                rgb path = cmul(prev_contrib, prev_path);
+                double mis = nee ? 0.0 : 1.0;
+                if (use_mis) {
+                    double lightPdfA = s->light_cdf.pdf[s->n_lights] * env_pdf(s, rd);
+                    mis = prevPdfW / (lightPdfA + prevPdfW);
+                }
+                rgb Le = env_eval(s, rd);
+                base[0] += mis * path.r * Le.r; base[1] += mis * path.g * Le.g; base[2] += mis * path.b * Le.b;
+            }
+            break; /* without an envmap: callback(nullptr) -> stop */
+        }
         v3 p = vadd(ro, vmul(rd, t));
         rgb path = cmul(cmul(prev_contrib, prev_path), seg);
         cNum++;
@@ -1027,7 +1194,23 @@ static void eval_contribution(const orc_scene *s, int N, int use_mis, int nee, u
             cPdfA *= fabs(vdot(ng, nuv)) / len2 * seg_pdf;
         }
         int is_light = s->geom_light[g] >= 0;
-        if (!is_light && cNum != N) {
+        if (!is_light && cNum != N && l.env) {
+            /* NEE toward an envmap vertex: G = |cos| at p, visibility to infinity */
+            v3 nuv = l.pos;
+            double G = fabs(vdot(ng, nuv));
+            if (nee && G != 0.0 && !occluded(s, p, nuv, RAY_EPS, INFINITY, cn)) {
+                const bsdf_t *b = &s->mats[s->geom_material[g]];
+                v3 wi = to_local(&f, vnorm(vsub(prev_pos, p)));
+                v3 wo = to_local(&f, nuv);
+                rgb fs = bsdf_eval(b, wi, wo);
+                if (!czero(fs)) {
+                    double mis = 1.0;
+                    if (use_mis) mis = lPdfA / (lPdfA + bsdf_pdf(b, wi, wo));
+                    rgb c = cmul(cmul(path, fs), l.contrib);
+                    base[0] += mis * G * c.r; base[1] += mis * G * c.g; base[2] += mis * G * c.b;
+                }
+            }
+        } else if (!is_light && cNum != N) {
             /* NEE (pathnextevent.h:106-127) */
             v3 uv = vsub(l.pos, p);
             double len2 = vlen2(uv);
@@ -1103,7 +1286,10 @@ static void visualize_sample(const orc_scene *s, uint32_t key, int px, int py, d
     v3 dir = camera_direction(s, lens_local, px, py, u01(key, DIM_PIXEL), u01(key, DIM_PIXEL + 1), raster, &pdfW);
     double t = INFINITY;
     uint32_t tri = closest(s, cpos, dir, &t, cn);
-    if (tri == UINT32_MAX) return;
+    if (tri == UINT32_MAX) {
+        if (s->env_on) out[0] = out[1] = out[2] = 1.0; /* the envmap vertex is a light vertex */
+        return;
+    }
     if (s->geom_light[s->tri_geom[tri]] >= 0) {
         out[0] = out[1] = out[2] = 1.0;
         return;
@@ -1137,7 +1323,7 @@ static void splat_add(const orc_scene *s, double *splat, const double raster[2],
 static void light_subpath(const orc_scene *s, int N, uint32_t key, double *splat, counters *cn) {
     if (N < 2) return;
     lvert l;
-    sample_light(s, u01(key, DIM_LIGHT), u01(key, DIM_LIGHT + 1), &l);
+    sample_light(s, u01(key, DIM_LIGHT), u01(key, DIM_LIGHT + 1), key, &l);
     v3 lens_local;
     v3 cpos = camera_position(s, u01(key, DIM_LENS), u01(key, DIM_LENS + 1), &lens_local);
     v3 cz = s->cam_frame.z;
@@ -1263,6 +1449,9 @@ static int render_mode(const orc_scene *s, const orc_render_params *p, int mode,
     if (!s || !p || !base || !splat) return fail("null argument");
     if (mode != 2 && p->num_path_vertices <= 1) { if (st) memset(st, 0, sizeof *st); return 0; } /* pathnextevent.h:163 */
     if (p->spp < 0) return fail("negative spp");
+    if ((mode == 0 || mode == 3) && s->light_cdf.n == 0) return fail("scene has no lights");
+    if (s->env_on && s->has_medium) return fail("an envmap with a medium is not supported");
+    if (s->env_on && mode == 3) return fail("the light tracer does not support envmap lights");
     job_t j;
     memset(&j, 0, sizeof j);
     j.s = s; j.p = p; j.mode = mode; j.base = base; j.splat = splat;

@@ -26,7 +26,8 @@ def main():
     data["_meta"] = {
         "generator": "oracle/ref_vectors.cpp compiled against the reference's src/ headers (asserts enabled)",
         "classes": ["CoordFrame3", "Mapping", "CdfTable", "LambertDiffuseBsdf", "OrenNayarBsdf", "PhongBsdf",
-                    "MixedBsdf", "TwoSidedBsdf", "DiffuseAreaLight", "TriangleMeshGeometry", "ThinlensCamera"],
+                    "MixedBsdf", "TwoSidedBsdf", "DiffuseAreaLight", "TriangleMeshGeometry", "ThinlensCamera",
+                    "EquiRectRep", "Fimage"],
     }
     with open(os.path.join(HERE, "ref_vectors.json"), "w") as f:
         json.dump(data, f, separators=(",", ":"))

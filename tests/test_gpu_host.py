@@ -71,3 +71,23 @@ def test_cli_primitive_and_visualize_equal_api(tmp_path, gpu_device):
     fl = wr.Film(gs)
     wr.render_light(gs, fl, 4, num_path_vertices=8, seed=7)  # FP64 splat atomics: summation order only
     np.testing.assert_allclose(read_pfm(tmp_path / "light.pfm"), fl.image().astype(np.float32), rtol=1e-5, atol=1e-6)
+
+
+def test_cli_envmap_scene_equals_api(tmp_path, gpu_device):
+    """A scene JSON with an "envmap" .pfm and no area light renders through the
+    command line as through the Python API (same image to FP32 rounding)."""
+    sc = scenes.icospheres(2, n_spheres=6, level=2, resolution=(40, 24))
+    sc.lights = sc.lights[:0]
+    sc.geom_light = np.full_like(sc.geom_light, -1)
+    sc.geom_material = np.where(sc.geom_material < 0, 0, sc.geom_material).astype(np.int32)
+    sc = sc.with_envmap(np.random.default_rng(2).uniform(0.1, 2.0, (16, 32, 3)).astype(np.float32))
+    scenes.write_obj(sc, str(tmp_path), "mesh")
+    j = scenes.scene_json(sc, "mesh.obj", [{"integrator": "nextevent_path_tracer", "output": "out.pfm",
+                                            "num_spp": 8, "num_path_vertices": 6, "seed": 3}])
+    (tmp_path / "scene.json").write_text(json.dumps(j))
+    r = subprocess.run([EXE, "--simple2parse", str(tmp_path / "scene.json")], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    gs = wr.GpuScene(gpu_device, sc)
+    film = wr.Film(gs)
+    wr.render_nee(gs, film, 8, num_path_vertices=6, seed=3)
+    np.testing.assert_allclose(read_pfm(tmp_path / "out.pfm"), film.image().astype(np.float32), rtol=1e-5, atol=1e-6)

@@ -165,3 +165,20 @@ def test_other_integrators_of_this_build_parse(tmp_path):
         p = write(tmp_path, sc, renders=({"integrator": name},))
         r = run(["--check", "--simple2parse", str(p)])
         assert r.returncode != 0 and "not part of this build" in r.stderr
+
+
+def test_envmap_pfm_is_loaded_and_checked(tmp_path):
+    """The scene JSON's "envmap" (a .pfm, as FimageIo::LoadPfm reads it) lights a
+    scene without area lights; other formats and a non-2:1 image are errors."""
+    sc = scenes.icospheres(2, n_spheres=4, level=1, resolution=(16, 8))
+    sc.lights = sc.lights[:0]
+    sc.geom_light = np.full_like(sc.geom_light, -1)
+    sc.geom_material = np.where(sc.geom_material < 0, 0, sc.geom_material).astype(np.int32)
+    r = run(["--check", "--simple2parse", str(write(tmp_path, sc))])
+    assert r.returncode == 1 and "no lights" in r.stderr
+    sky = sc.with_envmap(np.random.default_rng(1).uniform(0.1, 2.0, (8, 16, 3)))
+    r = run(["--check", "--simple2parse", str(write(tmp_path, sky))])
+    assert r.returncode == 0, r.stderr
+    bad = sc.with_envmap(np.ones((8, 8, 3)))
+    r = run(["--check", "--simple2parse", str(write(tmp_path, bad))])
+    assert r.returncode == 1 and "wrong size" in r.stderr

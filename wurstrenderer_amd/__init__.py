@@ -129,6 +129,7 @@ EXPORTS = {
     "wr_scene_info": (C.c_int, [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint32),
                                 C.POINTER(C.c_uint64)]),
     "wr_scene_set_medium": (C.c_int, [P, C.POINTER(Medium)]),
+    "wr_scene_set_envmap": (C.c_int, [P, P, C.c_uint32, C.c_uint32]),
     "wr_film_create": (C.c_int, [P, C.POINTER(P)]),
     "wr_film_destroy": (None, [P]),
     "wr_film_clear": (C.c_int, [P]),
@@ -267,6 +268,18 @@ class GpuScene:
         del keep
         if getattr(sc, "medium", None):
             self.set_medium(sc.medium["sigma_a"], sc.medium["sigma_s"], sc.medium["g"])
+        if getattr(sc, "envmap", None) is not None:
+            self.set_envmap(sc.envmap)
+
+    def set_envmap(self, rgb):
+        """Equirectangular envmap light: float (H, 2H, 3); None removes it."""
+        if rgb is None:
+            _check(lib().wr_scene_set_envmap(self.h, None, 0, 0))
+            return
+        a = np.ascontiguousarray(rgb, np.float32)
+        if a.ndim != 3 or a.shape[2] != 3:
+            raise ValueError(f"envmap must be (H, W, 3), got {a.shape}")
+        _check(lib().wr_scene_set_envmap(self.h, a.ctypes.data, a.shape[1], a.shape[0]))
 
     def set_medium(self, sigma_a=(0, 0, 0), sigma_s=(0, 0, 0), g=0.0):
         """Homogeneous medium filling the scene with an HG phase function (zeros: vacuum)."""

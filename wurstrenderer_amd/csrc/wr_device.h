@@ -63,6 +63,13 @@ struct DScene {
     uint32_t n_lights;
     DCamera cam;
     DMedium med;
+    // envmap light (EnvmapLight over an equirectangular image, light/envmap.h):
+    // texels [env_h][env_w] (w = 2h), the texel CdfTable's cdf and pdf; env_select =
+    // its light-selection pdf (0: no envmap)
+    const float4 *env_tex;
+    const float *env_cdf, *env_pdf;
+    int env_w, env_h;
+    float env_select;
 };
 
 // ---------------------------------------------------------------- float3 helpers
@@ -99,6 +106,10 @@ __device__ __forceinline__ float u01(uint32_t key, uint32_t dim) {
     return float(mix32(key + dim * 0x9e3779b9u + 0x632be5abu) >> 8) * 0x1p-24f;
 }
 enum : uint32_t { kDimLight = 0, kDimLens = 2, kDimPixel = 4, kDimBounce0 = 6, kDimsPerBounce = 6, kBdMedium = 0, kBdBsdf = 2 };
+// position inside the envmap texel of a light sample: fresh numbers (re-using the
+// twice-remapped selection number, as the FP64 reference does, would leave FP32
+// a few bits of it)
+constexpr uint32_t kDimEnv = 0x00fffff0u;
 
 // ---------------------------------------------------------------- frames and warps
 struct Frame { float3 x, y, z; };
@@ -271,11 +282,89 @@ __device__ __forceinline__ float3 bsdf_sample(const DMat &m, float3 wi, float u0
 }
 
 // ---------------------------------------------------------------- lights
-struct LightSample { float3 pos, n, contrib; float pdfA; };
+// env: an envmap vertex, whose pos is its direction (path.h:126-131)
+struct LightSample { float3 pos, n, contrib; float pdfA; bool env; };
+
+// ---------------------------------------------------------------- envmap (light/envmap.h, equirectangular.h)
+// Mapping::PanoramaFromWorld / WorldFromPanorama (mapping.h:91-96,302-323)
+__device__ __forceinline__ void panorama_from_world(float3 d, float &u, float &v) {
+    u = atan2f(-d.z, -d.x) * (0.5f * kInvPi) + 0.5f;
+    v = acosf(fminf(1.f, fmaxf(-1.f, d.y))) * kInvPi;
+}
+// Fimage::evalTexel, S repeat, T clamp (floatimage.h:91-112)
+__device__ __forceinline__ float3 env_texel(const DScene &s, int x, int y) {
+    x %= s.env_w;
+    if (x < 0) x += s.env_w;
+    y = min(max(y, 0), s.env_h - 1);
+    return xyz(s.env_tex[size_t(y) * s.env_w + x]);
+}
+// EquiRectRep::eval = Fimage::evalBilinear (floatimage.h:120-133)
+__device__ __forceinline__ float3 env_eval(const DScene &s, float3 d) {
+    float u, v;
+    panorama_from_world(d, u, v);
+    const float sx = u * float(s.env_w) - 0.5f, sy = v * float(s.env_h) - 0.5f;
+    const float fx = floorf(sx), fy = floorf(sy);
+    const int x = int(fx), y = int(fy);
+    const float dx1 = sx - fx, dy1 = sy - fy, dx2 = 1.f - dx1, dy2 = 1.f - dy1;
+    return env_texel(s, x, y) * (dx2 * dy2) + env_texel(s, x, y + 1) * (dx2 * dy1) +
+           env_texel(s, x + 1, y) * (dx1 * dy2) + env_texel(s, x + 1, y + 1) * (dx1 * dy1);
+}
+// EquiRectRep::pdfA: solid-angle pdf of sampling d (equirectangular.h:64-71)
+__device__ __forceinline__ float env_pdf_dir(const DScene &s, float3 d) {
+    float u, v;
+    panorama_from_world(d, u, v);
+    // clamped at both ends: a non-finite direction still reads inside the table
+    const int x = min(max(int(u * float(s.env_w)), 0), s.env_w - 1), y = min(max(int(v * float(s.env_h)), 0), s.env_h - 1);
+    const float sine = sqrtf(fmaxf(0.f, 1.f - d.y * d.y));
+    if (!(sine > 1e-12f)) return 0.f;
+    return s.env_pdf[size_t(y) * s.env_w + x] * float(s.env_w * s.env_h) / sine * (0.5f * kInvPi * kInvPi);
+}
+// EquiRectRep::sample (equirectangular.h:85-117): texel by the CdfTable
+// (lower bound), (ox, oy) inside it; returns eval / pdf, or 0 where pdf = 0
+__device__ __forceinline__ float3 env_sample(const DScene &s, float u, float ox, float oy, float3 &d, float &pdf) {
+    uint32_t lo = 0, hi = uint32_t(s.env_w * s.env_h) - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (s.env_cdf[mid] < u) lo = mid + 1; else hi = mid;
+    }
+    const int x = int(lo % uint32_t(s.env_w)), y = int(lo / uint32_t(s.env_w));
+    const float th = (float(x) + ox) / float(s.env_w) * (2.f * kPi), ph = (float(y) + oy) / float(s.env_h) * kPi;
+    float st, ct, sp, cp;
+    sincosf(th, &st, &ct);
+    sincosf(ph, &sp, &cp);
+    d = f3(ct * sp, cp, st * sp);
+    const float sine = sqrtf(fmaxf(0.f, 1.f - d.y * d.y));
+    pdf = sine > 1e-12f ? s.env_pdf[lo] * float(s.env_w * s.env_h) / sine * (0.5f * kInvPi * kInvPi) : 0.f;
+    return pdf > 0.f ? env_eval(s, d) * (1.f / pdf) : f3(0.f, 0.f, 0.f);
+}
 // createLightSubpath(numVertices = 1) -> CdfTable::sample -> DiffuseAreaLight::sampleLe0ByPdf
 // -> TriangleMeshGeometry::sample (generalsubpathsampler.h:199-255, diffusearealight.h:53-72,
 // trianglemesh.h:131-159, cdftable.h:36-44, mapping.h:21-27)
-__device__ __forceinline__ LightSample sample_light(const DScene &s, float u0, float u1) {
+// ENV: the kernel variant for scenes with an envmap (the branch costs registers
+// that the kernels of scenes without one keep).
+template <bool ENV = false>
+__device__ __forceinline__ LightSample sample_light(const DScene &s, float u0, float u1, uint32_t key) {
+    LightSample ls;
+    ls.env = false;
+    if (ENV && s.env_select > 0.f && (s.n_lights == 0 || u0 > s.lights[s.n_lights - 1].cdf_select)) {
+        // the envmap, last entry of the selection CdfTable (EnvmapLight::sampleLe0ByPdf, envmap.h:61-77)
+        const float prev = s.n_lights == 0 ? 0.f : s.lights[s.n_lights - 1].cdf_select;
+        const float r = fminf((u0 - prev) / s.env_select, kOneMinusEps);
+        float pdfW;
+        const float3 v = env_sample(s, r, u01(key, kDimEnv), u01(key, kDimEnv + 1), ls.pos, pdfW);
+        ls.n = f3(0.f, 0.f, 0.f);
+        ls.contrib = v * (1.f / s.env_select);
+        ls.pdfA = s.env_select * pdfW;
+        ls.env = true;
+        return ls;
+    }
+    if (s.n_lights == 0) {  // nothing to sample: a zero contribution
+        ls.pos = f3(0.f, 0.f, 0.f);
+        ls.n = f3(0.f, 1.f, 0.f);
+        ls.contrib = f3(0.f, 0.f, 0.f);
+        ls.pdfA = 1.f;
+        return ls;
+    }
     uint32_t li = 0;
     while (li + 1 < s.n_lights && s.lights[li].cdf_select < u0) li++;
     const DLight &L = s.lights[li];
@@ -295,7 +384,6 @@ __device__ __forceinline__ LightSample sample_light(const DScene &s, float u0, f
     float r2 = fminf((lo == 0 ? r : (r - prev_t)) / p1.w, kOneMinusEps);
     float sq = sqrtf(r2);
     float b1 = sq * (1.f - u1), b2 = u1 * sq;
-    LightSample ls;
     float3 a = xyz(p0), b = xyz(p1), c = xyz(p2);
     ls.pos = a * (1.f - b1 - b2) + b * b1 + c * b2;
     ls.n = normalize3(cross3(b - a, c - a));

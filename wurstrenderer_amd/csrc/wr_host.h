@@ -112,6 +112,10 @@ struct wr_scene {
     int2 *d_geom = nullptr;
     DMat *d_mats = nullptr;
     DLight *d_lights = nullptr;
+    float4 *d_env = nullptr;            // envmap texels; cdf and pdf follow in d_env_cdf
+    float *d_env_cdf = nullptr;
+    std::vector<DLight> h_lights;       // host copy of the light table
+    std::vector<double> light_w;        // area-light selection weights |L|_2
     uint64_t n_nodes = 0, n_tris = 0, device_bytes = 0;
     uint32_t max_stack = 0;
     int builder = WR_BVH_GPU;           // which builder made the BVH

@@ -195,7 +195,7 @@ __device__ __forceinline__ float3 primary_dir(const DCamera &cam, uint32_t key, 
 // generalsubpathsampler.h:185-268): its light vertex connects to the camera
 // with weight 1, and its first ray leaves the light in a cosine-distributed
 // direction (DiffuseAreaLight::sampleLe1CosByPdf, contribution 1).
-template <bool BATCH>
+template <bool BATCH, bool ENV = false>
 __global__ __launch_bounds__(kShadeBlock) void k_regen(DScene s1, const BatchJob *__restrict__ jobs, uint32_t n_jobs,
                                                        PathBuffers pb, RenderArgs a) {
     __shared__ uint32_t agg[kWavesPerShadeBlock + 1];
@@ -235,8 +235,31 @@ __global__ __launch_bounds__(kShadeBlock) void k_regen(DScene s1, const BatchJob
             const float3 cpos = cam_to_world(cam, lens) + f3(cam.ox, cam.oy, cam.oz);
             const float3 cz = f3(cam.zx, cam.zy, cam.zz);
             // t = 1 connection
-            const LightSample l = sample_light(s, u01(key, kDimLight), u01(key, kDimLight + 1));
-            if (a.nee || a.light) {
+            const LightSample l = sample_light<ENV>(s, u01(key, kDimLight), u01(key, kDimLight + 1), key);
+            if (ENV && a.nee && l.env) {
+                // an envmap vertex: the camera looks along its direction; G = |cos| at
+                // the camera only, Le1 = 1, visibility to infinity (path.h:64-76, scene.h:98-101)
+                const float3 d = l.pos;
+                const float G = fabsf(dot3(cz, d));
+                float rx = 0.f, ry = 0.f;
+                const float we1 = G != 0.f ? camera_we1(cam, lens, d, rx, ry) : 0.f;
+                if (we1 != 0.f) {
+                    float mis = 1.f;
+                    if (use_mis) {
+                        const float c = cam_to_local(cam, d).z;
+                        const float camPdfA = cam.per_pixel_norm / (c * c * c);
+                        mis = l.pdfA / (l.pdfA + camPdfA);
+                    }
+                    const float3 c = l.contrib * (mis * we1 * G);
+                    if (!is_black(c)) {
+                        const uint32_t target = kToSplat | (uint32_t(ry) * uint32_t(cam.W) + uint32_t(rx));
+                        so = make_float4(cpos.x, cpos.y, cpos.z, INFINITY);
+                        sd = make_float4(d.x, d.y, d.z, __uint_as_float(target));
+                        sc = make_float4(c.x, c.y, c.z, __uint_as_float(job));
+                        splat = true;
+                    }
+                }
+            } else if (a.nee || a.light) {
                 const float3 uv = cpos - l.pos;
                 const float len2 = dot3(uv, uv);
                 const float len = sqrtf(len2);
@@ -423,7 +446,7 @@ __global__ __launch_bounds__(kBlock) WR_EXTEND_ATTR void k_extend_fast(DScene s1
 // (generalsubpathsampler.h:34-130, pathnextevent.h:100-149).  Continuing
 // paths are compacted into the next state; a finished path stores its
 // radiance in its sample's slot of the pass.
-template <bool BATCH>
+template <bool BATCH, bool ENV = false>
 __global__ __launch_bounds__(kShadeBlock) void k_shade(DScene s1, const BatchJob *__restrict__ jobs, PathBuffers pb,
                                                        RenderArgs a) {
     __shared__ uint32_t agg[kWavesPerShadeBlock + 1];
@@ -447,6 +470,21 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(DScene s1, const BatchJob
             const uint32_t bounce = meta.z & kBounceMask;
             const int cNum = int(bounce) + 2;  // camera is vertex 1
             const uint2 h = pb.hit[i];
+            if (ENV && h.x == kInvalid && s.env_select > 0.f) {
+                // a miss is the envmap vertex (scene.h:82-88): an emitter hit with Le1 = 1,
+                // Le0 = the lookup, and area pdfs equal to the solid-angle ones
+                const float3 rd = xyz(cs.ray_d[i]);
+                const float4 thr = cs.thr[i];
+                float mis = a.nee ? 0.f : 1.f;
+                if (use_mis) {
+                    const float lightPdfA = s.env_select * env_pdf_dir(s, rd);
+                    mis = thr.w / (lightPdfA + thr.w);
+                }
+                const float3 c = xyz(thr) * env_eval(s, rd) * mis;
+                rad.x += c.x;
+                rad.y += c.y;
+                rad.z += c.z;
+            }
             if (h.x != kInvalid) {
                 const float t = __uint_as_float(h.y);
                 const float3 ro = xyz(cs.ray_o[i]), rd = xyz(cs.ray_d[i]);
@@ -480,14 +518,31 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(DScene s1, const BatchJob
                     const Frame f = make_frame(ng);
                     const float3 wi = to_local(f, -rd);
                     // NEE with MIS (pathnextevent.h:106-127)
-                    const LightSample l = sample_light(s, u01(key, kDimLight), u01(key, kDimLight + 1));
+                    const LightSample l = sample_light<ENV>(s, u01(key, kDimLight), u01(key, kDimLight + 1), key);
+                    if (ENV && a.nee && l.env) {
+                        // toward an envmap vertex: G = |cos| at the surface, visibility to infinity
+                        const float3 nuv = l.pos;
+                        const float G = fabsf(dot3(ng, nuv));
+                        const float3 wo = to_local(f, nuv);
+                        const float3 fs = G != 0.f ? bsdf_eval(m, wi, wo) : f3(0.f, 0.f, 0.f);
+                        if (!is_black(fs)) {
+                            const float mis = use_mis ? l.pdfA / (l.pdfA + bsdf_pdf(m, wi, wo)) : 1.f;
+                            const float3 c = T * fs * l.contrib * (mis * G);
+                            if (!is_black(c)) {
+                                so = make_float4(pos.x, pos.y, pos.z, INFINITY);
+                                sd = make_float4(nuv.x, nuv.y, nuv.z, 0.f);
+                                sc = make_float4(c.x, c.y, c.z, 0.f);
+                                push_shadow = true;
+                            }
+                        }
+                    }
                     const float3 uv = l.pos - pos;
                     const float len2 = dot3(uv, uv);
                     const float len = sqrtf(len2);
                     const float3 nuv = uv * (1.0f / len);
                     const float cos_lv = -dot3(l.n, nuv);
                     const float G = fabsf(dot3(ng, nuv)) * fabsf(cos_lv) / len2;
-                    if (a.nee && G != 0.f && cos_lv >= 0.f) {
+                    if (a.nee && !(ENV && l.env) && G != 0.f && cos_lv >= 0.f) {
                         const float3 wo = to_local(f, nuv);
                         const float3 fs = bsdf_eval(m, wi, wo);
                         if (!is_black(fs)) {
@@ -746,7 +801,7 @@ __global__ __launch_bounds__(kInferBlock) __attribute__((amdgpu_waves_per_eu(WR_
                 float wl[guide::kLobes];
                 guide::decode(L, wl);
                 const uint32_t key = meta.x;
-                const LightSample l = sample_light(s, u01(key, kDimLight), u01(key, kDimLight + 1));
+                const LightSample l = sample_light(s, u01(key, kDimLight), u01(key, kDimLight + 1), key);
                 const float3 dl = normalize3(l.pos - pos);
                 const uint32_t dim = kDimBounce0 + kDimsPerBounce * meta.z + kBdBsdf;
                 const float u0 = u01(key, dim), u1 = u01(key, dim + 1), us = u01(key, dim + 2);
@@ -884,7 +939,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade_ext(DScene s, PathBuffers
             }
             const float alpha = need ? (surf ? guide_alpha(m, g.alpha) : g.alpha) : 0.f;
             // NEE with MIS against the density that samples directions here (pathnextevent.h:106-127)
-            const LightSample l = sample_light(s, u01(key, kDimLight), u01(key, kDimLight + 1));
+            const LightSample l = sample_light(s, u01(key, kDimLight), u01(key, kDimLight + 1), key);
             const float3 uv = l.pos - pos;
             const float len2 = dot3(uv, uv);
             const float len = sqrtf(len2);
@@ -1122,7 +1177,10 @@ __global__ __launch_bounds__(kBlock) void k_visualize(DScene s, uint32_t wh, uin
             const float3 dir = primary_dir(cam, key, px, py, lens, pdfW);
             float t = INFINITY;
             const uint32_t tri = traverse<false, false, kBlock>(s, setup_ray(cpos, dir, kRayEps), t, st, cnt);
-            if (tri == kInvalid) continue;
+            if (tri == kInvalid) {
+                if (s.env_select > 0.f) { r += 1.0; g += 1.0; b += 1.0; }  // the envmap vertex is a light vertex
+                continue;
+            }
             const float4 v0 = s.tris[3 * size_t(tri) + 0], v1 = s.tris[3 * size_t(tri) + 1],
                          v2 = s.tris[3 * size_t(tri) + 2];
             if (s.geom_info[__float_as_uint(v1.w)].y >= 0) {
@@ -1192,6 +1250,8 @@ void wr_scene_destroy(wr_scene *s) {
     (void)hipFree(s->d_geom);
     (void)hipFree(s->d_mats);
     (void)hipFree(s->d_lights);
+    (void)hipFree(s->d_env);
+    (void)hipFree(s->d_env_cdf);
     (void)hipFree(s->d_spill);
     (void)hipFree(s->pool);
     if (s->h_counters) (void)hipHostFree(s->h_counters);
@@ -1206,7 +1266,6 @@ static int validate_desc(const wr_scene_desc *d) {
     if (d->n_tris > 0 && (!d->verts || !d->tris || !d->tri_geom))
         return set_error(WR_ERR_INVALID_ARGUMENT, "null geometry arrays");
     if (d->n_tris >= kMaxTriangles) return set_error(WR_ERR_UNSUPPORTED, "too many triangles (limit 2^28)");
-    if (d->n_lights == 0) return set_error(WR_ERR_INVALID_ARGUMENT, "scene has no lights");
     if (d->n_verts > 0 && !d->verts) return set_error(WR_ERR_INVALID_ARGUMENT, "null vertex array");
     for (size_t i = 0; i < 3 * size_t(d->n_verts); i++)
         if (!std::isfinite(d->verts[i])) return set_error(WR_ERR_INVALID_ARGUMENT, "non-finite vertex coordinate");
@@ -1371,7 +1430,8 @@ static int scene_create_impl(wr_device *dev, const wr_scene_desc *d, int builder
         w[l] = std::sqrt(double(L[0]) * L[0] + double(L[1]) * L[1] + double(L[2]) * L[2]);
         wsum += w[l];
     }
-    if (!(wsum > 0.0)) return set_error(WR_ERR_INVALID_ARGUMENT, "all lights have zero radiance");
+    if (d->n_lights > 0 && !(wsum > 0.0)) return set_error(WR_ERR_INVALID_ARGUMENT, "all lights have zero radiance");
+    s->light_w = w;
     double cdf = 0.0;
     for (uint32_t l = 0; l < d->n_lights; l++) {
         DLight &dl = lights[l];
@@ -1413,7 +1473,8 @@ static int scene_create_impl(wr_device *dev, const wr_scene_desc *d, int builder
             ltris.push_back(make_float4(c[0], c[1], c[2], 0.f));
         }
     }
-    rc = upload(&s->d_lights, lights.data(), sizeof(DLight) * lights.size(), s->device_bytes);
+    s->h_lights = lights;
+    rc = upload(&s->d_lights, lights.data(), sizeof(DLight) * lights.size(), s->device_bytes);  // >= 128 B allocated
     if (!rc) rc = upload(&s->d_ltris, ltris.data(), sizeof(float4) * ltris.size(), s->device_bytes);
     if (rc) return rc;
     // camera (ThinlensCamera ctor, thinlens.h:40-55; Camera ctor, camera.h:9-17), FP64 then rounded
@@ -1575,6 +1636,89 @@ int wr_scene_set_medium(wr_scene *s, const wr_medium *m) {
     }
     s->ds.med = dm;
     return WR_OK;
+}
+
+// The light-selection CdfTable over the area lights' |L|_2 and, last, the
+// envmap's weight 1.23 (generalsubpathsampler.h:167-176; envmap.h:22-26).
+static int rebuild_light_select(wr_scene *s, bool env) {
+    double wsum = env ? 1.23 : 0.0;
+    for (double w : s->light_w) wsum += w;
+    double cdf = 0.0;
+    for (size_t l = 0; l < s->h_lights.size(); l++) {
+        const double pdf = s->light_w[l] / wsum;
+        cdf += pdf;
+        s->h_lights[l].pdf_select = float(pdf);
+        s->h_lights[l].cdf_select = (!env && l + 1 == s->h_lights.size()) ? 1.0f : float(cdf);
+    }
+    s->ds.env_select = env ? float(1.23 / wsum) : 0.f;
+    if (!s->h_lights.empty())
+        WR_HIP(hipMemcpy(s->d_lights, s->h_lights.data(), sizeof(DLight) * s->h_lights.size(), hipMemcpyHostToDevice));
+    return WR_OK;
+}
+
+int wr_scene_set_envmap(wr_scene *s, const float *rgb, uint32_t width, uint32_t height) {
+    if (!s) return set_error(WR_ERR_INVALID_ARGUMENT, "scene is null");
+    int rc = set_device(s->dev);
+    if (rc) return rc;
+    WR_HIP(hipStreamSynchronize(s->stream));
+    if (!rgb || width == 0) {
+        (void)hipFree(s->d_env);
+        (void)hipFree(s->d_env_cdf);
+        s->d_env = nullptr;
+        s->d_env_cdf = nullptr;
+        s->ds.env_tex = nullptr;
+        s->ds.env_cdf = s->ds.env_pdf = nullptr;
+        s->ds.env_w = s->ds.env_h = 0;
+        return rebuild_light_select(s, false);
+    }
+    if (height == 0 || width != 2 * height) return set_error(WR_ERR_INVALID_ARGUMENT, "envmap has wrong size");
+    if (uint64_t(width) * height > (uint64_t(1) << 26)) return set_error(WR_ERR_UNSUPPORTED, "envmap too large");
+    const size_t n = size_t(width) * height;
+    std::vector<float4> tex(n);
+    std::vector<double> w(n);
+    double sum = 0.0;
+    for (uint32_t y = 0; y < height; y++) {
+        const double sine = std::sin((double(y) + 0.5) / height * 3.14159265358979323846);
+        for (uint32_t x = 0; x < width; x++) {
+            const size_t i = size_t(y) * width + x;
+            const float *t = rgb + 3 * i;
+            for (int c = 0; c < 3; c++)
+                if (!(t[c] >= 0.f) || !std::isfinite(t[c]))
+                    return set_error(WR_ERR_INVALID_ARGUMENT, "envmap texels must be finite and >= 0");
+            tex[i] = make_float4(t[0], t[1], t[2], 0.f);
+            // EquiRectRep's texel weight: sin(theta) |rgb|_2 (equirectangular.h:40-58)
+            w[i] = sine * std::sqrt(double(t[0]) * t[0] + double(t[1]) * t[1] + double(t[2]) * t[2]);
+            sum += w[i];
+        }
+    }
+    if (!(sum > 0.0)) return set_error(WR_ERR_INVALID_ARGUMENT, "envmap is black");
+    std::vector<float> cp(2 * n);  // cdf, then pdf (CdfTable in FP64, rounded)
+    double cdf = 0.0;
+    for (size_t i = 0; i < n; i++) {
+        const double pdf = w[i] / sum;
+        cdf += pdf;
+        cp[i] = i + 1 == n ? 1.0f : float(cdf);
+        cp[n + i] = float(pdf);
+    }
+    float4 *d_tex = nullptr;
+    float *d_cp = nullptr;
+    uint64_t bytes = 0;
+    rc = upload(&d_tex, tex.data(), sizeof(float4) * n, bytes);
+    if (!rc) rc = upload(&d_cp, cp.data(), sizeof(float) * 2 * n, bytes);
+    if (rc) {
+        (void)hipFree(d_tex);
+        return rc;
+    }
+    (void)hipFree(s->d_env);
+    (void)hipFree(s->d_env_cdf);
+    s->d_env = d_tex;
+    s->d_env_cdf = d_cp;
+    s->ds.env_tex = d_tex;
+    s->ds.env_cdf = d_cp;
+    s->ds.env_pdf = d_cp + n;
+    s->ds.env_w = int(width);
+    s->ds.env_h = int(height);
+    return rebuild_light_select(s, true);
 }
 
 int wr_film_create(wr_scene *s, wr_film **out) {
@@ -1776,6 +1920,11 @@ static int render_impl(wr_scene *s, wr_film *f, const wr_render_params *prm, wr_
     if (g && guide_scene(g) != s) return set_error(WR_ERR_INVALID_ARGUMENT, "guide belongs to another scene");
     if (g && !(gp->guide_fraction >= 0.f && gp->guide_fraction < 1.f))
         return set_error(WR_ERR_INVALID_ARGUMENT, "guide_fraction must be in [0, 1)");
+    const bool env = s->ds.env_select > 0.f;
+    if (est != kEstPrimitive && s->ds.n_lights == 0 && !env) return set_error(WR_ERR_INVALID_ARGUMENT, "scene has no lights");
+    if (env && est == kEstLight) return set_error(WR_ERR_UNSUPPORTED, "the light tracer does not support envmap lights");
+    if (env && g) return set_error(WR_ERR_UNSUPPORTED, "guided rendering does not support envmap lights");
+    if (env && s->ds.med.on) return set_error(WR_ERR_UNSUPPORTED, "an envmap with a medium is not supported");
     wr_guide_stats gst{};
     const auto t0 = std::chrono::steady_clock::now();
     wr_render_stats st{};
@@ -1836,6 +1985,7 @@ static int render_impl(wr_scene *s, wr_film *f, const wr_render_params *prm, wr_
     if (est == kEstLight) ds.med.on = 0;
     const bool medium = ds.med.on != 0;
     const void *shade_fn = est == kEstLight ? reinterpret_cast<const void *>(&k_shade_light)
+                           : env            ? reinterpret_cast<const void *>(&k_shade<false, true>)
                                             : reinterpret_cast<const void *>(&k_shade<false>);
     if (g && medium) shade_fn = reinterpret_cast<const void *>(&k_shade_ext<true, true>);
     else if (g) shade_fn = reinterpret_cast<const void *>(&k_shade_ext<true, false>);
@@ -1886,7 +2036,8 @@ static int render_impl(wr_scene *s, wr_film *f, const wr_render_params *prm, wr_
         const uint64_t max_iters = (uint64_t(a.total) / P + 2) * uint64_t(N + 1) + 64;
         auto iteration = [&](const PathBuffers &q) {
             begin(3);
-            k_regen<false><<<g_regen, kShadeBlock, 0, s->stream>>>(ds, nullptr, 0, q, a);
+            if (env) k_regen<false, true><<<g_regen, kShadeBlock, 0, s->stream>>>(ds, nullptr, 0, q, a);
+            else k_regen<false><<<g_regen, kShadeBlock, 0, s->stream>>>(ds, nullptr, 0, q, a);
             end();
             begin(0);
             if (count) k_extend<true, false><<<g_ext, kBlock, 0, s->stream>>>(ds, nullptr, q, s->d_spill, tune);
@@ -1911,6 +2062,7 @@ static int render_impl(wr_scene *s, wr_film *f, const wr_render_params *prm, wr_
             else if (g) k_shade_ext<true, false><<<g_shade, kShadeBlock, 0, s->stream>>>(ds, q, a, gs);
             else if (medium) k_shade_ext<false, true><<<g_shade, kShadeBlock, 0, s->stream>>>(ds, q, a, gs);
             else if (est == kEstLight) k_shade_light<<<g_shade, kShadeBlock, 0, s->stream>>>(ds, q, a);
+            else if (env) k_shade<false, true><<<g_shade, kShadeBlock, 0, s->stream>>>(ds, nullptr, q, a);
             else k_shade<false><<<g_shade, kShadeBlock, 0, s->stream>>>(ds, nullptr, q, a);
             end();
             begin(1);
@@ -2030,6 +2182,9 @@ int wr_render_batch(const wr_batch_job *jobs, uint32_t n_jobs, const wr_render_p
         if (J.num_path_vertices > int(kBounceMask)) return set_error(WR_ERR_UNSUPPORTED, "num_path_vertices above 255");
         if (uint64_t(J.scene->W) * J.scene->H > kMaxSamplesPerPass)
             return set_error(WR_ERR_UNSUPPORTED, "resolution too large for a batch pass");
+        if (J.scene->ds.med.on) return set_error(WR_ERR_UNSUPPORTED, "batch scenes with a medium are not supported");
+        if (J.scene->ds.n_lights == 0 && !(J.scene->ds.env_select > 0.f))
+            return set_error(WR_ERR_INVALID_ARGUMENT, "scene has no lights");
     }
     const auto t0 = std::chrono::steady_clock::now();
     wr_render_stats st{};
@@ -2087,8 +2242,12 @@ int wr_render_batch(const wr_batch_job *jobs, uint32_t n_jobs, const wr_render_p
     auto grid = [&](const void *fn, int block) {
         return std::max(1, std::min<int>((P + block - 1) / block, dev->n_cu * occupancy(dev, fn, block)));
     };
-    const int g_regen = grid(reinterpret_cast<const void *>(&k_regen<true>), kShadeBlock);
-    const int g_shade = grid(reinterpret_cast<const void *>(&k_shade<true>), kShadeBlock);
+    bool env = false;
+    for (uint32_t j = 0; j < n_jobs; j++) env = env || jobs[j].scene->ds.env_select > 0.f;
+    const int g_regen = grid(env ? reinterpret_cast<const void *>(&k_regen<true, true>)
+                                 : reinterpret_cast<const void *>(&k_regen<true>), kShadeBlock);
+    const int g_shade = grid(env ? reinterpret_cast<const void *>(&k_shade<true, true>)
+                                 : reinterpret_cast<const void *>(&k_shade<true>), kShadeBlock);
     const int g_ext = trav_grid(s0, P, occupancy(dev, reinterpret_cast<const void *>(&k_extend<false, true>), kBlock));
     const int g_shd = trav_grid(s0, 2 * P, occupancy(dev, reinterpret_cast<const void *>(&k_shadow<false, true>), kBlock));
     std::vector<BatchJob> tab;
@@ -2133,14 +2292,16 @@ int wr_render_batch(const wr_batch_job *jobs, uint32_t n_jobs, const wr_render_p
         const uint64_t max_iters = (uint64_t(a.total) / P + 2) * uint64_t(maxN + 1) + 64;
         auto iteration = [&](const PathBuffers &q) {
             begin(3);
-            k_regen<true><<<g_regen, kShadeBlock, 0, s0->stream>>>(s0->ds, d_tab, nt, q, a);
+            if (env) k_regen<true, true><<<g_regen, kShadeBlock, 0, s0->stream>>>(s0->ds, d_tab, nt, q, a);
+            else k_regen<true><<<g_regen, kShadeBlock, 0, s0->stream>>>(s0->ds, d_tab, nt, q, a);
             end();
             begin(0);
             if (count) k_extend<true, true><<<g_ext, kBlock, 0, s0->stream>>>(s0->ds, d_tab, q, s0->d_spill, tune);
             else k_extend<false, true><<<g_ext, kBlock, 0, s0->stream>>>(s0->ds, d_tab, q, s0->d_spill, tune);
             end();
             begin(2);
-            k_shade<true><<<g_shade, kShadeBlock, 0, s0->stream>>>(s0->ds, d_tab, q, a);
+            if (env) k_shade<true, true><<<g_shade, kShadeBlock, 0, s0->stream>>>(s0->ds, d_tab, q, a);
+            else k_shade<true><<<g_shade, kShadeBlock, 0, s0->stream>>>(s0->ds, d_tab, q, a);
             end();
             begin(1);
             if (count) k_shadow<true, true><<<g_shd, kBlock, 0, s0->stream>>>(s0->ds, d_tab, q, s0->d_spill, tune);

@@ -284,6 +284,7 @@ void PathNeeRenderer::render() {
     wr_guide *g = nullptr;
     check(wr_scene_create(ctx->get(), &d, &sc));
     int rc = camera->has_medium ? wr_scene_set_medium(sc, &camera->medium) : WR_OK;
+    if (rc == WR_OK && !scene->env_rgb.empty()) rc = wr_scene_set_envmap(sc, scene->env_rgb.data(), scene->env_w, scene->env_h);
     if (rc == WR_OK) rc = wr_film_create(sc, &f);
     if (rc == WR_OK && guided) rc = wr_guide_create(sc, &guide_config, &g);
     if (rc == WR_OK) {
@@ -311,6 +312,7 @@ static void render_on_gpu(const ThinlensCamera &camera, const SceneData &scene, 
     wr_film *f = nullptr;
     check(wr_scene_create(ctx->get(), &d, &sc));
     int rc = camera.has_medium ? wr_scene_set_medium(sc, &camera.medium) : WR_OK;
+    if (rc == WR_OK && !scene.env_rgb.empty()) rc = wr_scene_set_envmap(sc, scene.env_rgb.data(), scene.env_w, scene.env_h);
     if (rc == WR_OK) rc = wr_film_create(sc, &f);
     if (rc == WR_OK) rc = fn(sc, f);
     if (rc == WR_OK) rc = wr_film_read(f, camera.film->base.data(), camera.film->splat.data());
@@ -403,6 +405,28 @@ void VisualizeGeometryNormal::render() {
     });
 }
 
+void LoadPfm(const std::string &path, std::vector<float> *rgb, uint32_t *width, uint32_t *height) {
+    std::ifstream is(path, std::ios::binary);
+    if (!is.is_open()) throw std::runtime_error("cannot open envmap " + path);
+    std::string tag, scale;
+    long w = -1, h = -1;
+    is >> tag >> w >> h >> scale;
+    int c = is.get();
+    if (c == '\r') c = is.get();
+    if ((tag != "PF" && tag != "Pf") || w <= 0 || h <= 0 || !is) throw std::runtime_error("not a PFM image: " + path);
+    const int ch = tag == "PF" ? 3 : 1;
+    std::vector<float> row(size_t(ch) * size_t(w));
+    rgb->assign(3 * size_t(w) * size_t(h), 0.f);
+    for (long y = h; y > 0; y--) {  // rows bottom-up: the first row read is y = h - 1
+        is.read(reinterpret_cast<char *>(row.data()), std::streamsize(sizeof(float) * row.size()));
+        if (!is) throw std::runtime_error("truncated PFM image: " + path);
+        for (long x = 0; x < w; x++)
+            for (int k = 0; k < 3; k++) (*rgb)[3 * (size_t(y - 1) * size_t(w) + size_t(x)) + k] = row[size_t(x) * ch + (ch == 3 ? k : 0)];
+    }
+    *width = uint32_t(w);
+    *height = uint32_t(h);
+}
+
 // ------------------------------------------------------------------ scene JSON
 namespace {
 
@@ -445,7 +469,7 @@ void Simple2Parser::Parse(const std::string &json_path, const ParseOptions &opt)
 
 void Simple2Parser::Parse(const wj::Value &j, const std::string &root_dir, const ParseOptions &opt) {
     if (j.type != wj::Value::Object) throw std::runtime_error("scene json must be an object");
-    if (!j.string("envmap", "").empty()) throw std::runtime_error("envmap lights are not part of this build");
+
     if (j.has("render2d")) throw std::runtime_error("2-D rendering is not part of this build");
     auto scene = std::make_shared<SceneData>();
     if (const wj::Value *meshes = j.find("meshes")) {
@@ -455,6 +479,15 @@ void Simple2Parser::Parse(const wj::Value &j, const std::string &root_dir, const
             LoadObj(root_dir + "/" + m.string("path", ""), m.boolean("mitsuba_srgb", false), scene.get());
         }
     }
+    const std::string envmap = j.string("envmap", "");
+    if (!envmap.empty()) {
+        const size_t dot = envmap.find_last_of('.');
+        const std::string ext = dot == std::string::npos ? "" : envmap.substr(dot);
+        if (ext != ".pfm") throw std::runtime_error("envmap \"" + envmap + "\": only .pfm images are part of this build");
+        LoadPfm(root_dir + "/" + envmap, &scene->env_rgb, &scene->env_w, &scene->env_h);
+        if (scene->env_w != 2 * scene->env_h) throw std::runtime_error("envmap has wrong size");
+    }
+    if (scene->lights.empty() && scene->env_rgb.empty()) throw std::runtime_error("scene has no lights");
     if (!j.has("render")) return;
     auto camera = ThinlensCamera::FromSimple2ParseJson(j.at("camera"));
     if (const wj::Value *med = j.find("medium")) camera->has_medium = ParseMedium(*med, &camera->medium);

@@ -47,8 +47,16 @@ struct SceneData {
     std::vector<int32_t> geom_material, geom_light;
     std::vector<wr_material> materials;
     std::vector<wr_light> lights;
+    // envmap (scene JSON "envmap", simple2parser.h:110-119): RGB [env_h][env_w], row 0 = top
+    std::vector<float> env_rgb;
+    uint32_t env_w = 0, env_h = 0;
     wr_scene_desc desc(const wr_camera &camera) const;
 };
+
+// FimageIo::LoadPfm (src/common/floatimage.h:238-281): "PF" (RGB) or "Pf"
+// (grey, replicated), little-endian FP32 rows stored bottom-up.  Other image
+// formats of the reference's loader (.hdr, .exr, LDR) are not part of this build.
+void LoadPfm(const std::string &path, std::vector<float> *rgb, uint32_t *width, uint32_t *height);
 
 // OBJ + MTL reader with the reference's material mapping:
 //   Ke != 0 and Ns == 0 -> DiffuseAreaLight(Ke) (Ke with Ns != 0 is the reference's

@@ -41,6 +41,8 @@ class Scene:
     name: str = "scene"
     # homogeneous medium filling the scene: {"sigma_a": rgb, "sigma_s": rgb, "g": HG asymmetry} or None
     medium: dict | None = None
+    # envmap light (EnvmapLight over an equirectangular image): float32 (H, 2H, 3) or None
+    envmap: np.ndarray | None = None
 
     @property
     def n_tris(self) -> int:
@@ -54,6 +56,14 @@ class Scene:
         """The scene filled with a homogeneous medium (a number broadcasts to RGB)."""
         rgb = lambda v: tuple(float(x) for x in np.broadcast_to(np.asarray(v, np.float64), (3,)))
         return dataclasses.replace(self, medium={"sigma_a": rgb(sigma_a), "sigma_s": rgb(sigma_s), "g": float(g)})
+
+    def with_envmap(self, rgb) -> "Scene":
+        """The scene lit by an equirectangular envmap: an (H, 2H, 3) image, or an RGB
+        triple for a constant sky (a 2 x 1 image)."""
+        a = np.asarray(rgb, np.float32)
+        if a.ndim == 1:
+            a = np.broadcast_to(a, (1, 2, 3)).copy()
+        return dataclasses.replace(self, envmap=np.ascontiguousarray(a))
 
     def with_camera(self, **kw) -> "Scene":
         cam = self.camera.copy()
@@ -460,7 +470,18 @@ def write_obj(scene: Scene, directory: str, name: str = "scene") -> str:
         f.write("\n".join(mtl_lines) + "\n")
     with open(obj_path, "w") as f:
         f.write("\n".join(obj_lines) + "\n")
+    if scene.envmap is not None:
+        write_pfm(os.path.join(directory, f"{name}_env.pfm"), scene.envmap)
     return obj_path
+
+
+def write_pfm(path: str, rgb) -> None:
+    """A PFM image ("PF", little endian, rows bottom-up: floatimage.h:435-462) of
+    an (H, W, 3) array whose row 0 is the top."""
+    a = np.asarray(rgb, np.float32)
+    with open(path, "wb") as f:
+        f.write(f"PF\n{a.shape[1]} {a.shape[0]}\n-1\n".encode())
+        f.write(np.ascontiguousarray(a[::-1]).astype("<f4").tobytes())
 
 
 def scene_json(scene: Scene, obj_name: str, renders: list[dict]) -> dict:
@@ -475,6 +496,8 @@ def scene_json(scene: Scene, obj_name: str, renders: list[dict]) -> dict:
         if float(cam["aperture_radius"]) != float(cam["focal_distance"]):
             raise ValueError("the JSON key \"radius\" sets both the aperture and the focal distance")
         j["camera"]["radius"] = float(cam["aperture_radius"])
+    if scene.envmap is not None:  # written by write_obj next to the OBJ
+        j["envmap"] = obj_name[:-4] + "_env.pfm"
     if scene.medium:
         j["medium"] = {"type": "homogeneous", "sigma_a": list(scene.medium["sigma_a"]),
                        "sigma_s": list(scene.medium["sigma_s"]), "g": scene.medium["g"]}
