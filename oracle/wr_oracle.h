@@ -128,6 +128,15 @@ int orc_render_light(const orc_scene *s, const orc_render_params *p,
  * Henyey-Greenstein phase function of asymmetry g (0 = isotropic,
  * src/phasefunction/isotropic.h).  All-zero coefficients remove it. */
 int orc_scene_set_medium(orc_scene *s, const double sigma_a[3], const double sigma_s[3], double g);
+/* Envmap light (EnvmapLight over EquiRectRep, light/envmap.h,
+ * sphericalrepresentation/equirectangular.h): rgb[h][w][3] FP32, w = 2 h; a null
+ * rgb removes it.  The last entry of the light CdfTable, weight 1.23. */
+int orc_scene_set_envmap(orc_scene *s, const float *rgb, int w, int h);
+/* EquiRectRep::eval (bilinear) and ::pdfA at direction d. */
+void orc_env_eval(const orc_scene *s, const double d[3], double rgb[3], double *pdf);
+/* EquiRectRep::sample with the reference's use of u (the in-texel u is the
+ * remapped texel-selection number): direction, solid-angle pdf, eval / pdf. */
+void orc_env_sample_ref(const orc_scene *s, const double u[2], double dir[3], double *pdf, double value[3]);
 double orc_hg_pdf(double g, double cos_theta);
 void orc_hg_sample(double g, const double fwd[3], const double u[2], double out[3]);
 
