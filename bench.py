@@ -49,6 +49,8 @@ def parse():
     ap.add_argument("--ttr-runs", type=int, default=3)
     ap.add_argument("--ttr-train-spp", type=int, default=128)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-scene measurement")
+    ap.add_argument("--no-integrators", action="store_true",
+                    help="skip primitive / light tracing / visualize / envmap on the headline frame")
     ap.add_argument("--batch-scenes", type=int, default=32)
     ap.add_argument("--no-medium", action="store_true", help="skip the medium BF16/FP8 inference measurement")
     ap.add_argument("--med-width", type=int, default=1280)
@@ -322,6 +324,39 @@ def cpu_baseline(sc, N, target_s):
             "samples_per_s": round(total_paths / total_s, 1)}
 
 
+def other_integrators(wr, scenes, dev, gs, sc, N, a):
+    """The reference's other integrators on the same kernels, on the headline
+    scene and frame: primitive_path_tracer and light_splatting at the headline
+    depth (light_splatting traces spp * W * H light subpaths), visualize one
+    primary hit per sample, and NEE+MIS with a 1024 x 512 envmap sky added.
+    Mrays/s = (closest-hit + shadow rays) / wall second of one blocking call."""
+    import numpy as np
+    out = {}
+    spp = max(1, a.spp // 4)
+    sky = np.ones((512, 1024, 3), np.float32) * np.float32([0.3, 0.4, 0.6])
+    sky[40:48, 300:308] = 200.0  # a sun
+    for name, fn, scene in (("primitive_path_tracer", lambda g, f: wr.render_primitive(g, f, spp, num_path_vertices=N, seed=3), gs),
+                            ("light_splatting", lambda g, f: wr.render_light(g, f, spp, num_path_vertices=N, seed=3), gs),
+                            ("visualize", lambda g, f: wr.render_visualize(g, f, spp, seed=3), gs),
+                            ("nee_envmap_sky", lambda g, f: wr.render_nee(g, f, spp, num_path_vertices=N, seed=3), None)):
+        g = scene
+        if g is None:
+            g = wr.GpuScene(dev, sc.with_envmap(sky))
+        f = wr.Film(g)
+        fn(g, f)  # warm-up (pool allocation, first launches)
+        f.clear()
+        st = fn(g, f)
+        rays = st.extend_rays + st.shadow_rays
+        out[name] = {"spp": spp, "paths": st.paths, "rays": rays, "ms": round(st.wall_ms, 2),
+                     "mrays_per_s": round(rays / st.wall_ms / 1e3, 1)}
+        print(f"# {name}: {spp} spp in {st.wall_ms:.1f} ms, {rays / st.wall_ms / 1e3:.0f} Mrays/s",
+              file=sys.stderr, flush=True)
+        f.close()
+        if g is not gs:
+            g.close()
+    return out
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -399,6 +434,9 @@ def main():
                 "avg_launch_ms": round(avg_ms, 4), "launches": ext_launches,
                 "nodes_per_ray": round(nodes_per_ray, 2), "tris_per_ray": round(tris_per_ray, 2),
                 "bytes_per_ray": round(EXT_RAY_BYTES + nodes_per_ray * NODE_BYTES + tris_per_ray * TRI_BYTES, 1)}
+    others = None
+    if not a.no_integrators:
+        others = other_integrators(wr, scenes, dev, gs, sc, N, a)
     bat = None
     if not a.no_batch:
         bat = batched(wr, scenes, dev, a)
@@ -438,6 +476,7 @@ def main():
         "batched": bat,
         "medium_bf16_fp8": med,
         "bvh_build": bvh,
+        "other_integrators": others,
     }
     print(json.dumps(out))
 
