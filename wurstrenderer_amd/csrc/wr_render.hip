@@ -350,7 +350,7 @@ struct TravTune {
 // which admits 7 waves per SIMD instead of 6 (3% faster extension on the
 // 1.31M-triangle frame); the counting and batched kernels would spill at that
 // size and keep the compiler's choice.  The plain any-hit kernel (k_shadow)
-// gets 72 VGPRs and 7 waves by itself: its stack holds node refs only.
+// gets 70 VGPRs and 7 waves by itself: its stack holds node refs only.
 #ifndef WR_EXTEND_WAVES
 #define WR_EXTEND_WAVES 7
 #endif
@@ -418,8 +418,9 @@ __device__ __forceinline__ void extend_body(DScene s1, const BatchJob *__restric
             if (exhausted) break;
             continue;
         }
-        if (active && (BATCH ? trav_step<false, COUNT, kBlock>(geo, ts, st, cnt)
-                             : trav_step<false, COUNT, kBlock>(s1, ts, st, cnt))) {
+        const bool fin = BATCH ? trav_round<COUNT, kBlock>(geo, ts, st, cnt, active)
+                               : trav_round<COUNT, kBlock>(s1, ts, st, cnt, active);
+        if (active && fin) {
             pb.hit[p] = make_uint2(ts.hit, __float_as_uint(ts.t));
             active = false;
         }
@@ -1087,8 +1088,9 @@ __global__ __launch_bounds__(kBlock) WR_TRAV_ATTR void k_shadow(DScene s1, const
             if (exhausted) break;
             continue;
         }
-        if (active && (BATCH ? trav_step<true, COUNT, kBlock>(geo, ts, st, cnt)
-                             : trav_step<true, COUNT, kBlock>(s1, ts, st, cnt))) {
+        const bool fin = BATCH ? trav_round_any<COUNT, kBlock>(geo, ts, st, cnt, active)
+                               : trav_round_any<COUNT, kBlock>(s1, ts, st, cnt, active);
+        if (active && fin) {
             active = false;
             if (ts.hit == kInvalid) {
                 const float4 c = pb.sh_c[i];

@@ -244,6 +244,183 @@ __device__ __forceinline__ bool trav_step(const G &s, TravState &ts, LaneStack<k
     }
 }
 
+// Closest hit, one speculative while-while round (Aila and Laine, "Understanding
+// the Efficiency of Ray Traversal on GPUs", HPG 2009): each lane descends inner
+// nodes and postpones the first leaf it meets, continuing speculatively with
+// the next node, until every lane of the wave holds a leaf or has finished (or
+// kSpecSteps steps have run); then the wave tests its postponed leaves
+// together, so the box-test and triangle-test code each run with dense lanes
+// (one node per step with both branches left a third of the lanes active).
+// Speculation visits ~3% more nodes; short rounds keep that small: 4 steps
+// were fastest of 2..32 (extension 12-18% faster on the 1.31M-triangle and
+// gap-room frames, identical hits).  The push/pop order is the one-node-per-
+// step order, so the builders' stack bound holds.  All lanes of the wave must
+// call it (it takes a ballot); `live` marks lanes with a ray.  Returns true
+// when the lane's ray is finished.
+constexpr int kSpecSteps = 4;
+template <bool COUNT, int kBlock, class G>
+__device__ __forceinline__ bool trav_round(const G &s, TravState &ts, LaneStack<kBlock> &st, TraversalCounters &cnt,
+                                           bool live) {
+    const RaySetup &r = ts.r;
+    uint32_t leaf = kInvalid;  // postponed leaf
+    bool more = live;          // ts.cur is a node still to process
+    for (int it = 0; it < kSpecSteps; it++) {
+        // the wave stops once every lane holds a leaf or has no node left
+        if (__ballot(more && leaf == kInvalid) == 0) break;
+        const uint32_t cur = ts.cur;
+        if (!more || ((cur & kLeafBit) && leaf != kInvalid)) continue;  // waits with its leaf
+        bool pop = false;
+        if (cur & kLeafBit) {
+            leaf = cur;
+            pop = true;
+        } else {
+            const float4 *np = reinterpret_cast<const float4 *>(s.nodes + cur);
+            const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
+            const uint4 ch = reinterpret_cast<const uint4 *>(np)[6];
+            if (COUNT) cnt.nodes++;
+            const float ox = r.o.x, oy = r.o.y, oz = r.o.z;
+            const float4 nx_ = r.nx ? hx : lx, fx_ = r.nx ? lx : hx;
+            const float4 ny_ = r.ny ? hy : ly, fy_ = r.ny ? ly : hy;
+            const float4 nz_ = r.nz ? hz : lz, fz_ = r.nz ? lz : hz;
+            float tn[4], tf[4];
+            tn[0] = fmaxf(fmaxf((nx_.x - ox) * r.ix, (ny_.x - oy) * r.iy), fmaxf((nz_.x - oz) * r.iz, r.tmin));
+            tn[1] = fmaxf(fmaxf((nx_.y - ox) * r.ix, (ny_.y - oy) * r.iy), fmaxf((nz_.y - oz) * r.iz, r.tmin));
+            tn[2] = fmaxf(fmaxf((nx_.z - ox) * r.ix, (ny_.z - oy) * r.iy), fmaxf((nz_.z - oz) * r.iz, r.tmin));
+            tn[3] = fmaxf(fmaxf((nx_.w - ox) * r.ix, (ny_.w - oy) * r.iy), fmaxf((nz_.w - oz) * r.iz, r.tmin));
+            tf[0] = fminf(fminf((fx_.x - ox) * r.ix, (fy_.x - oy) * r.iy), (fz_.x - oz) * r.iz);
+            tf[1] = fminf(fminf((fx_.y - ox) * r.ix, (fy_.y - oy) * r.iy), (fz_.y - oz) * r.iz);
+            tf[2] = fminf(fminf((fx_.z - ox) * r.ix, (fy_.z - oy) * r.iy), (fz_.z - oz) * r.iz);
+            tf[3] = fminf(fminf((fx_.w - ox) * r.ix, (fy_.w - oy) * r.iy), (fz_.w - oz) * r.iz);
+            uint32_t c[4] = {ch.x, ch.y, ch.z, ch.w};
+            float key[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const bool h = c[i] != kEmptyChild && tn[i] <= fminf(tf[i] * 1.00000048f, ts.t);
+                key[i] = h ? tn[i] : INFINITY;
+            }
+#define WR_CSWAP(a, b)                                                          \
+    if (key[b] < key[a]) {                                                      \
+        float tk = key[a]; key[a] = key[b]; key[b] = tk;                        \
+        uint32_t tc = c[a]; c[a] = c[b]; c[b] = tc;                             \
+    }
+            WR_CSWAP(0, 1) WR_CSWAP(2, 3) WR_CSWAP(0, 2) WR_CSWAP(1, 3) WR_CSWAP(1, 2)
+#undef WR_CSWAP
+            uint32_t next = kInvalid;
+            float next_t = 0.f;
+#pragma unroll
+            for (int i = 3; i >= 0; i--) {
+                if (key[i] != INFINITY) {
+                    if (next != kInvalid) st.push(next, next_t);
+                    next = c[i];
+                    next_t = key[i];
+                }
+            }
+            if (next != kInvalid) ts.cur = next;
+            else pop = true;
+        }
+        if (pop) {
+            more = false;
+            while (st.sp > 0) {
+                st.sp--;
+                uint32_t ref;
+                float tn;
+                st.top(ref, tn);
+                if (tn <= ts.t) {
+                    ts.cur = ref;
+                    more = true;
+                    break;
+                }
+            }
+        }
+    }
+    if (leaf != kInvalid) {
+        const uint32_t first = leaf & 0x0FFFFFFFu;
+        const uint32_t count = ((leaf >> 28) & 0x7u) + 1u;
+        if (COUNT) cnt.tris += count;
+        for (uint32_t k = 0; k < count; k++) {
+            const float4 *tp = s.tris + 3 * size_t(first + k);
+            float t;
+            if (tri_test(r, tp[0], tp[1], tp[2], ts.t, t)) {
+                ts.t = t;
+                ts.hit = first + k;
+            }
+        }
+    }
+    return live && !more;
+}
+
+// Any hit, the same round (10% faster shadow rays on the 1.31M-triangle frame):
+// children in slot order (no sort), a stack of node refs only, and the
+// postponed leaf ends the ray at its first hit.
+template <bool COUNT, int kBlock, class G>
+__device__ __forceinline__ bool trav_round_any(const G &s, TravState &ts, LaneStack<kBlock> &st, TraversalCounters &cnt,
+                                               bool live) {
+    const RaySetup &r = ts.r;
+    uint32_t leaf = kInvalid;
+    bool more = live;
+    for (int it = 0; it < kSpecSteps; it++) {
+        if (__ballot(more && leaf == kInvalid) == 0) break;
+        const uint32_t cur = ts.cur;
+        if (!more || ((cur & kLeafBit) && leaf != kInvalid)) continue;
+        bool pop = false;
+        if (cur & kLeafBit) {
+            leaf = cur;
+            pop = true;
+        } else {
+            const float4 *np = reinterpret_cast<const float4 *>(s.nodes + cur);
+            const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
+            const uint4 ch = reinterpret_cast<const uint4 *>(np)[6];
+            if (COUNT) cnt.nodes++;
+            const float ox = r.o.x, oy = r.o.y, oz = r.o.z;
+            const float4 nx_ = r.nx ? hx : lx, fx_ = r.nx ? lx : hx;
+            const float4 ny_ = r.ny ? hy : ly, fy_ = r.ny ? ly : hy;
+            const float4 nz_ = r.nz ? hz : lz, fz_ = r.nz ? lz : hz;
+            float tn[4], tf[4];
+            tn[0] = fmaxf(fmaxf((nx_.x - ox) * r.ix, (ny_.x - oy) * r.iy), fmaxf((nz_.x - oz) * r.iz, r.tmin));
+            tn[1] = fmaxf(fmaxf((nx_.y - ox) * r.ix, (ny_.y - oy) * r.iy), fmaxf((nz_.y - oz) * r.iz, r.tmin));
+            tn[2] = fmaxf(fmaxf((nx_.z - ox) * r.ix, (ny_.z - oy) * r.iy), fmaxf((nz_.z - oz) * r.iz, r.tmin));
+            tn[3] = fmaxf(fmaxf((nx_.w - ox) * r.ix, (ny_.w - oy) * r.iy), fmaxf((nz_.w - oz) * r.iz, r.tmin));
+            tf[0] = fminf(fminf((fx_.x - ox) * r.ix, (fy_.x - oy) * r.iy), (fz_.x - oz) * r.iz);
+            tf[1] = fminf(fminf((fx_.y - ox) * r.ix, (fy_.y - oy) * r.iy), (fz_.y - oz) * r.iz);
+            tf[2] = fminf(fminf((fx_.z - ox) * r.ix, (fy_.z - oy) * r.iy), (fz_.z - oz) * r.iz);
+            tf[3] = fminf(fminf((fx_.w - ox) * r.ix, (fy_.w - oy) * r.iy), (fz_.w - oz) * r.iz);
+            const uint32_t c[4] = {ch.x, ch.y, ch.z, ch.w};
+            uint32_t next = kInvalid;
+#pragma unroll
+            for (int i = 3; i >= 0; i--) {
+                if (c[i] != kEmptyChild && tn[i] <= fminf(tf[i] * 1.00000048f, ts.t)) {
+                    if (next != kInvalid) st.push_ref(next);
+                    next = c[i];
+                }
+            }
+            if (next != kInvalid) ts.cur = next;
+            else pop = true;
+        }
+        if (pop) {
+            more = st.sp > 0;
+            if (more) {
+                st.sp--;
+                ts.cur = st.top_ref();
+            }
+        }
+    }
+    if (leaf != kInvalid) {
+        const uint32_t first = leaf & 0x0FFFFFFFu;
+        const uint32_t count = ((leaf >> 28) & 0x7u) + 1u;
+        if (COUNT) cnt.tris += count;
+        for (uint32_t k = 0; k < count; k++) {
+            const float4 *tp = s.tris + 3 * size_t(first + k);
+            float t;
+            if (tri_test(r, tp[0], tp[1], tp[2], ts.t, t)) {
+                ts.t = t;
+                ts.hit = first + k;
+                return live;  // occluded: finished
+            }
+        }
+    }
+    return live && !more;
+}
+
 // Whole traversal of one ray (used where rays are not refilled per lane).
 template <bool ANY, bool COUNT, int kBlock, class G>
 __device__ __forceinline__ uint32_t traverse(const G &s, const RaySetup &r, float &t_best, LaneStack<kBlock> &st,
