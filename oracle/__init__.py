@@ -45,6 +45,14 @@ class OrcStats(C.Structure):
                 ("seconds", C.c_double)]
 
 
+class OrcBdptRecord(C.Structure):
+    _fields_ = [("s", C.c_int32), ("t", C.c_int32), ("n", C.c_int32), ("pad", C.c_int32),
+                ("weight", C.c_double), ("contrib", C.c_double * 3),
+                ("pos", (C.c_double * 3) * 64), ("nrm", (C.c_double * 3) * 64),
+                ("kind", C.c_int32 * 64), ("id", C.c_int32 * 64),
+                ("lp", C.c_double * 64), ("cp", C.c_double * 64)]
+
+
 def build() -> str:
     """Compile liboracle.so (gcc); the reference harness only where the reference exists."""
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
@@ -68,6 +76,9 @@ def lib():
         L.orc_render_primitive.argtypes = [P, C.POINTER(OrcRenderParams), P, P, C.POINTER(OrcStats)]
         L.orc_render_visualize.argtypes = [P, C.POINTER(OrcRenderParams), P, P, C.POINTER(OrcStats)]
         L.orc_render_light.argtypes = [P, C.POINTER(OrcRenderParams), P, P, C.POINTER(OrcStats)]
+        L.orc_render_bdpt.argtypes = [P, C.POINTER(OrcRenderParams), P, P, C.POINTER(OrcStats)]
+        L.orc_bdpt_records.argtypes = [P, C.c_int, C.c_uint32, C.c_int, C.c_int, C.c_uint32,
+                                       C.POINTER(OrcBdptRecord), C.c_int, C.POINTER(C.c_int)]
         L.orc_trace_closest.argtypes = [P, P, C.c_uint32, P, P, P]
         L.orc_trace_any.argtypes = [P, P, C.c_uint32, P]
         L.orc_trace_closest_brute.argtypes = [P, P, C.c_uint32, P, P, P, P, P]
@@ -154,9 +165,10 @@ class OracleScene:
                integrator="nextevent_path_tracer"):
         """Returns (base, splat, stats); image = base/spp + splat/(spp*W*H).  `integrator`
         is the reference's JSON name: nextevent_path_tracer, primitive_path_tracer,
-        light_splatting or visualize."""
+        light_splatting, visualize or bidirectional_path_tracer."""
         fn = {"nextevent_path_tracer": lib().orc_render_nee, "primitive_path_tracer": lib().orc_render_primitive,
-              "light_splatting": lib().orc_render_light, "visualize": lib().orc_render_visualize}[integrator]
+              "light_splatting": lib().orc_render_light, "visualize": lib().orc_render_visualize,
+              "bidirectional_path_tracer": lib().orc_render_bdpt}[integrator]
         base = np.zeros((self.H, self.W, 3), np.float64)
         splat = np.zeros((self.H, self.W, 3), np.float64)
         p = OrcRenderParams(spp, num_path_vertices, seed, int(bool(use_mis)), first_sample, threads, 0, 0, 0, 0)
@@ -166,6 +178,22 @@ class OracleScene:
         if fn(self.h, C.byref(p), base.ctypes.data, splat.ctypes.data, C.byref(st)) != 0:
             raise RuntimeError("oracle: " + lib().orc_last_error().decode())
         return base, splat, st
+
+    def bdpt_records(self, num_path_vertices, seed, px, py, sample, max_records=4096):
+        """The strategies of one bidirectional camera sample (orc_bdpt_records) as dicts
+        with s, t, n, weight, contrib, pos (n, 3), nrm (n, 3), kind, id, lp, cp."""
+        buf = (OrcBdptRecord * max_records)()
+        n = C.c_int()
+        if lib().orc_bdpt_records(self.h, num_path_vertices, seed, px, py, sample, buf, max_records, C.byref(n)) != 0:
+            raise RuntimeError("oracle: " + lib().orc_last_error().decode())
+        out = []
+        for r in buf[:n.value]:
+            k = r.n
+            out.append(dict(s=r.s, t=r.t, n=k, weight=r.weight, contrib=np.array(r.contrib[:]),
+                            pos=np.array([r.pos[i][:] for i in range(k)]), nrm=np.array([r.nrm[i][:] for i in range(k)]),
+                            kind=np.array(r.kind[:k]), id=np.array(r.id[:k]),
+                            lp=np.array(r.lp[:k]), cp=np.array(r.cp[:k])))
+        return out
 
     def image(self, spp, **kw):
         base, splat, st = self.render(spp, **kw)

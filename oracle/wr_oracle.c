@@ -1381,11 +1381,268 @@ static void light_subpath(const orc_scene *s, int N, uint32_t key, double *splat
     }
 }
 
+/* ------------------------------------------------------------------ bidirectional path tracing
+ * BidirPathRenderer (bidirpath.h:44-421) over GeneralSubpathSampler with
+ * PdfA requests and no volumetric sampling (generalsubpathsampler.h:13-160,
+ * 185-322; the renderer's sampler is built non-volumetric, so a medium
+ * changes nothing).  Vertices are 1-based as in the reference: L[1] is the
+ * light point, C[1] the camera vertex.  pf is the area pdf with which a
+ * vertex's own subpath made it (the reference's mPdfFromLight on the light
+ * side, mPdfFromCamera on the camera side), pr the area pdf with which the
+ * other subpath would have made it. */
+#define BDPT_MAXV 64
+enum { BV_CAMERA = 0, BV_LIGHT = 1, BV_SURFACE = 2 };
+typedef struct {
+    v3 p, n;     /* position, geometric normal (the camera's: its viewing axis) */
+    frame3 f;
+    v3 wi;       /* surfaces: unit direction to the previous vertex of its own subpath */
+    rgb T;       /* mPathContrib */
+    double pf, pr;
+    int kind, id; /* id: material (surface) or light index (light) */
+} bvert;
+/* the light subpath's walk draws from its own stream */
+static inline uint32_t bdpt_light_key(uint32_t key) { return mix32(key ^ 0x3c6ef372u); }
+/* Vertex::ConvertToAreaFromSolidAngle (path.h:47-60) for surface, light and camera vertices */
+static double to_area(v3 from, v3 to, v3 to_n) {
+    v3 uv = vsub(to, from);
+    double l2 = vlen2(uv);
+    return fabs(vdot(to_n, vmul(uv, 1.0 / sqrt(l2)))) / l2;
+}
+/* Vertex::pdfA(from, to) of vertex x (path.h:101, 197-205, 307-315, 381-390):
+ * BSDF pdf, DiffuseAreaLight::evalPdfLe1 (diffusearealight.h:34-38) or the
+ * camera's per-pixel pdf (generalsubpathsampler.h:324-327); `from` is unused
+ * (the imaginary vertex) for lights and the camera */
+static double bv_pdfA(const orc_scene *s, const bvert *x, const bvert *from, const bvert *to) {
+    v3 w = vnorm(vsub(to->p, x->p));
+    double pw;
+    if (x->kind == BV_SURFACE) {
+        pw = bsdf_pdf(&s->mats[x->id], to_local(&x->f, vnorm(vsub(from->p, x->p))), to_local(&x->f, w));
+    } else if (x->kind == BV_LIGHT) {
+        double c = to_local(&x->f, w).y;
+        pw = c > 0.0 ? c * INV_PI : 0.0;
+    } else {
+        double c = to_local(&s->cam_frame, w).z;
+        pw = s->per_pixel_norm / (c * c * c);
+    }
+    return pw * to_area(x->p, to->p, to->n);
+}
+/* computeMisWeight with power 1 and the ratio form (bidirpath.h:125-194):
+ * connected vertex i <= s is L[i], i > s is C[s + t - i + 1]; lp[i] / cp[i]
+ * are the area pdfs with which the light / camera side makes vertex i, those
+ * at the two vertices each side of the connection re-evaluated for it.
+ * Returns 0 (weight 0) unless vertex 1 is on a light. */
+static int bdpt_pdf_arrays(const orc_scene *s, const bvert *L, const bvert *C, int sN, int tN, double *lp,
+                           double *cp) {
+    const int n = sN + tN;
+    for (int i = 1; i <= n; i++) {
+        const bvert *x = i <= sN ? &L[i] : &C[n - i + 1];
+        lp[i] = i <= sN ? x->pf : x->pr;
+        cp[i] = i <= sN ? x->pr : x->pf;
+    }
+    if ((sN >= 1 ? L[1].kind : C[n].kind) != BV_LIGHT) return 0;
+    const bvert *c = &C[tN], *cq = tN >= 2 ? &C[tN - 1] : NULL;
+    const bvert *l = sN >= 1 ? &L[sN] : NULL, *lq = sN >= 2 ? &L[sN - 1] : NULL;
+    if (sN >= 1) cp[sN] = bv_pdfA(s, c, cq, l);
+    if (sN >= 2) cp[sN - 1] = bv_pdfA(s, l, c, lq);
+    if (sN >= 1 && tN >= 2) lp[sN + 1] = bv_pdfA(s, l, lq, c);
+    if (sN >= 1 && tN >= 2) lp[sN + 2] = bv_pdfA(s, c, l, cq);
+    return 1;
+}
+static double bdpt_mis(const orc_scene *s, const bvert *L, const bvert *C, int sN, int tN) {
+    const int n = sN + tN;
+    double lp[2 * BDPT_MAXV + 3], cp[2 * BDPT_MAXV + 3];
+    if (!bdpt_pdf_arrays(s, L, C, sN, tN, lp, cp)) return 0.0;
+    double inv = 1.0, r = 1.0;
+    for (int i = sN; i > 0; i--) {
+        if (lp[i] == 0.0) return 0.0;
+        r *= cp[i] / lp[i];
+        inv += r;
+    }
+    r = 1.0;
+    for (int i = sN + 1; i < n; i++) {
+        if (cp[i] == 0.0) return 0.0;
+        r *= lp[i] / cp[i];
+        inv += r;
+    }
+    return 1.0 / inv;
+}
+/* bdptConnect (bidirpath.h:79-123): s = 0 is the camera subpath ending on an
+ * emitter, t = 1 a connection to the camera vertex (raster from We1), else a
+ * connection of two surface vertices; visibility from the light side */
+static rgb bdpt_connect(const orc_scene *s, const bvert *L, const bvert *C, int sN, int tN, double raster[2],
+                        counters *cn) {
+    if (sN == 0) {
+        const bvert *c = &C[tN], *cq = &C[tN - 1];
+        if (c->kind != BV_LIGHT) return BLACK;
+        if (to_local(&c->f, vnorm(vsub(cq->p, c->p))).y < 0.0) return BLACK;
+        return cscale(cmul(c->T, s->lights[c->id].L), INV_PI * PI); /* Le1 * Le0 */
+    }
+    const bvert *l = &L[sN], *c = &C[tN];
+    if (tN == 1 ? c->kind != BV_CAMERA : c->kind != BV_SURFACE) return BLACK;
+    v3 uv = vsub(c->p, l->p);
+    double len2 = vlen2(uv), len = sqrt(len2);
+    v3 nuv = vmul(uv, 1.0 / len);
+    double G = fabs(vdot(l->n, nuv)) * fabs(vdot(c->n, nuv)) / len2;
+    if (G == 0.0) return BLACK;
+    rgb cs;
+    if (tN == 1) {
+        double we1 = camera_we1(s, c->p, vneg(nuv), raster);
+        if (we1 == 0.0) return BLACK;
+        cs = RGB(we1, we1, we1);
+    } else {
+        cs = bsdf_eval(&s->mats[c->id], to_local(&c->f, c->wi), to_local(&c->f, vneg(nuv)));
+        if (czero(cs)) return BLACK;
+    }
+    rgb ls;
+    if (sN == 1) {
+        if (to_local(&l->f, nuv).y < 0.0) return BLACK;
+        ls = RGB(INV_PI, INV_PI, INV_PI);
+    } else {
+        ls = bsdf_eval(&s->mats[l->id], to_local(&l->f, l->wi), to_local(&l->f, nuv));
+        if (czero(ls)) return BLACK;
+    }
+    if (occluded(s, l->p, nuv, RAY_EPS, len - RAY_EPS, cn)) return BLACK;
+    return cmul(cscale(cmul(l->T, ls), G), cmul(cs, c->T));
+}
+/* one camera sample (bidirpath.h:364-395): light subpath, camera subpath, all
+ * strategies with s + t <= N; t = 1 goes to the splat film, the rest to base.
+ * The light subpath stops at N - 1 vertices: the reference traces an N-th one
+ * that no strategy uses (s <= N - 1). */
+static void bdpt_sample(const orc_scene *s, int N, uint32_t key, int px, int py, double base[3], double *splat,
+                        counters *cn, orc_bdpt_record *rec, int max_rec, int *n_rec) {
+    bvert L[BDPT_MAXV + 1], C[BDPT_MAXV + 1];
+    base[0] = base[1] = base[2] = 0.0;
+    /* light subpath (traceLightSubpath, bidirpath.h:196-219) */
+    lvert lv;
+    sample_light(s, u01(key, DIM_LIGHT), u01(key, DIM_LIGHT + 1), key, &lv);
+    memset(&L[1], 0, sizeof L[1]);
+    L[1].p = lv.pos; L[1].n = lv.n; L[1].f = lv.f; L[1].T = lv.contrib; L[1].pf = lv.pdfA;
+    L[1].kind = BV_LIGHT; L[1].id = lv.light;
+    int nL = 1;
+    const uint32_t kl = bdpt_light_key(key);
+    if (N - 1 >= 2) {
+        v3 ro = lv.pos, rd = to_world(&lv.f, cosine_hemisphere(u01(kl, DIM_PIXEL), u01(kl, DIM_PIXEL + 1)));
+        double pdfW = fmax(to_local(&lv.f, rd).y, 0.0) * INV_PI; /* sampleLe1CosByPdf, contribution 1 */
+        rgb w = RGB(1, 1, 1);
+        for (int bounce = 0;; bounce++) {
+            double t = INFINITY;
+            uint32_t tri = closest(s, ro, rd, &t, cn);
+            if (tri == UINT32_MAX) break;
+            uint32_t g = s->tri_geom[tri];
+            if (s->geom_light[g] >= 0) break; /* the callback keeps surfaces only; a light ends the walk */
+            bvert *x = &L[nL + 1], *prev = &L[nL];
+            v3 p0 = tri_v(s, tri, 0), p1 = tri_v(s, tri, 1), p2 = tri_v(s, tri, 2);
+            x->p = vadd(ro, vmul(rd, t));
+            x->n = vnorm(vcross(vsub(p1, p0), vsub(p2, p0)));
+            x->f = make_frame(x->n);
+            x->wi = vneg(rd);
+            x->T = cmul(prev->T, w);
+            x->pf = pdfW * to_area(prev->p, x->p, x->n);
+            x->pr = 0.0;
+            x->kind = BV_SURFACE; x->id = (int)s->geom_material[g];
+            nL++;
+            if (nL == N - 1) break;
+            const bsdf_t *b = &s->mats[x->id];
+            uint32_t dim = DIM_BOUNCE0 + DIMS_PER_BOUNCE * (uint32_t)bounce + BD_BSDF;
+            v3 wi = to_local(&x->f, x->wi), wo;
+            double pdf = 0.0;
+            w = bsdf_sample(b, wi, u01(kl, dim), u01(kl, dim + 1), &wo, &pdf);
+            if (czero(w)) break;
+            /* the previous vertex's pdf from the camera side, given this direction (bidirpath.h:207-212) */
+            prev->pr = bsdf_pdf(b, wo, wi) * to_area(x->p, prev->p, prev->n);
+            pdfW = pdf;
+            ro = x->p;
+            rd = to_world(&x->f, wo);
+        }
+    }
+    /* camera subpath (traceCameraSubpath, bidirpath.h:221-257) */
+    v3 lens_local;
+    memset(&C[1], 0, sizeof C[1]);
+    C[1].p = camera_position(s, u01(key, DIM_LENS), u01(key, DIM_LENS + 1), &lens_local);
+    C[1].n = s->cam_frame.z;
+    C[1].T = RGB(1, 1, 1);
+    C[1].kind = BV_CAMERA;
+    int nC = 1;
+    {
+        double raster0[2], pdfW;
+        v3 ro = C[1].p;
+        v3 rd = camera_direction(s, lens_local, px, py, u01(key, DIM_PIXEL), u01(key, DIM_PIXEL + 1), raster0, &pdfW);
+        rgb w = RGB(1, 1, 1);
+        for (int bounce = 0; nC < N; bounce++) {
+            double t = INFINITY;
+            uint32_t tri = closest(s, ro, rd, &t, cn);
+            if (tri == UINT32_MAX) break;
+            uint32_t g = s->tri_geom[tri];
+            bvert *x = &C[nC + 1], *prev = &C[nC];
+            v3 p0 = tri_v(s, tri, 0), p1 = tri_v(s, tri, 1), p2 = tri_v(s, tri, 2);
+            x->p = vadd(ro, vmul(rd, t));
+            x->n = vnorm(vcross(vsub(p1, p0), vsub(p2, p0)));
+            x->f = make_frame(x->n);
+            x->wi = vneg(rd);
+            x->T = cmul(prev->T, w);
+            x->pf = pdfW * to_area(prev->p, x->p, x->n);
+            nC++;
+            if (s->geom_light[g] >= 0) {
+                /* emitter: its light-side pdfs (bidirpath.h:240-249) end the walk */
+                const int li = s->geom_light[g];
+                x->kind = BV_LIGHT; x->id = li;
+                x->pr = s->light_cdf.pdf[li] * s->lights[li].inv_area;
+                prev->pr = bv_pdfA(s, x, NULL, prev);
+                break;
+            }
+            x->kind = BV_SURFACE; x->id = (int)s->geom_material[g];
+            x->pr = 0.0;
+            if (nC == N) break;
+            const bsdf_t *b = &s->mats[x->id];
+            uint32_t dim = DIM_BOUNCE0 + DIMS_PER_BOUNCE * (uint32_t)bounce + BD_BSDF;
+            v3 wi = to_local(&x->f, x->wi), wo;
+            double pdf = 0.0;
+            w = bsdf_sample(b, wi, u01(key, dim), u01(key, dim + 1), &wo, &pdf);
+            if (czero(w)) break;
+            prev->pr = bsdf_pdf(b, wo, wi) * to_area(x->p, prev->p, prev->n); /* bidirpath.h:234-239 */
+            pdfW = pdf;
+            ro = x->p;
+            rd = to_world(&x->f, wo);
+        }
+    }
+    /* every strategy (evalContribution, bidirpath.h:259-301) */
+    for (int sN = 0; sN <= nL; sN++)
+        for (int tN = 1; tN <= nC; tN++) {
+            if (sN + tN <= 1 || sN + tN > N) continue;
+            double raster[2];
+            rgb c = bdpt_connect(s, L, C, sN, tN, raster, cn);
+            if (czero(c)) continue;
+            const double w = bdpt_mis(s, L, C, sN, tN);
+            if (rec && *n_rec < max_rec) {
+                orc_bdpt_record *r = &rec[(*n_rec)++];
+                memset(r, 0, sizeof *r);
+                r->s = sN; r->t = tN; r->n = sN + tN; r->weight = w;
+                r->contrib[0] = c.r; r->contrib[1] = c.g; r->contrib[2] = c.b;
+                double lp[2 * BDPT_MAXV + 3], cp[2 * BDPT_MAXV + 3];
+                bdpt_pdf_arrays(s, L, C, sN, tN, lp, cp);
+                for (int i = 1; i <= r->n; i++) {
+                    const bvert *x = i <= sN ? &L[i] : &C[r->n - i + 1];
+                    r->pos[i - 1][0] = x->p.x; r->pos[i - 1][1] = x->p.y; r->pos[i - 1][2] = x->p.z;
+                    r->nrm[i - 1][0] = x->n.x; r->nrm[i - 1][1] = x->n.y; r->nrm[i - 1][2] = x->n.z;
+                    r->kind[i - 1] = x->kind; r->id[i - 1] = x->id;
+                    r->lp[i - 1] = lp[i]; r->cp[i - 1] = cp[i];
+                }
+            }
+            c = cscale(c, w);
+            if (tN == 1) {
+                if (splat) splat_add(s, splat, raster, c);
+            } else {
+                base[0] += c.r; base[1] += c.g; base[2] += c.b;
+            }
+        }
+}
+
 /* ------------------------------------------------------------------ render driver (pathnextevent.h:152-191) */
 typedef struct {
     const orc_scene *s;
     const orc_render_params *p;
-    int mode;  /* 0: PathNeeRenderer, 1: PrimitivePathRenderer, 2: VisualizeGeometryNormal, 3: LightSplattingRenderer */
+    int mode;  /* 0: PathNeeRenderer, 1: PrimitivePathRenderer, 2: VisualizeGeometryNormal, 3: LightSplattingRenderer,
+                  4: BidirPathRenderer */
     double *base, *splat;
     int tiles_x, tiles_y, x0, y0, x1, y1;
     volatile int next_tile;
@@ -1424,6 +1681,11 @@ static void *worker(void *arg) {
                         light_subpath(s, p->num_path_vertices, key, j->splat, &cn);
                         continue;
                     }
+                    if (j->mode == 4) {
+                        bdpt_sample(s, p->num_path_vertices, key, px, py, b, j->splat, &cn, NULL, 0, NULL);
+                        acc[0] += b[0]; acc[1] += b[1]; acc[2] += b[2];
+                        continue;
+                    }
                     const int nee = j->mode == 0;
                     eval_contribution(s, p->num_path_vertices, nee ? p->use_mis : 0, nee, key, px, py, b, &ok, &sx, &sy,
                                       sp, &cn);
@@ -1449,7 +1711,9 @@ static int render_mode(const orc_scene *s, const orc_render_params *p, int mode,
     if (!s || !p || !base || !splat) return fail("null argument");
     if (mode != 2 && p->num_path_vertices <= 1) { if (st) memset(st, 0, sizeof *st); return 0; } /* pathnextevent.h:163 */
     if (p->spp < 0) return fail("negative spp");
-    if ((mode == 0 || mode == 3) && s->light_cdf.n == 0) return fail("scene has no lights");
+    if ((mode == 0 || mode == 3 || mode == 4) && s->light_cdf.n == 0) return fail("scene has no lights");
+    if (mode == 4 && p->num_path_vertices > BDPT_MAXV) return fail("bidirectional path tracing supports at most 64 path vertices");
+    if (s->env_on && mode == 4) return fail("the bidirectional path tracer does not support envmap lights");
     if (s->env_on && s->has_medium) return fail("an envmap with a medium is not supported");
     if (s->env_on && mode == 3) return fail("the light tracer does not support envmap lights");
     job_t j;
@@ -1484,6 +1748,22 @@ int orc_render_nee(const orc_scene *s, const orc_render_params *p, double *base,
 }
 int orc_render_primitive(const orc_scene *s, const orc_render_params *p, double *base, double *splat, orc_stats *st) {
     return render_mode(s, p, 1, base, splat, st);
+}
+int orc_render_bdpt(const orc_scene *s, const orc_render_params *p, double *base, double *splat, orc_stats *st) {
+    return render_mode(s, p, 4, base, splat, st);
+}
+int orc_bdpt_records(const orc_scene *s, int num_path_vertices, uint32_t seed, int px, int py, uint32_t sample,
+                     orc_bdpt_record *out, int max_out, int *n_out) {
+    if (!s || !out || !n_out) return fail("null argument");
+    if (num_path_vertices < 2 || num_path_vertices > BDPT_MAXV) return fail("num_path_vertices must be in [2, 64]");
+    if (s->light_cdf.n == 0 || s->env_on) return fail("needs area lights and no envmap");
+    if (px < 0 || py < 0 || px >= s->W || py >= s->H) return fail("pixel outside the film");
+    counters cn = {0, 0};
+    double b[3];
+    *n_out = 0;
+    bdpt_sample(s, num_path_vertices, path_key(seed, (uint32_t)(py * s->W + px), sample), px, py, b, NULL, &cn, out,
+                max_out, n_out);
+    return 0;
 }
 int orc_render_visualize(const orc_scene *s, const orc_render_params *p, double *base, double *splat, orc_stats *st) {
     return render_mode(s, p, 2, base, splat, st);

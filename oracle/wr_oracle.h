@@ -122,6 +122,27 @@ int orc_render_visualize(const orc_scene *s, const orc_render_params *p,
  * medium are ignored (the reference's sampler there is not volumetric). */
 int orc_render_light(const orc_scene *s, const orc_render_params *p,
                      double *film_base, double *film_splat, orc_stats *stats);
+/* BidirPathRenderer (bidirpath.h:44-421): per camera sample one light and one
+ * camera subpath and every strategy (s, t) with s + t <= N, balance-heuristic
+ * weights; t = 1 into film_splat, the rest into film_base (the NEE film
+ * layout).  N <= 64; use_mis and the medium are ignored (the reference's BDPT
+ * always weights, and its subpath sampler is not volumetric); no envmap. */
+int orc_render_bdpt(const orc_scene *s, const orc_render_params *p,
+                    double *film_base, double *film_splat, orc_stats *stats);
+/* Test hook: the strategies of one camera sample of orc_render_bdpt with a
+ * nonzero unweighted contribution, each with its connected path (vertex 1 the
+ * light point ... vertex n the camera; kind 0 camera, 1 light, 2 surface; id
+ * the material or light index), the area pdfs lp / cp with which the light /
+ * camera side makes each vertex as the weight uses them, and the weight. */
+typedef struct {
+    int s, t, n, pad;
+    double weight, contrib[3];
+    double pos[64][3], nrm[64][3];
+    int kind[64], id[64];
+    double lp[64], cp[64];
+} orc_bdpt_record;
+int orc_bdpt_records(const orc_scene *s, int num_path_vertices, uint32_t seed, int px, int py, uint32_t sample,
+                     orc_bdpt_record *out, int max_out, int *n_out);
 
 /* A homogeneous medium filling the whole scene (src/medium/homogeneous.h:11-66,
  * JSON "medium" {"sigma_a", "sigma_s"}, simple2parser.h:163-167) with a
