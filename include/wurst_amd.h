@@ -257,6 +257,20 @@ int wr_render_light(wr_scene *scene, wr_film *film, const wr_render_params *para
  * Blocking. */
 int wr_render_visualize(wr_scene *scene, wr_film *film, const wr_render_params *params, wr_render_stats *stats);
 
+/* BidirPathRenderer::render (bidirpath.h:316-395; JSON "integrator":
+ * "bidirectional_path_tracer", simple2parser.h:176-179): per camera sample one
+ * light subpath and one camera subpath (GeneralSubpathSampler, not volumetric)
+ * and every strategy (s, t) with s + t <= num_path_vertices, weighted by the
+ * balance heuristic (computeMisWeight, power 1).  t = 1 connections go to the
+ * splat buffer at their raster, the rest to the base buffer at the pixel:
+ * image = base / spp + splat / (spp * W * H), the NEE film layout.  N <= 1
+ * renders nothing; N above 64, an envmap or a scene without lights is an
+ * error; `use_mis` is ignored (the reference always weights) and so is the
+ * medium.  The light subpath stops at N - 1 vertices (an N-th, which the
+ * reference traces, enters no strategy).  Keeps a per-scene buffer of
+ * 64 B x (N - 1) light-vertex records per traversal lane.  Blocking. */
+int wr_render_bdpt(wr_scene *scene, wr_film *film, const wr_render_params *params, wr_render_stats *stats);
+
 /* Closest hit for n host rays (EmbreeAccel::intersect -> rtcIntersect1,
  * embreeaccel.h:353-422).  Misses give geom = prim = WR_INVALID_ID, t = +inf. */
 int wr_trace_closest(wr_scene *scene, const float *rays, uint32_t n, uint32_t *geom, uint32_t *prim, float *t);

@@ -138,6 +138,7 @@ EXPORTS = {
     "wr_render_primitive": (C.c_int, [P, P, C.POINTER(RenderParams), C.POINTER(RenderStats)]),
     "wr_render_visualize": (C.c_int, [P, P, C.POINTER(RenderParams), C.POINTER(RenderStats)]),
     "wr_render_light": (C.c_int, [P, P, C.POINTER(RenderParams), C.POINTER(RenderStats)]),
+    "wr_render_bdpt": (C.c_int, [P, P, C.POINTER(RenderParams), C.POINTER(RenderStats)]),
     "wr_trace_closest": (C.c_int, [P, P, C.c_uint32, P, P, P]),
     "wr_trace_any": (C.c_int, [P, P, C.c_uint32, P]),
     "wr_trace_closest_device": (C.c_int, [P, P, C.c_uint32, P, C.POINTER(C.c_float)]),
@@ -424,6 +425,20 @@ def render_light(scene: GpuScene, film: Film, spp: int, num_path_vertices: int =
     p = render_params(spp, num_path_vertices, seed, False, first_sample, max_paths_in_flight, profile, count_traversal)
     st = RenderStats()
     _check(lib().wr_render_light(scene.h, film.h, C.byref(p), C.byref(st)))
+    film.spp += spp
+    return st
+
+
+def render_bdpt(scene: GpuScene, film: Film, spp: int, num_path_vertices: int = 10, seed: int = 0,
+                first_sample: int | None = None, profile: bool = False) -> RenderStats:
+    """BidirPathRenderer::render on the GPU (bidirpath.h:316-395): per camera sample a
+    light and a camera subpath and every strategy s + t <= N, balance-heuristic
+    weights; film.image() = base/spp + splat/(spp*W*H) as for NEE."""
+    if first_sample is None:
+        first_sample = film.spp
+    p = render_params(spp, num_path_vertices, seed, True, first_sample, 0, profile)
+    st = RenderStats()
+    _check(lib().wr_render_bdpt(scene.h, film.h, C.byref(p), C.byref(st)))
     film.spp += spp
     return st
 

@@ -108,6 +108,8 @@ struct wr_scene {
     Node4 *d_nodes = nullptr;
     float4 *d_tris = nullptr, *d_ltris = nullptr;
     uint2 *d_spill = nullptr;   // traversal stack spill: kSpillDepth entries per lane of the largest grid
+    float4 *d_bdpt = nullptr;   // bidirectional light-vertex records (+ 2 ray counters), grown on demand
+    size_t bdpt_bytes = 0;
     int max_trav_blocks = 0;
     int2 *d_geom = nullptr;
     DMat *d_mats = nullptr;

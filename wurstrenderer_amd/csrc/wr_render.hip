@@ -1197,6 +1197,294 @@ __global__ __launch_bounds__(kBlock) void k_visualize(DScene s, uint32_t wh, uin
     }
 }
 
+// ============================================================== bidirectional path tracing
+// BidirPathRenderer::render (bidirpath.h:316-395) for one lane per pixel and
+// its samples in sample order: a light subpath (traceLightSubpath,
+// bidirpath.h:196-219) whose vertices go to a per-lane record array in HBM,
+// then a camera subpath (traceCameraSubpath, bidirpath.h:221-257) whose every
+// surface vertex connects to every stored light vertex (bdptConnect,
+// bidirpath.h:79-123); t = 1 connections (light vertex -> camera vertex) are
+// FP64 atomic splats, the rest sum into the pixel's FP64 base film.
+//
+// Balance-heuristic weights (computeMisWeight, bidirpath.h:125-194, power 1):
+// 1 / w = 1 + sum of the light-side ratio products + sum of the camera-side
+// ones.  With b_i = (camera-side pdf / light-side pdf) of light vertex i and
+// a_j = (light-side pdf / camera-side pdf) of camera vertex j, the light sum of
+// strategy (s, t) is b_s'(1 + b_{s-1}'(1 + E_{s-2})) with E_k = b_k (1 + E_{k-1}),
+// E_0 = 0, and the camera sum a_t'(1 + a_{t-1}'(1 + D_{t-2})) with
+// D_k = a_k (1 + D_{k-1}), D_1 = 0 (the camera vertex never enters); primes
+// are the pdfs re-evaluated at the connection, the rest are fixed once the
+// subpath has moved two vertices on.  Each connection therefore costs O(1)
+// instead of the reference's loop over the path.
+//
+// Light vertex record (4 float4 per vertex, one plane per field, lane-major):
+//   0: position, pdf with which the light subpath made it (area)
+//   1: geometric normal, -
+//   2: throughput (mPathContrib), E_{k-2}
+//   3: direction to the previous light vertex, material (bits)
+constexpr int kBdptMaxVertices = 64;
+constexpr uint32_t kBdptKeySalt = 0x3c6ef372u;  // the light subpath's walk draws from its own stream
+
+__device__ __forceinline__ float bd_conv(float3 from, float3 to, float3 to_n) {
+    const float3 u = to - from;
+    const float l2 = dot3(u, u);
+    return fabsf(dot3(to_n, u)) / (sqrtf(l2) * l2);
+}
+__device__ __forceinline__ float bd_cam_pdfW(const DCamera &cam, float3 w) {
+    const float c = cam_to_local(cam, w).z;
+    return cam.per_pixel_norm / (c * c * c);
+}
+__device__ __forceinline__ float bd_weight(float inv) {
+    const float w = 1.f / inv;
+    return w >= 0.f && w <= 1.f ? w : 0.f;  // a zero forward pdf (grazing FP32 hit) weighs 0
+}
+
+__global__ __launch_bounds__(kBlock) void k_bdpt(DScene s, uint32_t wh, uint32_t spp, uint32_t first_sample,
+                                                 uint32_t seed, int N, double *film_base, double *film_splat,
+                                                 float4 *__restrict__ recs, unsigned long long *counts, uint2 *spill) {
+    __shared__ uint32_t st_node[kLdsStackDepth * kBlock];
+    __shared__ float st_t[kLdsStackDepth * kBlock];
+    LaneStack<kBlock> st{st_node + lane_id(), st_t + lane_id(),
+                         spill + size_t(blockIdx.x * kBlock + lane_id()) * kSpillDepth, 0};
+    TraversalCounters cnt;
+    const DCamera &cam = s.cam;
+    const float3 cz = f3(cam.zx, cam.zy, cam.zz);
+    const size_t lanes = size_t(gridDim.x) * kBlock, lane = size_t(blockIdx.x) * kBlock + lane_id();
+    auto rec = [&](int k, int f) -> float4 & { return recs[(size_t(k - 1) * 4 + f) * lanes + lane]; };
+    unsigned long long n_ext = 0, n_shd = 0;
+    auto occluded = [&](float3 o, float3 d, float len) {
+        n_shd++;
+        float t = len - kRayEps;
+        return traverse<true, false, kBlock>(s, setup_ray(o, d, kRayEps), t, st, cnt) != kInvalid;
+    };
+    auto closest = [&](float3 o, float3 d, float &t) {
+        n_ext++;
+        t = INFINITY;
+        return traverse<false, false, kBlock>(s, setup_ray(o, d, kRayEps), t, st, cnt);
+    };
+    for (uint32_t pixel = blockIdx.x * kBlock + lane_id(); pixel < wh; pixel += gridDim.x * kBlock) {
+        const uint32_t px = pixel % uint32_t(cam.W), py = pixel / uint32_t(cam.W);
+        double acc_r = 0.0, acc_g = 0.0, acc_b = 0.0;
+        for (uint32_t k = 0; k < spp; k++) {
+            const uint32_t key = path_key(seed, pixel, first_sample + k);
+            const uint32_t kl = mix32(key ^ kBdptKeySalt);
+            const float3 lens = lens_point(cam, key);
+            const float3 cpos = cam_to_world(cam, lens) + f3(cam.ox, cam.oy, cam.oz);
+            // ---- light subpath
+            const LightSample l = sample_light<false>(s, u01(key, kDimLight), u01(key, kDimLight + 1), key);
+            rec(1, 0) = make_float4(l.pos.x, l.pos.y, l.pos.z, l.pdfA);
+            rec(1, 1) = make_float4(l.n.x, l.n.y, l.n.z, 0.f);
+            rec(1, 2) = make_float4(l.contrib.x, l.contrib.y, l.contrib.z, 0.f);
+            int nL = 1;
+            // t = 1 connection of vertex sN (light or surface) to the camera vertex
+            auto splat_camera = [&](float3 p, float3 n, float3 T, float pf, bool is_light, const DMat &m, float3 wi_w,
+                                    float3 q_p, float3 q_n, float q_pf, float E2, int sN) {
+                const float3 uv = cpos - p;
+                const float len2 = dot3(uv, uv), len = sqrtf(len2);
+                const float3 nuv = uv * (1.f / len);
+                const float G = fabsf(dot3(n, nuv)) * fabsf(dot3(cz, nuv)) / len2;
+                if (G == 0.f) return;
+                float rx = 0.f, ry = 0.f;
+                const float we1 = camera_we1(cam, lens, -nuv, rx, ry);
+                if (we1 == 0.f) return;
+                float3 fl;
+                const Frame fr = make_frame(n);
+                if (is_light) {
+                    if (to_local(fr, nuv).y < 0.f) return;
+                    fl = f3(kInvPi, kInvPi, kInvPi);
+                } else {
+                    fl = bsdf_eval(m, to_local(fr, wi_w), to_local(fr, nuv));
+                    if (is_black(fl)) return;
+                }
+                if (occluded(p, nuv, len)) return;
+                // weight: light-side ratios only (t = 1)
+                const float bs = bd_cam_pdfW(cam, -nuv) * fabsf(dot3(n, nuv)) / len2 / pf;
+                float inv;
+                if (sN == 1) {
+                    inv = 1.f + bs;
+                } else {
+                    const float cq = bsdf_pdf(m, to_local(fr, nuv), to_local(fr, wi_w)) * bd_conv(p, q_p, q_n);
+                    inv = 1.f + bs * (1.f + (cq / q_pf) * (1.f + E2));
+                }
+                const float c = bd_weight(inv) * G * we1;
+                const float3 v = T * fl * c;
+                double *dst = film_splat + 3 * (size_t(uint32_t(ry)) * uint32_t(cam.W) + uint32_t(rx));
+                atomicAdd(dst + 0, double(v.x));
+                atomicAdd(dst + 1, double(v.y));
+                atomicAdd(dst + 2, double(v.z));
+            };
+            const DMat none{};
+            splat_camera(l.pos, l.n, l.contrib, l.pdfA, true, none, f3(0, 0, 0), f3(0, 0, 0), f3(0, 0, 0), 1.f, 0.f, 1);
+            if (N - 1 >= 2) {
+                const Frame lf = make_frame(l.n);
+                float3 ro = l.pos;
+                float3 rd = to_world(lf, cosine_hemisphere(u01(kl, kDimPixel), u01(kl, kDimPixel + 1)));
+                float pdfW = fmaxf(to_local(lf, rd).y, 0.f) * kInvPi;
+                float3 w = f3(1.f, 1.f, 1.f), T = l.contrib;
+                // the previous vertex q: position, normal, forward pdf
+                float3 q_p = l.pos, q_n = l.n;
+                float q_pf = l.pdfA;
+                float E2 = 0.f;  // E_{k-2} for the vertex about to be made
+                for (uint32_t bounce = 0;; bounce++) {
+                    float t;
+                    const uint32_t tri = closest(ro, rd, t);
+                    if (tri == kInvalid) break;
+                    const float4 v0 = s.tris[3 * size_t(tri) + 0], v1 = s.tris[3 * size_t(tri) + 1],
+                                 v2 = s.tris[3 * size_t(tri) + 2];
+                    const int2 gi = s.geom_info[__float_as_uint(v1.w)];
+                    if (gi.y >= 0) break;  // a light ends the light subpath
+                    const float3 p = ro + rd * t;
+                    const float3 ng = normalize3(cross3(xyz(v1) - xyz(v0), xyz(v2) - xyz(v0)));
+                    const float3 wiw = -rd;
+                    T = T * w;
+                    const float pf = pdfW * bd_conv(q_p, p, ng);
+                    nL++;
+                    rec(nL, 0) = make_float4(p.x, p.y, p.z, pf);
+                    rec(nL, 1) = make_float4(ng.x, ng.y, ng.z, 0.f);
+                    rec(nL, 2) = make_float4(T.x, T.y, T.z, E2);
+                    rec(nL, 3) = make_float4(wiw.x, wiw.y, wiw.z, __int_as_float(gi.x));
+                    const DMat m = s.mats[gi.x];
+                    splat_camera(p, ng, T, pf, false, m, wiw, q_p, q_n, q_pf, E2, nL);
+                    if (nL == N - 1) break;
+                    const Frame f = make_frame(ng);
+                    const float3 wi = to_local(f, wiw);
+                    const uint32_t dim = kDimBounce0 + kDimsPerBounce * bounce + kBdBsdf;
+                    float3 wo;
+                    float pdf = 0.f;
+                    w = bsdf_sample(m, wi, u01(kl, dim), u01(kl, dim + 1), wo, pdf);
+                    if (is_black(w)) break;
+                    // the previous vertex's camera-side pdf given this direction -> E_{k-1}
+                    const float pr_q = bsdf_pdf(m, wo, wi) * bd_conv(p, q_p, q_n);
+                    E2 = (pr_q / q_pf) * (1.f + E2);  // E_{k-1}, stored with the next vertex
+                    q_p = p; q_n = ng; q_pf = pf;
+                    pdfW = pdf;
+                    ro = p;
+                    rd = to_world(f, wo);
+                }
+            }
+            // ---- camera subpath
+            float pdfW;
+            float3 rd = primary_dir(cam, key, px, py, lens, pdfW);
+            float3 ro = cpos;
+            float3 q_p = cpos, q_n = cz;  // previous camera-subpath vertex
+            float q_pf = 1.f;
+            float3 T = f3(1.f, 1.f, 1.f), w = f3(1.f, 1.f, 1.f);
+            float D2 = 0.f;  // D_{t-2}
+            float sr = 0.f, sg = 0.f, sb = 0.f;
+            for (int tv = 2; tv <= N; tv++) {
+                float th;
+                const uint32_t tri = closest(ro, rd, th);
+                if (tri == kInvalid) break;
+                const float4 v0 = s.tris[3 * size_t(tri) + 0], v1 = s.tris[3 * size_t(tri) + 1],
+                             v2 = s.tris[3 * size_t(tri) + 2];
+                const int2 gi = s.geom_info[__float_as_uint(v1.w)];
+                const float3 p = ro + rd * th;
+                const float3 ng = normalize3(cross3(xyz(v1) - xyz(v0), xyz(v2) - xyz(v0)));
+                const float3 wiw = -rd;
+                T = T * w;
+                const float pf = pdfW * bd_conv(q_p, p, ng);
+                if (gi.y >= 0) {
+                    // s = 0: the camera subpath ends on an emitter (Le1 * Le0 = L on its front)
+                    const float cl = dot3(ng, wiw);
+                    if (cl >= 0.f) {
+                        const DLight &L = s.lights[gi.y];
+                        const float at = L.pdf_select * L.inv_area / pf;
+                        float inv;
+                        if (tv == 2) {
+                            inv = 1.f + at;
+                        } else {
+                            const float pr_q = cl * kInvPi * bd_conv(p, q_p, q_n);
+                            inv = 1.f + at * (1.f + (pr_q / q_pf) * (1.f + D2));
+                        }
+                        const float c = bd_weight(inv) * (kInvPi * kPi);
+                        sr += T.x * L.L[0] * c; sg += T.y * L.L[1] * c; sb += T.z * L.L[2] * c;
+                    }
+                    break;
+                }
+                if (tv == N) break;
+                const DMat m = s.mats[gi.x];
+                const Frame f = make_frame(ng);
+                const float3 wi = to_local(f, wiw);
+                // connections to the stored light vertices, s + t <= N
+                const int smax = min(nL, N - tv);
+                for (int sN = 1; sN <= smax; sN++) {
+                    const float4 A = rec(sN, 0), B = rec(sN, 1), Cc = rec(sN, 2);
+                    const float3 lp = xyz(A), ln = xyz(B);
+                    const float3 uv = p - lp;  // light vertex -> camera vertex
+                    const float len2 = dot3(uv, uv), len = sqrtf(len2);
+                    const float3 nuv = uv * (1.f / len);
+                    const float G = fabsf(dot3(ln, nuv)) * fabsf(dot3(ng, nuv)) / len2;
+                    if (G == 0.f) continue;
+                    const float3 wo_c = to_local(f, -nuv);
+                    const float3 fc = bsdf_eval(m, wi, wo_c);
+                    if (is_black(fc)) continue;
+                    const Frame lfr = make_frame(ln);
+                    float3 fl;
+                    float lpdf_c;  // light-side pdf of the camera vertex (solid angle at the light vertex)
+                    DMat lm{};
+                    float3 lwi_l = f3(0, 0, 0);
+                    if (sN == 1) {
+                        const float cl = to_local(lfr, nuv).y;
+                        if (cl < 0.f) continue;
+                        fl = f3(kInvPi, kInvPi, kInvPi);
+                        lpdf_c = cl * kInvPi;
+                    } else {
+                        const float4 Dd = rec(sN, 3);
+                        lm = s.mats[__float_as_int(Dd.w)];
+                        lwi_l = to_local(lfr, xyz(Dd));
+                        const float3 lwo = to_local(lfr, nuv);
+                        fl = bsdf_eval(lm, lwi_l, lwo);
+                        if (is_black(fl)) continue;
+                        lpdf_c = bsdf_pdf(lm, lwi_l, lwo);
+                    }
+                    if (occluded(lp, nuv, len)) continue;
+                    // light-side sum
+                    const float bs = bsdf_pdf(m, wi, wo_c) * fabsf(dot3(ln, nuv)) / len2 / A.w;
+                    float lsum;
+                    if (sN == 1) {
+                        lsum = bs;
+                    } else {
+                        const float4 Aq = rec(sN - 1, 0), Bq = rec(sN - 1, 1);
+                        const float cq = bsdf_pdf(lm, to_local(lfr, nuv), lwi_l) * bd_conv(lp, xyz(Aq), xyz(Bq));
+                        lsum = bs * (1.f + (cq / Aq.w) * (1.f + Cc.w));
+                    }
+                    // camera-side sum
+                    const float at = lpdf_c * fabsf(dot3(ng, nuv)) / len2 / pf;
+                    float csum;
+                    if (tv == 2) {
+                        csum = at;
+                    } else {
+                        const float lq = bsdf_pdf(m, wo_c, wi) * bd_conv(p, q_p, q_n);
+                        csum = at * (1.f + (lq / q_pf) * (1.f + D2));
+                    }
+                    const float c = bd_weight(1.f + lsum + csum) * G;
+                    const float3 v = xyz(Cc) * fl * fc * T * c;
+                    sr += v.x; sg += v.y; sb += v.z;
+                }
+                // continue the camera subpath
+                const uint32_t dim = kDimBounce0 + kDimsPerBounce * uint32_t(tv - 2) + kBdBsdf;
+                float3 wo;
+                float pdf = 0.f;
+                w = bsdf_sample(m, wi, u01(key, dim), u01(key, dim + 1), wo, pdf);
+                if (is_black(w)) break;
+                if (tv >= 3) {  // D_{t-1} from the previous vertex's light-side pdf given this direction
+                    const float pr_q = bsdf_pdf(m, wo, wi) * bd_conv(p, q_p, q_n);
+                    D2 = (pr_q / q_pf) * (1.f + D2);
+                }
+                q_p = p; q_n = ng; q_pf = pf;
+                pdfW = pdf;
+                ro = p;
+                rd = to_world(f, wo);
+            }
+            acc_r += sr; acc_g += sg; acc_b += sb;
+        }
+        double *dst = film_base + 3 * size_t(pixel);
+        dst[0] += acc_r; dst[1] += acc_g; dst[2] += acc_b;
+    }
+    atomicAdd(counts + 0, n_ext);
+    atomicAdd(counts + 1, n_shd);
+}
+
 // Resident blocks per CU of a kernel (HIP occupancy API), cached per device.
 static int occupancy(wr_device *d, const void *fn, int block) {
     auto it = d->occ_cache.find(fn);
@@ -1255,6 +1543,7 @@ void wr_scene_destroy(wr_scene *s) {
     (void)hipFree(s->d_env);
     (void)hipFree(s->d_env_cdf);
     (void)hipFree(s->d_spill);
+    (void)hipFree(s->d_bdpt);
     (void)hipFree(s->pool);
     if (s->h_counters) (void)hipHostFree(s->h_counters);
     for (auto &e : s->ev)
@@ -2164,6 +2453,64 @@ int wr_render_visualize(wr_scene *s, wr_film *f, const wr_render_params *prm, wr
     }
     st.paths = uint64_t(prm->spp) * wh;
     st.extend_rays = st.paths;
+    st.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (stats) *stats = st;
+    return WR_OK;
+}
+
+int wr_render_bdpt(wr_scene *s, wr_film *f, const wr_render_params *prm, wr_render_stats *stats) {
+    if (!s || !f || !prm) return set_error(WR_ERR_INVALID_ARGUMENT, "null argument");
+    if (f->scene != s) return set_error(WR_ERR_INVALID_ARGUMENT, "film belongs to another scene");
+    if (prm->spp < 0 || prm->first_sample < 0) return set_error(WR_ERR_INVALID_ARGUMENT, "spp and first_sample must be >= 0");
+    const auto t0 = std::chrono::steady_clock::now();
+    wr_render_stats st{};
+    const int N = prm->num_path_vertices;
+    if (N <= 1 || prm->spp == 0) {  // BidirPathRenderer::render returns at once for N <= 1 (bidirpath.h:318)
+        st.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        if (stats) *stats = st;
+        return WR_OK;
+    }
+    if (N > kBdptMaxVertices) return set_error(WR_ERR_UNSUPPORTED, "bidirectional path tracing supports at most 64 path vertices");
+    if (s->ds.n_lights == 0) return set_error(WR_ERR_INVALID_ARGUMENT, "scene has no lights");
+    if (s->ds.env_select > 0.f) return set_error(WR_ERR_UNSUPPORTED, "the bidirectional path tracer does not support envmap lights");
+    int rc = set_device(s->dev);
+    if (rc) return rc;
+    const uint32_t wh = uint32_t(s->W) * uint32_t(s->H);
+    const int grid = trav_grid(s, wh, occupancy(s->dev, reinterpret_cast<const void *>(&k_bdpt), kBlock));
+    // records of N - 1 light vertices per lane, behind 256 bytes of counters
+    const size_t need = 256 + sizeof(float4) * 4 * size_t(N - 1) * size_t(grid) * kBlock;
+    if (need > s->bdpt_bytes) {
+        WR_HIP(hipStreamSynchronize(s->stream));
+        (void)hipFree(s->d_bdpt);
+        s->device_bytes -= s->bdpt_bytes;
+        s->d_bdpt = nullptr;
+        s->bdpt_bytes = 0;
+        WR_HIP(hipMalloc(&s->d_bdpt, need));
+        s->bdpt_bytes = need;
+        s->device_bytes += need;
+    }
+    auto *counts = reinterpret_cast<unsigned long long *>(s->d_bdpt);
+    WR_HIP(hipMemsetAsync(counts, 0, 16, s->stream));
+    // the medium is not passed on: the reference's subpath sampler here is not volumetric
+    DScene ds = s->ds;
+    ds.med.on = 0;
+    if (prm->profile) WR_HIP(hipEventRecord(s->ev[0], s->stream));
+    k_bdpt<<<grid, kBlock, 0, s->stream>>>(ds, wh, uint32_t(prm->spp), uint32_t(prm->first_sample), prm->seed, N,
+                                           f->d_base, f->d_splat, s->d_bdpt + 16, counts, s->d_spill);
+    WR_HIP(hipGetLastError());
+    if (prm->profile) WR_HIP(hipEventRecord(s->ev[1], s->stream));
+    unsigned long long h[2] = {0, 0};
+    WR_HIP(hipMemcpyAsync(h, counts, sizeof h, hipMemcpyDeviceToHost, s->stream));
+    WR_HIP(hipStreamSynchronize(s->stream));
+    if (prm->profile) {
+        float ms = 0.f;
+        WR_HIP(hipEventElapsedTime(&ms, s->ev[0], s->ev[1]));
+        st.extend_ms = ms;
+        st.extend_launches = 1;
+    }
+    st.paths = uint64_t(prm->spp) * wh;
+    st.extend_rays = h[0];
+    st.shadow_rays = h[1];
     st.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (stats) *stats = st;
     return WR_OK;
