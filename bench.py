@@ -328,17 +328,23 @@ def other_integrators(wr, scenes, dev, gs, sc, N, a):
     """The reference's other integrators on the same kernels, on the headline
     scene and frame: primitive_path_tracer and light_splatting at the headline
     depth (light_splatting traces spp * W * H light subpaths), visualize one
-    primary hit per sample, and NEE+MIS with a 1024 x 512 envmap sky added.
-    Mrays/s = (closest-hit + shadow rays) / wall second of one blocking call."""
+    primary hit per sample, NEE+MIS with a 1024 x 512 envmap sky added, and
+    bidirectional_path_tracer at a quarter of their spp (every camera vertex
+    connects to every light vertex).  Mrays/s = (closest-hit + shadow rays) /
+    wall second of one blocking call."""
     import numpy as np
     out = {}
     spp = max(1, a.spp // 4)
+    bspp = max(1, a.spp // 16)
     sky = np.ones((512, 1024, 3), np.float32) * np.float32([0.3, 0.4, 0.6])
     sky[40:48, 300:308] = 200.0  # a sun
-    for name, fn, scene in (("primitive_path_tracer", lambda g, f: wr.render_primitive(g, f, spp, num_path_vertices=N, seed=3), gs),
-                            ("light_splatting", lambda g, f: wr.render_light(g, f, spp, num_path_vertices=N, seed=3), gs),
-                            ("visualize", lambda g, f: wr.render_visualize(g, f, spp, seed=3), gs),
-                            ("nee_envmap_sky", lambda g, f: wr.render_nee(g, f, spp, num_path_vertices=N, seed=3), None)):
+    for name, n_spp, fn, scene in (
+            ("primitive_path_tracer", spp, lambda g, f: wr.render_primitive(g, f, spp, num_path_vertices=N, seed=3), gs),
+            ("light_splatting", spp, lambda g, f: wr.render_light(g, f, spp, num_path_vertices=N, seed=3), gs),
+            ("visualize", spp, lambda g, f: wr.render_visualize(g, f, spp, seed=3), gs),
+            ("nee_envmap_sky", spp, lambda g, f: wr.render_nee(g, f, spp, num_path_vertices=N, seed=3), None),
+            ("bidirectional_path_tracer", bspp, lambda g, f: wr.render_bdpt(g, f, bspp, num_path_vertices=N, seed=3),
+             gs)):
         g = scene
         if g is None:
             g = wr.GpuScene(dev, sc.with_envmap(sky))
@@ -347,9 +353,9 @@ def other_integrators(wr, scenes, dev, gs, sc, N, a):
         f.clear()
         st = fn(g, f)
         rays = st.extend_rays + st.shadow_rays
-        out[name] = {"spp": spp, "paths": st.paths, "rays": rays, "ms": round(st.wall_ms, 2),
+        out[name] = {"spp": n_spp, "paths": st.paths, "rays": rays, "ms": round(st.wall_ms, 2),
                      "mrays_per_s": round(rays / st.wall_ms / 1e3, 1)}
-        print(f"# {name}: {spp} spp in {st.wall_ms:.1f} ms, {rays / st.wall_ms / 1e3:.0f} Mrays/s",
+        print(f"# {name}: {n_spp} spp in {st.wall_ms:.1f} ms, {rays / st.wall_ms / 1e3:.0f} Mrays/s",
               file=sys.stderr, flush=True)
         f.close()
         if g is not gs:

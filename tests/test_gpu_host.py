@@ -55,12 +55,15 @@ def test_cli_primitive_and_visualize_equal_api(tmp_path, gpu_device):
     j = scenes.scene_json(sc, "mesh.obj", [
         {"integrator": "primitive_path_tracer", "output": "prim.pfm", "num_spp": 8, "num_path_vertices": 10, "seed": 5},
         {"integrator": "light_splatting", "output": "light.pfm", "num_spp": 4, "num_path_vertices": 8, "seed": 7},
-        {"integrator": "visualize", "output": "vis.pfm", "num_spp": 4, "seed": 6}])
+        {"integrator": "visualize", "output": "vis.pfm", "num_spp": 4, "seed": 6},
+        {"integrator": "bidirectional_path_tracer", "output": "bdpt.pfm", "num_spp": 4, "num_path_vertices": 7,
+         "seed": 9}])
     (tmp_path / "scene.json").write_text(json.dumps(j))
     r = subprocess.run([EXE, "--simple2parse", str(tmp_path / "scene.json")], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
-    lines = [json.loads(x) for x in r.stdout.strip().splitlines()[-3:]]
-    assert [x["integrator"] for x in lines] == ["primitive_path_tracer", "light_splatting", "visualize"]
+    lines = [json.loads(x) for x in r.stdout.strip().splitlines()[-4:]]
+    assert [x["integrator"] for x in lines] == ["primitive_path_tracer", "light_splatting", "visualize",
+                                                "bidirectional_path_tracer"]
     assert lines[0]["paths"] == 48 * 40 * 8 and lines[2]["paths"] == 48 * 40 * 4
     gs = wr.GpuScene(gpu_device, sc)
     fp, fv = wr.Film(gs), wr.Film(gs)
@@ -71,6 +74,9 @@ def test_cli_primitive_and_visualize_equal_api(tmp_path, gpu_device):
     fl = wr.Film(gs)
     wr.render_light(gs, fl, 4, num_path_vertices=8, seed=7)  # FP64 splat atomics: summation order only
     np.testing.assert_allclose(read_pfm(tmp_path / "light.pfm"), fl.image().astype(np.float32), rtol=1e-5, atol=1e-6)
+    fb = wr.Film(gs)
+    wr.render_bdpt(gs, fb, 4, num_path_vertices=7, seed=9)  # t = 1 splats: FP64 atomics, summation order only
+    np.testing.assert_allclose(read_pfm(tmp_path / "bdpt.pfm"), fb.image().astype(np.float32), rtol=1e-5, atol=1e-6)
 
 
 def test_cli_envmap_scene_equals_api(tmp_path, gpu_device):

@@ -91,7 +91,7 @@ def test_mitsuba_srgb_converts_reflectances(tmp_path):
 
 @pytest.mark.parametrize("mutate,msg", [
     (lambda j: j["render"][0].update(integrator="nonsense"), "unknown integrator"),
-    (lambda j: j["render"][0].update(integrator="bidirectional_path_tracer"), "not part of this build"),
+    (lambda j: j["render"][0].update(integrator="photon_map"), "not part of this build"),
     (lambda j: j["camera"].update(radius=0), "focal distance"),
     (lambda j: j["meshes"][0].update(path="missing.obj"), "cannot open file"),
     (lambda j: j.update(envmap="sky.hdr"), "envmap"),
@@ -152,16 +152,17 @@ def test_medium_and_guiding_keys_are_accepted(tmp_path):
 
 
 def test_other_integrators_of_this_build_parse(tmp_path):
-    """primitive_path_tracer, light_splatting and visualize are dispatched by
-    name as the reference's parser does (simple2parser.h:184-199); the rest
-    stay rejected."""
+    """primitive_path_tracer, light_splatting, visualize and
+    bidirectional_path_tracer are dispatched by name as the reference's parser
+    does (simple2parser.h:176-199); the rest stay rejected."""
     sc = scenes.cornell(1, resolution=(16, 16))
     p = write(tmp_path, sc, renders=({"integrator": "primitive_path_tracer", "num_spp": 2, "num_path_vertices": 5},
                                      {"integrator": "light_splatting", "num_spp": 2, "num_path_vertices": 5},
-                                     {"integrator": "visualize", "num_spp": 2}))
+                                     {"integrator": "visualize", "num_spp": 2},
+                                     {"integrator": "bidirectional_path_tracer", "num_spp": 2, "num_path_vertices": 5}))
     r = run(["--check", "--simple2parse", str(p)])
     assert r.returncode == 0, r.stderr
-    for name in ("photon_map", "bidirectional_path_tracer", "diffuse_prt"):
+    for name in ("photon_map", "diffuse_prt", "glossy_prt2002"):
         p = write(tmp_path, sc, renders=({"integrator": name},))
         r = run(["--check", "--simple2parse", str(p)])
         assert r.returncode != 0 and "not part of this build" in r.stderr

@@ -350,6 +350,35 @@ void PrimitivePathRenderer::render() {
     });
 }
 
+std::shared_ptr<BidirPathRenderer> BidirPathRenderer::FromSimple2Json(
+    const std::shared_ptr<ThinlensCamera> &camera, const std::shared_ptr<const SceneData> &scene,
+    const wj::Value &json, RenderContext *ctx) {
+    auto r = std::make_shared<BidirPathRenderer>();
+    r->camera = camera;
+    r->scene = scene;
+    r->ctx = ctx;
+    r->num_path_vertices = int(json.number("num_path_vertices", 10));
+    r->num_spp = int(json.number("num_spp", 16));
+    r->seed = uint32_t(json.number("seed", 0));
+    return r;
+}
+
+// render (bidirpath.h:316-395): base scaled 1/spp, atomic buffer 1/(spp W H)
+void BidirPathRenderer::render() {
+    Film &film = *camera->film;
+    film.base_scale = 1.0 / double(num_spp);
+    film.splat_scale = 1.0 / double(num_spp) / (double(film.W) * double(film.H));
+    if (num_path_vertices <= 1 || num_spp <= 0) return;
+    render_on_gpu(*camera, *scene, ctx, [&](wr_scene *sc, wr_film *f) {
+        wr_render_params p{};
+        p.spp = num_spp;
+        p.num_path_vertices = num_path_vertices;
+        p.seed = seed;
+        p.use_mis = 1;
+        return wr_render_bdpt(sc, f, &p, &stats);
+    });
+}
+
 std::shared_ptr<LightSplattingRenderer> LightSplattingRenderer::FromSimple2Json(
     const std::shared_ptr<ThinlensCamera> &camera, const std::shared_ptr<const SceneData> &scene,
     const wj::Value &json, RenderContext *ctx) {
@@ -438,7 +467,7 @@ void dump_array(const std::string &dir, const char *name, const std::vector<T> &
 }
 
 // integrator names the reference's dispatch knows (simple2parser.h:176-207)
-const char *kOtherIntegrators[] = {"bidirectional_path_tracer", "photon_map", "diffuse_prt", "glossy_prt2002"};
+const char *kOtherIntegrators[] = {"photon_map", "diffuse_prt", "glossy_prt2002"};
 
 }  // namespace
 
@@ -518,6 +547,8 @@ void Simple2Parser::Parse(const wj::Value &j, const std::string &root_dir, const
             renderer = LightSplattingRenderer::FromSimple2Json(camera, scene, rj, &ctx);
         } else if (name == "visualize") {
             renderer = VisualizeGeometryNormal::FromSimple2Json(camera, scene, rj, &ctx);
+        } else if (name == "bidirectional_path_tracer") {
+            renderer = BidirPathRenderer::FromSimple2Json(camera, scene, rj, &ctx);
         } else {
             for (const char *other : kOtherIntegrators)
                 if (name == other) throw std::runtime_error("integrator \"" + name + "\" is not part of this build");
@@ -543,6 +574,7 @@ void Simple2Parser::Parse(const wj::Value &j, const std::string &root_dir, const
             if (auto *pr = dynamic_cast<PrimitivePathRenderer *>(renderer.get())) { st = &pr->stats; spp = pr->num_spp; }
             if (auto *vz = dynamic_cast<VisualizeGeometryNormal *>(renderer.get())) { st = &vz->stats; spp = vz->num_spp; }
             if (auto *ls = dynamic_cast<LightSplattingRenderer *>(renderer.get())) { st = &ls->stats; spp = ls->num_spp; }
+            if (auto *bd = dynamic_cast<BidirPathRenderer *>(renderer.get())) { st = &bd->stats; spp = bd->num_spp; }
             if (st)
                 std::printf("{\"integrator\": \"%s\", \"output\": \"%s\", \"spp\": %d, \"paths\": %llu, "
                             "\"extend_rays\": %llu, \"ms\": %.3f}\n",

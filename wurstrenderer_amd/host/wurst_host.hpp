@@ -135,6 +135,23 @@ struct LightSplattingRenderer : Renderer {
     wr_render_stats stats{};
 };
 
+// BidirPathRenderer (src/renderer/bidirpath.h:44-429) on the GPU: light and
+// camera subpaths joined by every strategy, balance-heuristic weights; JSON
+// keys "num_path_vertices" (10), "num_spp" (16), "seed".
+struct BidirPathRenderer : Renderer {
+    static std::shared_ptr<BidirPathRenderer> FromSimple2Json(const std::shared_ptr<ThinlensCamera> &camera,
+                                                              const std::shared_ptr<const SceneData> &scene,
+                                                              const wj::Value &json, RenderContext *ctx);
+    void render() override;
+
+    std::shared_ptr<ThinlensCamera> camera;
+    std::shared_ptr<const SceneData> scene;
+    RenderContext *ctx = nullptr;
+    int num_path_vertices = 10, num_spp = 16;
+    uint32_t seed = 0;
+    wr_render_stats stats{};
+};
+
 // VisualizeGeometryNormal (src/renderer/visualizegeomnormal.h:14-76) on the
 // GPU: the primary hit's geometric normal as RGB, 1 on emitters; JSON keys
 // "num_spp" (16), "seed".
