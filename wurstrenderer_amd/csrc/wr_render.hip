@@ -1239,7 +1239,11 @@ __device__ __forceinline__ float bd_weight(float inv) {
     return w >= 0.f && w <= 1.f ? w : 0.f;  // a zero forward pdf (grazing FP32 hit) weighs 0
 }
 
-__global__ __launch_bounds__(kBlock) void k_bdpt(DScene s, uint32_t wh, uint32_t spp, uint32_t first_sample,
+// Capped at 128 VGPRs (4 waves per SIMD; 328 B/lane spill to scratch): 227 VGPRs
+// and 2 waves measured 676 / 4,351 Mrays/s on the 1.31M-triangle frame at 4 spp /
+// a 512^2 Cornell box at 16 spp, 4 waves 912-945 / 5,341, 6 waves 1,046 / 3,530,
+// 8 waves 803 / 2,932; the film is bit-identical across all of them.
+__global__ __launch_bounds__(kBlock, 4) void k_bdpt(DScene s, uint32_t wh, uint32_t spp, uint32_t first_sample,
                                                  uint32_t seed, int N, double *film_base, double *film_splat,
                                                  float4 *__restrict__ recs, unsigned long long *counts, uint2 *spill) {
     __shared__ uint32_t st_node[kLdsStackDepth * kBlock];
