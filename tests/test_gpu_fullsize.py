@@ -10,8 +10,10 @@
   - the image mean equals the FP64 oracle's mean over the same full frame at
     1 spp within 5 standard errors of that oracle estimate (its own two seeds
     give the noise);
-  - primitive path tracing and light tracing of the same frame agree with
-    NEE+MIS in the mean (2%)."""
+  - primitive path tracing, light tracing and bidirectional path tracing of
+    the same frame agree with NEE+MIS in the mean (2%); the bidirectional
+    image is finite and non-negative and makes no more than N - 1 extension
+    rays per subpath."""
 import numpy as np
 import pytest
 
@@ -76,3 +78,17 @@ def test_full_frame_other_integrators_agree(frame):
     print(f"  means: nee {m}, primitive {fp.image().mean((0, 1))}, light {fl.image().mean((0, 1))}", flush=True)
     np.testing.assert_allclose(fp.image().mean((0, 1)), m, rtol=0.02)
     np.testing.assert_allclose(fl.image().mean((0, 1)), m, rtol=0.02)
+
+
+def test_full_frame_bdpt_agrees(frame):
+    sc, gs = frame
+    fn, fb = wr.Film(gs), wr.Film(gs)
+    wr.render_nee(gs, fn, 16, num_path_vertices=N, seed=3)
+    st = wr.render_bdpt(gs, fb, 4, num_path_vertices=N, seed=6)
+    img = fb.image()
+    print(f"  bdpt 4 spp: {st.extend_rays} extension, {st.shadow_rays} shadow rays in {st.wall_ms:.1f} ms; "
+          f"means nee {fn.image().mean((0, 1))}, bdpt {img.mean((0, 1))}", flush=True)
+    assert np.isfinite(img).all() and img.min() >= 0.0
+    assert st.paths == 1920 * 1080 * 4
+    assert st.paths <= st.extend_rays <= st.paths * 2 * (N - 1)
+    np.testing.assert_allclose(img.mean((0, 1)), fn.image().mean((0, 1)), rtol=0.02)

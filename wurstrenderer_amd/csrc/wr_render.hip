@@ -1147,7 +1147,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(DScene s, const float4 *__rest
             const float4 a = rays[2 * size_t(i)], b = rays[2 * size_t(i) + 1];
             const RaySetup r = setup_ray(xyz(a), xyz(b), a.w);
             float t = b.w;
-            const uint32_t tri = traverse<ANY, false, kBlock>(s, r, t, st, cnt);
+            const uint32_t tri = traverse_rounds<ANY, kBlock>(s, r, t, st, cnt);
             if (ANY) occ[i] = tri != kInvalid;
             else hits[i] = make_uint2(tri, __float_as_uint(tri == kInvalid ? INFINITY : t));
         }
@@ -1178,7 +1178,7 @@ __global__ __launch_bounds__(kBlock) void k_visualize(DScene s, uint32_t wh, uin
             float pdfW;
             const float3 dir = primary_dir(cam, key, px, py, lens, pdfW);
             float t = INFINITY;
-            const uint32_t tri = traverse<false, false, kBlock>(s, setup_ray(cpos, dir, kRayEps), t, st, cnt);
+            const uint32_t tri = traverse_rounds<false, kBlock>(s, setup_ray(cpos, dir, kRayEps), t, st, cnt);
             if (tri == kInvalid) {
                 if (s.env_select > 0.f) { r += 1.0; g += 1.0; b += 1.0; }  // the envmap vertex is a light vertex
                 continue;
@@ -1259,12 +1259,12 @@ __global__ __launch_bounds__(kBlock, 4) void k_bdpt(DScene s, uint32_t wh, uint3
     auto occluded = [&](float3 o, float3 d, float len) {
         n_shd++;
         float t = len - kRayEps;
-        return traverse<true, false, kBlock>(s, setup_ray(o, d, kRayEps), t, st, cnt) != kInvalid;
+        return traverse_rounds<true, kBlock>(s, setup_ray(o, d, kRayEps), t, st, cnt) != kInvalid;
     };
     auto closest = [&](float3 o, float3 d, float &t) {
         n_ext++;
         t = INFINITY;
-        return traverse<false, false, kBlock>(s, setup_ray(o, d, kRayEps), t, st, cnt);
+        return traverse_rounds<false, kBlock>(s, setup_ray(o, d, kRayEps), t, st, cnt);
     };
     for (uint32_t pixel = blockIdx.x * kBlock + lane_id(); pixel < wh; pixel += gridDim.x * kBlock) {
         const uint32_t px = pixel % uint32_t(cam.W), py = pixel / uint32_t(cam.W);

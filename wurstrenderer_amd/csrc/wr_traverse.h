@@ -437,4 +437,24 @@ __device__ __forceinline__ uint32_t traverse(const G &s, const RaySetup &r, floa
     return ts.hit;
 }
 
+// Whole traversal of one ray in speculative rounds (trav_round / trav_round_any):
+// the same hits as traverse().  The rounds' ballots count the lanes that are
+// active, so lanes of a wave may enter and leave it at different times.
+template <bool ANY, int kBlock, class G>
+__device__ __forceinline__ uint32_t traverse_rounds(const G &s, const RaySetup &r, float &t_best, LaneStack<kBlock> &st,
+                                                    TraversalCounters &cnt) {
+    TravState ts;
+    ts.r = r;
+    ts.t = t_best;
+    ts.hit = kInvalid;
+    ts.cur = 0;
+    st.sp = 0;
+    bool done;
+    do {
+        done = ANY ? trav_round_any<false, kBlock>(s, ts, st, cnt, true) : trav_round<false, kBlock>(s, ts, st, cnt, true);
+    } while (!done);
+    t_best = ts.t;
+    return ts.hit;
+}
+
 }  // namespace wr
