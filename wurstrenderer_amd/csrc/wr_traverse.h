@@ -438,8 +438,10 @@ __device__ __forceinline__ uint32_t traverse(const G &s, const RaySetup &r, floa
 }
 
 // Whole traversal of one ray in speculative rounds (trav_round / trav_round_any):
-// the same hits as traverse().  The rounds' ballots count the lanes that are
-// active, so lanes of a wave may enter and leave it at different times.
+// the same hits as traverse() except where two triangles meet the ray at exactly
+// the same t (the leaf tested first wins, and rounds test leaves in another
+// order).  The rounds' ballots count the lanes that are active, so lanes of a
+// wave may enter and leave it at different times.
 template <bool ANY, int kBlock, class G>
 __device__ __forceinline__ uint32_t traverse_rounds(const G &s, const RaySetup &r, float &t_best, LaneStack<kBlock> &st,
                                                     TraversalCounters &cnt) {
