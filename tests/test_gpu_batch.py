@@ -4,11 +4,9 @@ A batch gives every job's film what wr_render_nee with the same parameters
 gives it (the same random numbers per pixel and sample); only the grouping of
 the FP64 sums (passes) and the order of the splat atomics differ, so films
 agree to rtol 1e-9.  Graph replay and traversal scheduling parameters change
-the order of work, not the result: same bar.  (One exception exists: where two
-triangles meet a ray at exactly the same t, the speculative traversal rounds'
-order of leaf tests can depend on the other rays of the wave.  Over 1.5G paths
-of the batch benchmark it changed one sample of one scene in one run of five,
-per-scene relMSE 1e-15; these scenes trace ~4000x fewer paths.)"""
+the order of work, never the result: same bar (two triangles at exactly the
+same t resolve to the lower leaf-order index, whatever the order of leaf tests
+or the other rays of a wave)."""
 import numpy as np
 import pytest
 

@@ -219,7 +219,9 @@ __device__ __forceinline__ bool trav_step(const G &s, TravState &ts, LaneStack<k
         for (uint32_t k = 0; k < count; k++) {
             const float4 *tp = s.tris + 3 * size_t(first + k);
             float t;
-            if (tri_test(r, tp[0], tp[1], tp[2], ts.t, t)) {
+            // at equal t the lower leaf-order index wins (kInvalid is the largest), so
+            // the closest hit does not depend on the order in which leaves are tested
+            if (tri_test(r, tp[0], tp[1], tp[2], ts.t, t) && (ANY || t < ts.t || first + k < ts.hit)) {
                 ts.t = t;
                 ts.hit = first + k;
                 if (ANY) return true;
@@ -340,7 +342,7 @@ __device__ __forceinline__ bool trav_round(const G &s, TravState &ts, LaneStack<
         for (uint32_t k = 0; k < count; k++) {
             const float4 *tp = s.tris + 3 * size_t(first + k);
             float t;
-            if (tri_test(r, tp[0], tp[1], tp[2], ts.t, t)) {
+            if (tri_test(r, tp[0], tp[1], tp[2], ts.t, t) && (t < ts.t || first + k < ts.hit)) {  // ties: lower index
                 ts.t = t;
                 ts.hit = first + k;
             }
@@ -438,9 +440,8 @@ __device__ __forceinline__ uint32_t traverse(const G &s, const RaySetup &r, floa
 }
 
 // Whole traversal of one ray in speculative rounds (trav_round / trav_round_any):
-// the same hits as traverse() except where two triangles meet the ray at exactly
-// the same t (the leaf tested first wins, and rounds test leaves in another
-// order).  The rounds' ballots count the lanes that are active, so lanes of a
+// the same hits as traverse() (ties in t go to the lower leaf-order index in
+// both).  The rounds' ballots count the lanes that are active, so lanes of a
 // wave may enter and leave it at different times.
 template <bool ANY, int kBlock, class G>
 __device__ __forceinline__ uint32_t traverse_rounds(const G &s, const RaySetup &r, float &t_best, LaneStack<kBlock> &st,
