@@ -309,7 +309,12 @@ def cpu_baseline(sc, N, target_s):
     """The oracle (FP64 restatement of the reference path) on the host cores:
     whole frames at 1 spp, successive sample indices, until ~target_s of work."""
     import oracle
-    threads = min(16, os.cpu_count() or 1)
+    # every core this process may run on, capped by the CPU share the host grants
+    # one GPU's job (OMP_NUM_THREADS where it is set)
+    avail = len(os.sched_getaffinity(0))
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    threads = min(avail, share) if share > 0 else avail
+    nproc = os.cpu_count() or avail
     osc = oracle.OracleScene(sc)
     total_rays, total_s, total_paths, k = 0, 0.0, 0, 0
     while total_s < target_s:
@@ -321,7 +326,11 @@ def cpu_baseline(sc, N, target_s):
     return {"value": round(total_rays / total_s / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
             "sample": f"oracle FP64 restatement, {k} full frame(s) at 1 spp = {total_paths} paths of the same "
                       f"scene and depth, {total_s:.1f} s on {threads} threads",
-            "samples_per_s": round(total_paths / total_s, 1)}
+            "samples_per_s": round(total_paths / total_s, 1),
+            "nproc": nproc, "affinity_cpus": avail, "cpu_share": share or None,
+            "note": "the project's FP64 CPU restatement of the reference estimator (the reference integrator "
+                    "needs Embree's CMake-generated headers and cannot be built here); threads = the CPUs this "
+                    "process may use, capped by the host's CPU share for one GPU (OMP_NUM_THREADS)"}
 
 
 def other_integrators(wr, scenes, dev, gs, sc, N, a):

@@ -101,7 +101,7 @@ typedef struct {
     /* GPU-only scheduling (no reference counterpart; the CPU path has none):
      * 0 = library default; wr_autotune picks them per workload */
     int32_t use_graph;           /* 1: replay the per-iteration launches as a captured hipGraph */
-    uint32_t fetch_chunk;        /* traversal work items claimed per atomic (default 128) */
+    uint32_t fetch_chunk;        /* traversal work items claimed per atomic (default 128, max 4096) */
     uint32_t refill_lanes;       /* idle lanes that trigger a wave's refill (default 24, max 64) */
 } wr_render_params;
 
@@ -225,6 +225,14 @@ int wr_film_clear(wr_film *film);
  * image = base / spp + splat / (spp * W * H)   (pathnextevent.h:154-157). */
 int wr_film_read(wr_film *film, double *base, double *splat);
 
+/* Checks render parameters without touching a GPU: spp, first_sample and
+ * max_paths_in_flight >= 0, fetch_chunk <= 4096, refill_lanes <= 64.  Every
+ * wr_render_* entry point, wr_render_batch and wr_autotune run it before any
+ * launch.  The reference takes num_spp / num_path_vertices from the renderer
+ * JSON unchecked (pathnextevent.h:17-26); the scheduling fields have no
+ * reference counterpart. */
+int wr_render_params_validate(const wr_render_params *params);
+
 /* PathNeeRenderer::render (pathnextevent.h:152-191): adds spp samples per pixel
  * of the NEE + MIS estimator into `film`.  Blocking. */
 int wr_render_nee(wr_scene *scene, wr_film *film, const wr_render_params *params, wr_render_stats *stats);
@@ -283,6 +291,35 @@ int wr_trace_any(wr_scene *scene, const float *rays, uint32_t n, uint8_t *occlud
  * measured with HIP events on that stream. */
 int wr_trace_closest_device(wr_scene *scene, const float *d_rays, uint32_t n, void *d_hits, float *ms);
 int wr_trace_any_device(wr_scene *scene, const float *d_rays, uint32_t n, uint8_t *d_occluded, float *ms);
+
+/* ------------------------------------------------------------------ per-call checks
+ * The device functions the kernels inline, evaluated one record per GPU lane on
+ * host inputs (staged through device buffers allocated per call), so they can
+ * be compared call by call with the reference's classes.  Diagnostics; not on
+ * the render path.  Vectors are float[3], uniforms float[2], n records each. */
+/* Bsdf::evalBsdf / evalPdfW (bsdf.h:20-42; lambertdiffuse.h:16-51,
+ * orennayar.h:14-83, phong.h:22-93, mixed.h:20-60, twosided.h:25-63) in the
+ * local frame (y = normal): f [n*3] and pdf [n] for (wi, wo), material per record. */
+int wr_debug_bsdf_eval(wr_device *dev, const wr_material *mats, const float *wi, const float *wo, uint32_t n, float *f,
+                       float *pdf);
+/* Bsdf::sampleBsdfCosByPdf: sampled direction wo [n*3], weight f cos / pdf
+ * [n*3] and pdf [n] for incoming wi and uniforms u. */
+int wr_debug_bsdf_sample(wr_device *dev, const wr_material *mats, const float *wi, const float *u, uint32_t n, float *wo,
+                         float *weight, float *pdf);
+/* createLightSubpath with one vertex (generalsubpathsampler.h:199-255 ->
+ * DiffuseAreaLight::sampleLe0ByPdf, diffusearealight.h:53-72): point, unit
+ * normal, contribution (pi L / pdfA_point) / pdfLight and pdfA for uniforms u
+ * on the scene's area lights. */
+int wr_debug_light_sample(wr_scene *scene, const float *u, uint32_t n, float *pos, float *nrm, float *contrib,
+                          float *pdfA);
+/* ThinlensCamera::sampleWe0ByPdf + samplePerPixelWe1CosByPdf (thinlens.h:121-166):
+ * lens point org, unit direction dir and per-pixel pdfW of the sample at
+ * pixel (x, y) = pixel[2i..2i+1] with lens uniforms ulens and jitter upix. */
+int wr_debug_camera_ray(wr_scene *scene, const uint32_t *pixel, const float *ulens, const float *upix, uint32_t n,
+                        float *org, float *dir, float *pdfW);
+/* ThinlensCamera::evalWe1 (thinlens.h:62-80) from lens point org (world) toward
+ * direction w: We1 [n] (0 outside the film) and the raster position [n*2]. */
+int wr_debug_camera_we1(wr_scene *scene, const float *org, const float *w, uint32_t n, float *we1, float *raster);
 
 /* ------------------------------------------------------------------ batches
  * Several independent renders (scene, film, spp, depth, seed) in one path pool:

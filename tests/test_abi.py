@@ -67,6 +67,24 @@ def test_scene_validation_errors():
     _expect_invalid(order, "contiguous")
 
 
+def test_render_params_validation():
+    """Scheduling fields out of range are errors before any launch: a fetch chunk
+    above 4096 could wrap the 32-bit work counter (entries handed out twice)."""
+    good = wr.RenderParams(spp=4, num_path_vertices=10, seed=1, use_mis=1, fetch_chunk=4096, refill_lanes=64)
+    wr.validate_params(good)
+    wr.validate_params(wr.RenderParams(spp=0, num_path_vertices=10))  # zeros select the defaults
+    for field, value, fragment in [("fetch_chunk", 4097, "fetch_chunk"), ("fetch_chunk", 1 << 31, "fetch_chunk"),
+                                   ("refill_lanes", 65, "refill_lanes"), ("spp", -1, "spp"),
+                                   ("first_sample", -3, "first_sample"),
+                                   ("max_paths_in_flight", -1, "max_paths_in_flight")]:
+        p = wr.RenderParams(spp=4, num_path_vertices=10)
+        setattr(p, field, value)
+        with pytest.raises(wr.WrError) as e:
+            wr.validate_params(p)
+        assert e.value.code == 1 and fragment in str(e.value), str(e.value)
+    assert wr.lib().wr_render_params_validate(None) == 1
+
+
 def test_zero_area_light_is_rejected_at_scene_creation_checks():
     b = scenes.SceneBuilder()
     m = b.material(scenes.LAMBERT, rho_d=(0.5, 0.5, 0.5))

@@ -1,0 +1,180 @@
+"""Per-call GPU parity of the device BSDF, light and camera functions against
+the reference's own classes (tests/golden/ref_vectors.json, written by the
+reference's LambertDiffuseBsdf / OrenNayarBsdf / PhongBsdf / MixedBsdf /
+TwoSidedBsdf / DiffuseAreaLight / ThinlensCamera through oracle/ref_vectors.cpp).
+
+The renders exercise these functions only in aggregate; here each record is
+evaluated once by the very device functions the kernels inline
+(wr_debug_* entry points).  The reference is FP64 and the device FP32, so the
+bars are FP32 error bounds, written per quantity:
+  - values: relative 1e-5 (Lambert, Oren-Nayar; square roots and divisions),
+    and for Phong lobes 2e-5 + 6e-7 n: cos^n amplifies the ~1e-7 relative
+    error of the FP32 cosine n-fold (n = 900 in the vectors);
+  - directions: 2e-6 absolute per component, 4e-5 for Phong-sampled ones
+    (sin = sqrt(1 - cos^2) of a cosine within 1e-3 of 1 loses digits);
+  - records whose FP32-rounded input sits within 1e-6 of a branch (a cosine
+    near 0, a light-selection CDF step, the film border) may fall on the other
+    side and are counted, not compared: at most a few per set."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import wurstrenderer_amd as wr
+from wurstrenderer_amd import scenes
+
+pytestmark = pytest.mark.gpu
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "ref_vectors.json")))
+
+
+def _mats(recs):
+    m = np.zeros(len(recs), scenes.MATERIAL_DTYPE)
+    for i, r in enumerate(recs):
+        q = r["mat"]
+        m[i] = (q["type"], q["twosided"], q["rho_d"], q["rho_s"], q["sigma"], q["exponent"])
+    return m
+
+
+def _rtol(m):
+    n = np.where((m["type"] == scenes.PHONG) | (m["type"] == scenes.MIXED), m["exponent"], 0.0)
+    return np.where(n > 0, 2e-5 + 6e-7 * n, 1e-5)
+
+
+def _close(got, ref, rtol, atol):
+    """per-record: every component within rtol * |ref| + atol"""
+    got, ref = np.asarray(got, np.float64), np.asarray(ref, np.float64)
+    rt = np.asarray(rtol, np.float64).reshape(-1, *([1] * (ref.ndim - 1)))
+    ok = np.abs(got - ref) <= rt * np.abs(ref) + atol
+    return ok.reshape(len(ref), -1).all(axis=1)
+
+
+def test_bsdf_eval_per_call(gpu_device):
+    recs = GOLD["bsdf_eval"]
+    m = _mats(recs)
+    wi = np.array([r["wi"] for r in recs])
+    wo = np.array([r["wo"] for r in recs])
+    f_ref = np.array([r["f"] for r in recs])
+    p_ref = np.array([r["pdf"] for r in recs])
+    f, p = wr.debug_bsdf_eval(gpu_device, m, wi, wo)
+    scale = np.abs(f_ref).max()
+    ok = _close(f, f_ref, _rtol(m), 1e-7 * scale) & _close(p, p_ref, _rtol(m), 1e-7 * np.abs(p_ref).max())
+    # a branch input near zero: cosines against the normal, or the Phong lobe's cosine
+    refl = wi * np.array([-1.0, 1.0, -1.0])
+    edge = (np.minimum(np.abs(wi[:, 1]), np.abs(wo[:, 1])) < 1e-6) | (np.abs((refl * wo).sum(1)) < 1e-6)
+    bad = ~ok & ~edge
+    rel = np.abs(f - f_ref).max(1) / np.maximum(np.abs(f_ref).max(1), 1e-30)
+    print(f"bsdf_eval: {len(recs)} records, {int((f_ref != 0).any(1).sum())} non-zero, "
+          f"max rel err {rel[(f_ref != 0).any(1)].max():.2e}, edge records {int(edge.sum())}", flush=True)
+    assert not bad.any(), [(int(i), recs[i]["mat"]["type"], f[i].tolist(), f_ref[i].tolist(), float(p[i]),
+                            float(p_ref[i])) for i in np.nonzero(bad)[0][:5]]
+    assert edge.sum() <= 4
+    for t in (scenes.LAMBERT, scenes.OREN_NAYAR, scenes.PHONG, scenes.MIXED):  # every BSDF, one- and two-sided
+        for two in (0, 1):
+            sel = (m["type"] == t) & (m["twosided"] == two)
+            assert sel.sum() > 0 and (f_ref[sel] != 0).any(), (t, two)
+
+
+def test_bsdf_sample_per_call(gpu_device):
+    recs = GOLD["bsdf_sample"]
+    m = _mats(recs)
+    wi = np.array([r["wi"] for r in recs])
+    u = np.array([r["u"] for r in recs])
+    wo_ref = np.array([r["wo"] for r in recs])
+    w_ref = np.array([r["w"] for r in recs])
+    p_ref = np.array([r["pdf"] for r in recs])
+    wo, w, p = wr.debug_bsdf_sample(gpu_device, m, wi, u)
+    glossy = (m["type"] == scenes.PHONG) | (m["type"] == scenes.MIXED)
+    live = p_ref != 0  # a zero-pdf sample leaves its direction unspecified
+    dir_ok = ~live | (np.abs(wo - wo_ref).max(1) <= np.where(glossy, 4e-5, 2e-6))
+    ok = dir_ok & _close(w, w_ref, _rtol(m), 1e-7 * np.abs(w_ref).max()) & \
+        _close(p, p_ref, _rtol(m), 1e-7 * np.abs(p_ref).max())
+    refl = wi * np.array([-1.0, 1.0, -1.0])
+    edge = (np.abs(wi[:, 1]) < 1e-6) | (np.abs(wo_ref[:, 1]) < 1e-6) | (np.abs((refl * wo_ref).sum(1)) < 1e-6) | \
+        (np.abs(u[:, 0] - 0.5) < 1e-6)  # Mixed picks its component by u0 < 1/2
+    bad = ~ok & ~edge
+    print(f"bsdf_sample: {len(recs)} records, {int(live.sum())} with pdf > 0, max dir err "
+          f"{np.abs(wo - wo_ref).max(1)[live].max():.2e}, edge records {int(edge.sum())}", flush=True)
+    assert not bad.any(), [(int(i), recs[i]["mat"]["type"], wo[i].tolist(), wo_ref[i].tolist(), w[i].tolist(),
+                            w_ref[i].tolist(), float(p[i]), float(p_ref[i])) for i in np.nonzero(bad)[0][:5]]
+    assert edge.sum() <= 4
+    assert (w_ref != 0).any(1).sum() > 100
+
+
+def _light_scene():
+    b = scenes.SceneBuilder()
+    for lm in GOLD["light_meshes"]:
+        b.mesh(np.array(lm["verts"]).reshape(-1, 3), np.array(lm["tris"]).reshape(-1, 3), light_L=lm["L"])
+    return b.build(dict(pos=(0, 0, 5), lookat=(0, 0, 0), fovy=40.0, resolution=(8, 8)), "lights")
+
+
+def test_light_sample_per_call(gpu_device):
+    sc = _light_scene()
+    gs = wr.GpuScene(gpu_device, sc)
+    recs = GOLD["light_sample"]
+    u = np.array([r["u"] for r in recs])
+    pos, nrm, con, pdf = gs.debug_light_sample(u)
+    pos_ref = np.array([r["pos"] for r in recs])
+    nrm_ref = np.array([r["normal"] for r in recs])
+    con_ref = np.array([r["contrib"] for r in recs])
+    pdf_ref = np.array([r["pdfA"] for r in recs])
+    C = float(np.abs(sc.verts).max())
+    ok = (np.abs(pos - pos_ref).max(1) <= 4e-7 * C * 8) & (np.abs(nrm - nrm_ref).max(1) <= 2e-6) & \
+        _close(con, con_ref, 1e-5, 0.0) & _close(pdf, pdf_ref, 1e-5, 0.0)
+    # the light-selection CDF steps (weights |L|_2)
+    w = np.array([np.linalg.norm(lm["L"]) for lm in GOLD["light_meshes"]])
+    cdf = np.cumsum(w) / w.sum()
+    edge = (np.abs(u[:, 0][:, None] - cdf[None, :-1]) < 1e-6).any(1)
+    bad = ~ok & ~edge
+    print(f"light_sample: {len(recs)} records, max pos err {np.abs(pos - pos_ref).max():.2e}, "
+          f"max contrib rel err {(np.abs(con - con_ref) / np.abs(con_ref)).max():.2e}", flush=True)
+    assert not bad.any(), [(int(i), pos[i].tolist(), pos_ref[i].tolist(), con[i].tolist(), con_ref[i].tolist())
+                           for i in np.nonzero(bad)[0][:5]]
+    assert len({r["light"] for r in recs}) == len(GOLD["light_meshes"])  # both lights are sampled
+
+
+def test_camera_per_call(gpu_device):
+    """Camera samples and We1 per record, for every camera of the vectors
+    (pinhole and thin lens, several resolutions and fields of view)."""
+    cams = {}
+    for r in GOLD["camera"]:
+        cams.setdefault(json.dumps(r["cam"], sort_keys=True), []).append(r)
+    assert len(cams) >= 2
+    n_we = 0
+    for key, recs in cams.items():
+        c = json.loads(key)
+        b = scenes.SceneBuilder()
+        b.mesh(*scenes.quad((0, 50, 0), (1, 0, 0), (0, 0, 1)), light_L=(1, 1, 1))
+        sc = b.build(dict(pos=c["pos"], lookat=c["lookat"], up=c["up"], fovy=c["fovy"],
+                          resolution=(c["width"], c["height"]), aperture=c["aperture"], focal=c["focal"]), "cam")
+        gs = wr.GpuScene(gpu_device, sc)
+        px = np.array([r["pixel"] for r in recs])
+        org, d, pdfW = gs.debug_camera_ray(px, [r["ulens"] for r in recs], [r["upix"] for r in recs])
+        org_ref = np.array([r["org"] for r in recs])
+        C = max(1.0, float(np.abs(org_ref).max()))
+        np.testing.assert_allclose(org, org_ref, rtol=0, atol=4e-7 * C, err_msg=key)
+        np.testing.assert_allclose(d, np.array([r["dir"] for r in recs]), rtol=0, atol=2e-6, err_msg=key)
+        np.testing.assert_allclose(pdfW, [r["pdfW"] for r in recs], rtol=1e-5, err_msg=key)
+        we1, raster = gs.debug_camera_we1(org_ref, [r["w"] for r in recs])
+        we_ref = np.array([r["we1"] for r in recs])
+        ras_ref = np.array([r["we1_raster"] if r["we1"] != 0 else [-1.0, -1.0] for r in recs])
+        W, H = c["width"], c["height"]
+        border = (np.abs(ras_ref[:, 0]) < 1e-3) | (np.abs(ras_ref[:, 0] - W) < 1e-3) | \
+            (np.abs(ras_ref[:, 1]) < 1e-3) | (np.abs(ras_ref[:, 1] - H) < 1e-3)
+        ok = _close(we1, we_ref, 1e-5, 0.0) & ((we_ref == 0) | (np.abs(raster - ras_ref).max(1) <= 2e-6 * max(W, H)))
+        assert (ok | border).all(), (key, np.nonzero(~ok)[0][:5], we1[~ok][:5], we_ref[~ok][:5])
+        n_we += int((we_ref != 0).sum())
+        gs.close()
+    assert n_we > 10
+    print(f"camera: {len(GOLD['camera'])} records over {len(cams)} cameras, {n_we} with We1 > 0", flush=True)
+
+
+def test_debug_entry_points_reject_bad_input(gpu_device):
+    sc = _light_scene()
+    gs = wr.GpuScene(gpu_device, sc)
+    with pytest.raises(wr.WrError, match="pixel outside"):
+        gs.debug_camera_ray([[8, 0]], [[0.5, 0.5]], [[0.5, 0.5]])
+    m = np.zeros(1, scenes.MATERIAL_DTYPE)
+    m["type"] = 7
+    with pytest.raises(wr.WrError, match="material type"):
+        wr.debug_bsdf_eval(gpu_device, m, [[0, 1, 0]], [[0, 1, 0]])

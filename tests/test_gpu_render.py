@@ -142,6 +142,14 @@ def test_degenerate_requests(gpu_device):
     assert not b.any() and not s.any()
     with pytest.raises(wr.WrError):
         wr.render_nee(gs, film, -1)
+    # an oversized work chunk is rejected before any launch (it could wrap the
+    # work counter), for single renders and batches alike
+    with pytest.raises(wr.WrError, match="fetch_chunk"):
+        wr.render_nee(gs, film, 4, num_path_vertices=10, fetch_chunk=1 << 20)
+    with pytest.raises(wr.WrError, match="fetch_chunk"):
+        wr.render_batch([dict(scene=gs, film=film, spp=4, num_path_vertices=10, seed=1)], fetch_chunk=4097)
+    b, s = film.read()
+    assert not b.any() and not s.any()
 
 
 def test_closing_a_scene_closes_its_films_and_guides(gpu_device):

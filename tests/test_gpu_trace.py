@@ -151,6 +151,46 @@ def test_watertight_closed_icosphere_far_from_origin(gpu_device):
     assert (g2 == 0).all()
 
 
+def test_watertight_shared_edges(gpu_device):
+    """Rays aimed at points on the edges two triangles share, from inside a
+    closed mesh far from the origin, must all hit: the two triangles' edge
+    functions for that edge are exact negations of each other only when neither
+    is computed with a fused multiply-add (a crack otherwise lets a ray within an
+    ulp of the edge miss both)."""
+    v, f = scenes.icosphere(4)
+    b = scenes.SceneBuilder()
+    m = b.material(scenes.LAMBERT, rho_d=(0.5, 0.5, 0.5))
+    centre = np.array([1000.0, -500.0, 250.0])
+    b.mesh(v * 2.0 + centre, f, material=m)
+    b.mesh(*scenes.quad(centre + [0, 50, 0], (1, 0, 0), (0, 0, 1)), light_L=(1, 1, 1))
+    sc = b.build(dict(pos=(0, 0, 0), lookat=(1, 0, 0), fovy=40.0, resolution=(8, 8)), "ico_far_edges")
+    gs = wr.GpuScene(gpu_device, sc)
+    verts = sc.verts.astype(np.float64)
+    tris = sc.tris[sc.tri_geom == 0]
+    e = np.sort(np.concatenate([tris[:, [0, 1]], tris[:, [1, 2]], tris[:, [2, 0]]]), axis=1)
+    edges = np.unique(e, axis=0)
+    assert len(edges) == len(tris) * 3 // 2  # closed: every edge shared by two triangles
+    rng = np.random.default_rng(17)
+    fr = np.concatenate([[0.5], rng.uniform(0.02, 0.98, 7)])
+    pts = (verts[edges[:, 0]][:, None, :] * (1 - fr)[None, :, None] +
+           verts[edges[:, 1]][:, None, :] * fr[None, :, None]).reshape(-1, 3)
+    origins = np.concatenate([centre[None], centre + rng.uniform(-0.6, 0.6, size=(3, 3))]).astype(np.float32)
+    rays = []
+    for o in origins:
+        d = pts - o.astype(np.float64)
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        r = np.zeros((len(pts), 8), np.float32)
+        r[:, :3] = o
+        r[:, 4:7] = d
+        r[:, 7] = np.inf
+        rays.append(r)
+    rays = np.concatenate(rays)
+    print(f"shared-edge rays: {len(rays)} ({len(edges)} edges x {len(fr)} points x {len(origins)} origins)", flush=True)
+    g, _, t = gs.trace_closest(rays)
+    assert (g == 0).all(), (g != 0).sum()
+    assert gs.trace_any(rays).all()
+
+
 def _hip():
     """The HIP runtime the library links (already loaded in this process; found
     through the process's own mappings so no second runtime is loaded)."""

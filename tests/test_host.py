@@ -183,3 +183,54 @@ def test_envmap_pfm_is_loaded_and_checked(tmp_path):
     bad = sc.with_envmap(np.ones((8, 8, 3)))
     r = run(["--check", "--simple2parse", str(write(tmp_path, bad))])
     assert r.returncode == 1 and "wrong size" in r.stderr
+
+
+def test_geometry_ids_follow_the_reference_attach_order(tmp_path):
+    """Per OBJ the reference attaches the surfaces first and then the emitters,
+    each in file order, OBJ after OBJ (assimpio.h:123-150 splits them,
+    simple2parser.h:38-61 attaches mGeometries before mLightGeometries); the
+    light table follows the emitters in the same order.  The OBJs here list an
+    emitter first."""
+    def quad_lines(k, y):
+        b = 4 * k
+        return [f"v -1 {y} -1", f"v 1 {y} -1", f"v 1 {y} 1", f"v -1 {y} 1", f"f {b + 1} {b + 2} {b + 3} {b + 4}"]
+    mtl = ["newmtl red", "Kd 0.5 0.1 0.1", "Ks 0 0 0", "Ns 0", "Ke 0 0 0",
+           "newmtl gray", "Kd 0.4 0.4 0.4", "Ks 0 0 0", "Ns 0", "Ke 0 0 0",
+           "newmtl lampA", "Kd 0 0 0", "Ks 0 0 0", "Ns 0", "Ke 1 2 3",
+           "newmtl lampB", "Kd 0 0 0", "Ks 0 0 0", "Ns 0", "Ke 4 5 6",
+           "newmtl lampC", "Kd 0 0 0", "Ks 0 0 0", "Ns 0", "Ke 7 8 9"]
+    (tmp_path / "m.mtl").write_text("\n".join(mtl) + "\n")
+    a = ["mtllib m.mtl"]
+    for k, (name, y) in enumerate([("lampA", 3.0), ("red", 0.0), ("lampB", 4.0), ("gray", 1.0)]):
+        ql = quad_lines(k, y)
+        a += [f"o part{k}", f"usemtl {name}"] + ql[:4] + [ql[4]]
+    (tmp_path / "a.obj").write_text("\n".join(a) + "\n")
+    b = ["mtllib m.mtl"]
+    for k, (name, y) in enumerate([("lampC", 5.0), ("gray", 2.0)]):
+        ql = quad_lines(k, y)
+        b += [f"o part{k}", f"usemtl {name}"] + ql[:4] + [ql[4]]
+    (tmp_path / "b.obj").write_text("\n".join(b) + "\n")
+    j = {"camera": {"pos": [0, 0, 6], "lookat": [0, 0, 0], "up": [0, 1, 0], "resolution": [8, 8], "fovy": 40.0},
+         "meshes": [{"path": "a.obj"}, {"path": "b.obj"}], "render": [NEE]}
+    p = tmp_path / "scene.json"
+    p.write_text(json.dumps(j))
+    d = tmp_path / "dump"
+    d.mkdir()
+    r = run(["--check", "--dump", str(d), "--simple2parse", str(p)])
+    assert r.returncode == 0, r.stderr
+    gm = np.fromfile(d / "geom_material.i32", np.int32)
+    gl = np.fromfile(d / "geom_light.i32", np.int32)
+    lights = np.fromfile(d / "lights.bin", scenes.LIGHT_DTYPE)
+    mats = np.fromfile(d / "materials.bin", scenes.MATERIAL_DTYPE)
+    verts = np.fromfile(d / "verts.f32", np.float32).reshape(-1, 3)
+    tris = np.fromfile(d / "tris.u32", np.uint32).reshape(-1, 3)
+    tg = np.fromfile(d / "tri_geom.u32", np.uint32)
+    # a.obj: red, gray, lampA, lampB; then b.obj: gray, lampC
+    np.testing.assert_array_equal(gl, [-1, -1, 0, 1, -1, 2])
+    assert (gm[[0, 1, 4]] >= 0).all() and (gm[[2, 3, 5]] == -1).all()
+    np.testing.assert_allclose(mats[gm[0]]["rho_d"], [0.5, 0.1, 0.1])
+    np.testing.assert_allclose(mats[gm[1]]["rho_d"], [0.4, 0.4, 0.4])
+    np.testing.assert_array_equal(lights["geom"], [2, 3, 5])
+    np.testing.assert_array_equal(lights["L"], [[1, 2, 3], [4, 5, 6], [7, 8, 9]])
+    heights = [float(verts[tris[tg == g]][..., 1].mean()) for g in range(6)]
+    np.testing.assert_allclose(heights, [0.0, 1.0, 3.0, 4.0, 2.0, 5.0])

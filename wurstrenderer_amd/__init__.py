@@ -134,6 +134,7 @@ EXPORTS = {
     "wr_film_destroy": (None, [P]),
     "wr_film_clear": (C.c_int, [P]),
     "wr_film_read": (C.c_int, [P, P, P]),
+    "wr_render_params_validate": (C.c_int, [C.POINTER(RenderParams)]),
     "wr_render_nee": (C.c_int, [P, P, C.POINTER(RenderParams), C.POINTER(RenderStats)]),
     "wr_render_primitive": (C.c_int, [P, P, C.POINTER(RenderParams), C.POINTER(RenderStats)]),
     "wr_render_visualize": (C.c_int, [P, P, C.POINTER(RenderParams), C.POINTER(RenderStats)]),
@@ -143,6 +144,11 @@ EXPORTS = {
     "wr_trace_any": (C.c_int, [P, P, C.c_uint32, P]),
     "wr_trace_closest_device": (C.c_int, [P, P, C.c_uint32, P, C.POINTER(C.c_float)]),
     "wr_trace_any_device": (C.c_int, [P, P, C.c_uint32, P, C.POINTER(C.c_float)]),
+    "wr_debug_bsdf_eval": (C.c_int, [P, P, P, P, C.c_uint32, P, P]),
+    "wr_debug_bsdf_sample": (C.c_int, [P, P, P, P, C.c_uint32, P, P, P]),
+    "wr_debug_light_sample": (C.c_int, [P, P, C.c_uint32, P, P, P, P]),
+    "wr_debug_camera_ray": (C.c_int, [P, P, P, P, C.c_uint32, P, P, P]),
+    "wr_debug_camera_we1": (C.c_int, [P, P, P, C.c_uint32, P, P]),
     "wr_render_batch": (C.c_int, [C.POINTER(BatchJob), C.c_uint32, C.POINTER(RenderParams), C.POINTER(RenderStats)]),
     "wr_autotune": (C.c_int, [C.POINTER(BatchJob), C.c_uint32, C.POINTER(RenderParams), C.POINTER(RenderParams),
                               C.POINTER(C.c_double)]),
@@ -241,6 +247,38 @@ def validate(sc: scenes.Scene) -> None:
     _check(lib().wr_scene_validate(C.byref(d)))
 
 
+def validate_params(params: RenderParams) -> None:
+    """Host-side render-parameter checks (raises WrError)."""
+    _check(lib().wr_render_params_validate(C.byref(params)))
+
+
+def debug_bsdf_eval(device: Device, mats, wi, wo):
+    """wr_debug_bsdf_eval: the device BSDF's f and pdf per record (local frame, y = normal);
+    mats is an array of scenes.MATERIAL_DTYPE, one per record."""
+    m = np.ascontiguousarray(mats, scenes.MATERIAL_DTYPE)
+    a = np.ascontiguousarray(wi, np.float32).reshape(-1, 3)
+    b = np.ascontiguousarray(wo, np.float32).reshape(-1, 3)
+    n = len(m)
+    assert len(a) == n and len(b) == n
+    f, pdf = np.zeros((n, 3), np.float32), np.zeros(n, np.float32)
+    _check(lib().wr_debug_bsdf_eval(device.h, m.ctypes.data, a.ctypes.data, b.ctypes.data, n, f.ctypes.data,
+                                    pdf.ctypes.data))
+    return f, pdf
+
+
+def debug_bsdf_sample(device: Device, mats, wi, u):
+    """wr_debug_bsdf_sample: the device BSDF's sampled direction, f cos / pdf and pdf per record."""
+    m = np.ascontiguousarray(mats, scenes.MATERIAL_DTYPE)
+    a = np.ascontiguousarray(wi, np.float32).reshape(-1, 3)
+    uu = np.ascontiguousarray(u, np.float32).reshape(-1, 2)
+    n = len(m)
+    assert len(a) == n and len(uu) == n
+    wo, w, pdf = np.zeros((n, 3), np.float32), np.zeros((n, 3), np.float32), np.zeros(n, np.float32)
+    _check(lib().wr_debug_bsdf_sample(device.h, m.ctypes.data, a.ctypes.data, uu.ctypes.data, n, wo.ctypes.data,
+                                      w.ctypes.data, pdf.ctypes.data))
+    return wo, w, pdf
+
+
 def bvh_selftest(sc: scenes.Scene) -> dict:
     """Builds the BVH on the host and verifies its invariants (raises WrError)."""
     d, keep = scene_desc(sc)
@@ -318,6 +356,36 @@ class GpuScene:
         _check(lib().wr_scene_bvh_check(self.h, C.byref(d), C.byref(st)))
         del keep
         return st.as_dict()
+
+    def debug_light_sample(self, u):
+        """wr_debug_light_sample: the device's light sampling at uniforms u (n, 2)."""
+        u = np.ascontiguousarray(u, np.float32).reshape(-1, 2)
+        n = len(u)
+        pos, nrm, con, pdf = (np.zeros((n, 3), np.float32), np.zeros((n, 3), np.float32),
+                              np.zeros((n, 3), np.float32), np.zeros(n, np.float32))
+        _check(lib().wr_debug_light_sample(self.h, u.ctypes.data, n, pos.ctypes.data, nrm.ctypes.data,
+                                           con.ctypes.data, pdf.ctypes.data))
+        return pos, nrm, con, pdf
+
+    def debug_camera_ray(self, pixel, ulens, upix):
+        """wr_debug_camera_ray: lens point, direction and per-pixel pdfW of camera samples."""
+        px = np.ascontiguousarray(pixel, np.uint32).reshape(-1, 2)
+        ul = np.ascontiguousarray(ulens, np.float32).reshape(-1, 2)
+        up = np.ascontiguousarray(upix, np.float32).reshape(-1, 2)
+        n = len(px)
+        org, d, pdf = np.zeros((n, 3), np.float32), np.zeros((n, 3), np.float32), np.zeros(n, np.float32)
+        _check(lib().wr_debug_camera_ray(self.h, px.ctypes.data, ul.ctypes.data, up.ctypes.data, n, org.ctypes.data,
+                                         d.ctypes.data, pdf.ctypes.data))
+        return org, d, pdf
+
+    def debug_camera_we1(self, org, w):
+        """wr_debug_camera_we1: We1 and raster position toward directions w from lens points org."""
+        o = np.ascontiguousarray(org, np.float32).reshape(-1, 3)
+        ww = np.ascontiguousarray(w, np.float32).reshape(-1, 3)
+        n = len(o)
+        we1, r = np.zeros(n, np.float32), np.zeros((n, 2), np.float32)
+        _check(lib().wr_debug_camera_we1(self.h, o.ctypes.data, ww.ctypes.data, n, we1.ctypes.data, r.ctypes.data))
+        return we1, r
 
     def trace_closest(self, rays: np.ndarray):
         rays = np.ascontiguousarray(rays, np.float32)

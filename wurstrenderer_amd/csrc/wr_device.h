@@ -463,4 +463,27 @@ __device__ __forceinline__ float camera_we1(const DCamera &c, float3 lens_local,
     return c.per_film_norm / (c2 * c2);
 }
 
+// Lens point of a camera sample (sampleWe0ByPdf, thinlens.h:121-139): the
+// pinhole, or the point of the aperture disk at uniforms (u0, u1).
+__device__ __forceinline__ float3 lens_point_u(const DCamera &cam, float u0, float u1) {
+    float lx = 0.f, ly = 0.f;
+    if (cam.aperture > 0.f) {
+        disk_from_square(u0, u1, lx, ly);
+        lx *= cam.aperture; ly *= cam.aperture;
+    }
+    return f3(lx, ly, 0.f);
+}
+// Primary direction through pixel (px, py) jittered by (j0, j1)
+// (samplePerPixelWe1CosByPdf, thinlens.h:141-166) in world space, and its
+// per-pixel pdfW.
+__device__ __forceinline__ float3 primary_dir_u(const DCamera &cam, float j0, float j1, uint32_t px, uint32_t py,
+                                                float3 lens, float &pdfW) {
+    const float ns0 = (float(px) + j0) * cam.invW, ns1 = (float(py) + j1) * cam.invH;
+    const float3 film = f3((ns0 - 0.5f) * float(cam.W), (ns1 - 0.5f) * float(cam.H), -cam.d);
+    const float3 focal = film * (cam.focal / film.z);
+    const float3 dl = normalize3(focal - lens);
+    pdfW = cam.per_pixel_norm / (dl.z * dl.z * dl.z);
+    return cam_to_world(cam, dl);
+}
+
 }  // namespace wr

@@ -163,27 +163,14 @@ struct Geo {
     const float4 *tris;
 };
 
-// Lens point of a camera sample (sampleWe0ByPdf, thinlens.h:121-139): the
-// pinhole, or a point of the aperture disk.
+// Camera sample of path `key`: lens point and primary direction from its
+// lens and pixel-jitter dimensions (lens_point_u / primary_dir_u).
 __device__ __forceinline__ float3 lens_point(const DCamera &cam, uint32_t key) {
-    float lx = 0.f, ly = 0.f;
-    if (cam.aperture > 0.f) {
-        disk_from_square(u01(key, kDimLens), u01(key, kDimLens + 1), lx, ly);
-        lx *= cam.aperture; ly *= cam.aperture;
-    }
-    return f3(lx, ly, 0.f);
+    return lens_point_u(cam, u01(key, kDimLens), u01(key, kDimLens + 1));
 }
-// Primary direction through the jittered pixel (samplePerPixelWe1CosByPdf,
-// thinlens.h:141-166) in world space, and its per-pixel pdfW.
 __device__ __forceinline__ float3 primary_dir(const DCamera &cam, uint32_t key, uint32_t px, uint32_t py, float3 lens,
                                               float &pdfW) {
-    const float j0 = u01(key, kDimPixel), j1 = u01(key, kDimPixel + 1);
-    const float ns0 = (float(px) + j0) * cam.invW, ns1 = (float(py) + j1) * cam.invH;
-    const float3 film = f3((ns0 - 0.5f) * float(cam.W), (ns1 - 0.5f) * float(cam.H), -cam.d);
-    const float3 focal = film * (cam.focal / film.z);
-    const float3 dl = normalize3(focal - lens);
-    pdfW = cam.per_pixel_norm / (dl.z * dl.z * dl.z);
-    return cam_to_world(cam, dl);
+    return primary_dir_u(cam, u01(key, kDimPixel), u01(key, kDimPixel + 1), px, py, lens, pdfW);
 }
 
 // ============================================================== regenerate
@@ -2202,6 +2189,22 @@ static int drain(wr_scene *s, PathBuffers &pb, uint64_t max_iters, bool use_grap
     return rc;  // an even number of iterations: pb is the current state again
 }
 
+// Largest traversal work chunk: every wave claims at least one chunk from a
+// 32-bit work counter, so chunk x waves must stay far below 2^32 (a wrapped
+// counter would hand out entries twice).
+constexpr uint32_t kMaxFetchChunk = 4096;
+
+int wr_render_params_validate(const wr_render_params *p) {
+    if (!p) return set_error(WR_ERR_INVALID_ARGUMENT, "params is null");
+    if (p->spp < 0) return set_error(WR_ERR_INVALID_ARGUMENT, "spp must be >= 0");
+    if (p->first_sample < 0) return set_error(WR_ERR_INVALID_ARGUMENT, "first_sample must be >= 0");
+    if (p->max_paths_in_flight < 0) return set_error(WR_ERR_INVALID_ARGUMENT, "max_paths_in_flight must be >= 0");
+    if (p->fetch_chunk > kMaxFetchChunk)
+        return set_error(WR_ERR_INVALID_ARGUMENT, "fetch_chunk must be 0 (default) or 1.." + std::to_string(kMaxFetchChunk));
+    if (p->refill_lanes > 64) return set_error(WR_ERR_INVALID_ARGUMENT, "refill_lanes must be 0 (default) or 1..64");
+    return WR_OK;
+}
+
 // Estimators of the frame loop.
 enum Estimator { kEstNee, kEstPrimitive, kEstLight };
 // The frame loop of wr_render_nee, wr_render_guided (g non-null),
@@ -2210,7 +2213,7 @@ static int render_impl(wr_scene *s, wr_film *f, const wr_render_params *prm, wr_
                        wr_render_stats *stats, wr_guide_stats *gstats, Estimator est = kEstNee) {
     if (!s || !f || !prm) return set_error(WR_ERR_INVALID_ARGUMENT, "null argument");
     if (f->scene != s) return set_error(WR_ERR_INVALID_ARGUMENT, "film belongs to another scene");
-    if (prm->spp < 0) return set_error(WR_ERR_INVALID_ARGUMENT, "spp must be >= 0");
+    if (int rc0 = wr_render_params_validate(prm)) return rc0;
     if (g && !gp) return set_error(WR_ERR_INVALID_ARGUMENT, "null guided parameters");
     if (g && guide_scene(g) != s) return set_error(WR_ERR_INVALID_ARGUMENT, "guide belongs to another scene");
     if (g && !(gp->guide_fraction >= 0.f && gp->guide_fraction < 1.f))
@@ -2260,7 +2263,7 @@ static int render_impl(wr_scene *s, wr_film *f, const wr_render_params *prm, wr_
     const bool use_graph = prm->use_graph != 0 && !prof;
     TravTune tune;
     if (prm->fetch_chunk) tune.chunk = prm->fetch_chunk;
-    if (prm->refill_lanes) tune.refill = std::min<uint32_t>(prm->refill_lanes, 64);
+    if (prm->refill_lanes) tune.refill = prm->refill_lanes;
     std::vector<hipEvent_t> evs;
     struct Mark { int kind; size_t e0; };
     std::vector<Mark> marks;
@@ -2434,7 +2437,7 @@ int wr_render_light(wr_scene *s, wr_film *f, const wr_render_params *prm, wr_ren
 int wr_render_visualize(wr_scene *s, wr_film *f, const wr_render_params *prm, wr_render_stats *stats) {
     if (!s || !f || !prm) return set_error(WR_ERR_INVALID_ARGUMENT, "null argument");
     if (f->scene != s) return set_error(WR_ERR_INVALID_ARGUMENT, "film belongs to another scene");
-    if (prm->spp < 0) return set_error(WR_ERR_INVALID_ARGUMENT, "spp must be >= 0");
+    if (int rc0 = wr_render_params_validate(prm)) return rc0;
     const auto t0 = std::chrono::steady_clock::now();
     int rc = set_device(s->dev);
     if (rc) return rc;
@@ -2465,7 +2468,7 @@ int wr_render_visualize(wr_scene *s, wr_film *f, const wr_render_params *prm, wr
 int wr_render_bdpt(wr_scene *s, wr_film *f, const wr_render_params *prm, wr_render_stats *stats) {
     if (!s || !f || !prm) return set_error(WR_ERR_INVALID_ARGUMENT, "null argument");
     if (f->scene != s) return set_error(WR_ERR_INVALID_ARGUMENT, "film belongs to another scene");
-    if (prm->spp < 0 || prm->first_sample < 0) return set_error(WR_ERR_INVALID_ARGUMENT, "spp and first_sample must be >= 0");
+    if (int rc0 = wr_render_params_validate(prm)) return rc0;
     const auto t0 = std::chrono::steady_clock::now();
     wr_render_stats st{};
     const int N = prm->num_path_vertices;
@@ -2523,6 +2526,7 @@ int wr_render_bdpt(wr_scene *s, wr_film *f, const wr_render_params *prm, wr_rend
 // ============================================================== batches (host)
 int wr_render_batch(const wr_batch_job *jobs, uint32_t n_jobs, const wr_render_params *prm, wr_render_stats *stats) {
     if (!jobs || !prm) return set_error(WR_ERR_INVALID_ARGUMENT, "null argument");
+    if (int rc0 = wr_render_params_validate(prm)) return rc0;
     if (n_jobs == 0) return set_error(WR_ERR_INVALID_ARGUMENT, "empty batch");
     if (n_jobs > uint32_t(kMaxBatchJobs)) return set_error(WR_ERR_UNSUPPORTED, "too many jobs in a batch");
     wr_scene *s0 = jobs[0].scene;
@@ -2578,7 +2582,7 @@ int wr_render_batch(const wr_batch_job *jobs, uint32_t n_jobs, const wr_render_p
     const bool use_graph = prm->use_graph != 0 && !prof;
     TravTune tune;
     if (prm->fetch_chunk) tune.chunk = prm->fetch_chunk;
-    if (prm->refill_lanes) tune.refill = std::min<uint32_t>(prm->refill_lanes, 64);
+    if (prm->refill_lanes) tune.refill = prm->refill_lanes;
     std::vector<hipEvent_t> evs;
     struct Mark { int kind; size_t e0; };
     std::vector<Mark> marks;
@@ -2704,6 +2708,7 @@ int wr_render_batch(const wr_batch_job *jobs, uint32_t n_jobs, const wr_render_p
 int wr_autotune(const wr_batch_job *jobs, uint32_t n_jobs, const wr_render_params *base, wr_render_params *best,
                 double *mrays) {
     if (!jobs || !base || !best) return set_error(WR_ERR_INVALID_ARGUMENT, "null argument");
+    if (int rc0 = wr_render_params_validate(base)) return rc0;
     if (n_jobs == 0) return set_error(WR_ERR_INVALID_ARGUMENT, "empty batch");
     // trial jobs: 1 spp each into scratch films
     std::vector<wr_batch_job> trial(jobs, jobs + n_jobs);

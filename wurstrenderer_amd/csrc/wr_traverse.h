@@ -47,6 +47,9 @@ __device__ __forceinline__ RaySetup setup_ray(float3 o, float3 d, float tmin) {
 __device__ __forceinline__ double pickd(double x, double y, double z, int k) { return k == 0 ? x : (k == 1 ? y : z); }
 __device__ __forceinline__ bool tri_test_fp64(const RaySetup &r, float3 a, float3 b, float3 c, double tmin, double tmax,
                                               double &t_out) {
+    // no fused multiply-add: an edge shared by two triangles must give them
+    // exactly opposite edge values (the watertightness argument of Woop et al.)
+#pragma clang fp contract(off)
     const double ox = r.o.x, oy = r.o.y, oz = r.o.z;
     const double ax = a.x - ox, ay = a.y - oy, az = a.z - oz;
     const double bx = b.x - ox, by = b.y - oy, bz = b.z - oz;
@@ -75,9 +78,18 @@ __device__ __forceinline__ bool tri_test(const RaySetup &r, float4 va, float4 vb
     const float Ax = fmaf(-r.Sx, Akz, pick(A, r.kx)), Ay = fmaf(-r.Sy, Akz, pick(A, r.ky));
     const float Bx = fmaf(-r.Sx, Bkz, pick(B, r.kx)), By = fmaf(-r.Sy, Bkz, pick(B, r.ky));
     const float Cx = fmaf(-r.Sx, Ckz, pick(C, r.kx)), Cy = fmaf(-r.Sy, Ckz, pick(C, r.ky));
-    float U = Cx * By - Cy * Bx;
-    float V = Ax * Cy - Ay * Cx;
-    float W = Bx * Ay - By * Ax;
+    float U, V, W;
+    {
+        // The per-vertex projections above may fuse (a vertex then projects to
+        // the same value in every triangle that holds it), but the edge
+        // functions must not: Cx*By - Cy*Bx with one product fused is not the
+        // exact negation of the same edge's value in the neighbouring triangle,
+        // Bx*Cy - By*Cx, and a ray through the shared edge could miss both.
+#pragma clang fp contract(off)
+        U = Cx * By - Cy * Bx;
+        V = Ax * Cy - Ay * Cx;
+        W = Bx * Ay - By * Ax;
+    }
     if (U == 0.f || V == 0.f || W == 0.f) {
         double t;
         if (!tri_test_fp64(r, xyz(va), xyz(vb), xyz(vc), r.tmin, tmax, t)) return false;
@@ -93,6 +105,13 @@ __device__ __forceinline__ bool tri_test(const RaySetup &r, float4 va, float4 vb
     t_out = t;
     return true;
 }
+
+// Far limit against which a closest-hit traversal culls boxes and stack
+// entries: the best t so far widened by 2^-21 (4 ulp).  A triangle that ties
+// with the best hit in t may sit in a box whose FP32 entry distance rounds a
+// few ulp above that t; culling such a box would make the equal-t rule (lower
+// leaf-order index wins) depend on the order in which leaves are reached.
+__device__ __forceinline__ float cull_limit(float t) { return fmaf(fabsf(t), 4.76837158e-7f, t); }
 
 // Per-lane traversal stack of (node ref, entry distance).  The first
 // kLdsStackDepth entries live in LDS at [k * kBlock + lane] (bank-conflict
@@ -182,9 +201,10 @@ __device__ __forceinline__ bool trav_step(const G &s, TravState &ts, LaneStack<k
         tf[3] = fminf(fminf((fx_.w - ox) * r.ix, (fy_.w - oy) * r.iy), (fz_.w - oz) * r.iz);
         uint32_t c[4] = {ch.x, ch.y, ch.z, ch.w};
         float key[4];
+        const float tc = ANY ? ts.t : cull_limit(ts.t);
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            const bool h = c[i] != kEmptyChild && tn[i] <= fminf(tf[i] * 1.00000048f, ts.t);
+            const bool h = c[i] != kEmptyChild && tn[i] <= fminf(tf[i] * 1.00000048f, tc);
             key[i] = h ? tn[i] : INFINITY;
         }
         if (!ANY) {
@@ -242,7 +262,7 @@ __device__ __forceinline__ bool trav_step(const G &s, TravState &ts, LaneStack<k
         uint32_t ref;
         float tn;
         st.top(ref, tn);
-        if (tn <= ts.t) { ts.cur = ref; return false; }
+        if (tn <= cull_limit(ts.t)) { ts.cur = ref; return false; }
     }
 }
 
@@ -295,9 +315,10 @@ __device__ __forceinline__ bool trav_round(const G &s, TravState &ts, LaneStack<
             tf[3] = fminf(fminf((fx_.w - ox) * r.ix, (fy_.w - oy) * r.iy), (fz_.w - oz) * r.iz);
             uint32_t c[4] = {ch.x, ch.y, ch.z, ch.w};
             float key[4];
+            const float tc = cull_limit(ts.t);
 #pragma unroll
             for (int i = 0; i < 4; i++) {
-                const bool h = c[i] != kEmptyChild && tn[i] <= fminf(tf[i] * 1.00000048f, ts.t);
+                const bool h = c[i] != kEmptyChild && tn[i] <= fminf(tf[i] * 1.00000048f, tc);
                 key[i] = h ? tn[i] : INFINITY;
             }
 #define WR_CSWAP(a, b)                                                          \
@@ -327,7 +348,7 @@ __device__ __forceinline__ bool trav_round(const G &s, TravState &ts, LaneStack<
                 uint32_t ref;
                 float tn;
                 st.top(ref, tn);
-                if (tn <= ts.t) {
+                if (tn <= cull_limit(ts.t)) {
                     ts.cur = ref;
                     more = true;
                     break;

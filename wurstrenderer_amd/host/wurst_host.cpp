@@ -170,13 +170,25 @@ void LoadObj(const std::string &path, bool srgb, SceneData *scene) {
             load_mtl(dir_of(path) + "/" + trim(f), &mtl);
         }
     }
-    // materials -> BSDFs / emitters (assimpio.h:79-121)
-    for (const Group &g : groups) {
-        if (g.faces.empty()) continue;
+    // materials -> BSDFs / emitters (assimpio.h:79-121).  Geometry ids follow
+    // the reference's attach order: the OBJ's surfaces first, then its emitters
+    // (AssimpInOut::LoadFromObj splits them, assimpio.h:123-150, and
+    // Simple2Parser::ParseScene attaches mGeometries before mLightGeometries,
+    // simple2parser.h:38-61), each in file order.
+    auto is_emissive = [&](const Group &g) {
         auto it = mtl.find(g.material);
         if (it == mtl.end()) throw std::runtime_error("obj: unknown material \"" + g.material + "\"");
         const MtlEntry &m = it->second;
-        const bool emissive = m.Ke[0] != 0.0 || m.Ke[1] != 0.0 || m.Ke[2] != 0.0;
+        return m.Ke[0] != 0.0 || m.Ke[1] != 0.0 || m.Ke[2] != 0.0;
+    };
+    std::vector<const Group *> order;
+    for (int pass = 0; pass < 2; pass++)
+        for (const Group &g : groups)
+            if (!g.faces.empty() && is_emissive(g) == (pass == 1)) order.push_back(&g);
+    for (const Group *gp : order) {
+        const Group &g = *gp;
+        const MtlEntry &m = mtl.find(g.material)->second;
+        const bool emissive = is_emissive(g);
         const uint32_t geom = uint32_t(scene->geom_material.size());
         if (emissive) {
             if (m.Ns != 0.0) throw std::runtime_error("obj: emitter with a Phong exponent (cone-emission area light) is not supported");
