@@ -47,9 +47,6 @@ __device__ __forceinline__ RaySetup setup_ray(float3 o, float3 d, float tmin) {
 __device__ __forceinline__ double pickd(double x, double y, double z, int k) { return k == 0 ? x : (k == 1 ? y : z); }
 __device__ __forceinline__ bool tri_test_fp64(const RaySetup &r, float3 a, float3 b, float3 c, double tmin, double tmax,
                                               double &t_out) {
-    // no fused multiply-add: an edge shared by two triangles must give them
-    // exactly opposite edge values (the watertightness argument of Woop et al.)
-#pragma clang fp contract(off)
     const double ox = r.o.x, oy = r.o.y, oz = r.o.z;
     const double ax = a.x - ox, ay = a.y - oy, az = a.z - oz;
     const double bx = b.x - ox, by = b.y - oy, bz = b.z - oz;
@@ -61,7 +58,15 @@ __device__ __forceinline__ bool tri_test_fp64(const RaySetup &r, float3 a, float
     const double Ax = pickd(ax, ay, az, r.kx) - Sx * Akz, Ay = pickd(ax, ay, az, r.ky) - Sy * Akz;
     const double Bx = pickd(bx, by, bz, r.kx) - Sx * Bkz, By = pickd(bx, by, bz, r.ky) - Sy * Bkz;
     const double Cx = pickd(cx, cy, cz, r.kx) - Sx * Ckz, Cy = pickd(cx, cy, cz, r.ky) - Sy * Ckz;
-    const double U = Cx * By - Cy * Bx, V = Ax * Cy - Ay * Cx, W = Bx * Ay - By * Ax;
+    double U, V, W;
+    {
+        // unfused, as in tri_test: a shared edge gives its two triangles
+        // exactly opposite values (the watertightness argument of Woop et al.)
+#pragma clang fp contract(off)
+        U = Cx * By - Cy * Bx;
+        V = Ax * Cy - Ay * Cx;
+        W = Bx * Ay - By * Ax;
+    }
     if ((U < 0 || V < 0 || W < 0) && (U > 0 || V > 0 || W > 0)) return false;
     const double det = U + V + W;
     if (det == 0.0) return false;
