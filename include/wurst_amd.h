@@ -237,6 +237,23 @@ int wr_render_params_validate(const wr_render_params *params);
  * of the NEE + MIS estimator into `film`.  Blocking. */
 int wr_render_nee(wr_scene *scene, wr_film *film, const wr_render_params *params, wr_render_stats *stats);
 
+/* wr_render_nee without any device allocation.  No counterpart in the
+ * reference; it is for callers that cannot allocate device memory during the
+ * call (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The film and scene are the caller's; the path pool,
+ * per-sample radiance, counters and traversal spill live in a fixed 77 MiB
+ * region that the library reserves on each GPU it runs on (held until the
+ * process ends), so the call makes no hipMalloc/hipFree, creates no events and
+ * instantiates no graph.  Limits: a frame of at most 2^20 pixels (passes of at
+ * most 2^20 samples), at most 2^17 paths in flight (max_paths_in_flight 0
+ * selects 2^17), profile = 0 and use_graph = 0; anything beyond is rejected
+ * with WR_ERR_UNSUPPORTED before any launch.  The film receives what
+ * wr_render_nee adds (same samples; the FP64 per-pixel sums are identical
+ * when both render the call's spp in one pass, e.g. W * H * spp <= 2^20).
+ * All calls on one GPU share the region and leave their contents in it: calls
+ * that overlap on one GPU (from different scene streams or threads) race. */
+int wr_render_nee_noalloc(wr_scene *scene, wr_film *film, const wr_render_params *params, wr_render_stats *stats);
+
 /* PrimitivePathRenderer::render (primitivepath.h:42-96; JSON "integrator":
  * "primitive_path_tracer", simple2parser.h:184-187): spp samples per pixel of
  * BSDF-sampled camera paths of up to num_path_vertices vertices that add the

@@ -136,6 +136,7 @@ EXPORTS = {
     "wr_film_read": (C.c_int, [P, P, P]),
     "wr_render_params_validate": (C.c_int, [C.POINTER(RenderParams)]),
     "wr_render_nee": (C.c_int, [P, P, C.POINTER(RenderParams), C.POINTER(RenderStats)]),
+    "wr_render_nee_noalloc": (C.c_int, [P, P, C.POINTER(RenderParams), C.POINTER(RenderStats)]),
     "wr_render_primitive": (C.c_int, [P, P, C.POINTER(RenderParams), C.POINTER(RenderStats)]),
     "wr_render_visualize": (C.c_int, [P, P, C.POINTER(RenderParams), C.POINTER(RenderStats)]),
     "wr_render_light": (C.c_int, [P, P, C.POINTER(RenderParams), C.POINTER(RenderStats)]),

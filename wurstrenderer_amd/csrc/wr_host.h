@@ -98,6 +98,7 @@ struct wr_device {
     // resident blocks per CU of each kernel (HIP occupancy API), used to size persistent grids
     int occ_extend = 8, occ_shadow = 8, occ_shade = 8, occ_generate = 8, occ_trace = 8;
     std::map<const void *, int> occ_cache;  // other kernels, filled on first use
+    void *scratch = nullptr;  // device address of wr_render_nee_noalloc's fixed region on this GPU
 };
 
 struct wr_scene {

@@ -1492,6 +1492,8 @@ static int occupancy(wr_device *d, const void *fn, int block) {
 const char *wr_last_error(void) { return wr::error_string().c_str(); }
 const char *wr_version(void) { return "wurst_amd 0.1 (gfx950, BVH4-128B, wavefront NEE+MIS)"; }
 
+static int scratch_setup(wr_device *d);
+
 int wr_device_create(int ordinal, wr_device **out) {
     if (!out) return set_error(WR_ERR_INVALID_ARGUMENT, "out is null");
     *out = nullptr;
@@ -1515,6 +1517,10 @@ int wr_device_create(int ordinal, wr_device **out) {
 
     occ(reinterpret_cast<const void *>(&k_regen<false>), kShadeBlock, d->occ_generate);
     occ(reinterpret_cast<const void *>(&k_trace<false>), kBlock, d->occ_trace);
+    if (int rc = scratch_setup(d)) {
+        delete d;
+        return rc;
+    }
     *out = d;
     return WR_OK;
 }
@@ -2059,6 +2065,37 @@ int wr_film_read(wr_film *f, double *base, double *splat) {
 // Path pool: two queue-ordered states of 16*5 bytes per path (ray_o, ray_d,
 // thr, rad, meta), the hit records (8 B), two shadow entries (2 * 48 B), and
 // the per-sample radiance of a pass (16 B per sample).
+// Byte layout of a path pool of `cap` paths and `samples` per-sample slots
+// (every array 256-byte aligned): the state pair, hits, per-sample radiance,
+// shadow queue (2 entries per path), counters (one 256-byte line each), stats.
+constexpr size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+constexpr size_t pool_bytes(size_t cap, size_t samples) {
+    return 10 * align256(16 * cap) + align256(8 * cap) + align256(16 * samples) + 3 * align256(32 * cap) +
+           align256(4 * kCounterStride * kNumCounterLines) + align256(64);
+}
+// Points pb's arrays into the pool at `base`; returns the bytes used (pool_bytes).
+static size_t pool_layout(char *base, uint32_t cap, uint32_t samples, PathBuffers *pb) {
+    const size_t c = cap;
+    size_t off = 0;
+    auto take = [&](size_t bytes) { char *q = base + off; off += align256(bytes); return q; };
+    PathState *sts[2] = {&pb->cur, &pb->next};
+    for (PathState *st : sts) {
+        st->ray_o = reinterpret_cast<float4 *>(take(16 * c));
+        st->ray_d = reinterpret_cast<float4 *>(take(16 * c));
+        st->thr = reinterpret_cast<float4 *>(take(16 * c));
+        st->rad = reinterpret_cast<float4 *>(take(16 * c));
+        st->meta = reinterpret_cast<uint4 *>(take(16 * c));
+    }
+    pb->hit = reinterpret_cast<uint2 *>(take(8 * c));
+    pb->sample_rad = reinterpret_cast<float4 *>(take(16 * size_t(samples)));
+    pb->sh_o = reinterpret_cast<float4 *>(take(32 * c));
+    pb->sh_d = reinterpret_cast<float4 *>(take(32 * c));
+    pb->sh_c = reinterpret_cast<float4 *>(take(32 * c));
+    pb->counters = reinterpret_cast<uint32_t *>(take(4 * kCounterStride * kNumCounterLines));
+    pb->stats = reinterpret_cast<unsigned long long *>(take(64));
+    return off;
+}
+
 static int ensure_capacity(wr_scene *s, uint32_t cap, uint32_t samples) {
     if (s->capacity >= cap && s->sample_capacity >= samples) return WR_OK;
     cap = std::max(cap, s->capacity);
@@ -2070,40 +2107,10 @@ static int ensure_capacity(wr_scene *s, uint32_t cap, uint32_t samples) {
         s->capacity = 0;
         s->sample_capacity = 0;
     }
-    auto al = [](size_t b) { return (b + 255) / 256 * 256; };
-    const size_t c = cap;
-    std::vector<size_t> sizes;
-    for (int k = 0; k < 2; k++)
-        for (int j = 0; j < 5; j++) sizes.push_back(al(16 * c));
-    sizes.push_back(al(8 * c));              // hit
-    sizes.push_back(al(16 * size_t(samples)));  // per-sample radiance
-    for (int j = 0; j < 3; j++) sizes.push_back(al(32 * c));  // shadow o, d, contribution (2 entries per path)
-    sizes.push_back(al(4 * kCounterStride * kNumCounterLines));  // counters, one line each
-    sizes.push_back(al(64));                 // stats
-    size_t total = 0;
-    for (size_t b : sizes) total += b;
     char *p = nullptr;
-    WR_HIP(hipMalloc(reinterpret_cast<void **>(&p), total));
+    WR_HIP(hipMalloc(reinterpret_cast<void **>(&p), pool_bytes(cap, samples)));
     s->pool = p;
-    PathBuffers &pb = s->pb;
-    size_t off = 0;
-    int idx = 0;
-    auto take = [&]() { char *q = p + off; off += sizes[idx++]; return q; };
-    PathState *sts[2] = {&pb.cur, &pb.next};
-    for (PathState *st : sts) {
-        st->ray_o = reinterpret_cast<float4 *>(take());
-        st->ray_d = reinterpret_cast<float4 *>(take());
-        st->thr = reinterpret_cast<float4 *>(take());
-        st->rad = reinterpret_cast<float4 *>(take());
-        st->meta = reinterpret_cast<uint4 *>(take());
-    }
-    pb.hit = reinterpret_cast<uint2 *>(take());
-    pb.sample_rad = reinterpret_cast<float4 *>(take());
-    pb.sh_o = reinterpret_cast<float4 *>(take());
-    pb.sh_d = reinterpret_cast<float4 *>(take());
-    pb.sh_c = reinterpret_cast<float4 *>(take());
-    pb.counters = reinterpret_cast<uint32_t *>(take());
-    pb.stats = reinterpret_cast<unsigned long long *>(take());
+    pool_layout(p, cap, samples, &s->pb);
     s->capacity = cap;
     s->sample_capacity = samples;
     if (!s->h_counters)
@@ -2133,13 +2140,14 @@ constexpr uint64_t kMaxSamplesPerPass = uint64_t(1) << 27;
 // kernel reads its work counts from device memory, which is what makes the
 // replays valid; nothing in an iteration is a memset.
 template <class Launch>
-static int drain(wr_scene *s, PathBuffers &pb, uint64_t max_iters, bool use_graph, Launch &&launch) {
+static int drain(wr_scene *s, PathBuffers &pb, uint64_t max_iters, bool use_graph, uint32_t *host_counters,
+                 Launch &&launch) {
     constexpr int kPoll = 4;
     auto poll = [&](bool &done) -> int {
-        WR_HIP(hipMemcpyAsync(s->h_counters, pb.counters, sizeof(uint32_t) * kCounterStride * kNumCounterLines,
+        WR_HIP(hipMemcpyAsync(host_counters, pb.counters, sizeof(uint32_t) * kCounterStride * kNumCounterLines,
                               hipMemcpyDeviceToHost, s->stream));
         WR_HIP(hipStreamSynchronize(s->stream));
-        done = s->h_counters[kQ] == 0 && s->h_counters[kFresh] == 0;
+        done = host_counters[kQ] == 0 && host_counters[kFresh] == 0;
         return WR_OK;
     };
     if (!use_graph) {
@@ -2207,10 +2215,19 @@ int wr_render_params_validate(const wr_render_params *p) {
 
 // Estimators of the frame loop.
 enum Estimator { kEstNee, kEstPrimitive, kEstLight };
+// A path pool and traversal spill region that the caller already holds (the
+// allocation-free entry point's scratch): render_impl then allocates nothing.
+struct FixedPool {
+    PathBuffers pb;
+    uint32_t capacity, sample_capacity;  // paths in flight, samples per pass
+    uint2 *spill;                        // kSpillDepth entries per lane of trav_blocks blocks
+    int trav_blocks;
+};
 // The frame loop of wr_render_nee, wr_render_guided (g non-null),
-// wr_render_primitive and wr_render_light.
+// wr_render_primitive, wr_render_light and wr_render_nee_noalloc (fixed).
 static int render_impl(wr_scene *s, wr_film *f, const wr_render_params *prm, wr_guide *g, const wr_guided_params *gp,
-                       wr_render_stats *stats, wr_guide_stats *gstats, Estimator est = kEstNee) {
+                       wr_render_stats *stats, wr_guide_stats *gstats, Estimator est = kEstNee,
+                       const FixedPool *fixed = nullptr) {
     if (!s || !f || !prm) return set_error(WR_ERR_INVALID_ARGUMENT, "null argument");
     if (f->scene != s) return set_error(WR_ERR_INVALID_ARGUMENT, "film belongs to another scene");
     if (int rc0 = wr_render_params_validate(prm)) return rc0;
@@ -2235,24 +2252,45 @@ static int render_impl(wr_scene *s, wr_film *f, const wr_render_params *prm, wr_
         if (gstats) *gstats = gst;
         return WR_OK;
     }
-    uint64_t spp_cap = std::min<uint64_t>(kMaxSamplesPerPass / wh, uint64_t(prm->spp));
+    if (fixed) {
+        // the pool is the caller's fixed region: nothing may grow, and nothing
+        // that allocates (events for profiling, graph instantiation) may run
+        if (prm->profile) return set_error(WR_ERR_UNSUPPORTED, "profile = 1 needs HIP events; the allocation-free entry point creates none");
+        if (prm->use_graph) return set_error(WR_ERR_UNSUPPORTED, "use_graph = 1 instantiates a graph, which allocates");
+        if (wh > fixed->sample_capacity)
+            return set_error(WR_ERR_UNSUPPORTED, "frame of " + std::to_string(wh) + " pixels exceeds the allocation-free "
+                             "pass capacity of " + std::to_string(fixed->sample_capacity) + " samples");
+        if (uint64_t(prm->max_paths_in_flight) > fixed->capacity)
+            return set_error(WR_ERR_UNSUPPORTED, "max_paths_in_flight exceeds the allocation-free pool of " +
+                             std::to_string(fixed->capacity) + " paths");
+    }
+    uint64_t spp_cap = std::min<uint64_t>((fixed ? fixed->sample_capacity : kMaxSamplesPerPass) / wh, uint64_t(prm->spp));
     if (g && gp->spp_per_pass > 0) spp_cap = std::min<uint64_t>(spp_cap, gp->spp_per_pass);
     const uint32_t spp_pass = uint32_t(std::max<uint64_t>(1, spp_cap));
     const uint32_t samples_pass = spp_pass * wh;
     // default pool: 32M paths (about 5.6 GB of state): large launches hide the
     // traversal latency and shrink the per-launch tail
     uint64_t pool = prm->max_paths_in_flight > 0 ? uint64_t(prm->max_paths_in_flight) : (uint64_t(1) << 25);
+    if (fixed) pool = std::min<uint64_t>(pool, fixed->capacity);
     pool = std::min<uint64_t>(pool, samples_pass);
     if (pool > (uint64_t(1) << 30)) return set_error(WR_ERR_UNSUPPORTED, "too many paths in flight");
     const uint32_t P = uint32_t(pool);
-    rc = ensure_capacity(s, P, samples_pass);
-    if (rc) return rc;
+    uint32_t fixed_counters[kCounterStride * kNumCounterLines];  // host copy of the counters (fixed pool)
+    uint32_t *host_counters = fixed_counters;
+    uint2 *spill = s->d_spill;
+    if (fixed) {
+        spill = fixed->spill;
+    } else {
+        rc = ensure_capacity(s, P, samples_pass);
+        if (rc) return rc;
+        host_counters = s->h_counters;
+    }
     if (g) {
         rc = guide_reserve_draws(g, P);
         if (!rc) rc = guide_render_begin(g);
         if (rc) return rc;
     }
-    PathBuffers pb = s->pb;
+    PathBuffers pb = fixed ? fixed->pb : s->pb;
     pb.film_base = f->d_base;
     pb.film_splat = f->d_splat;
     k_fill_u32<<<1, 64, 0, s->stream>>>(reinterpret_cast<uint32_t *>(pb.stats), 0u, 12);
@@ -2299,8 +2337,12 @@ static int render_impl(wr_scene *s, wr_film *f, const wr_render_params *prm, wr_
     const int g_infer = std::max(1, std::min<int>((P + kInferBlock - 1) / kInferBlock,
                                                   s->dev->n_cu * occupancy(s->dev, infer_fn, kInferBlock)));
     guide::Net net{};
-    const int g_ext = trav_grid(s, P, s->dev->occ_extend);
-    const int g_shd = trav_grid(s, 2 * P, s->dev->occ_shadow);
+    int g_ext = trav_grid(s, P, s->dev->occ_extend);
+    int g_shd = trav_grid(s, 2 * P, s->dev->occ_shadow);
+    if (fixed) {  // the fixed spill region covers trav_blocks blocks
+        g_ext = std::min(g_ext, fixed->trav_blocks);
+        g_shd = std::min(g_shd, fixed->trav_blocks);
+    }
     for (uint32_t done = 0; done < uint32_t(prm->spp); done += spp_pass) {
         const uint32_t k = std::min<uint32_t>(spp_pass, uint32_t(prm->spp) - done);
         RenderArgs a;
@@ -2338,8 +2380,8 @@ static int render_impl(wr_scene *s, wr_film *f, const wr_render_params *prm, wr_
             else k_regen<false><<<g_regen, kShadeBlock, 0, s->stream>>>(ds, nullptr, 0, q, a);
             end();
             begin(0);
-            if (count) k_extend<true, false><<<g_ext, kBlock, 0, s->stream>>>(ds, nullptr, q, s->d_spill, tune);
-            else k_extend_fast<<<g_ext, kBlock, 0, s->stream>>>(ds, q, s->d_spill, tune);
+            if (count) k_extend<true, false><<<g_ext, kBlock, 0, s->stream>>>(ds, nullptr, q, spill, tune);
+            else k_extend_fast<<<g_ext, kBlock, 0, s->stream>>>(ds, q, spill, tune);
             end();
             begin(2);
             if (g && gs.alpha > 0.f) {
@@ -2364,12 +2406,12 @@ static int render_impl(wr_scene *s, wr_film *f, const wr_render_params *prm, wr_
             else k_shade<false><<<g_shade, kShadeBlock, 0, s->stream>>>(ds, nullptr, q, a);
             end();
             begin(1);
-            if (count) k_shadow<true, false><<<g_shd, kBlock, 0, s->stream>>>(ds, nullptr, q, s->d_spill, tune);
-            else k_shadow<false, false><<<g_shd, kBlock, 0, s->stream>>>(ds, nullptr, q, s->d_spill, tune);
+            if (count) k_shadow<true, false><<<g_shd, kBlock, 0, s->stream>>>(ds, nullptr, q, spill, tune);
+            else k_shadow<false, false><<<g_shd, kBlock, 0, s->stream>>>(ds, nullptr, q, spill, tune);
             end();
             k_rotate<<<1, 1, 0, s->stream>>>(q, a);
         };
-        rc = drain(s, pb, max_iters, use_graph, iteration);
+        rc = drain(s, pb, max_iters, use_graph, host_counters, iteration);
         if (rc) return rc;
         begin(4);
         k_accumulate<<<std::max<uint32_t>(1, std::min<uint32_t>((wh + 255) / 256, uint32_t(s->dev->n_cu) * 8)), 256, 0,
@@ -2418,6 +2460,46 @@ static int render_impl(wr_scene *s, wr_film *f, const wr_render_params *prm, wr_
 
 int wr_render_nee(wr_scene *s, wr_film *f, const wr_render_params *prm, wr_render_stats *stats) {
     return render_impl(s, f, prm, nullptr, nullptr, stats, nullptr);
+}
+
+// ============================================================== allocation-free NEE + MIS
+// wr_render_nee_noalloc renders through a fixed region of device memory that
+// the library reserves once per GPU instead of the scene's lazily grown path
+// pool: a pool of 2^17 paths with 2^20 per-sample slots (passes of at most
+// 2^20 samples) and the traversal spill of 1024 blocks -- 77 MiB, held by each
+// GPU the library runs on until the process ends.  The region is a file-scope
+// device array; its device address on each GPU comes from hipGetSymbolAddress
+// at wr_device_create (the array's name in host code is not a device address).
+constexpr uint32_t kFixedPaths = 1u << 17;
+constexpr uint32_t kFixedSamples = 1u << 20;
+constexpr int kFixedTravBlocks = 1024;
+constexpr size_t kFixedPoolBytes = pool_bytes(kFixedPaths, kFixedSamples);
+constexpr size_t kFixedSpillBytes = sizeof(uint2) * size_t(kFixedTravBlocks) * kBlock * kSpillDepth;
+constexpr size_t kScratchBytes = kFixedPoolBytes + kFixedSpillBytes;
+static_assert(kFixedPoolBytes % 256 == 0, "the spill region starts 256-byte aligned");
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+// Device address of `scratch` on the device current in wr_device_create.
+static int scratch_setup(wr_device *d) {
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        return set_error(WR_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    d->scratch = ptr;
+    return WR_OK;
+}
+
+int wr_render_nee_noalloc(wr_scene *s, wr_film *f, const wr_render_params *prm, wr_render_stats *stats) {
+    if (!s || !f || !prm) return set_error(WR_ERR_INVALID_ARGUMENT, "null argument");
+    if (!s->dev->scratch) return set_error(WR_ERR_DEVICE, "the device has no scratch region");
+    char *base = static_cast<char *>(s->dev->scratch);
+    FixedPool fp{};
+    pool_layout(base, kFixedPaths, kFixedSamples, &fp.pb);
+    fp.capacity = kFixedPaths;
+    fp.sample_capacity = kFixedSamples;
+    fp.spill = reinterpret_cast<uint2 *>(base + kFixedPoolBytes);
+    fp.trav_blocks = kFixedTravBlocks;
+    return render_impl(s, f, prm, nullptr, nullptr, stats, nullptr, kEstNee, &fp);
 }
 
 int wr_render_guided(wr_scene *s, wr_film *f, wr_guide *g, const wr_render_params *prm, const wr_guided_params *gp,
@@ -2666,7 +2748,7 @@ int wr_render_batch(const wr_batch_job *jobs, uint32_t n_jobs, const wr_render_p
             end();
             k_rotate<<<1, 1, 0, s0->stream>>>(q, a);
         };
-        rc = drain(s0, pb, max_iters, use_graph, iteration);
+        rc = drain(s0, pb, max_iters, use_graph, s0->h_counters, iteration);
         if (rc) return rc;
         begin(4);
         for (uint32_t t = 0; t < nt; t++) {
