@@ -95,13 +95,13 @@ typedef struct {
     uint32_t seed;
     int32_t use_mis;             /* NEE_USE_MIS (pathnextevent.h:13); 1 = on */
     int32_t first_sample;        /* index of the first sample (progressive calls) */
-    int32_t max_paths_in_flight; /* 0 = library default */
+    int32_t max_paths_in_flight; /* 0 = library default (2^27, capped by the samples of a pass) */
     int32_t profile;             /* 1: time each kernel class with HIP events */
     int32_t count_traversal;     /* 1: count visited nodes / tested triangles */
     /* GPU-only scheduling (no reference counterpart; the CPU path has none):
      * 0 = library default; wr_autotune picks them per workload */
     int32_t use_graph;           /* 1: replay the per-iteration launches as a captured hipGraph */
-    uint32_t fetch_chunk;        /* traversal work items claimed per atomic (default 128, max 4096) */
+    uint32_t fetch_chunk;        /* traversal work items claimed per atomic (default 256, max 4096) */
     uint32_t refill_lanes;       /* idle lanes that trigger a wave's refill (default 24, max 64) */
 } wr_render_params;
 

@@ -315,13 +315,15 @@ __global__ __launch_bounds__(kShadeBlock) void k_regen(DScene s1, const BatchJob
 // and refills its lanes from that local range as soon as `refill` of them are
 // idle, so a wave never waits for its slowest ray and atomics stay rare.
 // Waves beyond the work exit at once.  Both are runtime parameters
-// (TravTune); 128 / 24 were the fastest of a sweep over 64..1024 and 16..64 on
-// the 1.31M-triangle scene, and wr_autotune searches them per workload.
+// (TravTune).  On the 1.31M-triangle frame with the default pool, chunks of
+// 128 / 256 / 512 with refill at 24 idle lanes gave 5,382 / 5,442 / 5,042
+// Mrays/s (64 is far slower: one counter serialises the atomics), refill 8..32
+// within 2%; wr_autotune searches them per workload.
 #ifndef WR_REFILL_LANES
 #define WR_REFILL_LANES 24
 #endif
 #ifndef WR_FETCH_CHUNK
-#define WR_FETCH_CHUNK 128
+#define WR_FETCH_CHUNK 256
 #endif
 struct TravTune {
     uint32_t chunk = WR_FETCH_CHUNK;
@@ -333,13 +335,13 @@ struct TravTune {
 #else
 #define WR_TRAV_ATTR
 #endif
-// The plain closest-hit kernel (k_extend_fast) fits 72 VGPRs without spills,
-// which admits 7 waves per SIMD instead of 6 (3% faster extension on the
-// 1.31M-triangle frame); the counting and batched kernels would spill at that
-// size and keep the compiler's choice.  The plain any-hit kernel (k_shadow)
-// gets 70 VGPRs and 7 waves by itself: its stack holds node refs only.
+// The plain closest-hit kernel (k_extend_fast) fits 64 VGPRs without spills,
+// which admits 8 waves per SIMD (7 and 8 measured within 0.2% on the
+// 1.31M-triangle frame); the counting and batched kernels keep the compiler's
+// choice.  The plain any-hit kernel (k_shadow) gets 60 VGPRs and 8 waves by
+// itself: its stack holds node refs only.
 #ifndef WR_EXTEND_WAVES
-#define WR_EXTEND_WAVES 7
+#define WR_EXTEND_WAVES 8
 #endif
 #define WR_EXTEND_ATTR __attribute__((amdgpu_waves_per_eu(WR_EXTEND_WAVES, WR_EXTEND_WAVES)))
 
@@ -2131,6 +2133,8 @@ static int trav_grid(const wr_scene *s, uint32_t n, int per_cu) {
 // Samples of one pass: bounds the per-sample radiance buffer to 2 GiB, enough
 // for a 1920x1080 frame at 64 spp, so such a frame drains the pool once.
 constexpr uint64_t kMaxSamplesPerPass = uint64_t(1) << 27;
+// Paths in flight when the caller leaves max_paths_in_flight at 0.
+constexpr uint64_t kDefaultPool = uint64_t(1) << 27;
 
 // Runs wavefront iterations until the pool drains.  launch(pb) issues one
 // iteration (refill, extend, shade, shadow, rotate) for the state pb; the host
@@ -2268,9 +2272,11 @@ static int render_impl(wr_scene *s, wr_film *f, const wr_render_params *prm, wr_
     if (g && gp->spp_per_pass > 0) spp_cap = std::min<uint64_t>(spp_cap, gp->spp_per_pass);
     const uint32_t spp_pass = uint32_t(std::max<uint64_t>(1, spp_cap));
     const uint32_t samples_pass = spp_pass * wh;
-    // default pool: 32M paths (about 5.6 GB of state): large launches hide the
-    // traversal latency and shrink the per-launch tail
-    uint64_t pool = prm->max_paths_in_flight > 0 ? uint64_t(prm->max_paths_in_flight) : (uint64_t(1) << 25);
+    // default pool: 2^27 paths, a whole 1080p 64-spp pass (about 35 GB of state
+    // at full size, capped by the pass): large launches hide the traversal
+    // latency and shrink the per-launch tail (2^25 / 2^26 / 2^27 paths:
+    // 5,090 / 5,308 / 5,442 Mrays/s on the 1.31M-triangle frame)
+    uint64_t pool = prm->max_paths_in_flight > 0 ? uint64_t(prm->max_paths_in_flight) : kDefaultPool;
     if (fixed) pool = std::min<uint64_t>(pool, fixed->capacity);
     pool = std::min<uint64_t>(pool, samples_pass);
     if (pool > (uint64_t(1) << 30)) return set_error(WR_ERR_UNSUPPORTED, "too many paths in flight");
@@ -2645,7 +2651,7 @@ int wr_render_batch(const wr_batch_job *jobs, uint32_t n_jobs, const wr_render_p
         return WR_OK;
     }
     max_pass = std::min<uint64_t>(max_pass, kMaxSamplesPerPass);
-    uint64_t pool = prm->max_paths_in_flight > 0 ? uint64_t(prm->max_paths_in_flight) : (uint64_t(1) << 25);
+    uint64_t pool = prm->max_paths_in_flight > 0 ? uint64_t(prm->max_paths_in_flight) : kDefaultPool;
     pool = std::min<uint64_t>(pool, max_pass);
     if (pool > (uint64_t(1) << 30)) return set_error(WR_ERR_UNSUPPORTED, "too many paths in flight");
     const uint32_t P = uint32_t(pool);
@@ -2807,8 +2813,8 @@ int wr_autotune(const wr_batch_job *jobs, uint32_t n_jobs, const wr_render_param
     }
     struct Candidate { uint32_t pool, chunk, refill; int graph; };
     std::vector<Candidate> cands;
-    for (uint32_t pool : {1u << 22, 1u << 23, 1u << 24, 1u << 25})
-        for (auto ck : {std::make_pair(64u, 16u), std::make_pair(128u, 24u), std::make_pair(256u, 24u), std::make_pair(128u, 40u)})
+    for (uint32_t pool : {1u << 23, 1u << 24, 1u << 25, 1u << 26, 1u << 27})
+        for (auto ck : {std::make_pair(128u, 24u), std::make_pair(256u, 24u), std::make_pair(256u, 32u), std::make_pair(128u, 40u)})
             for (int graph : {0, 1}) cands.push_back({pool, ck.first, ck.second, graph});
     double best_rate = -1.0;
     wr_render_params bp = *base;

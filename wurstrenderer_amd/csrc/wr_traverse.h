@@ -1,6 +1,6 @@
 // wr_traverse.h -- BVH4 traversal with a per-lane LDS stack and the
-// watertight ray/triangle test (Woop, Benthin, Wald, JCGT 2013), FP32 with an
-// FP64 fallback on edge hits.
+// watertight ray/triangle test (Woop, Benthin, Wald, JCGT 2013), FP32 with the
+// paper's FP64 recomputation of the edge functions on edge hits.
 //
 // Semantics follow rtcIntersect1 / rtcOccluded1 as the reference calls them
 // (src/accel/embreeaccel.h:336-422): double-sided triangles, hits accepted
@@ -15,65 +15,40 @@ namespace wr {
 constexpr uint32_t kInvalid = 0xFFFFFFFFu;
 
 struct RaySetup {
-    float3 o, d;
+    float3 o;
     float tmin;
     float ix, iy, iz;          // 1/d with zero components replaced by +-1e-20
     int kx, ky, kz;            // watertight axis permutation
     float Sx, Sy, Sz;
-    bool nx, ny, nz;           // direction sign per axis (picks near/far slab)
+    uint32_t onx, ony, onz;    // byte offsets of the near-plane rows in a Node4 (the far row is at offset ^ 16)
 };
 
 __device__ __forceinline__ float pick(float3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
 
+// Reciprocals use the hardware v_rcp_f32 (about 1 ulp): box exit distances are
+// widened by 4 ulp, and the watertight shear only has to be the same for every
+// triangle a ray meets, not correctly rounded.
+__device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
 __device__ __forceinline__ RaySetup setup_ray(float3 o, float3 d, float tmin) {
     RaySetup r;
-    r.o = o; r.d = d; r.tmin = tmin;
+    r.o = o; r.tmin = tmin;
     auto safe = [](float x) { return fabsf(x) < 1e-20f ? copysignf(1e-20f, x) : x; };
-    r.ix = 1.0f / safe(d.x); r.iy = 1.0f / safe(d.y); r.iz = 1.0f / safe(d.z);
-    r.nx = r.ix < 0.f; r.ny = r.iy < 0.f; r.nz = r.iz < 0.f;
+    r.ix = rcp(safe(d.x)); r.iy = rcp(safe(d.y)); r.iz = rcp(safe(d.z));
+    // a Node4 holds rows lo_x, hi_x, lo_y, hi_y, lo_z, hi_z of 16 bytes: the near
+    // plane of an axis is its lo row for a positive direction, its hi row otherwise
+    r.onx = r.ix < 0.f ? 16u : 0u;
+    r.ony = r.iy < 0.f ? 48u : 32u;
+    r.onz = r.iz < 0.f ? 80u : 64u;
     float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
     int kz = (ax >= ay) ? ((ax >= az) ? 0 : 2) : ((ay >= az) ? 1 : 2);
     int kx = kz == 2 ? 0 : kz + 1;
     int ky = kx == 2 ? 0 : kx + 1;
     if (pick(d, kz) < 0.f) { int t = kx; kx = ky; ky = t; }
     r.kx = kx; r.ky = ky; r.kz = kz;
-    float dz = pick(d, kz);
-    r.Sx = pick(d, kx) / dz; r.Sy = pick(d, ky) / dz; r.Sz = 1.0f / dz;
+    r.Sz = rcp(pick(d, kz));
+    r.Sx = pick(d, kx) * r.Sz; r.Sy = pick(d, ky) * r.Sz;
     return r;
-}
-
-// FP64 recomputation of the watertight test (edge and vertex hits, where an
-// FP32 edge function is exactly zero).
-__device__ __forceinline__ double pickd(double x, double y, double z, int k) { return k == 0 ? x : (k == 1 ? y : z); }
-__device__ __forceinline__ bool tri_test_fp64(const RaySetup &r, float3 a, float3 b, float3 c, double tmin, double tmax,
-                                              double &t_out) {
-    const double ox = r.o.x, oy = r.o.y, oz = r.o.z;
-    const double ax = a.x - ox, ay = a.y - oy, az = a.z - oz;
-    const double bx = b.x - ox, by = b.y - oy, bz = b.z - oz;
-    const double cx = c.x - ox, cy = c.y - oy, cz = c.z - oz;
-    const double dkx = pickd(r.d.x, r.d.y, r.d.z, r.kx), dky = pickd(r.d.x, r.d.y, r.d.z, r.ky);
-    const double dkz = pickd(r.d.x, r.d.y, r.d.z, r.kz);
-    const double Sx = dkx / dkz, Sy = dky / dkz, Sz = 1.0 / dkz;
-    const double Akz = pickd(ax, ay, az, r.kz), Bkz = pickd(bx, by, bz, r.kz), Ckz = pickd(cx, cy, cz, r.kz);
-    const double Ax = pickd(ax, ay, az, r.kx) - Sx * Akz, Ay = pickd(ax, ay, az, r.ky) - Sy * Akz;
-    const double Bx = pickd(bx, by, bz, r.kx) - Sx * Bkz, By = pickd(bx, by, bz, r.ky) - Sy * Bkz;
-    const double Cx = pickd(cx, cy, cz, r.kx) - Sx * Ckz, Cy = pickd(cx, cy, cz, r.ky) - Sy * Ckz;
-    double U, V, W;
-    {
-        // unfused, as in tri_test: a shared edge gives its two triangles
-        // exactly opposite values (the watertightness argument of Woop et al.)
-#pragma clang fp contract(off)
-        U = Cx * By - Cy * Bx;
-        V = Ax * Cy - Ay * Cx;
-        W = Bx * Ay - By * Ax;
-    }
-    if ((U < 0 || V < 0 || W < 0) && (U > 0 || V > 0 || W > 0)) return false;
-    const double det = U + V + W;
-    if (det == 0.0) return false;
-    const double t = Sz * (U * Akz + V * Bkz + W * Ckz) / det;
-    if (!(t > tmin && t <= tmax)) return false;
-    t_out = t;
-    return true;
 }
 
 // Watertight FP32 test; returns true and t for a hit in (tmin, tmax].
@@ -96,16 +71,28 @@ __device__ __forceinline__ bool tri_test(const RaySetup &r, float4 va, float4 vb
         W = Bx * Ay - By * Ax;
     }
     if (U == 0.f || V == 0.f || W == 0.f) {
-        double t;
-        if (!tri_test_fp64(r, xyz(va), xyz(vb), xyz(vc), r.tmin, tmax, t)) return false;
-        t_out = float(t);
-        return true;
+        // An edge function rounded to zero: recompute the three in FP64 from the
+        // same projected coordinates (Woop et al., section 3.3).  A product of
+        // two floats is exact in FP64, so each difference has the exact sign,
+        // and a shared edge still gives its two triangles opposite values.
+        double Ud, Vd, Wd;
+        {
+#pragma clang fp contract(off)
+            Ud = double(Cx) * double(By) - double(Cy) * double(Bx);
+            Vd = double(Ax) * double(Cy) - double(Ay) * double(Cx);
+            Wd = double(Bx) * double(Ay) - double(By) * double(Ax);
+        }
+        if ((Ud < 0.0 || Vd < 0.0 || Wd < 0.0) && (Ud > 0.0 || Vd > 0.0 || Wd > 0.0)) return false;
+        U = float(Ud);
+        V = float(Vd);
+        W = float(Wd);
+    } else if ((U < 0.f || V < 0.f || W < 0.f) && (U > 0.f || V > 0.f || W > 0.f)) {
+        return false;
     }
-    if ((U < 0.f || V < 0.f || W < 0.f) && (U > 0.f || V > 0.f || W > 0.f)) return false;
     const float det = U + V + W;
     if (det == 0.f) return false;
     const float T = r.Sz * (U * Akz + V * Bkz + W * Ckz);
-    const float t = T / det;
+    const float t = T * rcp(det);
     if (!(t > r.tmin && t <= tmax)) return false;
     t_out = t;
     return true;
@@ -158,6 +145,39 @@ struct LaneStack {
     }
 };
 
+// Entry and exit distances of the ray through the four child boxes of inner
+// node `cur`, and the child refs.  The near and far plane rows are loaded at
+// the ray's per-axis byte offsets, so no per-child select is needed.  Empty
+// slots hold inverted boxes (lo = FLT_MAX, hi = -FLT_MAX): their entry
+// distance exceeds their exit distance, so they fail the box test with no
+// check of the child ref.  `far_limit` culls boxes entered beyond it; exit
+// distances are scaled by 1 + 2^-21 so rounding never culls a box the ray
+// touches.  Returns the refs; hit[i] marks the children whose box is entered.
+template <class G>
+__device__ __forceinline__ uint4 node_test(const G &s, const RaySetup &r, uint32_t cur, float far_limit, float tn[4],
+                                           bool hit[4]) {
+    const char *nb = reinterpret_cast<const char *>(s.nodes);
+    const uint32_t base = cur * uint32_t(sizeof(Node4));
+    auto row = [&](uint32_t off) { return *reinterpret_cast<const float4 *>(nb + (base + off)); };
+    const float4 nx_ = row(r.onx), fx_ = row(r.onx ^ 16u);
+    const float4 ny_ = row(r.ony), fy_ = row(r.ony ^ 16u);
+    const float4 nz_ = row(r.onz), fz_ = row(r.onz ^ 16u);
+    const uint4 ch = *reinterpret_cast<const uint4 *>(nb + (base + 96u));
+    const float ox = r.o.x, oy = r.o.y, oz = r.o.z;
+    tn[0] = fmaxf(fmaxf((nx_.x - ox) * r.ix, (ny_.x - oy) * r.iy), fmaxf((nz_.x - oz) * r.iz, r.tmin));
+    tn[1] = fmaxf(fmaxf((nx_.y - ox) * r.ix, (ny_.y - oy) * r.iy), fmaxf((nz_.y - oz) * r.iz, r.tmin));
+    tn[2] = fmaxf(fmaxf((nx_.z - ox) * r.ix, (ny_.z - oy) * r.iy), fmaxf((nz_.z - oz) * r.iz, r.tmin));
+    tn[3] = fmaxf(fmaxf((nx_.w - ox) * r.ix, (ny_.w - oy) * r.iy), fmaxf((nz_.w - oz) * r.iz, r.tmin));
+    float tf[4];
+    tf[0] = fminf(fminf((fx_.x - ox) * r.ix, (fy_.x - oy) * r.iy), (fz_.x - oz) * r.iz);
+    tf[1] = fminf(fminf((fx_.y - ox) * r.ix, (fy_.y - oy) * r.iy), (fz_.y - oz) * r.iz);
+    tf[2] = fminf(fminf((fx_.z - ox) * r.ix, (fy_.z - oy) * r.iy), (fz_.z - oz) * r.iz);
+    tf[3] = fminf(fminf((fx_.w - ox) * r.ix, (fy_.w - oy) * r.iy), (fz_.w - oz) * r.iz);
+#pragma unroll
+    for (int i = 0; i < 4; i++) hit[i] = tn[i] <= fminf(tf[i] * 1.00000048f, far_limit);
+    return ch;
+}
+
 struct TraversalCounters {
     uint32_t nodes = 0, tris = 0;
 };
@@ -177,6 +197,27 @@ __device__ __forceinline__ void trav_begin(TravState &ts, float3 o, float3 d, fl
     ts.cur = 0;  // root (internal node 0)
 }
 
+// Orders the children near to far by entry distance (key: +inf for a child
+// whose box is not entered), pushes the entered ones but the nearest, farthest
+// first, and returns the nearest (kInvalid if none is entered).  The
+// 5-comparator network is branch-free: min/max of the keys, selects of the refs.
+template <int kBlock>
+__device__ __forceinline__ uint32_t sort_push(float key[4], uint32_t c[4], LaneStack<kBlock> &st) {
+    auto cswap = [&](int a, int b) {
+        const bool sw = key[b] < key[a];
+        const float lo = fminf(key[a], key[b]), hi = fmaxf(key[a], key[b]);
+        const uint32_t ca = sw ? c[b] : c[a], cb = sw ? c[a] : c[b];
+        key[a] = lo; key[b] = hi;
+        c[a] = ca; c[b] = cb;
+    };
+    cswap(0, 1); cswap(2, 3); cswap(0, 2); cswap(1, 3); cswap(1, 2);
+    // the entered children are now a prefix of the four
+    if (key[3] != INFINITY) st.push(c[3], key[3]);
+    if (key[2] != INFINITY) st.push(c[2], key[2]);
+    if (key[1] != INFINITY) st.push(c[1], key[1]);
+    return key[0] != INFINITY ? c[0] : kInvalid;
+}
+
 // Processes one node (internal: box tests and pushes; leaf: triangle tests),
 // then pops if needed.  Returns true when the lane's ray is finished.
 // Closest hit (ANY = false) visits children near-to-far and culls stack entries
@@ -187,53 +228,25 @@ __device__ __forceinline__ bool trav_step(const G &s, TravState &ts, LaneStack<k
     const RaySetup &r = ts.r;
     const uint32_t cur = ts.cur;
     if (!(cur & kLeafBit)) {
-        const float4 *np = reinterpret_cast<const float4 *>(s.nodes + cur);
-        const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
-        const uint4 ch = reinterpret_cast<const uint4 *>(np)[6];
+        float tn[4];
+        bool h[4];
+        const uint4 ch = node_test(s, r, cur, ANY ? ts.t : cull_limit(ts.t), tn, h);
         if (COUNT) cnt.nodes++;
-        const float ox = r.o.x, oy = r.o.y, oz = r.o.z;
-        const float4 nx_ = r.nx ? hx : lx, fx_ = r.nx ? lx : hx;
-        const float4 ny_ = r.ny ? hy : ly, fy_ = r.ny ? ly : hy;
-        const float4 nz_ = r.nz ? hz : lz, fz_ = r.nz ? lz : hz;
-        float tn[4], tf[4];
-        tn[0] = fmaxf(fmaxf((nx_.x - ox) * r.ix, (ny_.x - oy) * r.iy), fmaxf((nz_.x - oz) * r.iz, r.tmin));
-        tn[1] = fmaxf(fmaxf((nx_.y - ox) * r.ix, (ny_.y - oy) * r.iy), fmaxf((nz_.y - oz) * r.iz, r.tmin));
-        tn[2] = fmaxf(fmaxf((nx_.z - ox) * r.ix, (ny_.z - oy) * r.iy), fmaxf((nz_.z - oz) * r.iz, r.tmin));
-        tn[3] = fmaxf(fmaxf((nx_.w - ox) * r.ix, (ny_.w - oy) * r.iy), fmaxf((nz_.w - oz) * r.iz, r.tmin));
-        tf[0] = fminf(fminf((fx_.x - ox) * r.ix, (fy_.x - oy) * r.iy), (fz_.x - oz) * r.iz);
-        tf[1] = fminf(fminf((fx_.y - ox) * r.ix, (fy_.y - oy) * r.iy), (fz_.y - oz) * r.iz);
-        tf[2] = fminf(fminf((fx_.z - ox) * r.ix, (fy_.z - oy) * r.iy), (fz_.z - oz) * r.iz);
-        tf[3] = fminf(fminf((fx_.w - ox) * r.ix, (fy_.w - oy) * r.iy), (fz_.w - oz) * r.iz);
         uint32_t c[4] = {ch.x, ch.y, ch.z, ch.w};
         float key[4];
-        const float tc = ANY ? ts.t : cull_limit(ts.t);
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const bool h = c[i] != kEmptyChild && tn[i] <= fminf(tf[i] * 1.00000048f, tc);
-            key[i] = h ? tn[i] : INFINITY;
-        }
-        if (!ANY) {
-            // sort (key, child) ascending: 5-comparator network
-#define WR_CSWAP(a, b)                                                          \
-    if (key[b] < key[a]) {                                                      \
-        float tk = key[a]; key[a] = key[b]; key[b] = tk;                        \
-        uint32_t tc = c[a]; c[a] = c[b]; c[b] = tc;                             \
-    }
-            WR_CSWAP(0, 1) WR_CSWAP(2, 3) WR_CSWAP(0, 2) WR_CSWAP(1, 3) WR_CSWAP(1, 2)
-#undef WR_CSWAP
-        }
-        // push hits farthest first; continue with the nearest
+        for (int i = 0; i < 4; i++) key[i] = h[i] ? tn[i] : INFINITY;
         uint32_t next = kInvalid;
-        float next_t = 0.f;
+        if (!ANY) {
+            next = sort_push(key, c, st);
+        } else {
+            // slot order (largest box first); push the others, continue with the first
 #pragma unroll
-        for (int i = 3; i >= 0; i--) {
-            if (key[i] != INFINITY) {
-                if (next != kInvalid) {
-                    if (ANY) st.push_ref(next);
-                    else st.push(next, next_t);
+            for (int i = 3; i >= 0; i--) {
+                if (key[i] != INFINITY) {
+                    if (next != kInvalid) st.push_ref(next);
+                    next = c[i];
                 }
-                next = c[i];
-                next_t = key[i];
             }
         }
         if (next != kInvalid) { ts.cur = next; return false; }
@@ -284,7 +297,10 @@ __device__ __forceinline__ bool trav_step(const G &s, TravState &ts, LaneStack<k
 // step order, so the builders' stack bound holds.  All lanes of the wave must
 // call it (it takes a ballot); `live` marks lanes with a ray.  Returns true
 // when the lane's ray is finished.
-constexpr int kSpecSteps = 4;
+#ifndef WR_SPEC_STEPS
+#define WR_SPEC_STEPS 4
+#endif
+constexpr int kSpecSteps = WR_SPEC_STEPS;
 template <bool COUNT, int kBlock, class G>
 __device__ __forceinline__ bool trav_round(const G &s, TravState &ts, LaneStack<kBlock> &st, TraversalCounters &cnt,
                                            bool live) {
@@ -301,48 +317,15 @@ __device__ __forceinline__ bool trav_round(const G &s, TravState &ts, LaneStack<
             leaf = cur;
             pop = true;
         } else {
-            const float4 *np = reinterpret_cast<const float4 *>(s.nodes + cur);
-            const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
-            const uint4 ch = reinterpret_cast<const uint4 *>(np)[6];
+            float tn[4];
+            bool h[4];
+            const uint4 ch = node_test(s, r, cur, cull_limit(ts.t), tn, h);
             if (COUNT) cnt.nodes++;
-            const float ox = r.o.x, oy = r.o.y, oz = r.o.z;
-            const float4 nx_ = r.nx ? hx : lx, fx_ = r.nx ? lx : hx;
-            const float4 ny_ = r.ny ? hy : ly, fy_ = r.ny ? ly : hy;
-            const float4 nz_ = r.nz ? hz : lz, fz_ = r.nz ? lz : hz;
-            float tn[4], tf[4];
-            tn[0] = fmaxf(fmaxf((nx_.x - ox) * r.ix, (ny_.x - oy) * r.iy), fmaxf((nz_.x - oz) * r.iz, r.tmin));
-            tn[1] = fmaxf(fmaxf((nx_.y - ox) * r.ix, (ny_.y - oy) * r.iy), fmaxf((nz_.y - oz) * r.iz, r.tmin));
-            tn[2] = fmaxf(fmaxf((nx_.z - ox) * r.ix, (ny_.z - oy) * r.iy), fmaxf((nz_.z - oz) * r.iz, r.tmin));
-            tn[3] = fmaxf(fmaxf((nx_.w - ox) * r.ix, (ny_.w - oy) * r.iy), fmaxf((nz_.w - oz) * r.iz, r.tmin));
-            tf[0] = fminf(fminf((fx_.x - ox) * r.ix, (fy_.x - oy) * r.iy), (fz_.x - oz) * r.iz);
-            tf[1] = fminf(fminf((fx_.y - ox) * r.ix, (fy_.y - oy) * r.iy), (fz_.y - oz) * r.iz);
-            tf[2] = fminf(fminf((fx_.z - ox) * r.ix, (fy_.z - oy) * r.iy), (fz_.z - oz) * r.iz);
-            tf[3] = fminf(fminf((fx_.w - ox) * r.ix, (fy_.w - oy) * r.iy), (fz_.w - oz) * r.iz);
             uint32_t c[4] = {ch.x, ch.y, ch.z, ch.w};
             float key[4];
-            const float tc = cull_limit(ts.t);
 #pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const bool h = c[i] != kEmptyChild && tn[i] <= fminf(tf[i] * 1.00000048f, tc);
-                key[i] = h ? tn[i] : INFINITY;
-            }
-#define WR_CSWAP(a, b)                                                          \
-    if (key[b] < key[a]) {                                                      \
-        float tk = key[a]; key[a] = key[b]; key[b] = tk;                        \
-        uint32_t tc = c[a]; c[a] = c[b]; c[b] = tc;                             \
-    }
-            WR_CSWAP(0, 1) WR_CSWAP(2, 3) WR_CSWAP(0, 2) WR_CSWAP(1, 3) WR_CSWAP(1, 2)
-#undef WR_CSWAP
-            uint32_t next = kInvalid;
-            float next_t = 0.f;
-#pragma unroll
-            for (int i = 3; i >= 0; i--) {
-                if (key[i] != INFINITY) {
-                    if (next != kInvalid) st.push(next, next_t);
-                    next = c[i];
-                    next_t = key[i];
-                }
-            }
+            for (int i = 0; i < 4; i++) key[i] = h[i] ? tn[i] : INFINITY;
+            const uint32_t next = sort_push(key, c, st);
             if (next != kInvalid) ts.cur = next;
             else pop = true;
         }
@@ -395,28 +378,15 @@ __device__ __forceinline__ bool trav_round_any(const G &s, TravState &ts, LaneSt
             leaf = cur;
             pop = true;
         } else {
-            const float4 *np = reinterpret_cast<const float4 *>(s.nodes + cur);
-            const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
-            const uint4 ch = reinterpret_cast<const uint4 *>(np)[6];
+            float tn[4];
+            bool h[4];
+            const uint4 ch = node_test(s, r, cur, ts.t, tn, h);
             if (COUNT) cnt.nodes++;
-            const float ox = r.o.x, oy = r.o.y, oz = r.o.z;
-            const float4 nx_ = r.nx ? hx : lx, fx_ = r.nx ? lx : hx;
-            const float4 ny_ = r.ny ? hy : ly, fy_ = r.ny ? ly : hy;
-            const float4 nz_ = r.nz ? hz : lz, fz_ = r.nz ? lz : hz;
-            float tn[4], tf[4];
-            tn[0] = fmaxf(fmaxf((nx_.x - ox) * r.ix, (ny_.x - oy) * r.iy), fmaxf((nz_.x - oz) * r.iz, r.tmin));
-            tn[1] = fmaxf(fmaxf((nx_.y - ox) * r.ix, (ny_.y - oy) * r.iy), fmaxf((nz_.y - oz) * r.iz, r.tmin));
-            tn[2] = fmaxf(fmaxf((nx_.z - ox) * r.ix, (ny_.z - oy) * r.iy), fmaxf((nz_.z - oz) * r.iz, r.tmin));
-            tn[3] = fmaxf(fmaxf((nx_.w - ox) * r.ix, (ny_.w - oy) * r.iy), fmaxf((nz_.w - oz) * r.iz, r.tmin));
-            tf[0] = fminf(fminf((fx_.x - ox) * r.ix, (fy_.x - oy) * r.iy), (fz_.x - oz) * r.iz);
-            tf[1] = fminf(fminf((fx_.y - ox) * r.ix, (fy_.y - oy) * r.iy), (fz_.y - oz) * r.iz);
-            tf[2] = fminf(fminf((fx_.z - ox) * r.ix, (fy_.z - oy) * r.iy), (fz_.z - oz) * r.iz);
-            tf[3] = fminf(fminf((fx_.w - ox) * r.ix, (fy_.w - oy) * r.iy), (fz_.w - oz) * r.iz);
             const uint32_t c[4] = {ch.x, ch.y, ch.z, ch.w};
             uint32_t next = kInvalid;
 #pragma unroll
             for (int i = 3; i >= 0; i--) {
-                if (c[i] != kEmptyChild && tn[i] <= fminf(tf[i] * 1.00000048f, ts.t)) {
+                if (h[i]) {
                     if (next != kInvalid) st.push_ref(next);
                     next = c[i];
                 }
