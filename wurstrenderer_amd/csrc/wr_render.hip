@@ -2135,6 +2135,20 @@ static int trav_grid(const wr_scene *s, uint32_t n, int per_cu) {
 constexpr uint64_t kMaxSamplesPerPass = uint64_t(1) << 27;
 // Paths in flight when the caller leaves max_paths_in_flight at 0.
 constexpr uint64_t kDefaultPool = uint64_t(1) << 27;
+// The default pool of a scene whose pool has to grow: `want` paths, but no
+// more than a quarter of the device memory that is free (counting the scene's
+// current pool, which the growth releases) -- many scenes can then keep their
+// pools in one process -- and never below the current pool or 2^20 paths.
+static uint64_t default_pool_fit(const wr_scene *s, uint64_t want, uint64_t samples) {
+    if (want <= s->capacity && samples <= s->sample_capacity) return want;
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return want;
+    uint64_t avail = free_b + (s->pool ? pool_bytes(s->capacity, s->sample_capacity) : 0);
+    const uint64_t per_path = pool_bytes(1 << 20, 0) >> 20;
+    const uint64_t budget = avail / 4 > 16 * samples ? (avail / 4 - 16 * samples) / per_path : 0;
+    const uint64_t floor = std::min<uint64_t>(want, std::max<uint64_t>(s->capacity, uint64_t(1) << 20));
+    return std::max(floor, std::min(want, budget));
+}
 
 // Runs wavefront iterations until the pool drains.  launch(pb) issues one
 // iteration (refill, extend, shade, shadow, rotate) for the state pb; the host
@@ -2279,6 +2293,7 @@ static int render_impl(wr_scene *s, wr_film *f, const wr_render_params *prm, wr_
     uint64_t pool = prm->max_paths_in_flight > 0 ? uint64_t(prm->max_paths_in_flight) : kDefaultPool;
     if (fixed) pool = std::min<uint64_t>(pool, fixed->capacity);
     pool = std::min<uint64_t>(pool, samples_pass);
+    if (!fixed && prm->max_paths_in_flight == 0) pool = default_pool_fit(s, pool, samples_pass);
     if (pool > (uint64_t(1) << 30)) return set_error(WR_ERR_UNSUPPORTED, "too many paths in flight");
     const uint32_t P = uint32_t(pool);
     uint32_t fixed_counters[kCounterStride * kNumCounterLines];  // host copy of the counters (fixed pool)
@@ -2653,6 +2668,7 @@ int wr_render_batch(const wr_batch_job *jobs, uint32_t n_jobs, const wr_render_p
     max_pass = std::min<uint64_t>(max_pass, kMaxSamplesPerPass);
     uint64_t pool = prm->max_paths_in_flight > 0 ? uint64_t(prm->max_paths_in_flight) : kDefaultPool;
     pool = std::min<uint64_t>(pool, max_pass);
+    if (prm->max_paths_in_flight == 0) pool = default_pool_fit(s0, pool, max_pass);
     if (pool > (uint64_t(1) << 30)) return set_error(WR_ERR_UNSUPPORTED, "too many paths in flight");
     const uint32_t P = uint32_t(pool);
     rc = ensure_capacity(s0, P, uint32_t(max_pass));

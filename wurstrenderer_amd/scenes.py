@@ -337,6 +337,18 @@ def batch_workload(seed: int = 3, n_scenes: int = 32, tri_range=(10_000, 2_000_0
     return out
 
 
+def medium_clutter(seed: int = 5, resolution=(1280, 720), tri_range=(10_000, 2_000_000), sigma_a=0.05, sigma_s=0.4,
+                   g=0.7) -> Scene:
+    """The batched-workload scene type filled with a homogeneous medium: a
+    triangle budget drawn log-uniform in tri_range from `seed`, the Cornell box
+    with that much icosphere clutter (cornell_clutter with the same seed), and
+    the medium (sigma_a, sigma_s RGB-equal, Henyey-Greenstein g) filling space."""
+    rng = np.random.default_rng(seed)
+    t = int(math.exp(rng.uniform(math.log(tri_range[0]), math.log(tri_range[1]))))
+    sc = cornell_clutter(seed, t, resolution)
+    return sc.with_medium(sigma_a=sigma_a, sigma_s=sigma_s, g=g)
+
+
 def emitting_box(resolution=(24, 24), L=1.0, half=1.0) -> Scene:
     """Camera at the centre of a closed cube whose six faces emit L inward (no
     reflecting surface): with a purely absorbing medium a pixel sees
