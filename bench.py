@@ -16,6 +16,7 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -47,17 +48,22 @@ def parse():
     ap.add_argument("--ttr-target", type=float, default=1e-3)
     ap.add_argument("--ttr-ref-spp", type=int, default=65536)
     ap.add_argument("--ttr-runs", type=int, default=3)
-    ap.add_argument("--ttr-train-spp", type=int, default=128)
+    ap.add_argument("--ttr-train-spp", type=int, default=64)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-scene measurement")
     ap.add_argument("--no-integrators", action="store_true",
                     help="skip primitive / light tracing / visualize / envmap on the headline frame")
     ap.add_argument("--batch-scenes", type=int, default=32)
+    ap.add_argument("--no-config4", action="store_true", help="skip the 1080p guided-vs-plain equal-relMSE measurement")
+    ap.add_argument("--c4-width", type=int, default=1920)
+    ap.add_argument("--c4-height", type=int, default=1080)
+    ap.add_argument("--c4-ref-spp", type=int, default=8192)
+    ap.add_argument("--c4-runs", type=int, default=3)
     ap.add_argument("--no-medium", action="store_true", help="skip the medium BF16/FP8 inference measurement")
     ap.add_argument("--med-width", type=int, default=1280)
     ap.add_argument("--med-height", type=int, default=720)
     ap.add_argument("--med-spp", type=int, default=512)
-    ap.add_argument("--med-ref-spp", type=int, default=2048)
-    ap.add_argument("--med-train-spp", type=int, default=128)
+    ap.add_argument("--med-ref-spp", type=int, default=6144)
+    ap.add_argument("--med-train-spp", type=int, default=64)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first copy 1 GiB device-to-device three times (known-size kernel for byte counters)")
     return ap.parse_args()
@@ -93,81 +99,123 @@ def rel_mse(img, ref, eps=1e-2):
     return float(np.mean((img - ref) ** 2 / (ref * ref + eps)))
 
 
-def progressive(render, film, ref, target, first_spp, max_spp):
-    """Progressive spp doubling: wall time of the render calls until the first
-    checkpoint with relMSE <= target, and the crossing time interpolated between
-    the last two checkpoints (relMSE ~ 1/time there)."""
-    import math
-    t_total, spp, step, curve = 0.0, 0, first_spp, []
-    while spp < max_spp:
+def checkpoints(first, max_spp, growth=1.25, step=8):
+    """spp checkpoints: every `step` samples up to 128 (so online training runs
+    whole 8-spp passes), then x growth, rounded up to a multiple of `step`."""
+    out, spp = [], first
+    while spp <= max_spp:
+        out.append(spp)
+        nxt = spp + step if spp < 128 else int(math.ceil(spp * growth / step) * step)
+        spp = max(nxt, spp + step)
+    return out
+
+
+def crossing(curve, target):
+    """Wall time at which relMSE reaches target, interpolated in log-log between
+    the two checkpoints around it (relMSE ~ 1/time there); None if never."""
+    for k, (_, t1, e1) in enumerate(curve):
+        if e1 <= target:
+            if k == 0:
+                return t1
+            _, t0, e0 = curve[k - 1]
+            if e0 > e1 > 0:
+                return math.exp(math.log(t0) + (math.log(t1) - math.log(t0)) * math.log(e0 / target) /
+                                math.log(e0 / e1))
+            return t1
+    return None
+
+
+def progressive(render, film, ref, floor, target, points):
+    """Renders up to each checkpoint in turn (only the render calls are timed) and
+    records relMSE against ref minus the reference's own noise floor (for
+    independent unbiased images E[(I-R)^2] = Var I + Var R); stops at the first
+    checkpoint at or below target.  Returns (seconds, crossing seconds, spp, curve)."""
+    t_total, done, curve = 0.0, 0, []
+    for spp in points:
         t = time.perf_counter()
-        render(film, step)
+        render(film, spp - done)
         t_total += time.perf_counter() - t
-        spp += step
-        e = rel_mse(film.image(), ref)
+        done = spp
+        e = rel_mse(film.image(), ref) - floor
         curve.append([spp, round(t_total, 4), e])
         if e <= target:
-            t_cross = t_total
-            if len(curve) > 1:
-                (_, t0, e0), (_, t1, e1) = curve[-2], curve[-1]
-                if e0 > e1 > 0:
-                    t_cross = math.exp(math.log(t0) + (math.log(t1) - math.log(t0)) * math.log(e0 / target) / math.log(e0 / e1))
-            return t_total, t_cross, spp, curve
-        step = spp
-    return None, None, spp, curve
+            return t_total, crossing(curve, target), spp, curve
+    return None, None, done, curve
+
+
+def reference(wr, gs, spp, N, seeds):
+    """Plain render of spp samples per pixel as two independent halves; returns
+    (image, noise floor = relMSE of the mean of the halves, seconds)."""
+    t = time.perf_counter()
+    fa, fb = wr.Film(gs), wr.Film(gs)
+    wr.render_nee(gs, fa, spp // 2, num_path_vertices=N, seed=seeds[0])
+    wr.render_nee(gs, fb, spp // 2, num_path_vertices=N, seed=seeds[1])
+    ia, ib = fa.image(), fb.image()
+    fa.close()
+    fb.close()
+    return 0.5 * (ia + ib), rel_mse(ia, ib) / 4.0, time.perf_counter() - t
+
+
+# The guided schedule every guided measurement uses: the network guides the
+# first 5 bounces with probability 0.4 (deeper vertices sample the BSDF only),
+# and trains online on the first 64 spp (8-spp passes, 64 Adam steps of 2^18
+# records at lr 1e-2 after each).  Picked on the 1080p gap room at equal
+# relMSE (256 spp, medians of 3): bounce limit 4/5/6/all 2.02/2.10/2.03/1.42x,
+# guide fraction 0.3/0.4/0.5 1.98/2.10/1.85x, 64 spp x 64 steps against
+# 128 x 32 2.10/1.85x.
+GUIDE = dict(guide_fraction=0.4, max_guided_bounce=5, train_spp=64, spp_per_pass=8, train_steps=64)
+
+
+def guided_renderer(wr, gs, N, seed, train_spp=GUIDE["train_spp"], fp8=False):
+    """Online guided rendering as benchmarked (GUIDE): a fresh network, trained
+    between the passes of the first train_spp samples per pixel, then used
+    without further training."""
+    g = wr.Guide(gs, learning_rate=1e-2, batch_size=1 << 18, record_capacity=1 << 20, seed=seed)
+    state = {"calls": 0}
+    kw = dict(num_path_vertices=N, seed=seed, guide_fraction=GUIDE["guide_fraction"],
+              max_guided_bounce=GUIDE["max_guided_bounce"], fp8_inference=fp8)
+
+    def render(film, spp):
+        left = max(0, train_spp - film.spp)
+        if left:
+            k = min(spp, left)
+            wr.render_guided(gs, film, g, k, spp_per_pass=GUIDE["spp_per_pass"], train_steps=GUIDE["train_steps"],
+                             warmup_passes=1 if state["calls"] == 0 else 0, **kw)
+            spp -= k
+        if spp:
+            wr.render_guided(gs, film, g, spp, spp_per_pass=spp, train_steps=0, warmup_passes=0, **kw)
+        state["calls"] += 1
+    return render, g
 
 
 def time_to_relmse(wr, scenes, dev, a):
     """Guided vs plain GPU NEE+MIS on the gap-lit room (1,003,550 triangles, light
     L = 500 hidden behind a 0.2 x 0.2 gap), max depth 16: wall time until the
-    progressively doubled image reaches relMSE <= target against a plain
-    reference at ttr_ref_spp; median of ttr_runs runs, each from an untrained
-    network.  Guided: 8-spp passes with 32 Adam steps of 2^18 records after each
-    until ttr_train_spp, then the trained network without further training."""
+    progressively rendered image reaches relMSE <= target (reference noise floor
+    subtracted) against a plain reference at ttr_ref_spp; checkpoints every
+    8 spp to 128, then x1.25; median of ttr_runs runs, each from an untrained
+    network."""
     import numpy as np
     N = 18
     sc = scenes.guide_room(4, resolution=(a.ttr_width, a.ttr_height))
     gs = wr.GpuScene(dev, sc)
-    t = time.perf_counter()
-    half = a.ttr_ref_spp // 2
-    fa, fb = wr.Film(gs), wr.Film(gs)
-    wr.render_nee(gs, fa, half, num_path_vertices=N, seed=9001)
-    wr.render_nee(gs, fb, half, num_path_vertices=N, seed=9002)
-    ia, ib = fa.image(), fb.image()
-    ref = 0.5 * (ia + ib)
-    ref_s = time.perf_counter() - t
-    floor = rel_mse(ia, ib) / 4.0  # relMSE of the mean of two halves against the truth, approximately
+    ref, floor, ref_s = reference(wr, gs, a.ttr_ref_spp, N, (9001, 9002))
     print(f"# ttr reference: {a.ttr_ref_spp} spp in {ref_s:.1f}s, noise floor ~{floor:.2e}", file=sys.stderr, flush=True)
-    max_spp = a.ttr_ref_spp
-
-    def plain(seed):
-        return lambda film, spp: wr.render_nee(gs, film, spp, num_path_vertices=N, seed=seed)
-
-    def guided(seed):
-        g = wr.Guide(gs, learning_rate=1e-2, batch_size=1 << 18, record_capacity=1 << 20, seed=seed)
-        state = {"calls": 0}
-
-        def render(film, spp):
-            # online training: 8-spp passes, 32 Adam steps after each, for the first
-            # TRAIN_SPP samples per pixel; then the trained network alone
-            left = max(0, a.ttr_train_spp - film.spp)
-            if left:
-                k = min(spp, left)
-                wr.render_guided(gs, film, g, k, num_path_vertices=N, seed=seed, guide_fraction=0.5,
-                                 spp_per_pass=8, train_steps=32, warmup_passes=1 if state["calls"] == 0 else 0)
-                spp -= k
-            if spp:
-                wr.render_guided(gs, film, g, spp, num_path_vertices=N, seed=seed, guide_fraction=0.5,
-                                 spp_per_pass=spp, train_steps=0, warmup_passes=0)
-            state["calls"] += 1
-        return render
-
+    points = checkpoints(16, a.ttr_ref_spp // 2)
     res = {}
-    for name, make, first in (("plain", plain, 16), ("guided", guided, 16)):
+    for name in ("plain", "guided"):
         times, crosses, curves = [], [], []
         for r in range(a.ttr_runs):
             film = wr.Film(gs)
-            tt, tc, spp, curve = progressive(make(100 + r), film, ref, a.ttr_target, first, max_spp)
+            if name == "plain":
+                render = (lambda seed: lambda f, spp: wr.render_nee(gs, f, spp, num_path_vertices=N, seed=seed))(100 + r)
+                g = None
+            else:
+                render, g = guided_renderer(wr, gs, N, 100 + r, a.ttr_train_spp)
+            tt, tc, spp, curve = progressive(render, film, ref, floor, a.ttr_target, points)
+            film.close()
+            if g is not None:
+                g.close()
             times.append(tt)
             crosses.append(tc)
             curves.append(curve)
@@ -178,80 +226,211 @@ def time_to_relmse(wr, scenes, dev, a):
         okc = [x for x in crosses if x is not None]
         res[name] = {"seconds_median": float(np.median(ok)) if len(ok) == len(times) else None,
                      "crossing_seconds_median": float(np.median(okc)) if len(okc) == len(crosses) else None,
-                     "seconds_runs": times, "crossing_seconds_runs": crosses, "curve_run0": curves[0]}
+                     "seconds_runs": times, "crossing_seconds_runs": crosses,
+                     "curve_run0": [[c[0], c[1], float(f"{c[2]:.4e}")] for c in curves[0]]}
     sp = spc = None
     if res["plain"]["seconds_median"] and res["guided"]["seconds_median"]:
         sp = res["plain"]["seconds_median"] / res["guided"]["seconds_median"]
     if res["plain"]["crossing_seconds_median"] and res["guided"]["crossing_seconds_median"]:
         spc = res["plain"]["crossing_seconds_median"] / res["guided"]["crossing_seconds_median"]
+    gs.close()
     return {"target_relmse": a.ttr_target, "workload": f"guide_room seed 4 (1,003,550 tris, gap 0.2, L 500), "
                                                        f"{a.ttr_width}x{a.ttr_height}, max depth 16",
-            "reference": {"spp": a.ttr_ref_spp, "estimator": "plain", "noise_floor_relmse": floor,
-                          "seconds": round(ref_s, 2)},
-            "guided_schedule": {"train_spp": a.ttr_train_spp, "spp_per_pass": 8, "adam_steps_per_pass": 32,
-                                "batch": 1 << 18, "lr": 1e-2, "guide_fraction": 0.5},
+            "reference": {"spp": a.ttr_ref_spp, "estimator": "plain GPU (pinned to the FP64 oracle in tests)",
+                          "noise_floor_relmse": floor, "seconds": round(ref_s, 2),
+                          "note": "relMSE values have the reference's noise floor subtracted"},
+            "checkpoints": "every 8 spp to 128, then x1.25",
+            "guided_schedule": {**GUIDE, "train_spp": a.ttr_train_spp, "batch": 1 << 18, "lr": 1e-2},
             "plain": res["plain"], "guided": res["guided"], "speedup_guided_vs_plain": sp,
             "speedup_guided_vs_plain_interpolated": spc}
 
 
-def medium_fp8(wr, scenes, dev, a):
-    """The gap-lit room filled with a homogeneous medium (sigma_s 0.4, sigma_a 0.05
-    RGB-equal, HG g 0.7), max depth 32, med_spp samples per pixel: plain NEE+MIS,
-    and guided with BF16 and with FP8 (e4m3) network inference -- each guided run
-    trains its own network online (BF16) for the first med_train_spp samples
-    (8-spp passes, 32 Adam steps after each) and renders the rest with it.
-    relMSE against a plain render at med_ref_spp (two halves, whose difference
-    estimates the reference's own noise floor)."""
-    N = 34
-    sc = scenes.guide_room(4, resolution=(a.med_width, a.med_height)).with_medium(sigma_a=0.05, sigma_s=0.4, g=0.7)
+def config4(wr, scenes, dev, a):
+    """The learned-guiding workload at its own size: the gap-lit room at
+    1920x1080, max depth 16.  Guided renders 256 spp (the GUIDE schedule:
+    online training over the first 64); plain renders
+    progressively (x1.25 checkpoints) until it reaches the guided image's
+    relMSE, and its time is interpolated there.  relMSE against a plain
+    reference of c4_ref_spp with its noise floor subtracted; medians of c4_runs."""
+    import numpy as np
+    N = 18
+    sc = scenes.guide_room(4, resolution=(a.c4_width, a.c4_height))
     gs = wr.GpuScene(dev, sc)
+    ref, floor, ref_s = reference(wr, gs, a.c4_ref_spp, N, (8001, 8002))
+    print(f"# config-4 reference: {a.c4_ref_spp} spp in {ref_s:.1f}s, noise floor ~{floor:.2e}", file=sys.stderr,
+          flush=True)
+    g_t, g_e = [], []
+    for r in range(a.c4_runs):
+        render, g = guided_renderer(wr, gs, N, 300 + r)
+        film = wr.Film(gs)
+        t = time.perf_counter()
+        render(film, GUIDE["train_spp"])
+        render(film, 256 - GUIDE["train_spp"])
+        g_t.append(time.perf_counter() - t)
+        g_e.append(rel_mse(film.image(), ref) - floor)
+        film.close()
+        g.close()
+        print(f"# config-4 guided run {r}: 256 spp in {g_t[-1]:.3f} s, relMSE {g_e[-1]:.4e}", file=sys.stderr,
+              flush=True)
+    target = float(np.median(g_e))
+    points = checkpoints(64, a.c4_ref_spp // 2)
+    p_t, p_spp, p_curves = [], [], []
+    for r in range(a.c4_runs):
+        film = wr.Film(gs)
+        render = (lambda seed: lambda f, spp: wr.render_nee(gs, f, spp, num_path_vertices=N, seed=seed))(400 + r)
+        tt, tc, spp, curve = progressive(render, film, ref, floor, target, points)
+        film.close()
+        p_t.append(tc)
+        p_spp.append(spp)
+        p_curves.append(curve)
+        print(f"# config-4 plain run {r}: relMSE {target:.4e} at {tc if tc is None else round(tc, 3)} s "
+              f"(checkpoint {spp} spp)", file=sys.stderr, flush=True)
+    gs.close()
+    tg = float(np.median(g_t))
+    tp = float(np.median(p_t)) if all(x is not None for x in p_t) else None
+    return {"workload": f"guide_room seed 4 (1,003,550 tris, gap 0.2, L 500), {a.c4_width}x{a.c4_height}, max "
+                        f"depth 16, guided 256 spp (64 training) vs plain at equal relMSE",
+            "guided_schedule": {**GUIDE, "batch": 1 << 18, "lr": 1e-2},
+            "reference": {"spp": a.c4_ref_spp, "estimator": "plain GPU (pinned to the FP64 oracle in tests)",
+                          "noise_floor_relmse": floor, "seconds": round(ref_s, 2)},
+            "guided": {"seconds_median": round(tg, 3), "relmse_median": target, "seconds_runs": g_t,
+                       "relmse_runs": g_e},
+            "plain": {"seconds_to_equal_relmse_median": tp, "seconds_runs": p_t, "checkpoint_spp": p_spp,
+                      "curve_run0": [[c[0], c[1], float(f"{c[2]:.4e}")] for c in p_curves[0]]},
+            "speedup_guided_vs_plain": None if tp is None else round(tp / tg, 3)}
+
+
+def medium_fp8(wr, scenes, dev, a):
+    """The medium workload: the seed-5 batched-workload scene (716,836
+    triangles) filled with a homogeneous medium (sigma_s 0.4, sigma_a 0.05
+    RGB-equal, HG g 0.7), 1280x720, max depth 32, med_spp samples per pixel.
+    Plain NEE+MIS; one network trained online (BF16) over med_train_spp; then
+    the same trained network renders med_spp with BF16 and with FP8 (e4m3)
+    inference from the same random numbers (paired: their difference is the
+    precision effect, not noise).  relMSE against a plain reference of
+    med_ref_spp, noise floor subtracted."""
+    import numpy as np
+    N = 34
+    sc = scenes.medium_clutter(5, resolution=(a.med_width, a.med_height))
+    gs = wr.GpuScene(dev, sc)
+    # reference: the guided estimator (unbiased; its mean is pinned to the plain
+    # renderer's in the GPU tests) with its own network, as two independent halves
     t = time.perf_counter()
-    half = a.med_ref_spp // 2
-    fa, fb = wr.Film(gs), wr.Film(gs)
-    wr.render_nee(gs, fa, half, num_path_vertices=N, seed=7001)
-    wr.render_nee(gs, fb, half, num_path_vertices=N, seed=7002)
-    ia, ib = fa.image(), fb.image()
-    ref = 0.5 * (ia + ib)
+    halves = []
+    for k in range(2):
+        render, gref = guided_renderer(wr, gs, N, 7001 + k)
+        f = wr.Film(gs)
+        render(f, GUIDE["train_spp"])
+        render(f, a.med_ref_spp // 2 - GUIDE["train_spp"])
+        halves.append(f.image())
+        f.close()
+        gref.close()
+    ref = 0.5 * (halves[0] + halves[1])
+    floor = rel_mse(halves[0], halves[1]) / 4.0
     ref_s = time.perf_counter() - t
-    floor = rel_mse(ia, ib) / 4.0
-    print(f"# medium reference: {a.med_ref_spp} spp in {ref_s:.1f}s, noise floor ~{floor:.2e}", file=sys.stderr,
+    print(f"# medium reference: {a.med_ref_spp} guided spp in {ref_s:.1f}s, noise floor ~{floor:.2e}", file=sys.stderr,
           flush=True)
     res = {}
     film = wr.Film(gs)
     t = time.perf_counter()
     st = wr.render_nee(gs, film, a.med_spp, num_path_vertices=N, seed=7100)
     sec = time.perf_counter() - t
-    res["plain"] = {"seconds": round(sec, 3), "relmse": rel_mse(film.image(), ref),
+    res["plain"] = {"seconds": round(sec, 3), "relmse": rel_mse(film.image(), ref) - floor,
                     "mrays_per_s": round((st.extend_rays + st.shadow_rays) / sec / 1e6, 1)}
+    film.close()
+    g = wr.Guide(gs, learning_rate=1e-2, batch_size=1 << 18, record_capacity=1 << 20, seed=7)
+    film = wr.Film(gs)
+    t = time.perf_counter()
+    wr.render_guided(gs, film, g, a.med_train_spp, num_path_vertices=N, seed=7150,
+                     guide_fraction=GUIDE["guide_fraction"], max_guided_bounce=GUIDE["max_guided_bounce"],
+                     spp_per_pass=GUIDE["spp_per_pass"], train_steps=GUIDE["train_steps"], warmup_passes=1)
+    train_s = time.perf_counter() - t
+    film.close()
+    trained = g.get_params()
+    imgs = {}
     for name, fp8 in (("guided_bf16", False), ("guided_fp8", True)):
-        g = wr.Guide(gs, learning_rate=1e-2, batch_size=1 << 18, record_capacity=1 << 20, seed=7)
+        g.set_params(trained)
         film = wr.Film(gs)
         t = time.perf_counter()
-        k = min(a.med_train_spp, a.med_spp)
-        s1, _ = wr.render_guided(gs, film, g, k, num_path_vertices=N, seed=7200, guide_fraction=0.5, spp_per_pass=8,
-                                 train_steps=32, warmup_passes=1, fp8_inference=fp8)
-        rays = s1.extend_rays + s1.shadow_rays
-        if a.med_spp > k:
-            s2, _ = wr.render_guided(gs, film, g, a.med_spp - k, num_path_vertices=N, seed=7200, guide_fraction=0.5,
-                                     spp_per_pass=a.med_spp - k, train_steps=0, warmup_passes=0, fp8_inference=fp8)
-            rays += s2.extend_rays + s2.shadow_rays
+        st, _ = wr.render_guided(gs, film, g, a.med_spp, num_path_vertices=N, seed=7200,
+                                 guide_fraction=GUIDE["guide_fraction"], max_guided_bounce=GUIDE["max_guided_bounce"],
+                                 spp_per_pass=a.med_spp, train_steps=0, warmup_passes=0, fp8_inference=fp8)
         sec = time.perf_counter() - t
-        res[name] = {"seconds": round(sec, 3), "relmse": rel_mse(film.image(), ref),
-                     "mrays_per_s": round(rays / sec / 1e6, 1)}
-        g.close()
+        imgs[name] = film.image()
+        res[name] = {"seconds": round(sec, 3), "relmse": rel_mse(imgs[name], ref) - floor,
+                     "mrays_per_s": round((st.extend_rays + st.shadow_rays) / sec / 1e6, 1)}
         film.close()
+    g.close()
+    gs.close()
+    paired = rel_mse(imgs["guided_fp8"], imgs["guided_bf16"])
+    # the FP8 - BF16 relMSE gap per pixel, at common random numbers: its mean and
+    # standard error over pixels (the reference's own noise enters both terms
+    # alike and largely cancels)
+    den = ref * ref + 1e-2
+    dpx = (((imgs["guided_fp8"] - ref) ** 2 - (imgs["guided_bf16"] - ref) ** 2) / den).mean(-1).ravel()
+    gap, gap_se = float(dpx.mean()), float(dpx.std() / np.sqrt(dpx.size))
     for k, v in res.items():
         print(f"# medium {k}: {v['seconds']} s, relMSE {v['relmse']:.4e}, {v['mrays_per_s']} Mrays/s", file=sys.stderr,
               flush=True)
-    gs.close()
-    return {"workload": f"guide_room seed 4 (1,003,550 tris, gap 0.2, L 500) in a homogeneous medium "
+    print(f"# medium: relMSE(fp8 image, bf16 image) at the same random numbers {paired:.3e}; relMSE gap fp8 - bf16 "
+          f"{gap:.3e} +- {gap_se:.1e} (z = {gap / gap_se:.1f}); training {train_s:.2f} s", file=sys.stderr, flush=True)
+    return {"workload": f"seed-5 batched-workload scene ({sc.n_tris:,} tris) in a homogeneous medium "
                         f"(sigma_s 0.4, sigma_a 0.05, HG g 0.7), {a.med_width}x{a.med_height}, {a.med_spp} spp, "
                         f"max depth 32",
-            "reference": {"spp": a.med_ref_spp, "estimator": "plain GPU (pinned to the FP64 oracle in tests)",
-                          "noise_floor_relmse": floor, "seconds": round(ref_s, 2)},
-            "guided_schedule": {"train_spp": a.med_train_spp, "spp_per_pass": 8, "adam_steps_per_pass": 32,
-                                "training": "bf16"},
+            "reference": {"spp": a.med_ref_spp, "estimator": "guided GPU, two independent networks and halves "
+                                                              "(unbiased: pinned to the plain mean in the GPU tests, "
+                                                              "plain to the FP64 oracle)",
+                          "noise_floor_relmse": floor, "seconds": round(ref_s, 2),
+                          "note": "relMSE values have the reference's noise floor subtracted"},
+            "guided_schedule": {**GUIDE, "train_spp": a.med_train_spp, "training": "bf16",
+                                "training_seconds": round(train_s, 3),
+                                "inference": "the same trained network, the same random numbers, BF16 vs FP8"},
+            "fp8_vs_bf16_paired_relmse": paired,
+            "fp8_minus_bf16_relmse": {"gap": gap, "standard_error": gap_se, "z": gap / gap_se,
+                                      "note": "per-pixel difference of squared relative errors at common random "
+                                              "numbers, mean and standard error over pixels"},
             **res}
+
+
+def batched_small(wr, scenes, dev):
+    """The same generator restricted to small jobs (32 scenes, 128^2 or 256^2,
+    16..64 spp, 10k..200k triangles): single renders of these leave the GPU
+    underfilled, which one batch does not.  Batch vs one scene at a time with
+    the same autotuned scheduling, every scene warmed up first."""
+    work = scenes.batch_workload(seed=3, n_scenes=32, resolutions=(128, 256), spp_range=(16, 64),
+                                 tri_range=(10_000, 200_000))
+    gsc = [wr.GpuScene(dev, sc) for sc, _, _ in work]
+    jobs = [dict(scene=g, spp=spp, num_path_vertices=N, seed=2000 + i) for i, (g, (sc, spp, N)) in
+            enumerate(zip(gsc, work))]
+    tuning, _ = wr.autotune(jobs)
+    for j in jobs:
+        j["film"] = wr.Film(j["scene"])
+    wr.render_batch(jobs, tuning=tuning)
+    for j in jobs:
+        j["film"].clear()
+    t0 = time.perf_counter()
+    st = wr.render_batch(jobs, tuning=tuning)
+    t_batch = time.perf_counter() - t0
+    t_ind, rel = 0.0, []
+    for j in jobs:
+        f = wr.Film(j["scene"])
+        wr.render_nee(j["scene"], f, 1, num_path_vertices=j["num_path_vertices"], seed=j["seed"], tuning=tuning)
+        f.clear()
+        t0 = time.perf_counter()
+        wr.render_nee(j["scene"], f, j["spp"], num_path_vertices=j["num_path_vertices"], seed=j["seed"], tuning=tuning)
+        t_ind += time.perf_counter() - t0
+        rel.append(rel_mse(j["film"].image(), f.image()))
+        f.close()
+        j["film"].close()
+        j["scene"].close()
+    rays = st.extend_rays + st.shadow_rays
+    print(f"# small-job batch: {len(jobs)} scenes in {t_batch * 1e3:.1f} ms ({rays / t_batch / 1e6:.0f} Mrays/s), "
+          f"individually {t_ind * 1e3:.1f} ms; max relMSE(batch, single) {max(rel):.2e}", file=sys.stderr, flush=True)
+    return {"workload": "32 seeded Cornell boxes with icosphere clutter (scene i seed 2000+i), 128^2 or 256^2, "
+                        "16..64 spp, 10k..200k triangles, max depth 8..16",
+            "seconds": round(t_batch, 4), "individual_seconds": round(t_ind, 4),
+            "mrays_per_s": round(rays / t_batch / 1e6, 1), "speedup_vs_individual": round(t_ind / t_batch, 3),
+            "max_relmse_vs_individual": max(rel)}
 
 
 def batched(wr, scenes, dev, a):
@@ -280,11 +459,16 @@ def batched(wr, scenes, dev, a):
     print(f"# batch: {len(jobs)} scenes, {tris} tris, {st.paths} paths in {t_batch:.2f}s "
           f"({rays_b / t_batch / 1e6:.0f} Mrays/s); build {build_s:.1f}s, autotune {tune_s:.1f}s", file=sys.stderr,
           flush=True)
+    # one scene at a time, with the batch's scheduling parameters; each scene is
+    # warmed up first (its path pool allocated, as the autotuner did for the batch)
     t_ind, rays_i, rel = 0.0, 0, []
     for j in jobs:
         f = wr.Film(j["scene"])
+        wr.render_nee(j["scene"], f, 1, num_path_vertices=j["num_path_vertices"], seed=j["seed"], tuning=tuning)
+        f.clear()
         t0 = time.perf_counter()
-        s1 = wr.render_nee(j["scene"], f, j["spp"], num_path_vertices=j["num_path_vertices"], seed=j["seed"])
+        s1 = wr.render_nee(j["scene"], f, j["spp"], num_path_vertices=j["num_path_vertices"], seed=j["seed"],
+                           tuning=tuning)
         t_ind += time.perf_counter() - t0
         rays_i += s1.extend_rays + s1.shadow_rays
         rel.append(rel_mse(j["film"].image(), f.image()))
@@ -302,7 +486,12 @@ def batched(wr, scenes, dev, a):
             "autotuned": {"max_paths_in_flight": tuning.max_paths_in_flight, "fetch_chunk": tuning.fetch_chunk,
                           "refill_lanes": tuning.refill_lanes, "use_graph": bool(tuning.use_graph),
                           "seconds": round(tune_s, 2)},
-            "per_scene_relmse_vs_individual": [float(f"{x:.3e}") for x in rel]}
+            "per_scene_relmse_vs_individual": [float(f"{x:.3e}") for x in rel],
+            "note": "one scene at a time uses the batch's autotuned scheduling and is warmed up first (pool "
+                    "allocated), as the batch is by the autotuner; the large jobs fill the GPU by themselves, so "
+                    "this mix gains little, the small-job batch below shows the batch where single renders cannot "
+                    "fill the GPU",
+            "small_jobs": batched_small(wr, scenes, dev)}
 
 
 def cpu_baseline(sc, N, target_s):
@@ -421,34 +610,54 @@ def main():
     ext_ms = sum(s.extend_ms for s in stats)
     ext_launches = sum(s.extend_launches for s in stats)
     shd_ms = sum(s.shadow_ms for s in stats)
-    print(f"# timed {a.steps} steps in {wall:.3f}s: ext {ext} shd {shd} rays; extend {ext_ms:.1f} ms "
+    print(f"# timed {a.steps} steps in {wall:.3f}s: ext {ext} shd {shd} rays in {ext_launches} / "
+          f"{sum(s.shadow_launches for s in stats)} launches; extend {ext_ms:.1f} ms "
           f"shadow {shd_ms:.1f} ms shade {sum(s.shade_ms for s in stats):.1f} ms "
           f"generate {sum(s.generate_ms for s in stats):.1f} ms", file=sys.stderr, flush=True)
-    # roofline of the closest-hit traversal kernel (algorithmic bytes per launch / mean launch time)
+    # roofline of the closest-hit traversal kernel: HBM-side bytes per launch
+    # (rocprof FETCH_SIZE / WRITE_SIZE of this build on this workload, per ray,
+    # times the live rays per launch) over the live mean launch time
     roof = None
     if not a.no_count:
         f2 = wr.Film(gs)
         cs = wr.render_nee(gs, f2, a.spp, num_path_vertices=N, seed=1, first_sample=0, count_traversal=True)
+        f2.close()
         nodes_per_ray = cs.extend_nodes / max(cs.extend_rays, 1)
         tris_per_ray = cs.extend_tris / max(cs.extend_rays, 1)
-        bytes_total = ext * (EXT_RAY_BYTES + nodes_per_ray * NODE_BYTES + tris_per_ray * TRI_BYTES)
-        per_launch_bytes = bytes_total / ext_launches
+        shd_nodes = cs.shadow_nodes / max(cs.shadow_rays, 1)
+        shd_tris = cs.shadow_tris / max(cs.shadow_rays, 1)
+        ext_bpr = EXT_RAY_BYTES + nodes_per_ray * NODE_BYTES + tris_per_ray * TRI_BYTES
+        shd_bpr = SHD_RAY_BYTES + shd_nodes * NODE_BYTES + shd_tris * TRI_BYTES
         avg_ms = ext_ms / ext_launches
-        achieved = per_launch_bytes / (avg_ms * 1e-3) / 1e9
-        traffic = None
+        rays_per_launch = ext / ext_launches
+        shd_launches = sum(s.shadow_launches for s in stats)
+        shd_avg_ms = shd_ms / shd_launches
+        shd_per_launch = shd / shd_launches
         tp = os.path.join(ROOT, "profiles", "extend_traffic.json")
-        if os.path.exists(tp):
-            traffic = json.load(open(tp)).get("hbm_bytes_per_launch")
-        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                # algorithmic bytes count node reads the L2 / MALL serve, so frac can pass 1;
-                # the DRAM-side rate of the same launches (rocprof counters):
-                "traffic_gbs": None if traffic is None else round(traffic / (avg_ms * 1e-3) / 1e9, 1),
-                "traffic_frac": None if traffic is None else round(traffic / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                "kernel": "k_extend_fast (closest-hit BVH4 traversal)",
-                "avg_launch_ms": round(avg_ms, 4), "launches": ext_launches,
-                "nodes_per_ray": round(nodes_per_ray, 2), "tris_per_ray": round(tris_per_ray, 2),
-                "bytes_per_ray": round(EXT_RAY_BYTES + nodes_per_ray * NODE_BYTES + tris_per_ray * TRI_BYTES, 1)}
+        prof = json.load(open(tp)) if os.path.exists(tp) else {}
+        roof = {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None, "traffic": None,
+                "kernel": "k_extend_fast (closest-hit BVH4 traversal)", "avg_launch_ms": round(avg_ms, 4),
+                "launches": ext_launches, "rays_per_launch": round(rays_per_launch),
+                "traffic_source": "profiles/extend_traffic.json: HBM bytes per extension ray from rocprofv3 FETCH_SIZE "
+                                  "and WRITE_SIZE of the same build on the same workload, scaled by a 1 GiB copy"}
+        if prof.get("extend_hbm_bytes_per_ray"):
+            traffic = prof["extend_hbm_bytes_per_ray"] * rays_per_launch
+            achieved = traffic / (avg_ms * 1e-3) / 1e9
+            roof.update(achieved=round(achieved, 1), frac=round(achieved / HBM_PEAK_GBS, 4), traffic=round(traffic))
+        # the cache-inclusive rate: every visited node and tested triangle counted
+        # as read (the L2 and MALL serve most of them), so it can pass the HBM peak
+        alg = rays_per_launch * ext_bpr / (avg_ms * 1e-3) / 1e9
+        roof["algorithmic_incl_cache"] = {
+            "bytes_per_ray": round(ext_bpr, 1), "nodes_per_ray": round(nodes_per_ray, 2),
+            "tris_per_ray": round(tris_per_ray, 2), "gbs": round(alg, 1), "frac_of_hbm_peak": round(alg / HBM_PEAK_GBS, 4)}
+        shadow = {"kernel": "k_shadow (any-hit BVH4 traversal)", "avg_launch_ms": round(shd_avg_ms, 4),
+                  "rays_per_launch": round(shd_per_launch), "nodes_per_ray": round(shd_nodes, 2),
+                  "algorithmic_incl_cache_gbs": round(shd_per_launch * shd_bpr / (shd_avg_ms * 1e-3) / 1e9, 1)}
+        if prof.get("shadow_hbm_bytes_per_ray"):
+            st_ = prof["shadow_hbm_bytes_per_ray"] * shd_per_launch
+            shadow.update(traffic=round(st_), achieved=round(st_ / (shd_avg_ms * 1e-3) / 1e9, 1),
+                          frac=round(st_ / (shd_avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4))
+        roof["shadow"] = shadow
     others = None
     if not a.no_integrators:
         others = other_integrators(wr, scenes, dev, gs, sc, N, a)
@@ -458,6 +667,9 @@ def main():
     ttr = None
     if not a.no_ttr:
         ttr = time_to_relmse(wr, scenes, dev, a)
+    c4 = None
+    if not a.no_config4:
+        c4 = config4(wr, scenes, dev, a)
     med = None
     if not a.no_medium:
         med = medium_fp8(wr, scenes, dev, a)
@@ -488,6 +700,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "time_to_relmse": ttr,
+        "guided_1080p_equal_relmse": c4,
         "batched": bat,
         "medium_bf16_fp8": med,
         "bvh_build": bvh,

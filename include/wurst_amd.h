@@ -401,6 +401,10 @@ typedef struct {
                                   is sampled from pass warmup_passes on, trained or not */
     int32_t fp8_inference;     /* 1: the renderer's network inference runs the e4m3 variant
                                   (wr_guide_forward_fp8); training stays BF16 */
+    uint32_t max_guided_bounce; /* vertices at bounce < this (0 = the camera ray's hit) are guided
+                                   and recorded; deeper ones sample the BSDF or phase function
+                                   only (probability 0 of the network, so still unbiased).
+                                   0 = every bounce */
 } wr_guided_params;
 
 typedef struct {

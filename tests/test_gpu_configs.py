@@ -18,7 +18,8 @@ properties that do not depend on size.
     code; a Cornell box gives 50x and 45%, the open icosphere scene 48000x and
     99%), where a systematic error would leave almost no pixel equal;
   - gap-lit room (1,003,550 triangles, 1920x1080, max depth 16): guided at
-    256 spp, trained online for the first 128, has the plain renderer's mean
+    256 spp with the benchmarked schedule (first 5 bounces guided, online
+    training over the first 64 spp), has the plain renderer's mean
     (per-pixel differences average to zero within 5 standard errors), a finite
     non-negative image, and the trained network's loss gradient on records
     built from the room's own camera-ray hits matches the FP64 oracle
@@ -159,16 +160,17 @@ def test_gap_room_full_size_guided(gpu_device):
     gs = wr.GpuScene(gpu_device, sc)
     g = wr.Guide(gs, learning_rate=1e-2, batch_size=1 << 18, record_capacity=1 << 20, seed=7)
     film = wr.Film(gs)
-    _, gst = wr.render_guided(gs, film, g, 128, num_path_vertices=18, seed=51, guide_fraction=0.5, spp_per_pass=8,
-                              train_steps=32, warmup_passes=1)
-    wr.render_guided(gs, film, g, 128, num_path_vertices=18, seed=51, guide_fraction=0.5, spp_per_pass=64,
-                     train_steps=0, warmup_passes=0)
+    # the benchmarked schedule: the first 5 bounces guided with probability 0.4,
+    # online training over the first 64 spp (64 Adam steps per 8-spp pass)
+    kw = dict(num_path_vertices=18, seed=51, guide_fraction=0.4, max_guided_bounce=5)
+    _, gst = wr.render_guided(gs, film, g, 64, spp_per_pass=8, train_steps=64, warmup_passes=1, **kw)
+    wr.render_guided(gs, film, g, 192, spp_per_pass=64, train_steps=0, warmup_passes=0, **kw)
     guided = film.image()
     pf = wr.Film(gs)
     wr.render_nee(gs, pf, 256, num_path_vertices=18, seed=52)
     plain = pf.image()
     print(f"  trained: {gst.train_steps} Adam steps on {gst.records} records, loss {gst.last_loss:.4f}", flush=True)
-    assert gst.train_steps == 16 * 32 and np.isfinite(gst.last_loss)
+    assert gst.train_steps == 8 * 64 and np.isfinite(gst.last_loss)
     assert np.isfinite(guided).all() and guided.min() >= 0
     ok, m, se = _mean_agrees(guided, plain)
     print(f"  mean guided - plain {m} (standard error {se}); image mean {plain.mean((0, 1))}", flush=True)

@@ -284,6 +284,34 @@ def test_guided_render_is_unbiased_with_any_network(gpu_device, fp8):
     assert np.all(np.abs(diff) < 4 * sig + 1e-6), (diff, sig)
 
 
+@pytest.mark.parametrize("max_bounce", [1, 3])
+def test_guided_render_with_bounce_limit_is_unbiased(gpu_device, max_bounce):
+    """Guiding only the first max_bounce vertices (deeper ones sample the BSDF
+    with probability 1) changes the noise, not the mean; deeper vertices take
+    no network draw, so they record no training data either."""
+    sc = scenes.cornell(1, resolution=(48, 48))
+    gs = wr.GpuScene(gpu_device, sc)
+    print(f"guided render, random network, first {max_bounce} bounces: cornell 48x48", flush=True)
+    g = wr.Guide(gs, seed=5)
+    spp = 1024
+    kw = dict(guide_fraction=0.5, spp_per_pass=spp, train_steps=0, warmup_passes=0, max_guided_bounce=max_bounce)
+    plain, _, _ = _render(gs, spp, 10, seed=25)
+    plain2, _, _ = _render(gs, spp, 10, seed=26)
+    guided, st, _ = _render(gs, spp, 10, seed=27, guide=g, **kw)
+    guided2, _, _ = _render(gs, spp, 10, seed=28, guide=g, **kw)
+    assert np.isfinite(guided).all() and st.extend_rays > 0
+    npx = guided.shape[0] * guided.shape[1]
+    sig = np.sqrt(((guided - guided2) ** 2).mean((0, 1)) / 2 / npx + ((plain - plain2) ** 2).mean((0, 1)) / 2 / npx)
+    diff = guided.mean((0, 1)) - plain.mean((0, 1))
+    print(f"  mean plain {plain.mean((0, 1))} guided {guided.mean((0, 1))} sigma {sig}")
+    assert np.all(np.abs(diff) < 4 * sig + 1e-6), (diff, sig)
+    # the limit takes effect: at the same random numbers the image differs from
+    # an unlimited guide's (deeper vertices draw from the BSDF instead)
+    if max_bounce == 1:
+        full, _, _ = _render(gs, spp, 10, seed=27, guide=g, **dict(kw, max_guided_bounce=0))
+        assert np.abs(guided - full).max() > 0
+
+
 def test_guided_render_trains_and_stays_unbiased(gpu_device):
     sc = scenes.guide_room(4, n_spheres=6, level=3, resolution=(64, 48))
     gs = wr.GpuScene(gpu_device, sc)

@@ -82,7 +82,7 @@ class GuideConfig(C.Structure):
 
 class GuidedParams(C.Structure):
     _fields_ = [("guide_fraction", C.c_float), ("spp_per_pass", C.c_uint32), ("train_steps", C.c_uint32),
-                ("warmup_passes", C.c_uint32), ("fp8_inference", C.c_int32)]
+                ("warmup_passes", C.c_uint32), ("fp8_inference", C.c_int32), ("max_guided_bounce", C.c_uint32)]
 
 
 class GuideStats(C.Structure):
@@ -611,7 +611,8 @@ class Guide:
 def render_guided(scene: GpuScene, film: Film, guide: Guide, spp: int, num_path_vertices: int = 10, seed: int = 0,
                   guide_fraction: float = 0.5, spp_per_pass: int = 4, train_steps: int = 4, warmup_passes: int = 1,
                   use_mis: bool = True, first_sample: int | None = None, max_paths_in_flight: int = 0,
-                  profile: bool = False, fp8_inference: bool = False, use_graph: bool = False):
+                  profile: bool = False, fp8_inference: bool = False, use_graph: bool = False,
+                  max_guided_bounce: int = 0):
     """wr_render_guided: progressive passes of spp_per_pass, training between passes;
     fp8_inference runs the renderer's network inference in e4m3; use_graph replays
     each pass's iterations as a captured hipGraph."""
@@ -619,7 +620,8 @@ def render_guided(scene: GpuScene, film: Film, guide: Guide, spp: int, num_path_
         first_sample = film.spp
     p = RenderParams(spp, num_path_vertices, seed, int(bool(use_mis)), first_sample, max_paths_in_flight,
                      int(bool(profile)), 0, int(bool(use_graph)))
-    gp = GuidedParams(guide_fraction, spp_per_pass, train_steps, warmup_passes, int(bool(fp8_inference)))
+    gp = GuidedParams(guide_fraction, spp_per_pass, train_steps, warmup_passes, int(bool(fp8_inference)),
+                      max_guided_bounce)
     st, gs = RenderStats(), GuideStats()
     _check(lib().wr_render_guided(scene.h, film.h, guide.h, C.byref(p), C.byref(gp), C.byref(st), C.byref(gs)))
     film.spp += spp

@@ -137,6 +137,7 @@ struct Shade {
     float alpha;        // one-sample MIS probability of sampling the network (0: BSDF only)
     float record_prob;  // fraction of paths whose vertices are recorded
     int collect;        // record training data
+    uint32_t max_bounce;  // vertices at bounce >= this are neither guided nor recorded
 };
 
 // Hardware exp2 / log2 (v_exp_f32 / v_log_f32, about 1 ulp) for the network's

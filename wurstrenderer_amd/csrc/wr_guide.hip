@@ -641,13 +641,15 @@ int guide_pass_net(wr_guide *g, int set, guide::Net *net) {
     return WR_OK;
 }
 
-int guide_shade_view(wr_guide *g, int set, float alpha, float record_prob, bool collect, guide::Shade *out) {
+int guide_shade_view(wr_guide *g, int set, float alpha, float record_prob, bool collect, uint32_t max_bounce,
+                     guide::Shade *out) {
     out->draw.dir = g->d_draw_dir;
     out->draw.q_nee = g->d_draw_qnee;
     out->rec = g->recs[set];
     out->alpha = alpha;
     out->record_prob = record_prob;
     out->collect = collect ? 1 : 0;
+    out->max_bounce = max_bounce;
     return WR_OK;
 }
 

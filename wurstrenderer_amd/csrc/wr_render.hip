@@ -720,32 +720,43 @@ __device__ __forceinline__ float guide_alpha(const DMat &m, float frac) {
 #ifndef WR_INFER_WAVES
 #define WR_INFER_WAVES 2
 #endif
+#ifndef WR_INFER_LDS_WEIGHTS
+#define WR_INFER_LDS_WEIGHTS 1
+#endif
 constexpr int kInferBlock = 256;  // 4 waves share one LDS copy of the weights
 // FP8: the e4m3 variant of the network (guide::forward8).
 template <bool MEDIUM, bool FP8>
 __global__ __launch_bounds__(kInferBlock) __attribute__((amdgpu_waves_per_eu(WR_INFER_WAVES, WR_INFER_WAVES))) void k_guide_infer(
     DScene s, PathState cs, const uint2 *__restrict__ hit, const uint32_t *__restrict__ counters, int N, guide::Net gnet,
-    float frac, guide::Draw out) {
+    float frac, uint32_t max_bounce, guide::Draw out) {
     using OpT = typename std::conditional<FP8, guide::u32x2, guide::u32x4>::type;
+#if WR_INFER_LDS_WEIGHTS
     __shared__ OpT w_ops[guide::kFwdOps];
+#endif
     __shared__ float w_bias[guide::kBiasPacked];
     __shared__ float w_scale[guide::kLayers];
     __shared__ float stage_all[kInferBlock * guide::kStageStride];
     const uint32_t n = active_count(counters);
     if (blockIdx.x * kInferBlock >= n) return;
+#if WR_INFER_LDS_WEIGHTS
     const OpT *src;
     if constexpr (FP8) src = gnet.fwd8;
     else src = gnet.fwd;
     for (int j = threadIdx.x; j < guide::kFwdOps; j += kInferBlock) w_ops[j] = src[j];
+#endif
     for (int j = threadIdx.x; j < guide::kBiasPacked; j += kInferBlock) w_bias[j] = gnet.bias[j];
     if (FP8 && threadIdx.x < guide::kLayers) w_scale[threadIdx.x] = gnet.wscale[threadIdx.x];
     __syncthreads();
     guide::Net net = gnet;
     if constexpr (FP8) {
+#if WR_INFER_LDS_WEIGHTS
         net.fwd8 = w_ops;
+#endif
         net.wscale = w_scale;
     } else {
+#if WR_INFER_LDS_WEIGHTS
         net.fwd = w_ops;
+#endif
     }
     net.bias = w_bias;
     float *stage = stage_all + (threadIdx.x & ~63u) * guide::kStageStride;
@@ -759,7 +770,7 @@ __global__ __launch_bounds__(kInferBlock) __attribute__((amdgpu_waves_per_eu(WR_
         if (i < n) {
             const uint2 h = hit[i];
             meta = cs.meta[i];
-            if (int(meta.z) + 2 != N) {
+            if (int(meta.z) + 2 != N && meta.z < max_bounce) {
                 float tf = 0.f;
                 med = MEDIUM && free_flight(s.med, meta.x, meta.z, h.x != kInvalid ? __uint_as_float(h.y) : INFINITY, tf);
                 if (med) {
@@ -873,7 +884,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade_ext(DScene s, PathBuffers
         const bool surf = hit_vertex && gi.y < 0 && !last;
         const bool med = valid && in_medium && !last;
         const float3 nf = in_medium ? -rd : (dot3(ng, rd) > 0.f ? -ng : ng);  // faces the incoming ray
-        const bool need = GUIDED && (surf || med) && g.alpha > 0.f;
+        const bool need = GUIDED && (surf || med) && g.alpha > 0.f && meta.z < g.max_bounce;
         float4 gdir = make_float4(0.f, 0.f, 0.f, 0.f);  // k_guide_infer's direction and q there
         float q_nee = 0.f;
         if (GUIDED && need) {
@@ -1007,7 +1018,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade_ext(DScene s, PathBuffers
         }
         uint32_t rec_id = 0;
         if (GUIDED && g.collect) {
-            const bool want = cont && float(mix32(key ^ 0x68bc21ebu) >> 8) * 0x1p-24f < g.record_prob;
+            const bool want = cont && bounce < g.max_bounce && float(mix32(key ^ 0x68bc21ebu) >> 8) * 0x1p-24f < g.record_prob;
             const uint32_t r = block_append(g.rec.count, want, agg);
             if (want && r < g.rec.capacity) {
                 g.rec.pos[r] = make_float4(pos.x, pos.y, pos.z, __uint_as_float(meta.y));
@@ -2385,7 +2396,8 @@ static int render_impl(wr_scene *s, wr_film *f, const wr_render_params *prm, wr_
             if (!rc && collect) rc = guide_begin_pass(g, set, a.total, &record_prob);
             if (rc) return rc;
             const bool sample_net = pass >= int(gp->warmup_passes);
-            rc = guide_shade_view(g, set, sample_net ? gp->guide_fraction : 0.f, record_prob, collect, &gs);
+            rc = guide_shade_view(g, set, sample_net ? gp->guide_fraction : 0.f, record_prob, collect,
+                                  gp->max_guided_bounce ? gp->max_guided_bounce : 0xFFFFFFFFu, &gs);
             if (rc) return rc;
         }
         k_init<<<1, 64, 0, s->stream>>>(pb, a);
@@ -2408,16 +2420,16 @@ static int render_impl(wr_scene *s, wr_film *f, const wr_render_params *prm, wr_
             if (g && gs.alpha > 0.f) {
                 if (medium && fp8)
                     k_guide_infer<true, true><<<g_infer, kInferBlock, 0, s->stream>>>(ds, q.cur, q.hit, q.counters, N,
-                                                                                      net, gs.alpha, gs.draw);
+                                                                                      net, gs.alpha, gs.max_bounce, gs.draw);
                 else if (medium)
                     k_guide_infer<true, false><<<g_infer, kInferBlock, 0, s->stream>>>(ds, q.cur, q.hit, q.counters, N,
-                                                                                       net, gs.alpha, gs.draw);
+                                                                                       net, gs.alpha, gs.max_bounce, gs.draw);
                 else if (fp8)
                     k_guide_infer<false, true><<<g_infer, kInferBlock, 0, s->stream>>>(ds, q.cur, q.hit, q.counters, N,
-                                                                                       net, gs.alpha, gs.draw);
+                                                                                       net, gs.alpha, gs.max_bounce, gs.draw);
                 else
                     k_guide_infer<false, false><<<g_infer, kInferBlock, 0, s->stream>>>(ds, q.cur, q.hit, q.counters, N,
-                                                                                        net, gs.alpha, gs.draw);
+                                                                                        net, gs.alpha, gs.max_bounce, gs.draw);
             }
             if (g && medium) k_shade_ext<true, true><<<g_shade, kShadeBlock, 0, s->stream>>>(ds, q, a, gs);
             else if (g) k_shade_ext<true, false><<<g_shade, kShadeBlock, 0, s->stream>>>(ds, q, a, gs);

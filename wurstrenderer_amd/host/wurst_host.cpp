@@ -256,8 +256,8 @@ std::shared_ptr<PathNeeRenderer> PathNeeRenderer::FromSimple2Json(const std::sha
     r->seed = uint32_t(json.number("seed", 0));
     r->use_mis = json.boolean("use_mis", true);
     // "guiding": {"guide_fraction", "spp_per_pass", "train_steps", "warmup_passes",
-    //             "learning_rate", "batch_size", "record_capacity", "seed",
-    //             "inference": "bf16" | "fp8"} (all optional)
+    //             "max_guided_bounce", "learning_rate", "batch_size", "record_capacity",
+    //             "seed", "inference": "bf16" | "fp8"} (all optional)
     if (const wj::Value *gj = json.find("guiding")) {
         if (gj->type != wj::Value::Object) throw std::runtime_error("\"guiding\" must be an object");
         r->guided = gj->boolean("enabled", true);
@@ -270,6 +270,7 @@ std::shared_ptr<PathNeeRenderer> PathNeeRenderer::FromSimple2Json(const std::sha
         r->guide_params.spp_per_pass = uint32_t(gj->number("spp_per_pass", 4));
         r->guide_params.train_steps = uint32_t(gj->number("train_steps", 4));
         r->guide_params.warmup_passes = uint32_t(gj->number("warmup_passes", 1));
+        r->guide_params.max_guided_bounce = uint32_t(gj->number("max_guided_bounce", 0));
         const std::string inference = gj->string("inference", "bf16");
         if (inference != "bf16" && inference != "fp8") throw std::runtime_error("guiding.inference must be bf16 or fp8");
         r->guide_params.fp8_inference = inference == "fp8" ? 1 : 0;
