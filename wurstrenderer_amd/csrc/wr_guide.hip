@@ -108,11 +108,14 @@ struct TrainArgs {
     float *loss;                        // [gridDim.x]
 };
 
-// Four waves per block (one per SIMD of a CU), each on its own tiles of 64
-// records, sharing one LDS gradient accumulator.  A wave keeps every layer's
-// input operands in registers (it has a SIMD's whole register file), so LDS
-// holds only the current layer's transposed tiles, which the loss stage
-// aliases: 4 x 18 KiB + 67 KiB of accumulator.
+// Four waves per block (one per SIMD of a CU) step through the records in
+// lockstep, a tile of 64 per wave.  A wave keeps every layer's input operands
+// in registers (it has a SIMD's whole register file), so LDS holds only the
+// current layer's transposed tiles, which the loss stage aliases, and the
+// block's gradient accumulator: 4 x 18 KiB + 67 KiB.  Each 32 x 32 tile of a
+// weight gradient belongs to one wave, which reduces it over all four waves'
+// staged tiles and adds it to the accumulator with plain loads and stores:
+// one writer per address, no LDS atomics.
 constexpr int kTrainWaves = 4;
 struct TrainLds {
     union {
@@ -140,20 +143,22 @@ __device__ __forceinline__ void store_xt(uint16_t *xt, const u32x4 (*x)[2]) {
                 base[kperm(ks, 0, e) * kXt + 32 * nt] = uint16_t(x[ks][nt][e >> 1] >> (16 * (e & 1)));
 }
 
-// Backward through layer l of one tile.  dY_l arrives as packed BF16 operands
-// (dy[ks][nt] element e = dY of output kperm(ks, h, e) of record 32 nt + p, the
-// values every product below uses); the layer's input operands are in x.
-// Adds dW_l and db_l into gacc and, for l > 0, leaves dY_{l-1} (the input's
-// ReLU mask applied) packed in dy.
+// Backward through layer l of the block's four tiles (one per wave).  dY_l
+// arrives as packed BF16 operands (dy[ks][nt] element e = dY of output
+// kperm(ks, h, e) of record 32 nt + p, the values every product below uses);
+// the layer's input operands are in x.  Adds dW_l and db_l into gacc and, for
+// l > 0, leaves dY_{l-1} (the input's ReLU mask applied) packed in dy.  Every
+// wave of the block calls it together (block barriers inside).
 template <int l>
-__device__ __forceinline__ void backward_layer(const Net &net, TrainLds &L, float *gacc, const u32x4 (*x)[2],
-                                               u32x4 dy[4][2]) {
+__device__ __forceinline__ void backward_layer(const Net &net, TrainLds *wl, float *gacc, float (*bpart)[64],
+                                               const u32x4 (*x)[2], u32x4 dy[4][2]) {
     constexpr int K = layer_in(l), N = layer_out(l);
-    const int lane = __lane_id(), p = lane & 31, h = lane >> 5;
+    const int lane = __lane_id(), p = lane & 31, h = lane >> 5, wave = threadIdx.x >> 6;
+    TrainLds &L = wl[wave];
     store_xt<4>(L.t.dyt, dy);
     store_xt<K / 16>(L.t.xt, x);
     wave_sync_lds();
-    // bias gradient: row sums of dY^T
+    // this wave's share of the bias gradient: row sums of its dY^T
     if (lane < N) {
         float s[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -162,28 +167,32 @@ __device__ __forceinline__ void backward_layer(const Net &net, TrainLds &L, floa
 #pragma unroll
             for (int e = 0; e < 8; e++) s[e & 3] += op_elem(v, e);
         }
-        atomicAdd(&gacc[b_offset(l) + lane], (s[0] + s[1]) + (s[2] + s[3]));
+        bpart[wave][lane] = (s[0] + s[1]) + (s[2] + s[3]);
     }
-    // weight gradient dW[i][o] = sum_r X[r][i] dY[r][o]: MFMA over the records
+    __syncthreads();
+    // weight gradient dW[i][o] = sum_r X[r][i] dY[r][o] over the block's 256
+    // records: output tile (mt, nt) = (wave / 2, wave % 2)
+    constexpr int kTiles = 2 * (K / 32);
+    if (wave < kTiles) {
+        const int mt = wave >> 1, nt = wave & 1;
+        f32x16 acc = f32x16{};
 #pragma unroll
-    for (int mt = 0; mt < K / 32; mt++)
-#pragma unroll
-        for (int nt = 0; nt < 2; nt++) {
-            if (32 * nt >= N) continue;
-            f32x16 acc = f32x16{};
+        for (int w2 = 0; w2 < kTrainWaves; w2++)
 #pragma unroll
             for (int ks = 0; ks < 4; ks++) {
-                const u32x4 av = *reinterpret_cast<const u32x4 *>(L.t.xt + (32 * mt + p) * kXt + 16 * ks + 8 * h);
-                const u32x4 bv = *reinterpret_cast<const u32x4 *>(L.t.dyt + (32 * nt + p) * kXt + 16 * ks + 8 * h);
+                const u32x4 av = *reinterpret_cast<const u32x4 *>(wl[w2].t.xt + (32 * mt + p) * kXt + 16 * ks + 8 * h);
+                const u32x4 bv = *reinterpret_cast<const u32x4 *>(wl[w2].t.dyt + (32 * nt + p) * kXt + 16 * ks + 8 * h);
                 acc = mfma(av, bv, acc);
             }
-            const int o = 32 * nt + p;
-            if (o < N) {
-                float *gb = gacc + w_offset(l) + 4 * h * N + p;
+        if (32 * nt + p < N) {
+            float *gb = gacc + w_offset(l) + (32 * mt + 4 * h) * N + 32 * nt + p;
 #pragma unroll
-                for (int v = 0; v < 16; v++) atomicAdd(&gb[(32 * mt + drow(v, 0)) * N + 32 * nt], acc[v]);
-            }
+            for (int v = 0; v < 16; v++) gb[drow(v, 0) * N] += acc[v];
         }
+    }
+    // bias: the wave with no weight tile for layer 0, else one per layer in turn
+    if (wave == (kTiles == 2 ? 2 : l & 3) && lane < N)
+        gacc[b_offset(l) + lane] += (bpart[0][lane] + bpart[1][lane]) + (bpart[2][lane] + bpart[3][lane]);
     if constexpr (l > 0) {
         // backward data: dX^T = W_l dY^T; the input's ReLU mask sits in the same lane
         // and slot as the result, which is packed into dy for the layer below
@@ -214,7 +223,7 @@ __device__ __forceinline__ void backward_layer(const Net &net, TrainLds &L, floa
 #pragma unroll
             for (int nt = 0; nt < 2; nt++) dy[ks][nt] = nd[ks][nt];
     }
-    wave_sync_lds();
+    __syncthreads();  // every wave's tiles are read before the next layer (or record tile) overwrites them
 }
 
 // One hidden layer of the wave's 64 records: input operands x (K inputs) ->
@@ -232,6 +241,7 @@ __device__ __forceinline__ void layer_to(const Net &net, int l, const u32x4 (*x)
 __global__ __launch_bounds__(64 * kTrainWaves) void k_train(Net net, TrainArgs a) {
     __shared__ __attribute__((aligned(16))) TrainLds wl[kTrainWaves];
     __shared__ float gacc[kNumParams];
+    __shared__ float bpart[kTrainWaves][64];
     __shared__ float wave_loss[kTrainWaves];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, p = lane & 31, h = lane >> 5;
     for (int j = threadIdx.x; j < kNumParams; j += blockDim.x) gacc[j] = 0.f;
@@ -239,8 +249,10 @@ __global__ __launch_bounds__(64 * kTrainWaves) void k_train(Net net, TrainArgs a
     TrainLds &L = wl[wave];
     float loss = 0.f;
     const uint32_t tiles = (a.batch + 63) / 64;
-    for (uint32_t tile = blockIdx.x * kTrainWaves + wave; tile < tiles; tile += gridDim.x * kTrainWaves) {
-        const uint32_t jr = tile * 64 + lane;
+    // groups of kTrainWaves tiles; every wave runs every group (the block
+    // synchronises inside the backward pass), a tile past the end as 64 invalid records
+    for (uint32_t group = blockIdx.x; group * kTrainWaves < tiles; group += gridDim.x) {
+        const uint32_t jr = (group * kTrainWaves + wave) * 64 + lane;
         const bool valid = jr < a.batch;
         Enc enc;
         float3 wi = f3(0.f, 1.f, 0.f);
@@ -311,19 +323,19 @@ __global__ __launch_bounds__(64 * kTrainWaves) void k_train(Net net, TrainArgs a
                     }
         }
         wave_sync_lds();  // the stage is read before the transposed tiles overwrite it
-        backward_layer<4>(net, L, gacc, x4, dacc);
+        backward_layer<4>(net, wl, gacc, bpart, x4, dacc);
         {
             u32x4 x3[4][2];
             layer_to<kHid>(net, 2, x2, x3);
-            backward_layer<3>(net, L, gacc, x3, dacc);
+            backward_layer<3>(net, wl, gacc, bpart, x3, dacc);
         }
-        backward_layer<2>(net, L, gacc, x2, dacc);
+        backward_layer<2>(net, wl, gacc, bpart, x2, dacc);
         {
             u32x4 x1[4][2];
             layer_to<kIn>(net, 0, x0, x1);
-            backward_layer<1>(net, L, gacc, x1, dacc);
+            backward_layer<1>(net, wl, gacc, bpart, x1, dacc);
         }
-        backward_layer<0>(net, L, gacc, x0, dacc);
+        backward_layer<0>(net, wl, gacc, bpart, x0, dacc);
     }
     for (int off = 32; off > 0; off >>= 1) loss += __shfl_xor(loss, off);
     if (lane == 0) wave_loss[wave] = loss;
