@@ -91,10 +91,17 @@ __device__ __forceinline__ uint32_t pack2(float lo, float hi) {
     const f32x2 v = {lo, hi};
     return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2));
 }
-__device__ __forceinline__ float unpack_lo(uint32_t w) { return __uint_as_float(w << 16); }
-__device__ __forceinline__ float unpack_hi(uint32_t w) { return __uint_as_float(w & 0xFFFF0000u); }
-// element e of an operand as FP32
-__device__ __forceinline__ float op_elem(const u32x4 &x, int e) { return (e & 1) ? unpack_hi(x[e >> 1]) : unpack_lo(x[e >> 1]); }
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+// pack2 of max(lo, 0) and max(hi, 0): a negative BF16 (or -0) has its sign bit
+// set and is negative as an int16, so one v_pk_max_i16 against 0 applies the
+// ReLU to both halves after the conversion.
+__device__ __forceinline__ uint32_t pack2_relu(float lo, float hi) {
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(s16x2, pack2(lo, hi)), s16x2{0, 0}));
+}
+// c + both BF16 halves of w: one v_dot2_f32_bf16 against (1, 1)
+__device__ __forceinline__ float add_halves(uint32_t w, float c) {
+    return __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, w), __builtin_bit_cast(bf16x2, 0x3F803F80u), c, false);
+}
 
 // Device view of a network.
 // Biases of all layers packed at 64 per layer (the last has 40).
@@ -199,9 +206,8 @@ __device__ __forceinline__ float feature_h(const Enc &e, int f0, int h) {
 // lobe k at 5k: (log w_k + log norm_k, mu_k xyz, lambda_k), with
 // norm = lambda / (2 pi (1 - exp(-2 lambda))).
 constexpr int kStageStride = 41;  // odd: the 64 rows of a wave fall in distinct banks
-__device__ __forceinline__ float softplus(float x) { return x > 15.f ? x : log1pf(expf(x)); }
-__device__ __forceinline__ float log_vmf_norm(float lam) { return logf(lam) - logf(2.f * kPi) - logf(-expm1f(-2.f * lam)); }
-// fast forms for rendering (lambda >= kLambdaMin, so 1 - e^{-2 lambda} >= 0.095)
+// softplus and log norm(lambda) from the hardware exp2 / log2 (lambda >= kLambdaMin,
+// so 1 - e^{-2 lambda} >= 0.095 and the difference loses no digits that matter)
 __device__ __forceinline__ float softplus_fast(float x) { return x > 15.f ? x : flog(1.f + fexp(x)); }
 __device__ __forceinline__ float log_vmf_norm_fast(float lam) {
     return flog(lam * (1.f / (2.f * kPi)) * frcp(1.f - fexp(-2.f * lam)));
@@ -313,8 +319,7 @@ __device__ __forceinline__ void tile_to_operands(const f32x16 acc[2], int mt, u3
         for (int i = 0; i < 4; i++)
 #pragma unroll
             for (int nt = 0; nt < 2; nt++)
-                xn[2 * mt + kk][nt][i] =
-                    pack2(fmaxf(acc[nt][8 * kk + 2 * i], 0.f), fmaxf(acc[nt][8 * kk + 2 * i + 1], 0.f));
+                xn[2 * mt + kk][nt][i] = pack2_relu(acc[nt][8 * kk + 2 * i], acc[nt][8 * kk + 2 * i + 1]);
 }
 
 __device__ __forceinline__ void wave_sync_lds() {
@@ -533,43 +538,41 @@ __device__ __forceinline__ void forward8(const Net &net, const Enc &mine, float 
 // o itself) and returns the loss.  Computed in log space: lobes underflow far
 // from their axis.
 __device__ __forceinline__ float loss_grad(const float *o, float3 wi, float w, float *g) {
+    // hardware exp2 / log2 / reciprocal throughout (about 1 ulp each): the
+    // outputs come from a BF16 network, so these errors are far below its own
     float mx = -INFINITY;
     for (int k = 0; k < kLobes; k++) mx = fmaxf(mx, o[5 * k]);
     float s = 0.f;
-    for (int k = 0; k < kLobes; k++) s += expf(o[5 * k] - mx);
-    const float lse = mx + logf(s);
-    float lc[kLobes], lam[kLobes], nrm[kLobes], dl[kLobes], cs[kLobes];
+    for (int k = 0; k < kLobes; k++) s += fexp(o[5 * k] - mx);
+    const float lse = mx + flog(s);
+    float lc[kLobes], lam[kLobes], inv[kLobes], dl[kLobes], cs[kLobes];
     float3 mu[kLobes];
     float lq_max = -INFINITY;
     for (int k = 0; k < kLobes; k++) {
         const float3 a = f3(o[5 * k + 1], o[5 * k + 2], o[5 * k + 3]);
-        nrm[k] = sqrtf(dot3(a, a));
-        mu[k] = nrm[k] > 1e-12f ? a * (1.f / nrm[k]) : f3(0.f, 1.f, 0.f);
+        const float n2 = dot3(a, a);
+        inv[k] = n2 > 1e-24f ? __builtin_amdgcn_rsqf(n2) : 0.f;  // 1 / |a|, 0 for a degenerate axis
+        mu[k] = n2 > 1e-24f ? a * inv[k] : f3(0.f, 1.f, 0.f);
         const float raw = o[5 * k + 4];
-        const float sp = softplus(raw);
+        const float sp = softplus_fast(raw);
         lam[k] = fminf(fmaxf(sp, kLambdaMin), kLambdaMax);
-        dl[k] = (sp > kLambdaMin && sp < kLambdaMax) ? 1.f / (1.f + expf(-raw)) : 0.f;  // dlambda/draw
+        dl[k] = (sp > kLambdaMin && sp < kLambdaMax) ? frcp(1.f + fexp(-raw)) : 0.f;  // dlambda/draw
         cs[k] = dot3(mu[k], wi);
-        lc[k] = (o[5 * k] - lse) + log_vmf_norm(lam[k]) + lam[k] * (cs[k] - 1.f);
+        lc[k] = (o[5 * k] - lse) + log_vmf_norm_fast(lam[k]) + lam[k] * (cs[k] - 1.f);
         lq_max = fmaxf(lq_max, lc[k]);
     }
     float sq = 0.f;
-    for (int k = 0; k < kLobes; k++) sq += expf(lc[k] - lq_max);
-    const float logq = lq_max + logf(sq);
-    float wk[kLobes];
-    for (int k = 0; k < kLobes; k++) wk[k] = expf(o[5 * k] - lse);
+    for (int k = 0; k < kLobes; k++) sq += fexp(lc[k] - lq_max);
+    const float logq = lq_max + flog(sq);
     for (int k = 0; k < kLobes; k++) {
-        const float gam = expf(lc[k] - logq);  // responsibility of lobe k
-        g[5 * k] = -w * (gam - wk[k]);
+        const float gam = fexp(lc[k] - logq);  // responsibility of lobe k
+        g[5 * k] = -w * (gam - fexp(o[5 * k] - lse));
         // d log c / d lambda = 1/lambda - 2 / (exp(2 lambda) - 1) + (mu.wi - 1)
-        const float dlogc = 1.f / lam[k] - 2.f / expm1f(2.f * lam[k]) + (cs[k] - 1.f);
+        const float dlogc = frcp(lam[k]) - 2.f * frcp(fexp(2.f * lam[k]) - 1.f) + (cs[k] - 1.f);
         g[5 * k + 4] = -w * gam * dlogc * dl[k];
-        // d log c / d mu = lambda wi, through mu = a / |a|
-        float3 ga = f3(0.f, 0.f, 0.f);
-        if (nrm[k] > 1e-12f) {
-            const float3 gm = wi * (gam * lam[k]);
-            ga = (gm - mu[k] * dot3(mu[k], gm)) * (1.f / nrm[k]);
-        }
+        // d log c / d mu = lambda wi, through mu = a / |a| (zero for a degenerate axis)
+        const float3 gm = wi * (gam * lam[k]);
+        const float3 ga = (gm - mu[k] * dot3(mu[k], gm)) * inv[k];
         g[5 * k + 1] = -w * ga.x;
         g[5 * k + 2] = -w * ga.y;
         g[5 * k + 3] = -w * ga.z;

@@ -110,12 +110,13 @@ struct TrainArgs {
 
 // Four waves per block (one per SIMD of a CU) step through the records in
 // lockstep, a tile of 64 per wave.  A wave keeps every layer's input operands
-// in registers (it has a SIMD's whole register file), so LDS holds only the
-// current layer's transposed tiles, which the loss stage aliases, and the
-// block's gradient accumulator: 4 x 18 KiB + 67 KiB.  Each 32 x 32 tile of a
-// weight gradient belongs to one wave, which reduces it over all four waves'
-// staged tiles and adds it to the accumulator with plain loads and stores:
-// one writer per address, no LDS atomics.
+// in registers (it has a SIMD's whole register file), so LDS holds the current
+// layer's transposed tiles (which the loss stage aliases), 4 x 18 KiB, and a
+// copy of the packed weights and biases, 69 KiB: at one wave per SIMD nothing
+// hides a load's latency, and LDS answers several times sooner than L2.  Each
+// 32 x 32 tile of a weight gradient belongs to one wave, which reduces it over
+// all four waves' staged tiles with MFMA and adds it into the block's slice of
+// the partial sums in global memory: one writer per address, no atomics.
 constexpr int kTrainWaves = 4;
 struct TrainLds {
     union {
@@ -146,15 +147,27 @@ __device__ __forceinline__ void store_xt(uint16_t *xt, const u32x4 (*x)[2]) {
 // Backward through layer l of the block's four tiles (one per wave).  dY_l
 // arrives as packed BF16 operands (dy[ks][nt] element e = dY of output
 // kperm(ks, h, e) of record 32 nt + p, the values every product below uses);
-// the layer's input operands are in x.  Adds dW_l and db_l into gacc and, for
+// the layer's input operands are in x.  Adds dW_l and db_l into grad (stores
+// them when first) and, for
 // l > 0, leaves dY_{l-1} (the input's ReLU mask applied) packed in dy.  Every
 // wave of the block calls it together (block barriers inside).
 template <int l>
-__device__ __forceinline__ void backward_layer(const Net &net, TrainLds *wl, float *gacc, float (*bpart)[64],
+__device__ __forceinline__ void backward_layer(const Net &net, TrainLds *wl, float *grad, bool first, float (*bpart)[64],
                                                const u32x4 (*x)[2], u32x4 dy[4][2]) {
     constexpr int K = layer_in(l), N = layer_out(l);
     const int lane = __lane_id(), p = lane & 31, h = lane >> 5, wave = threadIdx.x >> 6;
     TrainLds &L = wl[wave];
+    // weight gradient dW[i][o] = sum_r X[r][i] dY[r][o] over the block's 256
+    // records: output tile (mt, nt) = (wave / 2, wave % 2) belongs to this wave.
+    // Its running sum is the product's initial accumulator, loaded here so that
+    // the staging and the barrier hide the load's latency.
+    constexpr int kTiles = 2 * (K / 32);
+    const int mt = wave >> 1, nt = wave & 1;
+    const bool mine = wave < kTiles && 32 * nt + p < N;
+    float *gb = grad + w_offset(l) + (32 * mt + 4 * h) * N + 32 * nt + p;
+    f32x16 acc;
+#pragma unroll
+    for (int v = 0; v < 16; v++) acc[v] = !first && mine ? gb[drow(v, 0) * N] : 0.f;
     store_xt<4>(L.t.dyt, dy);
     store_xt<K / 16>(L.t.xt, x);
     wave_sync_lds();
@@ -165,17 +178,12 @@ __device__ __forceinline__ void backward_layer(const Net &net, TrainLds *wl, flo
         for (int r = 0; r < 64; r += 8) {
             const u32x4 v = *reinterpret_cast<const u32x4 *>(L.t.dyt + lane * kXt + r);
 #pragma unroll
-            for (int e = 0; e < 8; e++) s[e & 3] += op_elem(v, e);
+            for (int i = 0; i < 4; i++) s[i] = add_halves(v[i], s[i]);
         }
         bpart[wave][lane] = (s[0] + s[1]) + (s[2] + s[3]);
     }
     __syncthreads();
-    // weight gradient dW[i][o] = sum_r X[r][i] dY[r][o] over the block's 256
-    // records: output tile (mt, nt) = (wave / 2, wave % 2)
-    constexpr int kTiles = 2 * (K / 32);
     if (wave < kTiles) {
-        const int mt = wave >> 1, nt = wave & 1;
-        f32x16 acc = f32x16{};
 #pragma unroll
         for (int w2 = 0; w2 < kTrainWaves; w2++)
 #pragma unroll
@@ -184,15 +192,16 @@ __device__ __forceinline__ void backward_layer(const Net &net, TrainLds *wl, flo
                 const u32x4 bv = *reinterpret_cast<const u32x4 *>(wl[w2].t.dyt + (32 * nt + p) * kXt + 16 * ks + 8 * h);
                 acc = mfma(av, bv, acc);
             }
-        if (32 * nt + p < N) {
-            float *gb = gacc + w_offset(l) + (32 * mt + 4 * h) * N + 32 * nt + p;
+        if (mine) {
 #pragma unroll
-            for (int v = 0; v < 16; v++) gb[drow(v, 0) * N] += acc[v];
+            for (int v = 0; v < 16; v++) gb[drow(v, 0) * N] = acc[v];
         }
     }
     // bias: the wave with no weight tile for layer 0, else one per layer in turn
-    if (wave == (kTiles == 2 ? 2 : l & 3) && lane < N)
-        gacc[b_offset(l) + lane] += (bpart[0][lane] + bpart[1][lane]) + (bpart[2][lane] + bpart[3][lane]);
+    if (wave == (kTiles == 2 ? 2 : l & 3) && lane < N) {
+        const float s = (bpart[0][lane] + bpart[1][lane]) + (bpart[2][lane] + bpart[3][lane]);
+        grad[b_offset(l) + lane] = first ? s : grad[b_offset(l) + lane] + s;
+    }
     if constexpr (l > 0) {
         // backward data: dX^T = W_l dY^T; the input's ReLU mask sits in the same lane
         // and slot as the result, which is packed into dy for the layer below
@@ -238,14 +247,23 @@ __device__ __forceinline__ void layer_to(const Net &net, int l, const u32x4 (*x)
     }
 }
 
-__global__ __launch_bounds__(64 * kTrainWaves) void k_train(Net net, TrainArgs a) {
+__global__ __launch_bounds__(64 * kTrainWaves) void k_train(Net gnet, TrainArgs a) {
     __shared__ __attribute__((aligned(16))) TrainLds wl[kTrainWaves];
-    __shared__ float gacc[kNumParams];
+    __shared__ __attribute__((aligned(16))) u32x4 wfwd[kFwdOps];
+    __shared__ __attribute__((aligned(16))) u32x4 wbwd[kBwdOps];
+    __shared__ float wbias[kBiasPacked];
     __shared__ float bpart[kTrainWaves][64];
     __shared__ float wave_loss[kTrainWaves];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, p = lane & 31, h = lane >> 5;
-    for (int j = threadIdx.x; j < kNumParams; j += blockDim.x) gacc[j] = 0.f;
+    for (int j = threadIdx.x; j < kFwdOps; j += blockDim.x) wfwd[j] = gnet.fwd[j];
+    for (int j = threadIdx.x; j < kBwdOps; j += blockDim.x) wbwd[j] = gnet.bwd[j];
+    for (int j = threadIdx.x; j < kBiasPacked; j += blockDim.x) wbias[j] = gnet.bias[j];
     __syncthreads();
+    Net net = gnet;
+    net.fwd = wfwd;
+    net.bwd = wbwd;
+    net.bias = wbias;
+    float *grad = a.partials + size_t(blockIdx.x) * kNumParams;
     TrainLds &L = wl[wave];
     float loss = 0.f;
     const uint32_t tiles = (a.batch + 63) / 64;
@@ -253,6 +271,7 @@ __global__ __launch_bounds__(64 * kTrainWaves) void k_train(Net net, TrainArgs a
     // synchronises inside the backward pass), a tile past the end as 64 invalid records
     for (uint32_t group = blockIdx.x; group * kTrainWaves < tiles; group += gridDim.x) {
         const uint32_t jr = (group * kTrainWaves + wave) * 64 + lane;
+        const bool first = group == blockIdx.x;
         const bool valid = jr < a.batch;
         Enc enc;
         float3 wi = f3(0.f, 1.f, 0.f);
@@ -323,25 +342,23 @@ __global__ __launch_bounds__(64 * kTrainWaves) void k_train(Net net, TrainArgs a
                     }
         }
         wave_sync_lds();  // the stage is read before the transposed tiles overwrite it
-        backward_layer<4>(net, wl, gacc, bpart, x4, dacc);
+        backward_layer<4>(net, wl, grad, first, bpart, x4, dacc);
         {
             u32x4 x3[4][2];
             layer_to<kHid>(net, 2, x2, x3);
-            backward_layer<3>(net, wl, gacc, bpart, x3, dacc);
+            backward_layer<3>(net, wl, grad, first, bpart, x3, dacc);
         }
-        backward_layer<2>(net, wl, gacc, bpart, x2, dacc);
+        backward_layer<2>(net, wl, grad, first, bpart, x2, dacc);
         {
             u32x4 x1[4][2];
             layer_to<kIn>(net, 0, x0, x1);
-            backward_layer<1>(net, wl, gacc, bpart, x1, dacc);
+            backward_layer<1>(net, wl, grad, first, bpart, x1, dacc);
         }
-        backward_layer<0>(net, wl, gacc, bpart, x0, dacc);
+        backward_layer<0>(net, wl, grad, first, bpart, x0, dacc);
     }
     for (int off = 32; off > 0; off >>= 1) loss += __shfl_xor(loss, off);
     if (lane == 0) wave_loss[wave] = loss;
     __syncthreads();
-    float *dst = a.partials + size_t(blockIdx.x) * kNumParams;
-    for (int j = threadIdx.x; j < kNumParams; j += blockDim.x) dst[j] = gacc[j];
     if (threadIdx.x == 0) {
         float s = 0.f;
         for (int k = 0; k < kTrainWaves; k++) s += wave_loss[k];
