@@ -28,8 +28,7 @@ constexpr int kMaxTrainBlocks = 256;
 constexpr uint32_t kTestBlock = 64;
 
 // ---------------------------------------------------------------- pack
-__global__ void k_pack(const float *params, u32x4 *fwd, u32x4 *bwd, float *bias) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void pack_bf16(int j, const float *params, u32x4 *fwd, u32x4 *bwd, float *bias) {
     if (j < kBiasPacked) bias[j] = params[b_offset(j / 64) + j % 64];
     if (j < kFwdOps) {
         int l = 0;
@@ -65,9 +64,7 @@ __global__ void k_pack(const float *params, u32x4 *fwd, u32x4 *bwd, float *bias)
 // layer's largest |weight| sets its power-of-two scale (wscale[l] = 2^k, the
 // operands hold W / 2^k, at most 128 in magnitude).
 constexpr int kPack8Block = 256;
-__global__ __launch_bounds__(kPack8Block) void k_pack8(const float *params, u32x2 *fwd8, float *wscale) {
-    __shared__ float red[kPack8Block];
-    const int l = blockIdx.x;
+__device__ __forceinline__ void pack_e4m3(int l, float *red, const float *params, u32x2 *fwd8, float *wscale) {
     const int K = layer_in(l), N = layer_out(l), nks = K / 16;
     const float *W = params + w_offset(l);
     float m = 0.f;
@@ -89,6 +86,14 @@ __global__ __launch_bounds__(kPack8Block) void k_pack8(const float *params, u32x
         for (int e = 0; e < 8; e++) v[e] = o < N ? W[kperm(ks, h, e) * N + o] : 0.f;
         fwd8[fwd_offset(l) + jj] = quant8(v, inv);
     }
+}
+// Both packings in one launch: blocks 0 .. kLayers-1 quantise one layer each,
+// the rest convert 256 BF16 operands each.
+__global__ __launch_bounds__(kPack8Block) void k_pack(const float *params, u32x4 *fwd, u32x4 *bwd, float *bias,
+                                                    u32x2 *fwd8, float *wscale) {
+    __shared__ float red[kPack8Block];
+    if (blockIdx.x < kLayers) pack_e4m3(blockIdx.x, red, params, fwd8, wscale);
+    else pack_bf16((blockIdx.x - kLayers) * kPack8Block + threadIdx.x, params, fwd, bwd, bias);
 }
 
 __global__ void k_fill_u32_guide(uint32_t *p, uint32_t n) {
@@ -375,20 +380,29 @@ struct AdamArgs {
     float beta1, beta2, eps_t;  // eps * sqrt(1 - beta2^t)
     float *grad_out;            // non-null: store the summed gradient, no update
 };
-// 64 parameters per block: 4 threads per parameter sum interleaved block
-// partials (coalesced over the parameters), then combine in LDS.
-constexpr int kAdamParams = 64, kAdamBlock = 256;
+// 64 parameters per block, 16 threads per parameter: each sums every 16th
+// block partial (coalesced over the parameters, four loads in flight), then
+// one thread per parameter adds the 16 sums in a fixed order.
+constexpr int kAdamParams = 64, kAdamRows = 16, kAdamBlock = kAdamParams * kAdamRows;
 __global__ __launch_bounds__(kAdamBlock) void k_adam(AdamArgs a) {
     __shared__ float part[kAdamBlock];
     const int pl = threadIdx.x % kAdamParams, q = threadIdx.x / kAdamParams;
     const int j = blockIdx.x * kAdamParams + pl;
-    float s = 0.f;
-    if (j < kNumParams)
-        for (int b = q; b < a.blocks; b += kAdamBlock / kAdamParams) s += a.partials[size_t(b) * kNumParams + j];
-    part[threadIdx.x] = s;
+    float s[4] = {0.f, 0.f, 0.f, 0.f};
+    if (j < kNumParams) {
+        const float *col = a.partials + j;
+        int b = q;
+        for (; b + 3 * kAdamRows < a.blocks; b += 4 * kAdamRows)
+#pragma unroll
+            for (int u = 0; u < 4; u++) s[u] += col[size_t(b + u * kAdamRows) * kNumParams];
+        for (; b < a.blocks; b += kAdamRows) s[0] += col[size_t(b) * kNumParams];
+    }
+    part[threadIdx.x] = (s[0] + s[1]) + (s[2] + s[3]);
     __syncthreads();
     if (q != 0 || j >= kNumParams) return;
-    const float g = (part[pl] + part[pl + kAdamParams]) + (part[pl + 2 * kAdamParams] + part[pl + 3 * kAdamParams]);
+    float g = 0.f;
+#pragma unroll
+    for (int r = 0; r < kAdamRows; r++) g += part[r * kAdamParams + pl];
     if (a.grad_out) {
         a.grad_out[j] = g;
         return;
@@ -547,8 +561,8 @@ Net net_view(const wr_guide *g) {
 
 int pack(wr_guide *g, hipStream_t st) {
     const int n = kFwdOps + kBwdOps;
-    k_pack<<<(n + 255) / 256, 256, 0, st>>>(g->d_params, g->d_fwd, g->d_bwd, g->d_bias);
-    k_pack8<<<kLayers, kPack8Block, 0, st>>>(g->d_params, g->d_fwd8, g->d_wscale);
+    k_pack<<<kLayers + (n + kPack8Block - 1) / kPack8Block, kPack8Block, 0, st>>>(g->d_params, g->d_fwd, g->d_bwd,
+                                                                                    g->d_bias, g->d_fwd8, g->d_wscale);
     WR_HIP(hipGetLastError());
     return WR_OK;
 }
