@@ -356,7 +356,7 @@ __device__ __forceinline__ uint32_t refill(uint32_t *fetch_counter, uint32_t n, 
     if (wr.lo == wr.hi) {
         uint32_t base = 0;
         if (lane_id() == 0) base = atomicAdd(fetch_counter, chunk);
-        base = __shfl(base, 0);
+        base = __builtin_amdgcn_readfirstlane(base);  // every lane is active here, so this reads lane 0
         if (base >= n) {
             exhausted = true;
             return n;
@@ -377,10 +377,8 @@ __device__ __forceinline__ void extend_body(DScene s1, const BatchJob *__restric
                                             TravTune tune) {
     const uint32_t n = active_count(pb.counters);
     if (blockIdx.x * kBlock >= n) return;
-    __shared__ uint32_t st_node[kLdsStackDepth * kBlock];
-    __shared__ float st_t[kLdsStackDepth * kBlock];
-    LaneStack<kBlock> st{st_node + lane_id(), st_t + lane_id(),
-                         spill + size_t(blockIdx.x * kBlock + lane_id()) * kSpillDepth, 0};
+    __shared__ uint64_t st_ent[kLdsStackDepth * kBlock];
+    LaneStack<kBlock> st(st_ent + lane_id(), reinterpret_cast<uint32_t *>(st_ent) + lane_id(), spill);
     TraversalCounters cnt;
     TravState ts;
     uint32_t p = 0;
@@ -1061,7 +1059,7 @@ __global__ __launch_bounds__(kBlock) WR_TRAV_ATTR void k_shadow(DScene s1, const
     const uint32_t n = pb.counters[kS];
     if (blockIdx.x * kBlock >= n) return;
     __shared__ uint32_t st_node[kLdsStackDepth * kBlock];  // any hit: node refs only
-    LaneStack<kBlock> st{st_node + lane_id(), nullptr, spill + size_t(blockIdx.x * kBlock + lane_id()) * kSpillDepth, 0};
+    LaneStack<kBlock> st(nullptr, st_node + lane_id(), spill);
     TraversalCounters cnt;
     TravState ts;
     uint32_t i = 0;
@@ -1136,10 +1134,8 @@ __global__ void k_accumulate(const float4 *__restrict__ sample_rad, double *film
 template <bool ANY>
 __global__ __launch_bounds__(kBlock) void k_trace(DScene s, const float4 *__restrict__ rays, uint32_t n, uint2 *hits,
                                                   uint8_t *occ, uint2 *spill) {
-    __shared__ uint32_t st_node[kLdsStackDepth * kBlock];
-    __shared__ float st_t[kLdsStackDepth * kBlock];
-    LaneStack<kBlock> st{st_node + lane_id(), st_t + lane_id(),
-                         spill + size_t(blockIdx.x * kBlock + lane_id()) * kSpillDepth, 0};
+    __shared__ uint64_t st_ent[kLdsStackDepth * kBlock];
+    LaneStack<kBlock> st(st_ent + lane_id(), reinterpret_cast<uint32_t *>(st_ent) + lane_id(), spill);
     TraversalCounters cnt;
     for (uint32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
         const uint32_t i = base + lane_id();
@@ -1162,10 +1158,8 @@ __global__ __launch_bounds__(kBlock) void k_trace(DScene s, const float4 *__rest
 // order into the FP64 base film, so the film is bit-reproducible.
 __global__ __launch_bounds__(kBlock) void k_visualize(DScene s, uint32_t wh, uint32_t spp, uint32_t first_sample,
                                                       uint32_t seed, double *film_base, uint2 *spill) {
-    __shared__ uint32_t st_node[kLdsStackDepth * kBlock];
-    __shared__ float st_t[kLdsStackDepth * kBlock];
-    LaneStack<kBlock> st{st_node + lane_id(), st_t + lane_id(),
-                         spill + size_t(blockIdx.x * kBlock + lane_id()) * kSpillDepth, 0};
+    __shared__ uint64_t st_ent[kLdsStackDepth * kBlock];
+    LaneStack<kBlock> st(st_ent + lane_id(), reinterpret_cast<uint32_t *>(st_ent) + lane_id(), spill);
     TraversalCounters cnt;
     const DCamera &cam = s.cam;
     for (uint32_t pixel = blockIdx.x * kBlock + lane_id(); pixel < wh; pixel += gridDim.x * kBlock) {
@@ -1246,10 +1240,8 @@ __device__ __forceinline__ float bd_weight(float inv) {
 __global__ __launch_bounds__(kBlock, 4) void k_bdpt(DScene s, uint32_t wh, uint32_t spp, uint32_t first_sample,
                                                  uint32_t seed, int N, double *film_base, double *film_splat,
                                                  float4 *__restrict__ recs, unsigned long long *counts, uint2 *spill) {
-    __shared__ uint32_t st_node[kLdsStackDepth * kBlock];
-    __shared__ float st_t[kLdsStackDepth * kBlock];
-    LaneStack<kBlock> st{st_node + lane_id(), st_t + lane_id(),
-                         spill + size_t(blockIdx.x * kBlock + lane_id()) * kSpillDepth, 0};
+    __shared__ uint64_t st_ent[kLdsStackDepth * kBlock];
+    LaneStack<kBlock> st(st_ent + lane_id(), reinterpret_cast<uint32_t *>(st_ent) + lane_id(), spill);
     TraversalCounters cnt;
     const DCamera &cam = s.cam;
     const float3 cz = f3(cam.zx, cam.zy, cam.zz);

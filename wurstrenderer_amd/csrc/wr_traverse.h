@@ -5,8 +5,9 @@
 // Semantics follow rtcIntersect1 / rtcOccluded1 as the reference calls them
 // (src/accel/embreeaccel.h:336-422): double-sided triangles, hits accepted
 // for t in (tmin, tmax], closest hit for extension rays, any hit for shadow
-// rays.  Node boxes are tested conservatively (far distance scaled up by
-// 1 + 2^-21) so that no watertight hit is lost to rounding in the box test.
+// rays.  Node boxes are tested conservatively (exit distances widened by the
+// bound on the slab arithmetic's rounding error) so that no watertight hit is
+// lost to rounding in the box test.
 #pragma once
 #include "wr_device.h"
 
@@ -18,6 +19,8 @@ struct RaySetup {
     float3 o;
     float tmin;
     float ix, iy, iz;          // 1/d with zero components replaced by +-1e-20
+    float nnx, nny, nnz;       // per axis -(o/d), lowered (near planes) and raised (far planes) by
+    float nfx, nfy, nfz;       // 2^-22 |o/d|: a plane p is crossed at fma(p, 1/d, nn) or fma(p, 1/d, nf)
     int kx, ky, kz;            // watertight axis permutation
     float Sx, Sy, Sz;
     uint32_t onx, ony, onz;    // byte offsets of the near-plane rows in a Node4 (the far row is at offset ^ 16)
@@ -35,6 +38,10 @@ __device__ __forceinline__ RaySetup setup_ray(float3 o, float3 d, float tmin) {
     r.o = o; r.tmin = tmin;
     auto safe = [](float x) { return fabsf(x) < 1e-20f ? copysignf(1e-20f, x) : x; };
     r.ix = rcp(safe(d.x)); r.iy = rcp(safe(d.y)); r.iz = rcp(safe(d.z));
+    const float mx = o.x * r.ix, my = o.y * r.iy, mz = o.z * r.iz;
+    const float ex = 2.38418579e-7f * fabsf(mx), ey = 2.38418579e-7f * fabsf(my), ez = 2.38418579e-7f * fabsf(mz);
+    r.nnx = -mx - ex; r.nny = -my - ey; r.nnz = -mz - ez;
+    r.nfx = -mx + ex; r.nfy = -my + ey; r.nfz = -mz + ez;
     // a Node4 holds rows lo_x, hi_x, lo_y, hi_y, lo_z, hi_z of 16 bytes: the near
     // plane of an axis is its lo row for a positive direction, its hi row otherwise
     r.onx = r.ix < 0.f ? 16u : 0u;
@@ -105,54 +112,72 @@ __device__ __forceinline__ bool tri_test(const RaySetup &r, float4 va, float4 vb
 // leaf-order index wins) depend on the order in which leaves are reached.
 __device__ __forceinline__ float cull_limit(float t) { return fmaf(fabsf(t), 4.76837158e-7f, t); }
 
-// Per-lane traversal stack of (node ref, entry distance).  The first
-// kLdsStackDepth entries live in LDS at [k * kBlock + lane] (bank-conflict
-// free); deeper entries spill to a per-lane slice of global memory.  The BVH
-// builder guarantees at most kStackDepth live entries.
+// Per-lane traversal stack.  Closest hit keeps 64-bit entries (node ref in
+// the low half, entry distance bits in the high half), any hit node refs only.
+// The first kLdsStackDepth entries live in LDS at [k * kBlock + lane]
+// (bank-conflict free: one 64-bit or 32-bit LDS access per push and pop);
+// deeper entries spill to the block's slice of a global region, at
+// [(k - kLdsStackDepth) * kBlock + lane] (a wave's spills coalesce, and the
+// address is a scalar base plus a 32-bit lane offset).  The pointers carry
+// their address spaces, so the two branches of a pop stay an LDS load and a
+// global load (as generic pointers the compiler merged them into one flat load
+// and split the entry into two dependent loads).  The BVH builder guarantees
+// at most kStackDepth live entries.
+typedef __attribute__((address_space(3))) uint64_t LdsU64;
+typedef __attribute__((address_space(3))) uint32_t LdsU32;
+typedef __attribute__((address_space(1))) uint64_t GlobalU64;
+typedef __attribute__((address_space(1))) uint32_t GlobalU32;
 template <int kBlock>
 struct LaneStack {
-    uint32_t *node;
-    float *tnear;
-    uint2 *spill;
+    LdsU64 *ent;        // closest hit: LDS entries of this lane
+    LdsU32 *node;       // any hit: LDS node refs of this lane
+    GlobalU64 *spill;   // this block's spill slice (kSpillDepth * kBlock entries)
     int sp;
-    __device__ __forceinline__ void push(uint32_t ref, float t) {
-        if (sp < kLdsStackDepth) {
-            node[sp * kBlock] = ref;
-            tnear[sp * kBlock] = t;
-        } else {
-            spill[sp - kLdsStackDepth] = make_uint2(ref, __float_as_uint(t));
-        }
+    // spill_region: kSpillDepth * kBlock entries per block of the grid
+    __device__ __forceinline__ LaneStack(uint64_t *lds_ent, uint32_t *lds_node, uint2 *spill_region)
+        : ent((LdsU64 *)lds_ent), node((LdsU32 *)lds_node),
+          spill((GlobalU64 *)spill_region + size_t(blockIdx.x) * (kSpillDepth * kBlock)), sp(0) {}
+    __device__ __forceinline__ uint32_t spill_index() const { return uint32_t(sp - kLdsStackDepth) * kBlock + __lane_id(); }
+    __device__ __forceinline__ void push(uint64_t e) {
+        if (sp < kLdsStackDepth) ent[sp * kBlock] = e;
+        else spill[spill_index()] = e;
         sp++;
     }
-    __device__ __forceinline__ void top(uint32_t &ref, float &t) const {
-        if (sp < kLdsStackDepth) {
-            ref = node[sp * kBlock];
-            t = tnear[sp * kBlock];
-        } else {
-            const uint2 e = spill[sp - kLdsStackDepth];
-            ref = e.x;
-            t = __uint_as_float(e.y);
-        }
+    __device__ __forceinline__ uint64_t top() const {
+        return sp < kLdsStackDepth ? ent[sp * kBlock] : spill[spill_index()];
     }
-    // node refs only (any-hit traversal never culls a popped entry)
     __device__ __forceinline__ void push_ref(uint32_t ref) {
         if (sp < kLdsStackDepth) node[sp * kBlock] = ref;
-        else reinterpret_cast<uint32_t *>(spill + (sp - kLdsStackDepth))[0] = ref;
+        else reinterpret_cast<GlobalU32 *>(spill + spill_index())[0] = ref;
         sp++;
     }
     __device__ __forceinline__ uint32_t top_ref() const {
-        return sp < kLdsStackDepth ? node[sp * kBlock] : reinterpret_cast<const uint32_t *>(spill + (sp - kLdsStackDepth))[0];
+        return sp < kLdsStackDepth ? node[sp * kBlock] : reinterpret_cast<GlobalU32 *>(spill + spill_index())[0];
     }
 };
+__device__ __forceinline__ uint64_t stack_entry(uint32_t ref, uint32_t tn_bits) {
+    return uint64_t(ref) | (uint64_t(tn_bits) << 32);
+}
 
 // Entry and exit distances of the ray through the four child boxes of inner
 // node `cur`, and the child refs.  The near and far plane rows are loaded at
 // the ray's per-axis byte offsets, so no per-child select is needed.  Empty
 // slots hold inverted boxes (lo = FLT_MAX, hi = -FLT_MAX): their entry
 // distance exceeds their exit distance, so they fail the box test with no
-// check of the child ref.  `far_limit` culls boxes entered beyond it; exit
-// distances are scaled by 1 + 2^-21 so rounding never culls a box the ray
-// touches.  Returns the refs; hit[i] marks the children whose box is entered.
+// check of the child ref.
+//
+// A plane p is crossed at fma(p, 1/d, -(o/d)): one operation instead of a
+// subtraction and a product.  Against the exact (p - o)/d this form errs by
+// the rounding of o/d (at most 2^-24 |o/d|, large only along an axis the ray
+// hardly moves in) plus a relative part (1/d is v_rcp_f32 to 1 ulp, and the
+// final rounding) below 2^-22.  The absolute part is removed per axis: near
+// planes use -(o/d) lowered by 2^-22 |o/d| and far planes -(o/d) raised by as
+// much, so a computed entry never exceeds the exact one by more than 2^-22 of
+// its size, nor a computed exit falls short by more.  A box the ray touches
+// (entry <= exit, both >= tmin > 0) then passes entry <= exit * (1 + 2^-20),
+// and one entered before the far limit t passes entry <= cull_limit(t).
+// Returns the refs; tn holds the entry distances and hit[i] marks the
+// children whose box is entered before `far_limit`.
 template <class G>
 __device__ __forceinline__ uint4 node_test(const G &s, const RaySetup &r, uint32_t cur, float far_limit, float tn[4],
                                            bool hit[4]) {
@@ -163,18 +188,15 @@ __device__ __forceinline__ uint4 node_test(const G &s, const RaySetup &r, uint32
     const float4 ny_ = row(r.ony), fy_ = row(r.ony ^ 16u);
     const float4 nz_ = row(r.onz), fz_ = row(r.onz ^ 16u);
     const uint4 ch = *reinterpret_cast<const uint4 *>(nb + (base + 96u));
-    const float ox = r.o.x, oy = r.o.y, oz = r.o.z;
-    tn[0] = fmaxf(fmaxf((nx_.x - ox) * r.ix, (ny_.x - oy) * r.iy), fmaxf((nz_.x - oz) * r.iz, r.tmin));
-    tn[1] = fmaxf(fmaxf((nx_.y - ox) * r.ix, (ny_.y - oy) * r.iy), fmaxf((nz_.y - oz) * r.iz, r.tmin));
-    tn[2] = fmaxf(fmaxf((nx_.z - ox) * r.ix, (ny_.z - oy) * r.iy), fmaxf((nz_.z - oz) * r.iz, r.tmin));
-    tn[3] = fmaxf(fmaxf((nx_.w - ox) * r.ix, (ny_.w - oy) * r.iy), fmaxf((nz_.w - oz) * r.iz, r.tmin));
-    float tf[4];
-    tf[0] = fminf(fminf((fx_.x - ox) * r.ix, (fy_.x - oy) * r.iy), (fz_.x - oz) * r.iz);
-    tf[1] = fminf(fminf((fx_.y - ox) * r.ix, (fy_.y - oy) * r.iy), (fz_.y - oz) * r.iz);
-    tf[2] = fminf(fminf((fx_.z - ox) * r.ix, (fy_.z - oy) * r.iy), (fz_.z - oz) * r.iz);
-    tf[3] = fminf(fminf((fx_.w - ox) * r.ix, (fy_.w - oy) * r.iy), (fz_.w - oz) * r.iz);
+    const float nx[4] = {nx_.x, nx_.y, nx_.z, nx_.w}, fx[4] = {fx_.x, fx_.y, fx_.z, fx_.w};
+    const float ny[4] = {ny_.x, ny_.y, ny_.z, ny_.w}, fy[4] = {fy_.x, fy_.y, fy_.z, fy_.w};
+    const float nz[4] = {nz_.x, nz_.y, nz_.z, nz_.w}, fz[4] = {fz_.x, fz_.y, fz_.z, fz_.w};
 #pragma unroll
-    for (int i = 0; i < 4; i++) hit[i] = tn[i] <= fminf(tf[i] * 1.00000048f, far_limit);
+    for (int i = 0; i < 4; i++) {
+        tn[i] = fmaxf(fmaxf(fmaf(nx[i], r.ix, r.nnx), fmaf(ny[i], r.iy, r.nny)), fmaxf(fmaf(nz[i], r.iz, r.nnz), r.tmin));
+        const float tf = fminf(fminf(fmaf(fx[i], r.ix, r.nfx), fmaf(fy[i], r.iy, r.nfy)), fmaf(fz[i], r.iz, r.nfz));
+        hit[i] = tn[i] <= fminf(tf * 1.00000095f, far_limit);
+    }
     return ch;
 }
 
@@ -197,91 +219,28 @@ __device__ __forceinline__ void trav_begin(TravState &ts, float3 o, float3 d, fl
     ts.cur = 0;  // root (internal node 0)
 }
 
-// Orders the children near to far by entry distance (key: +inf for a child
-// whose box is not entered), pushes the entered ones but the nearest, farthest
-// first, and returns the nearest (kInvalid if none is entered).  The
-// 5-comparator network is branch-free: min/max of the keys, selects of the refs.
+// Orders the children near to far, pushes the entered ones but the nearest,
+// farthest first, and returns the nearest (kInvalid if none is entered).  A
+// key is a 64-bit (entry distance bits, child ref) pair, the distance bits
+// 0xFFFFFFFF for a child whose box is not entered; entry distances of entered
+// boxes are positive, so their bits order as the distances do.  Each of the
+// five comparators of the network is one 32-bit compare and four selects, and
+// a push is one 64-bit LDS store.
+constexpr uint32_t kNoEntry = 0xFFFFFFFFu;
 template <int kBlock>
-__device__ __forceinline__ uint32_t sort_push(float key[4], uint32_t c[4], LaneStack<kBlock> &st) {
+__device__ __forceinline__ uint32_t sort_push(uint2 key[4], LaneStack<kBlock> &st) {
     auto cswap = [&](int a, int b) {
-        const bool sw = key[b] < key[a];
-        const float lo = fminf(key[a], key[b]), hi = fmaxf(key[a], key[b]);
-        const uint32_t ca = sw ? c[b] : c[a], cb = sw ? c[a] : c[b];
-        key[a] = lo; key[b] = hi;
-        c[a] = ca; c[b] = cb;
+        const bool sw = key[b].y < key[a].y;
+        const uint2 lo = sw ? key[b] : key[a], hi = sw ? key[a] : key[b];
+        key[a] = lo;
+        key[b] = hi;
     };
     cswap(0, 1); cswap(2, 3); cswap(0, 2); cswap(1, 3); cswap(1, 2);
     // the entered children are now a prefix of the four
-    if (key[3] != INFINITY) st.push(c[3], key[3]);
-    if (key[2] != INFINITY) st.push(c[2], key[2]);
-    if (key[1] != INFINITY) st.push(c[1], key[1]);
-    return key[0] != INFINITY ? c[0] : kInvalid;
-}
-
-// Processes one node (internal: box tests and pushes; leaf: triangle tests),
-// then pops if needed.  Returns true when the lane's ray is finished.
-// Closest hit (ANY = false) visits children near-to-far and culls stack entries
-// farther than the current hit; any hit (ANY = true) stops at the first hit.
-// G is any scene handle with `nodes` and `tris` (DScene, or a per-lane Geo).
-template <bool ANY, bool COUNT, int kBlock, class G>
-__device__ __forceinline__ bool trav_step(const G &s, TravState &ts, LaneStack<kBlock> &st, TraversalCounters &cnt) {
-    const RaySetup &r = ts.r;
-    const uint32_t cur = ts.cur;
-    if (!(cur & kLeafBit)) {
-        float tn[4];
-        bool h[4];
-        const uint4 ch = node_test(s, r, cur, ANY ? ts.t : cull_limit(ts.t), tn, h);
-        if (COUNT) cnt.nodes++;
-        uint32_t c[4] = {ch.x, ch.y, ch.z, ch.w};
-        float key[4];
-#pragma unroll
-        for (int i = 0; i < 4; i++) key[i] = h[i] ? tn[i] : INFINITY;
-        uint32_t next = kInvalid;
-        if (!ANY) {
-            next = sort_push(key, c, st);
-        } else {
-            // slot order (largest box first); push the others, continue with the first
-#pragma unroll
-            for (int i = 3; i >= 0; i--) {
-                if (key[i] != INFINITY) {
-                    if (next != kInvalid) st.push_ref(next);
-                    next = c[i];
-                }
-            }
-        }
-        if (next != kInvalid) { ts.cur = next; return false; }
-    } else {
-        const uint32_t first = cur & 0x0FFFFFFFu;
-        const uint32_t count = ((cur >> 28) & 0x7u) + 1u;
-        if (COUNT) cnt.tris += count;
-        for (uint32_t k = 0; k < count; k++) {
-            const float4 *tp = s.tris + 3 * size_t(first + k);
-            float t;
-            // at equal t the lower leaf-order index wins (kInvalid is the largest), so
-            // the closest hit does not depend on the order in which leaves are tested
-            if (tri_test(r, tp[0], tp[1], tp[2], ts.t, t) && (ANY || t < ts.t || first + k < ts.hit)) {
-                ts.t = t;
-                ts.hit = first + k;
-                if (ANY) return true;
-            }
-        }
-    }
-    // pop, skipping entries farther than the current hit (any hit: the far
-    // limit has not moved since the push)
-    if (ANY) {
-        if (st.sp == 0) return true;
-        st.sp--;
-        ts.cur = st.top_ref();
-        return false;
-    }
-    for (;;) {
-        if (st.sp == 0) return true;
-        st.sp--;
-        uint32_t ref;
-        float tn;
-        st.top(ref, tn);
-        if (tn <= cull_limit(ts.t)) { ts.cur = ref; return false; }
-    }
+    if (key[3].y != kNoEntry) st.push(stack_entry(key[3].x, key[3].y));
+    if (key[2].y != kNoEntry) st.push(stack_entry(key[2].x, key[2].y));
+    if (key[1].y != kNoEntry) st.push(stack_entry(key[1].x, key[1].y));
+    return key[0].y != kNoEntry ? key[0].x : kInvalid;
 }
 
 // Closest hit, one speculative while-while round (Aila and Laine, "Understanding
@@ -321,23 +280,22 @@ __device__ __forceinline__ bool trav_round(const G &s, TravState &ts, LaneStack<
             bool h[4];
             const uint4 ch = node_test(s, r, cur, cull_limit(ts.t), tn, h);
             if (COUNT) cnt.nodes++;
-            uint32_t c[4] = {ch.x, ch.y, ch.z, ch.w};
-            float key[4];
-#pragma unroll
-            for (int i = 0; i < 4; i++) key[i] = h[i] ? tn[i] : INFINITY;
-            const uint32_t next = sort_push(key, c, st);
+            uint2 key[4] = {make_uint2(ch.x, h[0] ? __float_as_uint(tn[0]) : kNoEntry),
+                            make_uint2(ch.y, h[1] ? __float_as_uint(tn[1]) : kNoEntry),
+                            make_uint2(ch.z, h[2] ? __float_as_uint(tn[2]) : kNoEntry),
+                            make_uint2(ch.w, h[3] ? __float_as_uint(tn[3]) : kNoEntry)};
+            const uint32_t next = sort_push(key, st);
             if (next != kInvalid) ts.cur = next;
             else pop = true;
         }
         if (pop) {
             more = false;
+            const float lim = cull_limit(ts.t);
             while (st.sp > 0) {
                 st.sp--;
-                uint32_t ref;
-                float tn;
-                st.top(ref, tn);
-                if (tn <= cull_limit(ts.t)) {
-                    ts.cur = ref;
+                const uint64_t e = st.top();
+                if (__uint_as_float(uint32_t(e >> 32)) <= lim) {
+                    ts.cur = uint32_t(e);
                     more = true;
                     break;
                 }
@@ -380,7 +338,7 @@ __device__ __forceinline__ bool trav_round_any(const G &s, TravState &ts, LaneSt
         } else {
             float tn[4];
             bool h[4];
-            const uint4 ch = node_test(s, r, cur, ts.t, tn, h);
+            const uint4 ch = node_test(s, r, cur, cull_limit(ts.t), tn, h);
             if (COUNT) cnt.nodes++;
             const uint32_t c[4] = {ch.x, ch.y, ch.z, ch.w};
             uint32_t next = kInvalid;
@@ -419,25 +377,8 @@ __device__ __forceinline__ bool trav_round_any(const G &s, TravState &ts, LaneSt
     return live && !more;
 }
 
-// Whole traversal of one ray (used where rays are not refilled per lane).
-template <bool ANY, bool COUNT, int kBlock, class G>
-__device__ __forceinline__ uint32_t traverse(const G &s, const RaySetup &r, float &t_best, LaneStack<kBlock> &st,
-                                             TraversalCounters &cnt) {
-    TravState ts;
-    ts.r = r;
-    ts.t = t_best;
-    ts.hit = kInvalid;
-    ts.cur = 0;
-    st.sp = 0;
-    while (!trav_step<ANY, COUNT, kBlock>(s, ts, st, cnt)) {
-    }
-    t_best = ts.t;
-    return ts.hit;
-}
-
 // Whole traversal of one ray in speculative rounds (trav_round / trav_round_any):
-// the same hits as traverse() (ties in t go to the lower leaf-order index in
-// both).  The rounds' ballots count the lanes that are active, so lanes of a
+// ties in t go to the lower leaf-order index.  The rounds' ballots count the lanes that are active, so lanes of a
 // wave may enter and leave it at different times.
 template <bool ANY, int kBlock, class G>
 __device__ __forceinline__ uint32_t traverse_rounds(const G &s, const RaySetup &r, float &t_best, LaneStack<kBlock> &st,
