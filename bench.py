@@ -635,13 +635,16 @@ def main():
         shd_per_launch = shd / shd_launches
         tp = os.path.join(ROOT, "profiles", "extend_traffic.json")
         prof = json.load(open(tp)) if os.path.exists(tp) else {}
+        ext_prof, shd_prof = prof.get("extend", {}), prof.get("shadow", {})
         roof = {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None, "traffic": None,
                 "kernel": "k_extend_fast (closest-hit BVH4 traversal)", "avg_launch_ms": round(avg_ms, 4),
                 "launches": ext_launches, "rays_per_launch": round(rays_per_launch),
-                "traffic_source": "profiles/extend_traffic.json: HBM bytes per extension ray from rocprofv3 FETCH_SIZE "
-                                  "and WRITE_SIZE of the same build on the same workload, scaled by a 1 GiB copy"}
-        if prof.get("extend_hbm_bytes_per_ray"):
-            traffic = prof["extend_hbm_bytes_per_ray"] * rays_per_launch
+                "traffic_source": "profiles/extend_traffic.json (build " + str(prof.get("build")) + "): bytes leaving "
+                                  "the L2 toward the fabric per extension ray, from rocprofv3 TCC_EA0_RDREQ_128B/64B/32B "
+                                  "and WRITE_SIZE over the whole frame; the 256 MB MALL on that side serves part of "
+                                  "them before HBM, so this bounds the HBM bytes from above"}
+        if ext_prof.get("l2_egress_bytes_per_ray"):
+            traffic = ext_prof["l2_egress_bytes_per_ray"] * rays_per_launch
             achieved = traffic / (avg_ms * 1e-3) / 1e9
             roof.update(achieved=round(achieved, 1), frac=round(achieved / HBM_PEAK_GBS, 4), traffic=round(traffic))
         # the cache-inclusive rate: every visited node and tested triangle counted
@@ -650,11 +653,16 @@ def main():
         roof["algorithmic_incl_cache"] = {
             "bytes_per_ray": round(ext_bpr, 1), "nodes_per_ray": round(nodes_per_ray, 2),
             "tris_per_ray": round(tris_per_ray, 2), "gbs": round(alg, 1), "frac_of_hbm_peak": round(alg / HBM_PEAK_GBS, 4)}
+        # the compulsory rate: rays in and hits out, plus the whole BVH read once per launch
+        bvh_bytes = info["nodes"] * NODE_BYTES + sc.n_tris * TRI_BYTES
+        comp = (rays_per_launch * EXT_RAY_BYTES + bvh_bytes) / (avg_ms * 1e-3) / 1e9
+        roof["compulsory"] = {"bytes_per_launch": round(rays_per_launch * EXT_RAY_BYTES + bvh_bytes),
+                              "gbs": round(comp, 1), "frac_of_hbm_peak": round(comp / HBM_PEAK_GBS, 4)}
         shadow = {"kernel": "k_shadow (any-hit BVH4 traversal)", "avg_launch_ms": round(shd_avg_ms, 4),
                   "rays_per_launch": round(shd_per_launch), "nodes_per_ray": round(shd_nodes, 2),
                   "algorithmic_incl_cache_gbs": round(shd_per_launch * shd_bpr / (shd_avg_ms * 1e-3) / 1e9, 1)}
-        if prof.get("shadow_hbm_bytes_per_ray"):
-            st_ = prof["shadow_hbm_bytes_per_ray"] * shd_per_launch
+        if shd_prof.get("l2_egress_bytes_per_ray"):
+            st_ = shd_prof["l2_egress_bytes_per_ray"] * shd_per_launch
             shadow.update(traffic=round(st_), achieved=round(st_ / (shd_avg_ms * 1e-3) / 1e9, 1),
                           frac=round(st_ / (shd_avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4))
         roof["shadow"] = shadow
