@@ -48,7 +48,7 @@ def parse():
     ap.add_argument("--ttr-target", type=float, default=1e-3)
     ap.add_argument("--ttr-ref-spp", type=int, default=65536)
     ap.add_argument("--ttr-runs", type=int, default=3)
-    ap.add_argument("--ttr-train-spp", type=int, default=64)
+    ap.add_argument("--ttr-train-spp", type=int, default=16)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-scene measurement")
     ap.add_argument("--no-integrators", action="store_true",
                     help="skip primitive / light tracing / visualize / envmap on the headline frame")
@@ -63,7 +63,7 @@ def parse():
     ap.add_argument("--med-height", type=int, default=720)
     ap.add_argument("--med-spp", type=int, default=512)
     ap.add_argument("--med-ref-spp", type=int, default=6144)
-    ap.add_argument("--med-train-spp", type=int, default=64)
+    ap.add_argument("--med-train-spp", type=int, default=16)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first copy 1 GiB device-to-device three times (known-size kernel for byte counters)")
     return ap.parse_args()
@@ -101,7 +101,7 @@ def rel_mse(img, ref, eps=1e-2):
 
 def checkpoints(first, max_spp, growth=1.25, step=8):
     """spp checkpoints: every `step` samples up to 128 (so online training runs
-    whole 8-spp passes), then x growth, rounded up to a multiple of `step`."""
+    whole passes), then x growth, rounded up to a multiple of `step`."""
     out, spp = [], first
     while spp <= max_spp:
         out.append(spp)
@@ -157,20 +157,26 @@ def reference(wr, gs, spp, N, seeds):
 
 
 # The guided schedule every guided measurement uses: the network guides the
-# first 5 bounces with probability 0.4 (deeper vertices sample the BSDF only),
-# and trains online on the first 64 spp (8-spp passes, 64 Adam steps of 2^18
-# records at lr 1e-2 after each).  Picked on the 1080p gap room at equal
-# relMSE (256 spp, medians of 3): bounce limit 4/5/6/all 2.02/2.10/2.03/1.42x,
-# guide fraction 0.3/0.4/0.5 1.98/2.10/1.85x, 64 spp x 64 steps against
-# 128 x 32 2.10/1.85x.
-GUIDE = dict(guide_fraction=0.4, max_guided_bounce=5, train_spp=64, spp_per_pass=8, train_steps=64)
+# first 5 bounces with probability 0.4 (deeper vertices sample the BSDF only)
+# and trains online on the first 16 spp, as 2-spp passes each followed by 256
+# Adam steps (lr 1e-2, batches of 2^18) over that pass's records (up to 2^22,
+# a random subset of the pass's guided vertices), then renders the rest with
+# the trained network.  Picked on the 1080p gap room at 256 spp (guided
+# relMSE x time against plain's, two runs each): 8 x 8 spp with 64 steps
+# 2.12x; 16 x 2 spp with 64 steps over 2^20 records 2.61-2.91x; 8 x 2 spp
+# with 256 steps over 2^22 records 2.88-3.01x (192 steps 2.71-3.03x; 4 x 4 spp
+# with 512 steps over 2^23 records 2.93-3.08x; 12 x 2 spp 2.56-2.90x).  A
+# pass's record count is capped, so small passes give a full record set at a
+# fraction of the render time, and the steps, not the samples, train the net.
+GUIDE = dict(guide_fraction=0.4, max_guided_bounce=5, train_spp=16, spp_per_pass=2, train_steps=256,
+             record_capacity=1 << 22)
 
 
 def guided_renderer(wr, gs, N, seed, train_spp=GUIDE["train_spp"], fp8=False):
     """Online guided rendering as benchmarked (GUIDE): a fresh network, trained
     between the passes of the first train_spp samples per pixel, then used
     without further training."""
-    g = wr.Guide(gs, learning_rate=1e-2, batch_size=1 << 18, record_capacity=1 << 20, seed=seed)
+    g = wr.Guide(gs, learning_rate=1e-2, batch_size=1 << 18, record_capacity=GUIDE["record_capacity"], seed=seed)
     state = {"calls": 0}
     kw = dict(num_path_vertices=N, seed=seed, guide_fraction=GUIDE["guide_fraction"],
               max_guided_bounce=GUIDE["max_guided_bounce"], fp8_inference=fp8)
@@ -248,7 +254,7 @@ def time_to_relmse(wr, scenes, dev, a):
 def config4(wr, scenes, dev, a):
     """The learned-guiding workload at its own size: the gap-lit room at
     1920x1080, max depth 16.  Guided renders 256 spp (the GUIDE schedule:
-    online training over the first 64); plain renders
+    online training over the first 16); plain renders
     progressively (x1.25 checkpoints) until it reaches the guided image's
     relMSE, and its time is interpolated there.  relMSE against a plain
     reference of c4_ref_spp with its noise floor subtracted; medians of c4_runs."""
@@ -289,7 +295,7 @@ def config4(wr, scenes, dev, a):
     tg = float(np.median(g_t))
     tp = float(np.median(p_t)) if all(x is not None for x in p_t) else None
     return {"workload": f"guide_room seed 4 (1,003,550 tris, gap 0.2, L 500), {a.c4_width}x{a.c4_height}, max "
-                        f"depth 16, guided 256 spp (64 training) vs plain at equal relMSE",
+                        f"depth 16, guided 256 spp ({GUIDE['train_spp']} training) vs plain at equal relMSE",
             "guided_schedule": {**GUIDE, "batch": 1 << 18, "lr": 1e-2},
             "reference": {"spp": a.c4_ref_spp, "estimator": "plain GPU (pinned to the FP64 oracle in tests)",
                           "noise_floor_relmse": floor, "seconds": round(ref_s, 2)},
@@ -338,7 +344,7 @@ def medium_fp8(wr, scenes, dev, a):
     res["plain"] = {"seconds": round(sec, 3), "relmse": rel_mse(film.image(), ref) - floor,
                     "mrays_per_s": round((st.extend_rays + st.shadow_rays) / sec / 1e6, 1)}
     film.close()
-    g = wr.Guide(gs, learning_rate=1e-2, batch_size=1 << 18, record_capacity=1 << 20, seed=7)
+    g = wr.Guide(gs, learning_rate=1e-2, batch_size=1 << 18, record_capacity=GUIDE["record_capacity"], seed=7)
     film = wr.Film(gs)
     t = time.perf_counter()
     wr.render_guided(gs, film, g, a.med_train_spp, num_path_vertices=N, seed=7150,

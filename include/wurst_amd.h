@@ -391,6 +391,11 @@ typedef struct {
     uint32_t batch_size;       /* records per training step (2^18) */
     uint32_t record_capacity;  /* training records kept per pass (2^20) */
     uint32_t seed;             /* weight initialisation */
+    float avg_decay;           /* the renderer's inference uses the exponential average of the
+                                  parameters over the Adam steps, avg = decay avg + (1 - decay)
+                                  params after each step, zero-started at creation and
+                                  wr_guide_set_params and divided by 1 - decay^steps (0.99);
+                                  0: inference uses the parameters themselves */
 } wr_guide_config;
 
 typedef struct {
@@ -405,6 +410,9 @@ typedef struct {
                                    and recorded; deeper ones sample the BSDF or phase function
                                    only (probability 0 of the network, so still unbiased).
                                    0 = every bounce */
+    uint32_t overlap_training; /* 1: pass q renders with the network trained on pass q - 2, so
+                                  that the training on pass q - 1 runs beside it (give such a
+                                  render two warm-up passes); 0: pass q waits for that training */
 } wr_guided_params;
 
 typedef struct {

@@ -77,12 +77,14 @@ class Medium(C.Structure):
 
 class GuideConfig(C.Structure):
     _fields_ = [("learning_rate", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("epsilon", C.c_float),
-                ("batch_size", C.c_uint32), ("record_capacity", C.c_uint32), ("seed", C.c_uint32)]
+                ("batch_size", C.c_uint32), ("record_capacity", C.c_uint32), ("seed", C.c_uint32),
+                ("avg_decay", C.c_float)]
 
 
 class GuidedParams(C.Structure):
     _fields_ = [("guide_fraction", C.c_float), ("spp_per_pass", C.c_uint32), ("train_steps", C.c_uint32),
-                ("warmup_passes", C.c_uint32), ("fp8_inference", C.c_int32), ("max_guided_bounce", C.c_uint32)]
+                ("warmup_passes", C.c_uint32), ("fp8_inference", C.c_int32), ("max_guided_bounce", C.c_uint32),
+                ("overlap_training", C.c_uint32)]
 
 
 class GuideStats(C.Structure):
@@ -540,13 +542,17 @@ class Guide:
     """The path-guiding network bound to a scene (wr_guide_* of the C-ABI)."""
 
     def __init__(self, scene: GpuScene, learning_rate: float = 1e-2, batch_size: int = 1 << 18,
-                 record_capacity: int = 1 << 20, seed: int = 0):
+                 record_capacity: int = 1 << 20, seed: int = 0, avg_decay: float | None = None):
+        """avg_decay: decay of the parameter average the renderer's inference uses
+        (None: the library default, 0.99; 0: the parameters themselves)."""
         cfg = GuideConfig()
         lib().wr_guide_config_default(C.byref(cfg))
         cfg.learning_rate = learning_rate
         cfg.batch_size = batch_size
         cfg.record_capacity = record_capacity
         cfg.seed = seed
+        if avg_decay is not None:
+            cfg.avg_decay = avg_decay
         h = C.c_void_p()
         _check(lib().wr_guide_create(scene.h, C.byref(cfg), C.byref(h)))
         self.h = h
@@ -612,16 +618,17 @@ def render_guided(scene: GpuScene, film: Film, guide: Guide, spp: int, num_path_
                   guide_fraction: float = 0.5, spp_per_pass: int = 4, train_steps: int = 4, warmup_passes: int = 1,
                   use_mis: bool = True, first_sample: int | None = None, max_paths_in_flight: int = 0,
                   profile: bool = False, fp8_inference: bool = False, use_graph: bool = False,
-                  max_guided_bounce: int = 0):
+                  max_guided_bounce: int = 0, overlap_training: bool = False):
     """wr_render_guided: progressive passes of spp_per_pass, training between passes;
     fp8_inference runs the renderer's network inference in e4m3; use_graph replays
-    each pass's iterations as a captured hipGraph."""
+    each pass's iterations as a captured hipGraph; overlap_training renders pass q
+    with the network trained on pass q - 2 while the training on pass q - 1 runs."""
     if first_sample is None:
         first_sample = film.spp
     p = RenderParams(spp, num_path_vertices, seed, int(bool(use_mis)), first_sample, max_paths_in_flight,
                      int(bool(profile)), 0, int(bool(use_graph)))
     gp = GuidedParams(guide_fraction, spp_per_pass, train_steps, warmup_passes, int(bool(fp8_inference)),
-                      max_guided_bounce)
+                      max_guided_bounce, int(bool(overlap_training)))
     st, gs = RenderStats(), GuideStats()
     _check(lib().wr_render_guided(scene.h, film.h, guide.h, C.byref(p), C.byref(gp), C.byref(st), C.byref(gs)))
     film.spp += spp

@@ -28,8 +28,10 @@ constexpr int kMaxTrainBlocks = 256;
 constexpr uint32_t kTestBlock = 64;
 
 // ---------------------------------------------------------------- pack
-__device__ __forceinline__ void pack_bf16(int j, const float *params, u32x4 *fwd, u32x4 *bwd, float *bias) {
-    if (j < kBiasPacked) bias[j] = params[b_offset(j / 64) + j % 64];
+// `sc` scales every parameter as it is read (1, or the debiasing factor of the
+// parameter average); a null `bwd` skips the backward operands.
+__device__ __forceinline__ void pack_bf16(int j, const float *params, float sc, u32x4 *fwd, u32x4 *bwd, float *bias) {
+    if (j < kBiasPacked) bias[j] = sc * params[b_offset(j / 64) + j % 64];
     if (j < kFwdOps) {
         int l = 0;
         while (l + 1 < kLayers && j >= fwd_offset(l + 1)) l++;
@@ -39,11 +41,11 @@ __device__ __forceinline__ void pack_bf16(int j, const float *params, u32x4 *fwd
         const int o = 32 * mt + (lane & 31), h = lane >> 5;
         u32x4 v;
         for (int i = 0; i < 4; i++)
-            v[i] = o < N ? pack2(params[w_offset(l) + kperm(ks, h, 2 * i) * N + o],
-                                 params[w_offset(l) + kperm(ks, h, 2 * i + 1) * N + o])
+            v[i] = o < N ? pack2(sc * params[w_offset(l) + kperm(ks, h, 2 * i) * N + o],
+                                 sc * params[w_offset(l) + kperm(ks, h, 2 * i + 1) * N + o])
                          : 0u;
         fwd[j] = v;
-    } else if (j < kFwdOps + kBwdOps) {
+    } else if (bwd && j < kFwdOps + kBwdOps) {
         const int jb = j - kFwdOps;
         int l = 1;
         while (l + 1 < kLayers && jb >= bwd_offset(l + 1)) l++;
@@ -54,7 +56,8 @@ __device__ __forceinline__ void pack_bf16(int j, const float *params, u32x4 *fwd
         u32x4 v;
         for (int k = 0; k < 4; k++) {
             const int o0 = kperm(ks, h, 2 * k), o1 = kperm(ks, h, 2 * k + 1);
-            v[k] = pack2(o0 < N ? params[w_offset(l) + i * N + o0] : 0.f, o1 < N ? params[w_offset(l) + i * N + o1] : 0.f);
+            v[k] = pack2(o0 < N ? sc * params[w_offset(l) + i * N + o0] : 0.f,
+                         o1 < N ? sc * params[w_offset(l) + i * N + o1] : 0.f);
         }
         bwd[jb] = v;
     }
@@ -64,11 +67,11 @@ __device__ __forceinline__ void pack_bf16(int j, const float *params, u32x4 *fwd
 // layer's largest |weight| sets its power-of-two scale (wscale[l] = 2^k, the
 // operands hold W / 2^k, at most 128 in magnitude).
 constexpr int kPack8Block = 256;
-__device__ __forceinline__ void pack_e4m3(int l, float *red, const float *params, u32x2 *fwd8, float *wscale) {
+__device__ __forceinline__ void pack_e4m3(int l, float *red, const float *params, float sc, u32x2 *fwd8, float *wscale) {
     const int K = layer_in(l), N = layer_out(l), nks = K / 16;
     const float *W = params + w_offset(l);
     float m = 0.f;
-    for (int j = threadIdx.x; j < K * N; j += kPack8Block) m = fmaxf(m, fabsf(W[j]));
+    for (int j = threadIdx.x; j < K * N; j += kPack8Block) m = fmaxf(m, fabsf(sc * W[j]));
     red[threadIdx.x] = m;
     __syncthreads();
     for (int s = kPack8Block / 2; s > 0; s >>= 1) {
@@ -83,17 +86,17 @@ __device__ __forceinline__ void pack_e4m3(int l, float *red, const float *params
         const int mt = jj / (nks * 64), ks = (jj / 64) % nks, lane = jj % 64;
         const int o = 32 * mt + (lane & 31), h = lane >> 5;
         float v[8];
-        for (int e = 0; e < 8; e++) v[e] = o < N ? W[kperm(ks, h, e) * N + o] : 0.f;
+        for (int e = 0; e < 8; e++) v[e] = o < N ? sc * W[kperm(ks, h, e) * N + o] : 0.f;
         fwd8[fwd_offset(l) + jj] = quant8(v, inv);
     }
 }
 // Both packings in one launch: blocks 0 .. kLayers-1 quantise one layer each,
 // the rest convert 256 BF16 operands each.
-__global__ __launch_bounds__(kPack8Block) void k_pack(const float *params, u32x4 *fwd, u32x4 *bwd, float *bias,
+__global__ __launch_bounds__(kPack8Block) void k_pack(const float *params, float sc, u32x4 *fwd, u32x4 *bwd, float *bias,
                                                     u32x2 *fwd8, float *wscale) {
     __shared__ float red[kPack8Block];
-    if (blockIdx.x < kLayers) pack_e4m3(blockIdx.x, red, params, fwd8, wscale);
-    else pack_bf16((blockIdx.x - kLayers) * kPack8Block + threadIdx.x, params, fwd, bwd, bias);
+    if (blockIdx.x < kLayers) pack_e4m3(blockIdx.x, red, params, sc, fwd8, wscale);
+    else pack_bf16((blockIdx.x - kLayers) * kPack8Block + threadIdx.x, params, sc, fwd, bwd, bias);
 }
 
 __global__ void k_fill_u32_guide(uint32_t *p, uint32_t n) {
@@ -379,6 +382,8 @@ struct AdamArgs {
     float lr_t;  // lr * sqrt(1 - beta2^t) / (1 - beta1^t)
     float beta1, beta2, eps_t;  // eps * sqrt(1 - beta2^t)
     float *grad_out;            // non-null: store the summed gradient, no update
+    float *avg;                 // non-null: avg = decay avg + (1 - decay) params after the update
+    float avg_decay;
 };
 // 64 parameters per block, 16 threads per parameter: each sums every 16th
 // block partial (coalesced over the parameters, four loads in flight), then
@@ -411,7 +416,9 @@ __global__ __launch_bounds__(kAdamBlock) void k_adam(AdamArgs a) {
     const float v = a.beta2 * a.v[j] + (1.f - a.beta2) * g * g;
     a.m[j] = m;
     a.v[j] = v;
-    a.params[j] -= a.lr_t * m / (sqrtf(v) + a.eps_t);
+    const float p = a.params[j] - a.lr_t * m / (sqrtf(v) + a.eps_t);
+    a.params[j] = p;
+    if (a.avg) a.avg[j] = a.avg_decay * a.avg[j] + (1.f - a.avg_decay) * p;
 }
 
 // ---------------------------------------------------------------- finalize
@@ -479,6 +486,11 @@ struct wr_guide {
     wr_scene *scene = nullptr;
     wr_guide_config cfg{};
     float *d_params = nullptr, *d_m = nullptr, *d_v = nullptr;
+    // exponential average of the parameters after each Adam step (cfg.avg_decay),
+    // zero-started at creation and set_params and debiased by 1 / (1 - decay^avg_steps):
+    // the renderer's inference copies are packed from it
+    float *d_avg = nullptr;
+    uint64_t avg_steps = 0;
     u32x4 *d_fwd = nullptr, *d_bwd = nullptr;
     float *d_bias = nullptr;  // packed biases (kBiasPacked)
     u32x2 *d_fwd8 = nullptr;  // e4m3 forward operands (inference variant)
@@ -561,8 +573,19 @@ Net net_view(const wr_guide *g) {
 
 int pack(wr_guide *g, hipStream_t st) {
     const int n = kFwdOps + kBwdOps;
-    k_pack<<<kLayers + (n + kPack8Block - 1) / kPack8Block, kPack8Block, 0, st>>>(g->d_params, g->d_fwd, g->d_bwd,
+    k_pack<<<kLayers + (n + kPack8Block - 1) / kPack8Block, kPack8Block, 0, st>>>(g->d_params, 1.f, g->d_fwd, g->d_bwd,
                                                                                     g->d_bias, g->d_fwd8, g->d_wscale);
+    WR_HIP(hipGetLastError());
+    return WR_OK;
+}
+
+// Inference copy `set` <- the debiased parameter average (the parameters
+// themselves before the first averaged step, or with averaging off).
+int pack_inference(wr_guide *g, int set, hipStream_t st) {
+    const bool avg = g->cfg.avg_decay > 0.f && g->avg_steps > 0;
+    const float sc = avg ? float(1.0 / (1.0 - std::pow(double(g->cfg.avg_decay), double(g->avg_steps)))) : 1.f;
+    k_pack<<<kLayers + (kFwdOps + kPack8Block - 1) / kPack8Block, kPack8Block, 0, st>>>(
+        avg ? g->d_avg : g->d_params, sc, g->d_ifwd[set], nullptr, g->d_ibias[set], g->d_ifwd8[set], g->d_iwscale[set]);
     WR_HIP(hipGetLastError());
     return WR_OK;
 }
@@ -570,7 +593,9 @@ int pack(wr_guide *g, hipStream_t st) {
 int reset_adam(wr_guide *g) {
     WR_HIP(hipMemsetAsync(g->d_m, 0, sizeof(float) * kNumParams, g->scene->stream));
     WR_HIP(hipMemsetAsync(g->d_v, 0, sizeof(float) * kNumParams, g->scene->stream));
+    WR_HIP(hipMemsetAsync(g->d_avg, 0, sizeof(float) * kNumParams, g->scene->stream));
     g->step = 0;
+    g->avg_steps = 0;
     return WR_OK;
 }
 
@@ -593,6 +618,9 @@ int train_step(wr_guide *g, const float4 *pos, const float4 *nrm, const float4 *
     ad.partials = g->d_partials;
     ad.blocks = blocks;
     ad.grad_out = grad_out;
+    ad.avg = grad_out || !(g->cfg.avg_decay > 0.f) ? nullptr : g->d_avg;
+    ad.avg_decay = g->cfg.avg_decay;
+    if (ad.avg) g->avg_steps++;
     if (!grad_out) {
         g->step++;
         const double t = double(g->step);
@@ -655,10 +683,8 @@ int guide_render_begin(wr_guide *g) {
     hipStream_t st = g->scene->stream;
     WR_HIP(hipStreamSynchronize(g->tstream));
     for (int k = 0; k < 2; k++) {
-        WR_HIP(hipMemcpyAsync(g->d_ifwd[k], g->d_fwd, sizeof(u32x4) * kFwdOps, hipMemcpyDeviceToDevice, st));
-        WR_HIP(hipMemcpyAsync(g->d_ibias[k], g->d_bias, sizeof(float) * kBiasPacked, hipMemcpyDeviceToDevice, st));
-        WR_HIP(hipMemcpyAsync(g->d_ifwd8[k], g->d_fwd8, sizeof(u32x2) * kFwdOps, hipMemcpyDeviceToDevice, st));
-        WR_HIP(hipMemcpyAsync(g->d_iwscale[k], g->d_wscale, sizeof(float) * kLayers, hipMemcpyDeviceToDevice, st));
+        int rc = pack_inference(g, k, st);
+        if (rc) return rc;
         g->pending[k] = false;
     }
     g->train_marks.clear();
@@ -668,18 +694,21 @@ int guide_render_begin(wr_guide *g) {
 // Pass q (records set = q % 2) renders with the network trained on pass q - 1
 // (inference copy set ^ 1): the render stream waits for that training on the
 // GPU, which also orders it after the training of pass q - 2 that read this
-// pass's record set.  (Rendering with the network of pass q - 2 instead, so
-// that training overlaps the next pass, measured worse: the stale network cost
-// more variance than the overlap saved time.)
-int guide_pass_net(wr_guide *g, int set, guide::Net *net) {
-    const int prev = set ^ 1;
-    if (g->pending[prev]) WR_HIP(hipStreamWaitEvent(g->scene->stream, g->ev_trained[prev], 0));
-    g->pending[prev] = false;
+// pass's record set.  With `overlap`, pass q renders with the network trained
+// on pass q - 2 (inference copy `set`) and waits only for that training,
+// which read this pass's record set: the training on pass q - 1 then runs on
+// its own stream beside the pass.  (With passes of 8 spp and 32-64 steps the
+// stale network cost more variance than the overlap saved; with 2-spp passes
+// and 256 steps the training is as long as the pass.)
+int guide_pass_net(wr_guide *g, int set, bool overlap, guide::Net *net) {
+    const int use = overlap ? set : set ^ 1;
+    if (g->pending[use]) WR_HIP(hipStreamWaitEvent(g->scene->stream, g->ev_trained[use], 0));
+    g->pending[use] = false;
     *net = net_view(g);
-    net->fwd = g->d_ifwd[prev];
-    net->bias = g->d_ibias[prev];
-    net->fwd8 = g->d_ifwd8[prev];
-    net->wscale = g->d_iwscale[prev];
+    net->fwd = g->d_ifwd[use];
+    net->bias = g->d_ibias[use];
+    net->fwd8 = g->d_ifwd8[use];
+    net->wscale = g->d_iwscale[use];
     net->bwd = nullptr;
     return WR_OK;
 }
@@ -741,10 +770,8 @@ int guide_end_pass(wr_guide *g, int set, const float4 *sample_rad, uint64_t samp
         if (rc) return rc;
         gs->train_steps++;
     }
-    WR_HIP(hipMemcpyAsync(g->d_ifwd[set], g->d_fwd, sizeof(u32x4) * kFwdOps, hipMemcpyDeviceToDevice, ts));
-    WR_HIP(hipMemcpyAsync(g->d_ibias[set], g->d_bias, sizeof(float) * kBiasPacked, hipMemcpyDeviceToDevice, ts));
-    WR_HIP(hipMemcpyAsync(g->d_ifwd8[set], g->d_fwd8, sizeof(u32x2) * kFwdOps, hipMemcpyDeviceToDevice, ts));
-    WR_HIP(hipMemcpyAsync(g->d_iwscale[set], g->d_wscale, sizeof(float) * kLayers, hipMemcpyDeviceToDevice, ts));
+    int rc = pack_inference(g, set, ts);
+    if (rc) return rc;
     WR_HIP(hipEventRecord(t1, ts));
     WR_HIP(hipEventRecord(g->ev_trained[set], ts));
     g->pending[set] = true;
@@ -798,6 +825,7 @@ void wr_guide_config_default(wr_guide_config *c) {
     c->batch_size = 1u << 18;
     c->record_capacity = 1u << 20;
     c->seed = 0;
+    c->avg_decay = 0.99f;
 }
 
 void wr_guide_destroy(wr_guide *g) {
@@ -813,7 +841,7 @@ void wr_guide_destroy(wr_guide *g) {
         if (g->ev_trained[k]) (void)hipEventDestroy(g->ev_trained[k]);
     }
     if (g->tstream) (void)hipStreamDestroy(g->tstream);
-    for (void *p : {(void *)g->d_params, (void *)g->d_m, (void *)g->d_v, (void *)g->d_fwd, (void *)g->d_bwd, (void *)g->d_bias,
+    for (void *p : {(void *)g->d_params, (void *)g->d_m, (void *)g->d_v, (void *)g->d_avg, (void *)g->d_fwd, (void *)g->d_bwd, (void *)g->d_bias,
                     (void *)g->d_fwd8, (void *)g->d_wscale, (void *)g->d_partials, (void *)g->d_loss, (void *)g->d_wsum,
                     g->rec_pool, g->infer_pool, (void *)g->d_draw_dir})
         if (p) (void)hipFree(p);
@@ -828,6 +856,7 @@ static int guide_create_impl(wr_guide *g) {
     WR_HIP(hipMalloc(reinterpret_cast<void **>(&g->d_params), sizeof(float) * kNumParams));
     WR_HIP(hipMalloc(reinterpret_cast<void **>(&g->d_m), sizeof(float) * kNumParams));
     WR_HIP(hipMalloc(reinterpret_cast<void **>(&g->d_v), sizeof(float) * kNumParams));
+    WR_HIP(hipMalloc(reinterpret_cast<void **>(&g->d_avg), sizeof(float) * kNumParams));
     WR_HIP(hipMalloc(reinterpret_cast<void **>(&g->d_fwd), sizeof(u32x4) * kFwdOps));
     WR_HIP(hipMalloc(reinterpret_cast<void **>(&g->d_bwd), sizeof(u32x4) * kBwdOps));
     WR_HIP(hipMalloc(reinterpret_cast<void **>(&g->d_bias), sizeof(float) * kBiasPacked));
@@ -878,7 +907,7 @@ int wr_guide_create(wr_scene *s, const wr_guide_config *cfg, wr_guide **out) {
     if (c.batch_size == 0 || c.record_capacity == 0 || c.record_capacity > (1u << 26))
         return set_error(WR_ERR_INVALID_ARGUMENT, "batch_size must be > 0 and record_capacity in [1, 2^26]");
     if (!(c.learning_rate > 0.f) || !(c.beta1 >= 0.f && c.beta1 < 1.f) || !(c.beta2 >= 0.f && c.beta2 < 1.f) ||
-        !(c.epsilon > 0.f))
+        !(c.epsilon > 0.f) || !(c.avg_decay >= 0.f && c.avg_decay < 1.f))
         return set_error(WR_ERR_INVALID_ARGUMENT, "bad Adam hyper-parameters");
     wr_guide *g = new wr_guide;
     g->scene = s;

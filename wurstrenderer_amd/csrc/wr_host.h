@@ -175,7 +175,7 @@ int build_bvh4_gpu(const float *d_verts, const uint32_t *d_tris, const uint32_t 
 // enqueued on the guide's own stream); guide_render_end waits for the last
 // training.
 int guide_render_begin(wr_guide *g);
-int guide_pass_net(wr_guide *g, int set, wr::guide::Net *net);
+int guide_pass_net(wr_guide *g, int set, bool overlap, wr::guide::Net *net);
 int guide_shade_view(wr_guide *g, int set, float alpha, float record_prob, bool collect, uint32_t max_bounce,
                      wr::guide::Shade *out);
 // Clears the record set and picks the fraction of paths to record so that

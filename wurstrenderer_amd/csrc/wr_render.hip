@@ -2384,7 +2384,7 @@ static int render_impl(wr_scene *s, wr_film *f, const wr_render_params *prm, wr_
         const bool collect = g && gp->train_steps > 0;
         const int set = pass & 1;  // record set and inference copy of this pass
         if (g) {
-            rc = guide_pass_net(g, set, &net);
+            rc = guide_pass_net(g, set, gp->overlap_training != 0, &net);
             if (!rc && collect) rc = guide_begin_pass(g, set, a.total, &record_prob);
             if (rc) return rc;
             const bool sample_net = pass >= int(gp->warmup_passes);
