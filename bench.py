@@ -42,7 +42,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU time of the baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-count", action="store_true", help="skip the traversal-counting pass")
-    ap.add_argument("--no-ttr", action="store_true", help="skip the guided-vs-plain time-to-relMSE measurement")
+    ap.add_argument("--no-ttr", action="store_true", help="(the default) no 480x270 time-to-relMSE measurement")
+    ap.add_argument("--small-ttr", action="store_true",
+                    help="also measure time-to-relMSE on the 480x270 gap room against a 65,536-spp reference")
     ap.add_argument("--ttr-width", type=int, default=480)
     ap.add_argument("--ttr-height", type=int, default=270)
     ap.add_argument("--ttr-target", type=float, default=1e-3)
@@ -58,6 +60,7 @@ def parse():
     ap.add_argument("--c4-height", type=int, default=1080)
     ap.add_argument("--c4-ref-spp", type=int, default=8192)
     ap.add_argument("--c4-runs", type=int, default=3)
+    ap.add_argument("--c4-plain-runs", type=int, default=2)
     ap.add_argument("--no-medium", action="store_true", help="skip the medium BF16/FP8 inference measurement")
     ap.add_argument("--med-width", type=int, default=1280)
     ap.add_argument("--med-height", type=int, default=720)
@@ -125,12 +128,13 @@ def crossing(curve, target):
     return None
 
 
-def progressive(render, film, ref, floor, target, points):
+def progressive(render, film, ref, floor, target, points, done=0):
     """Renders up to each checkpoint in turn (only the render calls are timed) and
     records relMSE against ref minus the reference's own noise floor (for
     independent unbiased images E[(I-R)^2] = Var I + Var R); stops at the first
-    checkpoint at or below target.  Returns (seconds, crossing seconds, spp, curve)."""
-    t_total, done, curve = 0.0, 0, []
+    checkpoint at or below target.  `done`: samples per pixel the film already
+    holds.  Returns (seconds, crossing seconds, spp, curve)."""
+    t_total, curve = 0.0, []
     for spp in points:
         t = time.perf_counter()
         render(film, spp - done)
@@ -253,11 +257,15 @@ def time_to_relmse(wr, scenes, dev, a):
 
 def config4(wr, scenes, dev, a):
     """The learned-guiding workload at its own size: the gap-lit room at
-    1920x1080, max depth 16.  Guided renders 256 spp (the GUIDE schedule:
-    online training over the first 16); plain renders
-    progressively (x1.25 checkpoints) until it reaches the guided image's
-    relMSE, and its time is interpolated there.  relMSE against a plain
-    reference of c4_ref_spp with its noise floor subtracted; medians of c4_runs."""
+    1920x1080, max depth 16, against a plain reference of c4_ref_spp (two
+    independent halves; relMSE values have its noise floor subtracted).
+    Guided (the GUIDE schedule: online training over the first 16 spp) renders
+    256 spp, then continues at x1.25 checkpoints until relMSE <= ttr_target;
+    plain renders progressively from 64 spp at x1.25 checkpoints until it
+    reaches ttr_target.  Reported: plain's time to the guided 256-spp image's
+    relMSE (equal relMSE) and both estimators' times to ttr_target, crossings
+    interpolated in log-log between checkpoints; medians over c4_runs guided
+    and c4_plain_runs plain runs."""
     import numpy as np
     N = 18
     sc = scenes.guide_room(4, resolution=(a.c4_width, a.c4_height))
@@ -265,7 +273,7 @@ def config4(wr, scenes, dev, a):
     ref, floor, ref_s = reference(wr, gs, a.c4_ref_spp, N, (8001, 8002))
     print(f"# config-4 reference: {a.c4_ref_spp} spp in {ref_s:.1f}s, noise floor ~{floor:.2e}", file=sys.stderr,
           flush=True)
-    g_t, g_e = [], []
+    g_t, g_e, g_x, g_curves = [], [], [], []
     for r in range(a.c4_runs):
         render, g = guided_renderer(wr, gs, N, 300 + r)
         film = wr.Film(gs)
@@ -274,36 +282,51 @@ def config4(wr, scenes, dev, a):
         render(film, 256 - GUIDE["train_spp"])
         g_t.append(time.perf_counter() - t)
         g_e.append(rel_mse(film.image(), ref) - floor)
+        curve = [[256, round(g_t[-1], 4), g_e[-1]]]
+        if g_e[-1] > a.ttr_target:
+            _, _, _, more = progressive(render, film, ref, floor, a.ttr_target, checkpoints(320, a.c4_ref_spp, step=64),
+                                        done=256)
+            curve += [[c[0], round(c[1] + g_t[-1], 4), c[2]] for c in more]
+        g_x.append(crossing(curve, a.ttr_target))
+        g_curves.append(curve)
         film.close()
         g.close()
-        print(f"# config-4 guided run {r}: 256 spp in {g_t[-1]:.3f} s, relMSE {g_e[-1]:.4e}", file=sys.stderr,
-              flush=True)
+        print(f"# config-4 guided run {r}: 256 spp in {g_t[-1]:.3f} s, relMSE {g_e[-1]:.4e}; relMSE "
+              f"{a.ttr_target:g} at {g_x[-1] if g_x[-1] is None else round(g_x[-1], 3)} s ({curve[-1][0]} spp)",
+              file=sys.stderr, flush=True)
     target = float(np.median(g_e))
-    points = checkpoints(64, a.c4_ref_spp // 2)
-    p_t, p_spp, p_curves = [], [], []
-    for r in range(a.c4_runs):
+    points = checkpoints(64, a.c4_ref_spp, step=64)
+    p_eq, p_x, p_curves = [], [], []
+    for r in range(a.c4_plain_runs):
         film = wr.Film(gs)
         render = (lambda seed: lambda f, spp: wr.render_nee(gs, f, spp, num_path_vertices=N, seed=seed))(400 + r)
-        tt, tc, spp, curve = progressive(render, film, ref, floor, target, points)
+        _, _, spp, curve = progressive(render, film, ref, floor, a.ttr_target, points)
         film.close()
-        p_t.append(tc)
-        p_spp.append(spp)
+        p_eq.append(crossing(curve, target))
+        p_x.append(crossing(curve, a.ttr_target))
         p_curves.append(curve)
-        print(f"# config-4 plain run {r}: relMSE {target:.4e} at {tc if tc is None else round(tc, 3)} s "
-              f"(checkpoint {spp} spp)", file=sys.stderr, flush=True)
+        print(f"# config-4 plain run {r}: relMSE {target:.4e} at {p_eq[-1] if p_eq[-1] is None else round(p_eq[-1], 3)}"
+              f" s, relMSE {a.ttr_target:g} at {p_x[-1] if p_x[-1] is None else round(p_x[-1], 3)} s ({spp} spp)",
+              file=sys.stderr, flush=True)
     gs.close()
-    tg = float(np.median(g_t))
-    tp = float(np.median(p_t)) if all(x is not None for x in p_t) else None
+    med = lambda xs: float(np.median(xs)) if xs and all(x is not None for x in xs) else None
+    tg, tp, xg, xp = float(np.median(g_t)), med(p_eq), med(g_x), med(p_x)
     return {"workload": f"guide_room seed 4 (1,003,550 tris, gap 0.2, L 500), {a.c4_width}x{a.c4_height}, max "
-                        f"depth 16, guided 256 spp ({GUIDE['train_spp']} training) vs plain at equal relMSE",
-            "guided_schedule": {**GUIDE, "batch": 1 << 18, "lr": 1e-2},
+                        f"depth 16",
+            "guided_schedule": {**GUIDE, "batch": 1 << 18, "lr": 1e-2, "parameter_average_decay": 0.99},
             "reference": {"spp": a.c4_ref_spp, "estimator": "plain GPU (pinned to the FP64 oracle in tests)",
-                          "noise_floor_relmse": floor, "seconds": round(ref_s, 2)},
-            "guided": {"seconds_median": round(tg, 3), "relmse_median": target, "seconds_runs": g_t,
-                       "relmse_runs": g_e},
-            "plain": {"seconds_to_equal_relmse_median": tp, "seconds_runs": p_t, "checkpoint_spp": p_spp,
-                      "curve_run0": [[c[0], c[1], float(f"{c[2]:.4e}")] for c in p_curves[0]]},
-            "speedup_guided_vs_plain": None if tp is None else round(tp / tg, 3)}
+                          "noise_floor_relmse": floor, "seconds": round(ref_s, 2),
+                          "note": "relMSE values have the reference's noise floor subtracted"},
+            "equal_relmse_at_guided_256spp": {
+                "guided_seconds_median": round(tg, 3), "relmse_median": target, "guided_seconds_runs": g_t,
+                "guided_relmse_runs": g_e, "plain_seconds_median": tp, "plain_seconds_runs": p_eq,
+                "speedup_guided_vs_plain": None if tp is None else round(tp / tg, 3)},
+            "time_to_relmse": {
+                "target_relmse": a.ttr_target, "guided_seconds_median": xg, "guided_seconds_runs": g_x,
+                "plain_seconds_median": xp, "plain_seconds_runs": p_x,
+                "speedup_guided_vs_plain": None if xg is None or xp is None else round(xp / xg, 3),
+                "guided_curve_run0": [[c[0], c[1], float(f"{c[2]:.4e}")] for c in g_curves[0]],
+                "plain_curve_run0": [[c[0], c[1], float(f"{c[2]:.4e}")] for c in p_curves[0]]}}
 
 
 def medium_fp8(wr, scenes, dev, a):
@@ -679,7 +702,7 @@ def main():
     if not a.no_batch:
         bat = batched(wr, scenes, dev, a)
     ttr = None
-    if not a.no_ttr:
+    if a.small_ttr:
         ttr = time_to_relmse(wr, scenes, dev, a)
     c4 = None
     if not a.no_config4:
@@ -713,8 +736,8 @@ def main():
                       ("extend_ms", "shadow_ms", "shade_ms", "generate_ms", "accumulate_ms")},
         "roofline": roof,
         "cpu_baseline": cpu,
-        "time_to_relmse": ttr,
-        "guided_1080p_equal_relmse": c4,
+        "guided_vs_plain_1080p": c4,
+        "time_to_relmse_480x270": ttr,
         "batched": bat,
         "medium_bf16_fp8": med,
         "bvh_build": bvh,
