@@ -47,6 +47,32 @@ def test_cli_render_equals_api_and_oracle(tmp_path, gpu_device):
     assert oracle.rel_mse(img, ref) * 30 < oracle.rel_mse(other, ref)
 
 
+def test_cli_guided_schedule_renders_the_same_mean(tmp_path, gpu_device):
+    """"guiding" with train_spp: the host trains on the first train_spp samples
+    per pixel (passes of spp_per_pass, train_steps Adam steps after each), then
+    renders the rest in one pass with the trained network.  Every sample is
+    counted once, the image is finite and non-negative, and its mean is the
+    plain estimator's (the guided estimator is unbiased for any network)."""
+    sc = scenes.cornell(1, resolution=(64, 48))
+    scenes.write_obj(sc, str(tmp_path), "mesh")
+    guiding = {"guide_fraction": 0.4, "spp_per_pass": 2, "train_steps": 16, "train_spp": 4, "max_guided_bounce": 5,
+               "record_capacity": 65536, "batch_size": 8192, "seed": 11}
+    j = scenes.scene_json(sc, "mesh.obj", [{"integrator": "nextevent_path_tracer", "output": "out.pfm", "num_spp": 32,
+                                            "num_path_vertices": 8, "seed": 5, "guiding": guiding}])
+    (tmp_path / "scene.json").write_text(json.dumps(j))
+    r = subprocess.run([EXE, "--simple2parse", str(tmp_path / "scene.json")], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    stats = json.loads(r.stdout.strip().splitlines()[-1])
+    assert stats["paths"] == 64 * 48 * 32
+    img = read_pfm(tmp_path / "out.pfm")
+    assert np.all(np.isfinite(img)) and img.min() >= 0.0
+    gs = wr.GpuScene(gpu_device, sc)
+    film = wr.Film(gs)
+    wr.render_nee(gs, film, 256, num_path_vertices=8, seed=9)
+    plain = film.image()
+    assert abs(img.mean() / plain.mean() - 1.0) < 0.03, (img.mean(), plain.mean())
+
+
 def test_cli_primitive_and_visualize_equal_api(tmp_path, gpu_device):
     """One scene JSON with the other integrators of this build: each PFM equals
     the Python API's render of the same scene, integrator and seed."""

@@ -139,7 +139,8 @@ def test_default_txt_is_created_when_missing(tmp_path):
 
 def test_medium_and_guiding_keys_are_accepted(tmp_path):
     sc = scenes.cornell(1, resolution=(16, 16)).with_medium(sigma_a=(0.05, 0.05, 0.05), sigma_s=0.4, g=0.7)
-    p = write(tmp_path, sc, renders=(dict(NEE, guiding={"guide_fraction": 0.5, "train_steps": 2}),))
+    p = write(tmp_path, sc, renders=(dict(NEE, guiding={"guide_fraction": 0.5, "train_steps": 2, "train_spp": 4,
+                                                        "overlap_training": True, "avg_decay": 0.99}),))
     j = json.loads(p.read_text())
     assert j["medium"] == {"type": "homogeneous", "sigma_a": [0.05] * 3, "sigma_s": [0.4] * 3, "g": 0.7}
     r = run(["--check", "--simple2parse", str(p)])

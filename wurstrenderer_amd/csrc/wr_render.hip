@@ -796,7 +796,12 @@ __global__ __launch_bounds__(kInferBlock) __attribute__((amdgpu_waves_per_eu(WR_
             if constexpr (FP8) guide::forward8(net, guide::encode(pos, nf, -rd, net.box), stage);
             else guide::forward(net, guide::encode(pos, nf, -rd, net.box), stage);
             if (need) {
-                float *L = stage + lane * guide::kStageStride;
+                // the record's raw outputs into registers: decode, the draw and the
+                // four pdf evaluations read them there instead of from LDS
+                float L[guide::kOut];
+                const float *row = stage + lane * guide::kStageStride;
+#pragma unroll
+                for (int o = 0; o < guide::kOut; o++) L[o] = row[o];
                 float wl[guide::kLobes];
                 guide::decode(L, wl);
                 const uint32_t key = meta.x;

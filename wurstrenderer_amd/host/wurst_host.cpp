@@ -256,8 +256,10 @@ std::shared_ptr<PathNeeRenderer> PathNeeRenderer::FromSimple2Json(const std::sha
     r->seed = uint32_t(json.number("seed", 0));
     r->use_mis = json.boolean("use_mis", true);
     // "guiding": {"guide_fraction", "spp_per_pass", "train_steps", "warmup_passes",
-    //             "max_guided_bounce", "learning_rate", "batch_size", "record_capacity",
-    //             "seed", "inference": "bf16" | "fp8"} (all optional)
+    //             "max_guided_bounce", "train_spp", "overlap_training", "learning_rate",
+    //             "batch_size", "record_capacity", "avg_decay", "seed",
+    //             "inference": "bf16" | "fp8"} (all optional; train_spp > 0 trains on the
+    //             first train_spp samples per pixel only and renders the rest in one pass)
     if (const wj::Value *gj = json.find("guiding")) {
         if (gj->type != wj::Value::Object) throw std::runtime_error("\"guiding\" must be an object");
         r->guided = gj->boolean("enabled", true);
@@ -266,6 +268,9 @@ std::shared_ptr<PathNeeRenderer> PathNeeRenderer::FromSimple2Json(const std::sha
         r->guide_config.batch_size = uint32_t(gj->number("batch_size", r->guide_config.batch_size));
         r->guide_config.record_capacity = uint32_t(gj->number("record_capacity", r->guide_config.record_capacity));
         r->guide_config.seed = uint32_t(gj->number("seed", 0));
+        r->guide_config.avg_decay = float(gj->number("avg_decay", r->guide_config.avg_decay));
+        r->guide_train_spp = int(gj->number("train_spp", 0));
+        r->guide_params.overlap_training = gj->boolean("overlap_training", false) ? 1u : 0u;
         r->guide_params.guide_fraction = float(gj->number("guide_fraction", 0.5));
         r->guide_params.spp_per_pass = uint32_t(gj->number("spp_per_pass", 4));
         r->guide_params.train_steps = uint32_t(gj->number("train_steps", 4));
@@ -306,7 +311,30 @@ void PathNeeRenderer::render() {
         p.num_path_vertices = num_path_vertices;
         p.seed = seed;
         p.use_mis = use_mis ? 1 : 0;
-        rc = guided ? wr_render_guided(sc, f, g, &p, &guide_params, &stats, &guide_stats) : wr_render_nee(sc, f, &p, &stats);
+        if (!guided) {
+            rc = wr_render_nee(sc, f, &p, &stats);
+        } else if (guide_train_spp <= 0 || guide_train_spp >= num_spp) {
+            rc = wr_render_guided(sc, f, g, &p, &guide_params, &stats, &guide_stats);
+        } else {
+            // train on the first guide_train_spp samples, then render the rest with the trained network
+            p.spp = guide_train_spp;
+            rc = wr_render_guided(sc, f, g, &p, &guide_params, &stats, &guide_stats);
+            if (rc == WR_OK) {
+                wr_guided_params rest = guide_params;
+                rest.spp_per_pass = uint32_t(num_spp - guide_train_spp);
+                rest.train_steps = 0;
+                rest.warmup_passes = 0;
+                wr_render_stats st2{};
+                wr_guide_stats gs2{};
+                p.first_sample = guide_train_spp;
+                p.spp = num_spp - guide_train_spp;
+                rc = wr_render_guided(sc, f, g, &p, &rest, &st2, &gs2);
+                stats.paths += st2.paths;
+                stats.extend_rays += st2.extend_rays;
+                stats.shadow_rays += st2.shadow_rays;
+                stats.wall_ms += st2.wall_ms;
+            }
+        }
     }
     if (rc == WR_OK) rc = wr_film_read(f, film.base.data(), film.splat.data());
     const std::string err = rc == WR_OK ? "" : wr_last_error();

@@ -96,6 +96,7 @@ struct PathNeeRenderer : Renderer {
     // learned guiding: the optional "guiding" object of the render entry (no reference counterpart)
     bool guided = false;
     wr_guided_params guide_params{};
+    int guide_train_spp = 0;  // > 0: train on the first guide_train_spp samples only
     wr_guide_config guide_config{};
     wr_render_stats stats{};
     wr_guide_stats guide_stats{};
