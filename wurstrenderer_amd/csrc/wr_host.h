@@ -67,16 +67,18 @@ struct PathBuffers {
 
 // Each counter sits on its own 256-byte line: atomics on one line serialise.
 constexpr int kCounterStride = 64;
+constexpr int kXcds = 8;  // MI355X: 8 accelerator complex dies, each with its own L2
 enum Counter : int {
     kQ = 0 * kCounterStride,         // continuing paths at the head of the current state
     kFresh = 1 * kCounterStride,     // new samples started behind them this iteration
     kQn = 2 * kCounterStride,        // paths continuing into the next state
     kS = 3 * kCounterStride,         // shadow rays
-    kFetchExt = 4 * kCounterStride,  // work counter of the extend kernel
-    kFetchShd = 5 * kCounterStride,  // work counter of the shadow kernel
+    // lines 4 and 5 are spare
     kClaimed = 6 * kCounterStride,   // samples started so far in the pass
     kBase = 7 * kCounterStride,      // first sample index started this iteration
-    kNumCounterLines = 8
+    kFetchExtX = 8 * kCounterStride,   // extend: work counter of XCD x's share at kFetchExtX + x * kCounterStride
+    kFetchShdX = 16 * kCounterStride,  // shadow: likewise
+    kNumCounterLines = 24
 };
 
 struct RenderArgs {

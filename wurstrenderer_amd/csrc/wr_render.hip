@@ -117,8 +117,10 @@ __global__ void k_rotate(PathBuffers pb, RenderArgs a) {
     c[kQ] = c[kQn];
     c[kQn] = 0;
     c[kS] = 0;
-    c[kFetchExt] = 0;
-    c[kFetchShd] = 0;
+    for (int x = 0; x < kXcds; x++) {
+        c[kFetchExtX + x * kCounterStride] = 0;
+        c[kFetchShdX + x * kCounterStride] = 0;
+    }
     plan_fresh(c, a);
 }
 __global__ void k_fill_u32(uint32_t *p, uint32_t v, size_t n) {
@@ -347,22 +349,37 @@ struct TravTune {
 
 struct WaveRange {
     uint32_t lo = 0, hi = 0;  // wave-uniform
+    uint32_t tried = 0;       // XCD shares found empty so far (wave-uniform)
 };
-// Returns the entry for this lane (>= n when it got none).  A new chunk is
-// claimed only when the local range is empty; `exhausted` (wave-uniform) is
-// set when the queue has no chunk left.  All lanes of the wave must call it.
-__device__ __forceinline__ uint32_t refill(uint32_t *fetch_counter, uint32_t n, bool idle, unsigned long long idle_mask,
+// Returns the entry for this lane (>= n when it got none).  The queue's n
+// entries are split into kXcds contiguous shares, share x = [n x / 8,
+// n (x + 1) / 8), each with its own work counter (`xcd_counters` + x lines):
+// workgroups are dispatched to the XCDs round-robin, so block b runs on XCD
+// b % 8, and its waves claim chunks of that XCD's share first -- queue order
+// follows pixel order, so neighbouring rays, and the nodes they read, stay in
+// one XCD's L2 -- then of the other shares in turn, so no XCD idles while work
+// is left.  A new chunk is claimed only when the local range is empty;
+// `exhausted` (wave-uniform) is set when no share has a chunk left.  All lanes
+// of the wave must call it.
+__device__ __forceinline__ uint32_t refill(uint32_t *xcd_counters, uint32_t n, bool idle, unsigned long long idle_mask,
                                            WaveRange &wr, bool &exhausted, uint32_t chunk) {
     if (wr.lo == wr.hi) {
-        uint32_t base = 0;
-        if (lane_id() == 0) base = atomicAdd(fetch_counter, chunk);
-        base = __builtin_amdgcn_readfirstlane(base);  // every lane is active here, so this reads lane 0
-        if (base >= n) {
-            exhausted = true;
-            return n;
+        for (;;) {
+            const uint32_t x = (blockIdx.x + wr.tried) % kXcds;
+            const uint32_t lo = uint32_t(uint64_t(n) * x / kXcds), hi = uint32_t(uint64_t(n) * (x + 1) / kXcds);
+            uint32_t got = 0;
+            if (lane_id() == 0) got = atomicAdd(&xcd_counters[x * kCounterStride], chunk);
+            got = __builtin_amdgcn_readfirstlane(got);  // every lane is active here, so this reads lane 0
+            if (got < hi - lo) {
+                wr.lo = lo + got;
+                wr.hi = min(lo + got + chunk, hi);
+                break;
+            }
+            if (++wr.tried == kXcds) {
+                exhausted = true;
+                return n;
+            }
         }
-        wr.lo = base;
-        wr.hi = min(base + chunk, n);
     }
     const uint32_t k = (uint32_t)__popcll(idle_mask);
     const uint32_t avail = wr.hi - wr.lo;
@@ -388,7 +405,7 @@ __device__ __forceinline__ void extend_body(DScene s1, const BatchJob *__restric
     for (;;) {
         const unsigned long long idle = __ballot(!active);
         if (!exhausted && __popcll(idle) >= tune.refill) {
-            const uint32_t i = refill(&pb.counters[kFetchExt], n, !active, idle, wr, exhausted, tune.chunk);
+            const uint32_t i = refill(&pb.counters[kFetchExtX], n, !active, idle, wr, exhausted, tune.chunk);
             if (!active && i < n) {
                 p = i;
                 const float4 o = pb.cur.ray_o[p], d = pb.cur.ray_d[p];
@@ -1074,7 +1091,7 @@ __global__ __launch_bounds__(kBlock) WR_TRAV_ATTR void k_shadow(DScene s1, const
     for (;;) {
         const unsigned long long idle = __ballot(!active);
         if (!exhausted && __popcll(idle) >= tune.refill) {
-            const uint32_t j = refill(&pb.counters[kFetchShd], n, !active, idle, wr, exhausted, tune.chunk);
+            const uint32_t j = refill(&pb.counters[kFetchShdX], n, !active, idle, wr, exhausted, tune.chunk);
             if (!active && j < n) {
                 i = j;
                 const float4 o = pb.sh_o[i], d = pb.sh_d[i];
