@@ -328,11 +328,21 @@ __device__ __forceinline__ void wave_sync_lds() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Output row pair of one record: lane (p, h) holds, in r0 and r1, the rows
+// 32 mt + drow(v, h) of records p and 32 + p of output tile mt.
+// v_permlane32_swap exchanges the upper half-wave of r0 with the lower
+// half-wave of r1, after which lane (p, h) -- record 32 h + p -- holds rows
+// 32 mt + drow(v, 0) (lo) and 32 mt + drow(v, 0) + 4 (hi) of its own record.
+__device__ __forceinline__ void rows_to_record(float r0, float r1, float &lo, float &hi) {
+    const auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(r0), __float_as_uint(r1), false, false);
+    lo = __uint_as_float(sw[0]);
+    hi = __uint_as_float(sw[1]);
+}
+
 // Forward pass of the wave's 64 records (record r is the one encoded in lane
-// r).  Raw outputs of record r land in stage[r * kStageStride + o], o < 40.
-// Must be called by all 64 lanes of the wave.
-__device__ __forceinline__ void forward(const Net &net, const Enc &mine, float *stage) {
-    const int lane = __lane_id(), p = lane & 31, h = lane >> 5;
+// r).  Lane r receives the raw outputs of record r in out[0 .. kOut).  Must be
+// called by all 64 lanes of the wave.
+__device__ __forceinline__ void forward(const Net &net, const Enc &mine, float out[kOut]) {
     u32x4 x[4][2];
     {
         u32x4 x0[2][2];
@@ -359,7 +369,6 @@ __device__ __forceinline__ void forward(const Net &net, const Enc &mine, float *
 #pragma unroll
             for (int nt = 0; nt < 2; nt++) x[ks][nt] = xn[ks][nt];
     }
-    float *sb = stage + p * kStageStride + 4 * h;
 #pragma unroll
     for (int mt = 0; mt < 2; mt++) {
         f32x16 acc[2];
@@ -367,13 +376,14 @@ __device__ __forceinline__ void forward(const Net &net, const Enc &mine, float *
 #pragma unroll
         for (int v = 0; v < 16; v++) {
             const int o0 = 32 * mt + drow(v, 0);
-            if (o0 + 4 * h < kOut) {
-#pragma unroll
-                for (int nt = 0; nt < 2; nt++) sb[32 * nt * kStageStride + o0] = acc[nt][v];
+            if (o0 < kOut) {
+                float lo, hi;
+                rows_to_record(acc[0][v], acc[1][v], lo, hi);
+                out[o0] = lo;
+                if (o0 + 4 < kOut) out[o0 + 4] = hi;
             }
         }
     }
-    wave_sync_lds();
 }
 
 // ---------------------------------------------------------------- FP8 forward pass
@@ -460,8 +470,8 @@ __device__ __forceinline__ void hidden_to_operands8(f32x16 acc[2][2], const floa
     }
 }
 
-// forward() on e4m3 weights and activations: same inputs, outputs and staging.
-__device__ __forceinline__ void forward8(const Net &net, const Enc &mine, float *stage) {
+// forward() on e4m3 weights and activations: same inputs and outputs.
+__device__ __forceinline__ void forward8(const Net &net, const Enc &mine, float out[kOut]) {
     const int lane = __lane_id(), p = lane & 31, h = lane >> 5;
     u32x2 x[4][2];
     float xs[2];
@@ -518,19 +528,20 @@ __device__ __forceinline__ void forward8(const Net &net, const Enc &mine, float 
     layer8<kHid>(net.fwd8 + fwd_offset(kLayers - 1), x, acc);
     const float ws = net.wscale[kLayers - 1];
     const float *bb = net.bias + 64 * (kLayers - 1) + 4 * h;
-    float *sb = stage + p * kStageStride + 4 * h;
 #pragma unroll
     for (int mt = 0; mt < 2; mt++)
 #pragma unroll
         for (int v = 0; v < 16; v++) {
             const int o0 = 32 * mt + drow(v, 0);
-            if (o0 + 4 * h < kOut) {
-#pragma unroll
-                for (int nt = 0; nt < 2; nt++)
-                    sb[32 * nt * kStageStride + o0] = fmaf(acc[mt][nt][v], ws * xs[nt], bb[o0]);
+            if (o0 < kOut) {
+                // this lane's row is o0 + 4 h (its bias bb[o0]); rows past kOut are padding
+                const float b = o0 + 4 * h < kOut ? bb[o0] : 0.f;
+                float lo, hi;
+                rows_to_record(fmaf(acc[mt][0][v], ws * xs[0], b), fmaf(acc[mt][1][v], ws * xs[1], b), lo, hi);
+                out[o0] = lo;
+                if (o0 + 4 < kOut) out[o0 + 4] = hi;
             }
         }
-    wave_sync_lds();
 }
 
 // ---------------------------------------------------------------- loss gradient

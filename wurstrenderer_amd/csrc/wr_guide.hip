@@ -448,7 +448,6 @@ __global__ void k_finalize(Records rec, uint32_t n, const float4 *sample_rad, do
 template <bool FP8>
 __global__ __launch_bounds__(kTestBlock) void k_forward_test(Net net, const float *pos, const float *nrm, const float *wo,
                                                              uint32_t n, float *out) {
-    __shared__ float stage[64 * kStageStride];
     const uint32_t r = blockIdx.x * 64 + threadIdx.x;
     Enc e;
     if (r < n)
@@ -456,10 +455,11 @@ __global__ __launch_bounds__(kTestBlock) void k_forward_test(Net net, const floa
                    f3(wo[3 * r], wo[3 * r + 1], wo[3 * r + 2]), net.box);
     else
         for (int k = 0; k < 7; k++) e.v[k] = 0.f;
-    if (FP8) forward8(net, e, stage);
-    else forward(net, e, stage);
+    float o[kOut];
+    if (FP8) forward8(net, e, o);
+    else forward(net, e, o);
     if (r < n)
-        for (int k = 0; k < kOut; k++) out[size_t(r) * kOut + k] = stage[threadIdx.x * kStageStride + k];
+        for (int k = 0; k < kOut; k++) out[size_t(r) * kOut + k] = o[k];
 }
 
 __global__ __launch_bounds__(kTestBlock) void k_mixture_test(const float *raw, const float *dirs, const float *u, uint32_t n,

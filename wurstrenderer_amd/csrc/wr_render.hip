@@ -732,8 +732,13 @@ __device__ __forceinline__ float guide_alpha(const DMat &m, float frac) {
 // same random numbers k_shade would use) and returns q at that direction and
 // at the direction of the vertex's light sample.  The free-flight decision
 // repeats k_shade_ext's.
+// Network outputs reach their record's lane through v_permlane32_swap, so LDS
+// holds only the weights (37 KB BF16, 18 KB e4m3) and the kernel runs 3 waves
+// per SIMD (155-166 VGPRs): 2/3/4 waves took 91.7/82.4/81.4 ms of guided
+// shading per 32 spp at 1080p (4 spills registers), 88.7 with the former
+// 42 KB LDS stage at 2 waves.
 #ifndef WR_INFER_WAVES
-#define WR_INFER_WAVES 2
+#define WR_INFER_WAVES 3
 #endif
 #ifndef WR_INFER_LDS_WEIGHTS
 #define WR_INFER_LDS_WEIGHTS 1
@@ -750,7 +755,6 @@ __global__ __launch_bounds__(kInferBlock) __attribute__((amdgpu_waves_per_eu(WR_
 #endif
     __shared__ float w_bias[guide::kBiasPacked];
     __shared__ float w_scale[guide::kLayers];
-    __shared__ float stage_all[kInferBlock * guide::kStageStride];
     const uint32_t n = active_count(counters);
     if (blockIdx.x * kInferBlock >= n) return;
 #if WR_INFER_LDS_WEIGHTS
@@ -774,8 +778,6 @@ __global__ __launch_bounds__(kInferBlock) __attribute__((amdgpu_waves_per_eu(WR_
 #endif
     }
     net.bias = w_bias;
-    float *stage = stage_all + (threadIdx.x & ~63u) * guide::kStageStride;
-    const uint32_t lane = threadIdx.x & 63u;
     for (uint32_t base = blockIdx.x * kInferBlock; base < n; base += gridDim.x * kInferBlock) {
         const uint32_t i = base + threadIdx.x;
         bool need = false, med = false;
@@ -810,15 +812,10 @@ __global__ __launch_bounds__(kInferBlock) __attribute__((amdgpu_waves_per_eu(WR_
             }
         }
         if (__ballot(need)) {
-            if constexpr (FP8) guide::forward8(net, guide::encode(pos, nf, -rd, net.box), stage);
-            else guide::forward(net, guide::encode(pos, nf, -rd, net.box), stage);
+            float L[guide::kOut];  // this lane's record's raw outputs, then its decoded lobes
+            if constexpr (FP8) guide::forward8(net, guide::encode(pos, nf, -rd, net.box), L);
+            else guide::forward(net, guide::encode(pos, nf, -rd, net.box), L);
             if (need) {
-                // the record's raw outputs into registers: decode, the draw and the
-                // four pdf evaluations read them there instead of from LDS
-                float L[guide::kOut];
-                const float *row = stage + lane * guide::kStageStride;
-#pragma unroll
-                for (int o = 0; o < guide::kOut; o++) L[o] = row[o];
                 float wl[guide::kLobes];
                 guide::decode(L, wl);
                 const uint32_t key = meta.x;
