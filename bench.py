@@ -325,8 +325,8 @@ def config4(wr, scenes, dev, a):
                 "target_relmse": a.ttr_target, "guided_seconds_median": xg, "guided_seconds_runs": g_x,
                 "plain_seconds_median": xp, "plain_seconds_runs": p_x,
                 "speedup_guided_vs_plain": None if xg is None or xp is None else round(xp / xg, 3),
-                "guided_curve_run0": [[c[0], c[1], float(f"{c[2]:.4e}")] for c in g_curves[0]],
-                "plain_curve_run0": [[c[0], c[1], float(f"{c[2]:.4e}")] for c in p_curves[0]]}}
+                "guided_curve_run0": [[c[0], c[1], float(f"{c[2]:.4e}")] for c in (g_curves[0] if g_curves else [])],
+                "plain_curve_run0": [[c[0], c[1], float(f"{c[2]:.4e}")] for c in (p_curves[0] if p_curves else [])]}}
 
 
 def medium_fp8(wr, scenes, dev, a):

@@ -333,6 +333,31 @@ def test_guided_render_trains_and_stays_unbiased(gpu_device):
     assert np.all(np.abs(img.mean((0, 1)) - ref.mean((0, 1))) < 5 * sig + 1e-6)
 
 
+@pytest.mark.parametrize("overlap,avg", [(True, None), (False, 0.0)], ids=["overlap", "no-average"])
+def test_overlapped_training_and_parameter_average_stay_unbiased(gpu_device, overlap, avg):
+    """overlap_training renders pass q with the network trained on pass q - 2
+    while pass q - 1 trains on its own stream; the parameter average (avg_decay,
+    0 = the parameters themselves) decides which network the renderer samples.
+    Neither changes the estimator's mean, and both train the network."""
+    sc = scenes.guide_room(4, n_spheres=6, level=3, resolution=(64, 48))
+    gs = wr.GpuScene(gpu_device, sc)
+    print(f"guided render, overlap {overlap}, avg_decay {avg}: guide room 64x48", flush=True)
+    g = wr.Guide(gs, batch_size=1 << 14, record_capacity=1 << 16, avg_decay=avg)
+    before = g.get_params()
+    img, st, gst = _render(gs, 256, 18, seed=35, guide=g, guide_fraction=0.4, spp_per_pass=16, train_steps=16,
+                           warmup_passes=2 if overlap else 1, overlap_training=overlap, max_guided_bounce=5)
+    print(f"  records {gst.records} steps {gst.train_steps} loss {gst.last_loss:.4f} train_ms {gst.train_ms:.1f}")
+    assert gst.train_steps == 16 * 16 and np.isfinite(gst.last_loss) and np.abs(g.get_params() - before).max() > 0
+    assert np.isfinite(img).all() and img.min() >= 0.0
+    ref, _, _ = _render(gs, 2048, 18, seed=36)
+    plain, _, _ = _render(gs, 256, 18, seed=37)
+    plain2, _, _ = _render(gs, 256, 18, seed=38)
+    npx = img.shape[0] * img.shape[1]
+    sig = np.sqrt(2 * ((plain - plain2) ** 2).mean((0, 1)) / 2 / npx)
+    print(f"  mean ref {ref.mean((0, 1))} guided {img.mean((0, 1))} sigma {sig}")
+    assert np.all(np.abs(img.mean((0, 1)) - ref.mean((0, 1))) < 5 * sig + 1e-6)
+
+
 def test_trained_guiding_beats_plain_at_equal_spp(gpu_device):
     sc = scenes.guide_room(4, n_spheres=12, level=3, resolution=(96, 64))
     gs = wr.GpuScene(gpu_device, sc)
