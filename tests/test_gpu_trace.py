@@ -205,6 +205,30 @@ def _hip():
     return hip
 
 
+def test_closest_hit_matches_closed_form_quads(gpu_device):
+    """The FP32 watertight test against closed-form ray/quad hits (pins it
+    directly, not only through the oracle): hit or miss and the quad and
+    triangle exact away from edges, t within the bar above."""
+    from closed_form import quad_scene_and_rays
+    sc, rays, geom, prim, t, amb = quad_scene_and_rays()
+    print(f"closed-form quads: rays={len(rays)}", flush=True)
+    gs = wr.GpuScene(gpu_device, sc)
+    g, p, tt = gs.trace_closest(rays)
+    ok = ~amb
+    np.testing.assert_array_equal(g[ok], geom[ok])
+    hit = ok & (geom != INV)
+    np.testing.assert_array_equal(p[hit], prim[hit])
+    C = float(np.abs(sc.verts).max())
+    assert (np.abs(tt[hit] - t[hit]) <= 1e-5 * t[hit] + 4e-7 * C).all()
+    assert np.isinf(tt[ok & (geom == INV)]).all()
+    # any hit: occluded exactly when the closed-form closest hit lies before tmax
+    occ_rays = rays.copy()
+    occ_rays[:, 7] = np.random.default_rng(8).uniform(0.5, 10.0, len(rays))
+    occ = gs.trace_any(occ_rays)
+    sure = ok & (np.abs(t - occ_rays[:, 7]) > 1e-5 * occ_rays[:, 7])
+    np.testing.assert_array_equal(occ[sure].astype(bool), (t < occ_rays[:, 7])[sure])
+
+
 def test_device_resident_entry_point_matches_host_one(gpu_device):
     """wr_trace_*_device on device buffers (allocated with the library's own HIP
     runtime) gives the same hits as the host-buffer entry points."""

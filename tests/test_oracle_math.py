@@ -1,7 +1,8 @@
 """Pins the oracle by what the mathematics fixes: RNG statistics, BSDF sampling
 invariants and pdf normalisation, direct lighting of a plane by a polygonal
 luminaire (Lambert's formula), radiance of a directly visible emitter, MIS
-on/off agreement (pins the MIS weights), and BVH traversal against brute force."""
+on/off agreement (pins the MIS weights), BVH traversal against brute force,
+and the triangle test against closed-form ray/quad intersections."""
 import math
 
 import numpy as np
@@ -235,3 +236,22 @@ def test_closed_icosphere_is_watertight():
     rays[:, 7] = np.inf
     g, p, t = oracle.OracleScene(sc).trace_closest(rays)
     assert (g == 0).all()
+
+def test_triangle_test_matches_closed_form_quads():
+    from closed_form import quad_scene_and_rays
+    """The oracle's FP64 watertight test against closed-form ray/quad hits:
+    hit or miss, quad and triangle exact away from edges, t to FP32 rounding."""
+    sc, rays, geom, prim, t, amb = quad_scene_and_rays()
+    g, p, tt = oracle.OracleScene(sc).trace_closest(rays)
+    ok = ~amb
+    assert amb.mean() < 0.01 and 0.3 < (geom[ok] != 0xFFFFFFFF).mean() < 0.9
+    np.testing.assert_array_equal(g[ok], geom[ok])
+    hit = ok & (geom != 0xFFFFFFFF)
+    np.testing.assert_array_equal(p[hit], prim[hit])
+    assert (np.abs(tt[hit] - t[hit]) <= 2.4e-7 * t[hit]).all()
+    occ_rays = rays.copy()
+    occ_rays[:, 7] = np.random.default_rng(8).uniform(0.5, 10.0, len(rays))
+    occ = oracle.OracleScene(sc).trace_any(occ_rays)
+    sure = ok & (np.abs(t - occ_rays[:, 7]) > 1e-5 * occ_rays[:, 7])
+    np.testing.assert_array_equal(occ[sure].astype(bool), (t < occ_rays[:, 7])[sure])
+
