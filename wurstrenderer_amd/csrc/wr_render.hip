@@ -317,15 +317,18 @@ __global__ __launch_bounds__(kShadeBlock) void k_regen(DScene s1, const BatchJob
 // and refills its lanes from that local range as soon as `refill` of them are
 // idle, so a wave never waits for its slowest ray and atomics stay rare.
 // Waves beyond the work exit at once.  Both are runtime parameters
-// (TravTune).  On the 1.31M-triangle frame with the default pool, chunks of
-// 128 / 256 / 512 with refill at 24 idle lanes gave 5,382 / 5,442 / 5,042
-// Mrays/s (64 is far slower: one counter serialises the atomics), refill 8..32
-// within 2%; wr_autotune searches them per workload.
+// (TravTune).  With the queue split into per-XCD shares (refill below), a
+// smaller chunk keeps a wave's rays closer together in the queue: on the
+// 1.31M-triangle frame chunk x refill 64x32 / 96x32 / 128x32 / 256x24 /
+// 512x24 gave 5,902 / 5,899 / 5,863 / 5,672 / 5,338 Mrays/s, in the gap room
+// 5,981 / 6,123 / 6,099 / 6,008 / 5,974 (its shadow rays prefer larger
+// chunks, the icosphere frame's smaller); refill 8 and 16 are 2-6% slower.
+// wr_autotune searches them per workload.
 #ifndef WR_REFILL_LANES
-#define WR_REFILL_LANES 24
+#define WR_REFILL_LANES 32
 #endif
 #ifndef WR_FETCH_CHUNK
-#define WR_FETCH_CHUNK 256
+#define WR_FETCH_CHUNK 96
 #endif
 struct TravTune {
     uint32_t chunk = WR_FETCH_CHUNK;
@@ -2853,7 +2856,7 @@ int wr_autotune(const wr_batch_job *jobs, uint32_t n_jobs, const wr_render_param
     struct Candidate { uint32_t pool, chunk, refill; int graph; };
     std::vector<Candidate> cands;
     for (uint32_t pool : {1u << 23, 1u << 24, 1u << 25, 1u << 26, 1u << 27})
-        for (auto ck : {std::make_pair(128u, 24u), std::make_pair(256u, 24u), std::make_pair(256u, 32u), std::make_pair(128u, 40u)})
+        for (auto ck : {std::make_pair(64u, 32u), std::make_pair(96u, 32u), std::make_pair(128u, 40u), std::make_pair(256u, 32u)})
             for (int graph : {0, 1}) cands.push_back({pool, ck.first, ck.second, graph});
     double best_rate = -1.0;
     wr_render_params bp = *base;
