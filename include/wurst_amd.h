@@ -110,7 +110,10 @@ typedef struct {
     uint64_t extend_rays;        /* closest-hit rays traced */
     uint64_t shadow_rays;        /* any-hit rays traced (NEE + light->camera) */
     double wall_ms;              /* host wall time of the call */
-    /* with profile = 1: summed HIP-event time and launch count per kernel class */
+    /* with profile = 1: summed HIP-event time and launch count per kernel class.
+     * extend_*: the traversal launches, one per pool iteration, each tracing the
+     * iteration's closest-hit rays and the previous iteration's shadow rays;
+     * shadow_ms and shadow_launches stay 0 (no launch traces shadow rays alone). */
     double extend_ms, shadow_ms, shade_ms, generate_ms, accumulate_ms;
     uint64_t extend_launches, shadow_launches;
     /* with count_traversal = 1 */

@@ -124,9 +124,10 @@ def test_progressive_calls_and_batching_are_exact(gpu_device):
 def test_profile_and_counters(gpu_device):
     sc = scenes.cornell(1, resolution=(64, 64))
     _, st, _ = _gpu_image(gpu_device, sc, 4, num_path_vertices=10, seed=1, profile=True, count_traversal=True)
-    assert st.extend_ms > 0 and st.shadow_ms > 0 and st.shade_ms > 0
-    # one extend and one shadow launch per pool iteration (at least N-1 of them)
-    assert st.extend_launches >= 9 and st.shadow_launches == st.extend_launches
+    assert st.extend_ms > 0 and st.shade_ms > 0
+    # one traversal launch per pool iteration (at least N-1 of them); it traces
+    # the shadow rays of the previous iteration too, so none is timed apart
+    assert st.extend_launches >= 9 and st.shadow_launches == 0 and st.shadow_ms == 0
     assert st.extend_nodes >= st.extend_rays and st.extend_tris > 0 and st.shadow_nodes > 0
 
 

@@ -39,7 +39,8 @@ constexpr int kBlock = 64;  // one wavefront per block; LDS stack indexed by lan
 
 // Shadow-ray targets: where a visible connection's contribution goes.
 constexpr uint32_t kTargetMask = 0x3FFFFFFFu;
-constexpr uint32_t kToNextPath = 0u << 30;   // radiance of a continuing path (next-state index)
+constexpr uint32_t kToNextPath = 0u << 30;   // radiance of a continuing path (its index in the state that
+                                             // the shade kernel wrote, current when the ray is traced)
 constexpr uint32_t kToSample = 1u << 30;     // per-sample radiance of a finished path
 constexpr uint32_t kToSplat = 2u << 30;      // light->camera splat (pixel of the FP64 splat film)
 
@@ -53,6 +54,9 @@ struct PathState {
     float4 *rad;            // rgb: radiance gathered so far (FP32)
     uint4 *meta;            // x: RNG key, y: sample index in the pass, z: bounce of the next vertex,
                             // w: 1 + training record of the previous vertex (0: none)
+    uint32_t shadow_count;  // counter (kShadowA or kShadowB) of the shadow queue that the shade
+                            // kernel appends to while this state is current; the two states
+                            // hold the two counters and swap them with everything else
 };
 
 struct PathBuffers {
@@ -72,8 +76,15 @@ enum Counter : int {
     kQ = 0 * kCounterStride,         // continuing paths at the head of the current state
     kFresh = 1 * kCounterStride,     // new samples started behind them this iteration
     kQn = 2 * kCounterStride,        // paths continuing into the next state
-    kS = 3 * kCounterStride,         // shadow rays
-    // lines 4 and 5 are spare
+    // line 3 is spare
+    // Shadow queue: the shade kernel of iteration i appends its shadow rays,
+    // then the regeneration of iteration i + 1 its t = 1 connections, and the
+    // traversal launch of iteration i + 1 traces them beside that iteration's
+    // closest-hit rays.  Consecutive iterations alternate between the two
+    // counters (PathState::shadow_count), so a queue's count survives the end
+    // of its iteration while the next shade kernel appends from zero.
+    kShadowA = 4 * kCounterStride,
+    kShadowB = 5 * kCounterStride,
     kClaimed = 6 * kCounterStride,   // samples started so far in the pass
     kBase = 7 * kCounterStride,      // first sample index started this iteration
     kFetchExtX = 8 * kCounterStride,   // extend: work counter of XCD x's share at kFetchExtX + x * kCounterStride
@@ -98,7 +109,7 @@ struct wr_device {
     int ordinal = 0;
     int n_cu = 256;
     // resident blocks per CU of each kernel (HIP occupancy API), used to size persistent grids
-    int occ_extend = 8, occ_shadow = 8, occ_shade = 8, occ_generate = 8, occ_trace = 8;
+    int occ_rays = 8, occ_shade = 8, occ_generate = 8, occ_trace = 8;
     std::map<const void *, int> occ_cache;  // other kernels, filled on first use
     void *scratch = nullptr;  // device address of wr_render_nee_noalloc's fixed region on this GPU
 };

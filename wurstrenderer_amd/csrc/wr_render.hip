@@ -109,14 +109,17 @@ __global__ void k_init(PathBuffers pb, RenderArgs a) {
     if (threadIdx.x == 0) plan_fresh(pb.counters, a);
 }
 // End of an iteration: the next state becomes current, work counters restart,
-// and the next refill is planned.
+// the shadow queue traced in this iteration is emptied (the one the shade
+// kernel filled stays for the next traversal launch), and the next refill is
+// planned.
 __global__ void k_rotate(PathBuffers pb, RenderArgs a) {
     uint32_t *c = pb.counters;
+    const uint32_t traced = pb.next.shadow_count;
     pb.stats[0] += active_count(c);
-    pb.stats[1] += c[kS];
+    pb.stats[1] += c[traced];
     c[kQ] = c[kQn];
     c[kQn] = 0;
-    c[kS] = 0;
+    c[traced] = 0;
     for (int x = 0; x < kXcds; x++) {
         c[kFetchExtX + x * kCounterStride] = 0;
         c[kFetchShdX + x * kCounterStride] = 0;
@@ -303,7 +306,8 @@ __global__ __launch_bounds__(kShadeBlock) void k_regen(DScene s1, const BatchJob
             st.rad[e] = make_float4(0.f, 0.f, 0.f, 0.f);
             st.meta[e] = meta;
         }
-        const uint32_t si = block_append(&pb.counters[kS], splat, agg);
+        // t = 1 connections join the previous shade kernel's shadow rays
+        const uint32_t si = block_append(&pb.counters[pb.next.shadow_count], splat, agg);
         if (splat) {
             pb.sh_o[si] = so;
             pb.sh_d[si] = sd;
@@ -335,18 +339,13 @@ struct TravTune {
     uint32_t refill = WR_REFILL_LANES;
 };
 
-#ifdef WR_TRAV_WAVES
-#define WR_TRAV_ATTR __attribute__((amdgpu_waves_per_eu(WR_TRAV_WAVES, WR_TRAV_WAVES)))
-#else
-#define WR_TRAV_ATTR
-#endif
-// The plain closest-hit kernel (k_extend_fast) fits 64 VGPRs without spills,
-// which admits 8 waves per SIMD (7 and 8 measured within 0.2% on the
-// 1.31M-triangle frame); the counting and batched kernels keep the compiler's
-// choice.  The plain any-hit kernel (k_shadow) gets 60 VGPRs and 8 waves by
-// itself: its stack holds node refs only.
+// The plain traversal kernel (k_trace_bounce_fast) is capped at 7 waves per
+// SIMD (72 VGPRs): at 8 (64 VGPRs) two stack-address registers spill and one
+// is reloaded on every pop; for closest-hit traversal alone 7 and 8 waves
+// measured within 0.2% on the 1.31M-triangle frame.  The counting and batched
+// kernels keep the compiler's choice.
 #ifndef WR_EXTEND_WAVES
-#define WR_EXTEND_WAVES 8
+#define WR_EXTEND_WAVES 7
 #endif
 #define WR_EXTEND_ATTR __attribute__((amdgpu_waves_per_eu(WR_EXTEND_WAVES, WR_EXTEND_WAVES)))
 
@@ -392,12 +391,23 @@ __device__ __forceinline__ uint32_t refill(uint32_t *xcd_counters, uint32_t n, b
     return e;
 }
 
+// Chunk size for a queue of n entries.  Blocks past n / kBlock exit at once,
+// so a short queue (the last bounces of a frame) is worked by ceil(n / 64)
+// waves: a chunk of 64 then gives every lane one ray to start at once,
+// where a larger chunk leaves fewer waves with work and queues the rest of
+// each chunk behind their lanes' first rays.  Long queues keep `chunk`.
+__device__ __forceinline__ uint32_t spread_chunk(uint32_t n, uint32_t chunk) {
+    const uint32_t per_wave = n / gridDim.x + (n % gridDim.x != 0);
+    // wave-uniform; kept in a scalar register (the division runs on the VALU)
+    return __builtin_amdgcn_readfirstlane(min(chunk, max(uint32_t(kBlock), per_wave)));
+}
+
+// Closest hit for the n active paths of the current state.  `st_ent` is the
+// block's LDS stack (kLdsStackDepth x kBlock entries).
 template <bool COUNT, bool BATCH>
-__device__ __forceinline__ void extend_body(DScene s1, const BatchJob *__restrict__ jobs, PathBuffers pb, uint2 *spill,
-                                            TravTune tune) {
-    const uint32_t n = active_count(pb.counters);
-    if (blockIdx.x * kBlock >= n) return;
-    __shared__ uint64_t st_ent[kLdsStackDepth * kBlock];
+__device__ __forceinline__ void extend_rays(DScene s1, const BatchJob *__restrict__ jobs, PathBuffers pb, uint2 *spill,
+                                            TravTune tune, uint32_t n, uint64_t *st_ent) {
+    const uint32_t chunk = spread_chunk(n, tune.chunk);
     LaneStack<kBlock> st(st_ent + lane_id(), reinterpret_cast<uint32_t *>(st_ent) + lane_id(), spill);
     TraversalCounters cnt;
     TravState ts;
@@ -408,7 +418,7 @@ __device__ __forceinline__ void extend_body(DScene s1, const BatchJob *__restric
     for (;;) {
         const unsigned long long idle = __ballot(!active);
         if (!exhausted && __popcll(idle) >= tune.refill) {
-            const uint32_t i = refill(&pb.counters[kFetchExtX], n, !active, idle, wr, exhausted, tune.chunk);
+            const uint32_t i = refill(&pb.counters[kFetchExtX], n, !active, idle, wr, exhausted, chunk);
             if (!active && i < n) {
                 p = i;
                 const float4 o = pb.cur.ray_o[p], d = pb.cur.ray_d[p];
@@ -436,17 +446,6 @@ __device__ __forceinline__ void extend_body(DScene s1, const BatchJob *__restric
         wave_add_stat(&pb.stats[2], cnt.nodes);
         wave_add_stat(&pb.stats[3], cnt.tris);
     }
-}
-
-template <bool COUNT, bool BATCH>
-__global__ __launch_bounds__(kBlock) void k_extend(DScene s1, const BatchJob *__restrict__ jobs, PathBuffers pb,
-                                                   uint2 *spill, TravTune tune) {
-    extend_body<COUNT, BATCH>(s1, jobs, pb, spill, tune);
-}
-// The plain closest-hit kernel of wr_render_nee / wr_render_guided.
-__global__ __launch_bounds__(kBlock) WR_EXTEND_ATTR void k_extend_fast(DScene s1, PathBuffers pb, uint2 *spill,
-                                                                       TravTune tune) {
-    extend_body<false, false>(s1, nullptr, pb, spill, tune);
 }
 
 // ============================================================== shade
@@ -597,7 +596,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(DScene s1, const BatchJob
         const uint32_t target = cont ? (kToNextPath | qi) : (kToSample | meta.y);
         sd.w = __uint_as_float(target);
         sc.w = __uint_as_float(job);
-        const uint32_t si = block_append(&pb.counters[kS], push_shadow, agg);
+        const uint32_t si = block_append(&pb.counters[pb.cur.shadow_count], push_shadow, agg);
         if (push_shadow) {
             pb.sh_o[si] = so;
             pb.sh_d[si] = sd;
@@ -695,7 +694,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade_light(DScene s, PathBuffe
         } else if (valid) {
             pb.sample_rad[meta.y] = rad;
         }
-        const uint32_t si = block_append(&pb.counters[kS], push_shadow, agg);
+        const uint32_t si = block_append(&pb.counters[pb.cur.shadow_count], push_shadow, agg);
         if (push_shadow) {
             pb.sh_o[si] = so;
             pb.sh_d[si] = sd;
@@ -1062,7 +1061,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade_ext(DScene s, PathBuffers
         }
         const uint32_t target = cont ? (kToNextPath | qi) : (kToSample | meta.y);
         sd.w = __uint_as_float(target);
-        const uint32_t si = block_append(&pb.counters[kS], push_shadow, agg);
+        const uint32_t si = block_append(&pb.counters[pb.cur.shadow_count], push_shadow, agg);
         if (push_shadow) {
             pb.sh_o[si] = so;
             pb.sh_d[si] = sd;
@@ -1072,15 +1071,16 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade_ext(DScene s, PathBuffers
 }
 
 // ============================================================== shadow
-// Any-hit traversal of the shadow queue; a visible connection adds its
-// contribution to the radiance of its path (continuing or finished), or to the
-// FP64 splat film for light->camera (t = 1) connections.
+// Any hit for the n entries of the shadow queue the previous shade kernel (and
+// this iteration's regeneration) filled; a visible connection adds its
+// contribution to the radiance of its path (continuing: by then in the
+// current state; finished: its sample), or to the FP64 splat film for
+// light->camera (t = 1) connections.  `st_node` is the block's LDS stack of
+// node refs (any hit keeps no entry distances).
 template <bool COUNT, bool BATCH>
-__global__ __launch_bounds__(kBlock) WR_TRAV_ATTR void k_shadow(DScene s1, const BatchJob *__restrict__ jobs,
-                                                                PathBuffers pb, uint2 *spill, TravTune tune) {
-    const uint32_t n = pb.counters[kS];
-    if (blockIdx.x * kBlock >= n) return;
-    __shared__ uint32_t st_node[kLdsStackDepth * kBlock];  // any hit: node refs only
+__device__ __forceinline__ void shadow_rays(DScene s1, const BatchJob *__restrict__ jobs, PathBuffers pb, uint2 *spill,
+                                            TravTune tune, uint32_t n, uint32_t *st_node) {
+    const uint32_t chunk = spread_chunk(n, tune.chunk);
     LaneStack<kBlock> st(nullptr, st_node + lane_id(), spill);
     TraversalCounters cnt;
     TravState ts;
@@ -1091,7 +1091,7 @@ __global__ __launch_bounds__(kBlock) WR_TRAV_ATTR void k_shadow(DScene s1, const
     for (;;) {
         const unsigned long long idle = __ballot(!active);
         if (!exhausted && __popcll(idle) >= tune.refill) {
-            const uint32_t j = refill(&pb.counters[kFetchShdX], n, !active, idle, wr, exhausted, tune.chunk);
+            const uint32_t j = refill(&pb.counters[kFetchShdX], n, !active, idle, wr, exhausted, chunk);
             if (!active && j < n) {
                 i = j;
                 const float4 o = pb.sh_o[i], d = pb.sh_d[i];
@@ -1122,7 +1122,7 @@ __global__ __launch_bounds__(kBlock) WR_TRAV_ATTR void k_shadow(DScene s1, const
                     atomicAdd(dst + 1, double(c.y));
                     atomicAdd(dst + 2, double(c.z));
                 } else {
-                    float4 *r = (target & ~kTargetMask) == kToSample ? pb.sample_rad + idx : pb.next.rad + idx;
+                    float4 *r = (target & ~kTargetMask) == kToSample ? pb.sample_rad + idx : pb.cur.rad + idx;
                     float4 rr = *r;
                     rr.x += c.x; rr.y += c.y; rr.z += c.z;
                     *r = rr;
@@ -1134,6 +1134,41 @@ __global__ __launch_bounds__(kBlock) WR_TRAV_ATTR void k_shadow(DScene s1, const
         wave_add_stat(&pb.stats[4], cnt.nodes);
         wave_add_stat(&pb.stats[5], cnt.tris);
     }
+}
+
+// ============================================================== traversal launch
+// One launch per iteration traces the closest-hit rays of the current state
+// and the shadow queue of the previous shade kernel: the two are independent
+// (both read only what that shade kernel and this iteration's regeneration
+// wrote), and in one launch each queue's last, slowest rays overlap the other
+// queue's work instead of ending a launch of their own.  A block with
+// closest-hit work traces it first; the shadow queue is dealt to the blocks
+// from the first one without closest-hit work on (block index rotated by the
+// number of blocks the closest-hit rays occupy), so short queues run side by
+// side and long ones back to back.  Both share the block's LDS stack.
+template <bool COUNT, bool BATCH>
+__device__ __forceinline__ void trace_bounce(DScene s1, const BatchJob *__restrict__ jobs, PathBuffers pb, uint2 *spill,
+                                             TravTune tune) {
+    __shared__ uint64_t st_ent[kLdsStackDepth * kBlock];
+    const uint32_t n_ext = active_count(pb.counters);
+    const uint32_t n_shd = pb.counters[pb.next.shadow_count];
+    const uint32_t g = gridDim.x;
+    const uint32_t ext_blocks = min(g, n_ext / kBlock + (n_ext % kBlock != 0));
+    if (blockIdx.x < ext_blocks) extend_rays<COUNT, BATCH>(s1, jobs, pb, spill, tune, n_ext, st_ent);
+    const uint32_t sb = (blockIdx.x + g - ext_blocks) % g;
+    if (sb * kBlock < n_shd)
+        shadow_rays<COUNT, BATCH>(s1, jobs, pb, spill, tune, n_shd, reinterpret_cast<uint32_t *>(st_ent));
+}
+template <bool COUNT, bool BATCH>
+__global__ __launch_bounds__(kBlock) void k_trace_bounce(DScene s1, const BatchJob *__restrict__ jobs, PathBuffers pb,
+                                                         uint2 *spill, TravTune tune) {
+    trace_bounce<COUNT, BATCH>(s1, jobs, pb, spill, tune);
+}
+// The plain kernel of wr_render_nee / wr_render_guided and the other
+// integrators' loops.
+__global__ __launch_bounds__(kBlock) WR_EXTEND_ATTR void k_trace_bounce_fast(DScene s1, PathBuffers pb, uint2 *spill,
+                                                                             TravTune tune) {
+    trace_bounce<false, false>(s1, nullptr, pb, spill, tune);
 }
 
 // ============================================================== accumulate
@@ -1538,8 +1573,7 @@ int wr_device_create(int ordinal, wr_device **out) {
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, block, 0) == hipSuccess && nb > 0) out_blocks = nb;
         out_blocks = std::min(out_blocks, kMaxTraversalBlocksPerCU);
     };
-    occ(reinterpret_cast<const void *>(&k_extend_fast), kBlock, d->occ_extend);
-    occ(reinterpret_cast<const void *>(&k_shadow<false, false>), kBlock, d->occ_shadow);
+    occ(reinterpret_cast<const void *>(&k_trace_bounce_fast), kBlock, d->occ_rays);
     occ(reinterpret_cast<const void *>(&k_shade<false>), kShadeBlock, d->occ_shade);
 
     occ(reinterpret_cast<const void *>(&k_regen<false>), kShadeBlock, d->occ_generate);
@@ -2119,6 +2153,8 @@ static size_t pool_layout(char *base, uint32_t cap, uint32_t samples, PathBuffer
     pb->sh_d = reinterpret_cast<float4 *>(take(32 * c));
     pb->sh_c = reinterpret_cast<float4 *>(take(32 * c));
     pb->counters = reinterpret_cast<uint32_t *>(take(4 * kCounterStride * kNumCounterLines));
+    pb->cur.shadow_count = kShadowA;
+    pb->next.shadow_count = kShadowB;
     pb->stats = reinterpret_cast<unsigned long long *>(take(64));
     return off;
 }
@@ -2175,8 +2211,10 @@ static uint64_t default_pool_fit(const wr_scene *s, uint64_t want, uint64_t samp
     return std::max(floor, std::min(want, budget));
 }
 
-// Runs wavefront iterations until the pool drains.  launch(pb) issues one
-// iteration (refill, extend, shade, shadow, rotate) for the state pb; the host
+// Runs wavefront iterations until the pool drains and both shadow queues are
+// traced.  launch(pb) issues one iteration (refill, one traversal launch for
+// the closest-hit rays and the previous shadow queue, shade, rotate) for the
+// state pb; the host
 // polls the device counters every kPoll iterations.  With a graph, the two
 // iterations of a state round trip (cur -> next -> cur) are captured once and
 // replayed, so a pass costs one graph launch per two iterations.  Every
@@ -2190,7 +2228,9 @@ static int drain(wr_scene *s, PathBuffers &pb, uint64_t max_iters, bool use_grap
         WR_HIP(hipMemcpyAsync(host_counters, pb.counters, sizeof(uint32_t) * kCounterStride * kNumCounterLines,
                               hipMemcpyDeviceToHost, s->stream));
         WR_HIP(hipStreamSynchronize(s->stream));
-        done = host_counters[kQ] == 0 && host_counters[kFresh] == 0;
+        // done: no path left and both shadow queues traced
+        done = host_counters[kQ] == 0 && host_counters[kFresh] == 0 && host_counters[kShadowA] == 0 &&
+               host_counters[kShadowB] == 0;
         return WR_OK;
     };
     if (!use_graph) {
@@ -2383,12 +2423,9 @@ static int render_impl(wr_scene *s, wr_film *f, const wr_render_params *prm, wr_
     const int g_infer = std::max(1, std::min<int>((P + kInferBlock - 1) / kInferBlock,
                                                   s->dev->n_cu * occupancy(s->dev, infer_fn, kInferBlock)));
     guide::Net net{};
-    int g_ext = trav_grid(s, P, s->dev->occ_extend);
-    int g_shd = trav_grid(s, 2 * P, s->dev->occ_shadow);
-    if (fixed) {  // the fixed spill region covers trav_blocks blocks
-        g_ext = std::min(g_ext, fixed->trav_blocks);
-        g_shd = std::min(g_shd, fixed->trav_blocks);
-    }
+    // up to P closest-hit rays and 2P shadow rays (one per path and one per new sample)
+    int g_trav = trav_grid(s, 2 * P, s->dev->occ_rays);
+    if (fixed) g_trav = std::min(g_trav, fixed->trav_blocks);  // the fixed spill region covers trav_blocks blocks
     for (uint32_t done = 0; done < uint32_t(prm->spp); done += spp_pass) {
         const uint32_t k = std::min<uint32_t>(spp_pass, uint32_t(prm->spp) - done);
         RenderArgs a;
@@ -2417,9 +2454,11 @@ static int render_impl(wr_scene *s, wr_film *f, const wr_render_params *prm, wr_
         k_init<<<1, 64, 0, s->stream>>>(pb, a);
         WR_HIP(hipGetLastError());
         // Each iteration refills the pool, extends every active path by one
-        // vertex and resolves its shadow rays; iterations past the end find
-        // empty queues.  Every path ends within N-1 vertices, and each
-        // iteration that is not draining starts new samples in all free entries.
+        // vertex and resolves the shadow rays of the previous iteration's
+        // vertices (so the last shadow queue takes one iteration more);
+        // iterations past the end find empty queues.  Every path ends within
+        // N-1 vertices, and each iteration that is not draining starts new
+        // samples in all free entries.
         const uint64_t max_iters = (uint64_t(a.total) / P + 2) * uint64_t(N + 1) + 64;
         auto iteration = [&](const PathBuffers &q) {
             begin(3);
@@ -2427,8 +2466,8 @@ static int render_impl(wr_scene *s, wr_film *f, const wr_render_params *prm, wr_
             else k_regen<false><<<g_regen, kShadeBlock, 0, s->stream>>>(ds, nullptr, 0, q, a);
             end();
             begin(0);
-            if (count) k_extend<true, false><<<g_ext, kBlock, 0, s->stream>>>(ds, nullptr, q, spill, tune);
-            else k_extend_fast<<<g_ext, kBlock, 0, s->stream>>>(ds, q, spill, tune);
+            if (count) k_trace_bounce<true, false><<<g_trav, kBlock, 0, s->stream>>>(ds, nullptr, q, spill, tune);
+            else k_trace_bounce_fast<<<g_trav, kBlock, 0, s->stream>>>(ds, q, spill, tune);
             end();
             begin(2);
             if (g && gs.alpha > 0.f) {
@@ -2451,10 +2490,6 @@ static int render_impl(wr_scene *s, wr_film *f, const wr_render_params *prm, wr_
             else if (est == kEstLight) k_shade_light<<<g_shade, kShadeBlock, 0, s->stream>>>(ds, q, a);
             else if (env) k_shade<false, true><<<g_shade, kShadeBlock, 0, s->stream>>>(ds, nullptr, q, a);
             else k_shade<false><<<g_shade, kShadeBlock, 0, s->stream>>>(ds, nullptr, q, a);
-            end();
-            begin(1);
-            if (count) k_shadow<true, false><<<g_shd, kBlock, 0, s->stream>>>(ds, nullptr, q, spill, tune);
-            else k_shadow<false, false><<<g_shd, kBlock, 0, s->stream>>>(ds, nullptr, q, spill, tune);
             end();
             k_rotate<<<1, 1, 0, s->stream>>>(q, a);
         };
@@ -2735,8 +2770,8 @@ int wr_render_batch(const wr_batch_job *jobs, uint32_t n_jobs, const wr_render_p
                                  : reinterpret_cast<const void *>(&k_regen<true>), kShadeBlock);
     const int g_shade = grid(env ? reinterpret_cast<const void *>(&k_shade<true, true>)
                                  : reinterpret_cast<const void *>(&k_shade<true>), kShadeBlock);
-    const int g_ext = trav_grid(s0, P, occupancy(dev, reinterpret_cast<const void *>(&k_extend<false, true>), kBlock));
-    const int g_shd = trav_grid(s0, 2 * P, occupancy(dev, reinterpret_cast<const void *>(&k_shadow<false, true>), kBlock));
+    const int g_trav =
+        trav_grid(s0, 2 * P, occupancy(dev, reinterpret_cast<const void *>(&k_trace_bounce<false, true>), kBlock));
     std::vector<BatchJob> tab;
     for (;;) {
         // plan the pass: whole spp planes of each unfinished job, job-major, within the pass budget
@@ -2783,16 +2818,12 @@ int wr_render_batch(const wr_batch_job *jobs, uint32_t n_jobs, const wr_render_p
             else k_regen<true><<<g_regen, kShadeBlock, 0, s0->stream>>>(s0->ds, d_tab, nt, q, a);
             end();
             begin(0);
-            if (count) k_extend<true, true><<<g_ext, kBlock, 0, s0->stream>>>(s0->ds, d_tab, q, s0->d_spill, tune);
-            else k_extend<false, true><<<g_ext, kBlock, 0, s0->stream>>>(s0->ds, d_tab, q, s0->d_spill, tune);
+            if (count) k_trace_bounce<true, true><<<g_trav, kBlock, 0, s0->stream>>>(s0->ds, d_tab, q, s0->d_spill, tune);
+            else k_trace_bounce<false, true><<<g_trav, kBlock, 0, s0->stream>>>(s0->ds, d_tab, q, s0->d_spill, tune);
             end();
             begin(2);
             if (env) k_shade<true, true><<<g_shade, kShadeBlock, 0, s0->stream>>>(s0->ds, d_tab, q, a);
             else k_shade<true><<<g_shade, kShadeBlock, 0, s0->stream>>>(s0->ds, d_tab, q, a);
-            end();
-            begin(1);
-            if (count) k_shadow<true, true><<<g_shd, kBlock, 0, s0->stream>>>(s0->ds, d_tab, q, s0->d_spill, tune);
-            else k_shadow<false, true><<<g_shd, kBlock, 0, s0->stream>>>(s0->ds, d_tab, q, s0->d_spill, tune);
             end();
             k_rotate<<<1, 1, 0, s0->stream>>>(q, a);
         };
