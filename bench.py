@@ -638,14 +638,14 @@ def main():
     mrays = (ext + shd) / wall / 1e6
     ext_ms = sum(s.extend_ms for s in stats)
     ext_launches = sum(s.extend_launches for s in stats)
-    shd_ms = sum(s.shadow_ms for s in stats)
-    print(f"# timed {a.steps} steps in {wall:.3f}s: ext {ext} shd {shd} rays in {ext_launches} / "
-          f"{sum(s.shadow_launches for s in stats)} launches; extend {ext_ms:.1f} ms "
-          f"shadow {shd_ms:.1f} ms shade {sum(s.shade_ms for s in stats):.1f} ms "
+    print(f"# timed {a.steps} steps in {wall:.3f}s: ext {ext} shd {shd} rays in {ext_launches} "
+          f"traversal launches ({ext_ms:.1f} ms) shade {sum(s.shade_ms for s in stats):.1f} ms "
           f"generate {sum(s.generate_ms for s in stats):.1f} ms", file=sys.stderr, flush=True)
-    # roofline of the closest-hit traversal kernel: HBM-side bytes per launch
-    # (rocprof FETCH_SIZE / WRITE_SIZE of this build on this workload, per ray,
-    # times the live rays per launch) over the live mean launch time
+    # roofline of the traversal kernel (k_trace_bounce_fast: one launch per pool
+    # iteration traces that iteration's closest-hit rays and the previous one's
+    # shadow rays): bytes leaving the L2 per launch (rocprof request counters of
+    # this build on this workload, per traced ray, times the live rays per
+    # launch) over the live mean launch time
     roof = None
     if not a.no_count:
         f2 = wr.Film(gs)
@@ -658,43 +658,41 @@ def main():
         ext_bpr = EXT_RAY_BYTES + nodes_per_ray * NODE_BYTES + tris_per_ray * TRI_BYTES
         shd_bpr = SHD_RAY_BYTES + shd_nodes * NODE_BYTES + shd_tris * TRI_BYTES
         avg_ms = ext_ms / ext_launches
-        rays_per_launch = ext / ext_launches
-        shd_launches = sum(s.shadow_launches for s in stats)
-        shd_avg_ms = shd_ms / shd_launches
-        shd_per_launch = shd / shd_launches
+        ext_per_launch = ext / ext_launches
+        shd_per_launch = shd / ext_launches
+        rays_per_launch = ext_per_launch + shd_per_launch
         tp = os.path.join(ROOT, "profiles", "extend_traffic.json")
         prof = json.load(open(tp)) if os.path.exists(tp) else {}
-        ext_prof, shd_prof = prof.get("extend", {}), prof.get("shadow", {})
+        tr_prof = prof.get("trace", {})
         roof = {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None, "traffic": None,
-                "kernel": "k_extend_fast (closest-hit BVH4 traversal)", "avg_launch_ms": round(avg_ms, 4),
+                "kernel": "k_trace_bounce_fast (BVH4 traversal: closest-hit rays of a bounce with the shadow rays "
+                          "of the one before)", "avg_launch_ms": round(avg_ms, 4),
                 "launches": ext_launches, "rays_per_launch": round(rays_per_launch),
+                "extend_rays_per_launch": round(ext_per_launch), "shadow_rays_per_launch": round(shd_per_launch),
                 "traffic_source": "profiles/extend_traffic.json (build " + str(prof.get("build")) + "): bytes leaving "
-                                  "the L2 toward the fabric per extension ray, from rocprofv3 TCC_EA0_RDREQ_128B/64B/32B "
-                                  "and WRITE_SIZE over the whole frame; the 256 MB MALL on that side serves part of "
-                                  "them before HBM, so this bounds the HBM bytes from above"}
-        if ext_prof.get("l2_egress_bytes_per_ray"):
-            traffic = ext_prof["l2_egress_bytes_per_ray"] * rays_per_launch
+                                  "the L2 toward the fabric per traced ray (closest-hit and shadow), from rocprofv3 "
+                                  "TCC_EA0_RDREQ_128B/64B/32B and WRITE_SIZE over the whole frame; the 256 MB MALL on "
+                                  "that side serves part of them before HBM, so this bounds the HBM bytes from above"}
+        if tr_prof.get("l2_egress_bytes_per_ray"):
+            traffic = tr_prof["l2_egress_bytes_per_ray"] * rays_per_launch
             achieved = traffic / (avg_ms * 1e-3) / 1e9
             roof.update(achieved=round(achieved, 1), frac=round(achieved / HBM_PEAK_GBS, 4), traffic=round(traffic))
         # the cache-inclusive rate: every visited node and tested triangle counted
         # as read (the L2 and MALL serve most of them), so it can pass the HBM peak
-        alg = rays_per_launch * ext_bpr / (avg_ms * 1e-3) / 1e9
+        alg_bytes = ext_per_launch * ext_bpr + shd_per_launch * shd_bpr
+        alg = alg_bytes / (avg_ms * 1e-3) / 1e9
         roof["algorithmic_incl_cache"] = {
-            "bytes_per_ray": round(ext_bpr, 1), "nodes_per_ray": round(nodes_per_ray, 2),
-            "tris_per_ray": round(tris_per_ray, 2), "gbs": round(alg, 1), "frac_of_hbm_peak": round(alg / HBM_PEAK_GBS, 4)}
-        # the compulsory rate: rays in and hits out, plus the whole BVH read once per launch
+            "bytes_per_launch": round(alg_bytes),
+            "extend_bytes_per_ray": round(ext_bpr, 1), "nodes_per_ray": round(nodes_per_ray, 2),
+            "tris_per_ray": round(tris_per_ray, 2), "shadow_bytes_per_ray": round(shd_bpr, 1),
+            "shadow_nodes_per_ray": round(shd_nodes, 2), "shadow_tris_per_ray": round(shd_tris, 2),
+            "gbs": round(alg, 1), "frac_of_hbm_peak": round(alg / HBM_PEAK_GBS, 4)}
+        # the compulsory rate: rays in and results out, plus the whole BVH read once per launch
         bvh_bytes = info["nodes"] * NODE_BYTES + sc.n_tris * TRI_BYTES
-        comp = (rays_per_launch * EXT_RAY_BYTES + bvh_bytes) / (avg_ms * 1e-3) / 1e9
-        roof["compulsory"] = {"bytes_per_launch": round(rays_per_launch * EXT_RAY_BYTES + bvh_bytes),
+        comp_bytes = ext_per_launch * EXT_RAY_BYTES + shd_per_launch * SHD_RAY_BYTES + bvh_bytes
+        comp = comp_bytes / (avg_ms * 1e-3) / 1e9
+        roof["compulsory"] = {"bytes_per_launch": round(comp_bytes),
                               "gbs": round(comp, 1), "frac_of_hbm_peak": round(comp / HBM_PEAK_GBS, 4)}
-        shadow = {"kernel": "k_shadow (any-hit BVH4 traversal)", "avg_launch_ms": round(shd_avg_ms, 4),
-                  "rays_per_launch": round(shd_per_launch), "nodes_per_ray": round(shd_nodes, 2),
-                  "algorithmic_incl_cache_gbs": round(shd_per_launch * shd_bpr / (shd_avg_ms * 1e-3) / 1e9, 1)}
-        if shd_prof.get("l2_egress_bytes_per_ray"):
-            st_ = shd_prof["l2_egress_bytes_per_ray"] * shd_per_launch
-            shadow.update(traffic=round(st_), achieved=round(st_ / (shd_avg_ms * 1e-3) / 1e9, 1),
-                          frac=round(st_ / (shd_avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4))
-        roof["shadow"] = shadow
     others = None
     if not a.no_integrators:
         others = other_integrators(wr, scenes, dev, gs, sc, N, a)
@@ -732,8 +730,10 @@ def main():
         "samples_per_s": round(paths / wall, 1),
         "extend_rays": ext,
         "shadow_rays": shd,
-        "kernel_ms": {k: round(sum(getattr(s, k) for s in stats), 2) for k in
-                      ("extend_ms", "shadow_ms", "shade_ms", "generate_ms", "accumulate_ms")},
+        # trace_ms: the traversal launches (closest-hit and shadow rays together)
+        "kernel_ms": {name: round(sum(getattr(s, k) for s in stats), 2) for name, k in
+                      (("trace_ms", "extend_ms"), ("shade_ms", "shade_ms"), ("generate_ms", "generate_ms"),
+                       ("accumulate_ms", "accumulate_ms"))},
         "roofline": roof,
         "cpu_baseline": cpu,
         "guided_vs_plain_1080p": c4,
