@@ -661,7 +661,7 @@ def main():
         ext_per_launch = ext / ext_launches
         shd_per_launch = shd / ext_launches
         rays_per_launch = ext_per_launch + shd_per_launch
-        tp = os.path.join(ROOT, "profiles", "extend_traffic.json")
+        tp = os.path.join(ROOT, "profiles", "traversal_traffic.json")
         prof = json.load(open(tp)) if os.path.exists(tp) else {}
         tr_prof = prof.get("trace", {})
         roof = {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None, "traffic": None,
@@ -669,7 +669,7 @@ def main():
                           "of the one before)", "avg_launch_ms": round(avg_ms, 4),
                 "launches": ext_launches, "rays_per_launch": round(rays_per_launch),
                 "extend_rays_per_launch": round(ext_per_launch), "shadow_rays_per_launch": round(shd_per_launch),
-                "traffic_source": "profiles/extend_traffic.json (build " + str(prof.get("build")) + "): bytes leaving "
+                "traffic_source": "profiles/traversal_traffic.json (build " + str(prof.get("build")) + "): bytes leaving "
                                   "the L2 toward the fabric per traced ray (closest-hit and shadow), from rocprofv3 "
                                   "TCC_EA0_RDREQ_128B/64B/32B and WRITE_SIZE over the whole frame; the 256 MB MALL on "
                                   "that side serves part of them before HBM, so this bounds the HBM bytes from above"}
