@@ -488,6 +488,19 @@ def batched(wr, scenes, dev, a):
     print(f"# batch: {len(jobs)} scenes, {tris} tris, {st.paths} paths in {t_batch:.2f}s "
           f"({rays_b / t_batch / 1e6:.0f} Mrays/s); build {build_s:.1f}s, autotune {tune_s:.1f}s", file=sys.stderr,
           flush=True)
+    # the same batch with hipGraph replay switched the other way, so the line shows
+    # what the autotuner's choice is worth; the films must agree with the first run
+    alt = type(tuning).from_buffer_copy(tuning)
+    alt.use_graph = 0 if tuning.use_graph else 1
+    alt_jobs = [dict(j, film=wr.Film(j["scene"])) for j in jobs]
+    t0 = time.perf_counter()
+    wr.render_batch(alt_jobs, tuning=alt)
+    t_alt = time.perf_counter() - t0
+    alt_rel = max(rel_mse(j["film"].image(), aj["film"].image()) for j, aj in zip(jobs, alt_jobs))
+    for aj in alt_jobs:
+        aj["film"].close()
+    print(f"# batch with use_graph={int(alt.use_graph)}: {t_alt:.2f}s; max per-scene relMSE against the first run "
+          f"{alt_rel:.2e}", file=sys.stderr, flush=True)
     # one scene at a time, with the batch's scheduling parameters; each scene is
     # warmed up first (its path pool allocated, as the autotuner did for the batch)
     t_ind, rays_i, rel = 0.0, 0, []
@@ -515,6 +528,9 @@ def batched(wr, scenes, dev, a):
             "autotuned": {"max_paths_in_flight": tuning.max_paths_in_flight, "fetch_chunk": tuning.fetch_chunk,
                           "refill_lanes": tuning.refill_lanes, "use_graph": bool(tuning.use_graph),
                           "seconds": round(tune_s, 2)},
+            "graph_replay_switched": {"use_graph": bool(alt.use_graph), "seconds": round(t_alt, 3),
+                                      "mrays_per_s": round(rays_b / t_alt / 1e6, 1),
+                                      "max_relmse_vs_autotuned": float(f"{alt_rel:.3e}")},
             "per_scene_relmse_vs_individual": [float(f"{x:.3e}") for x in rel],
             "note": "one scene at a time uses the batch's autotuned scheduling and is warmed up first (pool "
                     "allocated), as the batch is by the autotuner; the large jobs fill the GPU by themselves, so "

@@ -54,7 +54,7 @@ struct DMedium {
 };
 
 struct DScene {
-    const Node4 *nodes;
+    const NodeQ *nodes;       // traversal nodes (quantised copy of the BVH4)
     const float4 *tris;       // 3 float4 per leaf-order triangle
     const int2 *geom_info;    // (material, light) per geometry
     const DMat *mats;

@@ -119,7 +119,8 @@ struct wr_scene {
     hipStream_t stream = nullptr;
     DScene ds{};
     // device allocations
-    Node4 *d_nodes = nullptr;
+    Node4 *d_nodes = nullptr;   // the build's BVH4 (checks and statistics)
+    NodeQ *d_qnodes = nullptr;  // its quantised copy, which the kernels traverse
     float4 *d_tris = nullptr, *d_ltris = nullptr;
     uint2 *d_spill = nullptr;   // traversal stack spill: kSpillDepth entries per lane of the largest grid
     float4 *d_bdpt = nullptr;   // bidirectional light-vertex records (+ 2 ray counters), grown on demand

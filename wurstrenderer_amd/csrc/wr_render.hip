@@ -25,6 +25,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cfloat>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -126,6 +127,62 @@ __global__ void k_rotate(PathBuffers pb, RenderArgs a) {
     }
     plan_fresh(c, a);
 }
+// The traversal copy of the BVH4 (NodeQ), one node per thread.  Per axis the
+// grid starts at the smallest lo of the node's children and its spacing is the
+// smallest power of two whose plane 255, decoded as the kernels decode it
+// (fmaf(q, scale, origin)), reaches the largest hi.  A child's low plane is the
+// largest q whose decoded value is <= its lo, its high plane the smallest q
+// whose decoded value is >= its hi: q = 0 decodes to the origin (<= every lo)
+// and q = 255 to at least the largest hi, so both exist and the decoded box
+// contains the child's box.
+__global__ void k_quantize_nodes(const Node4 *__restrict__ in, NodeQ *__restrict__ out, uint32_t n) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const Node4 &nd = in[k];
+    const float *lo[3] = {nd.lo_x, nd.lo_y, nd.lo_z}, *hi[3] = {nd.hi_x, nd.hi_y, nd.hi_z};
+    NodeQ q;
+    float scale[3];
+    for (int a = 0; a < 3; a++) {
+        float org = FLT_MAX, top = -FLT_MAX;
+        for (int c = 0; c < 4; c++)
+            if (nd.child[c] != kEmptyChild) {
+                org = fminf(org, lo[a][c]);
+                top = fmaxf(top, hi[a][c]);
+            }
+        if (!(org <= top)) org = top = 0.f;  // no child
+        const double ext = double(top) - double(org);
+        int e = ext > 0.0 ? max(-126, min(127, ilogb(ext / 255.0))) : -126;
+        float sc = ldexpf(1.f, e);
+        while (e < 127 && fmaf(255.f, sc, org) < top) sc = ldexpf(1.f, ++e);
+        uint32_t wl = 0, wh = 0;
+        for (int c = 0; c < 4; c++) {
+            uint32_t ql = 255, qh = 0;  // empty slot
+            if (nd.child[c] != kEmptyChild) {
+                const float l = lo[a][c], h = hi[a][c];
+                int a0 = int(fmin(255.0, fmax(0.0, floor((double(l) - org) / sc))));
+                while (a0 > 0 && fmaf(float(a0), sc, org) > l) a0--;
+                while (a0 < 255 && fmaf(float(a0 + 1), sc, org) <= l) a0++;
+                int b0 = int(fmin(255.0, fmax(0.0, ceil((double(h) - org) / sc))));
+                while (b0 < 255 && fmaf(float(b0), sc, org) < h) b0++;
+                while (b0 > 0 && fmaf(float(b0 - 1), sc, org) >= h) b0--;
+                ql = uint32_t(a0);
+                qh = uint32_t(b0);
+            }
+            wl |= ql << (8 * c);
+            wh |= qh << (8 * c);
+        }
+        q.org[a] = org;
+        scale[a] = sc;
+        q.q[2 * a] = wl;
+        q.q[2 * a + 1] = wh;
+    }
+    q.scale_x = scale[0];
+    q.scale_y = scale[1];
+    q.scale_z = scale[2];
+    for (int c = 0; c < 4; c++) q.child[c] = nd.child[c];
+    out[k] = q;
+}
+
 __global__ void k_fill_u32(uint32_t *p, uint32_t v, size_t n) {
     for (size_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) p[i] = v;
 }
@@ -164,7 +221,7 @@ __device__ __forceinline__ uint32_t find_job(const BatchJob *jobs, uint32_t n_jo
 }
 // Geometry of one lane's scene for traversal.
 struct Geo {
-    const Node4 *nodes;
+    const NodeQ *nodes;
     const float4 *tris;
 };
 
@@ -1593,6 +1650,7 @@ void wr_scene_destroy(wr_scene *s) {
     (void)hipSetDevice(s->dev->ordinal);
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     (void)hipFree(s->d_nodes);
+    (void)hipFree(s->d_qnodes);
     (void)hipFree(s->d_tris);
     (void)hipFree(s->d_ltris);
     (void)hipFree(s->d_geom);
@@ -1744,6 +1802,14 @@ static int scene_create_impl(wr_device *dev, const wr_scene_desc *d, int builder
     s->n_tris = d->n_tris;
     rc = builder == WR_BVH_HOST ? bvh_host(s, d) : bvh_gpu(s, d);
     if (rc) return rc;
+    WR_HIP(hipMalloc(&s->d_qnodes, sizeof(NodeQ) * std::max<uint64_t>(s->n_nodes, 1)));
+    s->device_bytes += sizeof(NodeQ) * s->n_nodes;
+    if (s->n_nodes) {
+        k_quantize_nodes<<<uint32_t((s->n_nodes + 255) / 256), 256, 0, s->stream>>>(s->d_nodes, s->d_qnodes,
+                                                                                    uint32_t(s->n_nodes));
+        WR_HIP(hipGetLastError());
+        WR_HIP(hipStreamSynchronize(s->stream));
+    }
     if (s->max_stack > (uint32_t)kStackDepth) return set_error(WR_ERR_UNSUPPORTED, "BVH deeper than the traversal stack");
     s->max_trav_blocks = dev->n_cu * kMaxTraversalBlocksPerCU;
     {
@@ -1861,7 +1927,7 @@ static int scene_create_impl(wr_device *dev, const wr_scene_desc *d, int builder
     dc.per_film_norm = float(dist * dist);
     dc.aperture = c.aperture_radius;
     dc.focal = c.focal_distance;
-    s->ds.nodes = s->d_nodes;
+    s->ds.nodes = s->d_qnodes;
     s->ds.tris = s->d_tris;
     s->ds.geom_info = s->d_geom;
     s->ds.mats = s->d_mats;
@@ -1928,6 +1994,31 @@ int wr_scene_bvh_check(wr_scene *s, const wr_scene_desc *d, wr_bvh_stats *st) {
     for (uint32_t i = 0; i < s->n_tris; i++)
         if (tris[i].geom != d->tri_geom[tris[i].orig] || tris[i].orig != s->leaf_to_orig[i])
             return set_error(WR_ERR_DEVICE, "BVH invariant violated: leaf triangle geometry id or order map");
+    // the traversal copy: same refs, and every decoded child box (the kernels'
+    // fmaf(q, scale, origin), IEEE single rounding on both sides) contains the Node4 box
+    {
+        std::vector<NodeQ> qn(s->n_nodes);
+        WR_HIP(hipMemcpy(qn.data(), s->d_qnodes, sizeof(NodeQ) * qn.size(), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < qn.size(); i++) {
+            const Node4 &a = nodes[i];
+            const NodeQ &b = qn[i];
+            const float *lo[3] = {a.lo_x, a.lo_y, a.lo_z}, *hi[3] = {a.hi_x, a.hi_y, a.hi_z};
+            const float sc[3] = {b.scale_x, b.scale_y, b.scale_z};
+            for (int c = 0; c < 4; c++) {
+                if (b.child[c] != a.child[c])
+                    return set_error(WR_ERR_DEVICE, "BVH invariant violated: quantised node refs differ");
+                if (a.child[c] == kEmptyChild) continue;
+                for (int k = 0; k < 3; k++) {
+                    const float pl = std::fma(float((b.q[2 * k] >> (8 * c)) & 0xFFu), sc[k], b.org[k]);
+                    const float ph = std::fma(float((b.q[2 * k + 1] >> (8 * c)) & 0xFFu), sc[k], b.org[k]);
+                    if (!(pl <= lo[k][c] && ph >= hi[k][c]))
+                        return set_error(WR_ERR_DEVICE, "BVH invariant violated: quantised box of node " +
+                                                            std::to_string(i) + " does not contain its child " +
+                                                            std::to_string(c));
+                }
+            }
+        }
+    }
     // SAH cost relative to the root box
     auto area = [](const Node4 &n, int c) {
         const double dx = double(n.hi_x[c]) - n.lo_x[c], dy = double(n.hi_y[c]) - n.lo_y[c],

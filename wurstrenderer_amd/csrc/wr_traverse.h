@@ -23,7 +23,6 @@ struct RaySetup {
     float nfx, nfy, nfz;       // 2^-22 |o/d|: a plane p is crossed at fma(p, 1/d, nn) or fma(p, 1/d, nf)
     int kx, ky, kz;            // watertight axis permutation
     float Sx, Sy, Sz;
-    uint32_t onx, ony, onz;    // byte offsets of the near-plane rows in a Node4 (the far row is at offset ^ 16)
 };
 
 __device__ __forceinline__ float pick(float3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
@@ -42,11 +41,6 @@ __device__ __forceinline__ RaySetup setup_ray(float3 o, float3 d, float tmin) {
     const float ex = 2.38418579e-7f * fabsf(mx), ey = 2.38418579e-7f * fabsf(my), ez = 2.38418579e-7f * fabsf(mz);
     r.nnx = -mx - ex; r.nny = -my - ey; r.nnz = -mz - ez;
     r.nfx = -mx + ex; r.nfy = -my + ey; r.nfz = -mz + ez;
-    // a Node4 holds rows lo_x, hi_x, lo_y, hi_y, lo_z, hi_z of 16 bytes: the near
-    // plane of an axis is its lo row for a positive direction, its hi row otherwise
-    r.onx = r.ix < 0.f ? 16u : 0u;
-    r.ony = r.iy < 0.f ? 48u : 32u;
-    r.onz = r.iz < 0.f ? 80u : 64u;
     float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
     int kz = (ax >= ay) ? ((ax >= az) ? 0 : 2) : ((ay >= az) ? 1 : 2);
     int kx = kz == 2 ? 0 : kz + 1;
@@ -160,11 +154,11 @@ __device__ __forceinline__ uint64_t stack_entry(uint32_t ref, uint32_t tn_bits) 
 }
 
 // Entry and exit distances of the ray through the four child boxes of inner
-// node `cur`, and the child refs.  The near and far plane rows are loaded at
-// the ray's per-axis byte offsets, so no per-child select is needed.  Empty
-// slots hold inverted boxes (lo = FLT_MAX, hi = -FLT_MAX): their entry
-// distance exceeds their exit distance, so they fail the box test with no
-// check of the child ref.
+// node `cur`, and the child refs.  The node is a NodeQ: four 16-byte loads,
+// then each plane is decoded to its FP32 value fmaf(q, scale, origin), which
+// lies outside the exact child box (the quantiser checks it with the same
+// fmaf), so the slab test below sees a box that contains the exact one.  An
+// empty slot is rejected by its ref (kEmptyChild).
 //
 // A plane p is crossed at fma(p, 1/d, -(o/d)): one operation instead of a
 // subtraction and a product.  Against the exact (p - o)/d this form errs by
@@ -181,21 +175,25 @@ __device__ __forceinline__ uint64_t stack_entry(uint32_t ref, uint32_t tn_bits) 
 template <class G>
 __device__ __forceinline__ uint4 node_test(const G &s, const RaySetup &r, uint32_t cur, float far_limit, float tn[4],
                                            bool hit[4]) {
-    const char *nb = reinterpret_cast<const char *>(s.nodes);
-    const uint32_t base = cur * uint32_t(sizeof(Node4));
-    auto row = [&](uint32_t off) { return *reinterpret_cast<const float4 *>(nb + (base + off)); };
-    const float4 nx_ = row(r.onx), fx_ = row(r.onx ^ 16u);
-    const float4 ny_ = row(r.ony), fy_ = row(r.ony ^ 16u);
-    const float4 nz_ = row(r.onz), fz_ = row(r.onz ^ 16u);
-    const uint4 ch = *reinterpret_cast<const uint4 *>(nb + (base + 96u));
-    const float nx[4] = {nx_.x, nx_.y, nx_.z, nx_.w}, fx[4] = {fx_.x, fx_.y, fx_.z, fx_.w};
-    const float ny[4] = {ny_.x, ny_.y, ny_.z, ny_.w}, fy[4] = {fy_.x, fy_.y, fy_.z, fy_.w};
-    const float nz[4] = {nz_.x, nz_.y, nz_.z, nz_.w}, fz[4] = {fz_.x, fz_.y, fz_.z, fz_.w};
+    // the node's four 16-byte rows (NodeQ)
+    const uint4 *nb = reinterpret_cast<const uint4 *>(s.nodes) + 4 * size_t(cur);
+    const uint4 r0 = nb[0], r1 = nb[1], r2 = nb[2], r3 = nb[3];
+    const float ox = __uint_as_float(r0.x), oy = __uint_as_float(r0.y), oz = __uint_as_float(r0.z);
+    const float sx = __uint_as_float(r0.w), sy = __uint_as_float(r3.z), sz = __uint_as_float(r3.w);
+    // the near plane of an axis is its low plane for a positive direction, its high plane otherwise
+    const uint32_t qnx = r.ix < 0.f ? r1.y : r1.x, qfx = r.ix < 0.f ? r1.x : r1.y;
+    const uint32_t qny = r.iy < 0.f ? r1.w : r1.z, qfy = r.iy < 0.f ? r1.z : r1.w;
+    const uint32_t qnz = r.iz < 0.f ? r2.y : r2.x, qfz = r.iz < 0.f ? r2.x : r2.y;
+    const uint4 ch = make_uint4(r2.z, r2.w, r3.x, r3.y);
+    const uint32_t c[4] = {ch.x, ch.y, ch.z, ch.w};
+    auto plane = [](uint32_t q, int i, float sc, float org) { return fmaf(float((q >> (8 * i)) & 0xFFu), sc, org); };
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-        tn[i] = fmaxf(fmaxf(fmaf(nx[i], r.ix, r.nnx), fmaf(ny[i], r.iy, r.nny)), fmaxf(fmaf(nz[i], r.iz, r.nnz), r.tmin));
-        const float tf = fminf(fminf(fmaf(fx[i], r.ix, r.nfx), fmaf(fy[i], r.iy, r.nfy)), fmaf(fz[i], r.iz, r.nfz));
-        hit[i] = tn[i] <= fminf(tf * 1.00000095f, far_limit);
+        const float nx = plane(qnx, i, sx, ox), ny = plane(qny, i, sy, oy), nz = plane(qnz, i, sz, oz);
+        const float fx = plane(qfx, i, sx, ox), fy = plane(qfy, i, sy, oy), fz = plane(qfz, i, sz, oz);
+        tn[i] = fmaxf(fmaxf(fmaf(nx, r.ix, r.nnx), fmaf(ny, r.iy, r.nny)), fmaxf(fmaf(nz, r.iz, r.nnz), r.tmin));
+        const float tf = fminf(fminf(fmaf(fx, r.ix, r.nfx), fmaf(fy, r.iy, r.nfy)), fmaf(fz, r.iz, r.nfz));
+        hit[i] = c[i] != kEmptyChild && tn[i] <= fminf(tf * 1.00000095f, far_limit);
     }
     return ch;
 }
