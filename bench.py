@@ -25,7 +25,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, published
-NODE_BYTES, TRI_BYTES = 128, 48
+NODE_BYTES, TRI_BYTES = 64, 48  # a traversal node (NodeQ) and a triangle record
 EXT_RAY_BYTES = 32 + 4 + 8     # ray (o, d) in + queue index + hit out
 SHD_RAY_BYTES = 48             # shadow entry (o+tmax, d+target, contribution)
 
