@@ -55,6 +55,7 @@ struct DMedium {
 
 struct DScene {
     const NodeQ *nodes;       // traversal nodes (quantised copy of the BVH4)
+    const Node4 *fnodes;      // the BVH4 itself (per-pixel kernels)
     const float4 *tris;       // 3 float4 per leaf-order triangle
     const int2 *geom_info;    // (material, light) per geometry
     const DMat *mats;

@@ -224,6 +224,11 @@ struct Geo {
     const NodeQ *nodes;
     const float4 *tris;
 };
+// The same over the full-precision BVH4 (per-pixel kernels, see node_test).
+struct Geo4 {
+    const Node4 *nodes;
+    const float4 *tris;
+};
 
 // Camera sample of path `key`: lens point and primary direction from its
 // lens and pixel-jitter dimensions (lens_point_u / primary_dir_u).
@@ -1285,7 +1290,8 @@ __global__ __launch_bounds__(kBlock) void k_visualize(DScene s, uint32_t wh, uin
             float pdfW;
             const float3 dir = primary_dir(cam, key, px, py, lens, pdfW);
             float t = INFINITY;
-            const uint32_t tri = traverse_rounds<false, kBlock>(s, setup_ray(cpos, dir, kRayEps), t, st, cnt);
+            const uint32_t tri = traverse_rounds<false, kBlock>(Geo4{s.fnodes, s.tris}, setup_ray(cpos, dir, kRayEps),
+                                                                t, st, cnt);
             if (tri == kInvalid) {
                 if (s.env_select > 0.f) { r += 1.0; g += 1.0; b += 1.0; }  // the envmap vertex is a light vertex
                 continue;
@@ -1364,12 +1370,12 @@ __global__ __launch_bounds__(kBlock, 4) void k_bdpt(DScene s, uint32_t wh, uint3
     auto occluded = [&](float3 o, float3 d, float len) {
         n_shd++;
         float t = len - kRayEps;
-        return traverse_rounds<true, kBlock>(s, setup_ray(o, d, kRayEps), t, st, cnt) != kInvalid;
+        return traverse_rounds<true, kBlock>(Geo4{s.fnodes, s.tris}, setup_ray(o, d, kRayEps), t, st, cnt) != kInvalid;
     };
     auto closest = [&](float3 o, float3 d, float &t) {
         n_ext++;
         t = INFINITY;
-        return traverse_rounds<false, kBlock>(s, setup_ray(o, d, kRayEps), t, st, cnt);
+        return traverse_rounds<false, kBlock>(Geo4{s.fnodes, s.tris}, setup_ray(o, d, kRayEps), t, st, cnt);
     };
     for (uint32_t pixel = blockIdx.x * kBlock + lane_id(); pixel < wh; pixel += gridDim.x * kBlock) {
         const uint32_t px = pixel % uint32_t(cam.W), py = pixel / uint32_t(cam.W);
@@ -1927,6 +1933,7 @@ static int scene_create_impl(wr_device *dev, const wr_scene_desc *d, int builder
     dc.aperture = c.aperture_radius;
     dc.focal = c.focal_distance;
     s->ds.nodes = s->d_qnodes;
+    s->ds.fnodes = s->d_nodes;
     s->ds.tris = s->d_tris;
     s->ds.geom_info = s->d_geom;
     s->ds.mats = s->d_mats;

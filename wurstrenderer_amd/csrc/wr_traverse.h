@@ -23,6 +23,7 @@ struct RaySetup {
     float nfx, nfy, nfz;       // 2^-22 |o/d|: a plane p is crossed at fma(p, 1/d, nn) or fma(p, 1/d, nf)
     int kx, ky, kz;            // watertight axis permutation
     float Sx, Sy, Sz;
+    uint32_t onx, ony, onz;    // byte offsets of the near-plane rows in a Node4 (the far row is at offset ^ 16)
 };
 
 __device__ __forceinline__ float pick(float3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
@@ -41,6 +42,11 @@ __device__ __forceinline__ RaySetup setup_ray(float3 o, float3 d, float tmin) {
     const float ex = 2.38418579e-7f * fabsf(mx), ey = 2.38418579e-7f * fabsf(my), ez = 2.38418579e-7f * fabsf(mz);
     r.nnx = -mx - ex; r.nny = -my - ey; r.nnz = -mz - ez;
     r.nfx = -mx + ex; r.nfy = -my + ey; r.nfz = -mz + ez;
+    // a Node4 holds rows lo_x, hi_x, lo_y, hi_y, lo_z, hi_z of 16 bytes: the near
+    // plane of an axis is its lo row for a positive direction, its hi row otherwise
+    r.onx = r.ix < 0.f ? 16u : 0u;
+    r.ony = r.iy < 0.f ? 48u : 32u;
+    r.onz = r.iz < 0.f ? 80u : 64u;
     float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
     int kz = (ax >= ay) ? ((ax >= az) ? 0 : 2) : ((ay >= az) ? 1 : 2);
     int kx = kz == 2 ? 0 : kz + 1;
@@ -172,11 +178,10 @@ __device__ __forceinline__ uint64_t stack_entry(uint32_t ref, uint32_t tn_bits) 
 // and one entered before the far limit t passes entry <= cull_limit(t).
 // Returns the refs; tn holds the entry distances and hit[i] marks the
 // children whose box is entered before `far_limit`.
-template <class G>
-__device__ __forceinline__ uint4 node_test(const G &s, const RaySetup &r, uint32_t cur, float far_limit, float tn[4],
-                                           bool hit[4]) {
-    // the node's four 16-byte rows (NodeQ)
-    const uint4 *nb = reinterpret_cast<const uint4 *>(s.nodes) + 4 * size_t(cur);
+__device__ __forceinline__ uint4 node_test(const NodeQ *nodes, const RaySetup &r, uint32_t cur, float far_limit,
+                                           float tn[4], bool hit[4]) {
+    // the node's four 16-byte rows
+    const uint4 *nb = reinterpret_cast<const uint4 *>(nodes) + 4 * size_t(cur);
     const uint4 r0 = nb[0], r1 = nb[1], r2 = nb[2], r3 = nb[3];
     const float ox = __uint_as_float(r0.x), oy = __uint_as_float(r0.y), oz = __uint_as_float(r0.z);
     const float sx = __uint_as_float(r0.w), sy = __uint_as_float(r3.z), sz = __uint_as_float(r3.w);
@@ -196,6 +201,37 @@ __device__ __forceinline__ uint4 node_test(const G &s, const RaySetup &r, uint32
         hit[i] = c[i] != kEmptyChild && tn[i] <= fminf(tf * 1.00000095f, far_limit);
     }
     return ch;
+}
+
+// The same test on a full-precision Node4: seven loads (the near and far rows
+// at the ray's per-axis byte offsets, and the refs), no decoding.  The kernels
+// that trace one coherent or register-heavy path per lane (k_visualize,
+// k_bdpt) use it, since their traversal is not bound by load requests.  Empty
+// slots hold inverted boxes (lo = FLT_MAX, hi = -FLT_MAX), which fail the test.
+__device__ __forceinline__ uint4 node_test(const Node4 *nodes, const RaySetup &r, uint32_t cur, float far_limit,
+                                           float tn[4], bool hit[4]) {
+    const char *nb = reinterpret_cast<const char *>(nodes);
+    const uint32_t base = cur * uint32_t(sizeof(Node4));
+    auto row = [&](uint32_t off) { return *reinterpret_cast<const float4 *>(nb + (base + off)); };
+    const float4 nx_ = row(r.onx), fx_ = row(r.onx ^ 16u);
+    const float4 ny_ = row(r.ony), fy_ = row(r.ony ^ 16u);
+    const float4 nz_ = row(r.onz), fz_ = row(r.onz ^ 16u);
+    const uint4 ch = *reinterpret_cast<const uint4 *>(nb + (base + 96u));
+    const float nx[4] = {nx_.x, nx_.y, nx_.z, nx_.w}, fx[4] = {fx_.x, fx_.y, fx_.z, fx_.w};
+    const float ny[4] = {ny_.x, ny_.y, ny_.z, ny_.w}, fy[4] = {fy_.x, fy_.y, fy_.z, fy_.w};
+    const float nz[4] = {nz_.x, nz_.y, nz_.z, nz_.w}, fz[4] = {fz_.x, fz_.y, fz_.z, fz_.w};
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        tn[i] = fmaxf(fmaxf(fmaf(nx[i], r.ix, r.nnx), fmaf(ny[i], r.iy, r.nny)), fmaxf(fmaf(nz[i], r.iz, r.nnz), r.tmin));
+        const float tf = fminf(fminf(fmaf(fx[i], r.ix, r.nfx), fmaf(fy[i], r.iy, r.nfy)), fmaf(fz[i], r.iz, r.nfz));
+        hit[i] = tn[i] <= fminf(tf * 1.00000095f, far_limit);
+    }
+    return ch;
+}
+template <class G>
+__device__ __forceinline__ uint4 node_test(const G &s, const RaySetup &r, uint32_t cur, float far_limit, float tn[4],
+                                           bool hit[4]) {
+    return node_test(s.nodes, r, cur, far_limit, tn, hit);
 }
 
 struct TraversalCounters {
