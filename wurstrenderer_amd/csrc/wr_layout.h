@@ -35,7 +35,7 @@ static_assert(sizeof(Node4) == 128, "Node4 must be one cache line");
 // The traversal copy of a Node4: 64 B in four 16-byte rows, so a node visit is
 // four loads instead of seven (traversal is bound by the L1's rate of lane
 // requests, not by arithmetic).  Each child box is stored as 8-bit planes on a
-// per-node, per-axis grid: plane q of axis a stands for the FP32 value
+// per-node, per-axis grid of 256 planes: plane q of axis a stands for the FP32 value
 // fmaf(q, scale_a, origin_a), computed exactly so by the kernels, and the
 // quantiser (k_quantize_nodes) picks every low plane so that this float is <=
 // the child's lo and every high plane so that it is >= its hi: the decoded box
@@ -46,7 +46,7 @@ static_assert(sizeof(Node4) == 128, "Node4 must be one cache line");
 // planes 255 / 0) and is rejected by its ref.
 struct alignas(64) NodeQ {
     float org[3];
-    float scale_x;       // grid spacing, a power of two
+    float scale_x;       // grid spacing
     uint32_t q[6];       // lo_x, hi_x, lo_y, hi_y, lo_z, hi_z
     uint32_t child[4];   // as Node4::child
     float scale_y, scale_z;

@@ -128,9 +128,9 @@ __global__ void k_rotate(PathBuffers pb, RenderArgs a) {
     plan_fresh(c, a);
 }
 // The traversal copy of the BVH4 (NodeQ), one node per thread.  Per axis the
-// grid starts at the smallest lo of the node's children and its spacing is the
-// smallest power of two whose plane 255, decoded as the kernels decode it
-// (fmaf(q, scale, origin)), reaches the largest hi.  A child's low plane is the
+// grid starts at the smallest lo of the node's children and its spacing is
+// (largest hi - origin) / 255, raised by ulps until plane 255, decoded as the
+// kernels decode it (fmaf(q, scale, origin)), reaches the largest hi.  A child's low plane is the
 // largest q whose decoded value is <= its lo, its high plane the smallest q
 // whose decoded value is >= its hi: q = 0 decodes to the origin (<= every lo)
 // and q = 255 to at least the largest hi, so both exist and the decoded box
@@ -150,10 +150,8 @@ __global__ void k_quantize_nodes(const Node4 *__restrict__ in, NodeQ *__restrict
                 top = fmaxf(top, hi[a][c]);
             }
         if (!(org <= top)) org = top = 0.f;  // no child
-        const double ext = double(top) - double(org);
-        int e = ext > 0.0 ? max(-126, min(127, ilogb(ext / 255.0))) : -126;
-        float sc = ldexpf(1.f, e);
-        while (e < 127 && fmaf(255.f, sc, org) < top) sc = ldexpf(1.f, ++e);
+        float sc = fmaxf(float((double(top) - double(org)) / 255.0), FLT_MIN);
+        while (fmaf(255.f, sc, org) < top) sc = nextafterf(sc, INFINITY);
         uint32_t wl = 0, wh = 0;
         for (int c = 0; c < 4; c++) {
             uint32_t ql = 255, qh = 0;  // empty slot
