@@ -101,8 +101,8 @@ typedef struct {
     /* GPU-only scheduling (no reference counterpart; the CPU path has none):
      * 0 = library default; wr_autotune picks them per workload */
     int32_t use_graph;           /* 1: replay the per-iteration launches as a captured hipGraph */
-    uint32_t fetch_chunk;        /* traversal work items claimed per atomic (default 96, max 4096) */
-    uint32_t refill_lanes;       /* idle lanes that trigger a wave's refill (default 32, max 64) */
+    uint32_t fetch_chunk;        /* traversal work items claimed per atomic (default 256, max 4096) */
+    uint32_t refill_lanes;       /* idle lanes that trigger a wave's refill (default 16, max 64) */
 } wr_render_params;
 
 typedef struct {

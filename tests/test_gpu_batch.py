@@ -88,7 +88,7 @@ def test_autotune_returns_a_working_configuration(gpu_device):
           f"graph {best.use_graph}: {rate:.1f} Mrays/s")
     assert rate > 0
     assert best.max_paths_in_flight in (1 << 23, 1 << 24, 1 << 25, 1 << 26, 1 << 27)
-    assert best.fetch_chunk in (64, 96, 128, 256) and best.refill_lanes in (32, 40)
+    assert best.fetch_chunk in (96, 128, 256) and best.refill_lanes in (16, 32)
     ref = []
     for j in jobs:
         j["film"] = wr.Film(j["scene"])

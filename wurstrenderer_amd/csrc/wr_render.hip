@@ -381,18 +381,17 @@ __global__ __launch_bounds__(kShadeBlock) void k_regen(DScene s1, const BatchJob
 // and refills its lanes from that local range as soon as `refill` of them are
 // idle, so a wave never waits for its slowest ray and atomics stay rare.
 // Waves beyond the work exit at once.  Both are runtime parameters
-// (TravTune).  With the queue split into per-XCD shares (refill below), a
-// smaller chunk keeps a wave's rays closer together in the queue: on the
-// 1.31M-triangle frame chunk x refill 64x32 / 96x32 / 128x32 / 256x24 /
-// 512x24 gave 5,902 / 5,899 / 5,863 / 5,672 / 5,338 Mrays/s, in the gap room
-// 5,981 / 6,123 / 6,099 / 6,008 / 5,974 (its shadow rays prefer larger
-// chunks, the icosphere frame's smaller); refill 8 and 16 are 2-6% slower.
-// wr_autotune searches them per workload.
+// (TravTune).  With the quantised nodes (traversal bound by L1 requests),
+// on the 1.31M-triangle frame chunk x refill 64x16 / 96x16 / 128x16 / 192x16 /
+// 256x16 gave 7,312 / 7,377 / 7,387 / 7,418 / 7,442 Mrays/s and 96x8 / 96x24 /
+// 96x32 / 96x48 7,355 / 7,326 / 7,205 / 6,911; in the gap room 256x16 7,938,
+// 96x16 7,862, 96x32 7,655: a wave now refills once a quarter of its lanes
+// are idle.  wr_autotune searches them per workload.
 #ifndef WR_REFILL_LANES
-#define WR_REFILL_LANES 32
+#define WR_REFILL_LANES 16
 #endif
 #ifndef WR_FETCH_CHUNK
-#define WR_FETCH_CHUNK 96
+#define WR_FETCH_CHUNK 256
 #endif
 struct TravTune {
     uint32_t chunk = WR_FETCH_CHUNK;
@@ -2982,7 +2981,7 @@ int wr_autotune(const wr_batch_job *jobs, uint32_t n_jobs, const wr_render_param
     struct Candidate { uint32_t pool, chunk, refill; int graph; };
     std::vector<Candidate> cands;
     for (uint32_t pool : {1u << 23, 1u << 24, 1u << 25, 1u << 26, 1u << 27})
-        for (auto ck : {std::make_pair(64u, 32u), std::make_pair(96u, 32u), std::make_pair(128u, 40u), std::make_pair(256u, 32u)})
+        for (auto ck : {std::make_pair(96u, 16u), std::make_pair(256u, 16u), std::make_pair(128u, 32u), std::make_pair(256u, 32u)})
             for (int graph : {0, 1}) cands.push_back({pool, ck.first, ck.second, graph});
     double best_rate = -1.0;
     wr_render_params bp = *base;
