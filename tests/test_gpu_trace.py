@@ -82,6 +82,39 @@ def test_closest_full_scene_camera_rays(gpu_device):
     _check_closest(gs, osc, sub, sc, max_tie_frac=0.05)
 
 
+def _tiny_far_soup(n, seed, offset):
+    """Triangles of ~0.02 in a 2-unit cube placed far from the origin: near the
+    leaves a traversal node's quantisation grid is finer than the FP32 spacing
+    at its origin, so its decoded planes snap to representable floats."""
+    rng = np.random.default_rng(seed)
+    b = scenes.SceneBuilder()
+    m = b.material(scenes.LAMBERT, rho_d=(0.5, 0.5, 0.5))
+    c = rng.uniform(-1.0, 1.0, size=(n, 1, 3))
+    v = c + rng.normal(scale=0.02, size=(n, 3, 3)) + np.asarray(offset)
+    b.mesh(v.reshape(-1, 3), np.arange(3 * n).reshape(-1, 3), material=m)
+    b.mesh(*scenes.quad(np.asarray(offset) + (0, 3, 0), (0.5, 0, 0), (0, 0, 0.5)), light_L=(1, 1, 1))
+    cam = dict(pos=tuple(np.asarray(offset) + (0, 0, 5)), lookat=tuple(offset), fovy=45.0, resolution=(16, 16))
+    return b.build(cam, f"tiny_far_{n}")
+
+
+def test_quantised_nodes_tiny_triangles_far_from_origin(gpu_device):
+    """The traversal nodes store 8-bit child planes; their decoded boxes must
+    contain the exact ones wherever FP32 is coarse relative to the boxes.
+    Closest and any hit against FP64 brute force, BVH containment checked."""
+    for offset in ((3.0e4, -2.0e4, 1.0e4), (-7.5e5, 0.0, 2.5e5)):
+        sc = _tiny_far_soup(3000, 21, offset)
+        gs, osc = wr.GpuScene(gpu_device, sc), oracle.OracleScene(sc)
+        gs.bvh_check(sc)
+        rays = np.concatenate([scenes.random_rays(sc, 15000, seed=22), scenes.random_rays(sc, 15000, seed=23, inside=True)])
+        _check_closest(gs, osc, rays, sc, max_tie_frac=0.05)
+        rays[:, 7] = np.random.default_rng(3).uniform(0.01, 3.0, len(rays))
+        occ = gs.trace_any(rays)
+        oocc, amb = osc.trace_any(rays, brute=True)
+        ok = amb == 0
+        assert (occ[ok] == oocc[ok]).all(), (offset, np.nonzero(occ[ok] != oocc[ok])[0][:10])
+        assert amb.mean() < 0.05
+
+
 def test_any_hit_soup_and_cornell_shadow_rays(gpu_device):
     sc = scenes.triangle_soup(3000, seed=12)
     gs, osc = wr.GpuScene(gpu_device, sc), oracle.OracleScene(sc)
