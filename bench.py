@@ -703,6 +703,32 @@ def main():
             "tris_per_ray": round(tris_per_ray, 2), "shadow_bytes_per_ray": round(shd_bpr, 1),
             "shadow_nodes_per_ray": round(shd_nodes, 2), "shadow_tris_per_ray": round(shd_tris, 2),
             "gbs": round(alg, 1), "frac_of_hbm_peak": round(alg / HBM_PEAK_GBS, 4)}
+        # the bound that traversal actually meets: the L1's rate of scattered lane
+        # requests.  Lane loads per launch (ray in, four 16-byte rows per node
+        # visit, three per triangle test, the result out; a shadow ray also reads
+        # its contribution and target and updates its radiance) against the rate
+        # that a full grid of independent scattered 16-byte loads reaches
+        # (wr_debug_l1_request_probe: a 16 KiB buffer, which every L1 holds, and
+        # a 4 MiB one, which the L2s hold and the L1s do not).
+        ext_loads = 3 + 4 * nodes_per_ray + 3 * tris_per_ray
+        shd_loads = 5 + 4 * shd_nodes + 3 * shd_tris
+        lane_loads = ext_per_launch * ext_loads + shd_per_launch * shd_loads
+        probes = {}
+        for fp in (1 << 14, 1 << 22):
+            p_ms, p_loads, n_cu = wr.l1_request_probe(dev, fp, 4096)
+            probes[f"{fp >> 10}KiB"] = {"ms": round(p_ms, 3), "loads": p_loads,
+                                        "per_ns_per_cu": round(p_loads / (p_ms * 1e6) / n_cu, 4)}
+        trace_rate = lane_loads / (avg_ms * 1e6) / n_cu
+        ceiling = probes["16KiB"]["per_ns_per_cu"]
+        roof["l1_requests"] = {
+            "lane_loads_per_launch": round(lane_loads), "extend_loads_per_ray": round(ext_loads, 2),
+            "shadow_loads_per_ray": round(shd_loads, 2), "per_ns_per_cu": round(trace_rate, 4),
+            "probe": probes, "frac_of_probe": round(trace_rate / ceiling, 4),
+            "note": "the traversal kernel is bound by the rate of scattered lane requests at the L1, not by HBM "
+                    "(with 128-byte nodes one extra 16-byte load per node visit cost ~7%, 30 extra VALU operations "
+                    "~1%); the 16 KiB probe is the rate of independent scattered 16-byte loads that all hit the L1, "
+                    "the 4 MiB one the rate when each must fetch its line from the L2; frac_of_probe counts the "
+                    "kernel's lane loads, of which the L1 merges those of one instruction to one line"}
         # the compulsory rate: rays in and results out, plus the whole BVH read once per launch
         bvh_bytes = info["nodes"] * NODE_BYTES + sc.n_tris * TRI_BYTES
         comp_bytes = ext_per_launch * EXT_RAY_BYTES + shd_per_launch * SHD_RAY_BYTES + bvh_bytes

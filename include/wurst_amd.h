@@ -340,6 +340,17 @@ int wr_debug_camera_ray(wr_scene *scene, const uint32_t *pixel, const float *ule
 /* ThinlensCamera::evalWe1 (thinlens.h:62-80) from lens point org (world) toward
  * direction w: We1 [n] (0 outside the film) and the raster position [n*2]. */
 int wr_debug_camera_we1(wr_scene *scene, const float *org, const float *w, uint32_t n, float *we1, float *raster);
+/* Measured ceiling of the rate at which a CU's L1 serves scattered lane
+ * requests, the bound of the traversal kernels (each lane fetches its own BVH
+ * node in 16-byte rows).  No counterpart in the reference: a hardware
+ * diagnostic for the benchmark's roofline.  A full grid (every CU at full
+ * occupancy) in which each lane issues `loads_per_lane` independent 16-byte
+ * loads, eight in flight, at pseudo-random 64-byte-aligned offsets of a
+ * device buffer of `footprint` bytes (a power of two, 64 B .. 4 GiB; a few MB
+ * stay L2-resident).  Reports the timed launch's duration (ms), the loads it
+ * issued and the GPU's compute units. */
+int wr_debug_l1_request_probe(wr_device *dev, uint64_t footprint, uint32_t loads_per_lane, double *ms, uint64_t *loads,
+                              uint32_t *n_cu);
 
 /* ------------------------------------------------------------------ batches
  * Several independent renders (scene, film, spp, depth, seed) in one path pool:

@@ -178,3 +178,16 @@ def test_debug_entry_points_reject_bad_input(gpu_device):
     m["type"] = 7
     with pytest.raises(wr.WrError, match="material type"):
         wr.debug_bsdf_eval(gpu_device, m, [[0, 1, 0]], [[0, 1, 0]])
+
+
+def test_l1_request_probe(gpu_device):
+    """The roofline probe: every lane of a full grid issues the requested loads
+    (so the count is grid x 256 x loads); an L1-resident buffer serves them
+    faster than one the L1s cannot hold; bad sizes are rejected."""
+    ms_small, loads, n_cu = wr.l1_request_probe(gpu_device, 1 << 14, 64)
+    ms_big, loads_big, _ = wr.l1_request_probe(gpu_device, 1 << 22, 64)
+    assert n_cu > 0 and loads == loads_big and loads % (n_cu * 256 * 64) == 0
+    assert ms_small > 0 and ms_big > 0
+    for fp, k in ((3000, 64), (1 << 14, 12), (1 << 14, 0), (32, 64)):
+        with pytest.raises(wr.WrError):
+            wr.l1_request_probe(gpu_device, fp, k)

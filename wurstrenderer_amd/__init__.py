@@ -152,6 +152,7 @@ EXPORTS = {
     "wr_debug_light_sample": (C.c_int, [P, P, C.c_uint32, P, P, P, P]),
     "wr_debug_camera_ray": (C.c_int, [P, P, P, P, C.c_uint32, P, P, P]),
     "wr_debug_camera_we1": (C.c_int, [P, P, P, C.c_uint32, P, P]),
+    "wr_debug_l1_request_probe": (C.c_int, [P, C.c_uint64, C.c_uint32, P, P, P]),
     "wr_render_batch": (C.c_int, [C.POINTER(BatchJob), C.c_uint32, C.POINTER(RenderParams), C.POINTER(RenderStats)]),
     "wr_autotune": (C.c_int, [C.POINTER(BatchJob), C.c_uint32, C.POINTER(RenderParams), C.POINTER(RenderParams),
                               C.POINTER(C.c_double)]),
@@ -267,6 +268,16 @@ def debug_bsdf_eval(device: Device, mats, wi, wo):
     _check(lib().wr_debug_bsdf_eval(device.h, m.ctypes.data, a.ctypes.data, b.ctypes.data, n, f.ctypes.data,
                                     pdf.ctypes.data))
     return f, pdf
+
+
+def l1_request_probe(device: Device, footprint: int = 1 << 22, loads_per_lane: int = 4096):
+    """wr_debug_l1_request_probe: one timed launch in which every lane of a full
+    grid issues loads_per_lane scattered 16-byte loads (eight in flight) into a
+    buffer of `footprint` bytes; returns (ms, loads, compute units)."""
+    ms, loads, ncu = C.c_double(), C.c_uint64(), C.c_uint32()
+    _check(lib().wr_debug_l1_request_probe(device.h, int(footprint), int(loads_per_lane), C.byref(ms), C.byref(loads),
+                                           C.byref(ncu)))
+    return ms.value, loads.value, ncu.value
 
 
 def debug_bsdf_sample(device: Device, mats, wi, u):

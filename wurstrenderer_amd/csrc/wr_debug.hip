@@ -9,6 +9,7 @@
 // Diagnostics only: inputs are staged through device buffers allocated per call.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
@@ -125,6 +126,27 @@ int to_dmats(const wr_material *m, uint32_t n, std::vector<DMat> &out) {
 
 inline unsigned blocks(uint32_t n) { return (n + kDebugBlock - 1) / kDebugBlock; }
 
+// Scattered 16-byte loads: lane l's k-th load reads the first 16 bytes of line
+// (h_k >> 8) & line_mask of a buffer of line_mask + 1 lines of 64 bytes (in
+// bounds by the mask), eight independent loads per step; the xor of
+// everything read decides an unlikely store, so no load can be dropped.
+__global__ __launch_bounds__(kDebugBlock) void k_debug_l1_probe(const uint4 *__restrict__ buf, uint32_t line_mask,
+                                                                uint32_t steps, uint32_t *sink) {
+    uint32_t h = (blockIdx.x * kDebugBlock + threadIdx.x) * 0x9E3779B9u + 0x7F4A7C15u;
+    uint32_t acc = 0;
+    for (uint32_t i = 0; i < steps; i++) {
+        uint4 v[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            h = h * 1664525u + 1013904223u;
+            v[k] = buf[size_t((h >> 8) & line_mask) * 4];
+        }
+#pragma unroll
+        for (int k = 0; k < 8; k++) acc ^= v[k].x ^ v[k].y ^ v[k].z ^ v[k].w;
+    }
+    if (acc == 0x9E3779B9u) sink[0] = acc;
+}
+
 }  // namespace
 
 int wr_debug_bsdf_eval(wr_device *dev, const wr_material *mats, const float *wi, const float *wo, uint32_t n, float *f,
@@ -239,5 +261,44 @@ int wr_debug_camera_we1(wr_scene *s, const float *org, const float *w, uint32_t 
     WR_HIP(hipGetLastError());
     WR_HIP(hipMemcpy(we1, d_we, sizeof(float) * n, hipMemcpyDeviceToHost));
     WR_HIP(hipMemcpy(raster, d_r, sizeof(float) * 2 * size_t(n), hipMemcpyDeviceToHost));
+    return WR_OK;
+}
+
+int wr_debug_l1_request_probe(wr_device *dev, uint64_t footprint, uint32_t loads_per_lane, double *ms, uint64_t *loads,
+                              uint32_t *n_cu) {
+    if (!dev || !ms || !loads || !n_cu) return set_error(WR_ERR_INVALID_ARGUMENT, "null argument");
+    if (footprint < 64 || footprint > (uint64_t(1) << 32) || (footprint & (footprint - 1)))
+        return set_error(WR_ERR_INVALID_ARGUMENT, "footprint must be a power of two in 64 B .. 4 GiB");
+    if (loads_per_lane < 8 || loads_per_lane % 8)
+        return set_error(WR_ERR_INVALID_ARGUMENT, "loads_per_lane must be a positive multiple of 8");
+    if (int rc = set_device(dev)) return rc;
+    int per_cu = 0;
+    WR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(&k_debug_l1_probe),
+                                                        kDebugBlock, 0));
+    const uint32_t grid = uint32_t(dev->n_cu) * uint32_t(std::max(1, per_cu));
+    Staging S;
+    uint4 *d_buf;
+    uint32_t *d_sink;
+    if (int rc = S.out(&d_buf, size_t(footprint))) return rc;
+    if (int rc = S.out(&d_sink, sizeof(uint32_t))) return rc;
+    WR_HIP(hipMemset(d_buf, 0, size_t(footprint)));
+    const uint32_t mask = uint32_t(footprint / 64 - 1), steps = loads_per_lane / 8;
+    hipEvent_t e0, e1;
+    WR_HIP(hipEventCreate(&e0));
+    WR_HIP(hipEventCreate(&e1));
+    k_debug_l1_probe<<<grid, kDebugBlock>>>(d_buf, mask, steps, d_sink);  // warm-up (caches, clocks)
+    (void)hipEventRecord(e0, nullptr);
+    k_debug_l1_probe<<<grid, kDebugBlock>>>(d_buf, mask, steps, d_sink);
+    (void)hipEventRecord(e1, nullptr);
+    const hipError_t err = hipEventSynchronize(e1);
+    float t = 0.f;
+    if (err == hipSuccess) (void)hipEventElapsedTime(&t, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    WR_HIP(err);
+    WR_HIP(hipGetLastError());
+    *ms = double(t);
+    *loads = uint64_t(grid) * kDebugBlock * loads_per_lane;
+    *n_cu = uint32_t(dev->n_cu);
     return WR_OK;
 }
