@@ -398,10 +398,11 @@ struct TravTune {
     uint32_t refill = WR_REFILL_LANES;
 };
 
-// The plain traversal kernel (k_trace_bounce_fast) runs 8 waves per SIMD in
-// 59 VGPRs without spills (7 waves measured 0.8% slower on the 1.31M-triangle
-// frame, 1.4% in the gap room).  The counting and batched kernels keep the
-// compiler's choice.
+// The traversal kernels run 8 waves per SIMD without spills: the plain one
+// (k_trace_bounce_fast) in 59 VGPRs (7 waves measured 0.8% slower on the
+// 1.31M-triangle frame, 1.4% in the gap room), the batched and counting
+// instances in 62-64 (the batched one at the compiler's 7 waves made the
+// 32-scene batch 3.5% slower: 1.627 against 1.570 s).
 #ifndef WR_EXTEND_WAVES
 #define WR_EXTEND_WAVES 8
 #endif
@@ -1218,8 +1219,8 @@ __device__ __forceinline__ void trace_bounce(DScene s1, const BatchJob *__restri
         shadow_rays<COUNT, BATCH>(s1, jobs, pb, spill, tune, n_shd, reinterpret_cast<uint32_t *>(st_ent));
 }
 template <bool COUNT, bool BATCH>
-__global__ __launch_bounds__(kBlock) void k_trace_bounce(DScene s1, const BatchJob *__restrict__ jobs, PathBuffers pb,
-                                                         uint2 *spill, TravTune tune) {
+__global__ __launch_bounds__(kBlock) WR_EXTEND_ATTR void k_trace_bounce(DScene s1, const BatchJob *__restrict__ jobs,
+                                                                        PathBuffers pb, uint2 *spill, TravTune tune) {
     trace_bounce<COUNT, BATCH>(s1, jobs, pb, spill, tune);
 }
 // The plain kernel of wr_render_nee / wr_render_guided and the other
