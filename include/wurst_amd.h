@@ -346,7 +346,7 @@ int wr_debug_camera_we1(wr_scene *scene, const float *org, const float *w, uint3
  * diagnostic for the benchmark's roofline.  A full grid (every CU at full
  * occupancy) in which each lane issues `loads_per_lane` independent 16-byte
  * loads, eight in flight, at pseudo-random 64-byte-aligned offsets of a
- * device buffer of `footprint` bytes (a power of two, 64 B .. 4 GiB; a few MB
+ * device buffer of `footprint` bytes (a power of two, 64 B .. 1 GiB; a few MB
  * stay L2-resident).  Reports the timed launch's duration (ms), the loads it
  * issued and the GPU's compute units. */
 int wr_debug_l1_request_probe(wr_device *dev, uint64_t footprint, uint32_t loads_per_lane, double *ms, uint64_t *loads,

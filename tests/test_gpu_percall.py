@@ -188,6 +188,6 @@ def test_l1_request_probe(gpu_device):
     ms_big, loads_big, _ = wr.l1_request_probe(gpu_device, 1 << 22, 64)
     assert n_cu > 0 and loads == loads_big and loads % (n_cu * 256 * 64) == 0
     assert ms_small > 0 and ms_big > 0
-    for fp, k in ((3000, 64), (1 << 14, 12), (1 << 14, 0), (32, 64)):
+    for fp, k in ((3000, 64), (1 << 14, 12), (1 << 14, 0), (32, 64), (1 << 31, 64)):
         with pytest.raises(wr.WrError):
             wr.l1_request_probe(gpu_device, fp, k)

@@ -267,8 +267,9 @@ int wr_debug_camera_we1(wr_scene *s, const float *org, const float *w, uint32_t 
 int wr_debug_l1_request_probe(wr_device *dev, uint64_t footprint, uint32_t loads_per_lane, double *ms, uint64_t *loads,
                               uint32_t *n_cu) {
     if (!dev || !ms || !loads || !n_cu) return set_error(WR_ERR_INVALID_ARGUMENT, "null argument");
-    if (footprint < 64 || footprint > (uint64_t(1) << 32) || (footprint & (footprint - 1)))
-        return set_error(WR_ERR_INVALID_ARGUMENT, "footprint must be a power of two in 64 B .. 4 GiB");
+    // the line index takes 24 random bits: 2^24 lines of 64 B
+    if (footprint < 64 || footprint > (uint64_t(1) << 30) || (footprint & (footprint - 1)))
+        return set_error(WR_ERR_INVALID_ARGUMENT, "footprint must be a power of two in 64 B .. 1 GiB");
     if (loads_per_lane < 8 || loads_per_lane % 8)
         return set_error(WR_ERR_INVALID_ARGUMENT, "loads_per_lane must be a positive multiple of 8");
     if (int rc = set_device(dev)) return rc;
